@@ -1,0 +1,110 @@
+"""Generate the high-precision golden fixtures of tests/golden/ (run once, committed).
+
+Independent of both the oracle and the GPU library: mpmath (40 digits) and sympy
+(exact) evaluate the closed forms.  Data files copied verbatim from the reference's
+own test data live next to this script: spherical_hankel.txt
+(crates/treams-core/references/), degrees.txt and harmonics.txt (formal/golden/).
+The end-to-end numbers of the reference's documented examples are in
+reference_examples.json (inputs restated from docs/examples/*.py, outputs from
+docs/examples/output/*.txt).
+
+    python tests/golden/make_fixtures.py
+"""
+import json
+import os
+
+import mpmath as mp
+from sympy.physics.wigner import wigner_3j
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+mp.mp.dps = 40
+
+
+def c2(z):
+    z = mp.mpc(z)
+    return [float(z.real), float(z.imag)]
+
+
+def sph_h(p, x):
+    x = mp.mpf(x)
+    return mp.sqrt(mp.pi / (2 * x)) * (mp.besselj(p + mp.mpf(1) / 2, x) + 1j * mp.bessely(p + mp.mpf(1) / 2, x))
+
+
+def sph_j(p, z):
+    z = mp.mpc(z)
+    return mp.sqrt(mp.pi / (2 * z)) * mp.besselj(p + mp.mpf(1) / 2, z)
+
+
+def hankel_table():
+    xs = [0.4, 0.455, 0.5, 0.65, 1.0, 1.3, 2.6, 5.0, 13.0, 40.0]
+    return {"x": xs, "pmax": 13, "values": [[c2(sph_h(p, x)) for p in range(14)] for x in xs]}
+
+
+def jn_table():
+    zs = [0.3, complex(0.45, 0.1), 0.7, 1.3, complex(3.0, 0.5), complex(5.66, 22.6), 12.0, complex(0.2, -0.3)]
+    return {"z": [[z.real, z.imag] if isinstance(z, complex) else [z, 0.0] for z in zs], "pmax": 13,
+            "values": [[c2(sph_j(p, z)) for p in range(14)] for z in zs]}
+
+
+def w3j_table():
+    rows = []
+    for j1 in range(0, 5):
+        for j2 in range(0, 5):
+            for j3 in range(abs(j1 - j2), j1 + j2 + 1):
+                for m1 in range(-j1, j1 + 1):
+                    for m2 in range(-j2, j2 + 1):
+                        m3 = -m1 - m2
+                        if abs(m3) > j3:
+                            continue
+                        rows.append([j1, j2, j3, m1, m2, m3, float(wigner_3j(j1, j2, j3, m1, m2, m3))])
+    for (j1, j2, m1, m2) in [(6, 6, 0, 0), (6, 6, 3, -5), (6, 5, -6, 2), (12, 6, 4, -1), (6, 6, 6, 6), (9, 7, -2, 5)]:
+        for j3 in range(max(abs(j1 - j2), abs(m1 + m2)), j1 + j2 + 1):
+            rows.append([j1, j2, j3, m1, m2, -m1 - m2, float(wigner_3j(j1, j2, j3, m1, m2, -m1 - m2))])
+    return rows
+
+
+def riccati_d(f, n, z):
+    """[z f_n(z)]' = z f_{n-1}(z) - n f_n(z)."""
+    return z * f(n - 1, z) - n * f(n, z)
+
+
+def mie_bh(n, x, eps):
+    """Bohren-Huffman a_n, b_n of a nonmagnetic sphere (relative index m) in vacuum."""
+    x = mp.mpf(x)
+    m = mp.sqrt(mp.mpc(eps))
+    mx = m * x
+    jx = sph_j(n, x)
+    jmx = sph_j(n, mx)
+    hx = sph_h(n, x)
+    djx = riccati_d(sph_j, n, x)
+    dhx = riccati_d(lambda p, z: sph_h(p, z), n, x)
+    djmx = riccati_d(sph_j, n, mx)
+    a = (m ** 2 * jmx * djx - jx * djmx) / (m ** 2 * jmx * dhx - hx * djmx)
+    b = (jmx * djx - jx * djmx) / (jmx * dhx - hx * djmx)
+    return a, b
+
+
+def mie_table():
+    cases = [(0.3, complex(4, 0.1)), (0.455, complex(6, 0.2)), (1.1, complex(2.5, 0)), (2.0, complex(4, 0.1)),
+             (0.26, complex(16, 0.5))]
+    out = []
+    for x, eps in cases:
+        for n in range(1, 7):
+            a, b = mie_bh(n, x, eps)
+            out.append({"x": x, "eps": [eps.real, eps.imag], "n": n, "a": c2(a), "b": c2(b)})
+    return out
+
+
+def main():
+    data = {
+        "hankel": hankel_table(),
+        "jn": jn_table(),
+        "wigner3j": w3j_table(),
+        "mie_bohren_huffman": mie_table(),
+    }
+    with open(os.path.join(HERE, "special_functions.json"), "w") as f:
+        json.dump(data, f, indent=0)
+
+
+if __name__ == "__main__":
+    main()
