@@ -1,0 +1,220 @@
+/*
+ * treams_gpu.h -- C ABI of the MI355X (gfx950) cluster-coupling library.
+ *
+ * The library replaces the CPU hot path of treams-rs for clusters of homogeneous
+ * spheres in vacuum: the pairwise spherical-wave translation blocks, the cluster
+ * operator (I - T C), its solves (GMRES, LU) and adjoint, and the cross sections.
+ * Reference paths below are relative to crates/treams-core/src/ of treams-rs unless
+ * they start with python/.
+ *
+ * Conventions (identical to the reference):
+ *  - modes per sphere in sw::modes order (sw/mod.rs:114-125): l = 1..lmax, m = -l..l,
+ *    pol 1 (positive helicity) before pol 0; M = 2 lmax (lmax + 2) modes per sphere,
+ *    n = N M unknowns, particle-major (global index p * M + mode);
+ *  - complex numbers are interleaved (re, im) doubles (tg_c128), matrices column-major;
+ *  - translations use displacement = destination - source (sw/mod.rs:19-22);
+ *  - C has exactly zero self blocks (cluster/spheres.rs:186-220).
+ *
+ * Memory: vectors and matrices passed to compute calls are DEVICE pointers owned by
+ * the caller (e.g. torch tensors); geometry passed to tg_cluster_create is host
+ * memory, copied into the handle.  Every call runs on the handle's HIP stream and
+ * returns after its work is complete on that stream.
+ *
+ * Errors: every function returns a tg_status; tg_last_error() gives the message of
+ * the calling thread's last failure (lib.rs:379-411 variants).  Inputs are validated
+ * before any device work (cluster/spheres.rs:99-133, linalg/gmres.rs:64-78).
+ */
+#ifndef TREAMS_GPU_H
+#define TREAMS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } tg_c128;
+
+typedef enum {
+    TG_OK = 0,
+    TG_INVALID_INPUT = 1,     /* Error::InvalidInput */
+    TG_SPECIAL_FUNCTION = 2,  /* Error::SpecialFunction */
+    TG_NON_FINITE = 3,        /* Error::NonFinite */
+    TG_NOT_CONVERGED = 4,     /* Error::NotConverged */
+    TG_SINGULAR = 5,          /* Error::Singular */
+    TG_OUT_OF_MEMORY = 6,     /* Error::OutOfMemory */
+    TG_DEVICE_ERROR = 7       /* HIP runtime failure (no reference counterpart) */
+} tg_status;
+
+/* Storage precision of the assembled coupling matrix. */
+typedef enum { TG_STORE_NONE = 0, TG_STORE_C64 = 1, TG_STORE_C128 = 2 } tg_storage;
+
+/* Operator used by tg_cluster_apply / the solvers. */
+typedef enum {
+    TG_OP_MATRIX_FREE_F64 = 0, /* recompute each pair block in FP64 (cluster/iterative.rs:187-221) */
+    TG_OP_STORED = 1           /* stream the stored coupling (c64 or c128) */
+} tg_operator;
+
+/* Preconditioner of the GMRES solves (right preconditioning; no reference counterpart). */
+typedef enum {
+    TG_PC_NONE = 0,            /* the reference's unpreconditioned GMRES */
+    TG_PC_BLOCK_JACOBI = 1,    /* exact inverses of spatial sphere groups */
+    TG_PC_LEARNED = 2          /* block-Jacobi plus MLP-weighted near-field coupling */
+} tg_preconditioner;
+
+/* linalg/gmres.rs:22-43 plus the precision and preconditioner choices of this library. */
+typedef struct {
+    double rtol;            /* default 1e-10 */
+    double atol;            /* default 0 */
+    int restart;            /* default 30 */
+    int max_iterations;     /* default 300 */
+    int mixed_precision;    /* 1: c64 stored inner solves refined by FP64 matrix-free residuals */
+    int preconditioner;     /* tg_preconditioner */
+} tg_gmres_options;
+
+/* linalg/gmres.rs:81-93. */
+typedef struct {
+    int iterations;
+    int refinements;        /* outer FP64 refinement steps (mixed precision), else 0 */
+    double residual_norm;   /* true residual |b - A x| recomputed in FP64 */
+    double rhs_norm;
+} tg_convergence;
+
+typedef struct tg_cluster tg_cluster;
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *tg_last_error(void);
+
+/* Number of modes per sphere, 2 lmax (lmax + 2); replaces sw::modes (sw/mod.rs:114). */
+int tg_modes_per_sphere(int lmax);
+
+/*
+ * Host-side translation plan between all modes up to lmax in the helicity basis;
+ * replaces TranslationPlan::between (sw/plan.rs:71-170).  Fills, for each of the
+ * H*H = (lmax (lmax+2))^2 (source (l,m), destination (lambda,mu)) pairs in
+ * column-major order, the same-helicity coupling as
+ *   C_pol = sum_same w T[idx] + (2 pol - 1) i sum_cross w T[idx],
+ * T[p^2 + p + q] = z_p(kd) P_p^q(cos theta) e^{i q phi}.
+ * starts has H*H+1 entries, nsame H*H; with w == NULL returns the term count only.
+ */
+int tg_plan_terms(int lmax, int32_t *starts, uint8_t *nsame, double *w, uint16_t *hidx, int64_t *count);
+
+/* Wigner 3j symbol used by the plan (host; special/wigner.rs:263). */
+double tg_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3);
+
+/*
+ * Create a sphere cluster: homogeneous nonmagnetic spheres in vacuum.  Replaces
+ * IterativeSphereCluster::new (cluster/iterative.rs:53-86) and the shared parts of
+ * sphere_cluster / sphere_cluster_factor (cluster/spheres.rs:56-184): validation,
+ * Mie T-matrix blocks (coeffs/mie.rs:209; tmatrix/sphere.rs:43) and the plan.
+ * stream: a hipStream_t (NULL = the null stream).
+ */
+int tg_cluster_create(int lmax, double k0, int n_spheres, const double *radii, const tg_c128 *epsilon,
+                      const double *positions, void *stream, tg_cluster **out);
+int tg_cluster_destroy(tg_cluster *cluster);
+/* Unknowns n = N M. */
+int64_t tg_cluster_dimension(const tg_cluster *cluster);
+
+/*
+ * Per-sphere, per-degree 2x2 Mie blocks in mode order (cluster/iterative.rs:99-103,
+ * coeffs/mie.rs:123-130) copied to host: out has N * lmax * 4 entries, row-major 2x2.
+ */
+int tg_cluster_mie_blocks(tg_cluster *cluster, tg_c128 *out_host);
+
+/*
+ * Dense coupling C (n x n, column-major, c128) in the reference layout, into a
+ * caller device buffer: replaces coupling_matrix (cluster/spheres.rs:188-220).
+ * With regular != 0 writes the regular overlap R instead (sw/expansion.rs:116-130
+ * with Radial::Regular; self blocks are the identity).
+ */
+int tg_pair_matrix(tg_cluster *cluster, int regular, tg_c128 *out_dev);
+
+/*
+ * Translation blocks of a list of displacements (device double[3 * count]), each
+ * an M x M column-major c128 block: replaces TranslationPlan::evaluate (sw/plan.rs:282).
+ */
+int tg_translation_blocks(int lmax, double k, int64_t count, const double *displacements_dev, int regular,
+                          tg_c128 *out_dev, void *stream);
+
+/*
+ * Assemble and keep the coupling on the device in the helicity-split layout
+ * (two N H x N H column-major matrices, H = M / 2), in c64 or c128.  No reference
+ * counterpart for the storage precision; the values are those of coupling_matrix.
+ */
+int tg_cluster_assemble(tg_cluster *cluster, int storage);
+
+/*
+ * y = (I - T C) x, or (I - C^H T^H) x with adjoint != 0, for `cols` column-major
+ * vectors of length n (device c128): replaces IterativeSphereCluster::apply
+ * (cluster/iterative.rs:225-232).  op selects the matrix-free FP64 or stored path.
+ */
+int tg_cluster_apply(tg_cluster *cluster, const tg_c128 *x_dev, tg_c128 *y_dev, int cols, int adjoint, int op);
+
+/* y = C x or C^H x (coupling only), cluster/iterative.rs:145-159. */
+int tg_cluster_coupling(tg_cluster *cluster, const tg_c128 *x_dev, tg_c128 *y_dev, int cols, int adjoint, int op);
+
+/* y = T x or T^H x with the Mie blocks, cluster/iterative.rs:99-136. */
+int tg_cluster_local(tg_cluster *cluster, const tg_c128 *x_dev, tg_c128 *y_dev, int cols, int adjoint);
+
+/*
+ * Local regular coefficients of a unit plane wave of helicity pol (1 positive,
+ * 0 negative) along the real direction dir (normalized): pw/expand.rs:23-66 times
+ * the position phase of pw/field.rs:20-26.  out_dev has n entries.
+ */
+int tg_cluster_plane_wave(tg_cluster *cluster, const double *dir, int pol, tg_c128 *out_dev);
+
+/*
+ * Solve (I - T C) X = T B for `cols` incident columns (device c128, n x cols):
+ * replaces IterativeSphereCluster::solve (cluster/iterative.rs:264-284).  With
+ * options->mixed_precision the inner GMRES runs on the c64 stored coupling and an
+ * FP64 matrix-free residual refines the answer; convergence is always certified
+ * by the FP64 true residual.  conv_host has cols entries.  With adjoint != 0
+ * solves (I - T C)^H X = B (cluster/iterative.rs:225-230, no T applied to B).
+ */
+int tg_cluster_solve(tg_cluster *cluster, const tg_c128 *incident_dev, int cols, const tg_gmres_options *options,
+                     int adjoint, tg_c128 *x_dev, tg_convergence *conv_host);
+
+/*
+ * y = R x or R^H x with the regular overlap R applied matrix-free (self blocks I):
+ * replaces `_radiation_overlap @ v` (python/treams_rs/_tmatrix.py:404-411).
+ */
+int tg_cluster_overlap(tg_cluster *cluster, const tg_c128 *x_dev, tg_c128 *y_dev, int cols, int adjoint);
+
+/*
+ * Cross sections of one illumination (python/treams_rs/_framework_tmatrix.py:231-289):
+ * scattering = Re(x^H R (x / k^2)) 0.5 / flux, extinction = -Re(a^H x / k^2) 0.5 / flux,
+ * with the regular overlap R applied matrix-free.
+ */
+int tg_cluster_cross_sections(tg_cluster *cluster, const tg_c128 *incident_dev, const tg_c128 *x_dev, double flux,
+                              double *scattering, double *extinction);
+
+/*
+ * Gradients of a real loss L with dL = Re<G, dX> through X = (I - T C)^{-1} T B:
+ * replaces IterativeResidual::pullback (cluster/iterative.rs:460-488) and
+ * SphereClusterResidual::pullback (cluster/spheres.rs:321-378).  Outputs (host):
+ * k0 gradient, radii[N], epsilon[N] (c128), positions[3N]; incident_grad_dev (n x cols)
+ * receives T^H Y.  One adjoint solve per column with the given options.
+ */
+int tg_cluster_pullback(tg_cluster *cluster, const tg_c128 *incident_dev, const tg_c128 *x_dev,
+                        const tg_c128 *cotangent_dev, int cols, const tg_gmres_options *options, double *grad_k0,
+                        double *grad_radii, tg_c128 *grad_eps, double *grad_positions, tg_c128 *incident_grad_dev,
+                        tg_convergence *conv_host);
+
+/*
+ * Position gradients of Re<Gbar, R> for the regular overlap with cotangent
+ * Gbar = sum_c u_c v_c^H (device c128 n x cols each): the R-dependence of the
+ * scattering cross section (sw/expansion.rs ExpansionResidual::pullback).
+ */
+int tg_cluster_overlap_pullback(tg_cluster *cluster, const tg_c128 *u_dev, const tg_c128 *v_dev, int cols,
+                                double *grad_positions);
+
+/* Kernel timing (HIP events on the handle's stream) for measurement. */
+int tg_cluster_timing_enable(tg_cluster *cluster, int enable);
+/* Total milliseconds and launch count of a timed kernel by name; 0 launches if unknown. */
+int tg_cluster_timing_get(tg_cluster *cluster, const char *kernel, double *total_ms, int64_t *launches);
+int tg_cluster_timing_reset(tg_cluster *cluster);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,226 @@
+"""GPU parity of the HIP path against the FP64 oracle and the reference goldens.
+
+Tolerances (FP64 paths): 1e-12 relative on translation blocks, Mie blocks and
+operator applications; 1e-10 on solutions; the reference example goldens at the
+digits they print.  c64 stored paths: 2e-6 relative (single-precision storage).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from __graft_entry__ import load_package
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    p = load_package()
+    assert torch.cuda.is_available()
+    return p
+
+
+def _rel(a, b):
+    a = a.detach().cpu().numpy() if torch.is_tensor(a) else np.asarray(a)
+    b = b.detach().cpu().numpy() if torch.is_tensor(b) else np.asarray(b)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _cluster(pkg, oracle, N=6, lmax=3, seed=1, side=3.0, eps=4 + 0.1j, k0=1.3):
+    radii, e, pos = pkg.random_cluster(N, side, 0.1, (0.2, 0.3), eps, seed=seed)
+    return pkg.IterativeSphereCluster(lmax, k0, radii, e, pos), oracle.Cluster(lmax, k0, radii, e, pos)
+
+
+def test_mie_blocks(pkg, oracle):
+    radii = np.array([0.15, 0.25, 0.35, 0.3])
+    eps = np.array([2.0 + 0.0j, 4 + 0.1j, 6 + 0.2j, 16 + 0.5j])
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1.2, 0], [0, 0, 1.5]])
+    cl = pkg.IterativeSphereCluster(6, 1.3, radii, eps, pos)
+    got = cl.mie_blocks()
+    ref = oracle.tmatrix_blocks(6, 1.3, radii, eps)
+    for i in range(4):
+        for l in range(6):
+            scale = np.abs(ref[i, l]).max()
+            assert np.abs(got[i, l] - ref[i, l]).max() <= 1e-12 * max(scale, 1e-30) + 1e-300, (i, l)
+
+
+def test_mie_metallic_golden(pkg, golden_dir):
+    g = json.load(open(os.path.join(golden_dir, "reference_examples.json")))["metallic_mie"]
+    # size x = k0 r = 80 with k0 = 1: one sphere, lmax up to 12 (rows l = 1, 3)
+    cl = pkg.IterativeSphereCluster(12, 1.0, [80.0], complex(*g["epsilon"]), [[0, 0, 0]])
+    blocks = cl.mie_blocks()[0]
+    for l, dre, dim, ore, oim in g["rows"]:
+        if l > 12:
+            continue
+        d, o = complex(dre, dim), complex(ore, oim)
+        expected = np.array([[d, o], [o, d]])
+        assert np.all(np.abs(blocks[l - 1] - expected) <= 2e-13 + 2e-11 * np.abs(expected)), l
+
+
+@pytest.mark.parametrize("regular", [False, True])
+@pytest.mark.parametrize("lmax", [1, 3, 6])
+def test_translation_blocks(pkg, oracle, lmax, regular):
+    rng = np.random.default_rng(lmax)
+    disp = rng.normal(size=(5, 3))
+    disp[0] = [0, 0, 0.7]
+    disp[1] = [0, 0, -1.1]
+    disp[2] *= 0.3  # kr ~ 0.4 (below the reference's recurrence domain)
+    got = pkg.translation_blocks(lmax, 1.3, disp, regular).cpu().numpy()
+    for i, d in enumerate(disp):
+        ref = oracle.translation_block(lmax, 1.3, d, regular)
+        assert np.abs(got[i] - ref).max() <= 1e-12 * np.abs(ref).max(), i
+    if regular:
+        origin = pkg.translation_blocks(lmax, 1.3, [[0.0, 0.0, 0.0]], True).cpu().numpy()[0]
+        assert np.abs(origin - np.eye(origin.shape[0])).max() <= 1e-13
+
+
+@pytest.mark.parametrize("N,lmax", [(8, 3), (5, 6)])
+def test_dense_coupling_and_overlap(pkg, oracle, N, lmax):
+    cl, ref = _cluster(pkg, oracle, N=N, lmax=lmax, seed=N + lmax)
+    C = cl.coupling_matrix().cpu().numpy()
+    assert _rel(C, ref.C) <= 1e-12
+    for i in range(N):  # self blocks exactly zero
+        M = ref.M
+        assert np.all(C[i * M:(i + 1) * M, i * M:(i + 1) * M] == 0)
+    R = cl.coupling_matrix(regular=True).cpu().numpy()
+    assert _rel(R, ref.R) <= 1e-12
+
+
+def test_operator_apply_matrix_free_and_stored(pkg, oracle):
+    cl, ref = _cluster(pkg, oracle, N=7, lmax=4, seed=3)
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(ref.n, 3)) + 1j * rng.normal(size=(ref.n, 3))
+    for adjoint in (False, True):
+        got = cl.apply(x, adjoint=adjoint)
+        exp = np.stack([ref.apply(x[:, c], adjoint) for c in range(3)], axis=1)
+        assert _rel(got, exp) <= 1e-12
+        gc = cl.coupling(x, adjoint=adjoint)
+        ec = (ref.C.conj().T if adjoint else ref.C) @ x
+        assert _rel(gc, ec) <= 1e-12
+    assert _rel(cl.local(x), ref.T @ x) <= 1e-13
+    assert _rel(cl.local(x, adjoint=True), ref.T.conj().T @ x) <= 1e-13
+    cl.assemble("c128")
+    assert _rel(cl.coupling(x, stored=True), ref.C @ x) <= 1e-12
+    assert _rel(cl.coupling(x, adjoint=True, stored=True), ref.C.conj().T @ x) <= 1e-12
+    cl.assemble("c64")
+    assert _rel(cl.coupling(x, stored=True), ref.C @ x) <= 2e-6
+    assert _rel(cl.apply(x[:, 0], stored=True), ref.apply(x[:, 0])) <= 2e-6
+
+
+def test_overlap_apply(pkg, oracle):
+    cl, ref = _cluster(pkg, oracle, N=5, lmax=3, seed=9)
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=ref.n) + 1j * rng.normal(size=ref.n)
+    assert _rel(cl.overlap(x), ref.R @ x) <= 1e-12
+    assert _rel(cl.overlap(x, adjoint=True), ref.R.conj().T @ x) <= 1e-12
+
+
+@pytest.mark.parametrize("direction", [[0, 0, 1], [0, 0, -1], [1, 0, 0], [0.2, -0.1, 1.0]])
+@pytest.mark.parametrize("pol", [0, 1])
+def test_plane_wave(pkg, oracle, direction, pol):
+    cl, ref = _cluster(pkg, oracle, N=4, lmax=6, seed=2)
+    assert _rel(cl.plane_wave(direction, pol), ref.plane_wave(direction, pol)) <= 1e-13
+
+
+def test_solve_fp64_matches_dense(pkg, oracle):
+    cl, ref = _cluster(pkg, oracle, N=8, lmax=3, seed=5, side=2.0)
+    a = torch.stack([cl.plane_wave([0, 0, 1], 1), cl.plane_wave([0, 0, -1], 0)], dim=1)
+    x, conv = cl.solve(a, rtol=1e-12)
+    an = a.cpu().numpy()
+    for c in range(2):
+        xr = ref.solve(an[:, c])
+        assert _rel(x[:, c], xr) <= 1e-10
+        its, resid, rhs = conv[c]
+        assert resid <= 1e-12 * rhs
+        # the certificate is the true residual
+        r = np.linalg.norm(ref.operator() @ x[:, c].cpu().numpy() - oracle.apply_local(ref.blocks, an[:, c]))
+        assert abs(r - resid) <= 1e-3 * resid + 1e-14 * rhs
+
+
+def test_solve_mixed_precision_reaches_fp64_tolerance(pkg, oracle):
+    cl, ref = _cluster(pkg, oracle, N=12, lmax=4, seed=8, side=3.5)
+    a = cl.plane_wave([0, 0, 1], 1)
+    x, conv = cl.solve(a, rtol=1e-10, restart=50, mixed_precision=True, convergence="full")
+    xr = ref.solve(a.cpu().numpy())
+    assert conv[0].residual_norm <= 1e-10 * conv[0].rhs_norm
+    assert conv[0].refinements >= 1
+    assert _rel(x, xr) <= 1e-8
+
+
+@pytest.mark.parametrize("pc", ["block_jacobi"])
+def test_preconditioned_solve_same_answer(pkg, oracle, pc):
+    cl, ref = _cluster(pkg, oracle, N=16, lmax=3, seed=11, side=3.0)
+    a = cl.plane_wave([0, 0, 1], 1)
+    x0, c0 = cl.solve(a, rtol=1e-10)
+    x1, c1 = cl.solve(a, rtol=1e-10, preconditioner=pc)
+    assert _rel(x1, x0) <= 1e-8
+    assert c1[0][1] <= 1e-10 * c1[0][2]
+    assert c1[0][0] <= c0[0][0]
+
+
+def test_not_converged_raises(pkg, oracle):
+    cl, _ = _cluster(pkg, oracle, N=8, lmax=3, seed=5, side=2.0)
+    a = cl.plane_wave([0, 0, 1], 1)
+    with pytest.raises(pkg.NotConverged):
+        cl.solve(a, rtol=1e-14, max_iterations=2)
+
+
+def test_invalid_input_rejected(pkg):
+    with pytest.raises(ValueError):
+        pkg.IterativeSphereCluster(3, 1.3, [0.3, 0.3], 4.0, [[0, 0, 0], [0.5, 0, 0]])  # overlapping
+    with pytest.raises(ValueError):
+        pkg.IterativeSphereCluster(3, -1.0, [0.3], 4.0, [[0, 0, 0]])
+
+
+def _example_cluster(pkg, golden_dir):
+    g = json.load(open(os.path.join(golden_dir, "reference_examples.json")))["cluster_example"]
+    pos = (220 / np.sqrt(24)) * np.array(
+        [[np.sqrt(8), 0, -1], [-np.sqrt(2), np.sqrt(6), -1], [-np.sqrt(2), -np.sqrt(6), -1], [0, 0, 3]])
+    k0 = 2 * np.pi / g["k0_over_2pi_inverse"]
+    return pkg.IterativeSphereCluster(g["lmax"], k0, g["radii"], complex(*g["epsilon"]), pos), g
+
+
+def test_cluster_example_golden(pkg, golden_dir):
+    cl, g = _example_cluster(pkg, golden_dir)
+    a = cl.plane_wave(g["direction"], g["pol"])
+    x, _ = cl.solve(a, rtol=1e-12)
+    sca, ext = cl.cross_sections(a, x)
+    e = g["expected"]
+    assert float(f"{sca:.6g}") == e["scattering"]
+    assert float(f"{ext:.6g}") == e["extinction"]
+    assert float(f"{ext - sca:.6g}") == e["absorption"]
+
+
+def test_reverse_gradient_golden(pkg, oracle, golden_dir):
+    g = json.load(open(os.path.join(golden_dir, "reference_examples.json")))["reverse_gradient_example"]
+    idx = np.arange(8)
+    positions = np.column_stack((180 * (idx % 2), 190 * ((idx // 2) % 2), 200 * (idx // 4))).astype(float)
+    positions += np.column_stack((9 * np.sin(idx), 7 * np.cos(idx), 11 * np.sin(2 * idx)))
+    positions -= positions.mean(axis=0)
+    radii = 45.0 + 3 * (idx % 4)
+    cl = pkg.IterativeSphereCluster(g["lmax"], 2 * np.pi / 700, radii, complex(*g["epsilon"]), positions)
+    sca, gr, gp, _ = pkg.scattering_gradient(cl, g["direction"], g["pol"], rtol=1e-13)
+    assert round(sca, 8) == g["expected_scattering"]
+    expected = np.array(g["expected_gradient_radius_xyz"])
+    assert np.abs(np.round(np.column_stack((gr, gp)), 6) - expected).max() <= 1.5e-6
+
+
+def test_pullback_matches_oracle(pkg, oracle):
+    cl, ref = _cluster(pkg, oracle, N=6, lmax=3, seed=13, side=2.5, eps=None)
+    rng = np.random.default_rng(2)
+    a = np.stack([ref.plane_wave([0, 0, 1], 1), ref.plane_wave([1, 0, 0], 0)], axis=1)
+    x, ctx, _ = cl.record(a, rtol=1e-13)
+    G = rng.normal(size=a.shape) + 1j * rng.normal(size=a.shape)
+    gk, gr, ge, gp, ginc, conv = ctx.pullback(G)
+    xr = np.stack([ref.solve(a[:, c]) for c in range(2)], axis=1)
+    r = ref.pullback(a, xr, G)
+    assert np.abs(gr - r["radii"]).max() <= 1e-9 * np.abs(r["radii"]).max()
+    assert np.abs(ge - r["eps"]).max() <= 1e-9 * np.abs(r["eps"]).max()
+    assert np.abs(gp - r["positions"]).max() <= 1e-9 * np.abs(r["positions"]).max()
+    assert abs(gk - r["k0"]) <= 1e-9 * abs(r["k0"])
+    assert _rel(ginc, r["incident"]) <= 1e-9
+    # zero total position gradient (translation invariance, iterative.rs:832-835)
+    assert np.abs(gp.sum(axis=0)).max() <= 1e-9 * np.abs(gp).max()

@@ -1,0 +1,294 @@
+"""Sphere clusters on the GPU, with the reference's Python-facing interface.
+
+``IterativeSphereCluster`` mirrors ``treams_rs._native.IterativeSphereCluster``
+(treams-py/src/iterative.rs:45-209, python/treams_rs/_native.pyi:910-1170): the
+same constructor arguments, ``solve`` with ``rtol/atol/restart/max_iterations``
+returning ``(X, [(iterations, residual_norm, rhs_norm)])``, ``record`` and
+``IterativeContext.pullback``.  Vectors follow the reference layout: particle-major,
+``sw::modes`` order inside a particle, one column per illumination.
+
+Arrays may be NumPy or torch; results are torch tensors on the device.  All
+numerical work runs in libtreams_gpu.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import GmresOptions as _Options, Convergence as _Conv, check, lib
+
+#: linalg/gmres.rs:34-43 / python/treams_rs/iterative.py:302-310
+DEFAULT_RTOL = 1e-10
+DEFAULT_ATOL = 0.0
+DEFAULT_RESTART = 30
+DEFAULT_MAX_ITERATIONS = 300
+
+_PC = {"none": _lib.TG_PC_NONE, "block_jacobi": _lib.TG_PC_BLOCK_JACOBI, "learned": _lib.TG_PC_LEARNED}
+
+
+def _device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("treams-rs_amd needs a ROCm GPU (torch.cuda is not available)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream_ptr() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _columns(a, n: int) -> tuple[torch.Tensor, bool]:
+    """(cols, n) contiguous complex128 device tensor = column-major n x cols."""
+    t = torch.as_tensor(a)
+    vector = t.ndim == 1
+    if vector:
+        t = t[:, None]
+    if t.ndim != 2 or t.shape[0] != n:
+        raise ValueError(f"expected shape ({n},) or ({n}, columns), got {tuple(t.shape)}")
+    t = t.to(device=_device(), dtype=torch.complex128)
+    return t.T.contiguous(), vector
+
+
+def _back(cm: torch.Tensor, vector: bool) -> torch.Tensor:
+    out = cm.T
+    return out[:, 0].contiguous() if vector else out.contiguous()
+
+
+@dataclass(frozen=True)
+class Convergence:
+    """linalg/gmres.rs:81-93 (plus the mixed-precision refinement count)."""
+
+    iterations: int
+    residual_norm: float
+    rhs_norm: float
+    refinements: int = 0
+
+    def as_tuple(self):
+        return (self.iterations, self.residual_norm, self.rhs_norm)
+
+
+class IterativeSphereCluster:
+    """Homogeneous nonmagnetic spheres in vacuum, solved on the GPU
+    (cluster/iterative.rs:53-86)."""
+
+    def __init__(self, lmax: int, k0: float, radii, epsilon, positions):
+        radii = np.ascontiguousarray(np.asarray(radii, np.float64).reshape(-1))
+        eps = np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, np.complex128), radii.shape))
+        pos = np.ascontiguousarray(np.asarray(positions, np.float64).reshape(-1, 3))
+        if pos.shape[0] != radii.size:
+            raise ValueError("require one position per sphere")
+        self.lmax, self.k0 = int(lmax), float(k0)
+        self.radii, self.epsilon, self.positions = radii.copy(), eps.copy(), pos.copy()
+        self._dev = _device()
+        h = ctypes.c_void_p()
+        check(lib().tg_cluster_create(self.lmax, self.k0, radii.size, radii.ctypes.data, eps.ctypes.data,
+                                      pos.ctypes.data, ctypes.c_void_p(_stream_ptr()), ctypes.byref(h)))
+        self._h = h
+        self.N = radii.size
+        self.modes_per_sphere = lib().tg_modes_per_sphere(self.lmax)
+        self.n = int(lib().tg_cluster_dimension(h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().tg_cluster_destroy(h)
+            self._h = None
+
+    # ---------------------------------------------------------------- queries
+    @property
+    def dimension(self) -> int:
+        return self.n
+
+    def mie_blocks(self) -> np.ndarray:
+        """(N, lmax, 2, 2) Mie blocks in mode order."""
+        out = np.zeros(self.N * self.lmax * 4, np.complex128)
+        check(lib().tg_cluster_mie_blocks(self._h, out.ctypes.data))
+        return out.reshape(self.N, self.lmax, 2, 2)
+
+    def coupling_matrix(self, regular: bool = False) -> torch.Tensor:
+        """Dense C (or the regular overlap R) as an (n, n) device tensor
+        (cluster/spheres.rs:188-220)."""
+        out = torch.empty((self.n, self.n), dtype=torch.complex128, device=self._dev)
+        check(lib().tg_pair_matrix(self._h, int(regular), out.data_ptr()))
+        return out.T  # memory is column-major
+
+    def assemble(self, storage: str = "c64") -> None:
+        """Keep the coupling on the device (helicity-split, c64 or c128)."""
+        s = {"c64": _lib.TG_STORE_C64, "c128": _lib.TG_STORE_C128}[storage]
+        check(lib().tg_cluster_assemble(self._h, s))
+
+    # -------------------------------------------------------------- operators
+    def _apply(self, fn, x, *args):
+        cm, vector = _columns(x, self.n)
+        out = torch.empty_like(cm)
+        check(fn(self._h, cm.data_ptr(), out.data_ptr(), cm.shape[0], *args))
+        return _back(out, vector)
+
+    def apply(self, x, adjoint: bool = False, stored: bool = False):
+        """(I - T C) x or (I - C^H T^H) x (cluster/iterative.rs:225-232)."""
+        return self._apply(lib().tg_cluster_apply, x, int(adjoint), int(stored))
+
+    def coupling(self, x, adjoint: bool = False, stored: bool = False):
+        return self._apply(lib().tg_cluster_coupling, x, int(adjoint), int(stored))
+
+    def local(self, x, adjoint: bool = False):
+        return self._apply(lib().tg_cluster_local, x, int(adjoint))
+
+    def overlap(self, x, adjoint: bool = False):
+        """R x or R^H x, the regular overlap (python/treams_rs/_tmatrix.py:404-411)."""
+        return self._apply(lib().tg_cluster_overlap, x, int(adjoint))
+
+    def plane_wave(self, direction, pol) -> torch.Tensor:
+        """Regular local coefficients of a unit plane wave (pw/expand.rs:23-66)."""
+        if isinstance(pol, str):
+            pol = {"positive_helicity": 1, "negative_helicity": 0}[pol]
+        d = np.ascontiguousarray(np.asarray(direction, np.float64).reshape(3))
+        out = torch.empty(self.n, dtype=torch.complex128, device=self._dev)
+        check(lib().tg_cluster_plane_wave(self._h, d.ctypes.data, int(pol), out.data_ptr()))
+        return out
+
+    # ------------------------------------------------------------------ solves
+    @staticmethod
+    def options(rtol=DEFAULT_RTOL, atol=DEFAULT_ATOL, restart=DEFAULT_RESTART,
+                max_iterations=DEFAULT_MAX_ITERATIONS, mixed_precision=False, preconditioner="none") -> _Options:
+        return _Options(float(rtol), float(atol), int(restart), int(max_iterations), int(bool(mixed_precision)),
+                        _PC[preconditioner])
+
+    def solve(self, incident, *, rtol=DEFAULT_RTOL, atol=DEFAULT_ATOL, restart=DEFAULT_RESTART,
+              max_iterations=DEFAULT_MAX_ITERATIONS, mixed_precision=False, preconditioner="none",
+              adjoint=False, convergence="tuples"):
+        """Solve (I - T C) X = T B (cluster/iterative.rs:264-284).  Returns
+        (X, [(iterations, residual_norm, rhs_norm), ...])."""
+        cm, vector = _columns(incident, self.n)
+        cols = cm.shape[0]
+        out = torch.empty_like(cm)
+        conv = (_Conv * cols)()
+        opts = self.options(rtol, atol, restart, max_iterations, mixed_precision, preconditioner)
+        check(lib().tg_cluster_solve(self._h, cm.data_ptr(), cols, ctypes.byref(opts), int(adjoint),
+                                     out.data_ptr(), conv))
+        reports = [Convergence(c.iterations, c.residual_norm, c.rhs_norm, c.refinements) for c in conv]
+        if convergence == "tuples":
+            reports = [r.as_tuple() for r in reports]
+        return _back(out, vector), reports
+
+    def record(self, incident, **options):
+        """Solve and keep what the pullback needs (cluster/iterative.rs:288-300)."""
+        x, conv = self.solve(incident, **options)
+        return x, IterativeContext(self, incident, x, options), conv
+
+    def cross_sections(self, incident, x, flux: float = 0.5):
+        """(scattering, extinction) of one illumination
+        (python/treams_rs/_framework_tmatrix.py:231-289)."""
+        a, _ = _columns(incident, self.n)
+        xc, _ = _columns(x, self.n)
+        sca, ext = ctypes.c_double(), ctypes.c_double()
+        check(lib().tg_cluster_cross_sections(self._h, a.data_ptr(), xc.data_ptr(), float(flux), ctypes.byref(sca),
+                                              ctypes.byref(ext)))
+        return sca.value, ext.value
+
+    def overlap_pullback(self, u, v) -> np.ndarray:
+        """Position gradient of Re<sum_c u_c v_c^H, R> (regular overlap pullback)."""
+        uc, _ = _columns(u, self.n)
+        vc, _ = _columns(v, self.n)
+        g = np.zeros(3 * self.N)
+        check(lib().tg_cluster_overlap_pullback(self._h, uc.data_ptr(), vc.data_ptr(), uc.shape[0], g.ctypes.data))
+        return g.reshape(self.N, 3)
+
+    def timing(self, enable: bool = True):
+        check(lib().tg_cluster_timing_enable(self._h, int(enable)))
+
+    def timing_get(self, kernel: str):
+        ms, count = ctypes.c_double(), ctypes.c_int64()
+        check(lib().tg_cluster_timing_get(self._h, kernel.encode(), ctypes.byref(ms), ctypes.byref(count)))
+        return ms.value, count.value
+
+    def timing_reset(self):
+        check(lib().tg_cluster_timing_reset(self._h))
+
+
+class IterativeContext:
+    """Residual of a recorded solve (cluster/iterative.rs:306-489)."""
+
+    def __init__(self, cluster: IterativeSphereCluster, incident, x, options):
+        self.cluster = cluster
+        self.incident = torch.as_tensor(incident).to(cluster._dev, torch.complex128)
+        self.x = x
+        self.options = dict(options)
+
+    def pullback(self, cotangent):
+        """Gradients from the cotangent of X: returns
+        (k0, radii[N], eps[N], positions[N, 3], incident, convergence)."""
+        c = self.cluster
+        a, vector = _columns(self.incident, c.n)
+        xc, _ = _columns(self.x, c.n)
+        g, _ = _columns(cotangent, c.n)
+        cols = a.shape[0]
+        inc = torch.empty_like(a)
+        gk = ctypes.c_double()
+        gr = np.zeros(c.N)
+        ge = np.zeros(c.N, np.complex128)
+        gp = np.zeros(3 * c.N)
+        conv = (_Conv * cols)()
+        opts = c.options(**{k: v for k, v in self.options.items() if k in (
+            "rtol", "atol", "restart", "max_iterations", "mixed_precision", "preconditioner")})
+        check(lib().tg_cluster_pullback(c._h, a.data_ptr(), xc.data_ptr(), g.data_ptr(), cols, ctypes.byref(opts),
+                                        ctypes.byref(gk), gr.ctypes.data, ge.ctypes.data, gp.ctypes.data,
+                                        inc.data_ptr(), conv))
+        reports = [(v.iterations, v.residual_norm, v.rhs_norm) for v in conv]
+        return gk.value, gr, ge, gp.reshape(c.N, 3), _back(inc, vector), reports
+
+
+def scattering_gradient(cluster: IterativeSphereCluster, direction, pol, **options):
+    """sigma_sca of one plane wave and its gradients with respect to the radii and the
+    positions: through the solve (cluster pullback), the regular overlap R and the
+    incident position phases e^{i k.r_j}
+    (python/treams_rs/_framework_tmatrix.py:231-289 differentiated)."""
+    c = cluster
+    kvec = c.k0 * np.asarray(direction, np.float64) / np.linalg.norm(direction)
+    a = c.plane_wave(direction, pol)
+    x, ctx, conv = c.record(a, **options)
+    w = 1.0 / c.k0 ** 2
+    rx = c.overlap(x)
+    rhx = c.overlap(x, adjoint=True)
+    sca = float(torch.vdot(x, rx).real) * w
+    G = (rx + rhx) * w
+    gk, gr, ge, gp, abar = ctx.pullback(G)
+    gp = gp + c.overlap_pullback(x * w, x)
+    ar = a.reshape(c.N, -1)
+    ab = abar.reshape(c.N, -1)
+    phase = torch.sum(torch.conj(ab) * 1j * ar, dim=1).real.cpu().numpy()
+    gp = gp + phase[:, None] * kvec[None, :]
+    return sca, gr, gp, dict(k0=gk, epsilon=ge, convergence=conv)
+
+
+def plan_terms(lmax: int):
+    """Host translation plan of the library (tg_plan_terms): starts, nsame, w, hidx."""
+    H = lmax * (lmax + 2)
+    count = ctypes.c_int64()
+    check(lib().tg_plan_terms(lmax, None, None, None, None, ctypes.byref(count)))
+    starts = np.zeros(H * H + 1, np.int32)
+    nsame = np.zeros(H * H, np.uint8)
+    w = np.zeros(count.value, np.float64)
+    hidx = np.zeros(count.value, np.uint16)
+    check(lib().tg_plan_terms(lmax, starts.ctypes.data, nsame.ctypes.data, w.ctypes.data, hidx.ctypes.data,
+                              ctypes.byref(count)))
+    return starts, nsame, w, hidx
+
+
+def wigner3j(j1, j2, j3, m1, m2, m3) -> float:
+    return lib().tg_wigner3j(j1, j2, j3, m1, m2, m3)
+
+
+def translation_blocks(lmax: int, k: float, displacements, regular: bool = False) -> torch.Tensor:
+    """M x M translation blocks (sw/plan.rs:282) for a list of displacements; returns
+    (count, M, M) indexed [destination, source]."""
+    d = torch.as_tensor(np.ascontiguousarray(np.asarray(displacements, np.float64).reshape(-1, 3)),
+                        device=_device())
+    M = lib().tg_modes_per_sphere(lmax)
+    out = torch.empty((d.shape[0], M, M), dtype=torch.complex128, device=d.device)
+    check(lib().tg_translation_blocks(lmax, float(k), d.shape[0], d.data_ptr(), int(regular), out.data_ptr(),
+                                      ctypes.c_void_p(_stream_ptr())))
+    return out.transpose(1, 2)

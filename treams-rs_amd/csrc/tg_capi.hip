@@ -1,0 +1,619 @@
+// C ABI of the cluster-coupling library (declarations and reference citations in
+// include/treams_gpu.h).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tg_gmres.h"
+#include "tg_internal.h"
+#include "tg_precond.h"
+
+namespace tg {
+
+static thread_local std::string g_error;
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+}
+
+int DevBuf::ensure(size_t need) {
+    if (need <= bytes && ptr) return TG_OK;
+    if (ptr) {
+        hipError_t e = hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        if (e != hipSuccess) {
+            set_error("hipFree: %s", hipGetErrorString(e));
+            return TG_DEVICE_ERROR;
+        }
+    }
+    if (need == 0) return TG_OK;
+    hipError_t e = hipMalloc(&ptr, need);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        set_error("device allocation of %zu bytes failed: %s", need, hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? TG_OUT_OF_MEMORY : TG_DEVICE_ERROR;
+    }
+    bytes = need;
+    return TG_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int Timer::begin(const char *name, hipStream_t s, Rec &r) {
+    if (!enabled) return TG_OK;
+    TG_HIP(hipEventCreate(&r.a));
+    TG_HIP(hipEventCreate(&r.b));
+    TG_HIP(hipEventRecord(r.a, s));
+    (void)name;
+    return TG_OK;
+}
+
+int Timer::end(const char *name, hipStream_t s, Rec &r) {
+    if (!enabled) return TG_OK;
+    TG_HIP(hipEventRecord(r.b, s));
+    pending[name].push_back(r);
+    return TG_OK;
+}
+
+int Timer::collect() {
+    for (auto &kv : pending) {
+        auto &tot = totals[kv.first];
+        for (auto &r : kv.second) {
+            TG_HIP(hipEventSynchronize(r.b));
+            float ms = 0.f;
+            TG_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            tot.first += ms;
+            tot.second += 1;
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+    pending.clear();
+    return TG_OK;
+}
+
+// Upload a host plan into one device buffer and fill the device view.
+static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp) {
+    const size_t HH = (size_t)hp.H * hp.H;
+    const size_t nterms = hp.w.size();
+    // transposed plan: terms grouped by harmonic index, entry order kept
+    std::vector<int32_t> tstarts(hp.table_len + 1, 0), tentry(nterms);
+    std::vector<double> tw(nterms);
+    for (size_t t = 0; t < nterms; ++t) tstarts[hp.hidx[t] + 1]++;
+    for (int i = 0; i < hp.table_len; ++i) tstarts[i + 1] += tstarts[i];
+    std::vector<int32_t> fill(tstarts.begin(), tstarts.end() - 1);
+    for (size_t e = 0; e < HH; ++e)
+        for (int32_t t = hp.starts[e]; t < hp.starts[e + 1]; ++t) {
+            int32_t dst = fill[hp.hidx[t]]++;
+            int cross = (t - hp.starts[e]) >= hp.nsame[e];
+            tentry[dst] = (int32_t)e | (cross ? (int32_t)0x40000000 : 0);
+            tw[dst] = hp.w[t];
+        }
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t off_starts = 0;
+    size_t off_nsame = al(off_starts + sizeof(int32_t) * (HH + 1));
+    size_t off_w = al(off_nsame + HH);
+    size_t off_hidx = al(off_w + sizeof(double) * nterms);
+    size_t off_refl = al(off_hidx + sizeof(uint16_t) * nterms);
+    size_t off_ts = al(off_refl + sizeof(double) * hp.table_len);
+    size_t off_te = al(off_ts + sizeof(int32_t) * (hp.table_len + 1));
+    size_t off_tw = al(off_te + sizeof(int32_t) * nterms);
+    size_t total = al(off_tw + sizeof(double) * nterms);
+    TG_TRY(buf.ensure(total));
+    char *base = buf.as<char>();
+    std::vector<char> host(total, 0);
+    memcpy(host.data() + off_starts, hp.starts.data(), sizeof(int32_t) * (HH + 1));
+    memcpy(host.data() + off_nsame, hp.nsame.data(), HH);
+    memcpy(host.data() + off_w, hp.w.data(), sizeof(double) * nterms);
+    memcpy(host.data() + off_hidx, hp.hidx.data(), sizeof(uint16_t) * nterms);
+    memcpy(host.data() + off_refl, hp.refl.data(), sizeof(double) * hp.table_len);
+    memcpy(host.data() + off_ts, tstarts.data(), sizeof(int32_t) * (hp.table_len + 1));
+    memcpy(host.data() + off_te, tentry.data(), sizeof(int32_t) * nterms);
+    memcpy(host.data() + off_tw, tw.data(), sizeof(double) * nterms);
+    TG_HIP(hipMemcpy(base, host.data(), total, hipMemcpyHostToDevice));
+    dp.H = hp.H;
+    dp.L2 = hp.L2;
+    dp.T = hp.table_len;
+    dp.starts = (const int32_t *)(base + off_starts);
+    dp.nsame = (const uint8_t *)(base + off_nsame);
+    dp.w = (const double *)(base + off_w);
+    dp.hidx = (const uint16_t *)(base + off_hidx);
+    dp.refl = (const double *)(base + off_refl);
+    dp.tstarts = (const int32_t *)(base + off_ts);
+    dp.tentry = (const int32_t *)(base + off_te);
+    dp.tw = (const double *)(base + off_tw);
+    return TG_OK;
+}
+
+// y = (I - T C) x or (I - C^H T^H) x, split layout.
+int apply_operator(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, int op) {
+    const int64_t count = (int64_t)cols * c->n;
+    TG_TRY(c->ws[6].ensure(sizeof(cd) * count * 2));
+    cd *t1 = c->ws[6].as<cd>();
+    cd *t2 = t1 + count;
+    auto coupling = [&](const cd *in, cd *out) -> int {
+        return op == TG_OP_STORED ? coupling_stored(c, in, out, cols) : coupling_matrix_free(c, in, out, cols, 0);
+    };
+    if (!adjoint) {
+        TG_TRY(coupling(x, t1));
+        TG_TRY(local_apply(c, t1, y, cols, 0, x));
+        return TG_OK;
+    }
+    // C^H z = P conj(C (P conj(z))) with z = T^H x
+    TG_TRY(local_apply(c, x, t1, cols, 1, nullptr));
+    TG_TRY(reflect_conj(c, t1, t2, cols));
+    TG_TRY(coupling(t2, t1));
+    TG_TRY(reflect_conj(c, t1, t2, cols));
+    TG_TRY(vec_axpby(c->stream, cd(1.0), x, cd(-1.0), t2, count));
+    TG_HIP(hipMemcpyAsync(y, t2, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream));
+    return TG_OK;
+}
+
+static int check_cluster(const tg_cluster *c) {
+    if (!c) {
+        set_error("null cluster handle");
+        return TG_INVALID_INPUT;
+    }
+    return TG_OK;
+}
+
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" {
+
+const char *tg_last_error(void) { return g_error.c_str(); }
+
+int tg_modes_per_sphere(int lmax) { return 2 * lmax * (lmax + 2); }
+
+double tg_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) { return wigner3j(j1, j2, j3, m1, m2, m3); }
+
+int tg_plan_terms(int lmax, int32_t *starts, uint8_t *nsame, double *w, uint16_t *hidx, int64_t *count) {
+    HostPlan hp;
+    if (build_plan(lmax, hp)) {
+        set_error("require 1 <= lmax <= %d", kMaxLmax);
+        return TG_INVALID_INPUT;
+    }
+    if (count) *count = (int64_t)hp.w.size();
+    if (starts) memcpy(starts, hp.starts.data(), sizeof(int32_t) * hp.starts.size());
+    if (nsame) memcpy(nsame, hp.nsame.data(), hp.nsame.size());
+    if (w) memcpy(w, hp.w.data(), sizeof(double) * hp.w.size());
+    if (hidx) memcpy(hidx, hp.hidx.data(), sizeof(uint16_t) * hp.hidx.size());
+    return TG_OK;
+}
+
+int tg_cluster_create(int lmax, double k0, int n_spheres, const double *radii, const tg_c128 *epsilon,
+                      const double *positions, void *stream, tg_cluster **out) {
+    if (!out || !radii || !epsilon || !positions) {
+        set_error("null argument");
+        return TG_INVALID_INPUT;
+    }
+    *out = nullptr;
+    if (lmax < 1 || lmax > kMaxLmax) {
+        set_error("require 1 <= lmax <= %d", kMaxLmax);
+        return TG_INVALID_INPUT;
+    }
+    // cluster/spheres.rs:99-133
+    bool ok = std::isfinite(k0) && k0 > 0.0 && n_spheres > 0;
+    for (int i = 0; ok && i < n_spheres; ++i) {
+        ok = std::isfinite(radii[i]) && radii[i] > 0.0 && std::isfinite(epsilon[i].re) &&
+             std::isfinite(epsilon[i].im);
+        for (int a = 0; a < 3; ++a) ok = ok && std::isfinite(positions[3 * i + a]);
+    }
+    if (!ok) {
+        set_error("require positive finite k0 and radii, and matching finite permittivities and positions");
+        return TG_INVALID_INPUT;
+    }
+    for (int i = 0; i < n_spheres; ++i)
+        for (int j = 0; j < i; ++j) {
+            double d = 0.0;
+            for (int a = 0; a < 3; ++a) d = std::hypot(d, positions[3 * i + a] - positions[3 * j + a]);
+            if (!(d > radii[i] + radii[j]) || !std::isfinite(d)) {
+                set_error("spherical particles must have finite separation and must not touch or overlap");
+                return TG_INVALID_INPUT;
+            }
+        }
+    tg_cluster *c = new tg_cluster();
+    c->lmax = lmax;
+    c->N = n_spheres;
+    c->M = 2 * lmax * (lmax + 2);
+    c->H = lmax * (lmax + 2);
+    c->n = (int64_t)c->N * c->M;
+    c->nh = (int64_t)c->N * c->H;
+    c->k0 = k0;
+    c->stream = (hipStream_t)stream;
+    c->radii.assign(radii, radii + n_spheres);
+    c->positions.assign(positions, positions + 3 * n_spheres);
+    for (int i = 0; i < n_spheres; ++i) c->eps.push_back(cd(epsilon[i].re, epsilon[i].im));
+    int st = TG_OK;
+    do {
+        if (build_plan(lmax, c->hplan)) {
+            set_error("plan construction failed");
+            st = TG_INVALID_INPUT;
+            break;
+        }
+        if ((st = upload_plan(c->hplan, c->b_plan, c->plan))) break;
+        std::vector<double> geo(6 * (size_t)n_spheres);
+        memcpy(geo.data(), positions, sizeof(double) * 3 * n_spheres);
+        memcpy(geo.data() + 3 * n_spheres, radii, sizeof(double) * n_spheres);
+        for (int i = 0; i < n_spheres; ++i) {
+            geo[4 * (size_t)n_spheres + 2 * i] = epsilon[i].re;
+            geo[4 * (size_t)n_spheres + 2 * i + 1] = epsilon[i].im;
+        }
+        if ((st = c->b_pos.ensure(sizeof(double) * geo.size()))) break;
+        hipError_t e = hipMemcpy(c->b_pos.ptr, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            set_error("upload: %s", hipGetErrorString(e));
+            st = TG_DEVICE_ERROR;
+            break;
+        }
+        if ((st = c->b_mie.ensure(sizeof(cd) * 12 * (size_t)n_spheres * lmax))) break;
+        if ((st = launch_mie(c))) break;
+        std::vector<cd> mie(4 * (size_t)n_spheres * lmax);
+        e = hipMemcpyAsync(mie.data(), c->b_mie.ptr, sizeof(cd) * mie.size(), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            set_error("Mie blocks: %s", hipGetErrorString(e));
+            st = TG_DEVICE_ERROR;
+            break;
+        }
+        for (auto &v : mie)
+            if (!std::isfinite(v.re) || !std::isfinite(v.im)) {
+                set_error("singular Mie coefficients");
+                st = TG_SINGULAR;
+                break;
+            }
+    } while (0);
+    if (st) {
+        tg_cluster_destroy(c);
+        return st;
+    }
+    *out = c;
+    return TG_OK;
+}
+
+int tg_cluster_destroy(tg_cluster *c) {
+    if (!c) return TG_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    destroy_preconditioner(c);
+    c->b_plan.release();
+    c->b_pos.release();
+    c->b_mie.release();
+    c->b_C.release();
+    for (auto &w : c->ws) w.release();
+    c->timer.collect();
+    delete c;
+    return TG_OK;
+}
+
+int64_t tg_cluster_dimension(const tg_cluster *c) { return c ? c->n : 0; }
+
+int tg_cluster_mie_blocks(tg_cluster *c, tg_c128 *out) {
+    TG_TRY(check_cluster(c));
+    TG_HIP(hipMemcpyAsync(out, c->b_mie.ptr, sizeof(cd) * 4 * c->N * c->lmax, hipMemcpyDeviceToHost, c->stream));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_pair_matrix(tg_cluster *c, int regular, tg_c128 *out) {
+    TG_TRY(check_cluster(c));
+    TG_TRY(launch_assemble_ref(c, regular, (cd *)out));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_translation_blocks(int lmax, double k, int64_t count, const double *disp, int regular, tg_c128 *out,
+                          void *stream) {
+    HostPlan hp;
+    if (build_plan(lmax, hp) || !(std::isfinite(k)) || count < 0) {
+        set_error("invalid lmax, wavenumber or count");
+        return TG_INVALID_INPUT;
+    }
+    DevBuf buf;
+    DevPlan dp;
+    int st = upload_plan(hp, buf, dp);
+    if (!st) st = launch_translation_blocks(lmax, dp, k, count, disp, regular, (cd *)out, (hipStream_t)stream);
+    if (!st) {
+        hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+        if (e != hipSuccess) {
+            set_error("%s", hipGetErrorString(e));
+            st = TG_DEVICE_ERROR;
+        }
+    }
+    buf.release();
+    return st;
+}
+
+int tg_cluster_assemble(tg_cluster *c, int storage) {
+    TG_TRY(check_cluster(c));
+    if (storage != TG_STORE_C64 && storage != TG_STORE_C128) {
+        set_error("storage must be TG_STORE_C64 or TG_STORE_C128");
+        return TG_INVALID_INPUT;
+    }
+    size_t elem = storage == TG_STORE_C64 ? 8 : 16;
+    size_t bytes = elem * 2 * (size_t)c->nh * (size_t)c->nh;
+    c->storage = TG_STORE_NONE;
+    TG_TRY(c->b_C.ensure(bytes));
+    TG_TRY(launch_assemble_split(c, storage, c->b_C.ptr));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    c->storage = storage;
+    return TG_OK;
+}
+
+// reference-layout in/out wrappers around split-layout work
+static int with_split(tg_cluster *c, const tg_c128 *x, tg_c128 *y, int cols,
+                      const std::function<int(const cd *, cd *)> &body) {
+    if (cols < 1) {
+        set_error("cols must be positive");
+        return TG_INVALID_INPUT;
+    }
+    const int64_t count = (int64_t)cols * c->n;
+    TG_TRY(c->ws[7].ensure(sizeof(cd) * count * 2));
+    cd *xs = c->ws[7].as<cd>(), *ys = xs + count;
+    TG_TRY(ref_to_split(c, (const cd *)x, xs, cols));
+    TG_TRY(body(xs, ys));
+    TG_TRY(split_to_ref(c, ys, (cd *)y, cols));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_cluster_coupling(tg_cluster *c, const tg_c128 *x, tg_c128 *y, int cols, int adjoint, int op) {
+    TG_TRY(check_cluster(c));
+    return with_split(c, x, y, cols, [&](const cd *xs, cd *ys) -> int {
+        auto cp = [&](const cd *in, cd *out) -> int {
+            return op == TG_OP_STORED ? coupling_stored(c, in, out, cols) : coupling_matrix_free(c, in, out, cols, 0);
+        };
+        if (!adjoint) return cp(xs, ys);
+        const int64_t count = (int64_t)cols * c->n;
+        TG_TRY(c->ws[6].ensure(sizeof(cd) * count * 2));
+        cd *t1 = c->ws[6].as<cd>(), *t2 = t1 + count;
+        TG_TRY(reflect_conj(c, xs, t1, cols));
+        TG_TRY(cp(t1, t2));
+        return reflect_conj(c, t2, ys, cols);
+    });
+}
+
+int tg_cluster_overlap(tg_cluster *c, const tg_c128 *x, tg_c128 *y, int cols, int adjoint) {
+    TG_TRY(check_cluster(c));
+    return with_split(c, x, y, cols, [&](const cd *xs, cd *ys) -> int {
+        if (!adjoint) return coupling_matrix_free(c, xs, ys, cols, 1);
+        const int64_t count = (int64_t)cols * c->n;
+        TG_TRY(c->ws[6].ensure(sizeof(cd) * count * 2));
+        cd *t1 = c->ws[6].as<cd>(), *t2 = t1 + count;
+        TG_TRY(reflect_conj(c, xs, t1, cols));
+        TG_TRY(coupling_matrix_free(c, t1, t2, cols, 1));
+        return reflect_conj(c, t2, ys, cols);
+    });
+}
+
+int tg_cluster_local(tg_cluster *c, const tg_c128 *x, tg_c128 *y, int cols, int adjoint) {
+    TG_TRY(check_cluster(c));
+    return with_split(c, x, y, cols,
+                      [&](const cd *xs, cd *ys) -> int { return local_apply(c, xs, ys, cols, adjoint, nullptr); });
+}
+
+int tg_cluster_apply(tg_cluster *c, const tg_c128 *x, tg_c128 *y, int cols, int adjoint, int op) {
+    TG_TRY(check_cluster(c));
+    return with_split(c, x, y, cols,
+                      [&](const cd *xs, cd *ys) -> int { return apply_operator(c, xs, ys, cols, adjoint, op); });
+}
+
+int tg_cluster_plane_wave(tg_cluster *c, const double *dir, int pol, tg_c128 *out) {
+    TG_TRY(check_cluster(c));
+    if (!dir || !(std::isfinite(dir[0]) && std::isfinite(dir[1]) && std::isfinite(dir[2])) ||
+        (dir[0] == 0 && dir[1] == 0 && dir[2] == 0) || (pol != 0 && pol != 1)) {
+        set_error("plane wave needs a finite nonzero direction and pol 0 or 1");
+        return TG_INVALID_INPUT;
+    }
+    TG_TRY(c->ws[7].ensure(sizeof(cd) * c->n));
+    cd *tmp = c->ws[7].as<cd>();
+    TG_TRY(plane_wave(c, dir, pol, tmp));
+    TG_TRY(split_to_ref(c, tmp, (cd *)out, 1));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+}  // extern "C"
+
+namespace tg {
+
+// Solve in the split layout: b (already T-applied for forward solves) -> x.
+int solve_split(tg_cluster *c, const cd *b, cd *x, int cols, const tg_gmres_options &o, int adjoint,
+                std::vector<tg_convergence> &conv) {
+    const int64_t count = (int64_t)cols * c->n;
+    GmresParams prm;
+    prm.rtol = o.rtol;
+    prm.atol = o.atol;
+    prm.restart = o.restart;
+    prm.max_iterations = o.max_iterations;
+    OpFn pc_fn;
+    const OpFn *pc = nullptr;
+    if (o.preconditioner != TG_PC_NONE) {
+        TG_TRY(ensure_preconditioner(c, o.preconditioner, o.mixed_precision ? TG_STORE_C64 : TG_STORE_C128));
+        pc_fn = [c, adjoint](const cd *in, cd *out, int nc) -> int { return apply_preconditioner(c, in, out, nc, adjoint); };
+        pc = &pc_fn;
+    }
+    if (!o.mixed_precision) {
+        const int op = c->storage == TG_STORE_C128 ? TG_OP_STORED : TG_OP_MATRIX_FREE_F64;
+        OpFn A = [c, adjoint, op](const cd *in, cd *out, int nc) -> int {
+            return apply_operator(c, in, out, nc, adjoint, op);
+        };
+        int its = 0;
+        int st = gmres_run(c, A, pc, b, x, cols, prm, conv, &its);
+        return st;
+    }
+    // Mixed precision: inner GMRES on the c64 stored operator, outer FP64 residuals.
+    if (c->storage != TG_STORE_C64) TG_TRY(tg_cluster_assemble(c, TG_STORE_C64));
+    OpFn A32 = [c, adjoint](const cd *in, cd *out, int nc) -> int {
+        return apply_operator(c, in, out, nc, adjoint, TG_OP_STORED);
+    };
+    DevBuf outer;
+    TG_TRY(outer.ensure(sizeof(cd) * count * 3));
+    cd *r = outer.as<cd>(), *d = r + count, *ax = d + count;
+    std::vector<double> bn(cols), rn(cols);
+    TG_TRY(vec_norms(c->stream, b, c->n, cols, c->n, bn.data(), c->ws[4]));
+    TG_HIP(hipMemsetAsync(x, 0, sizeof(cd) * count, c->stream));
+    TG_HIP(hipMemcpyAsync(r, b, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream));
+    conv.assign(cols, tg_convergence{0, 0, 0.0, 0.0});
+    int iterations = 0, refinements = 0;
+    int st = TG_OK;
+    while (true) {
+        TG_TRY(vec_norms(c->stream, r, c->n, cols, c->n, rn.data(), c->ws[4]));
+        double worst_ratio = 0.0;
+        bool done = true;
+        for (int k = 0; k < cols; ++k) {
+            double tol = std::max(o.atol, o.rtol * bn[k]);
+            conv[k].rhs_norm = bn[k];
+            conv[k].residual_norm = rn[k];
+            conv[k].iterations = iterations;
+            conv[k].refinements = refinements;
+            if (!std::isfinite(rn[k])) {
+                set_error("non-finite residual in mixed-precision refinement");
+                outer.release();
+                return TG_NOT_CONVERGED;
+            }
+            if (rn[k] > tol) done = false;
+            if (rn[k] > 0) worst_ratio = std::max(worst_ratio, rn[k] > 0 ? tol / rn[k] : 1.0);
+        }
+        if (done) break;
+        if (iterations >= o.max_iterations || refinements >= 20) {
+            set_error("mixed-precision GMRES did not converge after %d iterations (%d refinements)", iterations,
+                      refinements);
+            outer.release();
+            return TG_NOT_CONVERGED;
+        }
+        // inner relative tolerance: what the outer still needs, not below FP32 accuracy
+        double eta = 1e-6;
+        double need = 0.5 * worst_ratio;
+        for (int k = 0; k < cols; ++k) {
+            double tol = std::max(o.atol, o.rtol * bn[k]);
+            if (rn[k] > tol) need = std::min(need, 0.5 * tol / rn[k]);
+        }
+        if (need > eta) eta = need;
+        GmresParams ip = prm;
+        ip.rtol = eta;
+        ip.atol = 0.0;
+        ip.max_iterations = o.max_iterations - iterations;
+        std::vector<tg_convergence> ic;
+        int its = 0;
+        st = gmres_run(c, A32, pc, r, d, cols, ip, ic, &its);
+        iterations += its;
+        if (st != TG_OK && st != TG_NOT_CONVERGED) break;
+        st = TG_OK;
+        TG_TRY(vec_axpby(c->stream, cd(1.0), d, cd(1.0), x, count));
+        ++refinements;
+        const int op64 = TG_OP_MATRIX_FREE_F64;
+        TG_TRY(apply_operator(c, x, ax, cols, adjoint, op64));
+        TG_HIP(hipMemcpyAsync(r, b, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream));
+        TG_TRY(vec_axpby(c->stream, cd(-1.0), ax, cd(1.0), r, count));
+        if (its == 0) {
+            set_error("mixed-precision inner solve made no progress");
+            st = TG_NOT_CONVERGED;
+            break;
+        }
+    }
+    outer.release();
+    return st;
+}
+
+}  // namespace tg
+
+extern "C" {
+
+int tg_cluster_solve(tg_cluster *c, const tg_c128 *incident, int cols, const tg_gmres_options *o, int adjoint,
+                     tg_c128 *x, tg_convergence *conv_host) {
+    TG_TRY(check_cluster(c));
+    if (!o || !incident || !x || cols < 1) {
+        set_error("null argument or no columns");
+        return TG_INVALID_INPUT;
+    }
+    // linalg/gmres.rs:64-78
+    if (!std::isfinite(o->rtol) || !std::isfinite(o->atol) || o->rtol < 0 || o->atol < 0 || o->rtol + o->atol == 0 ||
+        o->restart <= 0 || o->max_iterations <= 0) {
+        set_error("GMRES needs finite nonnegative tolerances, a positive tolerance, restart and iteration limit");
+        return TG_INVALID_INPUT;
+    }
+    const int64_t count = (int64_t)cols * c->n;
+    DevBuf io;
+    TG_TRY(io.ensure(sizeof(cd) * count * 3));
+    cd *a = io.as<cd>(), *b = a + count, *xs = b + count;
+    int st = ref_to_split(c, (const cd *)incident, a, cols);
+    if (!st) st = adjoint ? (hipMemcpyAsync(b, a, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream) == hipSuccess
+                                 ? TG_OK
+                                 : TG_DEVICE_ERROR)
+                          : local_apply(c, a, b, cols, 0, nullptr);  // rhs = T a
+    std::vector<tg_convergence> conv;
+    if (!st) st = solve_split(c, b, xs, cols, *o, adjoint, conv);
+    if (!st || st == TG_NOT_CONVERGED) {
+        int s2 = split_to_ref(c, xs, (cd *)x, cols);
+        if (!st) st = s2;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !st) st = TG_DEVICE_ERROR;
+    if (conv_host)
+        for (int k = 0; k < cols && k < (int)conv.size(); ++k) conv_host[k] = conv[k];
+    io.release();
+    return st;
+}
+
+int tg_cluster_cross_sections(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, double flux, double *sca,
+                              double *ext) {
+    TG_TRY(check_cluster(c));
+    if (!std::isfinite(flux) || flux <= 0) {
+        set_error("require finite incident coefficients and positive flux");
+        return TG_INVALID_INPUT;
+    }
+    const int64_t n = c->n;
+    DevBuf io;
+    TG_TRY(io.ensure(sizeof(cd) * n * 3));
+    cd *xs = io.as<cd>(), *rx = xs + n, *as = rx + n;
+    int st = ref_to_split(c, (const cd *)x, xs, 1);
+    if (!st) st = ref_to_split(c, (const cd *)incident, as, 1);
+    if (!st) st = coupling_matrix_free(c, xs, rx, 1, 1);  // R x (self blocks identity)
+    cd d1, d2;
+    if (!st) st = vec_dots(c->stream, xs, 0, 1, rx, n, 1, n, n, &d1, c->ws[4]);
+    if (!st) st = vec_dots(c->stream, as, 0, 1, xs, n, 1, n, n, &d2, c->ws[4]);
+    io.release();
+    if (st) return st;
+    const double w = 1.0 / (c->k0 * c->k0);
+    *sca = d1.re * w * 0.5 / flux;
+    *ext = -d2.re * w * 0.5 / flux;
+    return TG_OK;
+}
+
+int tg_cluster_timing_enable(tg_cluster *c, int enable) {
+    TG_TRY(check_cluster(c));
+    c->timer.enabled = enable != 0;
+    return TG_OK;
+}
+
+int tg_cluster_timing_get(tg_cluster *c, const char *kernel, double *total_ms, int64_t *launches) {
+    TG_TRY(check_cluster(c));
+    TG_TRY(c->timer.collect());
+    auto it = c->timer.totals.find(kernel);
+    *total_ms = it == c->timer.totals.end() ? 0.0 : it->second.first;
+    *launches = it == c->timer.totals.end() ? 0 : it->second.second;
+    return TG_OK;
+}
+
+int tg_cluster_timing_reset(tg_cluster *c) {
+    TG_TRY(check_cluster(c));
+    TG_TRY(c->timer.collect());
+    c->timer.totals.clear();
+    return TG_OK;
+}
+
+}  // extern "C"

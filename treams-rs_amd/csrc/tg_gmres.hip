@@ -1,0 +1,197 @@
+// Restarted GMRES on the device, after linalg/gmres.rs:119-347: per column an
+// Arnoldi basis with two Gram-Schmidt passes (here classical, batched into two
+// dot/update kernels per pass), complex Givens rotations on the host, and
+// convergence judged only on the recomputed true residual b - A x.  All columns
+// advance in lockstep so one operator launch serves every column (the stored
+// coupling is streamed once per step whatever the column count); finished columns
+// carry zero vectors.  Right preconditioning keeps residual norms unpreconditioned.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <functional>
+#include <vector>
+
+#include "tg_gmres.h"
+
+namespace tg {
+
+using zc = std::complex<double>;
+
+static int check_finite_norms(const std::vector<double> &v) {
+    for (double x : v)
+        if (!std::isfinite(x)) return 0;
+    return 1;
+}
+
+int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x, int cols, const GmresParams &prm,
+              std::vector<tg_convergence> &conv, int *total_iterations) {
+    hipStream_t s = c->stream;
+    const int64_t n = c->n;
+    const int restart = (int)std::min<int64_t>(std::min(prm.restart, prm.max_iterations), n);
+    // buffers: V [cols][restart+1][n], W, X packed [cols][n], R residual [cols][n]
+    const int64_t csv = (int64_t)(restart + 1) * n;
+    TG_TRY(c->ws[2].ensure(sizeof(cd) * (size_t)cols * csv));
+    TG_TRY(c->ws[3].ensure(sizeof(cd) * (size_t)cols * n * 3));
+    TG_TRY(c->ws[5].ensure(sizeof(cd) * (size_t)cols * (restart + 2) + sizeof(double) * cols));
+    cd *V = c->ws[2].as<cd>();
+    cd *W = c->ws[3].as<cd>();
+    cd *X = W + (size_t)cols * n;
+    cd *R = X + (size_t)cols * n;
+    cd *coef = c->ws[5].as<cd>();
+    double *scale = reinterpret_cast<double *>(coef + (size_t)cols * (restart + 2));
+
+    std::vector<double> bnorm(cols), tol(cols), rn(cols);
+    TG_TRY(vec_norms(s, b, n, cols, n, bnorm.data(), c->ws[4]));
+    conv.assign(cols, tg_convergence{0, 0, 0.0, 0.0});
+    for (int k = 0; k < cols; ++k) {
+        tol[k] = std::max(prm.atol, prm.rtol * bnorm[k]);
+        conv[k].rhs_norm = bnorm[k];
+    }
+    TG_HIP(hipMemsetAsync(x, 0, sizeof(cd) * cols * n, s));
+    TG_HIP(hipMemcpyAsync(R, b, sizeof(cd) * cols * n, hipMemcpyDeviceToDevice, s));
+    int iterations = 0;
+    std::vector<zc> H((size_t)cols * (restart + 1) * restart), g((size_t)cols * (restart + 1));
+    std::vector<double> cs((size_t)cols * restart);
+    std::vector<zc> sn((size_t)cols * restart);
+    std::vector<int> used(cols), extending(cols), cycle(cols);
+    std::vector<cd> hbuf((size_t)cols * (restart + 1));
+    std::vector<double> nxt(cols), scale_h(cols);
+    while (true) {
+        TG_TRY(vec_norms(s, R, n, cols, n, rn.data(), c->ws[4]));
+        int any = 0;
+        for (int k = 0; k < cols; ++k) {
+            conv[k].residual_norm = rn[k];
+            conv[k].iterations = iterations;
+            if (std::isfinite(rn[k]) && rn[k] <= tol[k]) {
+                cycle[k] = 0;
+                continue;
+            }
+            if (!std::isfinite(rn[k]) || iterations >= prm.max_iterations) {
+                set_error("GMRES did not converge after %d iterations: residual %e, tolerance %e", iterations, rn[k],
+                          tol[k]);
+                if (total_iterations) *total_iterations = iterations;
+                return TG_NOT_CONVERGED;
+            }
+            cycle[k] = 1;
+            any = 1;
+        }
+        if (!any) break;
+        // V[k][0] = r_k / |r_k| for cycling columns, 0 otherwise
+        for (int k = 0; k < cols; ++k) scale_h[k] = cycle[k] ? 1.0 / rn[k] : 0.0;
+        TG_HIP(hipMemcpyAsync(scale, scale_h.data(), sizeof(double) * cols, hipMemcpyHostToDevice, s));
+        for (int k = 0; k < cols; ++k) TG_TRY(vec_scale_cols(s, scale + k, R + (int64_t)k * n, V + k * csv, n, 1));
+        std::fill(H.begin(), H.end(), zc(0.0));
+        std::fill(g.begin(), g.end(), zc(0.0));
+        for (int k = 0; k < cols; ++k) {
+            g[(size_t)k * (restart + 1)] = cycle[k] ? rn[k] : 0.0;
+            used[k] = 0;
+            extending[k] = cycle[k];
+        }
+        for (int j = 0; j < restart; ++j) {
+            // X = V[:, j]; W = A M^{-1} X
+            for (int k = 0; k < cols; ++k)
+                TG_HIP(hipMemcpyAsync(X + (int64_t)k * n, V + k * csv + (int64_t)j * n, sizeof(cd) * n,
+                                      hipMemcpyDeviceToDevice, s));
+            if (Minv) {
+                TG_TRY((*Minv)(X, R, cols));
+                TG_TRY(A(R, W, cols));
+            } else {
+                TG_TRY(A(X, W, cols));
+            }
+            // two classical Gram-Schmidt passes against V[:, 0..j]
+            for (int pass = 0; pass < 2; ++pass) {
+                TG_TRY(vec_dots(s, V, n, j + 1, W, n, cols, csv, n, hbuf.data(), c->ws[4]));
+                for (int k = 0; k < cols; ++k)
+                    for (int i = 0; i <= j; ++i) {
+                        cd h = extending[k] ? hbuf[(size_t)k * (j + 1) + i] : cd(0.0);
+                        hbuf[(size_t)k * (j + 1) + i] = h;
+                        H[((size_t)k * restart + j) * (restart + 1) + i] += zc(h.re, h.im);
+                    }
+                TG_HIP(hipMemcpyAsync(coef, hbuf.data(), sizeof(cd) * cols * (j + 1), hipMemcpyHostToDevice, s));
+                TG_TRY(vec_update(s, V, n, j + 1, coef, W, n, cols, csv, n, -1.0));
+            }
+            TG_TRY(vec_norms(s, W, n, cols, n, nxt.data(), c->ws[4]));
+            int more = 0;
+            for (int k = 0; k < cols; ++k) {
+                double &sc = scale_h[k];
+                sc = 0.0;
+                if (extending[k]) {
+                    zc *h = &H[((size_t)k * restart + j) * (restart + 1)];
+                    h[j + 1] = nxt[k];
+                    for (int i = 0; i < j; ++i) {
+                        double cc = cs[(size_t)k * restart + i];
+                        zc ss = sn[(size_t)k * restart + i];
+                        zc a = h[i], bb = h[i + 1];
+                        h[i] = cc * a + ss * bb;
+                        h[i + 1] = -std::conj(ss) * a + cc * bb;
+                    }
+                    zc a = h[j], bb = h[j + 1];
+                    double len = std::hypot(std::abs(a), std::abs(bb));
+                    if (!std::isfinite(len) || len == 0.0) {
+                        set_error("GMRES Arnoldi breakdown before convergence");
+                        return TG_NOT_CONVERGED;
+                    }
+                    zc phase = std::abs(a) == 0.0 ? zc(1.0) : a / std::abs(a);
+                    double cc = std::abs(a) / len;
+                    zc ss = phase * std::conj(bb) / len;
+                    h[j] = phase * len;
+                    h[j + 1] = 0.0;
+                    cs[(size_t)k * restart + j] = cc;
+                    sn[(size_t)k * restart + j] = ss;
+                    zc *gk = &g[(size_t)k * (restart + 1)];
+                    gk[j + 1] = -std::conj(ss) * gk[j];
+                    gk[j] *= cc;
+                    used[k] = j + 1;
+                    conv[k].iterations = iterations + 1;
+                    bool stop = std::abs(gk[j + 1]) <= tol[k] || nxt[k] == 0.0 ||
+                                iterations + 1 >= prm.max_iterations;
+                    if (!stop) {
+                        sc = 1.0 / nxt[k];
+                        more = 1;
+                    } else {
+                        extending[k] = 0;
+                    }
+                }
+            }
+            TG_HIP(hipMemcpyAsync(scale, scale_h.data(), sizeof(double) * cols, hipMemcpyHostToDevice, s));
+            ++iterations;
+            if (j + 1 < restart || more) {
+                if (j + 1 < restart)
+                    for (int k = 0; k < cols; ++k)
+                        TG_TRY(vec_scale_cols(s, scale + k, W + (int64_t)k * n, V + k * csv + (int64_t)(j + 1) * n, n,
+                                              1));
+            }
+            if (!more) break;
+        }
+        // back substitution and x += M^{-1} V y
+        for (int k = 0; k < cols; ++k) {
+            zc *gk = &g[(size_t)k * (restart + 1)];
+            for (int i = used[k] - 1; i >= 0; --i) {
+                zc tail = 0.0;
+                for (int jj = i + 1; jj < used[k]; ++jj) tail += H[((size_t)k * restart + jj) * (restart + 1) + i] * gk[jj];
+                gk[i] = (gk[i] - tail) / H[((size_t)k * restart + i) * (restart + 1) + i];
+            }
+            for (int i = 0; i < restart; ++i) {
+                zc v = (i < used[k] && cycle[k]) ? gk[i] : zc(0.0);
+                hbuf[(size_t)k * restart + i] = cd(v.real(), v.imag());
+            }
+        }
+        TG_HIP(hipMemcpyAsync(coef, hbuf.data(), sizeof(cd) * cols * restart, hipMemcpyHostToDevice, s));
+        TG_HIP(hipMemsetAsync(X, 0, sizeof(cd) * cols * n, s));
+        TG_TRY(vec_update(s, V, n, restart, coef, X, n, cols, csv, n, 1.0));
+        if (Minv) {
+            TG_TRY((*Minv)(X, W, cols));
+            TG_TRY(vec_axpby(s, cd(1.0), W, cd(1.0), x, (int64_t)cols * n));
+        } else {
+            TG_TRY(vec_axpby(s, cd(1.0), X, cd(1.0), x, (int64_t)cols * n));
+        }
+        // true residual R = b - A x
+        TG_TRY(A(x, R, cols));
+        TG_TRY(vec_axpby(s, cd(1.0), b, cd(-1.0), R, (int64_t)cols * n));
+    }
+    if (total_iterations) *total_iterations = iterations;
+    (void)check_finite_norms;
+    return TG_OK;
+}
+
+}  // namespace tg

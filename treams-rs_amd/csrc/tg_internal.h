@@ -1,0 +1,120 @@
+// Internal declarations of the gfx950 cluster-coupling library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/treams_gpu.h"
+#include "tg_complex.h"
+#include "tg_plan.h"
+
+namespace tg {
+
+void set_error(const char *fmt, ...);
+
+#define TG_HIP(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
+        if (_e != hipSuccess) {                                                                   \
+            ::tg::set_error("HIP error %s at %s:%d (%s)", hipGetErrorString(_e), __FILE__, __LINE__, \
+                            #expr);                                                               \
+            return _e == hipErrorOutOfMemory ? TG_OUT_OF_MEMORY : TG_DEVICE_ERROR;                 \
+        }                                                                                         \
+    } while (0)
+
+#define TG_TRY(expr)                   \
+    do {                               \
+        int _s = (expr);               \
+        if (_s != TG_OK) return _s;    \
+    } while (0)
+
+// Device view of the translation plan.
+struct DevPlan {
+    int H, L2, T;                // modes per helicity, largest harmonic degree, table length
+    const int32_t *starts;       // H*H + 1
+    const uint8_t *nsame;        // H*H
+    const double *w;             // terms
+    const uint16_t *hidx;        // terms
+    const double *refl;          // T
+    // transposed plan for pullbacks: for each harmonic index, its (entry, weight, kind)
+    const int32_t *tstarts;      // T + 1
+    const int32_t *tentry;       // entry index e (kind in bit 31)
+    const double *tw;            // weight
+};
+
+// Growable device buffer.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <class T> T *as() const { return reinterpret_cast<T *>(ptr); }
+};
+
+struct Timer {
+    bool enabled = false;
+    struct Rec { hipEvent_t a, b; };
+    std::map<std::string, std::vector<Rec>> pending;
+    std::map<std::string, std::pair<double, int64_t>> totals;
+    int begin(const char *name, hipStream_t s, Rec &r);
+    int end(const char *name, hipStream_t s, Rec &r);
+    int collect();
+};
+
+struct Preconditioner;  // defined in tg_precond.hip
+
+}  // namespace tg
+
+struct tg_cluster {
+    int lmax = 0, N = 0, M = 0, H = 0;
+    int64_t n = 0, nh = 0;
+    double k0 = 0;
+    hipStream_t stream = nullptr;
+    std::vector<double> radii, positions;
+    std::vector<cd> eps;
+    tg::HostPlan hplan;
+    tg::DevPlan plan{};
+    tg::DevBuf b_plan, b_pos, b_mie;   // b_mie: value, d/dsize, d/deps blocks
+    int storage = TG_STORE_NONE;
+    tg::DevBuf b_C;                    // stored split coupling: [2][nh][nh] (c64 or c128)
+    tg::DevBuf ws[8];                  // scratch
+    tg::Timer timer;
+    tg::Preconditioner *pc = nullptr;
+    int pc_kind = TG_PC_NONE;
+};
+
+namespace tg {
+
+// Kernel launchers (tg_kernels.hip).  Vectors are in the helicity-split layout
+// [col][h][N*H] (h = 0: pol 1, h = 1: pol 0) unless named "ref".
+int launch_mie(tg_cluster *c);
+int launch_assemble_split(tg_cluster *c, int storage, void *out);
+int launch_assemble_ref(tg_cluster *c, int regular, cd *out);
+int launch_translation_blocks(int lmax, const DevPlan &plan, double k, int64_t count, const double *disp,
+                              int regular, cd *out, hipStream_t s);
+int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular);
+int coupling_stored(tg_cluster *c, const cd *x, cd *y, int cols);
+int local_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *subtract_from);
+int ref_to_split(tg_cluster *c, const cd *ref, cd *split, int cols);
+int split_to_ref(tg_cluster *c, const cd *split, cd *ref, int cols);
+int reflect_conj(tg_cluster *c, const cd *x, cd *y, int cols);
+int plane_wave(tg_cluster *c, const double *dir, int pol, cd *out_split);
+int pair_gradients(tg_cluster *c, const cd *u, const cd *v, int cols, int regular, double *grad_pos_dev,
+                   double *grad_k_dev);
+
+// Operator y = (I - T C) x or its adjoint in the split layout.
+int apply_operator(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, int op);
+
+// Vector helpers (tg_kernels.hip)
+int vec_dots(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols,
+             int64_t colstride_v, int64_t colstride_w, cd *out_host, DevBuf &scratch);
+int vec_norms(hipStream_t s, const cd *w, int64_t n, int cols, int64_t colstride, double *out_host, DevBuf &scratch);
+int vec_update(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *coef_dev, cd *w, int64_t n, int cols,
+               int64_t colstride_v, int64_t colstride_w, double sign);
+int vec_axpby(hipStream_t s, cd alpha, const cd *x, cd beta, cd *y, int64_t count);
+int vec_scale_cols(hipStream_t s, const double *scale_dev, const cd *x, cd *y, int64_t n, int cols);
+
+}  // namespace tg

@@ -1,0 +1,856 @@
+// gfx950 kernels of the cluster-coupling path.
+//
+// Layouts.  A vector of n = N M unknowns is kept in the helicity-split layout
+// [h][N*H] with h = 0 for pol 1 (positive helicity) and h = 1 for pol 0, H = M / 2
+// modes (l, m) per sphere and helicity, so the same-helicity coupling is two
+// N H x N H matrices C_+ and C_-.  Stored couplings are column-major, source
+// column (j H + a), destination row (i H + b), with (l, m) index a = l^2 + l + m - 1.
+#include "tg_internal.h"
+#include "tg_special.h"
+
+namespace tg {
+
+// ---------------------------------------------------------------- geometry --
+
+// Length and unit direction of d, scaling by the largest component first;
+// returns 0 where every component is below DBL_MIN (the origin).
+__device__ __forceinline__ int unit_direction(const double d[3], double &r, double u[3]) {
+    double s = fmax(fabs(d[0]), fmax(fabs(d[1]), fabs(d[2])));
+    if (s < 2.2250738585072014e-308) return 0;
+    double a = d[0] / s, b = d[1] / s, c = d[2] / s;
+    double nrm = sqrt(a * a + b * b + c * c);
+    r = s * nrm;
+    u[0] = a / nrm;
+    u[1] = b / nrm;
+    u[2] = c / nrm;
+    return 1;
+}
+
+// Radial values z_p(kr), p = 0..L2: outgoing Hankel or regular Bessel.
+__device__ void radial_values(int L2, double kr, bool regular, cd *rad) {
+    if (!regular) {
+        tg_hankel_seq(L2, kr, rad);
+    } else {
+        double j[64];
+        tg_bessel_j_seq(L2, kr, j);
+        for (int p = 0; p <= L2; ++p) rad[p] = cd(j[p]);
+    }
+}
+
+// Entries (p, +-q), p = q..L2, of the harmonic table z_p(kr) P_p^q e^{i q phi} at the
+// unit vector u: diagonal ladder P_q^q = (-1)^q (2q-1)!! (x + i y)^q, then the degree
+// recurrence (p - q) S_p = (2p - 1) z S_{p-1} - (p + q - 1) S_{p-2}; negative orders by
+// S_p^{-q} = (-1)^q (p-q)!/(p+q)! conj(S_p^q).
+__device__ void table_column(int q, int L2, const double u[3], const cd *rad, const double *refl, cd *tab) {
+    cd xy(u[0], u[1]);
+    cd cur(1.0);
+    for (int k = 1; k <= q; ++k) cur = (-(double)(2 * k - 1)) * (cur * xy);
+    cd prev(0.0);
+    for (int p = q; p <= L2; ++p) {
+        if (p > q) {
+            double a = (double)(2 * p - 1) / (double)(p - q);
+            double b = (double)(p + q - 1) / (double)(p - q);
+            cd next = (a * u[2]) * cur - b * prev;
+            prev = cur;
+            cur = next;
+        }
+        tab[p * p + p + q] = rad[p] * cur;
+        if (q > 0) tab[p * p + p - q] = rad[p] * (refl[p * p + p - q] * conj(cur));
+    }
+}
+
+__device__ __forceinline__ void zero_column(int q, int L2, cd *tab) {
+    for (int p = q; p <= L2; ++p) {
+        tab[p * p + p + q] = cd(0.0);
+        if (q > 0) tab[p * p + p - q] = cd(0.0);
+    }
+}
+
+// Both helicity values of one block entry from the plan and a harmonic table.
+__device__ __forceinline__ void entry_values(const DevPlan &pl, int e, const cd *tab, cd &cp, cd &cm) {
+    const int s0 = pl.starts[e], s1 = pl.starts[e + 1], ns = pl.nsame[e];
+    cd same(0.0), cross(0.0);
+    for (int t = s0; t < s0 + ns; ++t) same += pl.w[t] * tab[pl.hidx[t]];
+    for (int t = s0 + ns; t < s1; ++t) cross += pl.w[t] * tab[pl.hidx[t]];
+    cp = cd(same.re - cross.im, same.im + cross.re);  // same + i cross
+    cm = cd(same.re + cross.im, same.im - cross.re);  // same - i cross
+}
+
+// ------------------------------------------------------------------- Mie --
+
+// One thread per sphere: 2x2 helicity Mie blocks (mode order, row-major) of every
+// degree and their derivatives along the size parameter and the permittivity.
+__global__ void k_mie(int N, int lmax, double k0, const double *radii, const double *eps, cd *val, cd *dsz, cd *deps) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    tg_dual a[kMaxLmax], b[kMaxLmax];
+    int rc = tg_mie_bh(lmax, k0 * radii[i], cd(eps[2 * i], eps[2 * i + 1]), a, b);
+    for (int l = 0; l < lmax; ++l) {
+        cd d = -0.5 * (a[l].v + b[l].v), o = -0.5 * (a[l].v - b[l].v);
+        cd ddx = -0.5 * (a[l].dx + b[l].dx), odx = -0.5 * (a[l].dx - b[l].dx);
+        cd dde = -0.5 * (a[l].de + b[l].de), ode = -0.5 * (a[l].de - b[l].de);
+        if (rc) d = o = ddx = odx = dde = ode = cd(NAN, NAN);
+        cd *v = val + (size_t)(i * lmax + l) * 4, *x = dsz + (size_t)(i * lmax + l) * 4,
+           *e = deps + (size_t)(i * lmax + l) * 4;
+        v[0] = d; v[1] = o; v[2] = o; v[3] = d;
+        x[0] = ddx; x[1] = odx; x[2] = odx; x[3] = ddx;
+        e[0] = dde; e[1] = ode; e[2] = ode; e[3] = dde;
+    }
+}
+
+int launch_mie(tg_cluster *c) {
+    const double *pos = c->b_pos.as<double>();
+    const double *radii = pos + 3 * c->N;
+    const double *eps = radii + c->N;
+    cd *val = c->b_mie.as<cd>();
+    size_t blk = (size_t)c->N * c->lmax * 4;
+    k_mie<<<(c->N + 63) / 64, 64, 0, c->stream>>>(c->N, c->lmax, c->k0, radii, eps, val, val + blk, val + 2 * blk);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+// -------------------------------------------------------------- assembly --
+
+struct GeoCluster {
+    const double *pos;
+    int N;
+    // 0: no row, 1: coincident (self), 2: regular pair with displacement d
+    __device__ int get(int col, int row, double d[3]) const {
+        if (row >= N) return 0;
+        if (row == col) return 1;
+        for (int a = 0; a < 3; ++a) d[a] = pos[3 * row + a] - pos[3 * col + a];
+        return 2;
+    }
+};
+
+struct GeoList {
+    const double *disp;
+    __device__ int get(int col, int row, double d[3]) const {
+        if (row != 0) return 0;
+        for (int a = 0; a < 3; ++a) d[a] = disp[3 * (size_t)col + a];
+        return 2;
+    }
+};
+
+template <class S> __device__ __forceinline__ S cvt(cd v);
+template <> __device__ __forceinline__ float2 cvt<float2>(cd v) { return make_float2((float)v.re, (float)v.im); }
+template <> __device__ __forceinline__ double2 cvt<double2>(cd v) { return make_double2(v.re, v.im); }
+
+template <class S>
+struct OutSplit {
+    S *cp, *cm;
+    int64_t nh;
+    int H;
+    __device__ void store(int i, int j, int a, int b, cd vp, cd vm) const {
+        int64_t idx = ((int64_t)j * H + a) * nh + (int64_t)i * H + b;
+        cp[idx] = cvt<S>(vp);
+        cm[idx] = cvt<S>(vm);
+    }
+};
+
+// Reference layout: rows i M + 2 b + (1 - pol), columns j M + 2 a + (1 - pol);
+// opposite-helicity entries stay at the caller's zero fill.
+struct OutRef {
+    double2 *out;
+    int64_t n;
+    int M;
+    __device__ void store(int i, int j, int a, int b, cd vp, cd vm) const {
+        int64_t row = (int64_t)i * M + 2 * b, col = (int64_t)j * M + 2 * a;
+        out[col * n + row] = make_double2(vp.re, vp.im);
+        out[(col + 1) * n + row + 1] = make_double2(vm.re, vm.im);
+    }
+};
+
+// One M x M block per list entry.
+struct OutBlocks {
+    double2 *out;
+    int M;
+    __device__ void store(int, int j, int a, int b, cd vp, cd vm) const {
+        double2 *blk = out + (size_t)j * M * M;
+        blk[(size_t)(2 * a) * M + 2 * b] = make_double2(vp.re, vp.im);
+        blk[(size_t)(2 * a + 1) * M + 2 * b + 1] = make_double2(vm.re, vm.im);
+    }
+};
+
+// Assembly: one workgroup per (source unit, group of P destination units).  The
+// harmonic tables of the P pairs sit in LDS; each thread owns block entries and
+// reuses every plan term it loads across the P pairs (FP64 accumulation).
+template <int P, class Geo, class Out, bool REG>
+__global__ __launch_bounds__(256) void k_assemble(DevPlan pl, double k, Geo geo, Out out) {
+    extern __shared__ cd smem[];
+    const int T = pl.T, L2 = pl.L2, H = pl.H;
+    cd *tab = smem;
+    cd *rad = smem + P * T;
+    __shared__ double su[P][3];
+    __shared__ int flag[P];
+    const int col = blockIdx.x;
+    const int row0 = blockIdx.y * P;
+    if (threadIdx.x < P) {
+        const int il = threadIdx.x;
+        double d[3], r, u[3];
+        int f = geo.get(col, row0 + il, d);
+        if (f == 2 && !unit_direction(d, r, u)) f = 1;
+        if (f == 2) {
+            radial_values(L2, k * r, REG, rad + il * (L2 + 1));
+            su[il][0] = u[0]; su[il][1] = u[1]; su[il][2] = u[2];
+        }
+        flag[il] = f;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < P * (L2 + 1); t += blockDim.x) {
+        const int il = t / (L2 + 1), q = t - il * (L2 + 1);
+        cd *tb = tab + il * T;
+        if (flag[il] == 2) {
+            double u[3] = {su[il][0], su[il][1], su[il][2]};
+            table_column(q, L2, u, rad + il * (L2 + 1), pl.refl, tb);
+        } else {
+            zero_column(q, L2, tb);
+            if (REG && q == 0 && flag[il] == 1) tb[0] = cd(1.0);  // regular plan at the origin
+        }
+    }
+    __syncthreads();
+    const int HH = H * H;
+    for (int e = threadIdx.x; e < HH; e += blockDim.x) {
+        const int a = e / H, b = e - a * H;
+        const int s0 = pl.starts[e], s1 = pl.starts[e + 1], ns = pl.nsame[e];
+        cd same[P], cross[P];
+#pragma unroll
+        for (int il = 0; il < P; ++il) { same[il] = cd(0.0); cross[il] = cd(0.0); }
+        for (int t = s0; t < s0 + ns; ++t) {
+            const double wt = pl.w[t];
+            const int h = pl.hidx[t];
+#pragma unroll
+            for (int il = 0; il < P; ++il) same[il] += wt * tab[il * T + h];
+        }
+        for (int t = s0 + ns; t < s1; ++t) {
+            const double wt = pl.w[t];
+            const int h = pl.hidx[t];
+#pragma unroll
+            for (int il = 0; il < P; ++il) cross[il] += wt * tab[il * T + h];
+        }
+#pragma unroll
+        for (int il = 0; il < P; ++il) {
+            if (flag[il] == 0) continue;
+            cd vp(same[il].re - cross[il].im, same[il].im + cross[il].re);
+            cd vm(same[il].re + cross[il].im, same[il].im - cross[il].re);
+            out.store(row0 + il, col, a, b, vp, vm);
+        }
+    }
+}
+
+static size_t assemble_smem(const DevPlan &pl, int P) { return (size_t)P * (pl.T + pl.L2 + 1) * sizeof(cd); }
+
+int launch_assemble_split(tg_cluster *c, int storage, void *outp) {
+    constexpr int P = 8;
+    dim3 grid(c->N, (c->N + P - 1) / P);
+    GeoCluster geo{c->b_pos.as<double>(), c->N};
+    size_t smem = assemble_smem(c->plan, P);
+    Timer::Rec rec;
+    TG_TRY(c->timer.begin("assemble", c->stream, rec));
+    if (storage == TG_STORE_C64) {
+        float2 *base = (float2 *)outp;
+        OutSplit<float2> out{base, base + c->nh * c->nh, c->nh, c->H};
+        k_assemble<P, GeoCluster, OutSplit<float2>, false><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo, out);
+    } else {
+        double2 *base = (double2 *)outp;
+        OutSplit<double2> out{base, base + c->nh * c->nh, c->nh, c->H};
+        k_assemble<P, GeoCluster, OutSplit<double2>, false><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo, out);
+    }
+    TG_HIP(hipGetLastError());
+    TG_TRY(c->timer.end("assemble", c->stream, rec));
+    return TG_OK;
+}
+
+int launch_assemble_ref(tg_cluster *c, int regular, cd *outp) {
+    constexpr int P = 4;
+    TG_HIP(hipMemsetAsync(outp, 0, sizeof(cd) * c->n * c->n, c->stream));
+    dim3 grid(c->N, (c->N + P - 1) / P);
+    GeoCluster geo{c->b_pos.as<double>(), c->N};
+    OutRef out{(double2 *)outp, c->n, c->M};
+    size_t smem = assemble_smem(c->plan, P);
+    if (regular)
+        k_assemble<P, GeoCluster, OutRef, true><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo, out);
+    else
+        k_assemble<P, GeoCluster, OutRef, false><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo, out);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+int launch_translation_blocks(int lmax, const DevPlan &pl, double k, int64_t count, const double *disp, int regular,
+                              cd *outp, hipStream_t s) {
+    const int M = 2 * lmax * (lmax + 2);
+    TG_HIP(hipMemsetAsync(outp, 0, sizeof(cd) * M * M * count, s));
+    if (count == 0) return TG_OK;
+    dim3 grid((unsigned)count, 1);
+    GeoList geo{disp};
+    OutBlocks out{(double2 *)outp, M};
+    size_t smem = assemble_smem(pl, 1);
+    if (regular)
+        k_assemble<1, GeoList, OutBlocks, true><<<grid, 256, smem, s>>>(pl, k, geo, out);
+    else
+        k_assemble<1, GeoList, OutBlocks, false><<<grid, 256, smem, s>>>(pl, k, geo, out);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+// --------------------------------------------------- matrix-free coupling (K2) --
+
+// y_i = sum_{j != i} C_ij x_j (or the regular overlap R with identity self blocks),
+// recomputing every pair block in FP64.  Workgroup (destination i, chunk of sources);
+// thread (g, b) owns destination row b and source columns a = g (mod G).  Partial
+// sums per chunk go to `part` [chunk][col][h][nh] and are added in chunk order.
+template <int NC, bool REG>
+__global__ __launch_bounds__(256) void k_coupling_mf(DevPlan pl, double k, const double *pos, int N, int cols,
+                                                      const cd *x, int64_t n, int64_t nh, int chunk, cd *part) {
+    extern __shared__ cd sm[];
+    const int T = pl.T, L2 = pl.L2, H = pl.H;
+    const int G = blockDim.x / H;
+    cd *tab = sm;
+    cd *rad = tab + T;
+    cd *xs = rad + (L2 + 1);  // [cols][2][H]
+    __shared__ double su[3];
+    __shared__ int flag;
+    const int tid = threadIdx.x;
+    const int g = tid / H, b = tid - g * H;
+    const int i = blockIdx.x;
+    const int j0 = blockIdx.y * chunk, j1 = min(N, j0 + chunk);
+    cd acc[NC][2];
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) { acc[cc][0] = cd(0.0); acc[cc][1] = cd(0.0); }
+    for (int j = j0; j < j1; ++j) {
+        __syncthreads();
+        if (tid == 0) {
+            double d[3], r, u[3];
+            for (int a = 0; a < 3; ++a) d[a] = pos[3 * i + a] - pos[3 * j + a];
+            int f = (i == j || !unit_direction(d, r, u)) ? 1 : 2;
+            if (f == 2) {
+                radial_values(L2, k * r, REG, rad);
+                su[0] = u[0]; su[1] = u[1]; su[2] = u[2];
+            }
+            flag = f;
+        }
+        for (int t = tid; t < cols * 2 * H; t += blockDim.x) {
+            const int cc = t / (2 * H), rem = t - cc * 2 * H, h = rem / H, a = rem - h * H;
+            xs[t] = x[(int64_t)cc * n + (int64_t)h * nh + (int64_t)j * H + a];
+        }
+        __syncthreads();
+        if (flag == 1) {
+            if (REG && g == 0 && tid < G * H) {
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc)
+                    if (cc < cols) {
+                        acc[cc][0] += xs[cc * 2 * H + b];
+                        acc[cc][1] += xs[cc * 2 * H + H + b];
+                    }
+            }
+            continue;
+        }
+        for (int q = tid; q <= L2; q += blockDim.x) {
+            double u[3] = {su[0], su[1], su[2]};
+            table_column(q, L2, u, rad, pl.refl, tab);
+        }
+        __syncthreads();
+        if (tid < G * H) {
+            for (int a = g; a < H; a += G) {
+                cd cp, cm;
+                entry_values(pl, a * H + b, tab, cp, cm);
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc)
+                    if (cc < cols) {
+                        acc[cc][0] += cp * xs[cc * 2 * H + a];
+                        acc[cc][1] += cm * xs[cc * 2 * H + H + a];
+                    }
+            }
+        }
+    }
+    // reduce the G column groups in LDS (fixed order)
+    __syncthreads();
+    cd *red = sm;  // [G][cols][2][H]
+    if (tid < G * H) {
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc)
+            if (cc < cols) {
+                red[((g * cols + cc) * 2 + 0) * H + b] = acc[cc][0];
+                red[((g * cols + cc) * 2 + 1) * H + b] = acc[cc][1];
+            }
+    }
+    __syncthreads();
+    for (int t = tid; t < cols * 2 * H; t += blockDim.x) {
+        cd s(0.0);
+        for (int gg = 0; gg < G; ++gg) s += red[gg * cols * 2 * H + t];
+        const int cc = t / (2 * H), rem = t - cc * 2 * H, h = rem / H, bb = rem - h * H;
+        part[((int64_t)blockIdx.y * cols + cc) * n + (int64_t)h * nh + (int64_t)i * H + bb] = s;
+    }
+}
+
+// out[t] = sum_c part[c * count + t], c = 0..nchunks-1 in order.
+template <class PT>
+__global__ void k_sum_chunks(const PT *part, int nchunks, int64_t count, cd *out) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    double re = 0.0, im = 0.0;
+    for (int c = 0; c < nchunks; ++c) {
+        PT v = part[(int64_t)c * count + t];
+        re += (double)v.x;
+        im += (double)v.y;
+    }
+    out[t] = cd(re, im);
+}
+
+static int grid1(int64_t count, int block = 256) { return (int)((count + block - 1) / block); }
+
+int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular) {
+    const int H = c->H;
+    const int G = 256 / H;
+    const int block = G * H;
+    // chunks of sources: enough workgroups to fill the chip
+    int chunk = c->N;
+    while ((int64_t)c->N * ((c->N + chunk - 1) / chunk) < 4096 && chunk > 8) chunk = (chunk + 1) / 2;
+    const int nchunks = (c->N + chunk - 1) / chunk;
+    const int64_t count = (int64_t)cols * c->n;
+    TG_TRY(c->ws[0].ensure(sizeof(cd) * count * nchunks));
+    cd *part = c->ws[0].as<cd>();
+    size_t smem = sizeof(cd) * (c->plan.T + c->plan.L2 + 1 + (size_t)cols * 2 * H);
+    size_t red = sizeof(cd) * (size_t)G * cols * 2 * H;
+    if (red > smem) smem = red;
+    dim3 grid(c->N, nchunks);
+    Timer::Rec rec;
+    const char *name = regular ? "overlap_mf" : "coupling_mf";
+    TG_TRY(c->timer.begin(name, c->stream, rec));
+    for (int c0 = 0; c0 < cols; c0 += 8) {
+        int nc = cols - c0 < 8 ? cols - c0 : 8;
+        const cd *xx = x + (int64_t)c0 * c->n;
+        cd *pp = part + (int64_t)c0 * c->n;  // chunk stride below uses `cols`
+        (void)pp;
+        // process column block [c0, c0+nc) with its own partial buffer layout
+        cd *pb = part;
+#define TG_MF(NCV)                                                                                        \
+    if (regular)                                                                                          \
+        k_coupling_mf<NCV, true><<<grid, block, smem, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), \
+                                                                   c->N, nc, xx, c->n, c->nh, chunk, pb); \
+    else                                                                                                  \
+        k_coupling_mf<NCV, false><<<grid, block, smem, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), \
+                                                                    c->N, nc, xx, c->n, c->nh, chunk, pb);
+        if (nc == 1) { TG_MF(1) } else if (nc == 2) { TG_MF(2) } else if (nc <= 4) { TG_MF(4) } else { TG_MF(8) }
+#undef TG_MF
+        TG_HIP(hipGetLastError());
+        int64_t cnt = (int64_t)nc * c->n;
+        k_sum_chunks<double2><<<grid1(cnt), 256, 0, c->stream>>>((const double2 *)pb, nchunks, cnt,
+                                                                  y + (int64_t)c0 * c->n);
+        TG_HIP(hipGetLastError());
+    }
+    TG_TRY(c->timer.end(name, c->stream, rec));
+    return TG_OK;
+}
+
+// ------------------------------------------------------ stored matvec (GEMV) --
+
+template <class S> struct acc_of;
+template <> struct acc_of<float2> { using T = float; using V = float2; };
+template <> struct acc_of<double2> { using T = double; using V = double2; };
+
+// y_h = C_h x_h for the stored split coupling, both helicities (grid.z), split over
+// column chunks (grid.y).  Each thread owns two consecutive rows and streams the
+// chunk's columns with 16/32-byte loads; x of the chunk sits in LDS in the
+// accumulation precision.  Partial sums [kc][col][h][nh] are added in chunk order.
+template <class S, int NC>
+__global__ __launch_bounds__(256) void k_matvec_stored(const S *C, int64_t nh, const cd *x, int64_t n, int cols,
+                                                        int kchunk, typename acc_of<S>::V *part) {
+    using A = typename acc_of<S>::T;
+    using AV = typename acc_of<S>::V;
+    extern __shared__ unsigned char smraw[];
+    AV *xs = reinterpret_cast<AV *>(smraw);  // [kchunk][NC]
+    const int h = blockIdx.z;
+    const S *Ch = C + (int64_t)h * nh * nh;
+    const int64_t c0 = (int64_t)blockIdx.y * kchunk;
+    const int64_t c1 = min(nh, c0 + kchunk);
+    const int kc = (int)(c1 - c0);
+    for (int t = threadIdx.x; t < kc * NC; t += blockDim.x) {
+        const int cc = t % NC, k = t / NC;
+        AV v;
+        if (cc < cols) {
+            cd xv = x[(int64_t)cc * n + (int64_t)h * nh + c0 + k];
+            v.x = (A)xv.re;
+            v.y = (A)xv.im;
+        } else {
+            v.x = 0;
+            v.y = 0;
+        }
+        xs[t] = v;
+    }
+    __syncthreads();
+    const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
+    if (r0 >= nh) return;
+    const bool pair = r0 + 1 < nh;
+    A acc[NC][2][2];
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) acc[cc][r][0] = acc[cc][r][1] = 0;
+    const S *p = Ch + c0 * nh + r0;
+    int k = 0;
+    if (pair) {
+#pragma unroll 8
+        for (; k < kc; ++k) {
+            S v0 = p[(int64_t)k * nh], v1 = p[(int64_t)k * nh + 1];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                AV xv = xs[k * NC + cc];
+                acc[cc][0][0] += (A)v0.x * xv.x - (A)v0.y * xv.y;
+                acc[cc][0][1] += (A)v0.x * xv.y + (A)v0.y * xv.x;
+                acc[cc][1][0] += (A)v1.x * xv.x - (A)v1.y * xv.y;
+                acc[cc][1][1] += (A)v1.x * xv.y + (A)v1.y * xv.x;
+            }
+        }
+    } else {
+        for (; k < kc; ++k) {
+            S v0 = p[(int64_t)k * nh];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                AV xv = xs[k * NC + cc];
+                acc[cc][0][0] += (A)v0.x * xv.x - (A)v0.y * xv.y;
+                acc[cc][0][1] += (A)v0.x * xv.y + (A)v0.y * xv.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) {
+        if (cc >= cols) break;
+        AV *o = part + ((int64_t)blockIdx.y * cols + cc) * n + (int64_t)h * nh + r0;
+        AV v;
+        v.x = acc[cc][0][0];
+        v.y = acc[cc][0][1];
+        o[0] = v;
+        if (pair) {
+            v.x = acc[cc][1][0];
+            v.y = acc[cc][1][1];
+            o[1] = v;
+        }
+    }
+}
+
+template <class S>
+static int matvec_stored_t(tg_cluster *c, const cd *x, cd *y, int cols) {
+    using AV = typename acc_of<S>::V;
+    const int64_t nh = c->nh;
+    // x tile of kchunk columns x 8 RHS in LDS: at most 64 KB
+    const int kchunk = sizeof(AV) == 8 ? 1024 : 512;
+    const int nk = (int)((nh + kchunk - 1) / kchunk);
+    TG_TRY(c->ws[1].ensure(sizeof(AV) * (size_t)nk * 8 * c->n));
+    AV *part = c->ws[1].as<AV>();
+    const S *C = c->b_C.as<S>();
+    Timer::Rec rec;
+    TG_TRY(c->timer.begin("matvec_stored", c->stream, rec));
+    for (int c0 = 0; c0 < cols; c0 += 8) {
+        const int nc = cols - c0 < 8 ? cols - c0 : 8;
+        dim3 grid((unsigned)((nh + 511) / 512), nk, 2);
+        const cd *xx = x + (int64_t)c0 * c->n;
+#define TG_MV(NCV)                                                                                           \
+    k_matvec_stored<S, NCV><<<grid, 256, sizeof(AV) * kchunk * NCV, c->stream>>>(C, nh, xx, c->n, nc, kchunk, \
+                                                                                  part);
+        if (nc == 1) { TG_MV(1) } else if (nc == 2) { TG_MV(2) } else if (nc <= 4) { TG_MV(4) } else { TG_MV(8) }
+#undef TG_MV
+        TG_HIP(hipGetLastError());
+        const int64_t cnt = (int64_t)nc * c->n;
+        k_sum_chunks<AV><<<grid1(cnt), 256, 0, c->stream>>>(part, nk, cnt, y + (int64_t)c0 * c->n);
+        TG_HIP(hipGetLastError());
+    }
+    TG_TRY(c->timer.end("matvec_stored", c->stream, rec));
+    return TG_OK;
+}
+
+int coupling_stored(tg_cluster *c, const cd *x, cd *y, int cols) {
+    if (c->storage == TG_STORE_C64) return matvec_stored_t<float2>(c, x, y, cols);
+    if (c->storage == TG_STORE_C128) return matvec_stored_t<double2>(c, x, y, cols);
+    set_error("no stored coupling: call tg_cluster_assemble first");
+    return TG_INVALID_INPUT;
+}
+
+// ------------------------------------------------------------- local T --
+
+// y = T x (or T^H x); with sub != NULL, y = sub - T x.  One thread per
+// (column, sphere, (l, m)); the 2x2 block of degree l mixes the two helicities.
+__global__ void k_local(const cd *mie, int lmax, int N, int H, int64_t n, int64_t nh, int cols, const cd *x, cd *y,
+                        int adjoint, const cd *sub) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)cols * nh) return;
+    const int64_t cc = t / nh, k = t - cc * nh;
+    const int p = (int)(k / H), a = (int)(k - (int64_t)p * H);
+    const int l = (int)sqrt((double)(a + 1) + 0.5);
+    const cd *blk = mie + (size_t)(p * lmax + (l - 1)) * 4;
+    cd t00 = blk[0], t01 = blk[1], t10 = blk[2], t11 = blk[3];
+    if (adjoint) {
+        cd s = t01;
+        t00 = conj(t00); t11 = conj(t11);
+        t01 = conj(t10); t10 = conj(s);
+    }
+    const int64_t ip = cc * n + k, im = ip + nh;
+    cd xp = x[ip], xm = x[im];
+    cd yp = t00 * xp + t01 * xm, ym = t10 * xp + t11 * xm;
+    if (sub) {
+        yp = sub[ip] - yp;
+        ym = sub[im] - ym;
+    }
+    y[ip] = yp;
+    y[im] = ym;
+}
+
+int local_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *sub) {
+    int64_t count = (int64_t)cols * c->nh;
+    k_local<<<grid1(count), 256, 0, c->stream>>>(c->b_mie.as<cd>(), c->lmax, c->N, c->H, c->n, c->nh, cols, x, y,
+                                                 adjoint, sub);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+// --------------------------------------------------------- permutations --
+
+__global__ void k_ref_split(const cd *in, cd *out, int M, int H, int64_t n, int64_t nh, int64_t count, int to_split) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int64_t cc = t / n, r = t - cc * n;  // r: reference index
+    const int64_t p = r / M;
+    const int mode = (int)(r - p * M);
+    const int a = mode >> 1, h = mode & 1;
+    const int64_t s = cc * n + (int64_t)h * nh + p * H + a;
+    if (to_split)
+        out[s] = in[t];
+    else
+        out[t] = in[s];
+}
+
+int ref_to_split(tg_cluster *c, const cd *ref, cd *split, int cols) {
+    int64_t count = (int64_t)cols * c->n;
+    k_ref_split<<<grid1(count), 256, 0, c->stream>>>(ref, split, c->M, c->H, c->n, c->nh, count, 1);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+int split_to_ref(tg_cluster *c, const cd *split, cd *ref, int cols) {
+    int64_t count = (int64_t)cols * c->n;
+    k_ref_split<<<grid1(count), 256, 0, c->stream>>>(split, ref, c->M, c->H, c->n, c->nh, count, 0);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+// y = P conj(x) with (P x)[p, (l, m)] = (-1)^m x[p, (l, -m)], per helicity.  The
+// coupling satisfies C_h = P C_h^T P (reciprocity), so C_h^H = P conj(C_h) P.
+__global__ void k_reflect_conj(const cd *x, cd *y, int H, int64_t count) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int64_t base = t - (t % H);
+    const int a = (int)(t % H);
+    const int l = (int)sqrt((double)(a + 1) + 0.5);
+    const int m = a + 1 - l * l - l;
+    const int ar = l * l + l - m - 1;
+    cd v = conj(x[base + ar]);
+    y[t] = (m & 1) ? -v : v;
+}
+
+int reflect_conj(tg_cluster *c, const cd *x, cd *y, int cols) {
+    int64_t count = (int64_t)cols * c->n;
+    k_reflect_conj<<<grid1(count), 256, 0, c->stream>>>(x, y, c->H, count);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+// ------------------------------------------------------------ plane wave --
+
+// Regular local coefficients of a unit plane wave (helicity pol) in the split
+// layout: 2 sqrt(pi (2l+1)/(l(l+1))) sqrt((l-m)!/(l+m)!) i^l e^{-i m phi}
+// (tau + (2 pol - 1) pi) e^{i k.r_p}, with the angular functions from
+// F_l^m = P_l^m / sin^|m| so the poles need no division.
+__global__ void k_plane_wave(int N, int H, int64_t nh, const double *pos, double kx, double ky, double kz, int pol,
+                             cd *out) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * nh) return;
+    const int h = (int)(t / nh);
+    const int64_t k = t - (int64_t)h * nh;
+    const int p = (int)(k / H), a = (int)(k - (int64_t)p * H);
+    const int wave_h = pol == 1 ? 0 : 1;
+    if (h != wave_h) {
+        out[t] = cd(0.0);
+        return;
+    }
+    const int l = (int)sqrt((double)(a + 1) + 0.5);
+    const int m = a + 1 - l * l - l;
+    const double kk = sqrt(kx * kx + ky * ky + kz * kz);
+    const double tr = sqrt(kx * kx + ky * ky);
+    const bool axis = tr == 0.0;
+    const double cx_ = axis ? 0.0 : kx / tr, sy_ = axis ? 0.0 : ky / tr;
+    const double cth = axis ? (kz / kk >= 0.0 ? 1.0 : -1.0) : kz / kk;
+    const double sth = tr / kk;
+    // F_l^mm and F_{l-1}^mm for mm = (m == 0 ? 1 : m)
+    const int mm = m == 0 ? 1 : m;
+    const int am = mm < 0 ? -mm : mm;
+    double f = 1.0;
+    if (mm >= 0)
+        for (int i = 1; i <= mm; ++i) f *= -(double)(2 * i - 1);
+    else
+        for (int i = 1; i <= -mm; ++i) f *= 1.0 / (double)(2 * i);
+    double fl = 0.0, flm1 = 0.0;
+    if (l >= am) {
+        double prev = 0.0, cur = f;
+        for (int d = am + 1; d <= l; ++d) {
+            double next = ((double)(2 * d - 1) * cth * cur - (double)(d + mm - 1) * prev) / (double)(d - mm);
+            prev = cur;
+            cur = next;
+        }
+        fl = cur;
+        flm1 = (l > am) ? prev : 0.0;
+    }
+    double pi_v, tau_v;
+    if (m == 0) {
+        pi_v = 0.0;
+        tau_v = sth * fl;
+    } else {
+        double power = 1.0;
+        for (int i = 1; i < am; ++i) power *= sth;
+        pi_v = (double)m * power * fl;
+        tau_v = power * ((double)l * cth * fl - (double)(l + m) * flm1);
+    }
+    const double ang = tau_v + (2.0 * pol - 1.0) * pi_v;
+    double ratio = 1.0;  // (l-m)!/(l+m)!
+    if (m > 0)
+        for (int i = l - m + 1; i <= l + m; ++i) ratio /= (double)i;
+    else
+        for (int i = l + m + 1; i <= l - m; ++i) ratio *= (double)i;
+    const double norm = 2.0 * sqrt(M_PI * (2 * l + 1) / (double)(l * (l + 1))) * sqrt(ratio);
+    cd az(1.0);
+    if (!axis) {
+        cd e(cx_, -sy_);
+        int am2 = m < 0 ? -m : m;
+        for (int i = 0; i < am2; ++i) az = az * e;
+        if (m < 0) az = conj(az);
+    }
+    const double phase = kx * pos[3 * p] + ky * pos[3 * p + 1] + kz * pos[3 * p + 2];
+    cd v = (norm * ang) * (ipow_(l) * az * cd(cos(phase), sin(phase)));
+    out[t] = v;
+}
+
+int plane_wave(tg_cluster *c, const double *dir, int pol, cd *out) {
+    double nrm = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    double kx = c->k0 * dir[0] / nrm, ky = c->k0 * dir[1] / nrm, kz = c->k0 * dir[2] / nrm;
+    k_plane_wave<<<grid1(2 * c->nh), 256, 0, c->stream>>>(c->N, c->H, c->nh, c->b_pos.as<double>(), kx, ky, kz, pol,
+                                                          out);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+// ------------------------------------------------------------ vector ops --
+
+// partial[((col * nvec + i) * gridDim.x + block] = sum_k conj(V[col][i][k]) w[col][k]
+__global__ __launch_bounds__(256) void k_dots(const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n,
+                                               int64_t csv, int64_t csw, cd *partial) {
+    __shared__ double sre[256], sim[256];
+    const int col = blockIdx.y;
+    const cd *wc = w + col * csw;
+    const cd *Vc = V + col * csv;
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t k0 = blockIdx.x * per, k1 = min(n, k0 + per);
+    for (int i = 0; i < nvec; ++i) {
+        double re = 0.0, im = 0.0;
+        const cd *v = Vc + (int64_t)i * vstride;
+        for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+            cd a = v[k], b = wc[k];
+            re += a.re * b.re + a.im * b.im;
+            im += a.re * b.im - a.im * b.re;
+        }
+        sre[threadIdx.x] = re;
+        sim[threadIdx.x] = im;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                sre[threadIdx.x] += sre[threadIdx.x + s];
+                sim[threadIdx.x] += sim[threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[((int64_t)col * nvec + i) * gridDim.x + blockIdx.x] = cd(sre[0], sim[0]);
+        __syncthreads();
+    }
+}
+
+__global__ void k_sum_rows(const cd *partial, int width, int rows, cd *out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    cd s(0.0);
+    for (int b = 0; b < width; ++b) s += partial[(int64_t)r * width + b];
+    out[r] = s;
+}
+
+static int dot_blocks(int64_t n) {
+    int64_t b = (n + 4095) / 4096;
+    if (b > 256) b = 256;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+int vec_dots(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols, int64_t csv,
+             int64_t csw, cd *out_host, DevBuf &scratch) {
+    const int nb = dot_blocks(n);
+    const int rows = cols * nvec;
+    TG_TRY(scratch.ensure(sizeof(cd) * ((size_t)rows * nb + rows)));
+    cd *partial = scratch.as<cd>();
+    cd *res = partial + (size_t)rows * nb;
+    k_dots<<<dim3(nb, cols), 256, 0, s>>>(V, vstride, nvec, w, n, csv, csw, partial);
+    TG_HIP(hipGetLastError());
+    k_sum_rows<<<(rows + 63) / 64, 64, 0, s>>>(partial, nb, rows, res);
+    TG_HIP(hipGetLastError());
+    TG_HIP(hipMemcpyAsync(out_host, res, sizeof(cd) * rows, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    return TG_OK;
+}
+
+int vec_norms(hipStream_t s, const cd *w, int64_t n, int cols, int64_t colstride, double *out_host, DevBuf &scratch) {
+    std::vector<cd> d(cols);
+    // |w|^2 as the dot of w with itself (nvec = 1, V = w)
+    TG_TRY(vec_dots(s, w, 0, 1, w, n, cols, colstride, colstride, d.data(), scratch));
+    for (int i = 0; i < cols; ++i) out_host[i] = sqrt(fmax(d[i].re, 0.0));
+    return TG_OK;
+}
+
+// w[col] += sign * sum_i coef[col * nvec + i] V[col][i]
+__global__ void k_update(const cd *V, int64_t vstride, int nvec, const cd *coef, cd *w, int64_t n, int64_t csv,
+                         int64_t csw, double sign) {
+    const int col = blockIdx.y;
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    cd acc(0.0);
+    for (int i = 0; i < nvec; ++i) acc += coef[col * nvec + i] * V[col * csv + (int64_t)i * vstride + k];
+    w[col * csw + k] += sign * acc;
+}
+
+int vec_update(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *coef_dev, cd *w, int64_t n, int cols,
+               int64_t csv, int64_t csw, double sign) {
+    k_update<<<dim3(grid1(n), cols), 256, 0, s>>>(V, vstride, nvec, coef_dev, w, n, csv, csw, sign);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+__global__ void k_axpby(cd alpha, const cd *x, cd beta, cd *y, int64_t count) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    y[t] = alpha * x[t] + beta * y[t];
+}
+
+int vec_axpby(hipStream_t s, cd alpha, const cd *x, cd beta, cd *y, int64_t count) {
+    k_axpby<<<grid1(count), 256, 0, s>>>(alpha, x, beta, y, count);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+__global__ void k_scale_cols(const double *scale, const cd *x, cd *y, int64_t n) {
+    const int col = blockIdx.y;
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    y[col * n + k] = scale[col] * x[col * n + k];
+}
+
+int vec_scale_cols(hipStream_t s, const double *scale_dev, const cd *x, cd *y, int64_t n, int cols) {
+    k_scale_cols<<<dim3(grid1(n), cols), 256, 0, s>>>(scale_dev, x, y, n);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+}  // namespace tg

@@ -1,0 +1,343 @@
+// Block-Jacobi preconditioner over spatial groups of spheres (right preconditioning).
+//
+// Group operator A_G = I - T_G C_GG (sphere-major, reference mode order inside a
+// sphere), inverted exactly on the device by in-place Gauss-Jordan elimination with
+// partial pivoting (one workgroup per group, column-major storage so that the
+// elimination sweep is coalesced over rows).
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "tg_precond.h"
+
+namespace tg {
+
+int launch_translation_blocks(int lmax, const DevPlan &pl, double k, int64_t count, const double *disp, int regular,
+                              cd *outp, hipStream_t s);
+
+struct Preconditioner {
+    int kind = TG_PC_NONE;
+    int group_size = 4;
+    int ngroups = 0, max_m = 0;
+    DevBuf mats;      // group inverses, column-major, c128
+    DevBuf meta;      // offsets (int64), sizes (int), member offsets (int), members (int)
+    const int64_t *d_off = nullptr;
+    const int *d_size = nullptr, *d_moff = nullptr, *d_members = nullptr;
+    // learned near-field correction (tg_learned.hip)
+    void *learned = nullptr;
+};
+
+std::vector<std::vector<int>> spatial_groups(const std::vector<double> &pos, int N, int size) {
+    std::vector<std::vector<int>> out;
+    std::vector<int> all(N);
+    std::iota(all.begin(), all.end(), 0);
+    std::vector<std::vector<int>> stack{all};
+    while (!stack.empty()) {
+        std::vector<int> g = std::move(stack.back());
+        stack.pop_back();
+        if ((int)g.size() <= size) {
+            out.push_back(g);
+            continue;
+        }
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int p : g)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], pos[3 * p + a]);
+                hi[a] = std::max(hi[a], pos[3 * p + a]);
+            }
+        int axis = 0;
+        for (int a = 1; a < 3; ++a)
+            if (hi[a] - lo[a] > hi[axis] - lo[axis]) axis = a;
+        std::stable_sort(g.begin(), g.end(), [&](int x, int y) { return pos[3 * x + axis] < pos[3 * y + axis]; });
+        // split so that the left part holds a multiple of `size` spheres where possible
+        size_t half = g.size() / 2;
+        size_t left = ((half + size - 1) / size) * size;
+        if (left >= g.size()) left = half;
+        std::vector<int> a(g.begin(), g.begin() + left), b(g.begin() + left, g.end());
+        stack.push_back(b);
+        stack.push_back(a);
+    }
+    return out;
+}
+
+// A_G[R, C] = delta - sum_k T_i[r, k] C_ij[k, c] for R = (li, r), C = (lj, c), i != j;
+// the identity on sphere-diagonal blocks.  pair blocks: [pair][M x M] column-major.
+__global__ void k_group_operator(const cd *pairs, const int64_t *pair_off, const int *gsize, const int64_t *mat_off,
+                                 const int *moff, const int *members, const cd *mie, int lmax, int M, cd *mats) {
+    const int g = blockIdx.y;
+    const int s = gsize[g];
+    const int m = s * M;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m * m) return;
+    const int C = (int)(t / m), R = (int)(t - (int64_t)C * m);
+    const int li = R / M, r = R - li * M, lj = C / M, cc = C - lj * M;
+    cd v(0.0);
+    if (li == lj) {
+        v = cd(r == cc ? 1.0 : 0.0);
+    } else {
+        // pair index within the group: ordered pairs (li, lj), li != lj, row-major over li
+        const int pidx = li * (s - 1) + (lj < li ? lj : lj - 1);
+        const cd *blk = pairs + (pair_off[g] + pidx) * (int64_t)M * M;
+        const int b = r >> 1;
+        const int l = (int)sqrt((double)(b + 1) + 0.5);
+        const int p = members[moff[g] + li];
+        const cd *t2 = mie + (size_t)(p * lmax + (l - 1)) * 4;
+        const int rr = r & 1;
+        cd tc = t2[rr * 2 + 0] * blk[(int64_t)cc * M + 2 * b] + t2[rr * 2 + 1] * blk[(int64_t)cc * M + 2 * b + 1];
+        v = -tc;
+    }
+    mats[mat_off[g] + t] = v;
+}
+
+// In-place Gauss-Jordan inversion with partial pivoting, one workgroup per matrix.
+__global__ __launch_bounds__(256) void k_gauss_jordan(cd *mats, const int64_t *mat_off, const int *gsize, int M,
+                                                      int *pivots, int piv_stride, int *status) {
+    extern __shared__ cd sh[];
+    const int g = blockIdx.x;
+    const int m = gsize[g] * M;
+    cd *a = mats + mat_off[g];
+    int *piv = pivots + (int64_t)g * piv_stride;
+    cd *colk = sh;
+    cd *rowk = sh + m;
+    __shared__ double bv[256];
+    __shared__ int bi[256];
+    const int tid = threadIdx.x;
+    for (int k = 0; k < m; ++k) {
+        double best = -1.0;
+        int besti = k;
+        for (int r = k + tid; r < m; r += blockDim.x) {
+            double v = abs2(a[(int64_t)k * m + r]);
+            if (v > best) { best = v; besti = r; }
+        }
+        bv[tid] = best;
+        bi[tid] = besti;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (tid < s) {
+                if (bv[tid + s] > bv[tid] || (bv[tid + s] == bv[tid] && bi[tid + s] < bi[tid])) {
+                    bv[tid] = bv[tid + s];
+                    bi[tid] = bi[tid + s];
+                }
+            }
+            __syncthreads();
+        }
+        const int p = bi[0];
+        if (!(bv[0] > 0.0)) {
+            if (tid == 0) status[g] = 1;
+            return;
+        }
+        if (tid == 0) piv[k] = p;
+        if (p != k)
+            for (int col = tid; col < m; col += blockDim.x) {
+                cd tmp = a[(int64_t)col * m + k];
+                a[(int64_t)col * m + k] = a[(int64_t)col * m + p];
+                a[(int64_t)col * m + p] = tmp;
+            }
+        __syncthreads();
+        const cd d = cd(1.0) / a[(int64_t)k * m + k];
+        for (int col = tid; col < m; col += blockDim.x) {
+            cd v = col == k ? d : a[(int64_t)col * m + k] * d;
+            a[(int64_t)col * m + k] = v;
+            rowk[col] = v;
+        }
+        for (int r = tid; r < m; r += blockDim.x) colk[r] = r == k ? cd(0.0) : a[(int64_t)k * m + r];
+        __syncthreads();
+        for (int64_t idx = tid; idx < (int64_t)m * m; idx += blockDim.x) {
+            const int col = (int)(idx / m), r = (int)(idx - (int64_t)col * m);
+            if (r == k) continue;
+            cd f = colk[r];
+            cd old = col == k ? cd(0.0) : a[idx];
+            a[idx] = old - f * rowk[col];
+        }
+        __syncthreads();
+    }
+    for (int k = m - 1; k >= 0; --k) {
+        const int p = piv[k];
+        if (p != k)
+            for (int r = tid; r < m; r += blockDim.x) {
+                cd tmp = a[(int64_t)k * m + r];
+                a[(int64_t)k * m + r] = a[(int64_t)p * m + r];
+                a[(int64_t)p * m + r] = tmp;
+            }
+        __syncthreads();
+    }
+}
+
+// y_G = A_G^{-1} x_G (or A_G^{-H} x_G) per group and column, split layout in and out.
+__global__ __launch_bounds__(256) void k_pc_apply(const cd *mats, const int64_t *mat_off, const int *gsize,
+                                                  const int *moff, const int *members, int M, int H, int64_t n,
+                                                  int64_t nh, const cd *x, cd *y, int adjoint) {
+    extern __shared__ cd v[];
+    const int g = blockIdx.x, col = blockIdx.y;
+    const int m = gsize[g] * M;
+    const cd *a = mats + mat_off[g];
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+        const int li = t / M, mode = t - li * M;
+        const int p = members[moff[g] + li];
+        v[t] = x[(int64_t)col * n + (int64_t)(mode & 1) * nh + (int64_t)p * H + (mode >> 1)];
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < m; r += blockDim.x) {
+        cd s(0.0);
+        if (!adjoint)
+            for (int k = 0; k < m; ++k) s += a[(int64_t)k * m + r] * v[k];
+        else
+            for (int k = 0; k < m; ++k) s += conj(a[(int64_t)r * m + k]) * v[k];
+        const int li = r / M, mode = r - li * M;
+        const int p = members[moff[g] + li];
+        y[(int64_t)col * n + (int64_t)(mode & 1) * nh + (int64_t)p * H + (mode >> 1)] = s;
+    }
+}
+
+int learned_setup(tg_cluster *c, Preconditioner *pc);
+int learned_apply(tg_cluster *c, Preconditioner *pc, const cd *x, cd *y, int cols, int adjoint);
+void learned_destroy(Preconditioner *pc);
+
+static int build_block_jacobi(tg_cluster *c, Preconditioner *pc) {
+    const int M = c->M;
+    auto groups = spatial_groups(c->positions, c->N, pc->group_size);
+    pc->ngroups = (int)groups.size();
+    std::vector<int64_t> mat_off(pc->ngroups + 1, 0), pair_off(pc->ngroups + 1, 0);
+    std::vector<int> gsize(pc->ngroups), moff(pc->ngroups + 1, 0), members;
+    std::vector<double> disp;
+    pc->max_m = 0;
+    for (int g = 0; g < pc->ngroups; ++g) {
+        const auto &G = groups[g];
+        const int s = (int)G.size();
+        gsize[g] = s;
+        moff[g + 1] = moff[g] + s;
+        for (int p : G) members.push_back(p);
+        mat_off[g + 1] = mat_off[g] + (int64_t)(s * M) * (s * M);
+        pair_off[g + 1] = pair_off[g] + (int64_t)s * (s - 1);
+        pc->max_m = std::max(pc->max_m, s * M);
+        for (int li = 0; li < s; ++li)
+            for (int lj = 0; lj < s; ++lj) {
+                if (li == lj) continue;
+                for (int a = 0; a < 3; ++a) disp.push_back(c->positions[3 * G[li] + a] - c->positions[3 * G[lj] + a]);
+            }
+    }
+    const int64_t npairs = pair_off[pc->ngroups];
+    // meta buffer
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t o_off = 0, o_poff = al(sizeof(int64_t) * (pc->ngroups + 1));
+    size_t o_size = al(o_poff + sizeof(int64_t) * (pc->ngroups + 1));
+    size_t o_moff = al(o_size + sizeof(int) * pc->ngroups);
+    size_t o_mem = al(o_moff + sizeof(int) * (pc->ngroups + 1));
+    size_t o_disp = al(o_mem + sizeof(int) * members.size());
+    size_t o_piv = al(o_disp + sizeof(double) * std::max<size_t>(disp.size(), 1));
+    size_t o_stat = al(o_piv + sizeof(int) * (size_t)pc->ngroups * pc->max_m);
+    size_t total = al(o_stat + sizeof(int) * pc->ngroups);
+    TG_TRY(pc->meta.ensure(total));
+    std::vector<char> host(total, 0);
+    memcpy(host.data() + o_off, mat_off.data(), sizeof(int64_t) * (pc->ngroups + 1));
+    memcpy(host.data() + o_poff, pair_off.data(), sizeof(int64_t) * (pc->ngroups + 1));
+    memcpy(host.data() + o_size, gsize.data(), sizeof(int) * pc->ngroups);
+    memcpy(host.data() + o_moff, moff.data(), sizeof(int) * (pc->ngroups + 1));
+    memcpy(host.data() + o_mem, members.data(), sizeof(int) * members.size());
+    if (!disp.empty()) memcpy(host.data() + o_disp, disp.data(), sizeof(double) * disp.size());
+    char *base = pc->meta.as<char>();
+    TG_HIP(hipMemcpy(base, host.data(), total, hipMemcpyHostToDevice));
+    pc->d_off = (const int64_t *)(base + o_off);
+    pc->d_size = (const int *)(base + o_size);
+    pc->d_moff = (const int *)(base + o_moff);
+    pc->d_members = (const int *)(base + o_mem);
+    TG_TRY(pc->mats.ensure(sizeof(cd) * (size_t)mat_off[pc->ngroups]));
+    DevBuf pairs;
+    TG_TRY(pairs.ensure(sizeof(cd) * (size_t)std::max<int64_t>(npairs, 1) * M * M));
+    int st = launch_translation_blocks(c->lmax, c->plan, c->k0, npairs, (const double *)(base + o_disp), 0,
+                                       pairs.as<cd>(), c->stream);
+    if (!st) {
+        int64_t maxel = (int64_t)pc->max_m * pc->max_m;
+        dim3 grid((unsigned)((maxel + 255) / 256), pc->ngroups);
+        k_group_operator<<<grid, 256, 0, c->stream>>>(pairs.as<cd>(), (const int64_t *)(base + o_poff), pc->d_size,
+                                                      pc->d_off, pc->d_moff, pc->d_members, c->b_mie.as<cd>(),
+                                                      c->lmax, M, pc->mats.as<cd>());
+        if (hipGetLastError() != hipSuccess) st = TG_DEVICE_ERROR;
+    }
+    if (!st) {
+        Timer::Rec rec;
+        c->timer.begin("pc_invert", c->stream, rec);
+        (void)hipMemsetAsync(base + o_stat, 0, sizeof(int) * pc->ngroups, c->stream);
+        k_gauss_jordan<<<pc->ngroups, 256, sizeof(cd) * 2 * pc->max_m, c->stream>>>(
+            pc->mats.as<cd>(), pc->d_off, pc->d_size, M, (int *)(base + o_piv), pc->max_m, (int *)(base + o_stat));
+        if (hipGetLastError() != hipSuccess) st = TG_DEVICE_ERROR;
+        c->timer.end("pc_invert", c->stream, rec);
+    }
+    if (!st) {
+        std::vector<int> stat(pc->ngroups);
+        TG_HIP(hipMemcpyAsync(stat.data(), base + o_stat, sizeof(int) * pc->ngroups, hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+        for (int v : stat)
+            if (v) {
+                set_error("singular group block in the block-Jacobi preconditioner");
+                st = TG_SINGULAR;
+                break;
+            }
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !st) st = TG_DEVICE_ERROR;
+    pairs.release();
+    return st;
+}
+
+int ensure_preconditioner(tg_cluster *c, int kind, int storage) {
+    (void)storage;
+    if (c->pc && c->pc_kind == kind) return TG_OK;
+    destroy_preconditioner(c);
+    Preconditioner *pc = new Preconditioner();
+    pc->kind = kind;
+    const char *env = getenv("TG_GROUP_SIZE");
+    if (env && atoi(env) > 0) pc->group_size = atoi(env);
+    int st = build_block_jacobi(c, pc);
+    if (!st && kind == TG_PC_LEARNED) st = learned_setup(c, pc);
+    if (st) {
+        pc->mats.release();
+        pc->meta.release();
+        learned_destroy(pc);
+        delete pc;
+        return st;
+    }
+    c->pc = pc;
+    c->pc_kind = kind;
+    return TG_OK;
+}
+
+int apply_preconditioner(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint) {
+    Preconditioner *pc = c->pc;
+    if (!pc) {
+        set_error("preconditioner not built");
+        return TG_INVALID_INPUT;
+    }
+    Timer::Rec rec;
+    TG_TRY(c->timer.begin("pc_apply", c->stream, rec));
+    k_pc_apply<<<dim3(pc->ngroups, cols), 256, sizeof(cd) * pc->max_m, c->stream>>>(
+        pc->mats.as<cd>(), pc->d_off, pc->d_size, pc->d_moff, pc->d_members, c->M, c->H, c->n, c->nh, x, y, adjoint);
+    TG_HIP(hipGetLastError());
+    if (pc->kind == TG_PC_LEARNED) TG_TRY(learned_apply(c, pc, x, y, cols, adjoint));
+    TG_TRY(c->timer.end("pc_apply", c->stream, rec));
+    return TG_OK;
+}
+
+void destroy_preconditioner(tg_cluster *c) {
+    if (!c->pc) return;
+    c->pc->mats.release();
+    c->pc->meta.release();
+    learned_destroy(c->pc);
+    delete c->pc;
+    c->pc = nullptr;
+    c->pc_kind = TG_PC_NONE;
+}
+
+// Learned part: filled in by tg_learned.hip when present.
+__attribute__((weak)) int learned_setup(tg_cluster *, Preconditioner *) {
+    set_error("learned preconditioner weights are not loaded");
+    return TG_INVALID_INPUT;
+}
+__attribute__((weak)) int learned_apply(tg_cluster *, Preconditioner *, const cd *, cd *, int, int) {
+    return TG_INVALID_INPUT;
+}
+__attribute__((weak)) void learned_destroy(Preconditioner *) {}
+
+}  // namespace tg

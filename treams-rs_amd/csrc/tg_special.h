@@ -1,0 +1,120 @@
+// Special functions of the cluster path, FP64, host and device.
+//
+//  - outgoing spherical Hankel h_p(x), real x: seed h_-1 = e^{ix}/x, h_0 = -i e^{ix}/x
+//    and upward recurrence (stable: h is the dominant solution);
+//  - regular spherical Bessel j_p(x), real x: Miller's backward ratio recurrence
+//    normalised by j_0 = sin x / x, power series below x = 0.5;
+//  - Mie coefficients of a homogeneous sphere in vacuum by the Bohren-Huffman
+//    logarithmic-derivative algorithm (Bohren & Huffman 1983, sec. 4.8), carried out
+//    on dual numbers so the same code gives dT/dx and dT/d(epsilon).
+#pragma once
+#include "tg_complex.h"
+
+// h_p(x), p = 0..last.
+TG_HD void tg_hankel_seq(int last, double x, cd *out) {
+    double inv = 1.0 / x, s = sin(x), c = cos(x);
+    cd prev(c * inv, s * inv);
+    cd cur(s * inv, -c * inv);
+    for (int p = 0; p <= last; ++p) {
+        out[p] = cur;
+        cd next = ((double)(2 * p + 1) * inv) * cur - prev;
+        prev = cur;
+        cur = next;
+    }
+}
+
+// j_p(x) by its power series x^p/(2p+1)!! sum_q (-x^2/2)^q / (q! (2p+3)...(2p+2q+1)).
+TG_HD double tg_sph_j_series(int p, double x) {
+    double lead = 1.0;
+    for (int i = 1; i <= p; ++i) lead *= x / (double)(2 * i + 1);
+    double term = lead, sum = lead, x2 = -0.5 * x * x;
+    for (int q = 1; q < 18; ++q) {
+        term *= x2 / ((double)q * (double)(2 * p + 2 * q + 1));
+        sum += term;
+    }
+    return sum;
+}
+
+// j_p(x), p = 0..last (last <= 63), real x >= 0.
+TG_HD void tg_bessel_j_seq(int last, double x, double *out) {
+    if (x < 0.5) {
+        for (int p = 0; p <= last; ++p) out[p] = tg_sph_j_series(p, x);
+        return;
+    }
+    int start = last + (int)x + 40;
+    double r = 0.0;
+    double ratios[72];
+    for (int p = start; p >= 1; --p) {
+        r = x / ((double)(2 * p + 1) - x * r);
+        if (p <= last) ratios[p] = r;
+    }
+    out[0] = sin(x) / x;
+    for (int p = 1; p <= last; ++p) out[p] = out[p - 1] * ratios[p];
+}
+
+// ------------------------------------------------------------------ dual numbers --
+// Value and derivatives along the size parameter x and the permittivity epsilon.
+struct tg_dual {
+    cd v, dx, de;
+    TG_HD tg_dual() {}
+    TG_HD tg_dual(cd a) : v(a) {}
+    TG_HD tg_dual(cd a, cd b, cd c) : v(a), dx(b), de(c) {}
+};
+TG_HD tg_dual operator+(tg_dual a, tg_dual b) { return {a.v + b.v, a.dx + b.dx, a.de + b.de}; }
+TG_HD tg_dual operator-(tg_dual a, tg_dual b) { return {a.v - b.v, a.dx - b.dx, a.de - b.de}; }
+TG_HD tg_dual operator-(tg_dual a) { return {-a.v, -a.dx, -a.de}; }
+TG_HD tg_dual operator*(tg_dual a, tg_dual b) {
+    return {a.v * b.v, a.dx * b.v + a.v * b.dx, a.de * b.v + a.v * b.de};
+}
+TG_HD tg_dual operator*(double s, tg_dual a) { return {s * a.v, s * a.dx, s * a.de}; }
+TG_HD tg_dual operator/(tg_dual a, tg_dual b) {
+    cd q = a.v / b.v;
+    return {q, (a.dx - q * b.dx) / b.v, (a.de - q * b.de) / b.v};
+}
+TG_HD tg_dual operator/(double s, tg_dual b) { return tg_dual(cd(s)) / b; }
+
+// Mie coefficients a_n, b_n (n = 1..lmax) of a nonmagnetic sphere of size x and
+// relative permittivity eps in vacuum, with derivatives along x and eps.
+// lmax <= 40.  Returns 0, or -1 for a zero denominator.
+TG_HD int tg_mie_bh(int lmax, double xv, cd eps, tg_dual *a, tg_dual *b) {
+    cd mv = csqrt_(eps);
+    tg_dual x(cd(xv), cd(1.0), cd(0.0));
+    tg_dual m(mv, cd(0.0), cd(0.5) / mv);  // dm/deps = 1/(2m)
+    tg_dual mx = m * x;
+    // D_n(mx) downward from zero.
+    double amx = cabs_(mx.v);
+    int nstart = (lmax > (int)amx ? lmax : (int)amx) + 16;
+    tg_dual D[48];
+    tg_dual d(cd(0.0));
+    for (int n = nstart; n >= 1; --n) {
+        tg_dual nz = (double)n / mx;
+        if (n <= lmax) D[n] = d;
+        d = nz - 1.0 / (d + nz);
+    }
+    // psi_n(x) = x j_n(x) from the ratio recurrence, xi_n(x) = x h_n(x) upward.
+    int jstart = lmax + (int)xv + 40;
+    tg_dual r(cd(0.0));
+    tg_dual ratio[48];
+    for (int n = jstart; n >= 1; --n) {
+        r = x / (tg_dual(cd((double)(2 * n + 1))) - x * r);
+        if (n <= lmax) ratio[n] = r;
+    }
+    double s = sin(xv), c = cos(xv);
+    tg_dual psi_prev(cd(s), cd(c), cd(0.0));                   // psi_0 = sin x
+    tg_dual xi_prev(cd(s, -c), cd(c, s), cd(0.0));             // xi_0 = -i e^{ix}
+    tg_dual xi_prev2(cd(c, s), cd(-s, c), cd(0.0));            // xi_-1 = e^{ix}
+    for (int n = 1; n <= lmax; ++n) {
+        tg_dual psi = psi_prev * ratio[n];
+        tg_dual xi = ((double)(2 * n - 1) / x) * xi_prev - xi_prev2;
+        tg_dual nx = (double)n / x;
+        tg_dual ga = D[n] / m + nx, gb = m * D[n] + nx;
+        tg_dual da = ga * xi - xi_prev, db = gb * xi - xi_prev;
+        if (abs2(da.v) == 0.0 || abs2(db.v) == 0.0) return -1;
+        a[n - 1] = (ga * psi - psi_prev) / da;
+        b[n - 1] = (gb * psi - psi_prev) / db;
+        psi_prev = psi;
+        xi_prev2 = xi_prev;
+        xi_prev = xi;
+    }
+    return 0;
+}
