@@ -57,6 +57,8 @@ def lib():
         L.or_pair_matrix.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _f64p, ctypes.c_int, _c128p]
         L.or_pair_matrix_pullback.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _f64p, ctypes.c_int,
                                               _c128p, _f64p, _f64p]
+        L.or_coupling_apply.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _f64p, ctypes.c_int, _c128p,
+                                        _c128p, ctypes.c_int, ctypes.c_int]
         L.or_mie.restype = ctypes.c_int
         L.or_mie.argtypes = [ctypes.c_int, ctypes.c_double, _C128, _c128p]
         L.or_mie_pullback.restype = ctypes.c_int
@@ -184,6 +186,21 @@ def pair_matrix_pullback(lmax: int, k0: float, positions, regular: bool, gbar: n
     gk = np.zeros(1)
     lib().or_pair_matrix_pullback(lmax, float(k0), N, pos.reshape(-1), int(regular), g, grad, gk)
     return grad.reshape(N, 3), float(gk[0])
+
+
+def coupling_apply(lmax: int, k0: float, positions, x: np.ndarray, rows=None) -> np.ndarray:
+    """Matrix-free C x (cluster/iterative.rs:187-221); x is (n,) or (n, cols).  With
+    rows = (i0, i1) only destination particles i0..i1-1 are computed (others zero)."""
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
+    N = pos.shape[0]
+    vec = x.ndim == 1
+    X = np.ascontiguousarray((x[:, None] if vec else x).T, np.complex128)  # column-major
+    cols = X.shape[0]
+    Y = np.zeros_like(X)
+    i0, i1 = rows if rows is not None else (0, N)
+    lib().or_coupling_apply(lmax, float(k0), N, pos.reshape(-1), cols, X.reshape(-1), Y.reshape(-1), i0, i1)
+    Y = Y.T
+    return Y[:, 0] if vec else Y
 
 
 # ------------------------------------------------------------------------- Mie

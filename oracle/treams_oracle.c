@@ -656,6 +656,47 @@ EXPORT void or_pair_matrix_pullback(int lmax, double k0, int N, const double *po
     plan_free(&plan);
 }
 
+/*
+ * Matrix-free y_i = sum_{j != i} C_ij x_j for `cols` column-major vectors, the
+ * per-pair plan application of cluster/iterative.rs:187-221 with sw/plan.rs:327-360
+ * (harmonic table once per pair, shared by the columns).  Only destination
+ * particles in [i0, i1) are computed (a bounded sample for timing); OpenMP over
+ * destinations, each thread owning its rows.
+ */
+EXPORT void or_coupling_apply(int lmax, double k0, int N, const double *positions, int cols, const cplx *x, cplx *y,
+                              int i0, int i1) {
+    Plan plan = plan_new(lmax);
+    int M = plan.M;
+    int64_t n = (int64_t)N * M;
+#pragma omp parallel
+    {
+        cplx *table = malloc(sizeof(cplx) * plan.len);
+#pragma omp for schedule(dynamic, 1)
+        for (int i = i0; i < i1; ++i) {
+            for (int c = 0; c < cols; ++c)
+                for (int r = 0; r < M; ++r) y[(int64_t)c * n + (int64_t)i * M + r] = 0.0;
+            for (int j = 0; j < N; ++j) {
+                if (j == i) continue;
+                double d[3];
+                for (int a = 0; a < 3; ++a) d[a] = positions[3 * i + a] - positions[3 * j + a];
+                plan_values(&plan, k0, d, 0, table);
+                for (int s = 0; s < M; ++s)
+                    for (int dd = 0; dd < M; ++dd) {
+                        int e = s * M + dd;
+                        if (plan.starts[e] == plan.starts[e + 1]) continue;
+                        cplx v = 0.0;
+                        for (int64_t t = plan.starts[e]; t < plan.starts[e + 1]; ++t)
+                            v += plan.weight[t] * table[plan.index[t]];
+                        for (int c = 0; c < cols; ++c)
+                            y[(int64_t)c * n + (int64_t)i * M + dd] += v * x[(int64_t)c * n + (int64_t)j * M + s];
+                    }
+            }
+        }
+        free(table);
+    }
+    plan_free(&plan);
+}
+
 /* --------------------------------------------------------------------- Mie -- */
 
 /* Riccati entries (f, f' + f/x) and their x-derivatives, coeffs/mie.rs:23-37. */

@@ -255,7 +255,7 @@ def scattering_gradient(cluster: IterativeSphereCluster, direction, pol, **optio
     rhx = c.overlap(x, adjoint=True)
     sca = float(torch.vdot(x, rx).real) * w
     G = (rx + rhx) * w
-    gk, gr, ge, gp, abar = ctx.pullback(G)
+    gk, gr, ge, gp, abar, adjoint_conv = ctx.pullback(G)
     gp = gp + c.overlap_pullback(x * w, x)
     ar = a.reshape(c.N, -1)
     ab = abar.reshape(c.N, -1)

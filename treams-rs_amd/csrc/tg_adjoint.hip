@@ -248,7 +248,8 @@ int tg_cluster_pullback(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x
     }
     const int64_t count = (int64_t)cols * c->n;
     DevBuf io;
-    TG_TRY(io.ensure(sizeof(cd) * count * 6 + sizeof(double) * (4 * (size_t)c->N + 2) + sizeof(cd) * c->N));
+    // 6 vectors, then gpos[3N], gr[N], gkp[N], gk[2], ge[N] (complex)
+    TG_TRY(io.ensure(sizeof(cd) * count * 6 + sizeof(double) * (5 * (size_t)c->N + 2) + sizeof(cd) * c->N));
     cd *a = io.as<cd>(), *X = a + count, *Gs = X + count, *Y = Gs + count, *U = Y + count, *Rsp = U + count;
     double *gpos = reinterpret_cast<double *>(Rsp + count);
     double *gr = gpos + 3 * c->N;
@@ -283,12 +284,13 @@ int tg_cluster_pullback(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x
         std::vector<double> h_pos(3 * c->N), h_r(c->N), h_kp(c->N);
         double h_gk = 0.0;
         std::vector<cd> h_e(c->N);
-        hipMemcpyAsync(h_pos.data(), gpos, sizeof(double) * 3 * c->N, hipMemcpyDeviceToHost, c->stream);
-        hipMemcpyAsync(h_r.data(), gr, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream);
-        hipMemcpyAsync(h_kp.data(), gkp, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream);
-        hipMemcpyAsync(&h_gk, gk, sizeof(double), hipMemcpyDeviceToHost, c->stream);
-        hipMemcpyAsync(h_e.data(), ge, sizeof(cd) * c->N, hipMemcpyDeviceToHost, c->stream);
-        if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        hipError_t e1 = hipMemcpyAsync(h_pos.data(), gpos, sizeof(double) * 3 * c->N, hipMemcpyDeviceToHost, c->stream);
+        hipError_t e2 = hipMemcpyAsync(h_r.data(), gr, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream);
+        hipError_t e3 = hipMemcpyAsync(h_kp.data(), gkp, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream);
+        hipError_t e4 = hipMemcpyAsync(&h_gk, gk, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        hipError_t e5 = hipMemcpyAsync(h_e.data(), ge, sizeof(cd) * c->N, hipMemcpyDeviceToHost, c->stream);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess || e5 != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
             set_error("pullback copy failed");
             st = TG_DEVICE_ERROR;
             break;
