@@ -1,0 +1,229 @@
+#!/usr/bin/env python
+"""Benchmark of the cluster-coupling hot path on one MI355X.
+
+Workload: N = 1024 spheres, lmax = 6 (98,304 unknowns), radii U[0.15, 0.35],
+permittivity U[2, 6] + i U[0, 0.2], centres in a cube of side 16 with surface gap
+>= 0.05, k0 = 1.3, four plane-wave right-hand sides (+-z x +-helicity).  One step =
+the whole job for one cluster: assemble the coupling (c64, helicity-split), build
+the preconditioner, and solve the four columns by mixed-precision GMRES(50) to a
+true FP64 relative residual <= 1e-8.  value = solves/s = 4 K / (time of K steps).
+
+Also measured in the same run: the plain FP64 block-Jacobi GMRES(50) on the same
+GPU (stored c128 coupling), the per-kernel roofline of the stored-coupling matvec
+and of the assembly (HIP events on the library's stream), and the CPU oracle
+(matrix-free FP64 apply, OpenMP) on a bounded sample, extrapolated to a solve.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s @N=1024,lmax=6 to 1e-8 resid: learned-PC vs plain; %HBM/MFMA peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def rhs_columns(cl, torch):
+    cols = [cl.plane_wave([0, 0, 1], 1), cl.plane_wave([0, 0, 1], 0), cl.plane_wave([0, 0, -1], 1),
+            cl.plane_wave([0, 0, -1], 0)]
+    return torch.stack(cols, dim=1)
+
+
+def run_method(cl, B, torch, *, mixed, pc, steps, warmup, rtol, restart, label):
+    """Time `steps` whole-job steps (storage + preconditioner + solve)."""
+    conv = None
+    x = None
+    for w in range(warmup):
+        cl.release()
+        x, conv = cl.solve(B, rtol=rtol, restart=restart, max_iterations=2000, mixed_precision=mixed,
+                           preconditioner=pc, convergence="full")
+        log(f"{label} warmup {w + 1}/{warmup}: iterations {[c.iterations for c in conv]}")
+    torch.cuda.synchronize()
+    cl.timing_reset()
+    cl.timing(True)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        ts = time.perf_counter()
+        cl.release()
+        if not mixed:
+            cl.assemble("c128")
+        x, conv = cl.solve(B, rtol=rtol, restart=restart, max_iterations=2000, mixed_precision=mixed,
+                           preconditioner=pc, convergence="full")
+        torch.cuda.synchronize()
+        log(f"{label} step {s + 1}/{steps}: {time.perf_counter() - ts:.3f} s, iterations "
+            f"{[c.iterations for c in conv]}, refinements {[c.refinements for c in conv]}")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    cl.timing(False)
+    kernels = {}
+    for name in ("assemble", "matvec_stored", "coupling_mf", "pc_apply", "pc_invert"):
+        ms, count = cl.timing_get(name)
+        if count:
+            kernels[name] = {"total_ms": ms, "launches": count, "avg_ms": ms / count}
+    return x, conv, elapsed, kernels
+
+
+def cpu_baseline(lmax, k0, pos, iterations, sample, oracle):
+    """Oracle matrix-free FP64 apply (OpenMP) on `sample` destination spheres x all
+    sources, 4 columns; extrapolated to a full apply and to the solve's iterations."""
+    N = pos.shape[0]
+    n = N * 2 * lmax * (lmax + 2)
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(n, 4)) + 1j * rng.normal(size=(n, 4))
+    oracle.coupling_apply(lmax, k0, pos, x, rows=(0, 1))  # warm the plan
+    t0 = time.perf_counter()
+    oracle.coupling_apply(lmax, k0, pos, x, rows=(0, sample))
+    dt = time.perf_counter() - t0
+    t_apply = dt * N / sample
+    t_solve = t_apply * (iterations + 2)  # Arnoldi applications plus true residuals
+    return {
+        "value": 4.0 / t_solve,
+        "unit": "solves/s",
+        "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+        "kind": "port",
+        "sample": (f"oracle matrix-free FP64 coupling apply (cluster/iterative.rs:187-221 restated, OpenMP) on "
+                   f"{sample} of {N} destination spheres x {N - 1} sources x 4 columns: {dt:.2f} s; "
+                   f"full apply extrapolated to {t_apply:.1f} s; one 4-column solve = {iterations + 2} applies "
+                   f"(iterations of unpreconditioned FP64 GMRES measured on the GPU)"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="n1024_l6")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--pc", default="block_jacobi")
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--restart", type=int, default=50)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-reference", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=24)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only "
+                                   "(see DESIGN.md), so --gpus must be 1"}))
+        return 2
+
+    import torch
+
+    from __graft_entry__ import load_package
+
+    pkg = load_package()
+    torch.cuda.set_device(0)
+    lmax, radii, eps, pos = pkg.workload(args.workload, seed=args.seed)
+    k0 = 1.3
+    N = radii.size
+    log(f"workload {args.workload}: N={N} lmax={lmax} n={N * 2 * lmax * (lmax + 2)}")
+    cl = pkg.IterativeSphereCluster(lmax, k0, radii, eps, pos)
+    n, H = cl.n, lmax * (lmax + 2)
+    nh = N * H
+    B = rhs_columns(cl, torch)
+    ncols = B.shape[1]
+
+    # mixed precision, learned / block-Jacobi preconditioner: the headline
+    x, conv, elapsed, kernels = run_method(cl, B, torch, mixed=True, pc=args.pc, steps=args.steps,
+                                           warmup=args.warmup, rtol=args.rtol, restart=args.restart,
+                                           label="mixed")
+    value = ncols * args.steps / elapsed
+    c64_bytes = 2 * nh * nh * 8
+    mv = kernels.get("matvec_stored", {})
+    roof = None
+    if mv:
+        achieved = c64_bytes / (mv["avg_ms"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": "k_matvec_stored<float2> (c64 helicity-split coupling, 4 RHS)",
+                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None, "bytes_per_launch": c64_bytes, "avg_ms": mv["avg_ms"]}
+    asm = kernels.get("assemble", {})
+    asm_roof = None
+    if asm:
+        achieved = c64_bytes / (asm["avg_ms"] * 1e-3) / 1e9
+        asm_roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "bytes_per_launch": c64_bytes, "avg_ms": asm["avg_ms"]}
+    iterations = [c.iterations for c in conv]
+    x_mixed = x
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 stored coupling and matvec (FP32 accumulate); c128 Krylov vectors, residuals, refinement",
+        "data": "synthetic",
+        "config": {"workload": f"N={N} spheres, lmax={lmax} ({n} unknowns), radii U[0.15,0.35], eps U[2,6]+iU[0,0.2], "
+                               f"cube side 16, gap 0.05, k0 1.3, 4 plane waves (+-z x +-helicity), "
+                               f"GMRES({args.restart}) to {args.rtol:g} true FP64 relative residual",
+                   "rhs": ncols, "seed": args.seed, "preconditioner": args.pc, "restart": args.restart},
+        "roofline": roof,
+        "assembly_roofline": asm_roof,
+        "kernels": kernels,
+        "iterations": iterations,
+        "refinements": [c.refinements for c in conv],
+        "true_relative_residual": [c.residual_norm / c.rhs_norm for c in conv],
+    }
+
+    ref_iterations = None
+    if not args.no_plain:
+        cl.release()
+        x64, conv64, el64, k64 = run_method(cl, B, torch, mixed=False, pc="block_jacobi", steps=1, warmup=0,
+                                            rtol=args.rtol, restart=args.restart, label="plain-fp64-bj")
+        plain = ncols / el64
+        result["plain_fp64_block_jacobi"] = {"value": plain, "unit": "solves/s", "seconds": el64,
+                                             "iterations": [c.iterations for c in conv64], "kernels": k64}
+        result["speedup_vs_plain_fp64_block_jacobi"] = value / plain
+        # cross sections of the mixed solution against the FP64 solution
+        errs = []
+        for c in range(ncols):
+            s1, e1 = cl.cross_sections(B[:, c], x_mixed[:, c])
+            s2, e2 = cl.cross_sections(B[:, c], x64[:, c])
+            errs.append(max(abs(s1 - s2) / abs(s2), abs(e1 - e2) / abs(e2)))
+        result["cross_section_rel_error_vs_fp64"] = max(errs)
+        result["solution_rel_error_vs_fp64"] = float(torch.linalg.norm(x_mixed - x64) / torch.linalg.norm(x64))
+        cl.release()
+    if not args.no_reference:
+        # the reference algorithm on the GPU: unpreconditioned matrix-free FP64 GMRES
+        t0 = time.perf_counter()
+        xr, convr = cl.solve(B, rtol=args.rtol, restart=args.restart, max_iterations=2000,
+                             convergence="full")
+        torch.cuda.synchronize()
+        elr = time.perf_counter() - t0
+        ref_iterations = max(c.iterations for c in convr)
+        result["reference_algorithm_gpu"] = {"value": ncols / elr, "unit": "solves/s", "seconds": elr,
+                                             "iterations": [c.iterations for c in convr],
+                                             "note": "unpreconditioned FP64 matrix-free GMRES, as the reference"}
+        log(f"reference algorithm on GPU: {elr:.2f} s, iterations {[c.iterations for c in convr]}")
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        its = ref_iterations if ref_iterations is not None else max(iterations)
+        result["cpu_baseline"] = cpu_baseline(lmax, k0, pos, its, args.cpu_sample, oracle)
+        log(f"cpu baseline: {result['cpu_baseline']['value']:.4g} solves/s")
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

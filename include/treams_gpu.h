@@ -144,6 +144,12 @@ int tg_translation_blocks(int lmax, double k, int64_t count, const double *displ
 int tg_cluster_assemble(tg_cluster *cluster, int storage);
 
 /*
+ * Free the stored coupling and the preconditioner; the next call that needs them
+ * rebuilds them.  No reference counterpart (device memory management).
+ */
+int tg_cluster_release(tg_cluster *cluster);
+
+/*
  * y = (I - T C) x, or (I - C^H T^H) x with adjoint != 0, for `cols` column-major
  * vectors of length n (device c128): replaces IterativeSphereCluster::apply
  * (cluster/iterative.rs:225-232).  op selects the matrix-free FP64 or stored path.

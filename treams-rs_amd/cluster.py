@@ -120,6 +120,10 @@ class IterativeSphereCluster:
         s = {"c64": _lib.TG_STORE_C64, "c128": _lib.TG_STORE_C128}[storage]
         check(lib().tg_cluster_assemble(self._h, s))
 
+    def release(self) -> None:
+        """Free the stored coupling and the preconditioner (rebuilt on demand)."""
+        check(lib().tg_cluster_release(self._h))
+
     # -------------------------------------------------------------- operators
     def _apply(self, fn, x, *args):
         cm, vector = _columns(x, self.n)

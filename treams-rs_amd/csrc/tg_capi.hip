@@ -354,6 +354,16 @@ int tg_cluster_assemble(tg_cluster *c, int storage) {
     return TG_OK;
 }
 
+int tg_cluster_release(tg_cluster *c) {
+    TG_TRY(check_cluster(c));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    destroy_preconditioner(c);
+    c->b_C.release();
+    c->storage = TG_STORE_NONE;
+    for (auto &w : c->ws) w.release();
+    return TG_OK;
+}
+
 // reference-layout in/out wrappers around split-layout work
 static int with_split(tg_cluster *c, const tg_c128 *x, tg_c128 *y, int cols,
                       const std::function<int(const cd *, cd *)> &body) {
