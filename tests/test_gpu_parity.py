@@ -110,6 +110,24 @@ def test_operator_apply_matrix_free_and_stored(pkg, oracle):
     assert _rel(cl.apply(x[:, 0], stored=True), ref.apply(x[:, 0])) <= 2e-6
 
 
+def test_matrix_free_apply_larger_cluster(pkg, oracle):
+    """N = 96, lmax = 6: pair-table matrix-free apply vs the oracle on a row subset
+    and vs the stored c128 product (an independent kernel) on every row."""
+    radii, e, pos = pkg.random_cluster(96, 7.0, 0.05, (0.15, 0.35), None, seed=21)
+    cl = pkg.IterativeSphereCluster(6, 1.3, radii, e, pos)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(cl.n, 3)) + 1j * rng.normal(size=(cl.n, 3))
+    y = cl.coupling(x).cpu().numpy()
+    rows = (10, 14)
+    yr = oracle.coupling_apply(6, 1.3, pos, x, rows=rows)
+    sl = slice(rows[0] * 96, rows[1] * 96)
+    assert _rel(y[sl], yr[sl]) <= 1e-12
+    cl.assemble("c128")
+    assert _rel(cl.coupling(x, stored=True), y) <= 1e-12
+    ya = cl.coupling(x, adjoint=True).cpu().numpy()
+    assert _rel(cl.coupling(x, adjoint=True, stored=True), ya) <= 1e-12
+
+
 def test_overlap_apply(pkg, oracle):
     cl, ref = _cluster(pkg, oracle, N=5, lmax=3, seed=9)
     rng = np.random.default_rng(4)

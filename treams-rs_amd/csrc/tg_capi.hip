@@ -294,6 +294,8 @@ int tg_cluster_destroy(tg_cluster *c) {
     c->b_pos.release();
     c->b_mie.release();
     c->b_C.release();
+    c->b_tab.release();
+    c->have_tables = false;
     for (auto &w : c->ws) w.release();
     c->timer.collect();
     delete c;
@@ -359,6 +361,8 @@ int tg_cluster_release(tg_cluster *c) {
     TG_HIP(hipStreamSynchronize(c->stream));
     destroy_preconditioner(c);
     c->b_C.release();
+    c->b_tab.release();
+    c->have_tables = false;
     c->storage = TG_STORE_NONE;
     for (auto &w : c->ws) w.release();
     return TG_OK;

@@ -80,6 +80,8 @@ struct tg_cluster {
     tg::DevBuf b_plan, b_pos, b_mie;   // b_mie: value, d/dsize, d/deps blocks
     int storage = TG_STORE_NONE;
     tg::DevBuf b_C;                    // stored split coupling: [2][nh][nh] (c64 or c128)
+    tg::DevBuf b_tab;                  // FP64 singular harmonic tables of the pairs i < j
+    bool have_tables = false;
     tg::DevBuf ws[8];                  // scratch
     tg::Timer timer;
     tg::Preconditioner *pc = nullptr;

@@ -5,6 +5,7 @@
 // modes (l, m) per sphere and helicity, so the same-helicity coupling is two
 // N H x N H matrices C_+ and C_-.  Stored couplings are column-major, source
 // column (j H + a), destination row (i H + b), with (l, m) index a = l^2 + l + m - 1.
+#include <algorithm>
 #include "tg_internal.h"
 #include "tg_special.h"
 
@@ -399,7 +400,203 @@ __global__ void k_sum_chunks(const PT *part, int nchunks, int64_t count, cd *out
 
 static int grid1(int64_t count, int block = 256) { return (int)((count + block - 1) / block); }
 
+// ------------------------------------------- matrix-free coupling with pair tables --
+
+// Index of the unordered pair (i < j) in row-major upper-triangle order.
+__device__ __forceinline__ int64_t pair_index(int i, int j, int N) {
+    return (int64_t)i * N - (int64_t)i * (i + 1) / 2 + (j - i - 1);
+}
+
+// Singular harmonic tables h_p(k r) P_p^q e^{i q phi} of every pair i < j at
+// d = pos_i - pos_j, one thread per (pair, q >= 0).  The pair (j, i) uses the same
+// table times (-1)^p, since the solid harmonic of degree p is odd or even in d.
+__global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, int64_t npairs, cd *tables) {
+    const int L2 = pl.L2;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs * (L2 + 1)) return;
+    const int64_t pidx = t / (L2 + 1);
+    const int q = (int)(t - pidx * (L2 + 1));
+    // invert pair_index: row i with start(i) <= pidx < start(i + 1)
+    int lo = 0, hi = N - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if ((int64_t)mid * N - (int64_t)mid * (mid + 1) / 2 <= pidx) lo = mid; else hi = mid - 1;
+    }
+    const int i = lo;
+    const int j = (int)(pidx - ((int64_t)i * N - (int64_t)i * (i + 1) / 2)) + i + 1;
+    double d[3], r, u[3];
+    for (int a = 0; a < 3; ++a) d[a] = pos[3 * i + a] - pos[3 * j + a];
+    cd *tab = tables + pidx * pl.T;
+    if (!unit_direction(d, r, u)) {
+        zero_column(q, L2, tab);
+        return;
+    }
+    cd rad[2 * kMaxLmax + 2];
+    tg_hankel_seq(L2, k * r, rad);
+    table_column(q, L2, u, rad, pl.refl, tab);
+}
+
+int ensure_pair_tables(tg_cluster *c) {
+    if (c->have_tables) return TG_OK;
+    const int64_t npairs = (int64_t)c->N * (c->N - 1) / 2;
+    TG_TRY(c->b_tab.ensure(sizeof(cd) * (size_t)std::max<int64_t>(npairs, 1) * c->plan.T));
+    const int64_t work = npairs * (c->plan.L2 + 1);
+    if (work > 0) {
+        k_pair_tables<<<grid1(work), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N, npairs,
+                                                          c->b_tab.as<cd>());
+        TG_HIP(hipGetLastError());
+    }
+    c->have_tables = true;
+    return TG_OK;
+}
+
+// y_i = sum_{j != i} C_ij x_j in FP64 from the stored pair tables.  Workgroup
+// (destination i, chunk of sources); thread (g, b) owns destination row b and the
+// source columns a = g (mod G).  Sources are taken PS at a time so every plan term a
+// thread loads serves PS pairs; the PS tables (signed for j < i) and source vectors
+// sit in LDS.  Partial sums per chunk go to part [chunk][col][h][nh].
+template <int NC, int PS>
+__global__ __launch_bounds__(256) void k_coupling_tab(DevPlan pl, const cd *tables, int N, int cols, const cd *x,
+                                                       int64_t n, int64_t nh, int chunk, cd *part) {
+    extern __shared__ cd sm[];
+    const int T = pl.T, H = pl.H;
+    const int G = blockDim.x / H;
+    cd *tab = sm;              // [PS][T]
+    cd *xs = tab + PS * T;     // [PS][cols][2][H]
+    __shared__ int src[PS];
+    const int tid = threadIdx.x;
+    const int g = tid / H, b = tid - g * H;
+    const int i = blockIdx.x;
+    const int j0 = blockIdx.y * chunk, j1 = min(N, j0 + chunk);
+    cd acc[NC][2];
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) { acc[cc][0] = cd(0.0); acc[cc][1] = cd(0.0); }
+    for (int jb = j0; jb < j1; jb += PS) {
+        __syncthreads();
+        if (tid < PS) {
+            int j = jb + tid;
+            src[tid] = (j < j1 && j != i) ? j : -1;
+        }
+        __syncthreads();
+        for (int t = tid; t < PS * T; t += blockDim.x) {
+            const int s = t / T, idx = t - s * T;
+            const int j = src[s];
+            cd v(0.0);
+            if (j >= 0) {
+                const int p = (int)sqrt((double)idx + 0.5);
+                if (i < j) {
+                    v = tables[pair_index(i, j, N) * T + idx];
+                } else {
+                    v = tables[pair_index(j, i, N) * T + idx];
+                    if (p & 1) v = -v;
+                }
+            }
+            tab[t] = v;
+        }
+        for (int t = tid; t < PS * cols * 2 * H; t += blockDim.x) {
+            const int s = t / (cols * 2 * H), rem = t - s * cols * 2 * H;
+            const int cc = rem / (2 * H), r2 = rem - cc * 2 * H, h = r2 / H, a = r2 - h * H;
+            const int j = src[s];
+            xs[t] = j >= 0 ? x[(int64_t)cc * n + (int64_t)h * nh + (int64_t)j * H + a] : cd(0.0);
+        }
+        __syncthreads();
+        if (tid < G * H) {
+            for (int a = g; a < H; a += G) {
+                const int e = a * H + b;
+                const int s0 = pl.starts[e], s1 = pl.starts[e + 1], ns = pl.nsame[e];
+                cd same[PS], cross[PS];
+#pragma unroll
+                for (int s = 0; s < PS; ++s) { same[s] = cd(0.0); cross[s] = cd(0.0); }
+                for (int t = s0; t < s0 + ns; ++t) {
+                    const double wt = pl.w[t];
+                    const int hh = pl.hidx[t];
+#pragma unroll
+                    for (int s = 0; s < PS; ++s) same[s] += wt * tab[s * T + hh];
+                }
+                for (int t = s0 + ns; t < s1; ++t) {
+                    const double wt = pl.w[t];
+                    const int hh = pl.hidx[t];
+#pragma unroll
+                    for (int s = 0; s < PS; ++s) cross[s] += wt * tab[s * T + hh];
+                }
+#pragma unroll
+                for (int s = 0; s < PS; ++s) {
+                    const cd cp(same[s].re - cross[s].im, same[s].im + cross[s].re);
+                    const cd cm(same[s].re + cross[s].im, same[s].im - cross[s].re);
+                    const cd *xv = xs + (s * cols) * 2 * H;
+#pragma unroll
+                    for (int cc = 0; cc < NC; ++cc)
+                        if (cc < cols) {
+                            acc[cc][0] += cp * xv[cc * 2 * H + a];
+                            acc[cc][1] += cm * xv[cc * 2 * H + H + a];
+                        }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    cd *red = sm;  // [G][cols][2][H]
+    if (tid < G * H) {
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc)
+            if (cc < cols) {
+                red[((g * cols + cc) * 2 + 0) * H + b] = acc[cc][0];
+                red[((g * cols + cc) * 2 + 1) * H + b] = acc[cc][1];
+            }
+    }
+    __syncthreads();
+    for (int t = tid; t < cols * 2 * H; t += blockDim.x) {
+        cd s(0.0);
+        for (int gg = 0; gg < G; ++gg) s += red[gg * cols * 2 * H + t];
+        const int cc = t / (2 * H), rem = t - cc * 2 * H, h = rem / H, bb = rem - h * H;
+        part[((int64_t)blockIdx.y * cols + cc) * n + (int64_t)h * nh + (int64_t)i * H + bb] = s;
+    }
+}
+
+static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols) {
+    TG_TRY(ensure_pair_tables(c));
+    const int H = c->H;
+    const int G = 256 / H;
+    const int block = G * H;
+    constexpr int PS = 4;
+    int chunk = c->N;
+    while ((int64_t)c->N * ((c->N + chunk - 1) / chunk) < 8192 && chunk > 16) chunk = (chunk + 1) / 2;
+    const int nchunks = (c->N + chunk - 1) / chunk;
+    // columns per launch: LDS (tables + source vectors, or the group reduction) <= 64 KB
+    int ncmax = 8;
+    while (ncmax > 1 && ((size_t)PS * c->plan.T + (size_t)PS * ncmax * 2 * H > 4096 ||
+                         (size_t)G * ncmax * 2 * H > 4096))
+        --ncmax;
+    TG_TRY(c->ws[0].ensure(sizeof(cd) * (size_t)std::min(cols, ncmax) * c->n * nchunks));
+    cd *part = c->ws[0].as<cd>();
+    Timer::Rec rec;
+    TG_TRY(c->timer.begin("coupling_mf", c->stream, rec));
+    for (int c0 = 0; c0 < cols; c0 += ncmax) {
+        const int nc = cols - c0 < ncmax ? cols - c0 : ncmax;
+        const cd *xx = x + (int64_t)c0 * c->n;
+        size_t smem = sizeof(cd) * ((size_t)PS * c->plan.T + (size_t)PS * nc * 2 * H);
+        size_t red = sizeof(cd) * (size_t)G * nc * 2 * H;
+        if (red > smem) smem = red;
+        dim3 grid(c->N, nchunks);
+#define TG_TAB(NCV)                                                                                        \
+    k_coupling_tab<NCV, PS><<<grid, block, smem, c->stream>>>(c->plan, c->b_tab.as<cd>(), c->N, nc, xx, c->n, \
+                                                              c->nh, chunk, part);
+        if (nc == 1) { TG_TAB(1) } else if (nc == 2) { TG_TAB(2) } else if (nc <= 4) { TG_TAB(4) } else { TG_TAB(8) }
+#undef TG_TAB
+        TG_HIP(hipGetLastError());
+        const int64_t cnt = (int64_t)nc * c->n;
+        k_sum_chunks<double2><<<grid1(cnt), 256, 0, c->stream>>>((const double2 *)part, nchunks, cnt,
+                                                                  y + (int64_t)c0 * c->n);
+        TG_HIP(hipGetLastError());
+    }
+    TG_TRY(c->timer.end("coupling_mf", c->stream, rec));
+    return TG_OK;
+}
+
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular) {
+    // singular couplings reuse the cluster's pair tables; the regular overlap
+    // (cross sections only) recomputes its tables per pair
+    if (!regular && c->N > 1) return coupling_tables(c, x, y, cols);
     const int H = c->H;
     const int G = 256 / H;
     const int block = G * H;
@@ -449,6 +646,12 @@ template <class S> struct acc_of;
 template <> struct acc_of<float2> { using T = float; using V = float2; };
 template <> struct acc_of<double2> { using T = double; using V = double2; };
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <class S> struct vec16;
+template <> struct vec16<float2> { using T = f32x4; static constexpr int R = 2; };   // two c64 rows
+template <> struct vec16<double2> { using T = f64x2; static constexpr int R = 1; };  // one c128 row
+
 // y_h = C_h x_h for the stored split coupling, both helicities (grid.z), split over
 // column chunks (grid.y).  Each thread owns two consecutive rows and streams the
 // chunk's columns with 16/32-byte loads; x of the chunk sits in LDS in the
@@ -479,52 +682,60 @@ __global__ __launch_bounds__(256) void k_matvec_stored(const S *C, int64_t nh, c
         xs[t] = v;
     }
     __syncthreads();
-    const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
+    // R consecutive rows per thread, read as one 16-byte non-temporal load per column
+    constexpr int R = vec16<S>::R;
+    using V16 = typename vec16<S>::T;
+    const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * R;
     if (r0 >= nh) return;
-    const bool pair = r0 + 1 < nh;
-    A acc[NC][2][2];
+    const bool whole = (nh % R == 0);
+    A acc[NC][R][2];
 #pragma unroll
     for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
-        for (int r = 0; r < 2; ++r) acc[cc][r][0] = acc[cc][r][1] = 0;
-    const S *p = Ch + c0 * nh + r0;
-    int k = 0;
-    if (pair) {
+        for (int r = 0; r < R; ++r) acc[cc][r][0] = acc[cc][r][1] = 0;
+    if (whole) {
+        const V16 *p = reinterpret_cast<const V16 *>(Ch + c0 * nh + r0);
+        const int64_t stride = nh / R;
 #pragma unroll 8
-        for (; k < kc; ++k) {
-            S v0 = p[(int64_t)k * nh], v1 = p[(int64_t)k * nh + 1];
+        for (int k = 0; k < kc; ++k) {
+            V16 v = __builtin_nontemporal_load(p + (int64_t)k * stride);
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) {
                 AV xv = xs[k * NC + cc];
-                acc[cc][0][0] += (A)v0.x * xv.x - (A)v0.y * xv.y;
-                acc[cc][0][1] += (A)v0.x * xv.y + (A)v0.y * xv.x;
-                acc[cc][1][0] += (A)v1.x * xv.x - (A)v1.y * xv.y;
-                acc[cc][1][1] += (A)v1.x * xv.y + (A)v1.y * xv.x;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const A re = (A)v[2 * r], im = (A)v[2 * r + 1];
+                    acc[cc][r][0] += re * xv.x - im * xv.y;
+                    acc[cc][r][1] += re * xv.y + im * xv.x;
+                }
             }
         }
     } else {
-        for (; k < kc; ++k) {
-            S v0 = p[(int64_t)k * nh];
+        const S *p = Ch + c0 * nh + r0;
+        for (int k = 0; k < kc; ++k)
 #pragma unroll
-            for (int cc = 0; cc < NC; ++cc) {
-                AV xv = xs[k * NC + cc];
-                acc[cc][0][0] += (A)v0.x * xv.x - (A)v0.y * xv.y;
-                acc[cc][0][1] += (A)v0.x * xv.y + (A)v0.y * xv.x;
+            for (int r = 0; r < R; ++r) {
+                if (r0 + r >= nh) break;
+                S v0 = p[(int64_t)k * nh + r];
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) {
+                    AV xv = xs[k * NC + cc];
+                    acc[cc][r][0] += (A)v0.x * xv.x - (A)v0.y * xv.y;
+                    acc[cc][r][1] += (A)v0.x * xv.y + (A)v0.y * xv.x;
+                }
             }
-        }
     }
 #pragma unroll
     for (int cc = 0; cc < NC; ++cc) {
         if (cc >= cols) break;
         AV *o = part + ((int64_t)blockIdx.y * cols + cc) * n + (int64_t)h * nh + r0;
-        AV v;
-        v.x = acc[cc][0][0];
-        v.y = acc[cc][0][1];
-        o[0] = v;
-        if (pair) {
-            v.x = acc[cc][1][0];
-            v.y = acc[cc][1][1];
-            o[1] = v;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r0 + r >= nh) break;
+            AV v;
+            v.x = acc[cc][r][0];
+            v.y = acc[cc][r][1];
+            o[r] = v;
         }
     }
 }
@@ -543,7 +754,8 @@ static int matvec_stored_t(tg_cluster *c, const cd *x, cd *y, int cols) {
     TG_TRY(c->timer.begin("matvec_stored", c->stream, rec));
     for (int c0 = 0; c0 < cols; c0 += 8) {
         const int nc = cols - c0 < 8 ? cols - c0 : 8;
-        dim3 grid((unsigned)((nh + 511) / 512), nk, 2);
+        const int64_t rows_per_block = 256 * vec16<S>::R;
+        dim3 grid((unsigned)((nh + rows_per_block - 1) / rows_per_block), nk, 2);
         const cd *xx = x + (int64_t)c0 * c->n;
 #define TG_MV(NCV)                                                                                           \
     k_matvec_stored<S, NCV><<<grid, 256, sizeof(AV) * kchunk * NCV, c->stream>>>(C, nh, xx, c->n, nc, kchunk, \
