@@ -214,6 +214,61 @@ int tg_cluster_pullback(tg_cluster *cluster, const tg_c128 *incident_dev, const 
 int tg_cluster_overlap_pullback(tg_cluster *cluster, const tg_c128 *u_dev, const tg_c128 *v_dev, int cols,
                                 double *grad_positions);
 
+/* ---------------------------------------------------------------------------
+ * Preconditioners (no reference counterpart: the reference GMRES is
+ * unpreconditioned).  Right preconditioning: iteration counts change, the answer
+ * and its FP64 true-residual certificate do not.
+ * ------------------------------------------------------------------------- */
+
+/* Spheres per block-Jacobi group (spatial recursive bisection), default 4. */
+int tg_cluster_set_group_size(tg_cluster *cluster, int spheres);
+
+/*
+ * Learned near-field preconditioner M^{-1} x = x + T sum_{(i,j)} D(alpha_ij) C_ij x_j
+ * over the k nearest neighbours j of each sphere i; D(alpha) scales destination
+ * degree l by alpha_ij[l].  tg_cluster_near_pairs builds the pair list, the pair
+ * blocks and the 64 FP32 features per pair (k d, 1/(k d), direction cosines, Mie
+ * blocks of both spheres for l <= 6).  Pair weights come from 16 FP32 outputs per
+ * pair, alpha[l] = 1 + out[2l] + i out[2l+1].
+ */
+int tg_cluster_near_pairs(tg_cluster *cluster, int k_nearest, int64_t *npairs);
+/* (i, j) of every pair, host int32[2 * npairs]. */
+int tg_cluster_pair_list(tg_cluster *cluster, int32_t *pairs_host);
+/* Copy the pair features (device float[npairs * 64]). */
+int tg_cluster_pair_features(tg_cluster *cluster, float *features_dev);
+/* Set the pair weights from network outputs (device float[npairs * 16]). */
+int tg_cluster_set_pair_output(tg_cluster *cluster, const float *out_dev);
+/* y = M^{-1} x (or M^{-H} x), reference layout, device vectors. */
+int tg_cluster_learned_apply(tg_cluster *cluster, const tg_c128 *x_dev, tg_c128 *y_dev, int cols, int adjoint);
+/*
+ * Training objective of the pair weights for probe columns V (device, n x cols):
+ * loss = |A M^{-1} V - V|_F^2 / |V|_F^2 with A = I - T C in FP64, and, when
+ * dout_dev != NULL, d loss / d out (device float[npairs * 16]).
+ */
+int tg_cluster_learned_loss(tg_cluster *cluster, const tg_c128 *probes_dev, int cols, double *loss, float *dout_dev);
+
+/* Network evaluated for TG_PC_LEARNED solves (not owned; NULL detaches). */
+typedef struct tg_mlp tg_mlp;
+int tg_cluster_set_mlp(tg_cluster *cluster, tg_mlp *mlp);
+
+/*
+ * Multilayer perceptron on MFMA (BF16 operands, FP32 accumulation), FP32 master
+ * weights: n_in -> width (x hidden_layers, ReLU) -> n_out.  The last layer starts
+ * at zero.  Parameters are ordered layer by layer, W (row-major out x in) then b.
+ */
+int tg_mlp_create(int n_in, int width, int n_out, int hidden_layers, uint64_t seed, void *stream, tg_mlp **out);
+int tg_mlp_destroy(tg_mlp *mlp);
+int64_t tg_mlp_param_count(const tg_mlp *mlp);
+int tg_mlp_get_params(tg_mlp *mlp, float *host);
+int tg_mlp_set_params(tg_mlp *mlp, const float *host);
+int tg_mlp_get_grads(tg_mlp *mlp, float *host);
+int tg_mlp_zero_grad(tg_mlp *mlp);
+/* y = network(x) for a batch (device float[batch * n_in] -> float[batch * n_out]). */
+int tg_mlp_forward(tg_mlp *mlp, const float *x_dev, int64_t batch, float *y_dev);
+/* Accumulate parameter gradients of sum(dy * y) for the last forward batch. */
+int tg_mlp_backward(tg_mlp *mlp, const float *dy_dev);
+int tg_mlp_adam_step(tg_mlp *mlp, float lr, float beta1, float beta2, float eps, int step);
+
 /* Kernel timing (HIP events on the handle's stream) for measurement. */
 int tg_cluster_timing_enable(tg_cluster *cluster, int enable);
 /* Total milliseconds and launch count of a timed kernel by name; 0 launches if unknown. */

@@ -31,7 +31,11 @@ EXPORTED = [
     "tg_translation_blocks", "tg_cluster_assemble", "tg_cluster_release", "tg_cluster_apply", "tg_cluster_coupling",
     "tg_cluster_local", "tg_cluster_overlap", "tg_cluster_plane_wave", "tg_cluster_solve", "tg_cluster_cross_sections",
     "tg_cluster_pullback", "tg_cluster_overlap_pullback", "tg_cluster_timing_enable",
-    "tg_cluster_timing_get", "tg_cluster_timing_reset",
+    "tg_cluster_timing_get", "tg_cluster_timing_reset", "tg_cluster_set_group_size", "tg_cluster_near_pairs",
+    "tg_cluster_pair_list", "tg_cluster_pair_features", "tg_cluster_set_pair_output", "tg_cluster_learned_apply",
+    "tg_cluster_learned_loss", "tg_cluster_set_mlp", "tg_mlp_create", "tg_mlp_destroy", "tg_mlp_param_count",
+    "tg_mlp_get_params", "tg_mlp_set_params", "tg_mlp_get_grads", "tg_mlp_zero_grad", "tg_mlp_forward",
+    "tg_mlp_backward", "tg_mlp_adam_step",
 ]
 
 
@@ -102,6 +106,24 @@ def lib():
         "tg_cluster_timing_enable": (_i, [_vp, _i]),
         "tg_cluster_timing_get": (_i, [_vp, ctypes.c_char_p, ctypes.POINTER(_d), ctypes.POINTER(_i64)]),
         "tg_cluster_timing_reset": (_i, [_vp]),
+        "tg_cluster_set_group_size": (_i, [_vp, _i]),
+        "tg_cluster_near_pairs": (_i, [_vp, _i, ctypes.POINTER(_i64)]),
+        "tg_cluster_pair_list": (_i, [_vp, _vp]),
+        "tg_cluster_pair_features": (_i, [_vp, _vp]),
+        "tg_cluster_set_pair_output": (_i, [_vp, _vp]),
+        "tg_cluster_learned_apply": (_i, [_vp, _vp, _vp, _i, _i]),
+        "tg_cluster_learned_loss": (_i, [_vp, _vp, _i, ctypes.POINTER(_d), _vp]),
+        "tg_cluster_set_mlp": (_i, [_vp, _vp]),
+        "tg_mlp_create": (_i, [_i, _i, _i, _i, ctypes.c_uint64, _vp, ctypes.POINTER(_vp)]),
+        "tg_mlp_destroy": (_i, [_vp]),
+        "tg_mlp_param_count": (_i64, [_vp]),
+        "tg_mlp_get_params": (_i, [_vp, _vp]),
+        "tg_mlp_set_params": (_i, [_vp, _vp]),
+        "tg_mlp_get_grads": (_i, [_vp, _vp]),
+        "tg_mlp_zero_grad": (_i, [_vp]),
+        "tg_mlp_forward": (_i, [_vp, _vp, _i64, _vp]),
+        "tg_mlp_backward": (_i, [_vp, _vp]),
+        "tg_mlp_adam_step": (_i, [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

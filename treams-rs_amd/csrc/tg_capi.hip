@@ -296,6 +296,7 @@ int tg_cluster_destroy(tg_cluster *c) {
     c->b_C.release();
     c->b_tab.release();
     c->have_tables = false;
+    learned_release(c);
     for (auto &w : c->ws) w.release();
     c->timer.collect();
     delete c;
@@ -365,6 +366,17 @@ int tg_cluster_release(tg_cluster *c) {
     c->have_tables = false;
     c->storage = TG_STORE_NONE;
     for (auto &w : c->ws) w.release();
+    return TG_OK;
+}
+
+int tg_cluster_set_group_size(tg_cluster *c, int spheres) {
+    TG_TRY(check_cluster(c));
+    if (spheres < 1 || spheres > 16) {
+        set_error("group size must be 1..16 spheres");
+        return TG_INVALID_INPUT;
+    }
+    c->group_size = spheres;
+    if (c->pc_kind == TG_PC_BLOCK_JACOBI) destroy_preconditioner(c);
     return TG_OK;
 }
 

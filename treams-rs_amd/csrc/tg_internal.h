@@ -86,6 +86,9 @@ struct tg_cluster {
     tg::Timer timer;
     tg::Preconditioner *pc = nullptr;
     int pc_kind = TG_PC_NONE;
+    struct LearnedState *learned = nullptr;  // near pairs, features, weights (tg_learned.hip)
+    struct tg_mlp *mlp = nullptr;            // network attached for TG_PC_LEARNED (not owned)
+    int group_size = 4;                      // block-Jacobi group size
 };
 
 namespace tg {

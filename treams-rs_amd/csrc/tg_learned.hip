@@ -1,0 +1,444 @@
+// Learned near-field preconditioner (no reference counterpart; right preconditioning,
+// so it changes iteration counts and never the answer, which the FP64 true residual
+// certifies).
+//
+//   M^{-1} x = x + T ( sum_{(i,j) in K-nearest pairs} D(alpha_ij) C_ij x_j )
+//
+// D(alpha) scales the rows of destination degree l by alpha_ij[l] (complex, both
+// helicities, all orders).  alpha = 1 is the first-order Neumann preconditioner of the
+// near field; an MLP maps per-pair features (k d, direction cosines, the two spheres'
+// Mie blocks) to alpha - 1.  Training minimises E_V |A M^{-1} V - V|^2 / |V|^2 over
+// random probe blocks V, whose gradient with respect to alpha_ij[l] is
+//   2 sum_{rows r of degree l of sphere i} Z_i[r] conj((C_ij V_j)[r]) / |V|^2,
+// Z = T^H A^H (A M^{-1} V - V).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "tg_gmres.h"
+#include "tg_internal.h"
+#include "tg_mlp.h"
+#include "tg_precond.h"
+
+struct LearnedState {
+    int K = 0;
+    int64_t npairs = 0;
+    tg::DevBuf meta;    // pi[np], pj[np], row_start[N+1], col_start[N+1], col_list[np]
+    tg::DevBuf blocks;  // [np][M x M] c128, reference layout, column-major
+    tg::DevBuf feats;   // [np][64] fp32
+    tg::DevBuf out16;   // [np][16] fp32 network output
+    tg::DevBuf alpha;   // [np][lmax] c128
+    tg::DevBuf work;    // training scratch
+    const int *pi = nullptr, *pj = nullptr, *row_start = nullptr, *col_start = nullptr, *col_list = nullptr;
+    bool have_alpha = false;
+};
+
+namespace tg {
+
+constexpr int kFeat = 64, kOut = 16;
+
+static inline int deg_of_row(int r) { return (int)std::sqrt((double)((r >> 1) + 1) + 0.5); }
+
+__device__ __forceinline__ int row_degree(int r) { return (int)sqrt((double)((r >> 1) + 1) + 0.5); }
+
+// features: k d, 1/(k d), d/|d| (d = pos_i - pos_j), then the 2x2 Mie blocks (diagonal
+// and off-diagonal, re/im) of sphere i and of sphere j for l = 1..6; zero padded.
+__global__ void k_pair_features(int64_t np, const int *pi, const int *pj, const double *pos, const cd *mie, int lmax,
+                                double k0, float *out) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int i = pi[p], j = pj[p];
+    double d[3];
+    for (int a = 0; a < 3; ++a) d[a] = pos[3 * i + a] - pos[3 * j + a];
+    const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    float *f = out + p * kFeat;
+    for (int t = 0; t < kFeat; ++t) f[t] = 0.f;
+    f[0] = (float)(k0 * r);
+    f[1] = (float)(1.0 / (k0 * r));
+    for (int a = 0; a < 3; ++a) f[2 + a] = (float)(d[a] / r);
+    for (int s = 0; s < 2; ++s) {
+        const int q = s == 0 ? i : j;
+        for (int l = 0; l < lmax && l < 6; ++l) {
+            const cd *b = mie + (size_t)(q * lmax + l) * 4;
+            float *g = f + 5 + s * 24 + l * 4;
+            g[0] = (float)b[0].re; g[1] = (float)b[0].im; g[2] = (float)b[1].re; g[3] = (float)b[1].im;
+        }
+    }
+}
+
+__global__ void k_alpha_from_output(int64_t np, int lmax, const float *out, cd *alpha) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np * lmax) return;
+    const int64_t p = t / lmax;
+    const int l = (int)(t - p * lmax);
+    const float re = l < kOut / 2 ? out[p * kOut + 2 * l] : 0.f, im = l < kOut / 2 ? out[p * kOut + 2 * l + 1] : 0.f;
+    alpha[t] = cd(1.0 + (double)re, (double)im);
+}
+
+// y_i = x_i + T_i sum_{pairs p of i} D(alpha_p) C_p x_{j_p}  (split layout; grid N x cols,
+// one thread per row r of sphere i in reference mode order).
+__global__ void k_near_apply(int N, int M, int H, int lmax, int64_t n, int64_t nh, const int *row_start, const int *pj,
+                             const cd *blocks, const cd *alpha, const cd *mie, const cd *x, cd *y) {
+    extern __shared__ cd acc_s[];
+    const int i = blockIdx.x, col = blockIdx.y;
+    const cd *xc = x + (int64_t)col * n;
+    const int r = threadIdx.x;
+    cd acc(0.0);
+    if (r < M) {
+        const int l = row_degree(r);
+        for (int p = row_start[i]; p < row_start[i + 1]; ++p) {
+            const int j = pj[p];
+            const cd *blk = blocks + (int64_t)p * M * M;
+            cd s(0.0);
+            for (int c = 0; c < M; ++c) s += blk[(int64_t)c * M + r] * xc[(int64_t)(c & 1) * nh + (int64_t)j * H + (c >> 1)];
+            acc += alpha[(int64_t)p * lmax + (l - 1)] * s;
+        }
+        acc_s[r] = acc;
+    }
+    __syncthreads();
+    if (r < M) {
+        const int b = r >> 1, h = r & 1, l = row_degree(r);
+        const cd *t2 = mie + (size_t)(i * lmax + l - 1) * 4;
+        const cd v = t2[h * 2 + 0] * acc_s[2 * b] + t2[h * 2 + 1] * acc_s[2 * b + 1];
+        const int64_t idx = (int64_t)col * n + (int64_t)h * nh + (int64_t)i * H + b;
+        y[idx] = x[idx] + v;
+    }
+}
+
+// y_j = x_j + sum_{pairs p with source j} C_p^H D(conj alpha_p) z_{i_p}, z = T^H x.
+__global__ void k_near_apply_adjoint(int N, int M, int H, int lmax, int64_t n, int64_t nh, const int *col_start,
+                                     const int *col_list, const int *pi, const cd *blocks, const cd *alpha,
+                                     const cd *z, const cd *x, cd *y) {
+    const int j = blockIdx.x, col = blockIdx.y;
+    const int c = threadIdx.x;
+    if (c >= M) return;
+    const cd *zc = z + (int64_t)col * n;
+    cd acc(0.0);
+    for (int q = col_start[j]; q < col_start[j + 1]; ++q) {
+        const int p = col_list[q];
+        const int i = pi[p];
+        const cd *blk = blocks + (int64_t)p * M * M + (int64_t)c * M;
+        for (int r = 0; r < M; ++r) {
+            const int l = row_degree(r);
+            acc += conj(blk[r]) * (conj(alpha[(int64_t)p * lmax + l - 1]) *
+                                   zc[(int64_t)(r & 1) * nh + (int64_t)i * H + (r >> 1)]);
+        }
+    }
+    const int64_t idx = (int64_t)col * n + (int64_t)(c & 1) * nh + (int64_t)j * H + (c >> 1);
+    y[idx] = x[idx] + acc;
+}
+
+// P_p[r, col] = sum_c C_p[r, c] V_{j_p}[c, col]   (grid np, one thread per row)
+__global__ void k_pair_products(int64_t np, int M, int H, int64_t n, int64_t nh, int cols, const int *pj,
+                                const cd *blocks, const cd *V, cd *P) {
+    const int64_t p = blockIdx.x;
+    const int r = threadIdx.x;
+    if (r >= M) return;
+    const int j = pj[p];
+    const cd *blk = blocks + p * M * M;
+    for (int cc = 0; cc < cols; ++cc) {
+        const cd *vc = V + (int64_t)cc * n;
+        cd s(0.0);
+        for (int c = 0; c < M; ++c) s += blk[(int64_t)c * M + r] * vc[(int64_t)(c & 1) * nh + (int64_t)j * H + (c >> 1)];
+        P[(p * cols + cc) * M + r] = s;
+    }
+}
+
+// dL/d(out_p) = 2 sum_{rows of degree l} Z_i conj(P_p) / |V|^2 as (re, im) pairs.
+__global__ void k_alpha_grad(int64_t np, int M, int H, int lmax, int64_t n, int64_t nh, int cols, const int *pi,
+                             const cd *Z, const cd *P, double scale, float *dout) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int i = pi[p];
+    for (int t = 0; t < kOut; ++t) dout[p * kOut + t] = 0.f;
+    for (int l = 1; l <= lmax && l <= kOut / 2; ++l) {
+        cd g(0.0);
+        for (int b = l * l - 1; b < l * l + 2 * l; ++b)
+            for (int h = 0; h < 2; ++h) {
+                const int r = 2 * b + h;
+                for (int cc = 0; cc < cols; ++cc)
+                    g += Z[(int64_t)cc * n + (int64_t)h * nh + (int64_t)i * H + b] * conj(P[(p * cols + cc) * M + r]);
+            }
+        dout[p * kOut + 2 * (l - 1)] = (float)(2.0 * scale * g.re);
+        dout[p * kOut + 2 * (l - 1) + 1] = (float)(2.0 * scale * g.im);
+    }
+}
+
+static LearnedState *state(tg_cluster *c) {
+    if (!c->learned) c->learned = new LearnedState();
+    return c->learned;
+}
+
+// K nearest neighbours of every sphere (host), near blocks and features (device).
+int learned_pairs(tg_cluster *c, int K) {
+    LearnedState *s = state(c);
+    if (s->npairs > 0 && s->K == K) return TG_OK;
+    const int N = c->N;
+    K = std::min(K, N - 1);
+    std::vector<int> pi, pj;
+    std::vector<std::pair<double, int>> dist(N);
+    for (int i = 0; i < N; ++i) {
+        int m = 0;
+        for (int j = 0; j < N; ++j) {
+            if (j == i) continue;
+            double d = 0;
+            for (int a = 0; a < 3; ++a) {
+                double t = c->positions[3 * i + a] - c->positions[3 * j + a];
+                d += t * t;
+            }
+            dist[m++] = {d, j};
+        }
+        std::partial_sort(dist.begin(), dist.begin() + K, dist.begin() + m);
+        std::vector<int> nb;
+        for (int t = 0; t < K; ++t) nb.push_back(dist[t].second);
+        std::sort(nb.begin(), nb.end());
+        for (int j : nb) {
+            pi.push_back(i);
+            pj.push_back(j);
+        }
+    }
+    const int64_t np = (int64_t)pi.size();
+    std::vector<int> row_start(N + 1, 0), col_start(N + 1, 0), col_list(np);
+    for (int64_t p = 0; p < np; ++p) {
+        row_start[pi[p] + 1]++;
+        col_start[pj[p] + 1]++;
+    }
+    for (int i = 0; i < N; ++i) {
+        row_start[i + 1] += row_start[i];
+        col_start[i + 1] += col_start[i];
+    }
+    std::vector<int> fill(col_start.begin(), col_start.end() - 1);
+    for (int64_t p = 0; p < np; ++p) col_list[fill[pj[p]]++] = (int)p;
+    std::vector<int> host;
+    host.insert(host.end(), pi.begin(), pi.end());
+    host.insert(host.end(), pj.begin(), pj.end());
+    host.insert(host.end(), row_start.begin(), row_start.end());
+    host.insert(host.end(), col_start.begin(), col_start.end());
+    host.insert(host.end(), col_list.begin(), col_list.end());
+    TG_TRY(s->meta.ensure(sizeof(int) * host.size()));
+    TG_HIP(hipMemcpy(s->meta.ptr, host.data(), sizeof(int) * host.size(), hipMemcpyHostToDevice));
+    int *base = s->meta.as<int>();
+    s->pi = base;
+    s->pj = base + np;
+    s->row_start = base + 2 * np;
+    s->col_start = s->row_start + N + 1;
+    s->col_list = s->col_start + N + 1;
+    s->npairs = np;
+    s->K = K;
+    s->have_alpha = false;
+    // near blocks C_ij = translation at pos_i - pos_j
+    std::vector<double> disp(3 * np);
+    for (int64_t p = 0; p < np; ++p)
+        for (int a = 0; a < 3; ++a) disp[3 * p + a] = c->positions[3 * pi[p] + a] - c->positions[3 * pj[p] + a];
+    DevBuf ddisp;
+    TG_TRY(ddisp.ensure(sizeof(double) * disp.size()));
+    TG_HIP(hipMemcpy(ddisp.ptr, disp.data(), sizeof(double) * disp.size(), hipMemcpyHostToDevice));
+    const int M = c->M;
+    TG_TRY(s->blocks.ensure(sizeof(cd) * (size_t)np * M * M));
+    TG_TRY(launch_translation_blocks(c->lmax, c->plan, c->k0, np, ddisp.as<double>(), 0, s->blocks.as<cd>(),
+                                     c->stream));
+    TG_TRY(s->feats.ensure(sizeof(float) * (size_t)np * kFeat));
+    k_pair_features<<<(unsigned)((np + 127) / 128), 128, 0, c->stream>>>(
+        np, s->pi, s->pj, c->b_pos.as<double>(), c->b_mie.as<cd>(), c->lmax, c->k0, s->feats.as<float>());
+    TG_HIP(hipGetLastError());
+    TG_TRY(s->alpha.ensure(sizeof(cd) * (size_t)np * c->lmax));
+    TG_TRY(s->out16.ensure(sizeof(float) * (size_t)np * kOut));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    ddisp.release();
+    return TG_OK;
+}
+
+int learned_set_output(tg_cluster *c, const float *out16_dev) {
+    LearnedState *s = state(c);
+    if (s->npairs == 0) {
+        set_error("near pairs not built: call tg_cluster_near_pairs first");
+        return TG_INVALID_INPUT;
+    }
+    if (out16_dev != s->out16.as<float>())
+        TG_HIP(hipMemcpyAsync(s->out16.ptr, out16_dev, sizeof(float) * s->npairs * kOut, hipMemcpyDeviceToDevice,
+                              c->stream));
+    const int64_t cnt = s->npairs * c->lmax;
+    k_alpha_from_output<<<(unsigned)((cnt + 255) / 256), 256, 0, c->stream>>>(s->npairs, c->lmax, s->out16.as<float>(),
+                                                                               s->alpha.as<cd>());
+    TG_HIP(hipGetLastError());
+    s->have_alpha = true;
+    return TG_OK;
+}
+
+int learned_near_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint) {
+    LearnedState *s = c->learned;
+    const int M = c->M;
+    if (!adjoint) {
+        k_near_apply<<<dim3(c->N, cols), ((M + 63) / 64) * 64, sizeof(cd) * M, c->stream>>>(
+            c->N, M, c->H, c->lmax, c->n, c->nh, s->row_start, s->pj, s->blocks.as<cd>(), s->alpha.as<cd>(),
+            c->b_mie.as<cd>(), x, y);
+        TG_HIP(hipGetLastError());
+        return TG_OK;
+    }
+    TG_TRY(s->work.ensure(sizeof(cd) * (size_t)cols * c->n));
+    cd *z = s->work.as<cd>();
+    TG_TRY(local_apply(c, x, z, cols, 1, nullptr));
+    k_near_apply_adjoint<<<dim3(c->N, cols), ((M + 63) / 64) * 64, 0, c->stream>>>(
+        c->N, M, c->H, c->lmax, c->n, c->nh, s->col_start, s->col_list, s->pi, s->blocks.as<cd>(), s->alpha.as<cd>(),
+        z, x, y);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+int learned_setup(tg_cluster *c, Preconditioner *) {
+    LearnedState *s = state(c);
+    if (c->mlp) {
+        TG_TRY(learned_pairs(c, s->K > 0 ? s->K : 8));
+        TG_TRY(mlp_forward(c->mlp, s->feats.as<float>(), s->npairs, s->out16.as<float>()));
+        TG_TRY(learned_set_output(c, s->out16.as<float>()));
+        return TG_OK;
+    }
+    if (s->npairs > 0 && s->have_alpha) return TG_OK;
+    set_error("learned preconditioner: attach a network (tg_cluster_set_mlp) or set the pair weights");
+    return TG_INVALID_INPUT;
+}
+
+int learned_apply(tg_cluster *c, Preconditioner *, const cd *x, cd *y, int cols, int adjoint) {
+    return learned_near_apply(c, x, y, cols, adjoint);
+}
+
+void learned_destroy(Preconditioner *) {}
+
+void learned_release(tg_cluster *c) {
+    if (!c->learned) return;
+    LearnedState *s = c->learned;
+    for (DevBuf *b : {&s->meta, &s->blocks, &s->feats, &s->out16, &s->alpha, &s->work}) b->release();
+    delete s;
+    c->learned = nullptr;
+}
+
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" {
+
+int tg_cluster_near_pairs(tg_cluster *c, int k_nearest, int64_t *npairs) {
+    if (!c || k_nearest < 1) {
+        set_error("need a cluster and k_nearest >= 1");
+        return TG_INVALID_INPUT;
+    }
+    if (c->N < 2) {
+        set_error("a single sphere has no near pairs");
+        return TG_INVALID_INPUT;
+    }
+    TG_TRY(learned_pairs(c, k_nearest));
+    if (npairs) *npairs = c->learned->npairs;
+    return TG_OK;
+}
+
+int tg_cluster_pair_list(tg_cluster *c, int32_t *pairs_host) {
+    if (!c || !c->learned || c->learned->npairs == 0) {
+        set_error("near pairs not built");
+        return TG_INVALID_INPUT;
+    }
+    const int64_t np = c->learned->npairs;
+    std::vector<int> a(np), b(np);
+    TG_HIP(hipMemcpy(a.data(), c->learned->pi, sizeof(int) * np, hipMemcpyDeviceToHost));
+    TG_HIP(hipMemcpy(b.data(), c->learned->pj, sizeof(int) * np, hipMemcpyDeviceToHost));
+    for (int64_t p = 0; p < np; ++p) {
+        pairs_host[2 * p] = a[p];
+        pairs_host[2 * p + 1] = b[p];
+    }
+    return TG_OK;
+}
+
+int tg_cluster_pair_features(tg_cluster *c, float *features_dev) {
+    if (!c || !c->learned || c->learned->npairs == 0) {
+        set_error("near pairs not built");
+        return TG_INVALID_INPUT;
+    }
+    TG_HIP(hipMemcpyAsync(features_dev, c->learned->feats.ptr, sizeof(float) * c->learned->npairs * kFeat,
+                          hipMemcpyDeviceToDevice, c->stream));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_cluster_set_pair_output(tg_cluster *c, const float *out_dev) {
+    if (!c) {
+        set_error("null cluster");
+        return TG_INVALID_INPUT;
+    }
+    TG_TRY(learned_set_output(c, out_dev));
+    if (c->pc_kind == TG_PC_LEARNED) destroy_preconditioner(c);
+    TG_HIP(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_cluster_set_mlp(tg_cluster *c, tg_mlp *m) {
+    if (!c) {
+        set_error("null cluster");
+        return TG_INVALID_INPUT;
+    }
+    c->mlp = m;
+    if (c->pc_kind == TG_PC_LEARNED) destroy_preconditioner(c);
+    return TG_OK;
+}
+
+int tg_cluster_learned_apply(tg_cluster *c, const tg_c128 *x, tg_c128 *y, int cols, int adjoint) {
+    if (!c || !c->learned || !c->learned->have_alpha || cols < 1) {
+        set_error("pair weights not set");
+        return TG_INVALID_INPUT;
+    }
+    const int64_t count = (int64_t)cols * c->n;
+    DevBuf io;
+    TG_TRY(io.ensure(sizeof(cd) * count * 2));
+    cd *xs = io.as<cd>(), *ys = xs + count;
+    int st = ref_to_split(c, (const cd *)x, xs, cols);
+    if (!st) st = learned_near_apply(c, xs, ys, cols, adjoint);
+    if (!st) st = split_to_ref(c, ys, (cd *)y, cols);
+    if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = TG_DEVICE_ERROR;
+    io.release();
+    return st;
+}
+
+int tg_cluster_learned_loss(tg_cluster *c, const tg_c128 *probes, int cols, double *loss, float *dout_dev) {
+    if (!c || !c->learned || !c->learned->have_alpha || cols < 1 || !loss) {
+        set_error("pair weights not set");
+        return TG_INVALID_INPUT;
+    }
+    LearnedState *s = c->learned;
+    const int64_t count = (int64_t)cols * c->n;
+    const int M = c->M;
+    DevBuf io;
+    TG_TRY(io.ensure(sizeof(cd) * (count * 4 + (size_t)s->npairs * cols * M)));
+    cd *V = io.as<cd>(), *U = V + count, *R = U + count, *Z = R + count, *P = Z + count;
+    int st = TG_OK;
+    do {
+        if ((st = ref_to_split(c, (const cd *)probes, V, cols))) break;
+        if ((st = learned_near_apply(c, V, U, cols, 0))) break;          // U = M^{-1} V
+        if ((st = apply_operator(c, U, R, cols, 0, TG_OP_MATRIX_FREE_F64))) break;
+        if ((st = vec_axpby(c->stream, cd(-1.0), V, cd(1.0), R, count))) break;  // R = A U - V
+        std::vector<double> vn(cols), rn(cols);
+        if ((st = vec_norms(c->stream, V, c->n, cols, c->n, vn.data(), c->ws[4]))) break;
+        if ((st = vec_norms(c->stream, R, c->n, cols, c->n, rn.data(), c->ws[4]))) break;
+        double v2 = 0, r2 = 0;
+        for (int k = 0; k < cols; ++k) {
+            v2 += vn[k] * vn[k];
+            r2 += rn[k] * rn[k];
+        }
+        *loss = r2 / v2;
+        if (!dout_dev) break;
+        if ((st = apply_operator(c, R, U, cols, 1, TG_OP_MATRIX_FREE_F64))) break;  // A^H R
+        if ((st = local_apply(c, U, Z, cols, 1, nullptr))) break;                     // Z = T^H A^H R
+        k_pair_products<<<(unsigned)s->npairs, ((M + 63) / 64) * 64, 0, c->stream>>>(
+            s->npairs, M, c->H, c->n, c->nh, cols, s->pj, s->blocks.as<cd>(), V, P);
+        if (hipGetLastError() != hipSuccess) { st = TG_DEVICE_ERROR; break; }
+        k_alpha_grad<<<(unsigned)((s->npairs + 127) / 128), 128, 0, c->stream>>>(
+            s->npairs, M, c->H, c->lmax, c->n, c->nh, cols, s->pi, Z, P, 1.0 / v2, dout_dev);
+        if (hipGetLastError() != hipSuccess) { st = TG_DEVICE_ERROR; break; }
+    } while (0);
+    if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = TG_DEVICE_ERROR;
+    if (st == TG_DEVICE_ERROR && tg_last_error()[0] == 0) set_error("learned loss kernels failed");
+    io.release();
+    return st;
+}
+
+}  // extern "C"

@@ -9,6 +9,8 @@ namespace tg {
 int ensure_preconditioner(tg_cluster *c, int kind, int storage);
 int apply_preconditioner(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint);
 void destroy_preconditioner(tg_cluster *c);
+// free the near pairs, blocks and weights of the learned preconditioner
+void learned_release(tg_cluster *c);
 
 // Spatial groups of at most `size` spheres by recursive coordinate bisection.
 std::vector<std::vector<int>> spatial_groups(const std::vector<double> &positions, int N, int size);

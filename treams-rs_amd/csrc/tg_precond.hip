@@ -286,12 +286,14 @@ int ensure_preconditioner(tg_cluster *c, int kind, int storage) {
     (void)storage;
     if (c->pc && c->pc_kind == kind) return TG_OK;
     destroy_preconditioner(c);
+    if (kind != TG_PC_BLOCK_JACOBI && kind != TG_PC_LEARNED) {
+        set_error("unknown preconditioner %d", kind);
+        return TG_INVALID_INPUT;
+    }
     Preconditioner *pc = new Preconditioner();
     pc->kind = kind;
-    const char *env = getenv("TG_GROUP_SIZE");
-    if (env && atoi(env) > 0) pc->group_size = atoi(env);
-    int st = build_block_jacobi(c, pc);
-    if (!st && kind == TG_PC_LEARNED) st = learned_setup(c, pc);
+    pc->group_size = c->group_size;
+    int st = kind == TG_PC_BLOCK_JACOBI ? build_block_jacobi(c, pc) : learned_setup(c, pc);
     if (st) {
         pc->mats.release();
         pc->meta.release();
@@ -312,10 +314,14 @@ int apply_preconditioner(tg_cluster *c, const cd *x, cd *y, int cols, int adjoin
     }
     Timer::Rec rec;
     TG_TRY(c->timer.begin("pc_apply", c->stream, rec));
-    k_pc_apply<<<dim3(pc->ngroups, cols), 256, sizeof(cd) * pc->max_m, c->stream>>>(
-        pc->mats.as<cd>(), pc->d_off, pc->d_size, pc->d_moff, pc->d_members, c->M, c->H, c->n, c->nh, x, y, adjoint);
-    TG_HIP(hipGetLastError());
-    if (pc->kind == TG_PC_LEARNED) TG_TRY(learned_apply(c, pc, x, y, cols, adjoint));
+    if (pc->kind == TG_PC_LEARNED) {
+        TG_TRY(learned_apply(c, pc, x, y, cols, adjoint));
+    } else {
+        k_pc_apply<<<dim3(pc->ngroups, cols), 256, sizeof(cd) * pc->max_m, c->stream>>>(
+            pc->mats.as<cd>(), pc->d_off, pc->d_size, pc->d_moff, pc->d_members, c->M, c->H, c->n, c->nh, x, y,
+            adjoint);
+        TG_HIP(hipGetLastError());
+    }
     TG_TRY(c->timer.end("pc_apply", c->stream, rec));
     return TG_OK;
 }
@@ -329,15 +335,5 @@ void destroy_preconditioner(tg_cluster *c) {
     c->pc = nullptr;
     c->pc_kind = TG_PC_NONE;
 }
-
-// Learned part: filled in by tg_learned.hip when present.
-__attribute__((weak)) int learned_setup(tg_cluster *, Preconditioner *) {
-    set_error("learned preconditioner weights are not loaded");
-    return TG_INVALID_INPUT;
-}
-__attribute__((weak)) int learned_apply(tg_cluster *, Preconditioner *, const cd *, cd *, int, int) {
-    return TG_INVALID_INPUT;
-}
-__attribute__((weak)) void learned_destroy(Preconditioner *) {}
 
 }  // namespace tg

@@ -1,0 +1,186 @@
+"""Learned near-field preconditioner: network, training on GPU, weights on disk.
+
+The preconditioner (libtreams_gpu.so, tg_learned.hip) is
+    M^{-1} x = x + T sum_{(i,j) in k-nearest pairs} D(alpha_ij) C_ij x_j,
+alpha_ij[l] = 1 + out[2l] + i out[2l+1], out = MLP(features_ij).  The network is the
+library's MFMA MLP (BF16 operands, FP32 accumulation); training minimises
+|A M^{-1} V - V|^2 / |V|^2 over random probe blocks V with gradients from
+tg_cluster_learned_loss and tg_mlp_backward, and Adam on the device.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import time
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+from .cluster import IterativeSphereCluster, _columns, _device, _stream_ptr
+from .geometry import random_cluster
+
+FEATURES = 64
+OUTPUTS = 16
+WIDTH = 256
+HIDDEN_LAYERS = 3
+K_NEAREST = 8
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "pc_mlp.npz")
+
+
+class MLP:
+    """FEATURES -> WIDTH (x HIDDEN_LAYERS, ReLU) -> OUTPUTS on the GPU."""
+
+    def __init__(self, n_in=FEATURES, width=WIDTH, n_out=OUTPUTS, hidden_layers=HIDDEN_LAYERS, seed=0):
+        _device()
+        h = ctypes.c_void_p()
+        check(lib().tg_mlp_create(n_in, width, n_out, hidden_layers, seed, ctypes.c_void_p(_stream_ptr()),
+                                  ctypes.byref(h)))
+        self._h = h
+        self.dims = (n_in, width, n_out, hidden_layers)
+        self.count = int(lib().tg_mlp_param_count(h))
+        self.step = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().tg_mlp_destroy(h)
+            self._h = None
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.count, np.float32)
+        check(lib().tg_mlp_get_params(self._h, out.ctypes.data))
+        return out
+
+    def set_params(self, values) -> None:
+        v = np.ascontiguousarray(values, np.float32)
+        assert v.size == self.count
+        check(lib().tg_mlp_set_params(self._h, v.ctypes.data))
+
+    def grads(self) -> np.ndarray:
+        out = np.zeros(self.count, np.float32)
+        check(lib().tg_mlp_get_grads(self._h, out.ctypes.data))
+        return out
+
+    def zero_grad(self):
+        check(lib().tg_mlp_zero_grad(self._h))
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        x = x.to(_device(), torch.float32).contiguous()
+        y = torch.empty((x.shape[0], self.dims[2]), dtype=torch.float32, device=x.device)
+        check(lib().tg_mlp_forward(self._h, x.data_ptr(), x.shape[0], y.data_ptr()))
+        return y
+
+    def backward(self, dy: torch.Tensor) -> None:
+        dy = dy.to(_device(), torch.float32).contiguous()
+        check(lib().tg_mlp_backward(self._h, dy.data_ptr()))
+
+    def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.step += 1
+        check(lib().tg_mlp_adam_step(self._h, lr, beta1, beta2, eps, self.step))
+
+    def save(self, path=DEFAULT_WEIGHTS, **meta):
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        np.savez(path, params=self.params(), dims=np.array(self.dims), **meta)
+
+    @classmethod
+    def load(cls, path=DEFAULT_WEIGHTS) -> "MLP":
+        with np.load(path, allow_pickle=False) as z:
+            dims = tuple(int(v) for v in z["dims"])
+            m = cls(*dims)
+            m.set_params(z["params"])
+        return m
+
+
+def near_pairs(cluster: IterativeSphereCluster, k_nearest: int = K_NEAREST) -> int:
+    npairs = ctypes.c_int64()
+    check(lib().tg_cluster_near_pairs(cluster._h, k_nearest, ctypes.byref(npairs)))
+    return npairs.value
+
+
+def pair_list(cluster: IterativeSphereCluster, npairs: int) -> np.ndarray:
+    out = np.zeros(2 * npairs, np.int32)
+    check(lib().tg_cluster_pair_list(cluster._h, out.ctypes.data))
+    return out.reshape(npairs, 2)
+
+
+def pair_features(cluster: IterativeSphereCluster, npairs: int) -> torch.Tensor:
+    f = torch.empty((npairs, FEATURES), dtype=torch.float32, device=_device())
+    check(lib().tg_cluster_pair_features(cluster._h, f.data_ptr()))
+    return f
+
+
+def set_pair_output(cluster: IterativeSphereCluster, out: torch.Tensor) -> None:
+    out = out.to(_device(), torch.float32).contiguous()
+    check(lib().tg_cluster_set_pair_output(cluster._h, out.data_ptr()))
+
+
+def learned_apply(cluster: IterativeSphereCluster, x, adjoint=False):
+    cm, vector = _columns(x, cluster.n)
+    y = torch.empty_like(cm)
+    check(lib().tg_cluster_learned_apply(cluster._h, cm.data_ptr(), y.data_ptr(), cm.shape[0], int(adjoint)))
+    return y[0] if vector else y.T.contiguous()
+
+
+def learned_loss(cluster: IterativeSphereCluster, probes, npairs: int, grad: bool = True):
+    cm, _ = _columns(probes, cluster.n)
+    loss = ctypes.c_double()
+    dout = torch.empty((npairs, OUTPUTS), dtype=torch.float32, device=_device()) if grad else None
+    check(lib().tg_cluster_learned_loss(cluster._h, cm.data_ptr(), cm.shape[0], ctypes.byref(loss),
+                                        dout.data_ptr() if grad else None))
+    return loss.value, dout
+
+
+def attach(cluster: IterativeSphereCluster, mlp: MLP | None, k_nearest: int = K_NEAREST) -> None:
+    """Use `mlp` for TG_PC_LEARNED solves of `cluster`."""
+    if mlp is not None:
+        near_pairs(cluster, k_nearest)
+    check(lib().tg_cluster_set_mlp(cluster._h, mlp._h if mlp is not None else None))
+    cluster._mlp = mlp  # keep the network alive as long as the cluster uses it
+
+
+def training_cluster(seed: int, rng: np.random.Generator | None = None):
+    """One training cluster: N in {64..256}, lmax in {3..6}, k0 U[0.8, 2.0], radii
+    U[0.15, 0.35], eps U[2,6] + i U[0,0.2], volume fill fraction U[0.05, 0.25]."""
+    rng = rng or np.random.default_rng(seed)
+    N = int(rng.integers(64, 257))
+    lmax = int(rng.integers(3, 7))
+    k0 = float(rng.uniform(0.8, 2.0))
+    fill = float(rng.uniform(0.05, 0.25))
+    mean_volume = 4.0 / 3.0 * np.pi * np.mean(np.array([0.15, 0.35]) ** 3)
+    side = (N * mean_volume / fill) ** (1.0 / 3.0)
+    radii, eps, pos = random_cluster(N, side, 0.05, (0.15, 0.35), None, seed=seed)
+    return lmax, k0, radii, eps, pos
+
+
+def train(n_clusters=48, epochs=6, probes=4, lr=2e-3, seed=1000, log=print, mlp: MLP | None = None) -> MLP:
+    """Train on clusters the method generates (seeds seed..seed+n_clusters-1)."""
+    mlp = mlp or MLP(seed=seed)
+    gen = torch.Generator(device=_device())
+    gen.manual_seed(seed)
+    clusters = [training_cluster(seed + c) for c in range(n_clusters)]
+    for epoch in range(epochs):
+        t0 = time.perf_counter()
+        losses, base = [], []
+        for (lmax, k0, radii, eps, pos) in clusters:
+            cl = IterativeSphereCluster(lmax, k0, radii, eps, pos)
+            npairs = near_pairs(cl)
+            feats = pair_features(cl, npairs)
+            V = torch.randn((cl.n, probes), dtype=torch.complex128, device=_device(), generator=gen)
+            out = mlp.forward(feats)
+            set_pair_output(cl, out)
+            loss, dout = learned_loss(cl, V, npairs)
+            mlp.zero_grad()
+            mlp.backward(dout)
+            mlp.adam(lr=lr)
+            losses.append(loss)
+            if epoch == 0:
+                set_pair_output(cl, torch.zeros_like(out))
+                base.append(learned_loss(cl, V, npairs, grad=False)[0])
+            del cl
+        msg = f"epoch {epoch + 1}/{epochs}: mean loss {np.mean(losses):.4e} ({time.perf_counter() - t0:.1f} s)"
+        if base:
+            msg += f"; plain Neumann (alpha = 1) {np.mean(base):.4e}"
+        log(msg)
+    return mlp
