@@ -108,7 +108,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="n1024_l6")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--pc", default="block_jacobi")
+    ap.add_argument("--pc", default="learned")
+    ap.add_argument("--weights", default=None)
+    ap.add_argument("--bj-group", type=int, default=4)
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--restart", type=int, default=50)
     ap.add_argument("--no-plain", action="store_true")
@@ -132,6 +134,13 @@ def main() -> int:
     N = radii.size
     log(f"workload {args.workload}: N={N} lmax={lmax} n={N * 2 * lmax * (lmax + 2)}")
     cl = pkg.IterativeSphereCluster(lmax, k0, radii, eps, pos)
+    cl.set_group_size(args.bj_group)
+    if args.pc == "learned":
+        import importlib
+
+        learned = importlib.import_module("treams_rs_amd.learned")
+        mlp = learned.MLP.load(args.weights or learned.DEFAULT_WEIGHTS)
+        learned.attach(cl, mlp)
     n, H = cl.n, lmax * (lmax + 2)
     nh = N * H
     B = rhs_columns(cl, torch)

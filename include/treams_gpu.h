@@ -144,8 +144,10 @@ int tg_translation_blocks(int lmax, double k, int64_t count, const double *displ
 int tg_cluster_assemble(tg_cluster *cluster, int storage);
 
 /*
- * Free the stored coupling and the preconditioner; the next call that needs them
- * rebuilds them.  No reference counterpart (device memory management).
+ * Free the stored coupling, the FP64 pair tables of the matrix-free operator and the
+ * preconditioners (including the learned one's near pairs; an attached network
+ * stays attached); the next call that needs them rebuilds them.  No reference
+ * counterpart (device memory management).
  */
 int tg_cluster_release(tg_cluster *cluster);
 

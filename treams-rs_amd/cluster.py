@@ -120,6 +120,10 @@ class IterativeSphereCluster:
         s = {"c64": _lib.TG_STORE_C64, "c128": _lib.TG_STORE_C128}[storage]
         check(lib().tg_cluster_assemble(self._h, s))
 
+    def set_group_size(self, spheres: int) -> None:
+        """Spheres per block-Jacobi group."""
+        check(lib().tg_cluster_set_group_size(self._h, int(spheres)))
+
     def release(self) -> None:
         """Free the stored coupling and the preconditioner (rebuilt on demand)."""
         check(lib().tg_cluster_release(self._h))

@@ -365,6 +365,7 @@ int tg_cluster_release(tg_cluster *c) {
     c->b_tab.release();
     c->have_tables = false;
     c->storage = TG_STORE_NONE;
+    learned_release(c);
     for (auto &w : c->ws) w.release();
     return TG_OK;
 }
