@@ -231,7 +231,8 @@ int tg_cluster_set_group_size(tg_cluster *cluster, int spheres);
  * degree l by alpha_ij[l].  tg_cluster_near_pairs builds the pair list, the pair
  * blocks and the 64 FP32 features per pair (k d, 1/(k d), direction cosines, Mie
  * blocks of both spheres for l <= 6).  Pair weights come from 16 FP32 outputs per
- * pair, alpha[l] = 1 + out[2l] + i out[2l+1].
+ * pair, alpha[l] = 1 + 0.5 tanh(out[2l]) + 0.5 i tanh(out[2l+1]) (out = 0: the
+ * first-order Neumann preconditioner of the near field).
  */
 int tg_cluster_near_pairs(tg_cluster *cluster, int k_nearest, int64_t *npairs);
 /* (i, j) of every pair, host int32[2 * npairs]. */

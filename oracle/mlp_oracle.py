@@ -70,7 +70,8 @@ def learned_inverse(cluster, pairs, out16):
     S = np.zeros((n, n), np.complex128)
     for p, (i, j) in enumerate(pairs):
         C = oracle.translation_block(lmax, cluster.k0, cluster.positions[i] - cluster.positions[j], False)
-        alpha = 1.0 + out16[p, 0:2 * lmax:2].astype(np.float64) + 1j * out16[p, 1:2 * lmax:2].astype(np.float64)
+        alpha = (1.0 + 0.5 * np.tanh(out16[p, 0:2 * lmax:2].astype(np.float64))
+                 + 0.5j * np.tanh(out16[p, 1:2 * lmax:2].astype(np.float64)))
         S[i * M:(i + 1) * M, j * M:(j + 1) * M] += alpha[l_of_mode - 1][:, None] * C
     return np.eye(n) + cluster.T @ S
 

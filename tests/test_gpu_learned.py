@@ -118,21 +118,20 @@ def test_learned_preconditioned_solve_same_answer(pkg, oracle, learned):
     assert c2[0][1] <= 1e-9 * c2[0][2]
 
 
-def test_training_reduces_loss(pkg, learned):
+def test_training_reduces_error_radius(pkg, learned):
     """A short GPU training run (the library's own forward, backward and Adam
-    kernels) lowers the preconditioner objective of its training clusters below the
-    same preconditioner without the network (alpha = 1)."""
+    kernels) lowers the estimated spectral radius of A M^{-1} - I on its training
+    clusters below that of the same preconditioner without the network (alpha = 1)."""
     log = []
-    mlp = learned.train(n_clusters=4, epochs=12, probes=2, lr=1e-3, seed=2000, log=log.append)
+    mlp = learned.train(n_clusters=4, epochs=10, probes=2, lr=1e-3, seed=2000, log=log.append)
     trained, plain = [], []
     for c in range(4):
         lmax, k0, radii, eps, pos = learned.training_cluster(2000 + c)
         cl = pkg.IterativeSphereCluster(lmax, k0, radii, eps, pos)
         npairs = learned.near_pairs(cl)
-        V = learned.probe_block(cl, 2, 2000 * 7919 + c)
         out = mlp.forward(learned.pair_features(cl, npairs))
         learned.set_pair_output(cl, out)
-        trained.append(learned.learned_loss(cl, V, npairs, grad=False)[0])
+        trained.append(learned.error_radius(cl, seed=c))
         learned.set_pair_output(cl, torch.zeros_like(out))
-        plain.append(learned.learned_loss(cl, V, npairs, grad=False)[0])
-    assert np.mean(trained) < np.mean(plain)
+        plain.append(learned.error_radius(cl, seed=c))
+    assert np.mean(trained) < np.mean(plain), (trained, plain, log)

@@ -68,13 +68,15 @@ __global__ void k_pair_features(int64_t np, const int *pi, const int *pj, const 
     }
 }
 
+// alpha[l] = 1 + 0.5 tanh(out[2l]) + 0.5 i tanh(out[2l+1]): bounded around the
+// first-order Neumann weights alpha = 1 (out = 0).
 __global__ void k_alpha_from_output(int64_t np, int lmax, const float *out, cd *alpha) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= np * lmax) return;
     const int64_t p = t / lmax;
     const int l = (int)(t - p * lmax);
     const float re = l < kOut / 2 ? out[p * kOut + 2 * l] : 0.f, im = l < kOut / 2 ? out[p * kOut + 2 * l + 1] : 0.f;
-    alpha[t] = cd(1.0 + (double)re, (double)im);
+    alpha[t] = cd(1.0 + 0.5 * tanh((double)re), 0.5 * tanh((double)im));
 }
 
 // y_i = x_i + T_i sum_{pairs p of i} D(alpha_p) C_p x_{j_p}  (split layout; grid N x cols,
@@ -146,9 +148,10 @@ __global__ void k_pair_products(int64_t np, int M, int H, int64_t n, int64_t nh,
     }
 }
 
-// dL/d(out_p) = 2 sum_{rows of degree l} Z_i conj(P_p) / |V|^2 as (re, im) pairs.
+// dL/d(out_p): dL/d(alpha) = 2 sum_{rows of degree l} Z_i conj(P_p) / |V|^2 as (re, im)
+// pairs, times d alpha / d out = 0.5 (1 - tanh^2 out) per component.
 __global__ void k_alpha_grad(int64_t np, int M, int H, int lmax, int64_t n, int64_t nh, int cols, const int *pi,
-                             const cd *Z, const cd *P, double scale, float *dout) {
+                             const cd *Z, const cd *P, double scale, const float *out, float *dout) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const int i = pi[p];
@@ -161,8 +164,9 @@ __global__ void k_alpha_grad(int64_t np, int M, int H, int lmax, int64_t n, int6
                 for (int cc = 0; cc < cols; ++cc)
                     g += Z[(int64_t)cc * n + (int64_t)h * nh + (int64_t)i * H + b] * conj(P[(p * cols + cc) * M + r]);
             }
-        dout[p * kOut + 2 * (l - 1)] = (float)(2.0 * scale * g.re);
-        dout[p * kOut + 2 * (l - 1) + 1] = (float)(2.0 * scale * g.im);
+        const double tr = tanh((double)out[p * kOut + 2 * (l - 1)]), ti = tanh((double)out[p * kOut + 2 * (l - 1) + 1]);
+        dout[p * kOut + 2 * (l - 1)] = (float)(2.0 * scale * g.re * 0.5 * (1.0 - tr * tr));
+        dout[p * kOut + 2 * (l - 1) + 1] = (float)(2.0 * scale * g.im * 0.5 * (1.0 - ti * ti));
     }
 }
 
@@ -432,7 +436,7 @@ int tg_cluster_learned_loss(tg_cluster *c, const tg_c128 *probes, int cols, doub
             s->npairs, M, c->H, c->n, c->nh, cols, s->pj, s->blocks.as<cd>(), V, P);
         if (hipGetLastError() != hipSuccess) { st = TG_DEVICE_ERROR; break; }
         k_alpha_grad<<<(unsigned)((s->npairs + 127) / 128), 128, 0, c->stream>>>(
-            s->npairs, M, c->H, c->lmax, c->n, c->nh, cols, s->pi, Z, P, 1.0 / v2, dout_dev);
+            s->npairs, M, c->H, c->lmax, c->n, c->nh, cols, s->pi, Z, P, 1.0 / v2, s->out16.as<float>(), dout_dev);
         if (hipGetLastError() != hipSuccess) { st = TG_DEVICE_ERROR; break; }
     } while (0);
     if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = TG_DEVICE_ERROR;

@@ -165,6 +165,24 @@ def probe_block(cluster: IterativeSphereCluster, probes: int, seed: int) -> torc
     return V / torch.linalg.norm(V, dim=0, keepdim=True)
 
 
+def dominant_probes(cluster: IterativeSphereCluster, probes: int, seed: int, steps: int = 4) -> torch.Tensor:
+    """Power iteration of E = A M^{-1} - I from T-range probes: columns close to the
+    dominant error modes of the current preconditioner, which limit GMRES."""
+    V = probe_block(cluster, probes, seed)
+    for _ in range(steps):
+        W = cluster.apply(learned_apply(cluster, V)) - V
+        V = W / torch.linalg.norm(W, dim=0, keepdim=True)
+    return V
+
+
+def error_radius(cluster: IterativeSphereCluster, probes: int = 4, seed: int = 0, steps: int = 12) -> float:
+    """Power-iteration estimate of the spectral radius of A M^{-1} - I for the weights
+    currently set on `cluster`."""
+    V = dominant_probes(cluster, probes, seed, steps)
+    W = cluster.apply(learned_apply(cluster, V)) - V
+    return float(torch.linalg.norm(W, dim=0).max())
+
+
 def train(n_clusters=48, epochs=6, probes=4, lr=1e-3, seed=1000, log=print, mlp: MLP | None = None) -> MLP:
     """Train on clusters the method generates (seeds seed..seed+n_clusters-1), each
     with a fixed block of Gaussian probe columns."""
@@ -177,9 +195,9 @@ def train(n_clusters=48, epochs=6, probes=4, lr=1e-3, seed=1000, log=print, mlp:
             cl = IterativeSphereCluster(lmax, k0, radii, eps, pos)
             npairs = near_pairs(cl)
             feats = pair_features(cl, npairs)
-            V = probe_block(cl, probes, seed * 7919 + c)
             out = mlp.forward(feats)
             set_pair_output(cl, out)
+            V = dominant_probes(cl, probes, seed * 7919 + c)
             loss, dout = learned_loss(cl, V, npairs)
             mlp.zero_grad()
             mlp.backward(dout)
