@@ -217,6 +217,46 @@ int tg_cluster_overlap_pullback(tg_cluster *cluster, const tg_c128 *u_dev, const
                                 double *grad_positions);
 
 /* ---------------------------------------------------------------------------
+ * Dense LU path.
+ * ------------------------------------------------------------------------- */
+
+typedef struct tg_factor tg_factor;
+
+/*
+ * Assemble A = I - T C and factor it on the device: replaces sphere_cluster_factor
+ * (cluster/spheres.rs:85-94) with InteractionFactor::from_blocks / operator / Lu::new
+ * (cluster/interaction.rs:155-205, linalg/mod.rs:398-428).  precision is TG_STORE_C64
+ * (complex FP32 factors on MFMA, every solve refined in FP64) or TG_STORE_C128;
+ * the factors take n^2 x 8 or 16 bytes.  Pivoting is partial within 64-row diagonal
+ * blocks (the reference pivots over whole columns); the FP64 residual certifies every
+ * solve.  TG_SINGULAR for a zero pivot.  The cluster must outlive the factor.
+ */
+int tg_factor_create(tg_cluster *cluster, int precision, tg_factor **out);
+int tg_factor_destroy(tg_factor *factor);
+
+/*
+ * adjoint == 0: X = (I - T C)^{-1} T A for `cols` incident columns (device c128,
+ * n x cols, reference layout): InteractionFactor::solve (cluster/interaction.rs:213-218).
+ * adjoint != 0: X = (I - T C)^{-H} A (Lu::solve_adjoint_in_place, linalg/mod.rs:436-438;
+ * InteractionFactor::pullback_incident is T^H of it, cluster/interaction.rs:220-226).
+ * Refined with the FP64 matrix-free residual until |b - A x| <= rtol |b|;
+ * TG_NOT_CONVERGED otherwise.  conv_host (cols entries) gets the residuals and the
+ * refinement count.
+ */
+int tg_factor_solve(tg_factor *factor, const tg_c128 *incident_dev, int cols, double rtol, int adjoint,
+                    tg_c128 *x_dev, tg_convergence *conv_host);
+
+/*
+ * tg_cluster_pullback with the adjoint solve by the LU factors: replaces
+ * SphereClusterResidual::pullback (cluster/spheres.rs:321-378) through
+ * InteractionResidual::pullback_blocks (cluster/interaction.rs:379-440).
+ */
+int tg_factor_pullback(tg_factor *factor, const tg_c128 *incident_dev, const tg_c128 *x_dev,
+                       const tg_c128 *cotangent_dev, int cols, double rtol, double *grad_k0, double *grad_radii,
+                       tg_c128 *grad_eps, double *grad_positions, tg_c128 *incident_grad_dev,
+                       tg_convergence *conv_host);
+
+/* ---------------------------------------------------------------------------
  * Preconditioners (no reference counterpart: the reference GMRES is
  * unpreconditioned).  Right preconditioning: iteration counts change, the answer
  * and its FP64 true-residual certificate do not.

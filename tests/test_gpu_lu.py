@@ -1,0 +1,178 @@
+"""GPU parity of the dense LU path (assembled I - T C, blocked LU with MFMA trailing
+updates, FP64 refinement) against the FP64 oracle (LAPACK solves of the oracle's
+dense operator) and the reference goldens.
+
+Tolerances: solutions 1e-10 relative to the oracle for both precisions (the c64
+factors are refined with the FP64 residual to rtol 1e-12); certificates are the
+FP64 residual itself; gradients 1e-9 relative like the GMRES pullback, and central
+finite differences (step 1e-6) to 1e-5 relative.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from __graft_entry__ import load_package
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    p = load_package()
+    assert torch.cuda.is_available()
+    return p
+
+
+def _rel(a, b):
+    a = a.detach().cpu().numpy() if torch.is_tensor(a) else np.asarray(a)
+    b = b.detach().cpu().numpy() if torch.is_tensor(b) else np.asarray(b)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _cluster(pkg, oracle, N=8, lmax=3, seed=1, side=2.0, eps=4 + 0.1j, k0=1.3):
+    radii, e, pos = pkg.random_cluster(N, side, 0.1, (0.2, 0.3), eps, seed=seed)
+    return pkg.IterativeSphereCluster(lmax, k0, radii, e, pos), oracle.Cluster(lmax, k0, radii, e, pos)
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_factor_solve_small_cluster(pkg, oracle, precision):
+    """8 spheres, lmax 3 (240 unknowns: three full 64-blocks and a ragged one)."""
+    cl, ref = _cluster(pkg, oracle, N=8, lmax=3, seed=5)
+    f = pkg.SphereClusterFactor(cl, precision)
+    a = torch.stack([cl.plane_wave([0, 0, 1], 1), cl.plane_wave([0, 0, -1], 0)], dim=1)
+    x, conv = f.solve(a)
+    an = a.cpu().numpy()
+    for c in range(2):
+        assert _rel(x[:, c], ref.solve(an[:, c])) <= 1e-10
+        assert conv[c].residual_norm <= 1e-12 * conv[c].rhs_norm
+        r = np.linalg.norm(ref.operator() @ x[:, c].cpu().numpy() - oracle.apply_local(ref.blocks, an[:, c]))
+        assert abs(r - conv[c].residual_norm) <= 1e-3 * conv[c].residual_norm + 1e-14 * conv[c].rhs_norm
+    if precision == "c128":
+        assert max(c.refinements for c in conv) <= 1
+    else:
+        assert max(c.refinements for c in conv) >= 1
+    f.close()
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_factor_adjoint_solve(pkg, oracle, precision):
+    cl, ref = _cluster(pkg, oracle, N=10, lmax=3, seed=7, side=2.6)
+    f = pkg.SphereClusterFactor(cl, precision)
+    rng = np.random.default_rng(3)
+    g = rng.normal(size=(ref.n, 3)) + 1j * rng.normal(size=(ref.n, 3))
+    y, conv = f.solve_adjoint(g)
+    yr = np.linalg.solve(ref.operator().conj().T, g)
+    assert _rel(y, yr) <= 1e-10
+    # pullback_incident = T^H A^{-H} G (cluster/interaction.rs:220-226)
+    assert _rel(f.pullback_incident(g), oracle.apply_local(ref.blocks, yr, True)) <= 1e-10
+
+
+def test_interacting_tmatrix_many_columns(pkg, oracle):
+    """All n columns (the interacting T-matrix X = (I - T C)^{-1} T of sphere_cluster,
+    cluster/spheres.rs:56-80), solved in chunks of 8 columns."""
+    cl, ref = _cluster(pkg, oracle, N=4, lmax=2, seed=2, side=1.6)
+    f = pkg.SphereClusterFactor(cl, "c128")
+    eye = np.eye(ref.n, dtype=np.complex128)
+    X, conv = f.solve(eye)
+    Xr = np.linalg.solve(ref.operator(), ref.T)
+    assert _rel(X, Xr) <= 1e-11
+    assert len(conv) == ref.n
+
+
+def test_factor_larger_cluster_c64(pkg, oracle):
+    """N = 40, lmax = 4 (1,920 unknowns, 30 blocks): c64 factors, FP64 refinement,
+    several trailing updates and 4 right-hand sides."""
+    cl, ref = _cluster(pkg, oracle, N=40, lmax=4, seed=21, side=5.0)
+    f = pkg.SphereClusterFactor(cl, "c64")
+    a = torch.stack([cl.plane_wave(d, p) for d in ([0, 0, 1], [0, 0, -1]) for p in (1, 0)], dim=1)
+    x, conv = f.solve(a)
+    xr = np.linalg.solve(ref.operator(), oracle.apply_local(ref.blocks, a.cpu().numpy()))
+    assert _rel(x, xr) <= 1e-10
+    for c in conv:
+        assert c.residual_norm <= 1e-12 * c.rhs_norm
+        assert 1 <= c.refinements <= 6
+    # the GMRES path gives the same answer
+    xg, _ = cl.solve(a, rtol=1e-12)
+    assert _rel(x, xg) <= 1e-10
+
+
+def test_factor_cluster_example_golden(pkg, golden_dir):
+    g = json.load(open(os.path.join(golden_dir, "reference_examples.json")))["cluster_example"]
+    pos = (220 / np.sqrt(24)) * np.array(
+        [[np.sqrt(8), 0, -1], [-np.sqrt(2), np.sqrt(6), -1], [-np.sqrt(2), -np.sqrt(6), -1], [0, 0, 3]])
+    k0 = 2 * np.pi / g["k0_over_2pi_inverse"]
+    f = pkg.sphere_cluster_factor(g["lmax"], k0, g["radii"], complex(*g["epsilon"]), pos)
+    a = f.cluster.plane_wave(g["direction"], g["pol"])
+    x, _ = f.solve(a)
+    sca, ext = f.cluster.cross_sections(a, x)
+    e = g["expected"]
+    assert float(f"{sca:.6g}") == e["scattering"]
+    assert float(f"{ext:.6g}") == e["extinction"]
+    assert float(f"{ext - sca:.6g}") == e["absorption"]
+
+
+def test_factor_pullback_matches_oracle_and_gmres(pkg, oracle):
+    cl, ref = _cluster(pkg, oracle, N=6, lmax=3, seed=13, side=2.5, eps=None)
+    f = pkg.SphereClusterFactor(cl, "c128")
+    rng = np.random.default_rng(2)
+    a = np.stack([ref.plane_wave([0, 0, 1], 1), ref.plane_wave([1, 0, 0], 0)], axis=1)
+    x, ctx, _ = f.record(a)
+    G = rng.normal(size=a.shape) + 1j * rng.normal(size=a.shape)
+    gk, gr, ge, gp, ginc, conv = ctx.pullback(G)
+    xr = np.stack([ref.solve(a[:, c]) for c in range(2)], axis=1)
+    r = ref.pullback(a, xr, G)
+    assert np.abs(gr - r["radii"]).max() <= 1e-9 * np.abs(r["radii"]).max()
+    assert np.abs(ge - r["eps"]).max() <= 1e-9 * np.abs(r["eps"]).max()
+    assert np.abs(gp - r["positions"]).max() <= 1e-9 * np.abs(r["positions"]).max()
+    assert abs(gk - r["k0"]) <= 1e-9 * abs(r["k0"])
+    assert _rel(ginc, r["incident"]) <= 1e-9
+    # the GMRES pullback of the same record agrees
+    _, ctx_g, _ = cl.record(a, rtol=1e-13)
+    gk2, gr2, ge2, gp2, ginc2, _ = ctx_g.pullback(G)
+    assert np.abs(gp - gp2).max() <= 1e-9 * np.abs(gp).max()
+    assert np.abs(gr - gr2).max() <= 1e-9 * np.abs(gr).max()
+
+
+def test_factor_scattering_gradient_finite_differences(pkg):
+    """d sigma_sca / d (radii, positions) through the LU adjoint against central
+    differences with step 1e-6 (each a fresh factorisation)."""
+    radii, eps, pos = pkg.random_cluster(5, 2.4, 0.1, (0.2, 0.3), None, seed=31)
+    k0, lmax = 1.3, 3
+
+    def sigma(r, p):
+        f = pkg.sphere_cluster_factor(lmax, k0, r, eps, p)
+        a = f.cluster.plane_wave([0, 0, 1], 1)
+        x, _ = f.solve(a)
+        return f.cluster.cross_sections(a, x, flux=0.5)[0]
+
+    f = pkg.sphere_cluster_factor(lmax, k0, radii, eps, pos)
+    sca, gr, gp, _ = pkg.scattering_gradient(f, [0, 0, 1], 1)
+    assert abs(sca - sigma(radii, pos)) <= 1e-12 * sca
+    h = 1e-6
+    for i in (0, 3):
+        rp, rm = radii.copy(), radii.copy()
+        rp[i] += h
+        rm[i] -= h
+        fd = (sigma(rp, pos) - sigma(rm, pos)) / (2 * h)
+        assert abs(fd - gr[i]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(gr).max())
+    for i, ax in ((1, 0), (2, 2), (4, 1)):
+        pp, pm = pos.copy(), pos.copy()
+        pp[i, ax] += h
+        pm[i, ax] -= h
+        fd = (sigma(radii, pp) - sigma(radii, pm)) / (2 * h)
+        assert abs(fd - gp[i, ax]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(gp).max())
+
+
+def test_factor_invalid_input(pkg, oracle):
+    cl, _ = _cluster(pkg, oracle, N=3, lmax=2, seed=1, side=1.5)
+    with pytest.raises(KeyError):
+        pkg.SphereClusterFactor(cl, "c32")
+    f = pkg.SphereClusterFactor(cl, "c128")
+    with pytest.raises(ValueError):
+        f.solve(np.zeros((cl.n + 1, 1), np.complex128))
+    f.close()
+    with pytest.raises(ValueError):
+        f.solve(np.zeros(cl.n, np.complex128))

@@ -10,10 +10,13 @@ from .cluster import (  # noqa: F401
     DEFAULT_RESTART,
     DEFAULT_RTOL,
     Convergence,
+    FactorContext,
     IterativeContext,
     IterativeSphereCluster,
+    SphereClusterFactor,
     plan_terms,
     scattering_gradient,
+    sphere_cluster_factor,
     translation_blocks,
     wigner3j,
 )

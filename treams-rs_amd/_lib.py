@@ -35,7 +35,8 @@ EXPORTED = [
     "tg_cluster_pair_list", "tg_cluster_pair_features", "tg_cluster_set_pair_output", "tg_cluster_learned_apply",
     "tg_cluster_learned_loss", "tg_cluster_set_mlp", "tg_mlp_create", "tg_mlp_destroy", "tg_mlp_param_count",
     "tg_mlp_get_params", "tg_mlp_set_params", "tg_mlp_get_grads", "tg_mlp_zero_grad", "tg_mlp_forward",
-    "tg_mlp_backward", "tg_mlp_adam_step",
+    "tg_mlp_backward", "tg_mlp_adam_step", "tg_factor_create", "tg_factor_destroy", "tg_factor_solve",
+    "tg_factor_pullback",
 ]
 
 
@@ -124,6 +125,10 @@ def lib():
         "tg_mlp_forward": (_i, [_vp, _vp, _i64, _vp]),
         "tg_mlp_backward": (_i, [_vp, _vp]),
         "tg_mlp_adam_step": (_i, [_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _i]),
+        "tg_factor_create": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
+        "tg_factor_destroy": (_i, [_vp]),
+        "tg_factor_solve": (_i, [_vp, _vp, _i, _d, _i, _vp, _vp]),
+        "tg_factor_pullback": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

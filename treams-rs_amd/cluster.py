@@ -249,15 +249,115 @@ class IterativeContext:
         return gk.value, gr, ge, gp.reshape(c.N, 3), _back(inc, vector), reports
 
 
-def scattering_gradient(cluster: IterativeSphereCluster, direction, pol, **options):
+class SphereClusterFactor:
+    """LU factors of (I - T C) on the device for any number of illuminations: mirrors
+    sphere_cluster_factor / InteractionFactor (cluster/spheres.rs:85-94,
+    cluster/interaction.rs:180-240).  ``precision`` "c128" or "c64" (complex FP32
+    factors on MFMA); every solve is refined with the FP64 residual to ``rtol``."""
+
+    def __init__(self, cluster: IterativeSphereCluster, precision: str = "c128", rtol: float = 1e-12):
+        p = {"c64": _lib.TG_STORE_C64, "c128": _lib.TG_STORE_C128}[precision]
+        self.cluster, self.precision, self.rtol = cluster, precision, float(rtol)
+        h = ctypes.c_void_p()
+        check(lib().tg_factor_create(cluster._h, p, ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        """Free the factors (n^2 x 8 or 16 bytes of device memory)."""
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().tg_factor_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def dimension(self) -> int:
+        return self.cluster.n
+
+    def _solve(self, b, adjoint: bool):
+        if self._h is None:
+            raise ValueError("the factor was closed")
+        cm, vector = _columns(b, self.cluster.n)
+        cols = cm.shape[0]
+        out = torch.empty_like(cm)
+        conv = (_Conv * cols)()
+        check(lib().tg_factor_solve(self._h, cm.data_ptr(), cols, self.rtol, int(adjoint), out.data_ptr(), conv))
+        reports = [Convergence(0, c.residual_norm, c.rhs_norm, c.refinements) for c in conv]
+        return _back(out, vector), reports
+
+    def solve(self, incident):
+        """X = (I - T C)^{-1} T A for the supplied columns (cluster/interaction.rs:213-218).
+        Returns (X, [Convergence])."""
+        return self._solve(incident, False)
+
+    def solve_adjoint(self, rhs):
+        """(I - T C)^{-H} rhs (linalg/mod.rs:436-438)."""
+        return self._solve(rhs, True)
+
+    def pullback_incident(self, cotangent):
+        """T^H (I - T C)^{-H} G (cluster/interaction.rs:220-226)."""
+        y, _ = self._solve(cotangent, True)
+        return self.cluster.local(y, adjoint=True)
+
+    def record(self, incident):
+        """Solve and keep what the pullback needs (cluster/interaction.rs:228-239)."""
+        x, conv = self.solve(incident)
+        return x, FactorContext(self, incident, x), conv
+
+
+class FactorContext:
+    """Pullback of a factored solve with respect to k0, radii, permittivities,
+    positions and the incident field, the adjoint solve reusing the LU factors
+    (cluster/spheres.rs:321-378)."""
+
+    def __init__(self, factor: SphereClusterFactor, incident, x):
+        self.factor = factor
+        self.incident = torch.as_tensor(incident).to(factor.cluster._dev, torch.complex128)
+        self.x = x
+
+    def pullback(self, cotangent):
+        """Returns (k0, radii[N], eps[N], positions[N, 3], incident, convergence)."""
+        f, c = self.factor, self.factor.cluster
+        a, vector = _columns(self.incident, c.n)
+        xc, _ = _columns(self.x, c.n)
+        g, _ = _columns(cotangent, c.n)
+        cols = a.shape[0]
+        inc = torch.empty_like(a)
+        gk = ctypes.c_double()
+        gr = np.zeros(c.N)
+        ge = np.zeros(c.N, np.complex128)
+        gp = np.zeros(3 * c.N)
+        conv = (_Conv * cols)()
+        check(lib().tg_factor_pullback(f._h, a.data_ptr(), xc.data_ptr(), g.data_ptr(), cols, f.rtol,
+                                       ctypes.byref(gk), gr.ctypes.data, ge.ctypes.data, gp.ctypes.data,
+                                       inc.data_ptr(), conv))
+        reports = [(v.refinements, v.residual_norm, v.rhs_norm) for v in conv]
+        return gk.value, gr, ge, gp.reshape(c.N, 3), _back(inc, vector), reports
+
+
+def sphere_cluster_factor(lmax: int, k0: float, radii, epsilon, positions, precision: str = "c128",
+                          rtol: float = 1e-12) -> SphereClusterFactor:
+    """Assemble and factor a sphere cluster (cluster/spheres.rs:85-94)."""
+    return SphereClusterFactor(IterativeSphereCluster(lmax, k0, radii, epsilon, positions), precision, rtol)
+
+
+def scattering_gradient(cluster, direction, pol, **options):
     """sigma_sca of one plane wave and its gradients with respect to the radii and the
     positions: through the solve (cluster pullback), the regular overlap R and the
     incident position phases e^{i k.r_j}
-    (python/treams_rs/_framework_tmatrix.py:231-289 differentiated)."""
-    c = cluster
+    (python/treams_rs/_framework_tmatrix.py:231-289 differentiated).  ``cluster`` is an
+    IterativeSphereCluster (GMRES with ``options``) or a SphereClusterFactor (LU)."""
+    if isinstance(cluster, SphereClusterFactor):
+        solver, c = cluster, cluster.cluster
+        a = c.plane_wave(direction, pol)
+        x, ctx, conv = solver.record(a)
+    else:
+        c = cluster
+        a = c.plane_wave(direction, pol)
+        x, ctx, conv = c.record(a, **options)
     kvec = c.k0 * np.asarray(direction, np.float64) / np.linalg.norm(direction)
-    a = c.plane_wave(direction, pol)
-    x, ctx, conv = c.record(a, **options)
     w = 1.0 / c.k0 ** 2
     rx = c.overlap(x)
     rhx = c.overlap(x, adjoint=True)

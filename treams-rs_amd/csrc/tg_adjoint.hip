@@ -237,12 +237,15 @@ __global__ void k_mie_pullback(const cd *mie, int N, int lmax, int H, int64_t n,
 
 using namespace tg;
 
-extern "C" {
+namespace tg {
 
-int tg_cluster_pullback(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent, int cols,
-                        const tg_gmres_options *o, double *grad_k0, double *grad_radii, tg_c128 *grad_eps,
-                        double *grad_positions, tg_c128 *incident_grad, tg_convergence *conv_host) {
-    if (!c || !incident || !x || !cotangent || !o || cols < 1) {
+// Gradients through X = (I - T C)^{-1} T B given an adjoint solver (GMRES or the LU
+// factors): Y = A^{-H} G, B-bar = T^H Y, the pair gradients of C-bar = T^H Y X^H and
+// the Mie gradients of T-bar = Y (B + C X)^H.
+int pullback_with(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent, int cols,
+                  const AdjointSolve &adjoint_solve, double *grad_k0, double *grad_radii, tg_c128 *grad_eps,
+                  double *grad_positions, tg_c128 *incident_grad, tg_convergence *conv_host) {
+    if (!c || !incident || !x || !cotangent || cols < 1) {
         set_error("null argument or no columns");
         return TG_INVALID_INPUT;
     }
@@ -262,11 +265,7 @@ int tg_cluster_pullback(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x
         if ((st = ref_to_split(c, (const cd *)incident, a, cols))) break;
         if ((st = ref_to_split(c, (const cd *)x, X, cols))) break;
         if ((st = ref_to_split(c, (const cd *)cotangent, Gs, cols))) break;
-        tg_gmres_options od = *o;
-        od.atol = 0.0;  // derivative solves are relative-only (linalg/gmres.rs:50-60)
-        if (!(od.rtol > 0)) od.rtol = 1e-10;
-        st = solve_split(c, Gs, Y, cols, od, 1, conv);
-        if (st) break;
+        if ((st = adjoint_solve(Gs, Y, cols, conv))) break;
         if ((st = local_apply(c, Y, U, cols, 1, nullptr))) break;  // T^H Y
         if ((st = split_to_ref(c, U, (cd *)incident_grad, cols))) break;
         if ((st = pair_gradients(c, U, X, cols, 0, gpos, gk))) break;
@@ -305,8 +304,30 @@ int tg_cluster_pullback(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x
     } while (0);
     if (conv_host)
         for (int k = 0; k < cols && k < (int)conv.size(); ++k) conv_host[k] = conv[k];
+    (void)hipStreamSynchronize(c->stream);
     io.release();
     return st;
+}
+
+}  // namespace tg
+
+extern "C" {
+
+int tg_cluster_pullback(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent, int cols,
+                        const tg_gmres_options *o, double *grad_k0, double *grad_radii, tg_c128 *grad_eps,
+                        double *grad_positions, tg_c128 *incident_grad, tg_convergence *conv_host) {
+    if (!o) {
+        set_error("null options");
+        return TG_INVALID_INPUT;
+    }
+    tg_gmres_options od = *o;
+    od.atol = 0.0;  // derivative solves are relative-only (linalg/gmres.rs:50-60)
+    if (!(od.rtol > 0)) od.rtol = 1e-10;
+    AdjointSolve solve = [c, od](const cd *g, cd *y, int k, std::vector<tg_convergence> &conv) {
+        return solve_split(c, g, y, k, od, 1, conv);
+    };
+    return pullback_with(c, incident, x, cotangent, cols, solve, grad_k0, grad_radii, grad_eps, grad_positions,
+                         incident_grad, conv_host);
 }
 
 int tg_cluster_overlap_pullback(tg_cluster *c, const tg_c128 *u, const tg_c128 *v, int cols, double *grad_positions) {

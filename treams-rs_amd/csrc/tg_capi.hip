@@ -523,7 +523,9 @@ int solve_split(tg_cluster *c, const cd *b, cd *x, int cols, const tg_gmres_opti
             outer.release();
             return TG_NOT_CONVERGED;
         }
-        // inner relative tolerance: what the outer still needs, not below FP32 accuracy
+        // inner relative tolerance: half of what the outer still needs, but not below
+        // 1e-6 of the current residual: the c64 operator differs from the FP64 one at
+        // ~1e-7 relative, so one more FP64 residual is needed in any case
         double eta = 1e-6;
         double need = 0.5 * worst_ratio;
         for (int k = 0; k < cols; ++k) {
@@ -534,6 +536,7 @@ int solve_split(tg_cluster *c, const cd *b, cd *x, int cols, const tg_gmres_opti
         GmresParams ip = prm;
         ip.rtol = eta;
         ip.atol = 0.0;
+        ip.inner = true;
         ip.max_iterations = o.max_iterations - iterations;
         std::vector<tg_convergence> ic;
         int its = 0;

@@ -185,6 +185,16 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
         } else {
             TG_TRY(vec_axpby(s, cd(1.0), X, cd(1.0), x, (int64_t)cols * n));
         }
+        if (prm.inner) {
+            bool all = true;
+            for (int k = 0; k < cols; ++k) {
+                if (!cycle[k]) continue;
+                const double est = std::abs(g[(size_t)k * (restart + 1) + used[k]]);
+                conv[k].residual_norm = est;
+                all = all && est <= tol[k];
+            }
+            if (all || iterations >= prm.max_iterations) break;
+        }
         // true residual R = b - A x
         TG_TRY(A(x, R, cols));
         TG_TRY(vec_axpby(s, cd(1.0), b, cd(-1.0), R, (int64_t)cols * n));

@@ -98,6 +98,8 @@ namespace tg {
 int launch_mie(tg_cluster *c);
 int launch_assemble_split(tg_cluster *c, int storage, void *out);
 int launch_assemble_ref(tg_cluster *c, int regular, cd *out);
+// A = I - T C in the split order, column-major n x n, c64 or c128 (precision: tg_storage)
+int launch_assemble_operator(tg_cluster *c, int precision, void *out);
 int launch_translation_blocks(int lmax, const DevPlan &plan, double k, int64_t count, const double *disp,
                               int regular, cd *out, hipStream_t s);
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular);

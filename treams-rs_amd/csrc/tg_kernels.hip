@@ -162,6 +162,28 @@ struct OutRef {
     }
 };
 
+// The operator A = I - T C itself, rows and columns in the split order
+// (h N H + i H + b), column-major with leading dimension n:
+// A[(h, i, b), (h', j, a)] = delta - T_i[h, h'](l_b) C_h'[(i, b), (j, a)].
+template <class S>
+struct OutOperator {
+    S *A;
+    int64_t n, nh;
+    int H, lmax;
+    const cd *mie;
+    __device__ void store(int i, int j, int a, int b, cd vp, cd vm) const {
+        int l = 1;
+        while ((l + 1) * (l + 1) <= b + 1) ++l;  // b = l^2 + l + m - 1
+        const cd *t = mie + (size_t)(i * lmax + l - 1) * 4;
+        const int64_t r = (int64_t)i * H + b, col = (int64_t)j * H + a;
+        const double d = (i == j && a == b) ? 1.0 : 0.0;
+        A[col * n + r] = cvt<S>(cd(d) - t[0] * vp);
+        A[(col + nh) * n + r] = cvt<S>(-(t[1] * vm));
+        A[col * n + r + nh] = cvt<S>(-(t[2] * vp));
+        A[(col + nh) * n + r + nh] = cvt<S>(cd(d) - t[3] * vm);
+    }
+};
+
 // One M x M block per list entry.
 struct OutBlocks {
     double2 *out;
@@ -259,6 +281,26 @@ int launch_assemble_split(tg_cluster *c, int storage, void *outp) {
     }
     TG_HIP(hipGetLastError());
     TG_TRY(c->timer.end("assemble", c->stream, rec));
+    return TG_OK;
+}
+
+int launch_assemble_operator(tg_cluster *c, int precision, void *outp) {
+    constexpr int P = 8;
+    dim3 grid(c->N, (c->N + P - 1) / P);
+    GeoCluster geo{c->b_pos.as<double>(), c->N};
+    size_t smem = assemble_smem(c->plan, P);
+    Timer::Rec rec;
+    TG_TRY(c->timer.begin("assemble_operator", c->stream, rec));
+    if (precision == TG_STORE_C64) {
+        OutOperator<float2> out{(float2 *)outp, c->n, c->nh, c->H, c->lmax, c->b_mie.as<cd>()};
+        k_assemble<P, GeoCluster, OutOperator<float2>, false><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo, out);
+    } else {
+        OutOperator<double2> out{(double2 *)outp, c->n, c->nh, c->H, c->lmax, c->b_mie.as<cd>()};
+        k_assemble<P, GeoCluster, OutOperator<double2>, false><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo,
+                                                                                               out);
+    }
+    TG_HIP(hipGetLastError());
+    TG_TRY(c->timer.end("assemble_operator", c->stream, rec));
     return TG_OK;
 }
 

@@ -1,0 +1,598 @@
+// Dense LU of the cluster operator A = I - T C on the device, in c64 (refined in FP64)
+// or c128: replaces Lu::new / solve_in_place / solve_adjoint_in_place
+// (linalg/mod.rs:398-531) behind InteractionFactor (cluster/interaction.rs:180-240)
+// and sphere_cluster_factor (cluster/spheres.rs:85-94).
+//
+// Storage: A in the helicity-split order (rows and columns h N H + i H + b),
+// column-major, leading dimension n, assembled directly by the pair kernel.
+// Blocked right-looking LU with block width NB = 64: the diagonal block is factored
+// in LDS by one wave with partial pivoting inside the block, its row swaps are
+// applied to the whole block row, the panels below and to the right are solved
+// against U11 / L11, and the trailing matrix is updated by a complex GEMM in the 3M
+// form (three real products per complex product) on MFMA:
+// v_mfma_f32_16x16x4f32 for c64, v_mfma_f64_16x16x4f64 for c128, FP32 / FP64
+// accumulation.  Pivoting inside diagonal blocks keeps every update a GEMM; the
+// reference pivots over the whole column (faer).  The operator's identity self
+// blocks make the diagonal blocks well conditioned for separated spheres, and every
+// solve is certified by the FP64 residual b - A x: an unstable factorisation shows
+// up as a refinement that fails with TG_NOT_CONVERGED, never as a wrong answer.
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "tg_gmres.h"
+#include "tg_internal.h"
+
+struct tg_factor {
+    tg_cluster *cluster = nullptr;
+    int precision = TG_STORE_C128;
+    int64_t n = 0;
+    tg::DevBuf lu, piv, perm, status, work;  // perm: P x = x[perm]; work: 8 n FP64 values
+};
+
+namespace tg {
+
+constexpr int NB = 64;
+
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+typedef double f64x4v __attribute__((ext_vector_type(4)));
+
+template <class T> struct mfma_acc;
+template <> struct mfma_acc<float> { using V = f32x4v; };
+template <> struct mfma_acc<double> { using V = f64x4v; };
+
+// D[16x16] += A[16x4] B[4x16]: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16].
+// The result D[i][j] sits in lane j + 16 (i / 4), element i % 4 for f32, and in lane
+// j + 16 (i % 4), element i / 4 for f64.
+template <class T>
+__device__ __forceinline__ int mfma_out_row(int lane, int v) {
+    return sizeof(T) == 8 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
+}
+__device__ __forceinline__ f32x4v mfma16x16x4(float a, float b, f32x4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4v mfma16x16x4(double a, double b, f64x4v c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Plain complex value for LDS arrays (no constructors).
+template <class T>
+struct lcx {
+    T re, im;
+    __device__ __forceinline__ cx<T> get() const { return cx<T>(re, im); }
+    __device__ __forceinline__ void set(cx<T> v) { re = v.re; im = v.im; }
+};
+
+// ------------------------------------------------------------ factor kernels --
+
+// Factor the nb x nb diagonal block at k0 in LDS, one wave, lane r owning row r for
+// the update; piv[k0 + k] = global row exchanged with row k0 + k.
+template <class T>
+__global__ __launch_bounds__(64) void k_lu_diag(cx<T> *A, int64_t n, int64_t k0, int nb, int *piv, int *status) {
+    __shared__ lcx<T> blk[NB][NB];  // [col][row]
+    const int lane = threadIdx.x;
+    for (int t = lane; t < nb * nb; t += 64) {
+        const int col = t / nb, row = t - col * nb;
+        blk[col][row].set(A[(k0 + col) * n + k0 + row]);
+    }
+    __syncthreads();
+    for (int k = 0; k < nb; ++k) {
+        double best = (lane >= k && lane < nb) ? (double)abs2(blk[k][lane].get()) : -1.0;
+        int besti = lane;
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(best, off);
+            const int oi = __shfl_xor(besti, off);
+            if (ov > best || (ov == best && oi < besti)) {
+                best = ov;
+                besti = oi;
+            }
+        }
+        if (!(best > 0.0)) {
+            if (lane == 0) *status = 1;
+            return;
+        }
+        const int p = besti;
+        if (lane == 0) piv[k0 + k] = (int)(k0 + p);
+        if (p != k && lane < nb) {
+            const lcx<T> tmp = blk[lane][k];
+            blk[lane][k] = blk[lane][p];
+            blk[lane][p] = tmp;
+        }
+        __syncthreads();
+        const cx<T> inv = cx<T>((T)1) / blk[k][k].get();
+        if (lane > k && lane < nb) {
+            const cx<T> lr = blk[k][lane].get() * inv;
+            blk[k][lane].set(lr);
+            for (int cc = k + 1; cc < nb; ++cc) blk[cc][lane].set(blk[cc][lane].get() - lr * blk[cc][k].get());
+        }
+        __syncthreads();
+    }
+    for (int t = lane; t < nb * nb; t += 64) {
+        const int col = t / nb, row = t - col * nb;
+        A[(k0 + col) * n + k0 + row] = blk[col][row].get();
+    }
+}
+
+// Apply the block's row swaps to every column outside the diagonal block.
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_swap(cx<T> *A, int64_t n, int64_t k0, int nb, const int *piv) {
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n || (col >= k0 && col < k0 + nb)) return;
+    cx<T> *c = A + col * n;
+    for (int k = 0; k < nb; ++k) {
+        const int64_t p = piv[k0 + k];
+        if (p != k0 + k) {
+            cx<T> tmp = c[k0 + k];
+            c[k0 + k] = c[p];
+            c[p] = tmp;
+        }
+    }
+}
+
+// L21 = A21 U11^{-1}, in place: one thread per row below the block (coalesced
+// column reads), U11 in LDS.
+template <class T>
+__global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A, int64_t n, int64_t k0, int nb) {
+    __shared__ lcx<T> U[NB][NB];  // [col][row]
+    for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
+        const int col = t / nb, row = t - col * nb;
+        U[col][row].set(A[(k0 + col) * n + k0 + row]);
+    }
+    __syncthreads();
+    const int64_t r = k0 + nb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (int c = 0; c < nb; ++c) {
+        cx<T> s = A[(k0 + c) * n + r];
+        for (int k = 0; k < c; ++k) s -= A[(k0 + k) * n + r] * U[c][k].get();
+        A[(k0 + c) * n + r] = s / U[c][c].get();
+    }
+}
+
+// U12 = L11^{-1} A12 (unit lower L11), in place: one wave per column, lane r holding
+// row r; each step broadcasts the solved entry and updates the rows below it.
+template <class T>
+__global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A, int64_t n, int64_t k0, int nb) {
+    __shared__ lcx<T> L[NB][NB];  // [col][row]
+    for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
+        const int col = t / nb, row = t - col * nb;
+        L[col][row].set(A[(k0 + col) * n + k0 + row]);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t col = k0 + nb + (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); col < n;
+         col += waves) {
+        cx<T> *c = A + col * n + k0;
+        cx<T> v = lane < nb ? c[lane] : cx<T>();
+        for (int k = 0; k < nb - 1; ++k) {
+            const cx<T> ck(__shfl(v.re, k), __shfl(v.im, k));
+            if (lane > k && lane < nb) v -= L[k][lane].get() * ck;
+        }
+        if (lane < nb) c[lane] = v;
+    }
+}
+
+// A22 -= L21 U12 with the 3M complex product on MFMA.  64 x 64 output tile per
+// workgroup (4 waves, 2 x 2 16x16 MFMA tiles each); the MFMA computes the transposed
+// tile (U12^T L21^T) so that lanes run along rows and the column-major read-modify-
+// write of A22 is coalesced.
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_update(cx<T> *A, int64_t n, int64_t k0, int nb) {
+    using V = typename mfma_acc<T>::V;
+    __shared__ T Lr[64][17], Li[64][17], Ur[16][65], Ui[16][65];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t r0 = k0 + nb + (int64_t)blockIdx.y * 64, c0 = k0 + nb + (int64_t)blockIdx.x * 64;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    V acc1[2][2], acc2[2][2], acc3[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int v = 0; v < 4; ++v) acc1[a][b][v] = acc2[a][b][v] = acc3[a][b][v] = (T)0;
+    const int rr = lane & 15, kq = lane >> 4;
+    for (int kk = 0; kk < nb; kk += 16) {
+        for (int t = tid; t < 64 * 16; t += 256) {
+            const int k = t >> 6, m = t & 63;  // L21 tile: rows m, columns k
+            const int64_t gr = r0 + m;
+            const cx<T> v = (gr < n && kk + k < nb) ? A[(k0 + kk + k) * n + gr] : cx<T>();
+            Lr[m][k] = v.re;
+            Li[m][k] = v.im;
+            const int k2 = t & 15, cc = t >> 4;  // U12 tile: rows k2, columns cc
+            const int64_t gc = c0 + cc;
+            const cx<T> u = (gc < n && kk + k2 < nb) ? A[gc * n + k0 + kk + k2] : cx<T>();
+            Ur[k2][cc] = u.re;
+            Ui[k2][cc] = u.im;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k4 = 0; k4 < 16; k4 += 4) {
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const T br = Ur[k4 + kq][wn + b * 16 + rr], bi = Ui[k4 + kq][wn + b * 16 + rr];
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    const T ar = Lr[wm + a * 16 + rr][k4 + kq], ai = Li[wm + a * 16 + rr][k4 + kq];
+                    acc1[a][b] = mfma16x16x4(br, ar, acc1[a][b]);
+                    acc2[a][b] = mfma16x16x4(bi, ai, acc2[a][b]);
+                    acc3[a][b] = mfma16x16x4(br + bi, ar + ai, acc3[a][b]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // transposed tile: element v of lane l is (row l % 16, column mfma_out_row(l, v))
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int v = 0; v < 4; ++v) {
+                const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<T>(lane, v);
+                if (gr < n && gc < n) {
+                    cx<T> &d = A[gc * n + gr];
+                    const T t1 = acc1[a][b][v], t2 = acc2[a][b][v], t3 = acc3[a][b][v];
+                    d.re -= t1 - t2;
+                    d.im -= t3 - t1 - t2;
+                }
+            }
+}
+
+// ------------------------------------------------------------- solve kernels --
+
+// Row permutation of `cols` vector columns: dst[i] = src[perm[i]] (forward, P x) or
+// dst[perm[i]] = src[i] (inverse, P^T x).
+__global__ void k_lu_permute(const cd *src, cd *dst, int64_t n, int cols, const int *perm, int inverse) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * cols) return;
+    const int64_t col = t / n, i = t - col * n;
+    const int64_t p = perm[i];
+    if (!inverse)
+        dst[col * n + i] = src[col * n + p];
+    else
+        dst[col * n + p] = src[col * n + i];
+}
+
+// Triangular solve with the diagonal block at k0, one wave per vector column, lane r
+// holding row r (FP64 vectors, factors in T).  which = 0: L y = b (unit lower);
+// 1: U x = y; 2: U^H z = g; 3: L^H w = z (unit upper).
+template <class T>
+__global__ __launch_bounds__(64) void k_lu_block_solve(const cx<T> *A, int64_t n, int64_t k0, int nb, cd *x,
+                                                       int which) {
+    const int lane = threadIdx.x;
+    cd *v = x + (int64_t)blockIdx.x * n + k0;
+    cd val = lane < nb ? v[lane] : cd();
+    auto a = [&](int r, int c) {
+        const cx<T> e = A[(k0 + c) * n + k0 + r];
+        return cd((double)e.re, (double)e.im);
+    };
+    auto bcast = [&](int k) { return cd(__shfl(val.re, k), __shfl(val.im, k)); };
+    if (which == 0) {
+        for (int k = 0; k < nb - 1; ++k) {
+            const cd vk = bcast(k);
+            if (lane > k && lane < nb) val -= a(lane, k) * vk;
+        }
+    } else if (which == 1) {
+        for (int k = nb - 1; k >= 0; --k) {
+            if (lane == k) val = val / a(k, k);
+            const cd vk = bcast(k);
+            if (lane < k) val -= a(lane, k) * vk;
+        }
+    } else if (which == 2) {
+        // U^H is lower triangular: (U^H)[r][k] = conj(U[k][r])
+        for (int k = 0; k < nb; ++k) {
+            if (lane == k) val = val / conj(a(k, k));
+            const cd vk = bcast(k);
+            if (lane > k && lane < nb) val -= conj(a(k, lane)) * vk;
+        }
+    } else {
+        for (int k = nb - 1; k > 0; --k) {
+            const cd vk = bcast(k);
+            if (lane < k) val -= conj(a(k, lane)) * vk;
+        }
+    }
+    if (lane < nb) v[lane] = val;
+}
+
+// Off-diagonal part of the solves, one thread per row and up to 8 vector columns:
+// which = 0: x[r] -= sum_k A[r, k0 + k] x[k0 + k] for r >= k0 + nb (L21);
+// which = 1: x[r] -= sum_k A[r, k0 + k] x[k0 + k] for r < k0 (U01);
+// which = 2: x[c] -= sum_k conj(A[k0 + k, c]) x[k0 + k] for c >= k0 + nb (U12^H);
+// which = 3: x[c] -= sum_k conj(A[k0 + k, c]) x[k0 + k] for c < k0 (L10^H).
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_panel_solve(const cx<T> *A, int64_t n, int64_t k0, int nb, cd *x,
+                                                        int cols, int which) {
+    __shared__ lcx<double> seg[8][NB];
+    const bool below = (which == 0 || which == 2);
+    const int64_t rcount = below ? n - k0 - nb : k0;
+    for (int t = threadIdx.x; t < cols * nb; t += blockDim.x)
+        seg[t / nb][t % nb].set(x[(int64_t)(t / nb) * n + k0 + t % nb]);
+    __syncthreads();
+    const int64_t rr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (rr >= rcount) return;
+    const int64_t r = below ? k0 + nb + rr : rr;
+    cd s[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[q] = cd(0.0);
+    if (which < 2) {
+        for (int k = 0; k < nb; ++k) {
+            const cx<T> e = A[(k0 + k) * n + r];
+            const cd ev((double)e.re, (double)e.im);
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (q < cols) s[q] += ev * seg[q][k].get();
+        }
+    } else {
+        const cx<T> *c = A + r * n + k0;
+        for (int k = 0; k < nb; ++k) {
+            const cd ev((double)c[k].re, -(double)c[k].im);
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (q < cols) s[q] += ev * seg[q][k].get();
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (q < cols) x[(int64_t)q * n + r] -= s[q];
+}
+
+template <class T>
+static int factor_t(tg_factor *f) {
+    tg_cluster *c = f->cluster;
+    const int64_t n = c->n;
+    cx<T> *A = f->lu.as<cx<T>>();
+    int *piv = f->piv.as<int>();
+    int *status = f->status.as<int>();
+    TG_HIP(hipMemsetAsync(status, 0, sizeof(int), c->stream));
+    TG_TRY(launch_assemble_operator(c, f->precision, A));
+    Timer::Rec rec;
+    TG_TRY(c->timer.begin("lu_factor", c->stream, rec));
+    for (int64_t k0 = 0; k0 < n; k0 += NB) {
+        const int nb = (int)std::min<int64_t>(NB, n - k0);
+        k_lu_diag<T><<<1, 64, 0, c->stream>>>(A, n, k0, nb, piv, status);
+        k_lu_swap<T><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(A, n, k0, nb, piv);
+        const int64_t rest = n - k0 - nb;
+        if (rest > 0) {
+            k_trsm_lower_panel<T><<<(unsigned)((rest + 255) / 256), 256, 0, c->stream>>>(A, n, k0, nb);
+            k_trsm_upper_panel<T>
+                <<<(unsigned)std::min<int64_t>((rest + 3) / 4, 1024), 256, 0, c->stream>>>(A, n, k0, nb);
+            dim3 grid((unsigned)((rest + 63) / 64), (unsigned)((rest + 63) / 64));
+            k_lu_update<T><<<grid, 256, 0, c->stream>>>(A, n, k0, nb);
+        }
+        TG_HIP(hipGetLastError());
+    }
+    TG_TRY(c->timer.end("lu_factor", c->stream, rec));
+    int st = 0;
+    TG_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    if (st) {
+        set_error("zero pivot in the LU factorisation");
+        return TG_SINGULAR;
+    }
+    // compose the exchanges into one gather: (P x)[i] = x[perm[i]]
+    std::vector<int> piv_h(n), perm_h(n);
+    TG_HIP(hipMemcpy(piv_h.data(), piv, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) perm_h[i] = (int)i;
+    for (int64_t k = 0; k < n; ++k) std::swap(perm_h[k], perm_h[piv_h[k]]);
+    TG_HIP(hipMemcpy(f->perm.as<int>(), perm_h.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    return TG_OK;
+}
+
+// x (split order, FP64, `cols` <= 8 columns) <- A^{-1} x or A^{-H} x with the factors.
+template <class T>
+static int lu_solve_t(tg_factor *f, cd *x, int cols, int adjoint) {
+    tg_cluster *c = f->cluster;
+    hipStream_t s = c->stream;
+    const int64_t n = f->n;
+    const cx<T> *A = f->lu.as<cx<T>>();
+    const int *perm = f->perm.as<int>();
+    cd *work = f->work.as<cd>();
+    auto grid = [](int64_t w) { return (unsigned)std::max<int64_t>(1, (w + 255) / 256); };
+    Timer::Rec rec;
+    TG_TRY(c->timer.begin("lu_solve", s, rec));
+    const int64_t last = ((n - 1) / NB) * NB;
+    if (!adjoint) {
+        // P A = L U: x <- P x, L y = x, U x = y
+        k_lu_permute<<<grid(n * cols), 256, 0, s>>>(x, work, n, cols, perm, 0);
+        TG_HIP(hipMemcpyAsync(x, work, sizeof(cd) * n * cols, hipMemcpyDeviceToDevice, s));
+        for (int64_t k0 = 0; k0 < n; k0 += NB) {
+            const int nb = (int)std::min<int64_t>(NB, n - k0);
+            k_lu_block_solve<T><<<cols, 64, 0, s>>>(A, n, k0, nb, x, 0);
+            if (n - k0 - nb > 0) k_lu_panel_solve<T><<<grid(n - k0 - nb), 256, 0, s>>>(A, n, k0, nb, x, cols, 0);
+        }
+        for (int64_t k0 = last; k0 >= 0; k0 -= NB) {
+            const int nb = (int)std::min<int64_t>(NB, n - k0);
+            k_lu_block_solve<T><<<cols, 64, 0, s>>>(A, n, k0, nb, x, 1);
+            if (k0 > 0) k_lu_panel_solve<T><<<grid(k0), 256, 0, s>>>(A, n, k0, nb, x, cols, 1);
+        }
+    } else {
+        // A^H = U^H L^H P: U^H z = g, L^H w = z, x = P^T w
+        for (int64_t k0 = 0; k0 < n; k0 += NB) {
+            const int nb = (int)std::min<int64_t>(NB, n - k0);
+            k_lu_block_solve<T><<<cols, 64, 0, s>>>(A, n, k0, nb, x, 2);
+            if (n - k0 - nb > 0) k_lu_panel_solve<T><<<grid(n - k0 - nb), 256, 0, s>>>(A, n, k0, nb, x, cols, 2);
+        }
+        for (int64_t k0 = last; k0 >= 0; k0 -= NB) {
+            const int nb = (int)std::min<int64_t>(NB, n - k0);
+            k_lu_block_solve<T><<<cols, 64, 0, s>>>(A, n, k0, nb, x, 3);
+            if (k0 > 0) k_lu_panel_solve<T><<<grid(k0), 256, 0, s>>>(A, n, k0, nb, x, cols, 3);
+        }
+        k_lu_permute<<<grid(n * cols), 256, 0, s>>>(x, work, n, cols, perm, 1);
+        TG_HIP(hipMemcpyAsync(x, work, sizeof(cd) * n * cols, hipMemcpyDeviceToDevice, s));
+    }
+    TG_HIP(hipGetLastError());
+    TG_TRY(c->timer.end("lu_solve", s, rec));
+    return TG_OK;
+}
+
+static int lu_solve(tg_factor *f, cd *x, int cols, int adjoint) {
+    for (int c0 = 0; c0 < cols; c0 += 8) {
+        const int w = std::min(8, cols - c0);
+        cd *xc = x + (int64_t)c0 * f->n;
+        TG_TRY(f->precision == TG_STORE_C64 ? lu_solve_t<float>(f, xc, w, adjoint)
+                                            : lu_solve_t<double>(f, xc, w, adjoint));
+    }
+    return TG_OK;
+}
+
+// x = A^{-1} b (or A^{-H} b) in the split order with FP64 iterative refinement on the
+// matrix-free FP64 operator, until |b - A x| <= rtol |b| for every column.
+int factor_solve_split(tg_factor *f, const cd *b, cd *x, int cols, int adjoint, double rtol,
+                       std::vector<tg_convergence> &conv) {
+    tg_cluster *c = f->cluster;
+    const int64_t count = (int64_t)cols * c->n;
+    DevBuf tmp;
+    TG_TRY(tmp.ensure(sizeof(cd) * count * 2));
+    cd *r = tmp.as<cd>(), *ax = r + count;
+    std::vector<double> bn(cols), rn(cols);
+    int st = TG_OK;
+    conv.assign(cols, tg_convergence{0, 0, 0.0, 0.0});
+    do {
+        if ((st = vec_norms(c->stream, b, c->n, cols, c->n, bn.data(), c->ws[4]))) break;
+        if (hipMemcpyAsync(x, b, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+            set_error("copy of the right-hand side failed");
+            st = TG_DEVICE_ERROR;
+            break;
+        }
+        if ((st = lu_solve(f, x, cols, adjoint))) break;
+        int refinements = 0;
+        const int max_refine = f->precision == TG_STORE_C64 ? 12 : 4;
+        while (true) {
+            if ((st = apply_operator(c, x, ax, cols, adjoint, TG_OP_MATRIX_FREE_F64))) break;
+            if (hipMemcpyAsync(r, b, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+                set_error("copy of the right-hand side failed");
+                st = TG_DEVICE_ERROR;
+                break;
+            }
+            if ((st = vec_axpby(c->stream, cd(-1.0), ax, cd(1.0), r, count))) break;
+            if ((st = vec_norms(c->stream, r, c->n, cols, c->n, rn.data(), c->ws[4]))) break;
+            bool done = true;
+            for (int k = 0; k < cols; ++k) {
+                conv[k].residual_norm = rn[k];
+                conv[k].rhs_norm = bn[k];
+                conv[k].refinements = refinements;
+                if (!std::isfinite(rn[k])) {
+                    set_error("non-finite residual after the LU solve (numerically singular operator)");
+                    st = TG_SINGULAR;
+                }
+                if (rn[k] > rtol * bn[k]) done = false;
+            }
+            if (st || done) break;
+            if (refinements >= max_refine) {
+                set_error("LU refinement did not reach rtol %g after %d steps (relative residual %e)", rtol,
+                          refinements, rn[0] / (bn[0] > 0 ? bn[0] : 1.0));
+                st = TG_NOT_CONVERGED;
+                break;
+            }
+            if ((st = lu_solve(f, r, cols, adjoint))) break;
+            if ((st = vec_axpby(c->stream, cd(1.0), r, cd(1.0), x, count))) break;
+            ++refinements;
+        }
+    } while (0);
+    tmp.release();
+    return st;
+}
+
+// Gradient body shared with the GMRES pullback (tg_adjoint.hip), here with the
+// adjoint solve by the factors.
+int pullback_with(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent, int cols,
+                  const AdjointSolve &adjoint_solve, double *grad_k0, double *grad_radii, tg_c128 *grad_eps,
+                  double *grad_positions, tg_c128 *incident_grad, tg_convergence *conv_host);
+
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" {
+
+int tg_factor_create(tg_cluster *c, int precision, tg_factor **out) {
+    if (!c || !out || (precision != TG_STORE_C64 && precision != TG_STORE_C128)) {
+        set_error("null argument or precision other than TG_STORE_C64 / TG_STORE_C128");
+        return TG_INVALID_INPUT;
+    }
+    *out = nullptr;
+    tg_factor *f = new tg_factor;
+    f->cluster = c;
+    f->precision = precision;
+    f->n = c->n;
+    const size_t esz = precision == TG_STORE_C64 ? sizeof(cf) : sizeof(cd);
+    int st = f->lu.ensure(esz * (size_t)c->n * (size_t)c->n);
+    if (!st) st = f->piv.ensure(sizeof(int) * (size_t)c->n);
+    if (!st) st = f->perm.ensure(sizeof(int) * (size_t)c->n);
+    if (!st) st = f->work.ensure(sizeof(cd) * 8 * (size_t)c->n);
+    if (!st) st = f->status.ensure(sizeof(int));
+    if (!st) st = precision == TG_STORE_C64 ? factor_t<float>(f) : factor_t<double>(f);
+    if (st) {
+        (void)hipStreamSynchronize(c->stream);
+        f->lu.release();
+        f->piv.release();
+        f->perm.release();
+        f->work.release();
+        f->status.release();
+        delete f;
+        return st;
+    }
+    *out = f;
+    return TG_OK;
+}
+
+int tg_factor_destroy(tg_factor *f) {
+    if (!f) return TG_OK;
+    (void)hipStreamSynchronize(f->cluster->stream);
+    f->lu.release();
+    f->piv.release();
+    f->perm.release();
+    f->work.release();
+    f->status.release();
+    delete f;
+    return TG_OK;
+}
+
+int tg_factor_solve(tg_factor *f, const tg_c128 *incident, int cols, double rtol, int adjoint, tg_c128 *x,
+                    tg_convergence *conv_host) {
+    if (!f || !incident || !x || cols < 1 || !(rtol > 0.0)) {
+        set_error("null argument, no columns or non-positive rtol");
+        return TG_INVALID_INPUT;
+    }
+    tg_cluster *c = f->cluster;
+    const int64_t count = (int64_t)cols * c->n;
+    DevBuf io;
+    TG_TRY(io.ensure(sizeof(cd) * count * 3));
+    cd *a = io.as<cd>(), *b = a + count, *xs = b + count;
+    std::vector<tg_convergence> conv;
+    int st = ref_to_split(c, (const cd *)incident, a, cols);
+    // rhs = T a for the forward solve (InteractionFactor::solve); the adjoint solve
+    // takes its right-hand side as it is (Lu::solve_adjoint_in_place)
+    if (!st) {
+        if (adjoint) {
+            if (hipMemcpyAsync(b, a, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+                set_error("copy of the right-hand side failed");
+                st = TG_DEVICE_ERROR;
+            }
+        } else {
+            st = local_apply(c, a, b, cols, 0, nullptr);
+        }
+    }
+    if (!st) st = factor_solve_split(f, b, xs, cols, adjoint, rtol, conv);
+    if (!st) st = split_to_ref(c, xs, (cd *)x, cols);
+    if (!st && hipStreamSynchronize(c->stream) != hipSuccess) {
+        set_error("device failure in the LU solve");
+        st = TG_DEVICE_ERROR;
+    }
+    if (conv_host)
+        for (int k = 0; k < cols && k < (int)conv.size(); ++k) conv_host[k] = conv[k];
+    (void)hipStreamSynchronize(c->stream);
+    io.release();
+    return st;
+}
+
+int tg_factor_pullback(tg_factor *f, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent, int cols,
+                       double rtol, double *grad_k0, double *grad_radii, tg_c128 *grad_eps, double *grad_positions,
+                       tg_c128 *incident_grad, tg_convergence *conv_host) {
+    if (!f || !(rtol > 0.0)) {
+        set_error("null factor or non-positive rtol");
+        return TG_INVALID_INPUT;
+    }
+    AdjointSolve solve = [f, rtol](const cd *g, cd *y, int k, std::vector<tg_convergence> &conv) {
+        return factor_solve_split(f, g, y, k, 1, rtol, conv);
+    };
+    return pullback_with(f->cluster, incident, x, cotangent, cols, solve, grad_k0, grad_radii, grad_eps,
+                         grad_positions, incident_grad, conv_host);
+}
+
+}  // extern "C"
