@@ -87,6 +87,7 @@ class IterativeSphereCluster:
         check(lib().tg_cluster_create(self.lmax, self.k0, radii.size, radii.ctypes.data, eps.ctypes.data,
                                       pos.ctypes.data, ctypes.c_void_p(_stream_ptr()), ctypes.byref(h)))
         self._h = h
+        self._destroy = lib().tg_cluster_destroy  # bound now: module globals are gone at exit
         self.N = radii.size
         self.modes_per_sphere = lib().tg_modes_per_sphere(self.lmax)
         self.n = int(lib().tg_cluster_dimension(h))
@@ -94,7 +95,7 @@ class IterativeSphereCluster:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().tg_cluster_destroy(h)
+            self._destroy(h)
             self._h = None
 
     # ---------------------------------------------------------------- queries
@@ -261,12 +262,13 @@ class SphereClusterFactor:
         h = ctypes.c_void_p()
         check(lib().tg_factor_create(cluster._h, p, ctypes.byref(h)))
         self._h = h
+        self._destroy = lib().tg_factor_destroy
 
     def close(self) -> None:
         """Free the factors (n^2 x 8 or 16 bytes of device memory)."""
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().tg_factor_destroy(h)
+            self._destroy(h)
         self._h = None
 
     def __del__(self):

@@ -176,7 +176,10 @@ __global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A, int64_t n, i
 // A22 -= L21 U12 with the 3M complex product on MFMA.  64 x 64 output tile per
 // workgroup (4 waves, 2 x 2 16x16 MFMA tiles each); the MFMA computes the transposed
 // tile (U12^T L21^T) so that lanes run along rows and the column-major read-modify-
-// write of A22 is coalesced.
+// write of A22 is coalesced.  The A22 tile is loaded into registers before the K loop
+// and the next 16-wide K slice of L21 / U12 is loaded while the current one is
+// multiplied, so no load waits on another.  Rows and columns beyond n are read
+// clamped (valid addresses) and never written.
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_update(cx<T> *A, int64_t n, int64_t k0, int nb) {
     using V = typename mfma_acc<T>::V;
@@ -184,25 +187,51 @@ __global__ __launch_bounds__(256) void k_lu_update(cx<T> *A, int64_t n, int64_t 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t r0 = k0 + nb + (int64_t)blockIdx.y * 64, c0 = k0 + nb + (int64_t)blockIdx.x * 64;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    V acc1[2][2], acc2[2][2], acc3[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-            for (int v = 0; v < 4; ++v) acc1[a][b][v] = acc2[a][b][v] = acc3[a][b][v] = (T)0;
     const int rr = lane & 15, kq = lane >> 4;
+    // this thread's 16 elements of the A22 tile, in the MFMA output layout
+    cx<T> creg[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t gr = min(r0 + wm + a * 16 + rr, n - 1);
+                const int64_t gc = min(c0 + wn + b * 16 + mfma_out_row<T>(lane, v), n - 1);
+                creg[a][b][v] = A[gc * n + gr];
+            }
+    // K-slice loads: L21 (64 rows x 16) at (m = tid & 63, k = (tid >> 6) + 4 j),
+    // U12 (16 x 64 columns) at (k = tid & 15, column = (tid >> 4) + 16 j)
+    const int lm = tid & 63, lk = tid >> 6, uk = tid & 15, uc = tid >> 4;
+    const int64_t lrow = min(r0 + lm, n - 1);
+    cx<T> lreg[4], ureg[4];
+    auto load_slice = [&](int kk) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            lreg[j] = A[(k0 + kk + lk + 4 * j) * n + lrow];
+            const int64_t gc = min(c0 + uc + 16 * j, n - 1);
+            ureg[j] = A[gc * n + k0 + kk + uk];
+        }
+    };
+    V acc1[2][2], acc2[2][2], acc3[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc1[a][b][v] = acc2[a][b][v] = acc3[a][b][v] = (T)0;
+    load_slice(0);
     for (int kk = 0; kk < nb; kk += 16) {
-        for (int t = tid; t < 64 * 16; t += 256) {
-            const int k = t >> 6, m = t & 63;  // L21 tile: rows m, columns k
-            const int64_t gr = r0 + m;
-            const cx<T> v = (gr < n && kk + k < nb) ? A[(k0 + kk + k) * n + gr] : cx<T>();
-            Lr[m][k] = v.re;
-            Li[m][k] = v.im;
-            const int k2 = t & 15, cc = t >> 4;  // U12 tile: rows k2, columns cc
-            const int64_t gc = c0 + cc;
-            const cx<T> u = (gc < n && kk + k2 < nb) ? A[gc * n + k0 + kk + k2] : cx<T>();
-            Ur[k2][cc] = u.re;
-            Ui[k2][cc] = u.im;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            Lr[lm][lk + 4 * j] = lreg[j].re;
+            Li[lm][lk + 4 * j] = lreg[j].im;
+            Ur[uk][uc + 16 * j] = ureg[j].re;
+            Ui[uk][uc + 16 * j] = ureg[j].im;
         }
         __syncthreads();
+        if (kk + 16 < nb) load_slice(kk + 16);
 #pragma unroll
         for (int k4 = 0; k4 < 16; k4 += 4) {
 #pragma unroll
@@ -217,19 +246,20 @@ __global__ __launch_bounds__(256) void k_lu_update(cx<T> *A, int64_t n, int64_t 
                 }
             }
         }
-        __syncthreads();
     }
     // transposed tile: element v of lane l is (row l % 16, column mfma_out_row(l, v))
+#pragma unroll
     for (int a = 0; a < 2; ++a)
+#pragma unroll
         for (int b = 0; b < 2; ++b)
+#pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<T>(lane, v);
-                if (gr < n && gc < n) {
-                    cx<T> &d = A[gc * n + gr];
-                    const T t1 = acc1[a][b][v], t2 = acc2[a][b][v], t3 = acc3[a][b][v];
-                    d.re -= t1 - t2;
-                    d.im -= t3 - t1 - t2;
-                }
+                const T t1 = acc1[a][b][v], t2 = acc2[a][b][v], t3 = acc3[a][b][v];
+                cx<T> d = creg[a][b][v];
+                d.re -= t1 - t2;
+                d.im -= t3 - t1 - t2;
+                if (gr < n && gc < n) A[gc * n + gr] = d;
             }
 }
 
@@ -248,87 +278,131 @@ __global__ void k_lu_permute(const cd *src, cd *dst, int64_t n, int cols, const 
         dst[col * n + p] = src[col * n + i];
 }
 
-// Triangular solve with the diagonal block at k0, one wave per vector column, lane r
-// holding row r (FP64 vectors, factors in T).  which = 0: L y = b (unit lower);
-// 1: U x = y; 2: U^H z = g; 3: L^H w = z (unit upper).
+// Triangular solve with the diagonal block at k0: one workgroup, wave w solving vector
+// column w (w < cols <= 8), lane r holding row r (FP64 vectors, factors in T).  which = 0: L y = b (unit lower);
+// 1: U x = y; 2: U^H z = g; 3: L^H w = z (unit upper).  The block is staged in LDS
+// oriented so that step k reads S[k][lane] (conflict-free): S[k][r] = A(r, k) for
+// which 0 / 1 and S[k][r] = A(k, r) for which 2 / 3.
 template <class T>
-__global__ __launch_bounds__(64) void k_lu_block_solve(const cx<T> *A, int64_t n, int64_t k0, int nb, cd *x,
-                                                       int which) {
-    const int lane = threadIdx.x;
-    cd *v = x + (int64_t)blockIdx.x * n + k0;
-    cd val = lane < nb ? v[lane] : cd();
-    auto a = [&](int r, int c) {
+__global__ __launch_bounds__(512) void k_lu_block_solve(const cx<T> *A, int64_t n, int64_t k0, int nb, cd *x,
+                                                        int cols, int which) {
+    __shared__ lcx<T> S[NB][NB];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool trans = which >= 2;
+#pragma unroll 8
+    for (int t = threadIdx.x; t < nb * nb; t += 512) {
+        const int c = t / nb, r = t - c * nb;  // coalesced over r
         const cx<T> e = A[(k0 + c) * n + k0 + r];
+        if (!trans)
+            S[c][r].set(e);
+        else
+            S[r][c].set(cx<T>(e.re, -e.im));  // conj(A(r, c)) at S[r][c]: (A^H)(c, r)
+    }
+    __syncthreads();
+    if (wave >= cols) return;
+    cd *v = x + (int64_t)wave * n + k0;
+    cd val = lane < nb ? v[lane] : cd();
+    auto s = [&](int k) {
+        const cx<T> e = S[k][lane].get();
+        return cd((double)e.re, (double)e.im);
+    };
+    auto diag = [&](int k) {
+        const cx<T> e = S[k][k].get();
         return cd((double)e.re, (double)e.im);
     };
     auto bcast = [&](int k) { return cd(__shfl(val.re, k), __shfl(val.im, k)); };
-    if (which == 0) {
-        for (int k = 0; k < nb - 1; ++k) {
-            const cd vk = bcast(k);
-            if (lane > k && lane < nb) val -= a(lane, k) * vk;
-        }
-    } else if (which == 1) {
-        for (int k = nb - 1; k >= 0; --k) {
-            if (lane == k) val = val / a(k, k);
-            const cd vk = bcast(k);
-            if (lane < k) val -= a(lane, k) * vk;
-        }
-    } else if (which == 2) {
-        // U^H is lower triangular: (U^H)[r][k] = conj(U[k][r])
+    if (which == 0 || which == 2) {
+        // lower triangular (L, or U^H with S[k][r] = conj(U(k, r)))
         for (int k = 0; k < nb; ++k) {
-            if (lane == k) val = val / conj(a(k, k));
+            if (which == 2 && lane == k) val = val / diag(k);
             const cd vk = bcast(k);
-            if (lane > k && lane < nb) val -= conj(a(k, lane)) * vk;
+            if (lane > k && lane < nb) val -= s(k) * vk;
         }
     } else {
-        for (int k = nb - 1; k > 0; --k) {
+        // upper triangular (U, or L^H with S[k][r] = conj(L(k, r)))
+        for (int k = nb - 1; k >= 0; --k) {
+            if (which == 1 && lane == k) val = val / diag(k);
             const cd vk = bcast(k);
-            if (lane < k) val -= conj(a(k, lane)) * vk;
+            if (lane < k) val -= s(k) * vk;
         }
     }
     if (lane < nb) v[lane] = val;
 }
 
-// Off-diagonal part of the solves, one thread per row and up to 8 vector columns:
+// Off-diagonal part of the solves for up to 8 vector columns:
 // which = 0: x[r] -= sum_k A[r, k0 + k] x[k0 + k] for r >= k0 + nb (L21);
 // which = 1: x[r] -= sum_k A[r, k0 + k] x[k0 + k] for r < k0 (U01);
-// which = 2: x[c] -= sum_k conj(A[k0 + k, c]) x[k0 + k] for c >= k0 + nb (U12^H);
-// which = 3: x[c] -= sum_k conj(A[k0 + k, c]) x[k0 + k] for c < k0 (L10^H).
+// one thread per row, 16 loads in flight, coalesced along the panel's columns.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_panel_solve(const cx<T> *A, int64_t n, int64_t k0, int nb, cd *x,
-                                                        int cols, int which) {
+__global__ __launch_bounds__(256) void k_lu_panel_rows(const cx<T> *A, int64_t n, int64_t k0, int nb, cd *x, int cols,
+                                                       int which) {
     __shared__ lcx<double> seg[8][NB];
-    const bool below = (which == 0 || which == 2);
-    const int64_t rcount = below ? n - k0 - nb : k0;
-    for (int t = threadIdx.x; t < cols * nb; t += blockDim.x)
-        seg[t / nb][t % nb].set(x[(int64_t)(t / nb) * n + k0 + t % nb]);
+    const int64_t rcount = which == 0 ? n - k0 - nb : k0;
+    // entries beyond nb are zero, so the loads below need no bounds branch
+    for (int t = threadIdx.x; t < cols * NB; t += blockDim.x) {
+        const int q = t / NB, k = t % NB;
+        seg[q][k].set(k < nb ? x[(int64_t)q * n + k0 + k] : cd(0.0));
+    }
     __syncthreads();
     const int64_t rr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (rr >= rcount) return;
-    const int64_t r = below ? k0 + nb + rr : rr;
+    const int64_t r = which == 0 ? k0 + nb + rr : rr;
     cd s[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) s[q] = cd(0.0);
-    if (which < 2) {
-        for (int k = 0; k < nb; ++k) {
-            const cx<T> e = A[(k0 + k) * n + r];
-            const cd ev((double)e.re, (double)e.im);
+    for (int k = 0; k < nb; k += 16) {
+        cx<T> e[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) e[u] = A[(k0 + min(k + u, nb - 1)) * n + r];  // clamped: seg is 0 there
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const cd ev((double)e[u].re, (double)e[u].im);
 #pragma unroll
             for (int q = 0; q < 8; ++q)
-                if (q < cols) s[q] += ev * seg[q][k].get();
-        }
-    } else {
-        const cx<T> *c = A + r * n + k0;
-        for (int k = 0; k < nb; ++k) {
-            const cd ev((double)c[k].re, -(double)c[k].im);
-#pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (q < cols) s[q] += ev * seg[q][k].get();
+                if (q < cols) s[q] += ev * seg[q][k + u].get();
         }
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q)
         if (q < cols) x[(int64_t)q * n + r] -= s[q];
+}
+
+// which = 2: x[c] -= sum_k conj(A[k0 + k, c]) x[k0 + k] for c >= k0 + nb (U12^H);
+// which = 3: x[c] -= sum_k conj(A[k0 + k, c]) x[k0 + k] for c < k0 (L10^H);
+// one wave per column c: lane k reads A[k0 + k, c] (a contiguous segment) and the
+// wave reduces the products with shuffles.
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_panel_cols(const cx<T> *A, int64_t n, int64_t k0, int nb, cd *x, int cols,
+                                                       int which) {
+    __shared__ lcx<double> seg[8][NB];
+    const int64_t ccount = which == 2 ? n - k0 - nb : k0;
+    for (int t = threadIdx.x; t < cols * nb; t += blockDim.x)
+        seg[t / nb][t % nb].set(x[(int64_t)(t / nb) * n + k0 + t % nb]);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t cc = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); cc < ccount; cc += waves) {
+        const int64_t c = which == 2 ? k0 + nb + cc : cc;
+        cx<T> e = lane < nb ? A[c * n + k0 + lane] : cx<T>();
+        const cd ev((double)e.re, -(double)e.im);
+        cd s[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[q] = q < cols ? ev * seg[q][lane < nb ? lane : 0].get() : cd(0.0);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                s[q].re += __shfl_xor(s[q].re, off);
+                s[q].im += __shfl_xor(s[q].im, off);
+            }
+        if (lane < cols) {
+            cd sq = s[0];
+#pragma unroll
+            for (int q = 1; q < 8; ++q)
+                if (q == lane) sq = s[q];
+            x[(int64_t)lane * n + c] -= sq;
+        }
+    }
 }
 
 template <class T>
@@ -383,6 +457,8 @@ static int lu_solve_t(tg_factor *f, cd *x, int cols, int adjoint) {
     const int *perm = f->perm.as<int>();
     cd *work = f->work.as<cd>();
     auto grid = [](int64_t w) { return (unsigned)std::max<int64_t>(1, (w + 255) / 256); };
+    // one wave per column, at most 2048 workgroups of 4 waves (grid-stride beyond)
+    auto wgrid = [](int64_t w) { return (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (w + 3) / 4)); };
     Timer::Rec rec;
     TG_TRY(c->timer.begin("lu_solve", s, rec));
     const int64_t last = ((n - 1) / NB) * NB;
@@ -392,25 +468,25 @@ static int lu_solve_t(tg_factor *f, cd *x, int cols, int adjoint) {
         TG_HIP(hipMemcpyAsync(x, work, sizeof(cd) * n * cols, hipMemcpyDeviceToDevice, s));
         for (int64_t k0 = 0; k0 < n; k0 += NB) {
             const int nb = (int)std::min<int64_t>(NB, n - k0);
-            k_lu_block_solve<T><<<cols, 64, 0, s>>>(A, n, k0, nb, x, 0);
-            if (n - k0 - nb > 0) k_lu_panel_solve<T><<<grid(n - k0 - nb), 256, 0, s>>>(A, n, k0, nb, x, cols, 0);
+            k_lu_block_solve<T><<<1, 512, 0, s>>>(A, n, k0, nb, x, cols, 0);
+            if (n - k0 - nb > 0) k_lu_panel_rows<T><<<grid(n - k0 - nb), 256, 0, s>>>(A, n, k0, nb, x, cols, 0);
         }
         for (int64_t k0 = last; k0 >= 0; k0 -= NB) {
             const int nb = (int)std::min<int64_t>(NB, n - k0);
-            k_lu_block_solve<T><<<cols, 64, 0, s>>>(A, n, k0, nb, x, 1);
-            if (k0 > 0) k_lu_panel_solve<T><<<grid(k0), 256, 0, s>>>(A, n, k0, nb, x, cols, 1);
+            k_lu_block_solve<T><<<1, 512, 0, s>>>(A, n, k0, nb, x, cols, 1);
+            if (k0 > 0) k_lu_panel_rows<T><<<grid(k0), 256, 0, s>>>(A, n, k0, nb, x, cols, 1);
         }
     } else {
         // A^H = U^H L^H P: U^H z = g, L^H w = z, x = P^T w
         for (int64_t k0 = 0; k0 < n; k0 += NB) {
             const int nb = (int)std::min<int64_t>(NB, n - k0);
-            k_lu_block_solve<T><<<cols, 64, 0, s>>>(A, n, k0, nb, x, 2);
-            if (n - k0 - nb > 0) k_lu_panel_solve<T><<<grid(n - k0 - nb), 256, 0, s>>>(A, n, k0, nb, x, cols, 2);
+            k_lu_block_solve<T><<<1, 512, 0, s>>>(A, n, k0, nb, x, cols, 2);
+            if (n - k0 - nb > 0) k_lu_panel_cols<T><<<wgrid(n - k0 - nb), 256, 0, s>>>(A, n, k0, nb, x, cols, 2);
         }
         for (int64_t k0 = last; k0 >= 0; k0 -= NB) {
             const int nb = (int)std::min<int64_t>(NB, n - k0);
-            k_lu_block_solve<T><<<cols, 64, 0, s>>>(A, n, k0, nb, x, 3);
-            if (k0 > 0) k_lu_panel_solve<T><<<grid(k0), 256, 0, s>>>(A, n, k0, nb, x, cols, 3);
+            k_lu_block_solve<T><<<1, 512, 0, s>>>(A, n, k0, nb, x, cols, 3);
+            if (k0 > 0) k_lu_panel_cols<T><<<wgrid(k0), 256, 0, s>>>(A, n, k0, nb, x, cols, 3);
         }
         k_lu_permute<<<grid(n * cols), 256, 0, s>>>(x, work, n, cols, perm, 1);
         TG_HIP(hipMemcpyAsync(x, work, sizeof(cd) * n * cols, hipMemcpyDeviceToDevice, s));

@@ -38,6 +38,7 @@ class MLP:
         check(lib().tg_mlp_create(n_in, width, n_out, hidden_layers, seed, ctypes.c_void_p(_stream_ptr()),
                                   ctypes.byref(h)))
         self._h = h
+        self._destroy = lib().tg_mlp_destroy  # bound now: module globals are gone at exit
         self.dims = (n_in, width, n_out, hidden_layers)
         self.count = int(lib().tg_mlp_param_count(h))
         self.step = 0
@@ -45,7 +46,7 @@ class MLP:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().tg_mlp_destroy(h)
+            self._destroy(h)
             self._h = None
 
     def params(self) -> np.ndarray:
