@@ -10,10 +10,12 @@ true FP64 relative residual <= 1e-8.  value = solves/s = 4 K / (time of K steps)
 
 Also measured in the same run: the plain FP64 block-Jacobi GMRES(50) on the same
 GPU (stored c128 coupling), the per-kernel roofline of the stored-coupling matvec
-and of the assembly (HIP events on the library's stream), and the CPU oracle
-(matrix-free FP64 apply, OpenMP) on a bounded sample, extrapolated to a solve.
+and of the assembly (HIP events on the library's stream), the dense LU at N = 512,
+lmax = 4 (c64 with FP64 refinement and c128; MFMA TFLOP/s of the factorisation),
+the cross-section gradient at N = 256, lmax = 4 through the LU adjoint, and the CPU
+oracle (matrix-free FP64 apply, OpenMP) on a bounded sample, extrapolated to a solve.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-lu] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -76,29 +78,92 @@ def run_method(cl, B, torch, *, mixed, pc, steps, warmup, rtol, restart, label):
     return x, conv, elapsed, kernels
 
 
-def cpu_baseline(lmax, k0, pos, iterations, sample, oracle):
-    """Oracle matrix-free FP64 apply (OpenMP) on `sample` destination spheres x all
-    sources, 4 columns; extrapolated to a full apply and to the solve's iterations."""
+def cpu_baseline(lmax, k0, pos, iterations, sample, reps, oracle):
+    """Oracle matrix-free FP64 apply (OpenMP) on `sample` destination spheres (all
+    when 0) x all sources, 4 columns, `reps` times; extrapolated to a full apply and
+    to the solve's operator applications."""
     N = pos.shape[0]
     n = N * 2 * lmax * (lmax + 2)
+    sample = N if sample <= 0 else min(sample, N)
     rng = np.random.default_rng(0)
     x = rng.normal(size=(n, 4)) + 1j * rng.normal(size=(n, 4))
     oracle.coupling_apply(lmax, k0, pos, x, rows=(0, 1))  # warm the plan
     t0 = time.perf_counter()
-    oracle.coupling_apply(lmax, k0, pos, x, rows=(0, sample))
+    for _ in range(reps):
+        oracle.coupling_apply(lmax, k0, pos, x, rows=(0, sample))
     dt = time.perf_counter() - t0
-    t_apply = dt * N / sample
+    t_apply = dt / reps * N / sample
     t_solve = t_apply * (iterations + 2)  # Arnoldi applications plus true residuals
     return {
         "value": 4.0 / t_solve,
         "unit": "solves/s",
         "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
         "kind": "port",
-        "sample": (f"oracle matrix-free FP64 coupling apply (cluster/iterative.rs:187-221 restated, OpenMP) on "
-                   f"{sample} of {N} destination spheres x {N - 1} sources x 4 columns: {dt:.2f} s; "
-                   f"full apply extrapolated to {t_apply:.1f} s; one 4-column solve = {iterations + 2} applies "
+        "sample": (f"oracle matrix-free FP64 coupling apply (cluster/iterative.rs:187-221 restated, OpenMP), "
+                   f"{reps} x ({sample} of {N} destination spheres x {N - 1} sources x 4 columns): {dt:.2f} s; "
+                   f"one full apply {t_apply:.2f} s; one 4-column solve = {iterations + 2} applies "
                    f"(iterations of unpreconditioned FP64 GMRES measured on the GPU)"),
     }
+
+
+def dense_lu(pkg, torch, seed):
+    """Dense LU at N = 512, lmax = 4 (24,576 unknowns), 4 plane waves: c64 factors on
+    MFMA with FP64 refinement against c128 factors, both on this GPU."""
+    lmax, radii, eps, pos = pkg.workload("n512_l4", seed=seed)
+    cl = pkg.IterativeSphereCluster(lmax, 1.3, radii, eps, pos)
+    B = rhs_columns(cl, torch)
+    n = cl.n
+    flops = 8.0 / 3.0 * n ** 3
+    out = {"workload": f"N={cl.N}, lmax={lmax} ({n} unknowns), 4 plane waves, rtol 1e-12",
+           "algorithmic_flops": flops}
+    xs = {}
+    for prec, peak in (("c64", 157.3), ("c128", 78.6)):
+        cl.timing_reset()
+        cl.timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f = pkg.SphereClusterFactor(cl, prec)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        x, conv = f.solve(B)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        cl.timing(False)
+        lu_ms = cl.timing_get("lu_factor")[0]
+        asm_ms = cl.timing_get("assemble_operator")[0]
+        tf = flops / (lu_ms * 1e-3) / 1e12
+        out[prec] = {"factor_ms": 1e3 * (t1 - t0), "assemble_operator_ms": asm_ms, "lu_kernels_ms": lu_ms,
+                     "tflops": tf, "peak_tflops": peak, "frac": tf / peak,
+                     "solve_ms_4rhs": 1e3 * (t2 - t1), "refinements": [c.refinements for c in conv],
+                     "true_relative_residual": [c.residual_norm / c.rhs_norm for c in conv]}
+        xs[prec] = x
+        log(f"dense LU {prec}: factor {1e3 * (t1 - t0):.0f} ms ({tf:.1f} TFLOP/s), solve {1e3 * (t2 - t1):.0f} ms")
+        f.close()
+    out["c64_vs_c128_solution_rel_diff"] = float(torch.linalg.norm(xs["c64"] - xs["c128"]) /
+                                                 torch.linalg.norm(xs["c128"]))
+    return out
+
+
+def gradient_lu(pkg, torch, seed):
+    """sigma_sca of a +z positive-helicity plane wave and its gradient with respect to
+    all 768 position components and 256 radii (N = 256, lmax = 4, 12,288 unknowns),
+    the adjoint reusing the c128 LU factors."""
+    lmax, radii, eps, pos = pkg.workload("n256_l4_grad", seed=seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f = pkg.sphere_cluster_factor(lmax, 1.3, radii, eps, pos)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sca, gr, gp, info = pkg.scattering_gradient(f, [0, 0, 1], 1)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    f.close()
+    # translation invariance: the position gradient sums to zero
+    drift = float(np.abs(gp.sum(axis=0)).max() / np.abs(gp).max())
+    log(f"gradient (LU): factor {1e3 * (t1 - t0):.0f} ms, solve + adjoint + pullback {1e3 * (t2 - t1):.0f} ms")
+    return {"workload": f"N={radii.size}, lmax={lmax} ({f.cluster.n} unknowns), c128 LU",
+            "factor_ms": 1e3 * (t1 - t0), "solve_adjoint_pullback_ms": 1e3 * (t2 - t1),
+            "scattering_cross_section": sca, "position_gradient_sum_rel": drift}
 
 
 def main() -> int:
@@ -116,7 +181,9 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-reference", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=24)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="destination spheres per CPU apply (0: all)")
+    ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--no-lu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only "
@@ -223,12 +290,16 @@ def main() -> int:
                                              "iterations": [c.iterations for c in convr],
                                              "note": "unpreconditioned FP64 matrix-free GMRES, as the reference"}
         log(f"reference algorithm on GPU: {elr:.2f} s, iterations {[c.iterations for c in convr]}")
+    if not args.no_lu:
+        del cl
+        result["dense_lu"] = dense_lu(pkg, torch, args.seed)
+        result["gradient_lu"] = gradient_lu(pkg, torch, args.seed)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
 
         its = ref_iterations if ref_iterations is not None else max(iterations)
-        result["cpu_baseline"] = cpu_baseline(lmax, k0, pos, its, args.cpu_sample, oracle)
+        result["cpu_baseline"] = cpu_baseline(lmax, k0, pos, its, args.cpu_sample, args.cpu_reps, oracle)
         log(f"cpu baseline: {result['cpu_baseline']['value']:.4g} solves/s")
     print(json.dumps(result), flush=True)
     return 0
