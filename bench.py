@@ -184,6 +184,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=0, help="destination spheres per CPU apply (0: all)")
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--no-lu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="bytes of a plain device copy run first (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only "
@@ -196,6 +198,16 @@ def main() -> int:
 
     pkg = load_package()
     torch.cuda.set_device(0)
+    if args.copy_calibration > 0:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch, divided
+        # by the size, is the counter scale applied to the workload's kernels
+        src = torch.empty(args.copy_calibration // 4, dtype=torch.float32, device="cuda").uniform_()
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {src.numel() * 4} bytes x 3")
+        del src, dst
     lmax, radii, eps, pos = pkg.workload(args.workload, seed=args.seed)
     k0 = 1.3
     N = radii.size

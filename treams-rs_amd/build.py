@@ -10,6 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtreams_gpu.so")
+DRIVER = os.path.join(HERE, "cluster_driver")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -49,6 +50,12 @@ def build(verbose: bool = False) -> str:
     if verbose:
         print(" ".join(link), flush=True)
     subprocess.check_call(link)
+    # C++ host driver over the C ABI (the call sequence of a Rust host)
+    drv = ["hipcc", "-O2", "-std=c++17", os.path.join(HERE, "host", "cluster_driver.cpp"), "-o", DRIVER,
+           "-L" + HERE, "-ltreams_gpu", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(drv), flush=True)
+    subprocess.check_call(drv)
     return OUT
 
 
