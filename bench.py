@@ -238,6 +238,15 @@ def main() -> int:
         roof = {"bound": "hbm", "kernel": "k_matvec_stored<float2> (c64 helicity-split coupling, 4 RHS)",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                 "traffic": None, "bytes_per_launch": c64_bytes, "avg_ms": mv["avg_ms"]}
+        # HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE counter runs of this
+        # workload, scaled by a plain copy of known size (the summary under profiles/)
+        tpath = os.path.join(ROOT, "profiles", "matvec_traffic_n1024_l6.json")
+        if args.workload == "n1024_l6" and os.path.exists(tpath):
+            with open(tpath) as fh:
+                t = json.load(fh)
+            roof["traffic"] = t["traffic_bytes"]
+            roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy "
+                                      "(profiles/matvec_assembly_n1024_l6.md)")
     asm = kernels.get("assemble", {})
     asm_roof = None
     if asm:
