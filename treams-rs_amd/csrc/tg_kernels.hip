@@ -694,74 +694,88 @@ template <class S> struct vec16;
 template <> struct vec16<float2> { using T = f32x4; static constexpr int R = 2; };   // two c64 rows
 template <> struct vec16<double2> { using T = f64x2; static constexpr int R = 1; };  // one c128 row
 
+// x packed for the matvec: xp[h][k][NC] in the accumulation precision, zero beyond
+// `cols`, so each workgroup copies its chunk with contiguous 16-byte loads.
+template <class AV, int NC>
+__global__ void k_pack_x(const cd *x, int64_t n, int64_t nh, int cols, AV *xp) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * nh * NC) return;
+    const int cc = (int)(t % NC);
+    const int64_t hk = t / NC;  // h * nh + k
+    AV v;
+    if (cc < cols) {
+        const cd xv = x[(int64_t)cc * n + hk];
+        v.x = xv.re;
+        v.y = xv.im;
+    } else {
+        v.x = 0;
+        v.y = 0;
+    }
+    xp[t] = v;
+}
+
 // y_h = C_h x_h for the stored split coupling, both helicities (grid.z), split over
-// column chunks (grid.y).  Each thread owns two consecutive rows and streams the
-// chunk's columns with 16/32-byte loads; x of the chunk sits in LDS in the
-// accumulation precision.  Partial sums [kc][col][h][nh] are added in chunk order.
+// column chunks (grid.y).  Each thread owns R consecutive rows and streams the
+// chunk's columns with one 16-byte non-temporal load per column, issued in batches
+// of 8 so that eight loads are in flight per wave.  The packed x is read with
+// wave-uniform addresses (scalar loads), so the kernel needs no LDS and runs at full
+// occupancy.  Partial sums [kc][col][h][nh] are added in chunk order.
 template <class S, int NC>
-__global__ __launch_bounds__(256) void k_matvec_stored(const S *C, int64_t nh, const cd *x, int64_t n, int cols,
-                                                        int kchunk, typename acc_of<S>::V *part) {
+__global__ __launch_bounds__(256) void k_matvec_stored(const S *C, int64_t nh,
+                                                        const typename acc_of<S>::V *__restrict__ xp, int64_t n,
+                                                        int cols, int kchunk, typename acc_of<S>::V *part) {
     using A = typename acc_of<S>::T;
     using AV = typename acc_of<S>::V;
-    extern __shared__ unsigned char smraw[];
-    AV *xs = reinterpret_cast<AV *>(smraw);  // [kchunk][NC]
+    constexpr int R = vec16<S>::R;
+    constexpr int B = 8;
+    using V16 = typename vec16<S>::T;
     const int h = blockIdx.z;
     const S *Ch = C + (int64_t)h * nh * nh;
     const int64_t c0 = (int64_t)blockIdx.y * kchunk;
     const int64_t c1 = min(nh, c0 + kchunk);
     const int kc = (int)(c1 - c0);
-    for (int t = threadIdx.x; t < kc * NC; t += blockDim.x) {
-        const int cc = t % NC, k = t / NC;
-        AV v;
-        if (cc < cols) {
-            cd xv = x[(int64_t)cc * n + (int64_t)h * nh + c0 + k];
-            v.x = (A)xv.re;
-            v.y = (A)xv.im;
-        } else {
-            v.x = 0;
-            v.y = 0;
-        }
-        xs[t] = v;
-    }
-    __syncthreads();
-    // R consecutive rows per thread, read as one 16-byte non-temporal load per column
-    constexpr int R = vec16<S>::R;
-    using V16 = typename vec16<S>::T;
+    const AV *xs = xp + ((int64_t)h * nh + c0) * NC;
     const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * R;
     if (r0 >= nh) return;
-    const bool whole = (nh % R == 0);
     A acc[NC][R][2];
 #pragma unroll
     for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
         for (int r = 0; r < R; ++r) acc[cc][r][0] = acc[cc][r][1] = 0;
-    if (whole) {
-        const V16 *p = reinterpret_cast<const V16 *>(Ch + c0 * nh + r0);
-        const int64_t stride = nh / R;
-#pragma unroll 8
-        for (int k = 0; k < kc; ++k) {
-            V16 v = __builtin_nontemporal_load(p + (int64_t)k * stride);
+    auto mac = [&](const V16 &v, int k) {
 #pragma unroll
-            for (int cc = 0; cc < NC; ++cc) {
-                AV xv = xs[k * NC + cc];
+        for (int cc = 0; cc < NC; ++cc) {
+            const AV xv = xs[k * NC + cc];
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const A re = (A)v[2 * r], im = (A)v[2 * r + 1];
-                    acc[cc][r][0] += re * xv.x - im * xv.y;
-                    acc[cc][r][1] += re * xv.y + im * xv.x;
-                }
+            for (int r = 0; r < R; ++r) {
+                const A re = (A)v[2 * r], im = (A)v[2 * r + 1];
+                acc[cc][r][0] += re * xv.x - im * xv.y;
+                acc[cc][r][1] += re * xv.y + im * xv.x;
             }
         }
+    };
+    if (nh % R == 0) {
+        const V16 *p = reinterpret_cast<const V16 *>(Ch + c0 * nh + r0);
+        const int64_t stride = nh / R;
+        int k = 0;
+        for (; k + B <= kc; k += B) {
+            V16 v[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) v[u] = __builtin_nontemporal_load(p + (int64_t)(k + u) * stride);
+#pragma unroll
+            for (int u = 0; u < B; ++u) mac(v[u], k + u);
+        }
+        for (; k < kc; ++k) mac(__builtin_nontemporal_load(p + (int64_t)k * stride), k);
     } else {
         const S *p = Ch + c0 * nh + r0;
         for (int k = 0; k < kc; ++k)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if (r0 + r >= nh) break;
-                S v0 = p[(int64_t)k * nh + r];
+                const S v0 = p[(int64_t)k * nh + r];
 #pragma unroll
                 for (int cc = 0; cc < NC; ++cc) {
-                    AV xv = xs[k * NC + cc];
+                    const AV xv = xs[k * NC + cc];
                     acc[cc][r][0] += (A)v0.x * xv.x - (A)v0.y * xv.y;
                     acc[cc][r][1] += (A)v0.x * xv.y + (A)v0.y * xv.x;
                 }
@@ -786,11 +800,13 @@ template <class S>
 static int matvec_stored_t(tg_cluster *c, const cd *x, cd *y, int cols) {
     using AV = typename acc_of<S>::V;
     const int64_t nh = c->nh;
-    // x tile of kchunk columns x 8 RHS in LDS: at most 64 KB
-    const int kchunk = sizeof(AV) == 8 ? 1024 : 512;
+    // column chunk per workgroup: long enough to amortise the partial sums, short
+    // enough to give every CU several workgroups (measured on config-5 shapes)
+    const int kchunk = sizeof(AV) == 8 ? 4096 : 2048;
     const int nk = (int)((nh + kchunk - 1) / kchunk);
-    TG_TRY(c->ws[1].ensure(sizeof(AV) * (size_t)nk * 8 * c->n));
+    TG_TRY(c->ws[1].ensure(sizeof(AV) * (size_t)nk * 8 * c->n + sizeof(AV) * 8 * (size_t)c->n));
     AV *part = c->ws[1].as<AV>();
+    AV *xp = part + (size_t)nk * 8 * c->n;  // packed x: [2][nh][NC]
     const S *C = c->b_C.as<S>();
     Timer::Rec rec;
     TG_TRY(c->timer.begin("matvec_stored", c->stream, rec));
@@ -799,9 +815,9 @@ static int matvec_stored_t(tg_cluster *c, const cd *x, cd *y, int cols) {
         const int64_t rows_per_block = 256 * vec16<S>::R;
         dim3 grid((unsigned)((nh + rows_per_block - 1) / rows_per_block), nk, 2);
         const cd *xx = x + (int64_t)c0 * c->n;
-#define TG_MV(NCV)                                                                                           \
-    k_matvec_stored<S, NCV><<<grid, 256, sizeof(AV) * kchunk * NCV, c->stream>>>(C, nh, xx, c->n, nc, kchunk, \
-                                                                                  part);
+#define TG_MV(NCV)                                                                                            \
+    k_pack_x<AV, NCV><<<grid1(2 * nh * NCV), 256, 0, c->stream>>>(xx, c->n, nh, nc, xp);                   \
+    k_matvec_stored<S, NCV><<<grid, 256, 0, c->stream>>>(C, nh, xp, c->n, nc, kchunk, part);
         if (nc == 1) { TG_MV(1) } else if (nc == 2) { TG_MV(2) } else if (nc <= 4) { TG_MV(4) } else { TG_MV(8) }
 #undef TG_MV
         TG_HIP(hipGetLastError());

@@ -58,6 +58,10 @@ int main(int argc, char **argv) {
         pos.insert(pos.end(), {x, y, z});
     }
     const int N = (int)radii.size();
+    if (N == 0) {
+        std::fprintf(stderr, "no spheres on stdin (radius eps_re eps_im x y z per line)\n");
+        return 2;
+    }
     tg_cluster *c = nullptr;
     CHECK(tg_cluster_create(lmax, k0, N, radii.data(), eps.data(), pos.data(), nullptr, &c));
     const int64_t n = tg_cluster_dimension(c);
