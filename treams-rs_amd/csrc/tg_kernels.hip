@@ -7,6 +7,7 @@
 // column (j H + a), destination row (i H + b), with (l, m) index a = l^2 + l + m - 1.
 #include <algorithm>
 #include "tg_internal.h"
+#include "tg_mfma.h"
 #include "tg_special.h"
 
 namespace tg {
@@ -42,7 +43,8 @@ __device__ void radial_values(int L2, double kr, bool regular, cd *rad) {
 // unit vector u: diagonal ladder P_q^q = (-1)^q (2q-1)!! (x + i y)^q, then the degree
 // recurrence (p - q) S_p = (2p - 1) z S_{p-1} - (p + q - 1) S_{p-2}; negative orders by
 // S_p^{-q} = (-1)^q (p-q)!/(p+q)! conj(S_p^q).
-__device__ void table_column(int q, int L2, const double u[3], const cd *rad, const double *refl, cd *tab) {
+__device__ void table_column(int q, int L2, const double u[3], const cd *rad, const double *refl, cd *tab,
+                             int64_t stride = 1) {
     cd xy(u[0], u[1]);
     cd cur(1.0);
     for (int k = 1; k <= q; ++k) cur = (-(double)(2 * k - 1)) * (cur * xy);
@@ -55,15 +57,15 @@ __device__ void table_column(int q, int L2, const double u[3], const cd *rad, co
             prev = cur;
             cur = next;
         }
-        tab[p * p + p + q] = rad[p] * cur;
-        if (q > 0) tab[p * p + p - q] = rad[p] * (refl[p * p + p - q] * conj(cur));
+        tab[(int64_t)(p * p + p + q) * stride] = rad[p] * cur;
+        if (q > 0) tab[(int64_t)(p * p + p - q) * stride] = rad[p] * (refl[p * p + p - q] * conj(cur));
     }
 }
 
-__device__ __forceinline__ void zero_column(int q, int L2, cd *tab) {
+__device__ __forceinline__ void zero_column(int q, int L2, cd *tab, int64_t stride = 1) {
     for (int p = q; p <= L2; ++p) {
-        tab[p * p + p + q] = cd(0.0);
-        if (q > 0) tab[p * p + p - q] = cd(0.0);
+        tab[(int64_t)(p * p + p + q) * stride] = cd(0.0);
+        if (q > 0) tab[(int64_t)(p * p + p - q) * stride] = cd(0.0);
     }
 }
 
@@ -443,192 +445,133 @@ __global__ void k_sum_chunks(const PT *part, int nchunks, int64_t count, cd *out
 static int grid1(int64_t count, int block = 256) { return (int)((count + block - 1) / block); }
 
 // ------------------------------------------- matrix-free coupling with pair tables --
+//
+// The FP64 coupling is the plan contraction C_ij[b, a] = sum_t w_t tab_ij[h_t] (same
+// and cross kinds, combined per helicity), so
+//   y_i[b] = sum_j sum_a C_ij[b, a] x_j[a] = sum_{a, t} w_t U_{h_t}[a][i],
+//   U_h[q][i] = sum_j TAB_h[i][j] X[q][j],
+// with TAB_h[i][j] = tab_ij[h] the singular harmonic table of the pair (zero for
+// i = j) and q = (column, helicity, source mode a).  U is one complex FP64 GEMM per
+// harmonic index h (M = N destinations, K = N sources, N = 2 H columns per right-hand
+// side) on MFMA, and the plan contraction that follows reads U once per term.
 
-// Index of the unordered pair (i < j) in row-major upper-triangle order.
-__device__ __forceinline__ int64_t pair_index(int i, int j, int N) {
-    return (int64_t)i * N - (int64_t)i * (i + 1) / 2 + (j - i - 1);
-}
-
-// Singular harmonic tables h_p(k r) P_p^q e^{i q phi} of every pair i < j at
-// d = pos_i - pos_j, one thread per (pair, q >= 0).  The pair (j, i) uses the same
-// table times (-1)^p, since the solid harmonic of degree p is odd or even in d.
-__global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, int64_t npairs, cd *tables) {
+// TAB[h][j][i] = h_p(k r) P_p^q e^{i q phi} at d = pos_i - pos_j, for every ordered
+// pair; one thread per (i, j) with i fastest, so each table entry is a coalesced
+// store across the wave.  The diagonal (i = j) is zero.
+__global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd *tab) {
     const int L2 = pl.L2;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npairs * (L2 + 1)) return;
-    const int64_t pidx = t / (L2 + 1);
-    const int q = (int)(t - pidx * (L2 + 1));
-    // invert pair_index: row i with start(i) <= pidx < start(i + 1)
-    int lo = 0, hi = N - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) / 2;
-        if ((int64_t)mid * N - (int64_t)mid * (mid + 1) / 2 <= pidx) lo = mid; else hi = mid - 1;
-    }
-    const int i = lo;
-    const int j = (int)(pidx - ((int64_t)i * N - (int64_t)i * (i + 1) / 2)) + i + 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * N) return;
+    const int j = (int)(t / N), i = (int)(t - (int64_t)j * N);
+    const int64_t NN = (int64_t)N * N;
+    cd *out = tab + (int64_t)j * N + i;
     double d[3], r, u[3];
     for (int a = 0; a < 3; ++a) d[a] = pos[3 * i + a] - pos[3 * j + a];
-    cd *tab = tables + pidx * pl.T;
-    if (!unit_direction(d, r, u)) {
-        zero_column(q, L2, tab);
+    if (i == j || !unit_direction(d, r, u)) {
+        for (int q = 0; q <= L2; ++q) zero_column(q, L2, out, NN);
         return;
     }
     cd rad[2 * kMaxLmax + 2];
     tg_hankel_seq(L2, k * r, rad);
-    table_column(q, L2, u, rad, pl.refl, tab);
+    for (int q = 0; q <= L2; ++q) table_column(q, L2, u, rad, pl.refl, out, NN);
 }
 
 int ensure_pair_tables(tg_cluster *c) {
     if (c->have_tables) return TG_OK;
-    const int64_t npairs = (int64_t)c->N * (c->N - 1) / 2;
-    TG_TRY(c->b_tab.ensure(sizeof(cd) * (size_t)std::max<int64_t>(npairs, 1) * c->plan.T));
-    const int64_t work = npairs * (c->plan.L2 + 1);
-    if (work > 0) {
-        k_pair_tables<<<grid1(work), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N, npairs,
-                                                          c->b_tab.as<cd>());
-        TG_HIP(hipGetLastError());
-    }
+    const int64_t NN = (int64_t)c->N * c->N;
+    TG_TRY(c->b_tab.ensure(sizeof(cd) * (size_t)NN * c->plan.T));
+    k_pair_tables<<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N, c->b_tab.as<cd>());
+    TG_HIP(hipGetLastError());
     c->have_tables = true;
     return TG_OK;
 }
 
-// y_i = sum_{j != i} C_ij x_j in FP64 from the stored pair tables.  Workgroup
-// (destination i, chunk of sources); thread (g, b) owns destination row b and the
-// source columns a = g (mod G).  Sources are taken PS at a time so every plan term a
-// thread loads serves PS pairs; the PS tables (signed for j < i) and source vectors
-// sit in LDS.  Partial sums per chunk go to part [chunk][col][h][nh].
-template <int NC, int PS>
-__global__ __launch_bounds__(256) void k_coupling_tab(DevPlan pl, const cd *tables, int N, int cols, const cd *x,
-                                                       int64_t n, int64_t nh, int chunk, cd *part) {
-    extern __shared__ cd sm[];
-    const int T = pl.T, H = pl.H;
-    const int G = blockDim.x / H;
-    cd *tab = sm;              // [PS][T]
-    cd *xs = tab + PS * T;     // [PS][cols][2][H]
-    __shared__ int src[PS];
-    const int tid = threadIdx.x;
-    const int g = tid / H, b = tid - g * H;
-    const int i = blockIdx.x;
-    const int j0 = blockIdx.y * chunk, j1 = min(N, j0 + chunk);
-    cd acc[NC][2];
+// X[q][j] with q = (column cc, helicity h, mode a): the GEMM's right operand.
+__global__ void k_pack_sources(const cd *x, int64_t n, int64_t nh, int N, int H, int cols, cd *X) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t Q = (int64_t)cols * 2 * H;
+    if (t >= Q * N) return;
+    const int64_t q = t / N;
+    const int j = (int)(t - q * N);
+    const int cc = (int)(q / (2 * H)), rem = (int)(q - (int64_t)cc * 2 * H), h = rem / H, a = rem - h * H;
+    X[t] = x[(int64_t)cc * n + (int64_t)h * nh + (int64_t)j * H + a];
+}
+
+// C[z] = A[z] B[z] for column-major complex FP64 matrices (M x K times K x N),
+// one 64 x 64 output tile per workgroup, batch z = grid.z with element strides sA,
+// sB, sC.  3M form on v_mfma_f64_16x16x4f64.
+template <bool KTAIL>
+__global__ __launch_bounds__(256) void k_zgemm(const cd *A, int64_t lda, int64_t sA, const cd *B, int64_t ldb,
+                                               int64_t sB, cd *C, int64_t ldc, int64_t sC, int64_t M, int64_t N,
+                                               int K) {
+    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    const int64_t z = blockIdx.z;
+    __shared__ TileLds<double> sm;
+    Tile3M<double> acc;
+    zgemm_tile_3m<double, KTAIL>(A + z * sA, lda, B + z * sB, ldb, r0, c0, M, N, K, sm, acc);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, rr = lane & 15;
+    cd *Cz = C + z * sC;
 #pragma unroll
-    for (int cc = 0; cc < NC; ++cc) { acc[cc][0] = cd(0.0); acc[cc][1] = cd(0.0); }
-    for (int jb = j0; jb < j1; jb += PS) {
-        __syncthreads();
-        if (tid < PS) {
-            int j = jb + tid;
-            src[tid] = (j < j1 && j != i) ? j : -1;
-        }
-        __syncthreads();
-        for (int t = tid; t < PS * T; t += blockDim.x) {
-            const int s = t / T, idx = t - s * T;
-            const int j = src[s];
-            cd v(0.0);
-            if (j >= 0) {
-                const int p = (int)sqrt((double)idx + 0.5);
-                if (i < j) {
-                    v = tables[pair_index(i, j, N) * T + idx];
-                } else {
-                    v = tables[pair_index(j, i, N) * T + idx];
-                    if (p & 1) v = -v;
-                }
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<double>(lane, v);
+                const double t1 = acc.t1[a][b][v], t2 = acc.t2[a][b][v], t3 = acc.t3[a][b][v];
+                if (gr < M && gc < N) Cz[gc * ldc + gr] = cd(t1 - t2, t3 - t1 - t2);
             }
-            tab[t] = v;
-        }
-        for (int t = tid; t < PS * cols * 2 * H; t += blockDim.x) {
-            const int s = t / (cols * 2 * H), rem = t - s * cols * 2 * H;
-            const int cc = rem / (2 * H), r2 = rem - cc * 2 * H, h = r2 / H, a = r2 - h * H;
-            const int j = src[s];
-            xs[t] = j >= 0 ? x[(int64_t)cc * n + (int64_t)h * nh + (int64_t)j * H + a] : cd(0.0);
-        }
-        __syncthreads();
-        if (tid < G * H) {
-            for (int a = g; a < H; a += G) {
-                const int e = a * H + b;
-                const int s0 = pl.starts[e], s1 = pl.starts[e + 1], ns = pl.nsame[e];
-                cd same[PS], cross[PS];
-#pragma unroll
-                for (int s = 0; s < PS; ++s) { same[s] = cd(0.0); cross[s] = cd(0.0); }
-                for (int t = s0; t < s0 + ns; ++t) {
-                    const double wt = pl.w[t];
-                    const int hh = pl.hidx[t];
-#pragma unroll
-                    for (int s = 0; s < PS; ++s) same[s] += wt * tab[s * T + hh];
-                }
-                for (int t = s0 + ns; t < s1; ++t) {
-                    const double wt = pl.w[t];
-                    const int hh = pl.hidx[t];
-#pragma unroll
-                    for (int s = 0; s < PS; ++s) cross[s] += wt * tab[s * T + hh];
-                }
-#pragma unroll
-                for (int s = 0; s < PS; ++s) {
-                    const cd cp(same[s].re - cross[s].im, same[s].im + cross[s].re);
-                    const cd cm(same[s].re + cross[s].im, same[s].im - cross[s].re);
-                    const cd *xv = xs + (s * cols) * 2 * H;
-#pragma unroll
-                    for (int cc = 0; cc < NC; ++cc)
-                        if (cc < cols) {
-                            acc[cc][0] += cp * xv[cc * 2 * H + a];
-                            acc[cc][1] += cm * xv[cc * 2 * H + H + a];
-                        }
-                }
-            }
-        }
+}
+
+// y[cc][h][i H + b] = sum_{a, t in (a, b)} w_t U[h_t][(cc, h, a)][i], same kind plus or
+// minus i times cross kind (h = 0: pol 1, +; h = 1: pol 0, -).  One thread per
+// destination sphere i (coalesced over i); the plan is read with wave-uniform
+// addresses.  grid.y = (cc, h, b).
+__global__ __launch_bounds__(64) void k_tab_contract(DevPlan pl, const cd *U, int N, int H, int cols, int64_t n,
+                                                     int64_t nh, cd *y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int yb = blockIdx.y;
+    const int cc = yb / (2 * H), rem = yb - cc * 2 * H, h = rem / H, b = rem - h * H;
+    if (i >= N) return;
+    const int64_t Q = (int64_t)cols * 2 * H;
+    const int64_t qbase = ((int64_t)cc * 2 + h) * H;
+    cd same(0.0), cross(0.0);
+    for (int a = 0; a < H; ++a) {
+        const int e = a * H + b;
+        const int s0 = pl.starts[e], s1 = pl.starts[e + 1], ns = pl.nsame[e];
+        const cd *Ua = U + (qbase + a) * N + i;
+        for (int t = s0; t < s0 + ns; ++t) same += pl.w[t] * Ua[(int64_t)pl.hidx[t] * Q * N];
+        for (int t = s0 + ns; t < s1; ++t) cross += pl.w[t] * Ua[(int64_t)pl.hidx[t] * Q * N];
     }
-    __syncthreads();
-    cd *red = sm;  // [G][cols][2][H]
-    if (tid < G * H) {
-#pragma unroll
-        for (int cc = 0; cc < NC; ++cc)
-            if (cc < cols) {
-                red[((g * cols + cc) * 2 + 0) * H + b] = acc[cc][0];
-                red[((g * cols + cc) * 2 + 1) * H + b] = acc[cc][1];
-            }
-    }
-    __syncthreads();
-    for (int t = tid; t < cols * 2 * H; t += blockDim.x) {
-        cd s(0.0);
-        for (int gg = 0; gg < G; ++gg) s += red[gg * cols * 2 * H + t];
-        const int cc = t / (2 * H), rem = t - cc * 2 * H, h = rem / H, bb = rem - h * H;
-        part[((int64_t)blockIdx.y * cols + cc) * n + (int64_t)h * nh + (int64_t)i * H + bb] = s;
-    }
+    const cd v = h == 0 ? cd(same.re - cross.im, same.im + cross.re) : cd(same.re + cross.im, same.im - cross.re);
+    y[(int64_t)cc * n + (int64_t)h * nh + (int64_t)i * H + b] = v;
 }
 
 static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols) {
     TG_TRY(ensure_pair_tables(c));
-    const int H = c->H;
-    const int G = 256 / H;
-    const int block = G * H;
-    constexpr int PS = 4;
-    int chunk = c->N;
-    while ((int64_t)c->N * ((c->N + chunk - 1) / chunk) < 8192 && chunk > 16) chunk = (chunk + 1) / 2;
-    const int nchunks = (c->N + chunk - 1) / chunk;
-    // columns per launch: LDS (tables + source vectors, or the group reduction) <= 64 KB
-    int ncmax = 8;
-    while (ncmax > 1 && ((size_t)PS * c->plan.T + (size_t)PS * ncmax * 2 * H > 4096 ||
-                         (size_t)G * ncmax * 2 * H > 4096))
-        --ncmax;
-    TG_TRY(c->ws[0].ensure(sizeof(cd) * (size_t)std::min(cols, ncmax) * c->n * nchunks));
-    cd *part = c->ws[0].as<cd>();
+    const int N = c->N, H = c->H, T = c->plan.T;
+    const int cmax = 8;  // right-hand sides per pass
+    const int64_t Qmax = (int64_t)std::min(cols, cmax) * 2 * H;
+    TG_TRY(c->ws[0].ensure(sizeof(cd) * ((size_t)Qmax * N + (size_t)T * Qmax * N)));
+    cd *X = c->ws[0].as<cd>();
+    cd *U = X + (size_t)Qmax * N;
     Timer::Rec rec;
     TG_TRY(c->timer.begin("coupling_mf", c->stream, rec));
-    for (int c0 = 0; c0 < cols; c0 += ncmax) {
-        const int nc = cols - c0 < ncmax ? cols - c0 : ncmax;
-        const cd *xx = x + (int64_t)c0 * c->n;
-        size_t smem = sizeof(cd) * ((size_t)PS * c->plan.T + (size_t)PS * nc * 2 * H);
-        size_t red = sizeof(cd) * (size_t)G * nc * 2 * H;
-        if (red > smem) smem = red;
-        dim3 grid(c->N, nchunks);
-#define TG_TAB(NCV)                                                                                        \
-    k_coupling_tab<NCV, PS><<<grid, block, smem, c->stream>>>(c->plan, c->b_tab.as<cd>(), c->N, nc, xx, c->n, \
-                                                              c->nh, chunk, part);
-        if (nc == 1) { TG_TAB(1) } else if (nc == 2) { TG_TAB(2) } else if (nc <= 4) { TG_TAB(4) } else { TG_TAB(8) }
-#undef TG_TAB
+    for (int c0 = 0; c0 < cols; c0 += cmax) {
+        const int nc = std::min(cmax, cols - c0);
+        const int64_t Q = (int64_t)nc * 2 * H;
+        k_pack_sources<<<grid1(Q * N), 256, 0, c->stream>>>(x + (int64_t)c0 * c->n, c->n, c->nh, N, H, nc, X);
+        dim3 grid((unsigned)((Q + 63) / 64), (unsigned)((N + 63) / 64), (unsigned)T);
+        if (N % 16)
+            k_zgemm<true><<<grid, 256, 0, c->stream>>>(c->b_tab.as<cd>(), N, (int64_t)N * N, X, N, 0, U, N, Q * N, N,
+                                                       Q, N);
+        else
+            k_zgemm<false><<<grid, 256, 0, c->stream>>>(c->b_tab.as<cd>(), N, (int64_t)N * N, X, N, 0, U, N, Q * N, N,
+                                                        Q, N);
         TG_HIP(hipGetLastError());
-        const int64_t cnt = (int64_t)nc * c->n;
-        k_sum_chunks<double2><<<grid1(cnt), 256, 0, c->stream>>>((const double2 *)part, nchunks, cnt,
-                                                                  y + (int64_t)c0 * c->n);
+        k_tab_contract<<<dim3((unsigned)((N + 63) / 64), (unsigned)Q), 64, 0, c->stream>>>(
+            c->plan, U, N, H, nc, c->n, c->nh, y + (int64_t)c0 * c->n);
         TG_HIP(hipGetLastError());
     }
     TG_TRY(c->timer.end("coupling_mf", c->stream, rec));

@@ -24,52 +24,44 @@
 #include "tg_gmres.h"
 #include "tg_internal.h"
 
+#include "tg_lu.h"
+#include "tg_mfma.h"
+
 struct tg_factor {
     tg_cluster *cluster = nullptr;
     int precision = TG_STORE_C128;
     int64_t n = 0;
-    tg::DevBuf lu, piv, perm, status, work;  // perm: P x = x[perm]; work: 8 n FP64 values
+    tg::DevBuf lu;   // the factors, n x n column-major
+    tg::LuSet set;   // a batch of one matrix
 };
 
 namespace tg {
 
 constexpr int NB = 64;
 
-typedef float f32x4v __attribute__((ext_vector_type(4)));
-typedef double f64x4v __attribute__((ext_vector_type(4)));
+// ------------------------------------------------------------ factor kernels --
+//
+// Every kernel works on a batch of matrices (grid.z = matrix z): matrix z has
+// dimension bn[z], starts at element boff[z] of the base pointer, and its rows (the
+// pivots and the packed vectors of the solves) start at bvoff[z].  A single matrix is
+// a batch of one.  Workgroups beyond a matrix's own dimension return at once.
 
-template <class T> struct mfma_acc;
-template <> struct mfma_acc<float> { using V = f32x4v; };
-template <> struct mfma_acc<double> { using V = f64x4v; };
-
-// D[16x16] += A[16x4] B[4x16]: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16].
-// The result D[i][j] sits in lane j + 16 (i / 4), element i % 4 for f32, and in lane
-// j + 16 (i % 4), element i / 4 for f64.
-template <class T>
-__device__ __forceinline__ int mfma_out_row(int lane, int v) {
-    return sizeof(T) == 8 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
-}
-__device__ __forceinline__ f32x4v mfma16x16x4(float a, float b, f32x4v c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ f64x4v mfma16x16x4(double a, double b, f64x4v c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-// Plain complex value for LDS arrays (no constructors).
-template <class T>
-struct lcx {
-    T re, im;
-    __device__ __forceinline__ cx<T> get() const { return cx<T>(re, im); }
-    __device__ __forceinline__ void set(cx<T> v) { re = v.re; im = v.im; }
+struct Batch {
+    const int64_t *off;   // element offset of each matrix
+    const int *n;         // dimension of each matrix
+    const int64_t *voff;  // row offset of each matrix in the packed vectors and pivots
 };
 
-// ------------------------------------------------------------ factor kernels --
-
 // Factor the nb x nb diagonal block at k0 in LDS, one wave, lane r owning row r for
-// the update; piv[k0 + k] = global row exchanged with row k0 + k.
+// the update; piv[k0 + k] = local row exchanged with row k0 + k.
 template <class T>
-__global__ __launch_bounds__(64) void k_lu_diag(cx<T> *A, int64_t n, int64_t k0, int nb, int *piv, int *status) {
+__global__ __launch_bounds__(64) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0, int *piv0, int *status) {
+    const int z = blockIdx.z;
+    const int64_t n = bt.n[z];
+    if (k0 >= n) return;
+    cx<T> *A = A0 + bt.off[z];
+    int *piv = piv0 + bt.voff[z];
+    const int nb = (int)min((int64_t)NB, n - k0);
     __shared__ lcx<T> blk[NB][NB];  // [col][row]
     const int lane = threadIdx.x;
     for (int t = lane; t < nb * nb; t += 64) {
@@ -89,7 +81,7 @@ __global__ __launch_bounds__(64) void k_lu_diag(cx<T> *A, int64_t n, int64_t k0,
             }
         }
         if (!(best > 0.0)) {
-            if (lane == 0) *status = 1;
+            if (lane == 0) status[z] = 1;
             return;
         }
         const int p = besti;
@@ -114,12 +106,17 @@ __global__ __launch_bounds__(64) void k_lu_diag(cx<T> *A, int64_t n, int64_t k0,
     }
 }
 
-// Apply the block's row swaps to every column outside the diagonal block.
+// Apply the block's row exchanges to every column outside the diagonal block.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_swap(cx<T> *A, int64_t n, int64_t k0, int nb, const int *piv) {
+__global__ __launch_bounds__(256) void k_lu_swap(cx<T> *A0, Batch bt, int64_t k0, const int *piv0) {
+    const int z = blockIdx.z;
+    const int64_t n = bt.n[z];
+    if (k0 >= n) return;
+    const int nb = (int)min((int64_t)NB, n - k0);
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= n || (col >= k0 && col < k0 + nb)) return;
-    cx<T> *c = A + col * n;
+    cx<T> *c = A0 + bt.off[z] + col * n;
+    const int *piv = piv0 + bt.voff[z];
     for (int k = 0; k < nb; ++k) {
         const int64_t p = piv[k0 + k];
         if (p != k0 + k) {
@@ -133,7 +130,12 @@ __global__ __launch_bounds__(256) void k_lu_swap(cx<T> *A, int64_t n, int64_t k0
 // L21 = A21 U11^{-1}, in place: one thread per row below the block (coalesced
 // column reads), U11 in LDS.
 template <class T>
-__global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A, int64_t n, int64_t k0, int nb) {
+__global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A0, Batch bt, int64_t k0) {
+    const int z = blockIdx.z;
+    const int64_t n = bt.n[z];
+    if (k0 + NB >= n) return;  // no rows below a full block
+    cx<T> *A = A0 + bt.off[z];
+    const int nb = NB;
     __shared__ lcx<T> U[NB][NB];  // [col][row]
     for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
         const int col = t / nb, row = t - col * nb;
@@ -152,7 +154,12 @@ __global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A, int64_t n, i
 // U12 = L11^{-1} A12 (unit lower L11), in place: one wave per column, lane r holding
 // row r; each step broadcasts the solved entry and updates the rows below it.
 template <class T>
-__global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A, int64_t n, int64_t k0, int nb) {
+__global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A0, Batch bt, int64_t k0) {
+    const int z = blockIdx.z;
+    const int64_t n = bt.n[z];
+    if (k0 + NB >= n) return;
+    cx<T> *A = A0 + bt.off[z];
+    const int nb = NB;
     __shared__ lcx<T> L[NB][NB];  // [col][row]
     for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
         const int col = t / nb, row = t - col * nb;
@@ -164,31 +171,29 @@ __global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A, int64_t n, i
     for (int64_t col = k0 + nb + (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); col < n;
          col += waves) {
         cx<T> *c = A + col * n + k0;
-        cx<T> v = lane < nb ? c[lane] : cx<T>();
+        cx<T> v = c[lane];
         for (int k = 0; k < nb - 1; ++k) {
             const cx<T> ck(__shfl(v.re, k), __shfl(v.im, k));
-            if (lane > k && lane < nb) v -= L[k][lane].get() * ck;
+            if (lane > k) v -= L[k][lane].get() * ck;
         }
-        if (lane < nb) c[lane] = v;
+        c[lane] = v;
     }
 }
 
-// A22 -= L21 U12 with the 3M complex product on MFMA.  64 x 64 output tile per
-// workgroup (4 waves, 2 x 2 16x16 MFMA tiles each); the MFMA computes the transposed
-// tile (U12^T L21^T) so that lanes run along rows and the column-major read-modify-
-// write of A22 is coalesced.  The A22 tile is loaded into registers before the K loop
-// and the next 16-wide K slice of L21 / U12 is loaded while the current one is
-// multiplied, so no load waits on another.  Rows and columns beyond n are read
-// clamped (valid addresses) and never written.
+// A22 -= L21 U12 with the 3M complex product on MFMA, one 64 x 64 tile of A22 per
+// workgroup.  The tile's A22 values are loaded into registers before the K loop so
+// their latency overlaps the MFMAs; rows and columns beyond n are never written.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_update(cx<T> *A, int64_t n, int64_t k0, int nb) {
-    using V = typename mfma_acc<T>::V;
-    __shared__ T Lr[64][17], Li[64][17], Ur[16][65], Ui[16][65];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+__global__ __launch_bounds__(256) void k_lu_update(cx<T> *A0, Batch bt, int64_t k0) {
+    const int z = blockIdx.z;
+    const int64_t n = bt.n[z];
+    const int nb = NB;
     const int64_t r0 = k0 + nb + (int64_t)blockIdx.y * 64, c0 = k0 + nb + (int64_t)blockIdx.x * 64;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int rr = lane & 15, kq = lane >> 4;
-    // this thread's 16 elements of the A22 tile, in the MFMA output layout
+    if (r0 >= n || c0 >= n) return;  // uniform over the workgroup
+    cx<T> *A = A0 + bt.off[z];
+    __shared__ TileLds<T> sm;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, rr = lane & 15;
     cx<T> creg[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -200,54 +205,9 @@ __global__ __launch_bounds__(256) void k_lu_update(cx<T> *A, int64_t n, int64_t 
                 const int64_t gc = min(c0 + wn + b * 16 + mfma_out_row<T>(lane, v), n - 1);
                 creg[a][b][v] = A[gc * n + gr];
             }
-    // K-slice loads: L21 (64 rows x 16) at (m = tid & 63, k = (tid >> 6) + 4 j),
-    // U12 (16 x 64 columns) at (k = tid & 15, column = (tid >> 4) + 16 j)
-    const int lm = tid & 63, lk = tid >> 6, uk = tid & 15, uc = tid >> 4;
-    const int64_t lrow = min(r0 + lm, n - 1);
-    cx<T> lreg[4], ureg[4];
-    auto load_slice = [&](int kk) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            lreg[j] = A[(k0 + kk + lk + 4 * j) * n + lrow];
-            const int64_t gc = min(c0 + uc + 16 * j, n - 1);
-            ureg[j] = A[gc * n + k0 + kk + uk];
-        }
-    };
-    V acc1[2][2], acc2[2][2], acc3[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc1[a][b][v] = acc2[a][b][v] = acc3[a][b][v] = (T)0;
-    load_slice(0);
-    for (int kk = 0; kk < nb; kk += 16) {
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            Lr[lm][lk + 4 * j] = lreg[j].re;
-            Li[lm][lk + 4 * j] = lreg[j].im;
-            Ur[uk][uc + 16 * j] = ureg[j].re;
-            Ui[uk][uc + 16 * j] = ureg[j].im;
-        }
-        __syncthreads();
-        if (kk + 16 < nb) load_slice(kk + 16);
-#pragma unroll
-        for (int k4 = 0; k4 < 16; k4 += 4) {
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const T br = Ur[k4 + kq][wn + b * 16 + rr], bi = Ui[k4 + kq][wn + b * 16 + rr];
-#pragma unroll
-                for (int a = 0; a < 2; ++a) {
-                    const T ar = Lr[wm + a * 16 + rr][k4 + kq], ai = Li[wm + a * 16 + rr][k4 + kq];
-                    acc1[a][b] = mfma16x16x4(br, ar, acc1[a][b]);
-                    acc2[a][b] = mfma16x16x4(bi, ai, acc2[a][b]);
-                    acc3[a][b] = mfma16x16x4(br + bi, ar + ai, acc3[a][b]);
-                }
-            }
-        }
-    }
-    // transposed tile: element v of lane l is (row l % 16, column mfma_out_row(l, v))
+    Tile3M<T> acc;
+    // L21 = A[:, k0 : k0 + nb] (rows absolute), U12 = A[k0 : k0 + nb, :] (columns absolute)
+    zgemm_tile_3m<T, false>(A + k0 * n, n, A + k0, n, r0, c0, n, n, nb, sm, acc);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -255,7 +215,7 @@ __global__ __launch_bounds__(256) void k_lu_update(cx<T> *A, int64_t n, int64_t 
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<T>(lane, v);
-                const T t1 = acc1[a][b][v], t2 = acc2[a][b][v], t3 = acc3[a][b][v];
+                const T t1 = acc.t1[a][b][v], t2 = acc.t2[a][b][v], t3 = acc.t3[a][b][v];
                 cx<T> d = creg[a][b][v];
                 d.re -= t1 - t2;
                 d.im -= t3 - t1 - t2;
@@ -264,28 +224,37 @@ __global__ __launch_bounds__(256) void k_lu_update(cx<T> *A, int64_t n, int64_t 
 }
 
 // ------------------------------------------------------------- solve kernels --
+//
+// Vectors are packed: column q of the batch is x + q ld, and matrix z's rows start at
+// x + q ld + voff[z].
 
-// Row permutation of `cols` vector columns: dst[i] = src[perm[i]] (forward, P x) or
-// dst[perm[i]] = src[i] (inverse, P^T x).
-__global__ void k_lu_permute(const cd *src, cd *dst, int64_t n, int cols, const int *perm, int inverse) {
+// Row permutation of `cols` packed vector columns: dst[i] = src[perm[i]] (forward,
+// P x) or dst[perm[i]] = src[i] (inverse, P^T x); perm holds packed indices.
+__global__ void k_lu_permute(const cd *src, cd *dst, int64_t ld, int cols, const int *perm, int inverse) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * cols) return;
-    const int64_t col = t / n, i = t - col * n;
+    if (t >= ld * cols) return;
+    const int64_t col = t / ld, i = t - col * ld;
     const int64_t p = perm[i];
     if (!inverse)
-        dst[col * n + i] = src[col * n + p];
+        dst[col * ld + i] = src[col * ld + p];
     else
-        dst[col * n + p] = src[col * n + i];
+        dst[col * ld + p] = src[col * ld + i];
 }
 
-// Triangular solve with the diagonal block at k0: one workgroup, wave w solving vector
-// column w (w < cols <= 8), lane r holding row r (FP64 vectors, factors in T).  which = 0: L y = b (unit lower);
-// 1: U x = y; 2: U^H z = g; 3: L^H w = z (unit upper).  The block is staged in LDS
-// oriented so that step k reads S[k][lane] (conflict-free): S[k][r] = A(r, k) for
-// which 0 / 1 and S[k][r] = A(k, r) for which 2 / 3.
+// Triangular solve with the diagonal block at k0: one workgroup per matrix, wave w
+// solving vector column w (w < cols <= 8), lane r holding row r (FP64 vectors,
+// factors in T).  which = 0: L y = b (unit lower); 1: U x = y; 2: U^H z = g;
+// 3: L^H w = z (unit upper).  The block is staged in LDS oriented so that step k
+// reads S[k][lane] (conflict-free): S[k][r] = A(r, k) for which 0 / 1 and
+// S[k][r] = conj(A(k, r)) for which 2 / 3.
 template <class T>
-__global__ __launch_bounds__(512) void k_lu_block_solve(const cx<T> *A, int64_t n, int64_t k0, int nb, cd *x,
+__global__ __launch_bounds__(512) void k_lu_block_solve(const cx<T> *A0, Batch bt, int64_t k0, cd *x0, int64_t ld,
                                                         int cols, int which) {
+    const int z = blockIdx.z;
+    const int64_t n = bt.n[z];
+    if (k0 >= n) return;
+    const cx<T> *A = A0 + bt.off[z];
+    const int nb = (int)min((int64_t)NB, n - k0);
     __shared__ lcx<T> S[NB][NB];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const bool trans = which >= 2;
@@ -300,7 +269,7 @@ __global__ __launch_bounds__(512) void k_lu_block_solve(const cx<T> *A, int64_t 
     }
     __syncthreads();
     if (wave >= cols) return;
-    cd *v = x + (int64_t)wave * n + k0;
+    cd *v = x0 + (int64_t)wave * ld + bt.voff[z] + k0;
     cd val = lane < nb ? v[lane] : cd();
     auto s = [&](int k) {
         const cx<T> e = S[k][lane].get();
@@ -334,17 +303,24 @@ __global__ __launch_bounds__(512) void k_lu_block_solve(const cx<T> *A, int64_t 
 // which = 1: x[r] -= sum_k A[r, k0 + k] x[k0 + k] for r < k0 (U01);
 // one thread per row, 16 loads in flight, coalesced along the panel's columns.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_panel_rows(const cx<T> *A, int64_t n, int64_t k0, int nb, cd *x, int cols,
-                                                       int which) {
-    __shared__ lcx<double> seg[8][NB];
+__global__ __launch_bounds__(256) void k_lu_panel_rows(const cx<T> *A0, Batch bt, int64_t k0, cd *x0, int64_t ld,
+                                                       int cols, int which) {
+    const int z = blockIdx.z;
+    const int64_t n = bt.n[z];
+    if (k0 >= n) return;
+    const int nb = (int)min((int64_t)NB, n - k0);
     const int64_t rcount = which == 0 ? n - k0 - nb : k0;
+    const int64_t rr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int64_t)blockIdx.x * blockDim.x >= rcount) return;  // uniform over the workgroup
+    const cx<T> *A = A0 + bt.off[z];
+    cd *x = x0 + bt.voff[z];
+    __shared__ lcx<double> seg[8][NB];
     // entries beyond nb are zero, so the loads below need no bounds branch
     for (int t = threadIdx.x; t < cols * NB; t += blockDim.x) {
         const int q = t / NB, k = t % NB;
-        seg[q][k].set(k < nb ? x[(int64_t)q * n + k0 + k] : cd(0.0));
+        seg[q][k].set(k < nb ? x[(int64_t)q * ld + k0 + k] : cd(0.0));
     }
     __syncthreads();
-    const int64_t rr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (rr >= rcount) return;
     const int64_t r = which == 0 ? k0 + nb + rr : rr;
     cd s[8];
@@ -364,7 +340,7 @@ __global__ __launch_bounds__(256) void k_lu_panel_rows(const cx<T> *A, int64_t n
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q)
-        if (q < cols) x[(int64_t)q * n + r] -= s[q];
+        if (q < cols) x[(int64_t)q * ld + r] -= s[q];
 }
 
 // which = 2: x[c] -= sum_k conj(A[k0 + k, c]) x[k0 + k] for c >= k0 + nb (U12^H);
@@ -372,12 +348,19 @@ __global__ __launch_bounds__(256) void k_lu_panel_rows(const cx<T> *A, int64_t n
 // one wave per column c: lane k reads A[k0 + k, c] (a contiguous segment) and the
 // wave reduces the products with shuffles.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_panel_cols(const cx<T> *A, int64_t n, int64_t k0, int nb, cd *x, int cols,
-                                                       int which) {
-    __shared__ lcx<double> seg[8][NB];
+__global__ __launch_bounds__(256) void k_lu_panel_cols(const cx<T> *A0, Batch bt, int64_t k0, cd *x0, int64_t ld,
+                                                       int cols, int which) {
+    const int z = blockIdx.z;
+    const int64_t n = bt.n[z];
+    if (k0 >= n) return;
+    const int nb = (int)min((int64_t)NB, n - k0);
     const int64_t ccount = which == 2 ? n - k0 - nb : k0;
+    if ((int64_t)blockIdx.x * (blockDim.x >> 6) >= ccount) return;  // uniform over the workgroup
+    const cx<T> *A = A0 + bt.off[z];
+    cd *x = x0 + bt.voff[z];
+    __shared__ lcx<double> seg[8][NB];
     for (int t = threadIdx.x; t < cols * nb; t += blockDim.x)
-        seg[t / nb][t % nb].set(x[(int64_t)(t / nb) * n + k0 + t % nb]);
+        seg[t / nb][t % nb].set(x[(int64_t)(t / nb) * ld + k0 + t % nb]);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
@@ -400,109 +383,168 @@ __global__ __launch_bounds__(256) void k_lu_panel_cols(const cx<T> *A, int64_t n
 #pragma unroll
             for (int q = 1; q < 8; ++q)
                 if (q == lane) sq = s[q];
-            x[(int64_t)lane * n + c] -= sq;
+            x[(int64_t)lane * ld + c] -= sq;
         }
     }
 }
 
+// ------------------------------------------------------------------ host side --
+
+int luset_init(LuSet &ls, int precision, void *A, const std::vector<int64_t> &off, const std::vector<int> &n) {
+    ls.precision = precision;
+    ls.A = A;
+    ls.count = (int)n.size();
+    ls.nmax = 0;
+    ls.ntot = 0;
+    ls.n_h = n;
+    ls.voff_h.assign(ls.count, 0);
+    for (int z = 0; z < ls.count; ++z) {
+        ls.voff_h[z] = ls.ntot;
+        ls.ntot += n[z];
+        ls.nmax = std::max(ls.nmax, n[z]);
+    }
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_off = 0, o_n = al(sizeof(int64_t) * ls.count), o_voff = al(o_n + sizeof(int) * ls.count);
+    const size_t o_piv = al(o_voff + sizeof(int64_t) * ls.count);
+    const size_t o_perm = al(o_piv + sizeof(int) * ls.ntot);
+    const size_t o_stat = al(o_perm + sizeof(int) * ls.ntot);
+    const size_t total = al(o_stat + sizeof(int) * ls.count);
+    TG_TRY(ls.meta.ensure(total));
+    std::vector<char> host(o_piv, 0);
+    memcpy(host.data() + o_off, off.data(), sizeof(int64_t) * ls.count);
+    memcpy(host.data() + o_n, n.data(), sizeof(int) * ls.count);
+    memcpy(host.data() + o_voff, ls.voff_h.data(), sizeof(int64_t) * ls.count);
+    char *base = ls.meta.as<char>();
+    TG_HIP(hipMemcpy(base, host.data(), o_piv, hipMemcpyHostToDevice));
+    ls.off = (const int64_t *)(base + o_off);
+    ls.n = (const int *)(base + o_n);
+    ls.voff = (const int64_t *)(base + o_voff);
+    ls.piv = (int *)(base + o_piv);
+    ls.perm = (int *)(base + o_perm);
+    ls.status = (int *)(base + o_stat);
+    TG_TRY(ls.work.ensure(sizeof(cd) * 8 * (size_t)ls.ntot));
+    return TG_OK;
+}
+
+void luset_release(LuSet &ls) {
+    ls.meta.release();
+    ls.work.release();
+}
+
 template <class T>
-static int factor_t(tg_factor *f) {
-    tg_cluster *c = f->cluster;
-    const int64_t n = c->n;
-    cx<T> *A = f->lu.as<cx<T>>();
-    int *piv = f->piv.as<int>();
-    int *status = f->status.as<int>();
-    TG_HIP(hipMemsetAsync(status, 0, sizeof(int), c->stream));
-    TG_TRY(launch_assemble_operator(c, f->precision, A));
-    Timer::Rec rec;
-    TG_TRY(c->timer.begin("lu_factor", c->stream, rec));
-    for (int64_t k0 = 0; k0 < n; k0 += NB) {
-        const int nb = (int)std::min<int64_t>(NB, n - k0);
-        k_lu_diag<T><<<1, 64, 0, c->stream>>>(A, n, k0, nb, piv, status);
-        k_lu_swap<T><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(A, n, k0, nb, piv);
-        const int64_t rest = n - k0 - nb;
+static int luset_factor_t(LuSet &ls, hipStream_t s) {
+    cx<T> *A = reinterpret_cast<cx<T> *>(ls.A);
+    const Batch bt{ls.off, ls.n, ls.voff};
+    const unsigned z = (unsigned)ls.count;
+    const int64_t nmax = ls.nmax;
+    TG_HIP(hipMemsetAsync(ls.status, 0, sizeof(int) * ls.count, s));
+    for (int64_t k0 = 0; k0 < nmax; k0 += NB) {
+        k_lu_diag<T><<<dim3(1, 1, z), 64, 0, s>>>(A, bt, k0, ls.piv, ls.status);
+        k_lu_swap<T><<<dim3((unsigned)((nmax + 255) / 256), 1, z), 256, 0, s>>>(A, bt, k0, ls.piv);
+        const int64_t rest = nmax - k0 - NB;
         if (rest > 0) {
-            k_trsm_lower_panel<T><<<(unsigned)((rest + 255) / 256), 256, 0, c->stream>>>(A, n, k0, nb);
+            k_trsm_lower_panel<T><<<dim3((unsigned)((rest + 255) / 256), 1, z), 256, 0, s>>>(A, bt, k0);
             k_trsm_upper_panel<T>
-                <<<(unsigned)std::min<int64_t>((rest + 3) / 4, 1024), 256, 0, c->stream>>>(A, n, k0, nb);
-            dim3 grid((unsigned)((rest + 63) / 64), (unsigned)((rest + 63) / 64));
-            k_lu_update<T><<<grid, 256, 0, c->stream>>>(A, n, k0, nb);
+                <<<dim3((unsigned)std::min<int64_t>((rest + 3) / 4, 1024), 1, z), 256, 0, s>>>(A, bt, k0);
+            const unsigned tiles = (unsigned)((rest + 63) / 64);
+            k_lu_update<T><<<dim3(tiles, tiles, z), 256, 0, s>>>(A, bt, k0);
         }
         TG_HIP(hipGetLastError());
     }
-    TG_TRY(c->timer.end("lu_factor", c->stream, rec));
-    int st = 0;
-    TG_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    TG_HIP(hipStreamSynchronize(c->stream));
-    if (st) {
-        set_error("zero pivot in the LU factorisation");
-        return TG_SINGULAR;
+    std::vector<int> stat(ls.count), piv_h(ls.ntot), perm_h(ls.ntot);
+    TG_HIP(hipMemcpyAsync(stat.data(), ls.status, sizeof(int) * ls.count, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipMemcpyAsync(piv_h.data(), ls.piv, sizeof(int) * ls.ntot, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    for (int v : stat)
+        if (v) {
+            set_error("zero pivot in the LU factorisation");
+            return TG_SINGULAR;
+        }
+    // compose each matrix's exchanges into one gather over the packed rows:
+    // (P x)[voff + i] = x[voff + perm_local[i]]
+    for (int z2 = 0; z2 < ls.count; ++z2) {
+        const int64_t v0 = ls.voff_h[z2], nz = ls.n_h[z2];
+        for (int64_t i = 0; i < nz; ++i) perm_h[v0 + i] = (int)(v0 + i);
+        for (int64_t k = 0; k < nz; ++k) std::swap(perm_h[v0 + k], perm_h[v0 + piv_h[v0 + k]]);
     }
-    // compose the exchanges into one gather: (P x)[i] = x[perm[i]]
-    std::vector<int> piv_h(n), perm_h(n);
-    TG_HIP(hipMemcpy(piv_h.data(), piv, sizeof(int) * n, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < n; ++i) perm_h[i] = (int)i;
-    for (int64_t k = 0; k < n; ++k) std::swap(perm_h[k], perm_h[piv_h[k]]);
-    TG_HIP(hipMemcpy(f->perm.as<int>(), perm_h.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    TG_HIP(hipMemcpyAsync(ls.perm, perm_h.data(), sizeof(int) * ls.ntot, hipMemcpyHostToDevice, s));
+    TG_HIP(hipStreamSynchronize(s));
     return TG_OK;
 }
 
-// x (split order, FP64, `cols` <= 8 columns) <- A^{-1} x or A^{-H} x with the factors.
+int luset_factor(LuSet &ls, hipStream_t s) {
+    return ls.precision == TG_STORE_C64 ? luset_factor_t<float>(ls, s) : luset_factor_t<double>(ls, s);
+}
+
+// x (packed rows, FP64, `cols` <= 8 columns of stride ld = ntot) <- A^{-1} x or A^{-H} x.
 template <class T>
-static int lu_solve_t(tg_factor *f, cd *x, int cols, int adjoint) {
-    tg_cluster *c = f->cluster;
-    hipStream_t s = c->stream;
-    const int64_t n = f->n;
-    const cx<T> *A = f->lu.as<cx<T>>();
-    const int *perm = f->perm.as<int>();
-    cd *work = f->work.as<cd>();
+static int luset_solve8(LuSet &ls, hipStream_t s, cd *x, int cols, int adjoint) {
+    const cx<T> *A = reinterpret_cast<const cx<T> *>(ls.A);
+    const Batch bt{ls.off, ls.n, ls.voff};
+    const unsigned z = (unsigned)ls.count;
+    const int64_t nmax = ls.nmax, ld = ls.ntot;
+    cd *work = ls.work.as<cd>();
     auto grid = [](int64_t w) { return (unsigned)std::max<int64_t>(1, (w + 255) / 256); };
     // one wave per column, at most 2048 workgroups of 4 waves (grid-stride beyond)
     auto wgrid = [](int64_t w) { return (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (w + 3) / 4)); };
-    Timer::Rec rec;
-    TG_TRY(c->timer.begin("lu_solve", s, rec));
-    const int64_t last = ((n - 1) / NB) * NB;
+    const int64_t last = ((nmax - 1) / NB) * NB;
     if (!adjoint) {
         // P A = L U: x <- P x, L y = x, U x = y
-        k_lu_permute<<<grid(n * cols), 256, 0, s>>>(x, work, n, cols, perm, 0);
-        TG_HIP(hipMemcpyAsync(x, work, sizeof(cd) * n * cols, hipMemcpyDeviceToDevice, s));
-        for (int64_t k0 = 0; k0 < n; k0 += NB) {
-            const int nb = (int)std::min<int64_t>(NB, n - k0);
-            k_lu_block_solve<T><<<1, 512, 0, s>>>(A, n, k0, nb, x, cols, 0);
-            if (n - k0 - nb > 0) k_lu_panel_rows<T><<<grid(n - k0 - nb), 256, 0, s>>>(A, n, k0, nb, x, cols, 0);
+        k_lu_permute<<<grid(ld * cols), 256, 0, s>>>(x, work, ld, cols, ls.perm, 0);
+        TG_HIP(hipMemcpyAsync(x, work, sizeof(cd) * ld * cols, hipMemcpyDeviceToDevice, s));
+        for (int64_t k0 = 0; k0 < nmax; k0 += NB) {
+            k_lu_block_solve<T><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 0);
+            if (nmax - k0 - NB > 0)
+                k_lu_panel_rows<T><<<dim3(grid(nmax - k0 - NB), 1, z), 256, 0, s>>>(A, bt, k0, x, ld, cols, 0);
         }
         for (int64_t k0 = last; k0 >= 0; k0 -= NB) {
-            const int nb = (int)std::min<int64_t>(NB, n - k0);
-            k_lu_block_solve<T><<<1, 512, 0, s>>>(A, n, k0, nb, x, cols, 1);
-            if (k0 > 0) k_lu_panel_rows<T><<<grid(k0), 256, 0, s>>>(A, n, k0, nb, x, cols, 1);
+            k_lu_block_solve<T><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 1);
+            if (k0 > 0) k_lu_panel_rows<T><<<dim3(grid(k0), 1, z), 256, 0, s>>>(A, bt, k0, x, ld, cols, 1);
         }
     } else {
         // A^H = U^H L^H P: U^H z = g, L^H w = z, x = P^T w
-        for (int64_t k0 = 0; k0 < n; k0 += NB) {
-            const int nb = (int)std::min<int64_t>(NB, n - k0);
-            k_lu_block_solve<T><<<1, 512, 0, s>>>(A, n, k0, nb, x, cols, 2);
-            if (n - k0 - nb > 0) k_lu_panel_cols<T><<<wgrid(n - k0 - nb), 256, 0, s>>>(A, n, k0, nb, x, cols, 2);
+        for (int64_t k0 = 0; k0 < nmax; k0 += NB) {
+            k_lu_block_solve<T><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 2);
+            if (nmax - k0 - NB > 0)
+                k_lu_panel_cols<T><<<dim3(wgrid(nmax - k0 - NB), 1, z), 256, 0, s>>>(A, bt, k0, x, ld, cols, 2);
         }
         for (int64_t k0 = last; k0 >= 0; k0 -= NB) {
-            const int nb = (int)std::min<int64_t>(NB, n - k0);
-            k_lu_block_solve<T><<<1, 512, 0, s>>>(A, n, k0, nb, x, cols, 3);
-            if (k0 > 0) k_lu_panel_cols<T><<<wgrid(k0), 256, 0, s>>>(A, n, k0, nb, x, cols, 3);
+            k_lu_block_solve<T><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 3);
+            if (k0 > 0) k_lu_panel_cols<T><<<dim3(wgrid(k0), 1, z), 256, 0, s>>>(A, bt, k0, x, ld, cols, 3);
         }
-        k_lu_permute<<<grid(n * cols), 256, 0, s>>>(x, work, n, cols, perm, 1);
-        TG_HIP(hipMemcpyAsync(x, work, sizeof(cd) * n * cols, hipMemcpyDeviceToDevice, s));
+        k_lu_permute<<<grid(ld * cols), 256, 0, s>>>(x, work, ld, cols, ls.perm, 1);
+        TG_HIP(hipMemcpyAsync(x, work, sizeof(cd) * ld * cols, hipMemcpyDeviceToDevice, s));
     }
     TG_HIP(hipGetLastError());
-    TG_TRY(c->timer.end("lu_solve", s, rec));
     return TG_OK;
 }
 
-static int lu_solve(tg_factor *f, cd *x, int cols, int adjoint) {
+int luset_solve(LuSet &ls, hipStream_t s, cd *x, int cols, int adjoint) {
     for (int c0 = 0; c0 < cols; c0 += 8) {
         const int w = std::min(8, cols - c0);
-        cd *xc = x + (int64_t)c0 * f->n;
-        TG_TRY(f->precision == TG_STORE_C64 ? lu_solve_t<float>(f, xc, w, adjoint)
-                                            : lu_solve_t<double>(f, xc, w, adjoint));
+        cd *xc = x + (int64_t)c0 * ls.ntot;
+        TG_TRY(ls.precision == TG_STORE_C64 ? luset_solve8<float>(ls, s, xc, w, adjoint)
+                                            : luset_solve8<double>(ls, s, xc, w, adjoint));
     }
+    return TG_OK;
+}
+
+static int factor_cluster(tg_factor *f) {
+    tg_cluster *c = f->cluster;
+    TG_TRY(launch_assemble_operator(c, f->precision, f->lu.ptr));
+    Timer::Rec rec;
+    TG_TRY(c->timer.begin("lu_factor", c->stream, rec));
+    const int st = luset_factor(f->set, c->stream);
+    TG_TRY(c->timer.end("lu_factor", c->stream, rec));
+    return st;
+}
+
+static int lu_solve(tg_factor *f, cd *x, int cols, int adjoint) {
+    tg_cluster *c = f->cluster;
+    Timer::Rec rec;
+    TG_TRY(c->timer.begin("lu_solve", c->stream, rec));
+    TG_TRY(luset_solve(f->set, c->stream, x, cols, adjoint));
+    TG_TRY(c->timer.end("lu_solve", c->stream, rec));
     return TG_OK;
 }
 
@@ -588,18 +630,12 @@ int tg_factor_create(tg_cluster *c, int precision, tg_factor **out) {
     f->n = c->n;
     const size_t esz = precision == TG_STORE_C64 ? sizeof(cf) : sizeof(cd);
     int st = f->lu.ensure(esz * (size_t)c->n * (size_t)c->n);
-    if (!st) st = f->piv.ensure(sizeof(int) * (size_t)c->n);
-    if (!st) st = f->perm.ensure(sizeof(int) * (size_t)c->n);
-    if (!st) st = f->work.ensure(sizeof(cd) * 8 * (size_t)c->n);
-    if (!st) st = f->status.ensure(sizeof(int));
-    if (!st) st = precision == TG_STORE_C64 ? factor_t<float>(f) : factor_t<double>(f);
+    if (!st) st = luset_init(f->set, precision, f->lu.ptr, {0}, {(int)c->n});
+    if (!st) st = factor_cluster(f);
     if (st) {
         (void)hipStreamSynchronize(c->stream);
         f->lu.release();
-        f->piv.release();
-        f->perm.release();
-        f->work.release();
-        f->status.release();
+        luset_release(f->set);
         delete f;
         return st;
     }
@@ -611,10 +647,7 @@ int tg_factor_destroy(tg_factor *f) {
     if (!f) return TG_OK;
     (void)hipStreamSynchronize(f->cluster->stream);
     f->lu.release();
-    f->piv.release();
-    f->perm.release();
-    f->work.release();
-    f->status.release();
+    luset_release(f->set);
     delete f;
     return TG_OK;
 }

@@ -1,9 +1,9 @@
 // Block-Jacobi preconditioner over spatial groups of spheres (right preconditioning).
 //
 // Group operator A_G = I - T_G C_GG (sphere-major, reference mode order inside a
-// sphere), inverted exactly on the device by in-place Gauss-Jordan elimination with
-// partial pivoting (one workgroup per group, column-major storage so that the
-// elimination sweep is coalesced over rows).
+// sphere), factored on the device by the batched LU (all groups in one set of
+// launches, MFMA trailing updates); applying it gathers each group's rows into a
+// packed vector, runs the batched triangular solves and scatters the result back.
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -11,6 +11,7 @@
 #include <numeric>
 #include <vector>
 
+#include "tg_lu.h"
 #include "tg_precond.h"
 
 namespace tg {
@@ -22,8 +23,10 @@ struct Preconditioner {
     int kind = TG_PC_NONE;
     int group_size = 4;
     int ngroups = 0, max_m = 0;
-    DevBuf mats;      // group inverses, column-major, c128
+    DevBuf mats;      // group operators, then their LU factors, column-major, c128
     DevBuf meta;      // offsets (int64), sizes (int), member offsets (int), members (int)
+    LuSet lu;         // the batched factorisation of `mats`
+    DevBuf packed;    // gathered group vectors [col][sum of group dimensions]
     const int64_t *d_off = nullptr;
     const int *d_size = nullptr, *d_moff = nullptr, *d_members = nullptr;
     // learned near-field correction (tg_learned.hip)
@@ -92,104 +95,28 @@ __global__ void k_group_operator(const cd *pairs, const int64_t *pair_off, const
     mats[mat_off[g] + t] = v;
 }
 
-// In-place Gauss-Jordan inversion with partial pivoting, one workgroup per matrix.
-__global__ __launch_bounds__(256) void k_gauss_jordan(cd *mats, const int64_t *mat_off, const int *gsize, int M,
-                                                      int *pivots, int piv_stride, int *status) {
-    extern __shared__ cd sh[];
-    const int g = blockIdx.x;
-    const int m = gsize[g] * M;
-    cd *a = mats + mat_off[g];
-    int *piv = pivots + (int64_t)g * piv_stride;
-    cd *colk = sh;
-    cd *rowk = sh + m;
-    __shared__ double bv[256];
-    __shared__ int bi[256];
-    const int tid = threadIdx.x;
-    for (int k = 0; k < m; ++k) {
-        double best = -1.0;
-        int besti = k;
-        for (int r = k + tid; r < m; r += blockDim.x) {
-            double v = abs2(a[(int64_t)k * m + r]);
-            if (v > best) { best = v; besti = r; }
-        }
-        bv[tid] = best;
-        bi[tid] = besti;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (tid < s) {
-                if (bv[tid + s] > bv[tid] || (bv[tid + s] == bv[tid] && bi[tid + s] < bi[tid])) {
-                    bv[tid] = bv[tid + s];
-                    bi[tid] = bi[tid + s];
-                }
-            }
-            __syncthreads();
-        }
-        const int p = bi[0];
-        if (!(bv[0] > 0.0)) {
-            if (tid == 0) status[g] = 1;
-            return;
-        }
-        if (tid == 0) piv[k] = p;
-        if (p != k)
-            for (int col = tid; col < m; col += blockDim.x) {
-                cd tmp = a[(int64_t)col * m + k];
-                a[(int64_t)col * m + k] = a[(int64_t)col * m + p];
-                a[(int64_t)col * m + p] = tmp;
-            }
-        __syncthreads();
-        const cd d = cd(1.0) / a[(int64_t)k * m + k];
-        for (int col = tid; col < m; col += blockDim.x) {
-            cd v = col == k ? d : a[(int64_t)col * m + k] * d;
-            a[(int64_t)col * m + k] = v;
-            rowk[col] = v;
-        }
-        for (int r = tid; r < m; r += blockDim.x) colk[r] = r == k ? cd(0.0) : a[(int64_t)k * m + r];
-        __syncthreads();
-        for (int64_t idx = tid; idx < (int64_t)m * m; idx += blockDim.x) {
-            const int col = (int)(idx / m), r = (int)(idx - (int64_t)col * m);
-            if (r == k) continue;
-            cd f = colk[r];
-            cd old = col == k ? cd(0.0) : a[idx];
-            a[idx] = old - f * rowk[col];
-        }
-        __syncthreads();
-    }
-    for (int k = m - 1; k >= 0; --k) {
-        const int p = piv[k];
-        if (p != k)
-            for (int r = tid; r < m; r += blockDim.x) {
-                cd tmp = a[(int64_t)k * m + r];
-                a[(int64_t)k * m + r] = a[(int64_t)p * m + r];
-                a[(int64_t)p * m + r] = tmp;
-            }
-        __syncthreads();
-    }
+// Packed group vectors: row (slot M + mode) of a packed column holds mode `mode`
+// (reference order) of sphere members[slot]; groups are consecutive slot ranges, so
+// group g's rows start at moff[g] M, as the batched LU expects.
+__device__ __forceinline__ int64_t packed_to_split(const int *members, int M, int H, int64_t nh, int64_t k) {
+    const int slot = (int)(k / M), mode = (int)(k - (int64_t)slot * M);
+    return (int64_t)(mode & 1) * nh + (int64_t)members[slot] * H + (mode >> 1);
 }
 
-// y_G = A_G^{-1} x_G (or A_G^{-H} x_G) per group and column, split layout in and out.
-__global__ __launch_bounds__(256) void k_pc_apply(const cd *mats, const int64_t *mat_off, const int *gsize,
-                                                  const int *moff, const int *members, int M, int H, int64_t n,
-                                                  int64_t nh, const cd *x, cd *y, int adjoint) {
-    extern __shared__ cd v[];
-    const int g = blockIdx.x, col = blockIdx.y;
-    const int m = gsize[g] * M;
-    const cd *a = mats + mat_off[g];
-    for (int t = threadIdx.x; t < m; t += blockDim.x) {
-        const int li = t / M, mode = t - li * M;
-        const int p = members[moff[g] + li];
-        v[t] = x[(int64_t)col * n + (int64_t)(mode & 1) * nh + (int64_t)p * H + (mode >> 1)];
-    }
-    __syncthreads();
-    for (int r = threadIdx.x; r < m; r += blockDim.x) {
-        cd s(0.0);
-        if (!adjoint)
-            for (int k = 0; k < m; ++k) s += a[(int64_t)k * m + r] * v[k];
-        else
-            for (int k = 0; k < m; ++k) s += conj(a[(int64_t)r * m + k]) * v[k];
-        const int li = r / M, mode = r - li * M;
-        const int p = members[moff[g] + li];
-        y[(int64_t)col * n + (int64_t)(mode & 1) * nh + (int64_t)p * H + (mode >> 1)] = s;
-    }
+__global__ void k_pc_gather(const int *members, int M, int H, int64_t n, int64_t nh, int64_t ld, int cols,
+                            const cd *x, cd *packed) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ld * cols) return;
+    const int64_t col = t / ld, k = t - col * ld;
+    packed[t] = x[col * n + packed_to_split(members, M, H, nh, k)];
+}
+
+__global__ void k_pc_scatter(const int *members, int M, int H, int64_t n, int64_t nh, int64_t ld, int cols,
+                             const cd *packed, cd *y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ld * cols) return;
+    const int64_t col = t / ld, k = t - col * ld;
+    y[col * n + packed_to_split(members, M, H, nh, k)] = packed[t];
 }
 
 int learned_setup(tg_cluster *c, Preconditioner *pc);
@@ -227,9 +154,7 @@ static int build_block_jacobi(tg_cluster *c, Preconditioner *pc) {
     size_t o_moff = al(o_size + sizeof(int) * pc->ngroups);
     size_t o_mem = al(o_moff + sizeof(int) * (pc->ngroups + 1));
     size_t o_disp = al(o_mem + sizeof(int) * members.size());
-    size_t o_piv = al(o_disp + sizeof(double) * std::max<size_t>(disp.size(), 1));
-    size_t o_stat = al(o_piv + sizeof(int) * (size_t)pc->ngroups * pc->max_m);
-    size_t total = al(o_stat + sizeof(int) * pc->ngroups);
+    size_t total = al(o_disp + sizeof(double) * std::max<size_t>(disp.size(), 1));
     TG_TRY(pc->meta.ensure(total));
     std::vector<char> host(total, 0);
     memcpy(host.data() + o_off, mat_off.data(), sizeof(int64_t) * (pc->ngroups + 1));
@@ -258,24 +183,18 @@ static int build_block_jacobi(tg_cluster *c, Preconditioner *pc) {
         if (hipGetLastError() != hipSuccess) st = TG_DEVICE_ERROR;
     }
     if (!st) {
-        Timer::Rec rec;
-        c->timer.begin("pc_invert", c->stream, rec);
-        (void)hipMemsetAsync(base + o_stat, 0, sizeof(int) * pc->ngroups, c->stream);
-        k_gauss_jordan<<<pc->ngroups, 256, sizeof(cd) * 2 * pc->max_m, c->stream>>>(
-            pc->mats.as<cd>(), pc->d_off, pc->d_size, M, (int *)(base + o_piv), pc->max_m, (int *)(base + o_stat));
-        if (hipGetLastError() != hipSuccess) st = TG_DEVICE_ERROR;
-        c->timer.end("pc_invert", c->stream, rec);
-    }
-    if (!st) {
-        std::vector<int> stat(pc->ngroups);
-        TG_HIP(hipMemcpyAsync(stat.data(), base + o_stat, sizeof(int) * pc->ngroups, hipMemcpyDeviceToHost, c->stream));
-        TG_HIP(hipStreamSynchronize(c->stream));
-        for (int v : stat)
-            if (v) {
-                set_error("singular group block in the block-Jacobi preconditioner");
-                st = TG_SINGULAR;
-                break;
-            }
+        // every group operator, LU-factored in one batch (rows of group g at moff[g] M)
+        std::vector<int> dims(pc->ngroups);
+        for (int g = 0; g < pc->ngroups; ++g) dims[g] = gsize[g] * M;
+        std::vector<int64_t> offs(mat_off.begin(), mat_off.end() - 1);
+        st = luset_init(pc->lu, TG_STORE_C128, pc->mats.ptr, offs, dims);
+        if (!st) {
+            Timer::Rec rec;
+            c->timer.begin("pc_factor", c->stream, rec);
+            st = luset_factor(pc->lu, c->stream);
+            c->timer.end("pc_factor", c->stream, rec);
+            if (st == TG_SINGULAR) set_error("singular group block in the block-Jacobi preconditioner");
+        }
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess && !st) st = TG_DEVICE_ERROR;
     pairs.release();
@@ -297,6 +216,8 @@ int ensure_preconditioner(tg_cluster *c, int kind, int storage) {
     if (st) {
         pc->mats.release();
         pc->meta.release();
+        luset_release(pc->lu);
+        pc->packed.release();
         learned_destroy(pc);
         delete pc;
         return st;
@@ -317,9 +238,14 @@ int apply_preconditioner(tg_cluster *c, const cd *x, cd *y, int cols, int adjoin
     if (pc->kind == TG_PC_LEARNED) {
         TG_TRY(learned_apply(c, pc, x, y, cols, adjoint));
     } else {
-        k_pc_apply<<<dim3(pc->ngroups, cols), 256, sizeof(cd) * pc->max_m, c->stream>>>(
-            pc->mats.as<cd>(), pc->d_off, pc->d_size, pc->d_moff, pc->d_members, c->M, c->H, c->n, c->nh, x, y,
-            adjoint);
+        const int64_t ld = pc->lu.ntot;  // = N M: every sphere sits in exactly one group
+        TG_TRY(pc->packed.ensure(sizeof(cd) * (size_t)ld * cols));
+        cd *pk = pc->packed.as<cd>();
+        const unsigned grid = (unsigned)((ld * cols + 255) / 256);
+        k_pc_gather<<<grid, 256, 0, c->stream>>>(pc->d_members, c->M, c->H, c->n, c->nh, ld, cols, x, pk);
+        TG_HIP(hipGetLastError());
+        TG_TRY(luset_solve(pc->lu, c->stream, pk, cols, adjoint));
+        k_pc_scatter<<<grid, 256, 0, c->stream>>>(pc->d_members, c->M, c->H, c->n, c->nh, ld, cols, pk, y);
         TG_HIP(hipGetLastError());
     }
     TG_TRY(c->timer.end("pc_apply", c->stream, rec));
@@ -330,6 +256,8 @@ void destroy_preconditioner(tg_cluster *c) {
     if (!c->pc) return;
     c->pc->mats.release();
     c->pc->meta.release();
+    luset_release(c->pc->lu);
+    c->pc->packed.release();
     learned_destroy(c->pc);
     delete c->pc;
     c->pc = nullptr;
