@@ -58,8 +58,8 @@ typedef enum {
 /* Preconditioner of the GMRES solves (right preconditioning; no reference counterpart). */
 typedef enum {
     TG_PC_NONE = 0,            /* the reference's unpreconditioned GMRES */
-    TG_PC_BLOCK_JACOBI = 1,    /* exact inverses of spatial sphere groups */
-    TG_PC_LEARNED = 2          /* block-Jacobi plus MLP-weighted near-field coupling */
+    TG_PC_BLOCK_JACOBI = 1,    /* exact (LU) inverses of spatial sphere groups */
+    TG_PC_LEARNED = 2          /* near-field Neumann correction with MLP-weighted pair blocks */
 } tg_preconditioner;
 
 /* linalg/gmres.rs:22-43 plus the precision and preconditioner choices of this library. */

@@ -199,13 +199,29 @@ struct OutBlocks {
 
 // Assembly: one workgroup per (source unit, group of P destination units).  The
 // harmonic tables of the P pairs sit in LDS; each thread owns block entries and
-// reuses every plan term it loads across the P pairs (FP64 accumulation).
-template <int P, class Geo, class Out, bool REG>
+// reuses every plan term it loads across the P pairs, accumulating in FP64.  The
+// radial and Legendre recurrences run in FP64; with TabT = float2 the finished tables
+// are staged in LDS rounded to FP32 (for the c64 stored coupling, whose entries are
+// rounded to FP32 anyway), which halves the LDS traffic that bounds the contraction.
+template <class TabT> __device__ __forceinline__ cd tab_get(const TabT *t, int i);
+template <> __device__ __forceinline__ cd tab_get<double2>(const double2 *t, int i) {
+    const double2 v = t[i];
+    return cd(v.x, v.y);
+}
+template <> __device__ __forceinline__ cd tab_get<float2>(const float2 *t, int i) {
+    const float2 v = t[i];
+    return cd((double)v.x, (double)v.y);
+}
+
+template <int P, class Geo, class Out, bool REG, class TabT = double2>
 __global__ __launch_bounds__(256) void k_assemble(DevPlan pl, double k, Geo geo, Out out) {
     extern __shared__ cd smem[];
     const int T = pl.T, L2 = pl.L2, H = pl.H;
-    cd *tab = smem;
+    constexpr bool copy = sizeof(TabT) != sizeof(cd);
+    cd *tab = smem;                 // FP64 tables [P][T] while they are built
     cd *rad = smem + P * T;
+    // [P][T] tables read by the contraction: the FP64 tables themselves, or an FP32 copy
+    TabT *tabs = copy ? reinterpret_cast<TabT *>(rad + P * (L2 + 1)) : reinterpret_cast<TabT *>(tab);
     __shared__ double su[P][3];
     __shared__ int flag[P];
     const int col = blockIdx.x;
@@ -234,6 +250,15 @@ __global__ __launch_bounds__(256) void k_assemble(DevPlan pl, double k, Geo geo,
         }
     }
     __syncthreads();
+    if (copy) {
+        for (int t = threadIdx.x; t < P * T; t += blockDim.x) {
+            TabT v;
+            v.x = tab[t].re;
+            v.y = tab[t].im;
+            tabs[t] = v;
+        }
+        __syncthreads();
+    }
     const int HH = H * H;
     for (int e = threadIdx.x; e < HH; e += blockDim.x) {
         const int a = e / H, b = e - a * H;
@@ -245,13 +270,13 @@ __global__ __launch_bounds__(256) void k_assemble(DevPlan pl, double k, Geo geo,
             const double wt = pl.w[t];
             const int h = pl.hidx[t];
 #pragma unroll
-            for (int il = 0; il < P; ++il) same[il] += wt * tab[il * T + h];
+            for (int il = 0; il < P; ++il) same[il] += wt * tab_get(tabs, il * T + h);
         }
         for (int t = s0 + ns; t < s1; ++t) {
             const double wt = pl.w[t];
             const int h = pl.hidx[t];
 #pragma unroll
-            for (int il = 0; il < P; ++il) cross[il] += wt * tab[il * T + h];
+            for (int il = 0; il < P; ++il) cross[il] += wt * tab_get(tabs, il * T + h);
         }
 #pragma unroll
         for (int il = 0; il < P; ++il) {
@@ -263,7 +288,21 @@ __global__ __launch_bounds__(256) void k_assemble(DevPlan pl, double k, Geo geo,
     }
 }
 
-static size_t assemble_smem(const DevPlan &pl, int P) { return (size_t)P * (pl.T + pl.L2 + 1) * sizeof(cd); }
+// FP64 tables and radial values, then the contraction copy of the tables (TabT)
+template <class TabT = double2>
+static size_t assemble_smem(const DevPlan &pl, int P) {
+    const size_t copy = sizeof(TabT) != sizeof(cd) ? (size_t)P * pl.T * sizeof(TabT) : 0;
+    return (size_t)P * (pl.T + pl.L2 + 1) * sizeof(cd) + copy;
+}
+
+// Dynamic LDS of one assembly workgroup must fit the 64 KB a workgroup may allocate.
+static int check_smem(size_t bytes, int lmax) {
+    if (bytes > 65536) {
+        set_error("assembly at lmax %d needs %zu bytes of LDS per workgroup (limit 65536)", lmax, bytes);
+        return TG_INVALID_INPUT;
+    }
+    return TG_OK;
+}
 
 int launch_assemble_split(tg_cluster *c, int storage, void *outp) {
     constexpr int P = 8;
@@ -272,10 +311,17 @@ int launch_assemble_split(tg_cluster *c, int storage, void *outp) {
     size_t smem = assemble_smem(c->plan, P);
     Timer::Rec rec;
     TG_TRY(c->timer.begin("assemble", c->stream, rec));
+    TG_TRY(check_smem(smem, c->lmax));
     if (storage == TG_STORE_C64) {
         float2 *base = (float2 *)outp;
         OutSplit<float2> out{base, base + c->nh * c->nh, c->nh, c->H};
-        k_assemble<P, GeoCluster, OutSplit<float2>, false><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo, out);
+        const size_t smem32 = assemble_smem<float2>(c->plan, P);
+        if (smem32 <= 65536)
+            k_assemble<P, GeoCluster, OutSplit<float2>, false, float2>
+                <<<grid, 256, smem32, c->stream>>>(c->plan, c->k0, geo, out);
+        else
+            k_assemble<P, GeoCluster, OutSplit<float2>, false><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo,
+                                                                                               out);
     } else {
         double2 *base = (double2 *)outp;
         OutSplit<double2> out{base, base + c->nh * c->nh, c->nh, c->H};
@@ -291,11 +337,18 @@ int launch_assemble_operator(tg_cluster *c, int precision, void *outp) {
     dim3 grid(c->N, (c->N + P - 1) / P);
     GeoCluster geo{c->b_pos.as<double>(), c->N};
     size_t smem = assemble_smem(c->plan, P);
+    TG_TRY(check_smem(smem, c->lmax));
     Timer::Rec rec;
     TG_TRY(c->timer.begin("assemble_operator", c->stream, rec));
     if (precision == TG_STORE_C64) {
         OutOperator<float2> out{(float2 *)outp, c->n, c->nh, c->H, c->lmax, c->b_mie.as<cd>()};
-        k_assemble<P, GeoCluster, OutOperator<float2>, false><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo, out);
+        const size_t smem32 = assemble_smem<float2>(c->plan, P);
+        if (smem32 <= 65536)
+            k_assemble<P, GeoCluster, OutOperator<float2>, false, float2>
+                <<<grid, 256, smem32, c->stream>>>(c->plan, c->k0, geo, out);
+        else
+            k_assemble<P, GeoCluster, OutOperator<float2>, false><<<grid, 256, smem, c->stream>>>(c->plan, c->k0,
+                                                                                                  geo, out);
     } else {
         OutOperator<double2> out{(double2 *)outp, c->n, c->nh, c->H, c->lmax, c->b_mie.as<cd>()};
         k_assemble<P, GeoCluster, OutOperator<double2>, false><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo,
@@ -313,6 +366,7 @@ int launch_assemble_ref(tg_cluster *c, int regular, cd *outp) {
     GeoCluster geo{c->b_pos.as<double>(), c->N};
     OutRef out{(double2 *)outp, c->n, c->M};
     size_t smem = assemble_smem(c->plan, P);
+    TG_TRY(check_smem(smem, c->lmax));
     if (regular)
         k_assemble<P, GeoCluster, OutRef, true><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo, out);
     else
@@ -330,6 +384,7 @@ int launch_translation_blocks(int lmax, const DevPlan &pl, double k, int64_t cou
     GeoList geo{disp};
     OutBlocks out{(double2 *)outp, M};
     size_t smem = assemble_smem(pl, 1);
+    TG_TRY(check_smem(smem, lmax));
     if (regular)
         k_assemble<1, GeoList, OutBlocks, true><<<grid, 256, smem, s>>>(pl, k, geo, out);
     else
