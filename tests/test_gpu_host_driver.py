@@ -21,7 +21,8 @@ def test_host_driver_against_oracle(oracle):
     assert os.path.exists(DRIVER), "cluster_driver is not built (treams-rs_amd/build.py)"
     lmax, k0 = 3, 1.3
     radii, eps, pos = pkg.random_cluster(8, 2.0, 0.1, (0.2, 0.3), 4 + 0.1j, seed=0)
-    lines = "".join(f"{r!r} {e.real!r} {e.imag!r} {p[0]!r} {p[1]!r} {p[2]!r}\n" for r, e, p in zip(radii, eps, pos))
+    lines = "".join(" ".join(f"{float(v):.17g}" for v in (r, e.real, e.imag, *p)) + "\n"
+                    for r, e, p in zip(radii, eps, pos))
     out = subprocess.run([DRIVER, str(lmax), repr(k0)], input=lines, capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     res = json.loads(out.stdout)

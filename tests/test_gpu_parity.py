@@ -179,6 +179,20 @@ def test_preconditioned_solve_same_answer(pkg, oracle, pc):
     assert c1[0][0] <= c0[0][0]
 
 
+def test_block_jacobi_ragged_groups(pkg, oracle):
+    """11 spheres in groups of at most 4 (sizes differ): the batched LU factors
+    matrices of different dimensions in one set of launches; the preconditioned
+    solve matches the dense solution, for the forward and the adjoint system."""
+    cl, ref = _cluster(pkg, oracle, N=11, lmax=3, seed=17, side=3.2)
+    cl.set_group_size(4)
+    a = cl.plane_wave([0, 0, 1], 1)
+    x, conv = cl.solve(a, rtol=1e-12, preconditioner="block_jacobi")
+    assert _rel(x, ref.solve(a.cpu().numpy())) <= 1e-10
+    g = np.random.default_rng(5).normal(size=ref.n) + 0j
+    y, _ = cl.solve(g, rtol=1e-12, preconditioner="block_jacobi", adjoint=True)
+    assert _rel(y, np.linalg.solve(ref.operator().conj().T, g)) <= 1e-10
+
+
 def test_not_converged_raises(pkg, oracle):
     cl, _ = _cluster(pkg, oracle, N=8, lmax=3, seed=5, side=2.0)
     a = cl.plane_wave([0, 0, 1], 1)

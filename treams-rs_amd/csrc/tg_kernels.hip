@@ -556,7 +556,7 @@ __global__ void k_pack_sources(const cd *x, int64_t n, int64_t nh, int N, int H,
 // one 64 x 64 output tile per workgroup, batch z = grid.z with element strides sA,
 // sB, sC.  3M form on v_mfma_f64_16x16x4f64.
 template <bool KTAIL>
-__global__ __launch_bounds__(256) void k_zgemm(const cd *A, int64_t lda, int64_t sA, const cd *B, int64_t ldb,
+__global__ __launch_bounds__(256, 2) void k_zgemm(const cd *A, int64_t lda, int64_t sA, const cd *B, int64_t ldb,
                                                int64_t sB, cd *C, int64_t ldc, int64_t sC, int64_t M, int64_t N,
                                                int K) {
     const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;

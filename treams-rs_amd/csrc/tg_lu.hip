@@ -181,10 +181,9 @@ __global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A0, Batch bt, i
 }
 
 // A22 -= L21 U12 with the 3M complex product on MFMA, one 64 x 64 tile of A22 per
-// workgroup.  The tile's A22 values are loaded into registers before the K loop so
-// their latency overlaps the MFMAs; rows and columns beyond n are never written.
+// workgroup; rows and columns beyond n are never written.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_update(cx<T> *A0, Batch bt, int64_t k0) {
+__global__ __launch_bounds__(256, 2) void k_lu_update(cx<T> *A0, Batch bt, int64_t k0) {
     const int z = blockIdx.z;
     const int64_t n = bt.n[z];
     const int nb = NB;
@@ -194,20 +193,27 @@ __global__ __launch_bounds__(256) void k_lu_update(cx<T> *A0, Batch bt, int64_t 
     __shared__ TileLds<T> sm;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, rr = lane & 15;
+    // c64: the tile's A22 values are loaded before the K loop (their latency overlaps the
+    // MFMAs); c128 loads them afterwards, which keeps two waves per SIMD without spills
+    constexpr bool prefetch = sizeof(T) == 4;
     cx<T> creg[2][2][4];
+    auto load_c = [&] {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t gr = min(r0 + wm + a * 16 + rr, n - 1);
-                const int64_t gc = min(c0 + wn + b * 16 + mfma_out_row<T>(lane, v), n - 1);
-                creg[a][b][v] = A[gc * n + gr];
-            }
+                for (int v = 0; v < 4; ++v) {
+                    const int64_t gr = min(r0 + wm + a * 16 + rr, n - 1);
+                    const int64_t gc = min(c0 + wn + b * 16 + mfma_out_row<T>(lane, v), n - 1);
+                    creg[a][b][v] = A[gc * n + gr];
+                }
+    };
+    if (prefetch) load_c();
     Tile3M<T> acc;
     // L21 = A[:, k0 : k0 + nb] (rows absolute), U12 = A[k0 : k0 + nb, :] (columns absolute)
     zgemm_tile_3m<T, false>(A + k0 * n, n, A + k0, n, r0, c0, n, n, nb, sm, acc);
+    if (!prefetch) load_c();
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
