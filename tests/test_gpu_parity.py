@@ -256,3 +256,19 @@ def test_pullback_matches_oracle(pkg, oracle):
     assert _rel(ginc, r["incident"]) <= 1e-9
     # zero total position gradient (translation invariance, iterative.rs:832-835)
     assert np.abs(gp.sum(axis=0)).max() <= 1e-9 * np.abs(gp).max()
+
+
+def test_pullback_many_columns_chunked(pkg, oracle):
+    """20 illuminations at lmax 6: the pair-gradient kernel takes the columns in LDS-sized
+    chunks and accumulates them in a fixed order; the result matches the oracle."""
+    cl, ref = _cluster(pkg, oracle, N=3, lmax=6, seed=3, side=1.8, eps=None)
+    rng = np.random.default_rng(4)
+    a = rng.normal(size=(ref.n, 20)) + 1j * rng.normal(size=(ref.n, 20))
+    f = pkg.SphereClusterFactor(cl, "c128")
+    x, ctx, _ = f.record(a)
+    G = rng.normal(size=a.shape) + 1j * rng.normal(size=a.shape)
+    gk, gr, ge, gp, ginc, _ = ctx.pullback(G)
+    r = ref.pullback(a, np.linalg.solve(ref.operator(), oracle.apply_local(ref.blocks, a)), G)
+    assert np.abs(gp - r["positions"]).max() <= 1e-9 * np.abs(r["positions"]).max()
+    assert np.abs(gr - r["radii"]).max() <= 1e-9 * np.abs(r["radii"]).max()
+    assert abs(gk - r["k0"]) <= 1e-9 * abs(r["k0"])

@@ -6,6 +6,7 @@
 // C-bar restricted to a pair block is contracted with the conjugate plan weights
 // into one cotangent per harmonic (sw/plan.rs:205-213), which pulls back through the
 // derivative table of z_p(k r) Y_p^q(d/r) to the displacement and k (sw/plan.rs:596-615).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -34,13 +35,13 @@ __device__ __forceinline__ int unit_dir(const double d[3], double &r, double u[3
 // cotangent sum_c U_i[c] V_j[c]^H (zero for i == j).
 __global__ __launch_bounds__(256) void k_pair_gradient(DevPlan pl, double k, const double *pos, int N, int cols,
                                                        const cd *U, const cd *V, int64_t n, int64_t nh, int regular,
-                                                       double *out) {
+                                                       double *out, int accumulate) {
     extern __shared__ cd sm[];
     const int H = pl.H, L2 = pl.L2, T = pl.T;
     const int i = blockIdx.y, j = blockIdx.x;
     double *o = out + ((int64_t)i * N + j) * 4;
     if (i == j) {
-        if (threadIdx.x < 4) o[threadIdx.x] = 0.0;
+        if (threadIdx.x < 4 && !accumulate) o[threadIdx.x] = 0.0;
         return;
     }
     cd *us = sm;                  // [cols][2][H]
@@ -156,22 +157,29 @@ __global__ __launch_bounds__(256) void k_pair_gradient(DevPlan pl, double k, con
             for (int c = 0; c < 4; ++c) red[c][tid] += red[c][tid + s];
         __syncthreads();
     }
-    if (tid < 4) o[tid] = red[tid][0];
+    if (tid < 4) o[tid] = (accumulate ? o[tid] : 0.0) + red[tid][0];
 }
 
-// grad_pos[i] = sum_j g(i, j) - sum_j g(j, i), in j order; gk = sum over all pairs.
-__global__ void k_pair_reduce(const double *pg, int N, double *grad_pos, double *gk_out) {
+// grad_pos[i] = sum_j g(i, j) - sum_j g(j, i), in j order; gk_rows[i] = sum_j of the
+// k parts of row i, in j order.
+__global__ void k_pair_reduce(const double *pg, int N, double *grad_pos, double *gk_rows) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) {
-        double g[3] = {0, 0, 0};
-        for (int j = 0; j < N; ++j)
-            for (int a = 0; a < 3; ++a) g[a] += pg[((int64_t)i * N + j) * 4 + a] - pg[((int64_t)j * N + i) * 4 + a];
-        for (int a = 0; a < 3; ++a) grad_pos[3 * i + a] = g[a];
+    if (i >= N) return;
+    double g[3] = {0, 0, 0}, gk = 0.0;
+    for (int j = 0; j < N; ++j) {
+        for (int a = 0; a < 3; ++a) g[a] += pg[((int64_t)i * N + j) * 4 + a] - pg[((int64_t)j * N + i) * 4 + a];
+        gk += pg[((int64_t)i * N + j) * 4 + 3];
     }
-    if (i == 0) {
+    for (int a = 0; a < 3; ++a) grad_pos[3 * i + a] = g[a];
+    gk_rows[i] = gk;
+}
+
+// gk = sum_i gk_rows[i] in i order (one thread: N values).
+__global__ void k_sum_in_order(const double *v, int N, double *out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         double s = 0.0;
-        for (int64_t t = 0; t < (int64_t)N * N; ++t) s += pg[t * 4 + 3];
-        *gk_out = s;
+        for (int i = 0; i < N; ++i) s += v[i];
+        *out = s;
     }
 }
 
@@ -179,18 +187,27 @@ int pair_gradients(tg_cluster *c, const cd *u, const cd *v, int cols, int regula
                    double *grad_k_dev) {
     const int H = c->H;
     DevBuf pg;
-    TG_TRY(pg.ensure(sizeof(double) * 4 * (size_t)c->N * c->N));
-    for (int c0 = 0; c0 < cols; c0 += 8) {
-        (void)c0;
-    }
-    size_t smem = sizeof(cd) * ((size_t)cols * 4 * H + 4 * (size_t)c->plan.T + c->plan.L2 + 2);
+    TG_TRY(pg.ensure(sizeof(double) * (4 * (size_t)c->N * c->N + c->N)));
+    double *gk_rows = pg.as<double>() + 4 * (size_t)c->N * c->N;
+    // columns per launch so that the cotangent vectors and tables fit the LDS left
+    // beside the kernel's static reduction arrays (8 KB + a few words); later chunks
+    // accumulate into the same pair gradients (fixed order)
+    const size_t fixed = sizeof(cd) * (4 * (size_t)c->plan.T + c->plan.L2 + 2);
+    const size_t budget = 65536 - 8192 - 256;
+    const int cmax = (int)std::max<size_t>(1, (budget - fixed) / (sizeof(cd) * 4 * H));
     Timer::Rec rec;
     TG_TRY(c->timer.begin("pair_gradient", c->stream, rec));
-    k_pair_gradient<<<dim3(c->N, c->N), 256, smem, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N, cols, u,
-                                                                v, c->n, c->nh, regular, pg.as<double>());
-    TG_HIP(hipGetLastError());
+    for (int c0 = 0; c0 < cols; c0 += cmax) {
+        const int nc = std::min(cmax, cols - c0);
+        const size_t smem = fixed + sizeof(cd) * (size_t)nc * 4 * H;
+        k_pair_gradient<<<dim3(c->N, c->N), 256, smem, c->stream>>>(
+            c->plan, c->k0, c->b_pos.as<double>(), c->N, nc, u + (int64_t)c0 * c->n, v + (int64_t)c0 * c->n, c->n,
+            c->nh, regular, pg.as<double>(), c0 > 0);
+        TG_HIP(hipGetLastError());
+    }
     TG_TRY(c->timer.end("pair_gradient", c->stream, rec));
-    k_pair_reduce<<<(c->N + 63) / 64, 64, 0, c->stream>>>(pg.as<double>(), c->N, grad_pos_dev, grad_k_dev);
+    k_pair_reduce<<<(c->N + 63) / 64, 64, 0, c->stream>>>(pg.as<double>(), c->N, grad_pos_dev, gk_rows);
+    k_sum_in_order<<<1, 64, 0, c->stream>>>(gk_rows, c->N, grad_k_dev);
     TG_HIP(hipGetLastError());
     TG_HIP(hipStreamSynchronize(c->stream));
     pg.release();

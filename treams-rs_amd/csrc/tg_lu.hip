@@ -64,9 +64,15 @@ __global__ __launch_bounds__(64) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0,
     const int nb = (int)min((int64_t)NB, n - k0);
     __shared__ lcx<T> blk[NB][NB];  // [col][row]
     const int lane = threadIdx.x;
-    for (int t = lane; t < nb * nb; t += 64) {
-        const int col = t / nb, row = t - col * nb;
-        blk[col][row].set(A[(k0 + col) * n + k0 + row]);
+    // lane r loads row r of 16 columns at a time (16 loads in flight), clamped rows for r >= nb
+    const int64_t row = k0 + min(lane, nb - 1);
+    for (int c0 = 0; c0 < nb; c0 += 16) {
+        cx<T> v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = A[(k0 + min(c0 + u, nb - 1)) * n + row];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (c0 + u < nb && lane < nb) blk[c0 + u][lane].set(v[u]);
     }
     __syncthreads();
     for (int k = 0; k < nb; ++k) {
@@ -144,10 +150,27 @@ __global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A0, Batch bt, i
     __syncthreads();
     const int64_t r = k0 + nb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    for (int c = 0; c < nb; ++c) {
-        cx<T> s = A[(k0 + c) * n + r];
-        for (int k = 0; k < c; ++k) s -= A[(k0 + k) * n + r] * U[c][k].get();
-        A[(k0 + c) * n + r] = s / U[c][c].get();
+    // x U11 = a for this row.  c64: the row is held in registers (all 64 loads in
+    // flight; the launch has at most a few waves per SIMD anyway); c128 re-reads the
+    // solved entries, which stay in L1.
+    if constexpr (sizeof(T) == 4) {
+        cx<T> x[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) x[c] = A[(k0 + c) * n + r];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            cx<T> s = x[c];
+#pragma unroll
+            for (int k = 0; k < c; ++k) s -= x[k] * U[c][k].get();
+            x[c] = s / U[c][c].get();
+            A[(k0 + c) * n + r] = x[c];
+        }
+    } else {
+        for (int c = 0; c < nb; ++c) {
+            cx<T> s = A[(k0 + c) * n + r];
+            for (int k = 0; k < c; ++k) s -= A[(k0 + k) * n + r] * U[c][k].get();
+            A[(k0 + c) * n + r] = s / U[c][c].get();
+        }
     }
 }
 
