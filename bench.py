@@ -44,8 +44,10 @@ def rhs_columns(cl, torch):
     return torch.stack(cols, dim=1)
 
 
-def run_method(cl, B, torch, *, mixed, pc, steps, warmup, rtol, restart, label):
-    """Time `steps` whole-job steps (storage + preconditioner + solve)."""
+def run_method(cl, B, torch, *, mixed, pc, steps, warmup, rtol, restart, label, stored=True):
+    """Time `steps` whole-job steps (storage or tables + preconditioner + solve).  FP64
+    solves (mixed = False) run on the stored c128 coupling, or matrix-free with
+    stored = False."""
     conv = None
     x = None
     for w in range(warmup):
@@ -60,7 +62,7 @@ def run_method(cl, B, torch, *, mixed, pc, steps, warmup, rtol, restart, label):
     for s in range(steps):
         ts = time.perf_counter()
         cl.release()
-        if not mixed:
+        if not mixed and stored:
             cl.assemble("c128")
         x, conv = cl.solve(B, rtol=rtol, restart=restart, max_iterations=2000, mixed_precision=mixed,
                            preconditioner=pc, convergence="full")
@@ -71,7 +73,7 @@ def run_method(cl, B, torch, *, mixed, pc, steps, warmup, rtol, restart, label):
     elapsed = time.perf_counter() - t0
     cl.timing(False)
     kernels = {}
-    for name in ("assemble", "matvec_stored", "coupling_mf", "pc_apply", "pc_factor"):
+    for name in ("assemble", "matvec_stored", "coupling_mf", "coupling_mf32", "pc_apply", "pc_factor"):
         ms, count = cl.timing_get(name)
         if count:
             kernels[name] = {"total_ms": ms, "launches": count, "avg_ms": ms / count}
@@ -184,6 +186,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=0, help="destination spheres per CPU apply (0: all)")
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--no-lu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="bytes of a plain device copy run first (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -282,14 +285,32 @@ def main() -> int:
     }
 
     ref_iterations = None
-    if not args.no_plain:
+    if not args.no_variants:
+        # the same learned-PC mixed solve with the matrix-free c64 inner operator (c64 pair
+        # tables on FP32 MFMA) instead of the stored coupling: no assembly, no 38.65 GB
         cl.release()
-        x64, conv64, el64, k64 = run_method(cl, B, torch, mixed=False, pc="block_jacobi", steps=1, warmup=0,
-                                            rtol=args.rtol, restart=args.restart, label="plain-fp64-bj")
-        plain = ncols / el64
-        result["plain_fp64_block_jacobi"] = {"value": plain, "unit": "solves/s", "seconds": el64,
-                                             "iterations": [c.iterations for c in conv64], "kernels": k64}
+        xv, convv, elv, kv = run_method(cl, B, torch, mixed="matrix_free", pc=args.pc, steps=args.steps,
+                                        warmup=args.warmup, rtol=args.rtol, restart=args.restart,
+                                        label="mixed-matrix-free")
+        result["mixed_matrix_free_c64"] = {
+            "value": ncols * args.steps / elv, "unit": "solves/s", "ms_per_step": 1e3 * elv / args.steps,
+            "iterations": [c.iterations for c in convv], "refinements": [c.refinements for c in convv],
+            "true_relative_residual": [c.residual_norm / c.rhs_norm for c in convv], "kernels": kv,
+            "note": "learned PC, inner GMRES on the matrix-free c64 operator (FP32 MFMA), FP64 refinement"}
+    if not args.no_plain:
+        plains = {}
+        for stored, key in ((True, "plain_fp64_block_jacobi"), (False, "plain_fp64_block_jacobi_matrix_free")):
+            cl.release()
+            x64, conv64, el64, k64 = run_method(cl, B, torch, mixed=False, pc="block_jacobi", steps=1, warmup=0,
+                                                rtol=args.rtol, restart=args.restart,
+                                                label="plain-fp64-bj" + ("" if stored else "-mf"), stored=stored)
+            plains[key] = ncols / el64
+            result[key] = {"value": ncols / el64, "unit": "solves/s", "seconds": el64,
+                           "iterations": [c.iterations for c in conv64], "kernels": k64,
+                           "operator": "stored c128 coupling" if stored else "FP64 matrix-free (pair-table GEMMs)"}
+        plain = max(plains.values())
         result["speedup_vs_plain_fp64_block_jacobi"] = value / plain
+        result["speedup_note"] = "against the faster of the two plain FP64 block-Jacobi variants"
         # cross sections of the mixed solution against the FP64 solution
         errs = []
         for c in range(ncols):

@@ -52,7 +52,8 @@ typedef enum { TG_STORE_NONE = 0, TG_STORE_C64 = 1, TG_STORE_C128 = 2 } tg_stora
 /* Operator used by tg_cluster_apply / the solvers. */
 typedef enum {
     TG_OP_MATRIX_FREE_F64 = 0, /* recompute each pair block in FP64 (cluster/iterative.rs:187-221) */
-    TG_OP_STORED = 1           /* stream the stored coupling (c64 or c128) */
+    TG_OP_STORED = 1,          /* stream the stored coupling (c64 or c128) */
+    TG_OP_MATRIX_FREE_C64 = 2  /* matrix-free from c64 pair tables on FP32 MFMA (no reference counterpart) */
 } tg_operator;
 
 /* Preconditioner of the GMRES solves (right preconditioning; no reference counterpart). */
@@ -68,7 +69,8 @@ typedef struct {
     double atol;            /* default 0 */
     int restart;            /* default 30 */
     int max_iterations;     /* default 300 */
-    int mixed_precision;    /* 1: c64 stored inner solves refined by FP64 matrix-free residuals */
+    int mixed_precision;    /* 1: inner solves on the c64 stored coupling, 2: inner solves on the
+                               matrix-free c64 operator (FP32 MFMA); both refined by FP64 residuals */
     int preconditioner;     /* tg_preconditioner */
 } tg_gmres_options;
 

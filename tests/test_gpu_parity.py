@@ -110,6 +110,33 @@ def test_operator_apply_matrix_free_and_stored(pkg, oracle):
     assert _rel(cl.apply(x[:, 0], stored=True), ref.apply(x[:, 0])) <= 2e-6
 
 
+def test_operator_apply_matrix_free_c64(pkg, oracle):
+    """The c64 matrix-free operator (c64 tables, FP32 MFMA, FP64 contraction) against
+    the FP64 oracle: single-precision level, forward and adjoint, ragged and full
+    GEMM tiles (N = 6 and N = 32)."""
+    for N, seed in ((6, 3), (32, 9)):
+        cl, ref = _cluster(pkg, oracle, N=N, lmax=3, seed=seed, side=0.9 * N ** (1 / 3) * 1.6)
+        rng = np.random.default_rng(N)
+        x = rng.normal(size=(ref.n, 3)) + 1j * rng.normal(size=(ref.n, 3))
+        assert _rel(cl.apply(x, c64=True), ref.apply(x)) <= 2e-6
+        assert _rel(cl.apply(x, adjoint=True, c64=True), ref.apply(x, adjoint=True)) <= 2e-6
+        assert _rel(cl.coupling(x, c64=True), ref.C @ x) <= 1e-5
+
+
+def test_solve_mixed_matrix_free_c64(pkg, oracle):
+    cl, ref = _cluster(pkg, oracle, N=12, lmax=4, seed=8, side=3.5)
+    a = cl.plane_wave([0, 0, 1], 1)
+    cl.timing(True)
+    x, conv = cl.solve(a, rtol=1e-10, restart=50, mixed_precision="matrix_free", convergence="full")
+    cl.timing(False)
+    assert conv[0].residual_norm <= 1e-10 * conv[0].rhs_norm
+    assert conv[0].refinements >= 1
+    assert _rel(x, ref.solve(a.cpu().numpy())) <= 1e-8
+    # the inner solves ran on the c64 matrix-free operator, never on a stored coupling
+    assert cl.timing_get("coupling_mf32")[1] > 0
+    assert cl.timing_get("matvec_stored")[1] == 0
+
+
 def test_matrix_free_apply_larger_cluster(pkg, oracle):
     """N = 96, lmax = 6: pair-table matrix-free apply vs the oracle on a row subset
     and vs the stored c128 product (an independent kernel) on every row."""

@@ -21,7 +21,7 @@ TG_OUT_OF_MEMORY = 6
 TG_DEVICE_ERROR = 7
 
 TG_STORE_NONE, TG_STORE_C64, TG_STORE_C128 = 0, 1, 2
-TG_OP_MATRIX_FREE_F64, TG_OP_STORED = 0, 1
+TG_OP_MATRIX_FREE_F64, TG_OP_STORED, TG_OP_MATRIX_FREE_C64 = 0, 1, 2
 TG_PC_NONE, TG_PC_BLOCK_JACOBI, TG_PC_LEARNED = 0, 1, 2
 
 #: every symbol include/treams_gpu.h declares

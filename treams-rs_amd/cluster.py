@@ -136,12 +136,17 @@ class IterativeSphereCluster:
         check(fn(self._h, cm.data_ptr(), out.data_ptr(), cm.shape[0], *args))
         return _back(out, vector)
 
-    def apply(self, x, adjoint: bool = False, stored: bool = False):
-        """(I - T C) x or (I - C^H T^H) x (cluster/iterative.rs:225-232)."""
-        return self._apply(lib().tg_cluster_apply, x, int(adjoint), int(stored))
+    @staticmethod
+    def _op(stored: bool, c64: bool) -> int:
+        """tg_operator: FP64 matrix-free, stored coupling, or matrix-free c64 (FP32 MFMA)."""
+        return _lib.TG_OP_STORED if stored else (_lib.TG_OP_MATRIX_FREE_C64 if c64 else _lib.TG_OP_MATRIX_FREE_F64)
 
-    def coupling(self, x, adjoint: bool = False, stored: bool = False):
-        return self._apply(lib().tg_cluster_coupling, x, int(adjoint), int(stored))
+    def apply(self, x, adjoint: bool = False, stored: bool = False, c64: bool = False):
+        """(I - T C) x or (I - C^H T^H) x (cluster/iterative.rs:225-232)."""
+        return self._apply(lib().tg_cluster_apply, x, int(adjoint), self._op(stored, c64))
+
+    def coupling(self, x, adjoint: bool = False, stored: bool = False, c64: bool = False):
+        return self._apply(lib().tg_cluster_coupling, x, int(adjoint), self._op(stored, c64))
 
     def local(self, x, adjoint: bool = False):
         return self._apply(lib().tg_cluster_local, x, int(adjoint))
@@ -163,7 +168,10 @@ class IterativeSphereCluster:
     @staticmethod
     def options(rtol=DEFAULT_RTOL, atol=DEFAULT_ATOL, restart=DEFAULT_RESTART,
                 max_iterations=DEFAULT_MAX_ITERATIONS, mixed_precision=False, preconditioner="none") -> _Options:
-        return _Options(float(rtol), float(atol), int(restart), int(max_iterations), int(bool(mixed_precision)),
+        """mixed_precision: False (FP64), True or "stored" (inner solves on the stored c64
+        coupling) or "matrix_free" (inner solves on the matrix-free c64 operator)."""
+        mp = {False: 0, True: 1, "stored": 1, "matrix_free": 2}[mixed_precision]
+        return _Options(float(rtol), float(atol), int(restart), int(max_iterations), mp,
                         _PC[preconditioner])
 
     def solve(self, incident, *, rtol=DEFAULT_RTOL, atol=DEFAULT_ATOL, restart=DEFAULT_RESTART,

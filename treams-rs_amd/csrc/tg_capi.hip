@@ -145,7 +145,9 @@ int apply_operator(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, int
     cd *t1 = c->ws[6].as<cd>();
     cd *t2 = t1 + count;
     auto coupling = [&](const cd *in, cd *out) -> int {
-        return op == TG_OP_STORED ? coupling_stored(c, in, out, cols) : coupling_matrix_free(c, in, out, cols, 0);
+        if (op == TG_OP_STORED) return coupling_stored(c, in, out, cols);
+        if (op == TG_OP_MATRIX_FREE_C64) return coupling_matrix_free32(c, in, out, cols);
+        return coupling_matrix_free(c, in, out, cols, 0);
     };
     if (!adjoint) {
         TG_TRY(coupling(x, t1));
@@ -296,6 +298,8 @@ int tg_cluster_destroy(tg_cluster *c) {
     c->b_C.release();
     c->b_tab.release();
     c->have_tables = false;
+    c->b_tab32.release();
+    c->have_tables32 = false;
     learned_release(c);
     for (auto &w : c->ws) w.release();
     c->timer.collect();
@@ -364,6 +368,8 @@ int tg_cluster_release(tg_cluster *c) {
     c->b_C.release();
     c->b_tab.release();
     c->have_tables = false;
+    c->b_tab32.release();
+    c->have_tables32 = false;
     c->storage = TG_STORE_NONE;
     learned_release(c);
     for (auto &w : c->ws) w.release();
@@ -402,7 +408,9 @@ int tg_cluster_coupling(tg_cluster *c, const tg_c128 *x, tg_c128 *y, int cols, i
     TG_TRY(check_cluster(c));
     return with_split(c, x, y, cols, [&](const cd *xs, cd *ys) -> int {
         auto cp = [&](const cd *in, cd *out) -> int {
-            return op == TG_OP_STORED ? coupling_stored(c, in, out, cols) : coupling_matrix_free(c, in, out, cols, 0);
+            if (op == TG_OP_STORED) return coupling_stored(c, in, out, cols);
+            if (op == TG_OP_MATRIX_FREE_C64) return coupling_matrix_free32(c, in, out, cols);
+            return coupling_matrix_free(c, in, out, cols, 0);
         };
         if (!adjoint) return cp(xs, ys);
         const int64_t count = (int64_t)cols * c->n;
@@ -483,10 +491,12 @@ int solve_split(tg_cluster *c, const cd *b, cd *x, int cols, const tg_gmres_opti
         int st = gmres_run(c, A, pc, b, x, cols, prm, conv, &its);
         return st;
     }
-    // Mixed precision: inner GMRES on the c64 stored operator, outer FP64 residuals.
-    if (c->storage != TG_STORE_C64) TG_TRY(tg_cluster_assemble(c, TG_STORE_C64));
-    OpFn A32 = [c, adjoint](const cd *in, cd *out, int nc) -> int {
-        return apply_operator(c, in, out, nc, adjoint, TG_OP_STORED);
+    // Mixed precision: inner GMRES on a c64 operator (the stored c64 coupling, or with
+    // mixed_precision == 2 the matrix-free coupling on FP32 MFMA), outer FP64 residuals.
+    const int op32 = o.mixed_precision == 2 ? TG_OP_MATRIX_FREE_C64 : TG_OP_STORED;
+    if (op32 == TG_OP_STORED && c->storage != TG_STORE_C64) TG_TRY(tg_cluster_assemble(c, TG_STORE_C64));
+    OpFn A32 = [c, adjoint, op32](const cd *in, cd *out, int nc) -> int {
+        return apply_operator(c, in, out, nc, adjoint, op32);
     };
     DevBuf outer;
     TG_TRY(outer.ensure(sizeof(cd) * count * 3));

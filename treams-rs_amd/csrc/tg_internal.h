@@ -80,8 +80,10 @@ struct tg_cluster {
     tg::DevBuf b_plan, b_pos, b_mie;   // b_mie: value, d/dsize, d/deps blocks
     int storage = TG_STORE_NONE;
     tg::DevBuf b_C;                    // stored split coupling: [2][nh][nh] (c64 or c128)
-    tg::DevBuf b_tab;                  // FP64 singular harmonic tables of the pairs i < j
+    tg::DevBuf b_tab;                  // FP64 singular harmonic tables TAB[h][j][i] of all ordered pairs
     bool have_tables = false;
+    tg::DevBuf b_tab32;                // the same tables rounded to c64 (FP32 MFMA coupling)
+    bool have_tables32 = false;
     tg::DevBuf ws[8];                  // scratch
     tg::Timer timer;
     tg::Preconditioner *pc = nullptr;
@@ -103,6 +105,7 @@ int launch_assemble_operator(tg_cluster *c, int precision, void *out);
 int launch_translation_blocks(int lmax, const DevPlan &plan, double k, int64_t count, const double *disp,
                               int regular, cd *out, hipStream_t s);
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular);
+int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols);
 int coupling_stored(tg_cluster *c, const cd *x, cd *y, int cols);
 int local_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *subtract_from);
 int ref_to_split(tg_cluster *c, const cd *ref, cd *split, int cols);

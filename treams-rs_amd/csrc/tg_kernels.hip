@@ -541,41 +541,50 @@ int ensure_pair_tables(tg_cluster *c) {
     return TG_OK;
 }
 
-// X[q][j] with q = (column cc, helicity h, mode a): the GEMM's right operand.
-__global__ void k_pack_sources(const cd *x, int64_t n, int64_t nh, int N, int H, int cols, cd *X) {
+// X[q][j] with q = (column cc, helicity h, mode a): the GEMM's right operand, in the
+// GEMM's precision.
+template <class T>
+__global__ void k_pack_sources(const cd *x, int64_t n, int64_t nh, int N, int H, int cols, cx<T> *X) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t Q = (int64_t)cols * 2 * H;
     if (t >= Q * N) return;
     const int64_t q = t / N;
     const int j = (int)(t - q * N);
     const int cc = (int)(q / (2 * H)), rem = (int)(q - (int64_t)cc * 2 * H), h = rem / H, a = rem - h * H;
-    X[t] = x[(int64_t)cc * n + (int64_t)h * nh + (int64_t)j * H + a];
+    const cd v = x[(int64_t)cc * n + (int64_t)h * nh + (int64_t)j * H + a];
+    X[t] = cx<T>((T)v.re, (T)v.im);
 }
 
-// C[z] = A[z] B[z] for column-major complex FP64 matrices (M x K times K x N),
-// one 64 x 64 output tile per workgroup, batch z = grid.z with element strides sA,
-// sB, sC.  3M form on v_mfma_f64_16x16x4f64.
-template <bool KTAIL>
-__global__ __launch_bounds__(256, 2) void k_zgemm(const cd *A, int64_t lda, int64_t sA, const cd *B, int64_t ldb,
-                                               int64_t sB, cd *C, int64_t ldc, int64_t sC, int64_t M, int64_t N,
-                                               int K) {
+// FP64 pair tables rounded to c64, for the FP32 MFMA coupling.
+__global__ void k_tables_to_c64(const cd *tab, int64_t count, cf *out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) out[t] = cf((float)tab[t].re, (float)tab[t].im);
+}
+
+// C[z] = A[z] B[z] for column-major complex matrices (M x K times K x N) in T =
+// double (FP64 MFMA) or float (FP32 MFMA), one 64 x 64 output tile per workgroup,
+// batch z = grid.z with element strides sA, sB, sC; 3M form.
+template <class T, bool KTAIL>
+__global__ __launch_bounds__(256, 2) void k_zgemm(const cx<T> *A, int64_t lda, int64_t sA, const cx<T> *B,
+                                                  int64_t ldb, int64_t sB, cx<T> *C, int64_t ldc, int64_t sC,
+                                                  int64_t M, int64_t N, int K) {
     const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
     const int64_t z = blockIdx.z;
-    __shared__ TileLds<double> sm;
-    Tile3M<double> acc;
-    zgemm_tile_3m<double, KTAIL>(A + z * sA, lda, B + z * sB, ldb, r0, c0, M, N, K, sm, acc);
+    __shared__ TileLds<T> sm;
+    Tile3M<T> acc;
+    zgemm_tile_3m<T, KTAIL>(A + z * sA, lda, B + z * sB, ldb, r0, c0, M, N, K, sm, acc);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, rr = lane & 15;
-    cd *Cz = C + z * sC;
+    cx<T> *Cz = C + z * sC;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<double>(lane, v);
-                const double t1 = acc.t1[a][b][v], t2 = acc.t2[a][b][v], t3 = acc.t3[a][b][v];
-                if (gr < M && gc < N) Cz[gc * ldc + gr] = cd(t1 - t2, t3 - t1 - t2);
+                const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<T>(lane, v);
+                const T t1 = acc.t1[a][b][v], t2 = acc.t2[a][b][v], t3 = acc.t3[a][b][v];
+                if (gr < M && gc < N) Cz[gc * ldc + gr] = cx<T>(t1 - t2, t3 - t1 - t2);
             }
 }
 
@@ -583,7 +592,8 @@ __global__ __launch_bounds__(256, 2) void k_zgemm(const cd *A, int64_t lda, int6
 // minus i times cross kind (h = 0: pol 1, +; h = 1: pol 0, -).  One thread per
 // destination sphere i (coalesced over i); the plan is read with wave-uniform
 // addresses.  grid.y = (cc, h, b).
-__global__ __launch_bounds__(64) void k_tab_contract(DevPlan pl, const cd *U, int N, int H, int cols, int64_t n,
+template <class T>
+__global__ __launch_bounds__(64) void k_tab_contract(DevPlan pl, const cx<T> *U, int N, int H, int cols, int64_t n,
                                                      int64_t nh, cd *y) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int yb = blockIdx.y;
@@ -595,48 +605,75 @@ __global__ __launch_bounds__(64) void k_tab_contract(DevPlan pl, const cd *U, in
     for (int a = 0; a < H; ++a) {
         const int e = a * H + b;
         const int s0 = pl.starts[e], s1 = pl.starts[e + 1], ns = pl.nsame[e];
-        const cd *Ua = U + (qbase + a) * N + i;
-        for (int t = s0; t < s0 + ns; ++t) same += pl.w[t] * Ua[(int64_t)pl.hidx[t] * Q * N];
-        for (int t = s0 + ns; t < s1; ++t) cross += pl.w[t] * Ua[(int64_t)pl.hidx[t] * Q * N];
+        const cx<T> *Ua = U + (qbase + a) * N + i;
+        for (int t = s0; t < s0 + ns; ++t) {
+            const cx<T> u = Ua[(int64_t)pl.hidx[t] * Q * N];
+            same += pl.w[t] * cd((double)u.re, (double)u.im);
+        }
+        for (int t = s0 + ns; t < s1; ++t) {
+            const cx<T> u = Ua[(int64_t)pl.hidx[t] * Q * N];
+            cross += pl.w[t] * cd((double)u.re, (double)u.im);
+        }
     }
     const cd v = h == 0 ? cd(same.re - cross.im, same.im + cross.re) : cd(same.re + cross.im, same.im - cross.re);
     y[(int64_t)cc * n + (int64_t)h * nh + (int64_t)i * H + b] = v;
 }
 
+// The tables in the GEMM precision: FP64 pair tables, or their c64 rounding (built
+// once from the FP64 tables and kept with them).
+template <class T> static const cx<T> *pair_tables(tg_cluster *c);
+template <> const cd *pair_tables<double>(tg_cluster *c) { return c->b_tab.as<cd>(); }
+template <> const cf *pair_tables<float>(tg_cluster *c) { return c->b_tab32.as<cf>(); }
+
+template <class R>
 static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols) {
     TG_TRY(ensure_pair_tables(c));
-    const int N = c->N, H = c->H, T = c->plan.T;
+    const int N = c->N, H = c->H, T = c->plan.T;  // T: harmonic indices
+    if (sizeof(R) == 4 && !c->have_tables32) {
+        const int64_t count = (int64_t)N * N * T;
+        TG_TRY(c->b_tab32.ensure(sizeof(cf) * (size_t)count));
+        k_tables_to_c64<<<grid1(count), 256, 0, c->stream>>>(c->b_tab.as<cd>(), count, c->b_tab32.as<cf>());
+        TG_HIP(hipGetLastError());
+        c->have_tables32 = true;
+    }
     const int cmax = 8;  // right-hand sides per pass
     const int64_t Qmax = (int64_t)std::min(cols, cmax) * 2 * H;
-    TG_TRY(c->ws[0].ensure(sizeof(cd) * ((size_t)Qmax * N + (size_t)T * Qmax * N)));
-    cd *X = c->ws[0].as<cd>();
-    cd *U = X + (size_t)Qmax * N;
+    TG_TRY(c->ws[0].ensure(sizeof(cx<R>) * ((size_t)Qmax * N + (size_t)T * Qmax * N)));
+    cx<R> *X = c->ws[0].as<cx<R>>();
+    cx<R> *U = X + (size_t)Qmax * N;
+    const cx<R> *TAB = pair_tables<R>(c);
     Timer::Rec rec;
-    TG_TRY(c->timer.begin("coupling_mf", c->stream, rec));
+    const char *name = sizeof(R) == 8 ? "coupling_mf" : "coupling_mf32";
+    TG_TRY(c->timer.begin(name, c->stream, rec));
     for (int c0 = 0; c0 < cols; c0 += cmax) {
         const int nc = std::min(cmax, cols - c0);
         const int64_t Q = (int64_t)nc * 2 * H;
-        k_pack_sources<<<grid1(Q * N), 256, 0, c->stream>>>(x + (int64_t)c0 * c->n, c->n, c->nh, N, H, nc, X);
+        k_pack_sources<R><<<grid1(Q * N), 256, 0, c->stream>>>(x + (int64_t)c0 * c->n, c->n, c->nh, N, H, nc, X);
         dim3 grid((unsigned)((Q + 63) / 64), (unsigned)((N + 63) / 64), (unsigned)T);
         if (N % 16)
-            k_zgemm<true><<<grid, 256, 0, c->stream>>>(c->b_tab.as<cd>(), N, (int64_t)N * N, X, N, 0, U, N, Q * N, N,
-                                                       Q, N);
+            k_zgemm<R, true><<<grid, 256, 0, c->stream>>>(TAB, N, (int64_t)N * N, X, N, 0, U, N, Q * N, N, Q, N);
         else
-            k_zgemm<false><<<grid, 256, 0, c->stream>>>(c->b_tab.as<cd>(), N, (int64_t)N * N, X, N, 0, U, N, Q * N, N,
-                                                        Q, N);
+            k_zgemm<R, false><<<grid, 256, 0, c->stream>>>(TAB, N, (int64_t)N * N, X, N, 0, U, N, Q * N, N, Q, N);
         TG_HIP(hipGetLastError());
-        k_tab_contract<<<dim3((unsigned)((N + 63) / 64), (unsigned)Q), 64, 0, c->stream>>>(
+        k_tab_contract<R><<<dim3((unsigned)((N + 63) / 64), (unsigned)Q), 64, 0, c->stream>>>(
             c->plan, U, N, H, nc, c->n, c->nh, y + (int64_t)c0 * c->n);
         TG_HIP(hipGetLastError());
     }
-    TG_TRY(c->timer.end("coupling_mf", c->stream, rec));
+    TG_TRY(c->timer.end(name, c->stream, rec));
     return TG_OK;
+}
+
+// C x with the c64 tables on FP32 MFMA (accumulated in FP32, contracted in FP64): the
+// inner operator of mixed-precision solves without a stored coupling.
+int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols) {
+    if (c->N < 2) return coupling_matrix_free(c, x, y, cols, 0);
+    return coupling_tables<float>(c, x, y, cols);
 }
 
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular) {
     // singular couplings reuse the cluster's pair tables; the regular overlap
     // (cross sections only) recomputes its tables per pair
-    if (!regular && c->N > 1) return coupling_tables(c, x, y, cols);
+    if (!regular && c->N > 1) return coupling_tables<double>(c, x, y, cols);
     const int H = c->H;
     const int G = 256 / H;
     const int block = G * H;
