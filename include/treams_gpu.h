@@ -51,7 +51,7 @@ typedef enum { TG_STORE_NONE = 0, TG_STORE_C64 = 1, TG_STORE_C128 = 2 } tg_stora
 
 /* Operator used by tg_cluster_apply / the solvers. */
 typedef enum {
-    TG_OP_MATRIX_FREE_F64 = 0, /* recompute each pair block in FP64 (cluster/iterative.rs:187-221) */
+    TG_OP_MATRIX_FREE_F64 = 0, /* FP64 matrix-free (cluster/iterative.rs:187-221): pair-table GEMMs */
     TG_OP_STORED = 1,          /* stream the stored coupling (c64 or c128) */
     TG_OP_MATRIX_FREE_C64 = 2  /* matrix-free from c64 pair tables on FP32 MFMA (no reference counterpart) */
 } tg_operator;
@@ -146,12 +146,22 @@ int tg_translation_blocks(int lmax, double k, int64_t count, const double *displ
 int tg_cluster_assemble(tg_cluster *cluster, int storage);
 
 /*
- * Free the stored coupling, the FP64 pair tables of the matrix-free operator and the
+ * Free the stored coupling, the pair tables of the matrix-free operators and the
  * preconditioners (including the learned one's near pairs; an attached network
  * stays attached); the next call that needs them rebuilds them.  No reference
  * counterpart (device memory management).
  */
 int tg_cluster_release(tg_cluster *cluster);
+
+/*
+ * Largest pair-table set the matrix-free operators keep, in bytes (default 64 GiB).
+ * One set holds N^2 (2 lmax + 1)^2 c128 values (2.8 GB at N = 1024, lmax = 6); the
+ * singular coupling, its c64 copy and the regular overlap each keep one.  Clusters
+ * whose set is larger recompute the translation coefficients of every pair in every
+ * apply (slower, no table memory).  Frees the current tables.  No reference
+ * counterpart (device memory policy).
+ */
+int tg_cluster_set_table_limit(tg_cluster *cluster, int64_t bytes);
 
 /*
  * y = (I - T C) x, or (I - C^H T^H) x with adjoint != 0, for `cols` column-major

@@ -163,6 +163,39 @@ def test_overlap_apply(pkg, oracle):
     assert _rel(cl.overlap(x, adjoint=True), ref.R.conj().T @ x) <= 1e-12
 
 
+def test_overlap_apply_tables_larger(pkg, oracle):
+    """Regular overlap from the regular pair tables (GEMM path) at N = 33 (ragged GEMM
+    tiles and K tail) against the oracle's dense R, both directions, 2 columns."""
+    cl, ref = _cluster(pkg, oracle, N=33, lmax=3, seed=13, side=5.0)
+    rng = np.random.default_rng(6)
+    x = rng.normal(size=(ref.n, 2)) + 1j * rng.normal(size=(ref.n, 2))
+    assert _rel(cl.overlap(x), ref.R @ x) <= 1e-12
+    assert _rel(cl.overlap(x, adjoint=True), ref.R.conj().T @ x) <= 1e-12
+
+
+def test_table_limit_selects_per_pair_path(pkg, oracle):
+    """With a table limit below one table set the matrix-free operators recompute the
+    translation coefficients per pair: same results as the table GEMMs and the
+    oracle (FP64), c64 solves included; a negative limit is rejected."""
+    cl, ref = _cluster(pkg, oracle, N=20, lmax=4, seed=17, side=4.0)
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=(ref.n, 2)) + 1j * rng.normal(size=(ref.n, 2))
+    with_tables = [cl.coupling(x), cl.coupling(x, adjoint=True), cl.overlap(x), cl.apply(x, c64=True)]
+    cl.set_table_limit(0)
+    per_pair = [cl.coupling(x), cl.coupling(x, adjoint=True), cl.overlap(x), cl.apply(x, c64=True)]
+    for a, b in zip(with_tables[:3], per_pair[:3]):
+        assert _rel(a, b) <= 1e-12
+    assert _rel(per_pair[0], ref.C @ x) <= 1e-12
+    assert _rel(per_pair[2], ref.R @ x) <= 1e-12
+    assert _rel(per_pair[3], ref.apply(x)) <= 1e-12  # c64 falls back to the FP64 per-pair path
+    assert _rel(with_tables[3], ref.apply(x)) <= 2e-6
+    a = cl.plane_wave([0, 0, 1], 1)
+    xs, conv = cl.solve(a, rtol=1e-10, mixed_precision="matrix_free", convergence="full")
+    assert _rel(xs, ref.solve(a.cpu().numpy())) <= 1e-8
+    with pytest.raises(Exception):
+        cl.set_table_limit(-1)
+
+
 @pytest.mark.parametrize("direction", [[0, 0, 1], [0, 0, -1], [1, 0, 0], [0.2, -0.1, 1.0]])
 @pytest.mark.parametrize("pol", [0, 1])
 def test_plane_wave(pkg, oracle, direction, pol):

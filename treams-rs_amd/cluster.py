@@ -125,8 +125,14 @@ class IterativeSphereCluster:
         """Spheres per block-Jacobi group."""
         check(lib().tg_cluster_set_group_size(self._h, int(spheres)))
 
+    def set_table_limit(self, nbytes: int) -> None:
+        """Largest pair-table set the matrix-free operators keep (bytes); larger
+        clusters recompute every pair's translation coefficients per apply."""
+        check(lib().tg_cluster_set_table_limit(self._h, int(nbytes)))
+
     def release(self) -> None:
-        """Free the stored coupling and the preconditioner (rebuilt on demand)."""
+        """Free the stored coupling, the pair tables and the preconditioner (rebuilt on
+        demand)."""
         check(lib().tg_cluster_release(self._h))
 
     # -------------------------------------------------------------- operators

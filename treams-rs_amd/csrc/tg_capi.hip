@@ -1,9 +1,12 @@
 // C ABI of the cluster-coupling library (declarations and reference citations in
 // include/treams_gpu.h).
 #include <cmath>
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -24,19 +27,84 @@ void set_error(const char *fmt, ...) {
     g_error = buf;
 }
 
+// Released device blocks are kept in a process-wide cache and handed to later requests
+// of at least half their size: hipFree synchronises the device and unmaps the pages,
+// which costs milliseconds per large block, and every solve allocates its scratch
+// again.  Work queued before a release may still use the block, so taking one from
+// the cache synchronises the device first.  Blocks above kCacheBlockMax are freed at
+// once, the cache holds at most kCacheBytes, it is emptied when an allocation fails
+// (and the allocation retried) and when the last cluster is destroyed.
+namespace {
+constexpr size_t kCacheBytes = (size_t)32 << 30;
+constexpr size_t kCacheBlockMax = (size_t)8 << 30;
+std::mutex g_cache_mu;
+std::multimap<size_t, void *> g_cache;  // block size -> block
+size_t g_cached = 0;
+int g_live_clusters = 0;
+
+void *cache_take(size_t need, size_t &got) {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    auto it = g_cache.lower_bound(need);
+    if (it == g_cache.end() || it->first / 2 > need) return nullptr;
+    void *p = it->second;
+    got = it->first;
+    g_cached -= got;
+    g_cache.erase(it);
+    return p;
+}
+
+bool cache_put(void *p, size_t bytes) {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    if (bytes > kCacheBlockMax || g_cached + bytes > kCacheBytes) return false;
+    g_cache.emplace(bytes, p);
+    g_cached += bytes;
+    return true;
+}
+
+void cache_clear() {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    for (auto &kv : g_cache) (void)hipFree(kv.second);
+    g_cache.clear();
+    g_cached = 0;
+}
+
+void cluster_born() {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    ++g_live_clusters;
+}
+
+void cluster_died() {
+    bool last;
+    {
+        std::lock_guard<std::mutex> lock(g_cache_mu);
+        last = --g_live_clusters == 0;
+    }
+    if (last) cache_clear();
+}
+}  // namespace
+
 int DevBuf::ensure(size_t need) {
     if (need <= bytes && ptr) return TG_OK;
-    if (ptr) {
-        hipError_t e = hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
+    release();
+    if (need == 0) return TG_OK;
+    size_t got = 0;
+    if (void *p = cache_take(need, got)) {
+        hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) {
-            set_error("hipFree: %s", hipGetErrorString(e));
+            (void)hipFree(p);
+            set_error("hipDeviceSynchronize: %s", hipGetErrorString(e));
             return TG_DEVICE_ERROR;
         }
+        ptr = p;
+        bytes = got;
+        return TG_OK;
     }
-    if (need == 0) return TG_OK;
     hipError_t e = hipMalloc(&ptr, need);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        cache_clear();
+        e = hipMalloc(&ptr, need);
+    }
     if (e != hipSuccess) {
         ptr = nullptr;
         set_error("device allocation of %zu bytes failed: %s", need, hipGetErrorString(e));
@@ -47,7 +115,7 @@ int DevBuf::ensure(size_t need) {
 }
 
 void DevBuf::release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && !cache_put(ptr, bytes)) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
 }
@@ -102,6 +170,30 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp) {
             tentry[dst] = (int32_t)e | (cross ? (int32_t)0x40000000 : 0);
             tw[dst] = hp.w[t];
         }
+    // source-mode subsets per harmonic index and the subset column of every term
+    const int T = hp.table_len;
+    std::vector<std::vector<char>> used(T, std::vector<char>(hp.H, 0));
+    for (size_t e = 0; e < HH; ++e)
+        for (int32_t t = hp.starts[e]; t < hp.starts[e + 1]; ++t) used[hp.hidx[t]][e / hp.H] = 1;
+    std::vector<int32_t> soff(T + 1, 0), smode, rank(T * hp.H, -1), ucol(nterms);
+    for (int t = 0; t < T; ++t) {
+        for (int a = 0; a < hp.H; ++a)
+            if (used[t][a]) {
+                rank[t * hp.H + a] = (int32_t)smode.size() - soff[t];
+                smode.push_back(a);
+            }
+        soff[t + 1] = (int32_t)smode.size();
+    }
+    for (size_t e = 0; e < HH; ++e)
+        for (int32_t t = hp.starts[e]; t < hp.starts[e + 1]; ++t)
+            ucol[t] = soff[hp.hidx[t]] + rank[hp.hidx[t] * hp.H + e / hp.H];
+    const int S = soff[T];
+    std::vector<int32_t> stiles((size_t)kGemmCols * (T + 1), 0);
+    for (int cols = 1; cols <= kGemmCols; ++cols)
+        for (int t = 0; t < T; ++t) {
+            const int ncol = 2 * cols * (soff[t + 1] - soff[t]);
+            stiles[(size_t)(cols - 1) * (T + 1) + t + 1] = stiles[(size_t)(cols - 1) * (T + 1) + t] + (ncol + 63) / 64;
+        }
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t off_starts = 0;
     size_t off_nsame = al(off_starts + sizeof(int32_t) * (HH + 1));
@@ -111,7 +203,11 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp) {
     size_t off_ts = al(off_refl + sizeof(double) * hp.table_len);
     size_t off_te = al(off_ts + sizeof(int32_t) * (hp.table_len + 1));
     size_t off_tw = al(off_te + sizeof(int32_t) * nterms);
-    size_t total = al(off_tw + sizeof(double) * nterms);
+    size_t off_soff = al(off_tw + sizeof(double) * nterms);
+    size_t off_smode = al(off_soff + sizeof(int32_t) * (T + 1));
+    size_t off_ucol = al(off_smode + sizeof(int32_t) * std::max(S, 1));
+    size_t off_stiles = al(off_ucol + sizeof(int32_t) * nterms);
+    size_t total = al(off_stiles + sizeof(int32_t) * stiles.size());
     TG_TRY(buf.ensure(total));
     char *base = buf.as<char>();
     std::vector<char> host(total, 0);
@@ -123,6 +219,10 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp) {
     memcpy(host.data() + off_ts, tstarts.data(), sizeof(int32_t) * (hp.table_len + 1));
     memcpy(host.data() + off_te, tentry.data(), sizeof(int32_t) * nterms);
     memcpy(host.data() + off_tw, tw.data(), sizeof(double) * nterms);
+    memcpy(host.data() + off_soff, soff.data(), sizeof(int32_t) * (T + 1));
+    if (S) memcpy(host.data() + off_smode, smode.data(), sizeof(int32_t) * S);
+    memcpy(host.data() + off_ucol, ucol.data(), sizeof(int32_t) * nterms);
+    memcpy(host.data() + off_stiles, stiles.data(), sizeof(int32_t) * stiles.size());
     TG_HIP(hipMemcpy(base, host.data(), total, hipMemcpyHostToDevice));
     dp.H = hp.H;
     dp.L2 = hp.L2;
@@ -135,6 +235,12 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp) {
     dp.tstarts = (const int32_t *)(base + off_ts);
     dp.tentry = (const int32_t *)(base + off_te);
     dp.tw = (const double *)(base + off_tw);
+    dp.S = S;
+    dp.soff = (const int32_t *)(base + off_soff);
+    dp.smode = (const int32_t *)(base + off_smode);
+    dp.ucol = (const int32_t *)(base + off_ucol);
+    dp.stiles = (const int32_t *)(base + off_stiles);
+    for (int cols = 1; cols <= kGemmCols; ++cols) dp.ntiles[cols - 1] = stiles[(size_t)(cols - 1) * (T + 1) + T];
     return TG_OK;
 }
 
@@ -230,6 +336,7 @@ int tg_cluster_create(int lmax, double k0, int n_spheres, const double *radii, c
             }
         }
     tg_cluster *c = new tg_cluster();
+    cluster_born();
     c->lmax = lmax;
     c->N = n_spheres;
     c->M = 2 * lmax * (lmax + 2);
@@ -300,10 +407,13 @@ int tg_cluster_destroy(tg_cluster *c) {
     c->have_tables = false;
     c->b_tab32.release();
     c->have_tables32 = false;
+    c->b_tabR.release();
+    c->have_tablesR = false;
     learned_release(c);
     for (auto &w : c->ws) w.release();
     c->timer.collect();
     delete c;
+    cluster_died();
     return TG_OK;
 }
 
@@ -370,9 +480,28 @@ int tg_cluster_release(tg_cluster *c) {
     c->have_tables = false;
     c->b_tab32.release();
     c->have_tables32 = false;
+    c->b_tabR.release();
+    c->have_tablesR = false;
     c->storage = TG_STORE_NONE;
     learned_release(c);
     for (auto &w : c->ws) w.release();
+    return TG_OK;
+}
+
+int tg_cluster_set_table_limit(tg_cluster *c, int64_t bytes) {
+    TG_TRY(check_cluster(c));
+    if (bytes < 0) {
+        set_error("table limit must be >= 0 bytes");
+        return TG_INVALID_INPUT;
+    }
+    TG_HIP(hipStreamSynchronize(c->stream));
+    c->table_limit = bytes;
+    c->b_tab.release();
+    c->have_tables = false;
+    c->b_tab32.release();
+    c->have_tables32 = false;
+    c->b_tabR.release();
+    c->have_tablesR = false;
     return TG_OK;
 }
 

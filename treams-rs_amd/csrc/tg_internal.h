@@ -31,6 +31,8 @@ void set_error(const char *fmt, ...);
         if (_s != TG_OK) return _s;    \
     } while (0)
 
+constexpr int kGemmCols = 8;  // right-hand sides per matrix-free GEMM pass
+
 // Device view of the translation plan.
 struct DevPlan {
     int H, L2, T;                // modes per helicity, largest harmonic degree, table length
@@ -43,6 +45,17 @@ struct DevPlan {
     const int32_t *tstarts;      // T + 1
     const int32_t *tentry;       // entry index e (kind in bit 31)
     const double *tw;            // weight
+    // source-mode subsets of the matrix-free GEMMs: harmonic index t multiplies only the
+    // S_t source modes a that some term (b, a, t) uses, listed in smode[soff[t] ..
+    // soff[t + 1]); term k reads subset column ucol[k] (= soff[hidx[k]] + rank of a)
+    int S;                       // soff[T]
+    const int32_t *soff;         // T + 1
+    const int32_t *smode;        // S
+    const int32_t *ucol;         // terms
+    // 64-wide GEMM column tiles before harmonic t for G = 2 cols column groups:
+    // stiles[(cols - 1) (T + 1) + t], cols = 1 .. kGemmCols
+    const int32_t *stiles;
+    int ntiles[kGemmCols];       // stiles[(cols - 1) (T + 1) + T]: the grid of a pass
 };
 
 // Growable device buffer.
@@ -84,6 +97,9 @@ struct tg_cluster {
     bool have_tables = false;
     tg::DevBuf b_tab32;                // the same tables rounded to c64 (FP32 MFMA coupling)
     bool have_tables32 = false;
+    tg::DevBuf b_tabR;                 // FP64 regular harmonic tables (overlap for cross sections)
+    bool have_tablesR = false;
+    int64_t table_limit = (int64_t)64 << 30;  // largest pair-table set kept (bytes)
     tg::DevBuf ws[8];                  // scratch
     tg::Timer timer;
     tg::Preconditioner *pc = nullptr;

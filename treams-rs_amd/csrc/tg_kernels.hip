@@ -506,13 +506,17 @@ static int grid1(int64_t count, int block = 256) { return (int)((count + block -
 //   y_i[b] = sum_j sum_a C_ij[b, a] x_j[a] = sum_{a, t} w_t U_{h_t}[a][i],
 //   U_h[q][i] = sum_j TAB_h[i][j] X[q][j],
 // with TAB_h[i][j] = tab_ij[h] the singular harmonic table of the pair (zero for
-// i = j) and q = (column, helicity, source mode a).  U is one complex FP64 GEMM per
-// harmonic index h (M = N destinations, K = N sources, N = 2 H columns per right-hand
-// side) on MFMA, and the plan contraction that follows reads U once per term.
+// i = j) and q = (column, helicity, source mode a).  U is one complex GEMM per
+// harmonic index h (M = N destinations, K = N sources) on MFMA, over only the source
+// modes a that some term of h uses (57% of all (h, a) at lmax = 6: the Gaunt
+// selection rules |l_a - l_b| <= p <= l_a + l_b and m_b = m_a + q leave the rest
+// zero), and the plan contraction that follows reads U once per term.
 
-// TAB[h][j][i] = h_p(k r) P_p^q e^{i q phi} at d = pos_i - pos_j, for every ordered
-// pair; one thread per (i, j) with i fastest, so each table entry is a coalesced
-// store across the wave.  The diagonal (i = j) is zero.
+// TAB[h][j][i] = z_p(k r) P_p^q e^{i q phi} at d = pos_i - pos_j for every ordered pair,
+// z_p = h_p (singular coupling) or j_p (REG: regular overlap, whose self blocks are the
+// identity: the harmonic table at the origin is 1 at p = q = 0).  One thread per
+// (i, j) with i fastest, so each table entry is a coalesced store across the wave.
+template <bool REG>
 __global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd *tab) {
     const int L2 = pl.L2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -522,22 +526,35 @@ __global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd
     cd *out = tab + (int64_t)j * N + i;
     double d[3], r, u[3];
     for (int a = 0; a < 3; ++a) d[a] = pos[3 * i + a] - pos[3 * j + a];
-    if (i == j || !unit_direction(d, r, u)) {
+    if ((!REG && i == j) || !unit_direction(d, r, u)) {
         for (int q = 0; q <= L2; ++q) zero_column(q, L2, out, NN);
+        if (REG) out[0] = cd(1.0);
         return;
     }
     cd rad[2 * kMaxLmax + 2];
-    tg_hankel_seq(L2, k * r, rad);
+    radial_values(L2, k * r, REG, rad);
     for (int q = 0; q <= L2; ++q) table_column(q, L2, u, rad, pl.refl, out, NN);
 }
 
-int ensure_pair_tables(tg_cluster *c) {
-    if (c->have_tables) return TG_OK;
+// Pair tables are kept when one set takes at most c->table_limit bytes.
+static bool tables_fit(const tg_cluster *c) {
+    return (int64_t)sizeof(cd) * c->N * c->N * c->plan.T <= c->table_limit;
+}
+
+static int ensure_pair_tables(tg_cluster *c, bool regular) {
+    bool &have = regular ? c->have_tablesR : c->have_tables;
+    DevBuf &buf = regular ? c->b_tabR : c->b_tab;
+    if (have) return TG_OK;
     const int64_t NN = (int64_t)c->N * c->N;
-    TG_TRY(c->b_tab.ensure(sizeof(cd) * (size_t)NN * c->plan.T));
-    k_pair_tables<<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N, c->b_tab.as<cd>());
+    TG_TRY(buf.ensure(sizeof(cd) * (size_t)NN * c->plan.T));
+    if (regular)
+        k_pair_tables<true><<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N,
+                                                              buf.as<cd>());
+    else
+        k_pair_tables<false><<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N,
+                                                               buf.as<cd>());
     TG_HIP(hipGetLastError());
-    c->have_tables = true;
+    have = true;
     return TG_OK;
 }
 
@@ -561,57 +578,71 @@ __global__ void k_tables_to_c64(const cd *tab, int64_t count, cf *out) {
     if (t < count) out[t] = cf((float)tab[t].re, (float)tab[t].im);
 }
 
-// C[z] = A[z] B[z] for column-major complex matrices (M x K times K x N) in T =
-// double (FP64 MFMA) or float (FP32 MFMA), one 64 x 64 output tile per workgroup,
-// batch z = grid.z with element strides sA, sB, sC; 3M form.
+// U[g][soff[t] + r][i] = sum_j TAB_t[i][j] X[g H + smode[soff[t] + r]][j] for every
+// harmonic index t, column group g = (column, helicity) < G and subset rank r: one
+// 64 x 64 tile per workgroup, grid.x running over the column tiles of all harmonic
+// indices (tiles[t] of them before t), grid.y over 64-row tiles of destinations.
 template <class T, bool KTAIL>
-__global__ __launch_bounds__(256, 2) void k_zgemm(const cx<T> *A, int64_t lda, int64_t sA, const cx<T> *B,
-                                                  int64_t ldb, int64_t sB, cx<T> *C, int64_t ldc, int64_t sC,
-                                                  int64_t M, int64_t N, int K) {
-    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
-    const int64_t z = blockIdx.z;
+__global__ __launch_bounds__(256, 2) void k_zgemm_subsets(const cx<T> *TAB, const cx<T> *X, cx<T> *U, int N, int H,
+                                                          int S, int G, int T_, const int32_t *tiles,
+                                                          const int32_t *soff, const int32_t *smode) {
+    const int bx = blockIdx.x;
+    int lo = 0, hi = T_ - 1;  // last t with tiles[t] <= bx
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tiles[mid] <= bx) lo = mid; else hi = mid - 1;
+    }
+    const int t = lo;
+    const int s0 = soff[t], nt = soff[t + 1] - s0, ncol = G * nt;
+    const int64_t c0 = (int64_t)(bx - tiles[t]) * 64, r0 = (int64_t)blockIdx.y * 64;
     __shared__ TileLds<T> sm;
     Tile3M<T> acc;
-    zgemm_tile_3m<T, KTAIL>(A + z * sA, lda, B + z * sB, ldb, r0, c0, M, N, K, sm, acc);
+    auto bcol = [=](int64_t c) {
+        const int q = (int)min(c, (int64_t)ncol - 1), g = q / nt, r = q - g * nt;
+        return X + ((int64_t)g * H + smode[s0 + r]) * N;
+    };
+    zgemm_tile_3m_cols<T, KTAIL>(TAB + (int64_t)t * N * N, N, bcol, r0, c0, N, N, sm, acc);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, rr = lane & 15;
-    cx<T> *Cz = C + z * sC;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int v = 0; v < 4; ++v) {
+            const int q = (int)c0 + wn + b * 16 + mfma_out_row<T>(lane, v);
+            if (q >= ncol) continue;
+            const int g = q / nt, r = q - g * nt;
+            cx<T> *Uc = U + ((int64_t)g * S + s0 + r) * N;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<T>(lane, v);
+            for (int a = 0; a < 2; ++a) {
+                const int64_t gr = r0 + wm + a * 16 + rr;
                 const T t1 = acc.t1[a][b][v], t2 = acc.t2[a][b][v], t3 = acc.t3[a][b][v];
-                if (gr < M && gc < N) Cz[gc * ldc + gr] = cx<T>(t1 - t2, t3 - t1 - t2);
+                if (gr < N) Uc[gr] = cx<T>(t1 - t2, t3 - t1 - t2);
             }
+        }
 }
 
-// y[cc][h][i H + b] = sum_{a, t in (a, b)} w_t U[h_t][(cc, h, a)][i], same kind plus or
-// minus i times cross kind (h = 0: pol 1, +; h = 1: pol 0, -).  One thread per
+// y[cc][h][i H + b] = sum_{a, t in (a, b)} w_t U[(cc, h)][ucol_t][i], same kind plus
+// or minus i times cross kind (h = 0: pol 1, +; h = 1: pol 0, -).  One thread per
 // destination sphere i (coalesced over i); the plan is read with wave-uniform
 // addresses.  grid.y = (cc, h, b).
 template <class T>
-__global__ __launch_bounds__(64) void k_tab_contract(DevPlan pl, const cx<T> *U, int N, int H, int cols, int64_t n,
+__global__ __launch_bounds__(64) void k_tab_contract(DevPlan pl, const cx<T> *U, int N, int H, int64_t n,
                                                      int64_t nh, cd *y) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int yb = blockIdx.y;
     const int cc = yb / (2 * H), rem = yb - cc * 2 * H, h = rem / H, b = rem - h * H;
     if (i >= N) return;
-    const int64_t Q = (int64_t)cols * 2 * H;
-    const int64_t qbase = ((int64_t)cc * 2 + h) * H;
+    const cx<T> *Ug = U + ((int64_t)cc * 2 + h) * pl.S * N + i;
     cd same(0.0), cross(0.0);
     for (int a = 0; a < H; ++a) {
         const int e = a * H + b;
         const int s0 = pl.starts[e], s1 = pl.starts[e + 1], ns = pl.nsame[e];
-        const cx<T> *Ua = U + (qbase + a) * N + i;
         for (int t = s0; t < s0 + ns; ++t) {
-            const cx<T> u = Ua[(int64_t)pl.hidx[t] * Q * N];
+            const cx<T> u = Ug[(int64_t)pl.ucol[t] * N];
             same += pl.w[t] * cd((double)u.re, (double)u.im);
         }
         for (int t = s0 + ns; t < s1; ++t) {
-            const cx<T> u = Ua[(int64_t)pl.hidx[t] * Q * N];
+            const cx<T> u = Ug[(int64_t)pl.ucol[t] * N];
             cross += pl.w[t] * cd((double)u.re, (double)u.im);
         }
     }
@@ -621,13 +652,15 @@ __global__ __launch_bounds__(64) void k_tab_contract(DevPlan pl, const cx<T> *U,
 
 // The tables in the GEMM precision: FP64 pair tables, or their c64 rounding (built
 // once from the FP64 tables and kept with them).
-template <class T> static const cx<T> *pair_tables(tg_cluster *c);
-template <> const cd *pair_tables<double>(tg_cluster *c) { return c->b_tab.as<cd>(); }
-template <> const cf *pair_tables<float>(tg_cluster *c) { return c->b_tab32.as<cf>(); }
+template <class T> static const cx<T> *pair_tables(tg_cluster *c, bool regular);
+template <> const cd *pair_tables<double>(tg_cluster *c, bool regular) {
+    return (regular ? c->b_tabR : c->b_tab).as<cd>();
+}
+template <> const cf *pair_tables<float>(tg_cluster *c, bool) { return c->b_tab32.as<cf>(); }
 
 template <class R>
-static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols) {
-    TG_TRY(ensure_pair_tables(c));
+static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool regular) {
+    TG_TRY(ensure_pair_tables(c, regular));
     const int N = c->N, H = c->H, T = c->plan.T;  // T: harmonic indices
     if (sizeof(R) == 4 && !c->have_tables32) {
         const int64_t count = (int64_t)N * N * T;
@@ -636,27 +669,31 @@ static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols) {
         TG_HIP(hipGetLastError());
         c->have_tables32 = true;
     }
-    const int cmax = 8;  // right-hand sides per pass
+    const int cmax = kGemmCols;  // right-hand sides per pass
     const int64_t Qmax = (int64_t)std::min(cols, cmax) * 2 * H;
-    TG_TRY(c->ws[0].ensure(sizeof(cx<R>) * ((size_t)Qmax * N + (size_t)T * Qmax * N)));
+    const int S = c->plan.S;
+    TG_TRY(c->ws[0].ensure(sizeof(cx<R>) * ((size_t)Qmax * N + (size_t)std::min(cols, cmax) * 2 * S * N)));
     cx<R> *X = c->ws[0].as<cx<R>>();
     cx<R> *U = X + (size_t)Qmax * N;
-    const cx<R> *TAB = pair_tables<R>(c);
+    const cx<R> *TAB = pair_tables<R>(c, regular);
     Timer::Rec rec;
-    const char *name = sizeof(R) == 8 ? "coupling_mf" : "coupling_mf32";
+    const char *name = regular ? "overlap_mf" : sizeof(R) == 8 ? "coupling_mf" : "coupling_mf32";
     TG_TRY(c->timer.begin(name, c->stream, rec));
     for (int c0 = 0; c0 < cols; c0 += cmax) {
         const int nc = std::min(cmax, cols - c0);
         const int64_t Q = (int64_t)nc * 2 * H;
         k_pack_sources<R><<<grid1(Q * N), 256, 0, c->stream>>>(x + (int64_t)c0 * c->n, c->n, c->nh, N, H, nc, X);
-        dim3 grid((unsigned)((Q + 63) / 64), (unsigned)((N + 63) / 64), (unsigned)T);
+        const int32_t *tiles = c->plan.stiles + (size_t)(nc - 1) * (T + 1);
+        dim3 grid((unsigned)c->plan.ntiles[nc - 1], (unsigned)((N + 63) / 64));
         if (N % 16)
-            k_zgemm<R, true><<<grid, 256, 0, c->stream>>>(TAB, N, (int64_t)N * N, X, N, 0, U, N, Q * N, N, Q, N);
+            k_zgemm_subsets<R, true><<<grid, 256, 0, c->stream>>>(TAB, X, U, N, H, S, 2 * nc, T, tiles,
+                                                                 c->plan.soff, c->plan.smode);
         else
-            k_zgemm<R, false><<<grid, 256, 0, c->stream>>>(TAB, N, (int64_t)N * N, X, N, 0, U, N, Q * N, N, Q, N);
+            k_zgemm_subsets<R, false><<<grid, 256, 0, c->stream>>>(TAB, X, U, N, H, S, 2 * nc, T, tiles,
+                                                                  c->plan.soff, c->plan.smode);
         TG_HIP(hipGetLastError());
         k_tab_contract<R><<<dim3((unsigned)((N + 63) / 64), (unsigned)Q), 64, 0, c->stream>>>(
-            c->plan, U, N, H, nc, c->n, c->nh, y + (int64_t)c0 * c->n);
+            c->plan, U, N, H, c->n, c->nh, y + (int64_t)c0 * c->n);
         TG_HIP(hipGetLastError());
     }
     TG_TRY(c->timer.end(name, c->stream, rec));
@@ -666,14 +703,14 @@ static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols) {
 // C x with the c64 tables on FP32 MFMA (accumulated in FP32, contracted in FP64): the
 // inner operator of mixed-precision solves without a stored coupling.
 int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols) {
-    if (c->N < 2) return coupling_matrix_free(c, x, y, cols, 0);
-    return coupling_tables<float>(c, x, y, cols);
+    if (!tables_fit(c)) return coupling_matrix_free(c, x, y, cols, 0);
+    return coupling_tables<float>(c, x, y, cols, false);
 }
 
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular) {
-    // singular couplings reuse the cluster's pair tables; the regular overlap
-    // (cross sections only) recomputes its tables per pair
-    if (!regular && c->N > 1) return coupling_tables<double>(c, x, y, cols);
+    // pair tables and GEMMs while the tables fit the cluster's limit; beyond it the
+    // translation coefficients are recomputed per pair in every apply
+    if (tables_fit(c)) return coupling_tables<double>(c, x, y, cols, regular != 0);
     const int H = c->H;
     const int G = 256 / H;
     const int block = G * H;
@@ -1095,7 +1132,10 @@ int vec_dots(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *w,
              int64_t csw, cd *out_host, DevBuf &scratch) {
     const int nb = dot_blocks(n);
     const int rows = cols * nvec;
-    TG_TRY(scratch.ensure(sizeof(cd) * ((size_t)rows * nb + rows)));
+    // room for 64 vectors per column at once: the Arnoldi basis grows by one vector per
+    // iteration, and the scratch should not be reallocated every time
+    const size_t cap = (size_t)cols * ((nvec + 63) / 64 * 64);
+    TG_TRY(scratch.ensure(sizeof(cd) * (cap * nb + cap)));
     cd *partial = scratch.as<cd>();
     cd *res = partial + (size_t)rows * nb;
     k_dots<<<dim3(nb, cols), 256, 0, s>>>(V, vstride, nvec, w, n, csv, csw, partial);

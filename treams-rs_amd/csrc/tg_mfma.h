@@ -51,15 +51,15 @@ struct Tile3M {
     typename mfma_acc<T>::V t1[2][2], t2[2][2], t3[2][2];
 };
 
-// t += A[r0 : r0 + 64, 0 : K] B[0 : K, c0 : c0 + 64] for column-major A (lda) and B
-// (ldb).  Rows >= M and columns >= N are read clamped (valid addresses): they only
-// feed output rows and columns that the caller does not store.  With KTAIL, k >= K is
-// read clamped and zeroed (K not a multiple of 16).  No load is conditional; the next
-// K slice is loaded while the current one is multiplied.  256 threads.
-template <class T, bool KTAIL>
-__device__ __forceinline__ void zgemm_tile_3m(const cx<T> *A, int64_t lda, const cx<T> *B, int64_t ldb, int64_t r0,
-                                              int64_t c0, int64_t M, int64_t N, int K, TileLds<T> &sm,
-                                              Tile3M<T> &acc) {
+// t += A[r0 : r0 + 64, 0 : K] B[0 : K, cols] for column-major A (lda); column c of B
+// (c = c0 .. c0 + 63) starts at bcol(c), which the caller keeps a valid address for
+// every c (clamped).  Rows >= M are read clamped: they only feed output rows that
+// the caller does not store.  With KTAIL, k >= K is read clamped and zeroed (K not a
+// multiple of 16).  No load is conditional; the next K slice is loaded while the
+// current one is multiplied.  256 threads.
+template <class T, bool KTAIL, class BCol>
+__device__ __forceinline__ void zgemm_tile_3m_cols(const cx<T> *A, int64_t lda, BCol bcol, int64_t r0, int64_t c0,
+                                                   int64_t M, int K, TileLds<T> &sm, Tile3M<T> &acc) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int rr = lane & 15, kq = lane >> 4;
@@ -73,20 +73,22 @@ __device__ __forceinline__ void zgemm_tile_3m(const cx<T> *A, int64_t lda, const
     // B (16 x 64 columns) at (k = tid & 15, column = (tid >> 4) + 16 j)
     const int lm = tid & 63, lk = tid >> 6, uk = tid & 15, uc = tid >> 4;
     const int64_t arow = min(r0 + lm, M - 1);
+    const cx<T> *bptr[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bptr[j] = bcol(c0 + uc + 16 * j);
     cx<T> areg[4], breg[4];
     auto load_slice = [&](int kk) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int ka = kk + lk + 4 * j, kb = kk + uk;
-            const int64_t bc = min(c0 + uc + 16 * j, N - 1);
             if (KTAIL) {
                 const cx<T> av = A[(int64_t)min(ka, K - 1) * lda + arow];
-                const cx<T> bv = B[bc * ldb + min(kb, K - 1)];
+                const cx<T> bv = bptr[j][min(kb, K - 1)];
                 areg[j] = ka < K ? av : cx<T>();
                 breg[j] = kb < K ? bv : cx<T>();
             } else {
                 areg[j] = A[(int64_t)ka * lda + arow];
-                breg[j] = B[bc * ldb + kb];
+                breg[j] = bptr[j][kb];
             }
         }
     };
@@ -117,6 +119,16 @@ __device__ __forceinline__ void zgemm_tile_3m(const cx<T> *A, int64_t lda, const
             }
         }
     }
+}
+
+// t += A[r0 : r0 + 64, 0 : K] B[0 : K, c0 : c0 + 64] for column-major A (lda) and B
+// (ldb); columns >= N are read clamped.
+template <class T, bool KTAIL>
+__device__ __forceinline__ void zgemm_tile_3m(const cx<T> *A, int64_t lda, const cx<T> *B, int64_t ldb, int64_t r0,
+                                              int64_t c0, int64_t M, int64_t N, int K, TileLds<T> &sm,
+                                              Tile3M<T> &acc) {
+    zgemm_tile_3m_cols<T, KTAIL>(
+        A, lda, [=](int64_t c) { return B + min(c, N - 1) * ldb; }, r0, c0, M, K, sm, acc);
 }
 
 }  // namespace tg
