@@ -4,13 +4,15 @@
 Workload: N = 1024 spheres, lmax = 6 (98,304 unknowns), radii U[0.15, 0.35],
 permittivity U[2, 6] + i U[0, 0.2], centres in a cube of side 16 with surface gap
 >= 0.05, k0 = 1.3, four plane-wave right-hand sides (+-z x +-helicity).  One step =
-the whole job for one cluster: assemble the coupling (c64, helicity-split), build
-the preconditioner, and solve the four columns by mixed-precision GMRES(50) to a
-true FP64 relative residual <= 1e-8.  value = solves/s = 4 K / (time of K steps).
+the whole job for one cluster: build the coupling (c64 and FP64 pair harmonic
+tables), build the learned preconditioner, and solve the four columns by
+mixed-precision GMRES(50) (inner solves on the c64 coupling, FP32 MFMA) to a true
+FP64 relative residual <= 1e-8.  value = solves/s = 4 K / (time of K steps).
 
-Also measured in the same run: the plain FP64 block-Jacobi GMRES(50) on the same
-GPU (stored c128 coupling), the per-kernel roofline of the stored-coupling matvec
-and of the assembly (HIP events on the library's stream), the dense LU at N = 512,
+Also measured in the same run: the same solve on the dense stored c64 coupling
+(assembly + streaming matvec, with their HBM rooflines), the plain FP64
+block-Jacobi GMRES(50) on the same GPU (stored c128 coupling and FP64 pair-table
+GEMMs), the per-kernel rooflines (HIP events on the library's stream), the dense LU at N = 512,
 lmax = 4 (c64 with FP64 refinement and c128; MFMA TFLOP/s of the factorisation),
 the cross-section gradient at N = 256, lmax = 4 through the LU adjoint, and the CPU
 oracle (matrix-free FP64 apply, OpenMP) on a bounded sample, extrapolated to a solve.
@@ -32,6 +34,19 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s @N=1024,lmax=6 to 1e-8 resid: learned-PC vs plain; %HBM/MFMA peak"
 HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def subset_modes(pkg, lmax: int) -> int:
+    """Sum over harmonic indices t of the source modes a that some plan term (b, a, t)
+    uses: the GEMM columns per helicity and right-hand side."""
+    starts, nsame, w, hidx = pkg.plan_terms(lmax)
+    H = lmax * (lmax + 2)
+    pairs = set()
+    for e in range(H * H):
+        for k in range(starts[e], starts[e + 1]):
+            pairs.add((int(hidx[k]), e // H))
+    return len(pairs)
 
 
 def log(msg: str) -> None:
@@ -76,7 +91,8 @@ def run_method(cl, B, torch, *, mixed, pc, steps, warmup, rtol, restart, label, 
     elapsed = time.perf_counter() - t0
     cl.timing(False)
     kernels = {}
-    for name in ("assemble", "matvec_stored", "coupling_mf", "coupling_mf32", "pc_apply", "pc_factor"):
+    for name in ("assemble", "matvec_stored", "coupling_mf", "coupling_mf32", "gemm_c64", "gemm_f64", "pc_apply",
+                 "pc_factor"):
         ms, count = cl.timing_get(name)
         if count:
             kernels[name] = {"total_ms": ms, "launches": count, "avg_ms": ms / count}
@@ -231,34 +247,30 @@ def main() -> int:
     B = rhs_columns(cl, torch)
     ncols = B.shape[1]
 
-    # mixed precision, learned / block-Jacobi preconditioner: the headline
-    x, conv, elapsed, kernels = run_method(cl, B, torch, mixed=True, pc=args.pc, steps=args.steps,
+    # the headline: learned-PC mixed-precision GMRES whose inner solves apply the c64
+    # coupling held as pair harmonic tables (FP32 MFMA GEMMs); FP64 refinement and the
+    # true-residual certificate on the FP64 tables
+    x, conv, elapsed, kernels = run_method(cl, B, torch, mixed="matrix_free", pc=args.pc, steps=args.steps,
                                            warmup=args.warmup, rtol=args.rtol, restart=args.restart,
                                            label="mixed")
     value = ncols * args.steps / elapsed
-    c64_bytes = 2 * nh * nh * 8
-    mv = kernels.get("matvec_stored", {})
+    gemm = kernels.get("gemm_c64", {})
     roof = None
-    if mv:
-        achieved = c64_bytes / (mv["avg_ms"] * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": "k_matvec_stored<float2> (c64 helicity-split coupling, 4 RHS)",
-                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None, "bytes_per_launch": c64_bytes, "avg_ms": mv["avg_ms"]}
-        # HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE counter runs of this
-        # workload, scaled by a plain copy of known size (the summary under profiles/)
-        tpath = os.path.join(ROOT, "profiles", "matvec_traffic_n1024_l6.json")
+    if gemm:
+        cmacs = float(N) * N * 2 * ncols * subset_modes(pkg, lmax)
+        achieved = 8.0 * cmacs / (gemm["avg_ms"] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "k_zgemm_subsets<float> (c64 pair-table GEMMs, 4 RHS, 3M form)",
+                "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP32_MFMA_PEAK_TFLOPS, "traffic": None,
+                "flops_per_launch": 8.0 * cmacs, "avg_ms": gemm["avg_ms"],
+                "flop_count": "8 real flops per complex multiply-add over the nonzero (harmonic, source mode) "
+                              "columns; the 3M form executes 6 on MFMA"}
+        tpath = os.path.join(ROOT, "profiles", "gemm_c64_traffic_n1024_l6.json")
         if args.workload == "n1024_l6" and os.path.exists(tpath):
             with open(tpath) as fh:
                 t = json.load(fh)
             roof["traffic"] = t["traffic_bytes"]
-            roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy "
-                                      "(profiles/matvec_assembly_n1024_l6.md)")
-    asm = kernels.get("assemble", {})
-    asm_roof = None
-    if asm:
-        achieved = c64_bytes / (asm["avg_ms"] * 1e-3) / 1e9
-        asm_roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "bytes_per_launch": c64_bytes, "avg_ms": asm["avg_ms"]}
+            roof["traffic_source"] = t["source"]
     iterations = [c.iterations for c in conv]
     x_mixed = x
 
@@ -273,14 +285,17 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 stored coupling and matvec (FP32 accumulate); c128 Krylov vectors, residuals, refinement",
+        "dtype": "c64 coupling (pair tables) with FP32 MFMA accumulate for the inner solves; c128 tables, "
+                 "Krylov vectors, residuals and refinement",
         "data": "synthetic",
         "config": {"workload": f"N={N} spheres, lmax={lmax} ({n} unknowns), radii U[0.15,0.35], eps U[2,6]+iU[0,0.2], "
                                f"cube side 16, gap 0.05, k0 1.3, 4 plane waves (+-z x +-helicity), "
                                f"GMRES({args.restart}) to {args.rtol:g} true FP64 relative residual",
+                   "coupling": "c64 pair harmonic tables (N^2 (2 lmax + 1)^2 values, 1.4 GB) applied by FP32 MFMA "
+                               "GEMMs; the dense c64 store (38.65 GB, helicity-split) is measured in "
+                               "stored_dense_c64",
                    "rhs": ncols, "seed": args.seed, "preconditioner": args.pc, "restart": args.restart},
         "roofline": roof,
-        "assembly_roofline": asm_roof,
         "kernels": kernels,
         "iterations": iterations,
         "refinements": [c.refinements for c in conv],
@@ -288,18 +303,39 @@ def main() -> int:
     }
 
     ref_iterations = None
+    x_dense = None
     if not args.no_variants:
-        # the same learned-PC mixed solve with the matrix-free c64 inner operator (c64 pair
-        # tables on FP32 MFMA) instead of the stored coupling: no assembly, no 38.65 GB
+        # the same learned-PC mixed solve on the dense stored c64 coupling: assembly
+        # (HBM-bound write of 38.65 GB) and the streaming matvec (HBM-bound read)
         cl.release()
-        xv, convv, elv, kv = run_method(cl, B, torch, mixed="matrix_free", pc=args.pc, steps=args.steps,
+        xv, convv, elv, kv = run_method(cl, B, torch, mixed=True, pc=args.pc, steps=args.steps,
                                         warmup=args.warmup, rtol=args.rtol, restart=args.restart,
-                                        label="mixed-matrix-free")
-        result["mixed_matrix_free_c64"] = {
-            "value": ncols * args.steps / elv, "unit": "solves/s", "ms_per_step": 1e3 * elv / args.steps,
-            "iterations": [c.iterations for c in convv], "refinements": [c.refinements for c in convv],
-            "true_relative_residual": [c.residual_norm / c.rhs_norm for c in convv], "kernels": kv,
-            "note": "learned PC, inner GMRES on the matrix-free c64 operator (FP32 MFMA), FP64 refinement"}
+                                        label="mixed-stored")
+        c64_bytes = 2 * nh * nh * 8
+        dense = {"value": ncols * args.steps / elv, "unit": "solves/s", "ms_per_step": 1e3 * elv / args.steps,
+                 "iterations": [c.iterations for c in convv], "refinements": [c.refinements for c in convv],
+                 "true_relative_residual": [c.residual_norm / c.rhs_norm for c in convv], "kernels": kv,
+                 "note": "learned PC, inner GMRES on the assembled dense c64 coupling (helicity-split halves)"}
+        mv = kv.get("matvec_stored", {})
+        if mv:
+            achieved = c64_bytes / (mv["avg_ms"] * 1e-3) / 1e9
+            dense["matvec_roofline"] = {
+                "bound": "hbm", "kernel": "k_matvec_stored<float2> (dense c64 coupling, 4 RHS)",
+                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None, "bytes_per_launch": c64_bytes, "avg_ms": mv["avg_ms"]}
+            tpath = os.path.join(ROOT, "profiles", "matvec_traffic_n1024_l6.json")
+            if args.workload == "n1024_l6" and os.path.exists(tpath):
+                with open(tpath) as fh:
+                    t = json.load(fh)
+                dense["matvec_roofline"]["traffic"] = t["traffic_bytes"]
+        asm = kv.get("assemble", {})
+        if asm:
+            achieved = c64_bytes / (asm["avg_ms"] * 1e-3) / 1e9
+            result["assembly_roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                                           "bytes_per_launch": c64_bytes, "avg_ms": asm["avg_ms"]}
+        result["stored_dense_c64"] = dense
+        x_dense = xv
     if not args.no_plain:
         plains = {}
         for stored, key in ((True, "plain_fp64_block_jacobi"), (False, "plain_fp64_block_jacobi_matrix_free")):
@@ -326,6 +362,13 @@ def main() -> int:
         result["cross_sections_ms_per_column"] = 1e3 * (time.perf_counter() - tx) / (2 * ncols)
         result["cross_section_rel_error_vs_fp64"] = max(errs)
         result["solution_rel_error_vs_fp64"] = float(torch.linalg.norm(x_mixed - x64) / torch.linalg.norm(x64))
+        if x_dense is not None:
+            errs = []
+            for c in range(ncols):
+                s1, e1 = cl.cross_sections(B[:, c], x_dense[:, c])
+                s2, e2 = cl.cross_sections(B[:, c], x64[:, c])
+                errs.append(max(abs(s1 - s2) / abs(s2), abs(e1 - e2) / abs(e2)))
+            result["stored_dense_c64"]["cross_section_rel_error_vs_fp64"] = max(errs)
         cl.release()
     if not args.no_reference:
         # the reference algorithm on the GPU: unpreconditioned matrix-free FP64 GMRES

@@ -102,6 +102,25 @@ def test_learned_apply_and_loss_against_dense_oracle(pkg, oracle, learned):
     assert np.all(dout[:, 2 * ref.lmax:] == 0)
 
 
+@pytest.mark.parametrize("lmax,cols", [(2, 5), (6, 6)])
+def test_learned_apply_many_columns(pkg, oracle, learned, lmax, cols):
+    """The forward apply reads each pair block once for 4 columns at a time: column
+    counts past one chunk, and block rows past one 64-lane slot (lmax = 6, 96 rows),
+    against the dense oracle."""
+    import mlp_oracle
+
+    cl, ref = _small_cluster(pkg, oracle, seed=11, N=7, lmax=lmax)
+    npairs = learned.near_pairs(cl, 5)
+    pairs = learned.pair_list(cl, npairs)
+    rng = np.random.default_rng(lmax)
+    out = (0.3 * rng.normal(size=(npairs, 16))).astype(np.float32)
+    learned.set_pair_output(cl, torch.from_numpy(out))
+    Minv = mlp_oracle.learned_inverse(ref, pairs, out)
+    x = rng.normal(size=(ref.n, cols)) + 1j * rng.normal(size=(ref.n, cols))
+    y = learned.learned_apply(cl, x).cpu().numpy()
+    assert np.linalg.norm(y - Minv @ x) <= 1e-12 * np.linalg.norm(Minv @ x)
+
+
 def test_learned_preconditioned_solve_same_answer(pkg, oracle, learned):
     cl, ref = _small_cluster(pkg, oracle, seed=6, N=24, lmax=4)
     mlp = learned.MLP(seed=2)

@@ -43,8 +43,10 @@ __device__ void radial_values(int L2, double kr, bool regular, cd *rad) {
 // unit vector u: diagonal ladder P_q^q = (-1)^q (2q-1)!! (x + i y)^q, then the degree
 // recurrence (p - q) S_p = (2p - 1) z S_{p-1} - (p + q - 1) S_{p-2}; negative orders by
 // S_p^{-q} = (-1)^q (p-q)!/(p+q)! conj(S_p^q).
-__device__ void table_column(int q, int L2, const double u[3], const cd *rad, const double *refl, cd *tab,
-                             int64_t stride = 1) {
+// put(h, value) stores harmonic index h.
+template <class Put>
+__device__ __forceinline__ void table_column_put(int q, int L2, const double u[3], const cd *rad, const double *refl,
+                                                 Put put) {
     cd xy(u[0], u[1]);
     cd cur(1.0);
     for (int k = 1; k <= q; ++k) cur = (-(double)(2 * k - 1)) * (cur * xy);
@@ -57,16 +59,26 @@ __device__ void table_column(int q, int L2, const double u[3], const cd *rad, co
             prev = cur;
             cur = next;
         }
-        tab[(int64_t)(p * p + p + q) * stride] = rad[p] * cur;
-        if (q > 0) tab[(int64_t)(p * p + p - q) * stride] = rad[p] * (refl[p * p + p - q] * conj(cur));
+        put(p * p + p + q, rad[p] * cur);
+        if (q > 0) put(p * p + p - q, rad[p] * (refl[p * p + p - q] * conj(cur)));
+    }
+}
+
+__device__ void table_column(int q, int L2, const double u[3], const cd *rad, const double *refl, cd *tab,
+                             int64_t stride = 1) {
+    table_column_put(q, L2, u, rad, refl, [&](int h, cd v) { tab[(int64_t)h * stride] = v; });
+}
+
+template <class Put>
+__device__ __forceinline__ void zero_column_put(int q, int L2, Put put) {
+    for (int p = q; p <= L2; ++p) {
+        put(p * p + p + q, cd(0.0));
+        if (q > 0) put(p * p + p - q, cd(0.0));
     }
 }
 
 __device__ __forceinline__ void zero_column(int q, int L2, cd *tab, int64_t stride = 1) {
-    for (int p = q; p <= L2; ++p) {
-        tab[(int64_t)(p * p + p + q) * stride] = cd(0.0);
-        if (q > 0) tab[(int64_t)(p * p + p - q) * stride] = cd(0.0);
-    }
+    zero_column_put(q, L2, [&](int h, cd v) { tab[(int64_t)h * stride] = v; });
 }
 
 // Both helicity values of one block entry from the plan and a harmonic table.
@@ -517,23 +529,34 @@ static int grid1(int64_t count, int block = 256) { return (int)((count + block -
 // identity: the harmonic table at the origin is 1 at p = q = 0).  One thread per
 // (i, j) with i fastest, so each table entry is a coalesced store across the wave.
 template <bool REG>
-__global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd *tab) {
+__global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd *tab, cf *tab32) {
     const int L2 = pl.L2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)N * N) return;
     const int j = (int)(t / N), i = (int)(t - (int64_t)j * N);
     const int64_t NN = (int64_t)N * N;
-    cd *out = tab + (int64_t)j * N + i;
+    const int64_t o = (int64_t)j * N + i;
+    // the FP64 entry and, when asked for, its c64 rounding (the FP32 MFMA tables)
+    auto put = [&](int h, cd v) {
+        tab[(int64_t)h * NN + o] = v;
+        if (tab32) tab32[(int64_t)h * NN + o] = cf((float)v.re, (float)v.im);
+    };
     double d[3], r, u[3];
     for (int a = 0; a < 3; ++a) d[a] = pos[3 * i + a] - pos[3 * j + a];
     if ((!REG && i == j) || !unit_direction(d, r, u)) {
-        for (int q = 0; q <= L2; ++q) zero_column(q, L2, out, NN);
-        if (REG) out[0] = cd(1.0);
+        for (int q = 0; q <= L2; ++q) zero_column_put(q, L2, put);
+        if (REG) put(0, cd(1.0));
         return;
     }
     cd rad[2 * kMaxLmax + 2];
     radial_values(L2, k * r, REG, rad);
-    for (int q = 0; q <= L2; ++q) table_column(q, L2, u, rad, pl.refl, out, NN);
+    for (int q = 0; q <= L2; ++q) table_column_put(q, L2, u, rad, pl.refl, put);
+}
+
+// FP64 pair tables rounded to c64, for the FP32 MFMA coupling.
+__global__ void k_tables_to_c64(const cd *tab, int64_t count, cf *out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) out[t] = cf((float)tab[t].re, (float)tab[t].im);
 }
 
 // Pair tables are kept when one set takes at most c->table_limit bytes.
@@ -541,20 +564,30 @@ static bool tables_fit(const tg_cluster *c) {
     return (int64_t)sizeof(cd) * c->N * c->N * c->plan.T <= c->table_limit;
 }
 
-static int ensure_pair_tables(tg_cluster *c, bool regular) {
+// The FP64 singular (or regular) tables, and with c64 also their c64 rounding: built
+// together in one pass when neither exists yet.
+static int ensure_pair_tables(tg_cluster *c, bool regular, bool c64) {
     bool &have = regular ? c->have_tablesR : c->have_tables;
     DevBuf &buf = regular ? c->b_tabR : c->b_tab;
-    if (have) return TG_OK;
-    const int64_t NN = (int64_t)c->N * c->N;
-    TG_TRY(buf.ensure(sizeof(cd) * (size_t)NN * c->plan.T));
-    if (regular)
-        k_pair_tables<true><<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N,
-                                                              buf.as<cd>());
-    else
-        k_pair_tables<false><<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N,
-                                                               buf.as<cd>());
+    const bool need32 = c64 && !regular && !c->have_tables32;
+    if (have && !need32) return TG_OK;
+    const int64_t NN = (int64_t)c->N * c->N, count = NN * c->plan.T;
+    if (need32) TG_TRY(c->b_tab32.ensure(sizeof(cf) * (size_t)count));
+    cf *t32 = need32 ? c->b_tab32.as<cf>() : nullptr;
+    if (have) {
+        k_tables_to_c64<<<grid1(count), 256, 0, c->stream>>>(buf.as<cd>(), count, t32);
+    } else {
+        TG_TRY(buf.ensure(sizeof(cd) * (size_t)count));
+        if (regular)
+            k_pair_tables<true><<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N,
+                                                                  buf.as<cd>(), nullptr);
+        else
+            k_pair_tables<false><<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N,
+                                                                   buf.as<cd>(), t32);
+    }
     TG_HIP(hipGetLastError());
     have = true;
+    if (need32) c->have_tables32 = true;
     return TG_OK;
 }
 
@@ -572,11 +605,6 @@ __global__ void k_pack_sources(const cd *x, int64_t n, int64_t nh, int N, int H,
     X[t] = cx<T>((T)v.re, (T)v.im);
 }
 
-// FP64 pair tables rounded to c64, for the FP32 MFMA coupling.
-__global__ void k_tables_to_c64(const cd *tab, int64_t count, cf *out) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < count) out[t] = cf((float)tab[t].re, (float)tab[t].im);
-}
 
 // U[g][soff[t] + r][i] = sum_j TAB_t[i][j] X[g H + smode[soff[t] + r]][j] for every
 // harmonic index t, column group g = (column, helicity) < G and subset rank r: one
@@ -660,15 +688,8 @@ template <> const cf *pair_tables<float>(tg_cluster *c, bool) { return c->b_tab3
 
 template <class R>
 static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool regular) {
-    TG_TRY(ensure_pair_tables(c, regular));
+    TG_TRY(ensure_pair_tables(c, regular, sizeof(R) == 4));
     const int N = c->N, H = c->H, T = c->plan.T;  // T: harmonic indices
-    if (sizeof(R) == 4 && !c->have_tables32) {
-        const int64_t count = (int64_t)N * N * T;
-        TG_TRY(c->b_tab32.ensure(sizeof(cf) * (size_t)count));
-        k_tables_to_c64<<<grid1(count), 256, 0, c->stream>>>(c->b_tab.as<cd>(), count, c->b_tab32.as<cf>());
-        TG_HIP(hipGetLastError());
-        c->have_tables32 = true;
-    }
     const int cmax = kGemmCols;  // right-hand sides per pass
     const int64_t Qmax = (int64_t)std::min(cols, cmax) * 2 * H;
     const int S = c->plan.S;
@@ -685,6 +706,9 @@ static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool reg
         k_pack_sources<R><<<grid1(Q * N), 256, 0, c->stream>>>(x + (int64_t)c0 * c->n, c->n, c->nh, N, H, nc, X);
         const int32_t *tiles = c->plan.stiles + (size_t)(nc - 1) * (T + 1);
         dim3 grid((unsigned)c->plan.ntiles[nc - 1], (unsigned)((N + 63) / 64));
+        Timer::Rec grec;
+        const char *gname = regular ? "gemm_overlap" : sizeof(R) == 8 ? "gemm_f64" : "gemm_c64";
+        TG_TRY(c->timer.begin(gname, c->stream, grec));
         if (N % 16)
             k_zgemm_subsets<R, true><<<grid, 256, 0, c->stream>>>(TAB, X, U, N, H, S, 2 * nc, T, tiles,
                                                                  c->plan.soff, c->plan.smode);
@@ -692,6 +716,7 @@ static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool reg
             k_zgemm_subsets<R, false><<<grid, 256, 0, c->stream>>>(TAB, X, U, N, H, S, 2 * nc, T, tiles,
                                                                   c->plan.soff, c->plan.smode);
         TG_HIP(hipGetLastError());
+        TG_TRY(c->timer.end(gname, c->stream, grec));
         k_tab_contract<R><<<dim3((unsigned)((N + 63) / 64), (unsigned)Q), 64, 0, c->stream>>>(
             c->plan, U, N, H, c->n, c->nh, y + (int64_t)c0 * c->n);
         TG_HIP(hipGetLastError());

@@ -79,33 +79,78 @@ __global__ void k_alpha_from_output(int64_t np, int lmax, const float *out, cd *
     alpha[t] = cd(1.0 + 0.5 * tanh((double)re), 0.5 * tanh((double)im));
 }
 
-// y_i = x_i + T_i sum_{pairs p of i} D(alpha_p) C_p x_{j_p}  (split layout; grid N x cols,
-// one thread per row r of sphere i in reference mode order).
-__global__ void k_near_apply(int N, int M, int H, int lmax, int64_t n, int64_t nh, const int *row_start, const int *pj,
-                             const cd *blocks, const cd *alpha, const cd *mie, const cd *x, cd *y) {
-    extern __shared__ cd acc_s[];
-    const int i = blockIdx.x, col = blockIdx.y;
-    const cd *xc = x + (int64_t)col * n;
-    const int r = threadIdx.x;
-    cd acc(0.0);
-    if (r < M) {
-        const int l = row_degree(r);
-        for (int p = row_start[i]; p < row_start[i + 1]; ++p) {
+// y_i = x_i + T_i sum_{pairs p of i} D(alpha_p) C_p x_{j_p} (split layout).  One
+// workgroup per sphere i reads each of its pair blocks once for 4 columns at a time:
+// wave w takes the pairs row_start[i] + w, + 4, ..., lane l the block rows l + 64 k
+// (k < RS, reference mode order); the waves' partial sums meet in LDS ([4][M]).
+template <int RS>
+__global__ __launch_bounds__(256) void k_near_apply(int M, int H, int lmax, int64_t n, int64_t nh, int cols,
+                                                    const int *row_start, const int *pj, const cd *blocks,
+                                                    const cd *alpha, const cd *mie, const cd *x, cd *y) {
+    constexpr int NC = 4;
+    extern __shared__ cd red[];
+    const int i = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p0 = row_start[i], p1 = row_start[i + 1];
+    for (int col0 = 0; col0 < cols; col0 += NC) {
+        const int nc = min(NC, cols - col0);
+        cd acc[RS][NC];
+#pragma unroll
+        for (int k = 0; k < RS; ++k)
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) acc[k][cc] = cd(0.0);
+        for (int p = p0 + wave; p < p1; p += 4) {
             const int j = pj[p];
             const cd *blk = blocks + (int64_t)p * M * M;
-            cd s(0.0);
-            for (int c = 0; c < M; ++c) s += blk[(int64_t)c * M + r] * xc[(int64_t)(c & 1) * nh + (int64_t)j * H + (c >> 1)];
-            acc += alpha[(int64_t)p * lmax + (l - 1)] * s;
+            cd sum[RS][NC];
+#pragma unroll
+            for (int k = 0; k < RS; ++k)
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) sum[k][cc] = cd(0.0);
+#pragma unroll 4
+            for (int c = 0; c < M; ++c) {
+                const int64_t xo = (int64_t)(c & 1) * nh + (int64_t)j * H + (c >> 1);
+                cd xv[NC];
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) xv[cc] = cc < nc ? x[(int64_t)(col0 + cc) * n + xo] : cd(0.0);
+#pragma unroll
+                for (int k = 0; k < RS; ++k) {
+                    const cd b = blk[(int64_t)c * M + min(lane + 64 * k, M - 1)];  // rows >= M unused
+#pragma unroll
+                    for (int cc = 0; cc < NC; ++cc) sum[k][cc] += b * xv[cc];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < RS; ++k) {
+                const int r = lane + 64 * k;
+                if (r < M) {
+                    const cd a = alpha[(int64_t)p * lmax + row_degree(r) - 1];
+#pragma unroll
+                    for (int cc = 0; cc < NC; ++cc) acc[k][cc] += a * sum[k][cc];
+                }
+            }
         }
-        acc_s[r] = acc;
-    }
-    __syncthreads();
-    if (r < M) {
-        const int b = r >> 1, h = r & 1, l = row_degree(r);
-        const cd *t2 = mie + (size_t)(i * lmax + l - 1) * 4;
-        const cd v = t2[h * 2 + 0] * acc_s[2 * b] + t2[h * 2 + 1] * acc_s[2 * b + 1];
-        const int64_t idx = (int64_t)col * n + (int64_t)h * nh + (int64_t)i * H + b;
-        y[idx] = x[idx] + v;
+        for (int w = 0; w < 4; ++w) {
+            if (wave == w) {
+#pragma unroll
+                for (int k = 0; k < RS; ++k) {
+                    const int r = lane + 64 * k;
+                    if (r < M)
+#pragma unroll
+                        for (int cc = 0; cc < NC; ++cc)
+                            red[cc * M + r] = (w == 0 ? cd(0.0) : red[cc * M + r]) + acc[k][cc];
+                }
+            }
+            __syncthreads();
+        }
+        for (int t = threadIdx.x; t < nc * M; t += blockDim.x) {
+            const int cc = t / M, r = t - cc * M;
+            const int b = r >> 1, h = r & 1, l = row_degree(r);
+            const cd *t2 = mie + (size_t)(i * lmax + l - 1) * 4;
+            const cd v = t2[h * 2 + 0] * red[cc * M + 2 * b] + t2[h * 2 + 1] * red[cc * M + 2 * b + 1];
+            const int64_t idx = (int64_t)(col0 + cc) * n + (int64_t)h * nh + (int64_t)i * H + b;
+            y[idx] = x[idx] + v;
+        }
+        __syncthreads();
     }
 }
 
@@ -275,9 +320,16 @@ int learned_near_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint)
     LearnedState *s = c->learned;
     const int M = c->M;
     if (!adjoint) {
-        k_near_apply<<<dim3(c->N, cols), ((M + 63) / 64) * 64, sizeof(cd) * M, c->stream>>>(
-            c->N, M, c->H, c->lmax, c->n, c->nh, s->row_start, s->pj, s->blocks.as<cd>(), s->alpha.as<cd>(),
-            c->b_mie.as<cd>(), x, y);
+        const size_t smem = sizeof(cd) * 4 * M;
+#define TG_NEAR(RS)                                                                                             \
+    k_near_apply<RS><<<c->N, 256, smem, c->stream>>>(M, c->H, c->lmax, c->n, c->nh, cols, s->row_start, s->pj, \
+                                                     s->blocks.as<cd>(), s->alpha.as<cd>(), c->b_mie.as<cd>(), x, y)
+        const int rs = (M + 63) / 64;
+        if (rs <= 1) TG_NEAR(1);
+        else if (rs == 2) TG_NEAR(2);
+        else if (rs <= 4) TG_NEAR(4);
+        else TG_NEAR(6);
+#undef TG_NEAR
         TG_HIP(hipGetLastError());
         return TG_OK;
     }
