@@ -206,6 +206,8 @@ def main() -> int:
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--no-lu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--operator", choices=("tables", "stored"), default="tables",
+                    help="c64 coupling of the headline inner solves: pair tables (GEMMs) or the dense store")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="bytes of a plain device copy run first (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -247,33 +249,67 @@ def main() -> int:
     B = rhs_columns(cl, torch)
     ncols = B.shape[1]
 
-    # the headline: learned-PC mixed-precision GMRES whose inner solves apply the c64
-    # coupling held as pair harmonic tables (FP32 MFMA GEMMs); FP64 refinement and the
-    # true-residual certificate on the FP64 tables
-    x, conv, elapsed, kernels = run_method(cl, B, torch, mixed="matrix_free", pc=args.pc, steps=args.steps,
-                                           warmup=args.warmup, rtol=args.rtol, restart=args.restart,
-                                           label="mixed")
-    value = ncols * args.steps / elapsed
-    gemm = kernels.get("gemm_c64", {})
-    roof = None
-    if gemm:
-        cmacs = float(N) * N * 2 * ncols * subset_modes(pkg, lmax)
-        achieved = 8.0 * cmacs / (gemm["avg_ms"] * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": "k_zgemm_subsets<float> (c64 pair-table GEMMs, 4 RHS, 3M form)",
+    def measure(operator, label):
+        """Learned-PC (or --pc) mixed-precision job on the c64 coupling held as pair
+        tables (FP32 MFMA GEMMs) or as the dense helicity-split store (streaming
+        matvec); with its rooflines."""
+        cl.release()
+        xm, cv, el, kern = run_method(cl, B, torch, mixed="matrix_free" if operator == "tables" else True,
+                                      pc=args.pc, steps=args.steps, warmup=args.warmup, rtol=args.rtol,
+                                      restart=args.restart, label=label)
+        out = {"value": ncols * args.steps / el, "unit": "solves/s", "ms_per_step": 1e3 * el / args.steps,
+               "iterations": [c.iterations for c in cv], "refinements": [c.refinements for c in cv],
+               "true_relative_residual": [c.residual_norm / c.rhs_norm for c in cv], "kernels": kern}
+        roofs = {}
+        gemm = kern.get("gemm_c64", {})
+        if gemm:
+            cmacs = float(N) * N * 2 * ncols * subset_modes(pkg, lmax)
+            achieved = 8.0 * cmacs / (gemm["avg_ms"] * 1e-3) / 1e12
+            roofs["gemm"] = {
+                "bound": "mfma", "kernel": "k_zgemm_subsets<float> (c64 pair-table GEMMs, 4 RHS, 3M form)",
                 "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP32_MFMA_PEAK_TFLOPS, "traffic": None,
-                "flops_per_launch": 8.0 * cmacs, "avg_ms": gemm["avg_ms"],
+                "frac": achieved / FP32_MFMA_PEAK_TFLOPS, "traffic": None, "flops_per_launch": 8.0 * cmacs,
+                "avg_ms": gemm["avg_ms"],
                 "flop_count": "8 real flops per complex multiply-add over the nonzero (harmonic, source mode) "
                               "columns; the 3M form executes 6 on MFMA"}
-        tpath = os.path.join(ROOT, "profiles", "gemm_c64_traffic_n1024_l6.json")
+            add_traffic(roofs["gemm"], "gemm_c64_traffic_n1024_l6.json")
+        mv = kern.get("matvec_stored", {})
+        if mv:
+            achieved = c64_bytes / (mv["avg_ms"] * 1e-3) / 1e9
+            roofs["matvec"] = {
+                "bound": "hbm", "kernel": "k_matvec_stored<float2> (dense c64 coupling, 4 RHS)",
+                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None, "bytes_per_launch": c64_bytes, "avg_ms": mv["avg_ms"]}
+            add_traffic(roofs["matvec"], "matvec_traffic_n1024_l6.json")
+        asm = kern.get("assemble", {})
+        if asm:
+            achieved = c64_bytes / (asm["avg_ms"] * 1e-3) / 1e9
+            roofs["assembly"] = {"bound": "hbm", "kernel": "k_assemble (dense c64 coupling, written once)",
+                                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": achieved / HBM_PEAK_GBS, "bytes_per_launch": c64_bytes,
+                                 "avg_ms": asm["avg_ms"]}
+        return xm, cv, out, roofs
+
+    def add_traffic(roof, name):
+        tpath = os.path.join(ROOT, "profiles", name)
         if args.workload == "n1024_l6" and os.path.exists(tpath):
             with open(tpath) as fh:
                 t = json.load(fh)
             roof["traffic"] = t["traffic_bytes"]
-            roof["traffic_source"] = t["source"]
-    iterations = [c.iterations for c in conv]
-    x_mixed = x
+            roof["traffic_source"] = t.get("source", f"profiles/{name}")
 
+    c64_bytes = 2 * nh * nh * 8
+    # the headline: inner solves on the c64 coupling held as pair harmonic tables
+    # (FP32 MFMA GEMMs) unless --operator stored; FP64 refinement and the true-residual
+    # certificate on the FP64 tables
+    x, conv, head, roofs = measure(args.operator, "mixed")
+    elapsed = head["ms_per_step"] * 1e-3 * args.steps
+    value = head["value"]
+    kernels = head["kernels"]
+    roof = roofs.get("gemm") if args.operator == "tables" else roofs.get("matvec")
+    iterations = head["iterations"]
+    x_mixed = x
+    tables = args.operator == "tables"
     result = {
         "metric": METRIC,
         "value": value,
@@ -285,56 +321,35 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 coupling (pair tables) with FP32 MFMA accumulate for the inner solves; c128 tables, "
-                 "Krylov vectors, residuals and refinement",
+        "dtype": ("c64 coupling (pair tables" if tables else "c64 coupling (dense store") +
+                 ") with FP32 accumulate for the inner solves; c128 tables, Krylov vectors, residuals and refinement",
         "data": "synthetic",
         "config": {"workload": f"N={N} spheres, lmax={lmax} ({n} unknowns), radii U[0.15,0.35], eps U[2,6]+iU[0,0.2], "
                                f"cube side 16, gap 0.05, k0 1.3, 4 plane waves (+-z x +-helicity), "
                                f"GMRES({args.restart}) to {args.rtol:g} true FP64 relative residual",
-                   "coupling": "c64 pair harmonic tables (N^2 (2 lmax + 1)^2 values, 1.4 GB) applied by FP32 MFMA "
-                               "GEMMs; the dense c64 store (38.65 GB, helicity-split) is measured in "
-                               "stored_dense_c64",
+                   "coupling": ("c64 pair harmonic tables (N^2 (2 lmax + 1)^2 values, 1.4 GB) applied by FP32 MFMA "
+                                "GEMMs; the dense c64 store (38.65 GB, helicity-split) is measured in "
+                                "stored_dense_c64") if tables else
+                               "dense c64 store (38.65 GB, helicity-split), streaming matvec",
                    "rhs": ncols, "seed": args.seed, "preconditioner": args.pc, "restart": args.restart},
         "roofline": roof,
         "kernels": kernels,
         "iterations": iterations,
-        "refinements": [c.refinements for c in conv],
-        "true_relative_residual": [c.residual_norm / c.rhs_norm for c in conv],
+        "refinements": head["refinements"],
+        "true_relative_residual": head["true_relative_residual"],
     }
+    if "assembly" in roofs:
+        result["assembly_roofline"] = roofs["assembly"]
 
     ref_iterations = None
     x_dense = None
     if not args.no_variants:
-        # the same learned-PC mixed solve on the dense stored c64 coupling: assembly
-        # (HBM-bound write of 38.65 GB) and the streaming matvec (HBM-bound read)
-        cl.release()
-        xv, convv, elv, kv = run_method(cl, B, torch, mixed=True, pc=args.pc, steps=args.steps,
-                                        warmup=args.warmup, rtol=args.rtol, restart=args.restart,
-                                        label="mixed-stored")
-        c64_bytes = 2 * nh * nh * 8
-        dense = {"value": ncols * args.steps / elv, "unit": "solves/s", "ms_per_step": 1e3 * elv / args.steps,
-                 "iterations": [c.iterations for c in convv], "refinements": [c.refinements for c in convv],
-                 "true_relative_residual": [c.residual_norm / c.rhs_norm for c in convv], "kernels": kv,
-                 "note": "learned PC, inner GMRES on the assembled dense c64 coupling (helicity-split halves)"}
-        mv = kv.get("matvec_stored", {})
-        if mv:
-            achieved = c64_bytes / (mv["avg_ms"] * 1e-3) / 1e9
-            dense["matvec_roofline"] = {
-                "bound": "hbm", "kernel": "k_matvec_stored<float2> (dense c64 coupling, 4 RHS)",
-                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None, "bytes_per_launch": c64_bytes, "avg_ms": mv["avg_ms"]}
-            tpath = os.path.join(ROOT, "profiles", "matvec_traffic_n1024_l6.json")
-            if args.workload == "n1024_l6" and os.path.exists(tpath):
-                with open(tpath) as fh:
-                    t = json.load(fh)
-                dense["matvec_roofline"]["traffic"] = t["traffic_bytes"]
-        asm = kv.get("assemble", {})
-        if asm:
-            achieved = c64_bytes / (asm["avg_ms"] * 1e-3) / 1e9
-            result["assembly_roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                                           "bytes_per_launch": c64_bytes, "avg_ms": asm["avg_ms"]}
-        result["stored_dense_c64"] = dense
+        other = "stored" if tables else "tables"
+        xv, convv, var, vroofs = measure(other, "mixed-" + other)
+        var["rooflines"] = vroofs
+        if "assembly" in vroofs:
+            result["assembly_roofline"] = vroofs["assembly"]
+        result["stored_dense_c64" if tables else "pair_tables_c64"] = var
         x_dense = xv
     if not args.no_plain:
         plains = {}
@@ -368,7 +383,7 @@ def main() -> int:
                 s1, e1 = cl.cross_sections(B[:, c], x_dense[:, c])
                 s2, e2 = cl.cross_sections(B[:, c], x64[:, c])
                 errs.append(max(abs(s1 - s2) / abs(s2), abs(e1 - e2) / abs(e2)))
-            result["stored_dense_c64"]["cross_section_rel_error_vs_fp64"] = max(errs)
+            result["stored_dense_c64" if tables else "pair_tables_c64"]["cross_section_rel_error_vs_fp64"] = max(errs)
         cl.release()
     if not args.no_reference:
         # the reference algorithm on the GPU: unpreconditioned matrix-free FP64 GMRES

@@ -608,21 +608,29 @@ __global__ void k_pack_sources(const cd *x, int64_t n, int64_t nh, int N, int H,
 
 // U[g][soff[t] + r][i] = sum_j TAB_t[i][j] X[g H + smode[soff[t] + r]][j] for every
 // harmonic index t, column group g = (column, helicity) < G and subset rank r: one
-// 64 x 64 tile per workgroup, grid.x running over the column tiles of all harmonic
-// indices (tiles[t] of them before t), grid.y over 64-row tiles of destinations.
+// 64 x 64 tile per workgroup.  Tiles are ordered (t, row tile, column tile), column
+// tile fastest (tiles[t] column tiles before t), and the 1-D grid is remapped so that
+// each XCD (workgroups are dealt to the 8 XCDs round-robin) runs a contiguous range
+// of them: the column tiles sharing one 64-row panel of TAB_t then run together on
+// one XCD and read the panel through its L2.
 template <class T, bool KTAIL>
 __global__ __launch_bounds__(256, 2) void k_zgemm_subsets(const cx<T> *TAB, const cx<T> *X, cx<T> *U, int N, int H,
                                                           int S, int G, int T_, const int32_t *tiles,
                                                           const int32_t *soff, const int32_t *smode) {
-    const int bx = blockIdx.x;
-    int lo = 0, hi = T_ - 1;  // last t with tiles[t] <= bx
+    const int nb = gridDim.x, bid = blockIdx.x;
+    const int xcd = bid & 7, q8 = nb >> 3, r8 = nb & 7;
+    const int L = xcd * q8 + min(xcd, r8) + (bid >> 3);
+    const int R = (N + 63) / 64;  // row tiles
+    int lo = 0, hi = T_ - 1;      // last t with tiles[t] R <= L
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (tiles[mid] <= bx) lo = mid; else hi = mid - 1;
+        if (tiles[mid] * R <= L) lo = mid; else hi = mid - 1;
     }
     const int t = lo;
+    const int nct = tiles[t + 1] - tiles[t], local = L - tiles[t] * R;
+    const int rt = local / nct, ct = local - rt * nct;
     const int s0 = soff[t], nt = soff[t + 1] - s0, ncol = G * nt;
-    const int64_t c0 = (int64_t)(bx - tiles[t]) * 64, r0 = (int64_t)blockIdx.y * 64;
+    const int64_t c0 = (int64_t)ct * 64, r0 = (int64_t)rt * 64;
     __shared__ TileLds<T> sm;
     Tile3M<T> acc;
     auto bcol = [=](int64_t c) {
@@ -705,7 +713,7 @@ static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool reg
         const int64_t Q = (int64_t)nc * 2 * H;
         k_pack_sources<R><<<grid1(Q * N), 256, 0, c->stream>>>(x + (int64_t)c0 * c->n, c->n, c->nh, N, H, nc, X);
         const int32_t *tiles = c->plan.stiles + (size_t)(nc - 1) * (T + 1);
-        dim3 grid((unsigned)c->plan.ntiles[nc - 1], (unsigned)((N + 63) / 64));
+        const unsigned grid = (unsigned)c->plan.ntiles[nc - 1] * (unsigned)((N + 63) / 64);
         Timer::Rec grec;
         const char *gname = regular ? "gemm_overlap" : sizeof(R) == 8 ? "gemm_f64" : "gemm_c64";
         TG_TRY(c->timer.begin(gname, c->stream, grec));
