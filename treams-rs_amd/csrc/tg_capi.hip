@@ -188,11 +188,14 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp) {
         for (int32_t t = hp.starts[e]; t < hp.starts[e + 1]; ++t)
             ucol[t] = soff[hp.hidx[t]] + rank[hp.hidx[t] * hp.H + e / hp.H];
     const int S = soff[T];
-    std::vector<int32_t> stiles((size_t)kGemmCols * (T + 1), 0);
-    for (int cols = 1; cols <= kGemmCols; ++cols)
-        for (int t = 0; t < T; ++t) {
-            const int ncol = 2 * cols * (soff[t + 1] - soff[t]);
-            stiles[(size_t)(cols - 1) * (T + 1) + t + 1] = stiles[(size_t)(cols - 1) * (T + 1) + t] + (ncol + 63) / 64;
+    std::vector<int32_t> stiles((size_t)2 * kGemmCols * (T + 1), 0);
+    for (int w = 0; w < 2; ++w)
+        for (int cols = 1; cols <= kGemmCols; ++cols) {
+            int32_t *st = stiles.data() + (size_t)(w * kGemmCols + cols - 1) * (T + 1);
+            for (int t = 0; t < T; ++t) {
+                const int ncol = 2 * cols * (soff[t + 1] - soff[t]);
+                st[t + 1] = st[t] + (ncol + kTileWidths[w] - 1) / kTileWidths[w];
+            }
         }
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t off_starts = 0;
@@ -240,7 +243,9 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp) {
     dp.smode = (const int32_t *)(base + off_smode);
     dp.ucol = (const int32_t *)(base + off_ucol);
     dp.stiles = (const int32_t *)(base + off_stiles);
-    for (int cols = 1; cols <= kGemmCols; ++cols) dp.ntiles[cols - 1] = stiles[(size_t)(cols - 1) * (T + 1) + T];
+    for (int w = 0; w < 2; ++w)
+        for (int cols = 1; cols <= kGemmCols; ++cols)
+            dp.ntiles[w][cols - 1] = stiles[(size_t)(w * kGemmCols + cols - 1) * (T + 1) + T];
     return TG_OK;
 }
 

@@ -32,6 +32,9 @@ void set_error(const char *fmt, ...);
     } while (0)
 
 constexpr int kGemmCols = 8;  // right-hand sides per matrix-free GEMM pass
+// Column-tile widths of the matrix-free GEMMs: 64 x 64 output tiles in FP32 (the
+// fewest loads per MFMA), 128 x 32 in FP64 (less column padding).
+constexpr int kTileWidths[2] = {64, 32};
 
 // Device view of the translation plan.
 struct DevPlan {
@@ -52,10 +55,10 @@ struct DevPlan {
     const int32_t *soff;         // T + 1
     const int32_t *smode;        // S
     const int32_t *ucol;         // terms
-    // 64-wide GEMM column tiles before harmonic t for G = 2 cols column groups:
-    // stiles[(cols - 1) (T + 1) + t], cols = 1 .. kGemmCols
+    // GEMM column tiles of width kTileWidths[w] before harmonic t for G = 2 cols column
+    // groups: stiles[((w kGemmCols) + cols - 1) (T + 1) + t], cols = 1 .. kGemmCols
     const int32_t *stiles;
-    int ntiles[kGemmCols];       // stiles[(cols - 1) (T + 1) + T]: the grid of a pass
+    int ntiles[2][kGemmCols];    // the totals (t = T): the column tiles of a pass
 };
 
 // Growable device buffer.

@@ -528,6 +528,39 @@ static int grid1(int64_t count, int block = 256) { return (int)((count + block -
 // z_p = h_p (singular coupling) or j_p (REG: regular overlap, whose self blocks are the
 // identity: the harmonic table at the origin is 1 at p = q = 0).  One thread per
 // (i, j) with i fastest, so each table entry is a coalesced store across the wave.
+// The singular table h_p(x) P_p^q e^{i q phi}: the table_column recurrences with the
+// Hankel upward recurrence run alongside each order q instead of from an array of
+// h_p (an array indexed by a run-time degree lives in scratch memory).  Same
+// operations in the same order, so the same values as table_column.
+template <class Put>
+__device__ __forceinline__ void singular_table_put(int L2, double x, const double u[3], const double *refl,
+                                                   Put put) {
+    const double inv = 1.0 / x, sx = sin(x), cx_ = cos(x);
+    const cd xy(u[0], u[1]);
+    cd ladder(1.0);
+    for (int q = 0; q <= L2; ++q) {
+        if (q > 0) ladder = (-(double)(2 * q - 1)) * (ladder * xy);
+        cd hprev(cx_ * inv, sx * inv), hcur(sx * inv, -cx_ * inv);  // h_{-1}, h_0
+        cd cur = ladder, prev(0.0);
+        for (int p = 0; p <= L2; ++p) {
+            if (p > q) {
+                const double a = (double)(2 * p - 1) / (double)(p - q);
+                const double b = (double)(p + q - 1) / (double)(p - q);
+                const cd next = (a * u[2]) * cur - b * prev;
+                prev = cur;
+                cur = next;
+            }
+            if (p >= q) {
+                put(p * p + p + q, hcur * cur);
+                if (q > 0) put(p * p + p - q, hcur * (refl[p * p + p - q] * conj(cur)));
+            }
+            const cd hnext = ((double)(2 * p + 1) * inv) * hcur - hprev;
+            hprev = hcur;
+            hcur = hnext;
+        }
+    }
+}
+
 template <bool REG>
 __global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd *tab, cf *tab32) {
     const int L2 = pl.L2;
@@ -546,6 +579,10 @@ __global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd
     if ((!REG && i == j) || !unit_direction(d, r, u)) {
         for (int q = 0; q <= L2; ++q) zero_column_put(q, L2, put);
         if (REG) put(0, cd(1.0));
+        return;
+    }
+    if (!REG) {
+        singular_table_put(L2, k * r, u, pl.refl, put);
         return;
     }
     cd rad[2 * kMaxLmax + 2];
@@ -608,19 +645,22 @@ __global__ void k_pack_sources(const cd *x, int64_t n, int64_t nh, int N, int H,
 
 // U[g][soff[t] + r][i] = sum_j TAB_t[i][j] X[g H + smode[soff[t] + r]][j] for every
 // harmonic index t, column group g = (column, helicity) < G and subset rank r: one
-// 64 x 64 tile per workgroup.  Tiles are ordered (t, row tile, column tile), column
+// BM x BN tile per workgroup (64 x 64, or 128 x 32: a harmonic's subset is 13-48
+// modes per column group, and 32-wide column tiles pad less).  Tiles are ordered
+// (t, row tile, column tile), column
 // tile fastest (tiles[t] column tiles before t), and the 1-D grid is remapped so that
 // each XCD (workgroups are dealt to the 8 XCDs round-robin) runs a contiguous range
-// of them: the column tiles sharing one 64-row panel of TAB_t then run together on
+// of them: the column tiles sharing one BM-row panel of TAB_t then run together on
 // one XCD and read the panel through its L2.
-template <class T, bool KTAIL>
+template <class T, bool KTAIL, int BM, int BN>
 __global__ __launch_bounds__(256, 2) void k_zgemm_subsets(const cx<T> *TAB, const cx<T> *X, cx<T> *U, int N, int H,
                                                           int S, int G, int T_, const int32_t *tiles,
                                                           const int32_t *soff, const int32_t *smode) {
+    constexpr int WC = BN / 32;
     const int nb = gridDim.x, bid = blockIdx.x;
     const int xcd = bid & 7, q8 = nb >> 3, r8 = nb & 7;
     const int L = xcd * q8 + min(xcd, r8) + (bid >> 3);
-    const int R = (N + 63) / 64;  // row tiles
+    const int R = (N + BM - 1) / BM;  // row tiles
     int lo = 0, hi = T_ - 1;      // last t with tiles[t] R <= L
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -630,16 +670,16 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets(const cx<T> *TAB, cons
     const int nct = tiles[t + 1] - tiles[t], local = L - tiles[t] * R;
     const int rt = local / nct, ct = local - rt * nct;
     const int s0 = soff[t], nt = soff[t + 1] - s0, ncol = G * nt;
-    const int64_t c0 = (int64_t)ct * 64, r0 = (int64_t)rt * 64;
-    __shared__ TileLds<T> sm;
+    const int64_t c0 = (int64_t)ct * BN, r0 = (int64_t)rt * BM;
+    __shared__ TileLds<T, BM, BN> sm;
     Tile3M<T> acc;
     auto bcol = [=](int64_t c) {
         const int q = (int)min(c, (int64_t)ncol - 1), g = q / nt, r = q - g * nt;
         return X + ((int64_t)g * H + smode[s0 + r]) * N;
     };
-    zgemm_tile_3m_cols<T, KTAIL>(TAB + (int64_t)t * N * N, N, bcol, r0, c0, N, N, sm, acc);
+    zgemm_tile_3m_cols<T, KTAIL, BM, BN>(TAB + (int64_t)t * N * N, N, bcol, r0, c0, N, N, sm, acc);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, rr = lane & 15;
+    const int wm = (wave / WC) * 32, wn = (wave % WC) * 32, rr = lane & 15;
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
@@ -712,17 +752,19 @@ static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool reg
         const int nc = std::min(cmax, cols - c0);
         const int64_t Q = (int64_t)nc * 2 * H;
         k_pack_sources<R><<<grid1(Q * N), 256, 0, c->stream>>>(x + (int64_t)c0 * c->n, c->n, c->nh, N, H, nc, X);
-        const int32_t *tiles = c->plan.stiles + (size_t)(nc - 1) * (T + 1);
-        const unsigned grid = (unsigned)c->plan.ntiles[nc - 1] * (unsigned)((N + 63) / 64);
+        // FP32: 64 x 64 tiles (w = 0); FP64: 128 x 32 (w = 1)
+        constexpr int w = sizeof(R) == 8 ? 1 : 0, BN = kTileWidths[w], BM = 64 * 64 / BN;
+        const int32_t *tiles = c->plan.stiles + (size_t)(w * kGemmCols + nc - 1) * (T + 1);
+        const unsigned grid = (unsigned)c->plan.ntiles[w][nc - 1] * (unsigned)((N + BM - 1) / BM);
         Timer::Rec grec;
         const char *gname = regular ? "gemm_overlap" : sizeof(R) == 8 ? "gemm_f64" : "gemm_c64";
         TG_TRY(c->timer.begin(gname, c->stream, grec));
         if (N % 16)
-            k_zgemm_subsets<R, true><<<grid, 256, 0, c->stream>>>(TAB, X, U, N, H, S, 2 * nc, T, tiles,
-                                                                 c->plan.soff, c->plan.smode);
+            k_zgemm_subsets<R, true, BM, BN><<<grid, 256, 0, c->stream>>>(TAB, X, U, N, H, S, 2 * nc, T, tiles,
+                                                                         c->plan.soff, c->plan.smode);
         else
-            k_zgemm_subsets<R, false><<<grid, 256, 0, c->stream>>>(TAB, X, U, N, H, S, 2 * nc, T, tiles,
-                                                                  c->plan.soff, c->plan.smode);
+            k_zgemm_subsets<R, false, BM, BN><<<grid, 256, 0, c->stream>>>(TAB, X, U, N, H, S, 2 * nc, T, tiles,
+                                                                          c->plan.soff, c->plan.smode);
         TG_HIP(hipGetLastError());
         TG_TRY(c->timer.end(gname, c->stream, grec));
         k_tab_contract<R><<<dim3((unsigned)((N + 63) / 64), (unsigned)Q), 64, 0, c->stream>>>(

@@ -35,16 +35,17 @@ struct lcx {
     __device__ __forceinline__ void set(cx<T> v) { re = v.re; im = v.im; }
 };
 
-// LDS staging of one 16-wide K slice: A tile 64 x 16, B tile 16 x 64, split re / im.
-template <class T>
+// LDS staging of one 16-wide K slice: A tile BM x 16, B tile 16 x BN, split re / im.
+template <class T, int BM = 64, int BN = 64>
 struct TileLds {
-    T Ar[64][17], Ai[64][17], Br[16][65], Bi[16][65];
+    T Ar[BM][17], Ai[BM][17], Br[16][BN + 1], Bi[16][BN + 1];
 };
 
-// Accumulators of a 64 x 64 output tile computed by 4 waves (2 x 2 16x16 MFMA tiles
-// each), for the transposed product (B^T A^T) so that lanes run along the output
-// rows: element v of lane l of tile (a, b) is row wm + 16 a + l % 16, column
-// wn + 16 b + mfma_out_row(l, v), with wm = 32 (wave / 2), wn = 32 (wave % 2).
+// Accumulators of a BM x BN output tile computed by 4 waves of 32 x 32 (2 x 2 16x16
+// MFMA tiles each; BN / 32 waves across, BM / 32 down), for the transposed product
+// (B^T A^T) so that lanes run along the output rows: element v of lane l of tile
+// (a, b) is row wm + 16 a + l % 16, column wn + 16 b + mfma_out_row(l, v), with
+// wm = 32 (wave / (BN / 32)), wn = 32 (wave % (BN / 32)).
 // Re(A B) = t1 - t2, Im(A B) = t3 - t1 - t2.
 template <class T>
 struct Tile3M {
@@ -57,11 +58,13 @@ struct Tile3M {
 // the caller does not store.  With KTAIL, k >= K is read clamped and zeroed (K not a
 // multiple of 16).  No load is conditional; the next K slice is loaded while the
 // current one is multiplied.  256 threads.
-template <class T, bool KTAIL, class BCol>
+template <class T, bool KTAIL, int BM, int BN, class BCol>
 __device__ __forceinline__ void zgemm_tile_3m_cols(const cx<T> *A, int64_t lda, BCol bcol, int64_t r0, int64_t c0,
-                                                   int64_t M, int K, TileLds<T> &sm, Tile3M<T> &acc) {
+                                                   int64_t M, int K, TileLds<T, BM, BN> &sm, Tile3M<T> &acc) {
+    static_assert(BM * BN == 64 * 64 && BM % 32 == 0 && BN % 32 == 0, "4 waves of 32 x 32");
+    constexpr int WC = BN / 32, NA = BM / 16, NB = BN / 16, KA = 256 / BM;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int wm = (wave / WC) * 32, wn = (wave % WC) * 32;
     const int rr = lane & 15, kq = lane >> 4;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -69,25 +72,32 @@ __device__ __forceinline__ void zgemm_tile_3m_cols(const cx<T> *A, int64_t lda, 
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc.t1[a][b][v] = acc.t2[a][b][v] = acc.t3[a][b][v] = (T)0;
-    // slice loads: A (64 rows x 16) at (m = tid & 63, k = (tid >> 6) + 4 j),
-    // B (16 x 64 columns) at (k = tid & 15, column = (tid >> 4) + 16 j)
-    const int lm = tid & 63, lk = tid >> 6, uk = tid & 15, uc = tid >> 4;
+    // slice loads: A (BM rows x 16) at (m = tid % BM, k = tid / BM + KA j), j < NA,
+    // B (16 x BN columns) at (k = tid & 15, column = (tid >> 4) + 16 j), j < NB
+    const int lm = tid % BM, lk = tid / BM, uk = tid & 15, uc = tid >> 4;
     const int64_t arow = min(r0 + lm, M - 1);
-    const cx<T> *bptr[4];
+    const cx<T> *bptr[NB];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bptr[j] = bcol(c0 + uc + 16 * j);
-    cx<T> areg[4], breg[4];
+    for (int j = 0; j < NB; ++j) bptr[j] = bcol(c0 + uc + 16 * j);
+    cx<T> areg[NA], breg[NB];
     auto load_slice = [&](int kk) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int ka = kk + lk + 4 * j, kb = kk + uk;
+        for (int j = 0; j < NA; ++j) {
+            const int ka = kk + lk + KA * j;
             if (KTAIL) {
                 const cx<T> av = A[(int64_t)min(ka, K - 1) * lda + arow];
-                const cx<T> bv = bptr[j][min(kb, K - 1)];
                 areg[j] = ka < K ? av : cx<T>();
-                breg[j] = kb < K ? bv : cx<T>();
             } else {
                 areg[j] = A[(int64_t)ka * lda + arow];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int kb = kk + uk;
+            if (KTAIL) {
+                const cx<T> bv = bptr[j][min(kb, K - 1)];
+                breg[j] = kb < K ? bv : cx<T>();
+            } else {
                 breg[j] = bptr[j][kb];
             }
         }
@@ -96,9 +106,12 @@ __device__ __forceinline__ void zgemm_tile_3m_cols(const cx<T> *A, int64_t lda, 
     for (int kk = 0; kk < K; kk += 16) {
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            sm.Ar[lm][lk + 4 * j] = areg[j].re;
-            sm.Ai[lm][lk + 4 * j] = areg[j].im;
+        for (int j = 0; j < NA; ++j) {
+            sm.Ar[lm][lk + KA * j] = areg[j].re;
+            sm.Ai[lm][lk + KA * j] = areg[j].im;
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
             sm.Br[uk][uc + 16 * j] = breg[j].re;
             sm.Bi[uk][uc + 16 * j] = breg[j].im;
         }
@@ -127,7 +140,7 @@ template <class T, bool KTAIL>
 __device__ __forceinline__ void zgemm_tile_3m(const cx<T> *A, int64_t lda, const cx<T> *B, int64_t ldb, int64_t r0,
                                               int64_t c0, int64_t M, int64_t N, int K, TileLds<T> &sm,
                                               Tile3M<T> &acc) {
-    zgemm_tile_3m_cols<T, KTAIL>(
+    zgemm_tile_3m_cols<T, KTAIL, 64, 64>(
         A, lda, [=](int64_t c) { return B + min(c, N - 1) * ldb; }, r0, c0, M, K, sm, acc);
 }
 
