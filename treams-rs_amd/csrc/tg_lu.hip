@@ -52,63 +52,102 @@ struct Batch {
     const int64_t *voff;  // row offset of each matrix in the packed vectors and pivots
 };
 
-// Factor the nb x nb diagonal block at k0 in LDS, one wave, lane r owning row r for
-// the update; piv[k0 + k] = local row exchanged with row k0 + k.
+// Value of x in lane `lane` (wave-uniform) for every lane.
+__device__ __forceinline__ float read_lane(float x, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+__device__ __forceinline__ double read_lane(double x, int lane) {
+    const long long bits = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)bits, lane), hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 template <class T>
-__global__ __launch_bounds__(64) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0, int *piv0, int *status) {
+__device__ __forceinline__ cx<T> read_lane(cx<T> x, int lane) {
+    return cx<T>(read_lane(x.re, lane), read_lane(x.im, lane));
+}
+
+// Factor the nb x nb diagonal block at k0 with partial pivoting inside the block;
+// piv[k0 + k] = row exchanged with row k0 + k.  Four waves: lane r holds row r, wave w
+// its columns 16 w .. 16 w + 15 in registers.  Step k: the wave owning column k finds
+// the pivot (wave argmax) and the multipliers and publishes them in LDS (double
+// buffered: one barrier per step); every wave then exchanges rows k and p and updates
+// its columns right of k, reading the pivot row with readlane.  Each step is a
+// latency chain (argmax, division, barrier): about 2 us whatever the layout.
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0, int *piv0, int *status) {
     const int z = blockIdx.z;
     const int64_t n = bt.n[z];
     if (k0 >= n) return;
     cx<T> *A = A0 + bt.off[z];
     int *piv = piv0 + bt.voff[z];
     const int nb = (int)min((int64_t)NB, n - k0);
-    __shared__ lcx<T> blk[NB][NB];  // [col][row]
-    const int lane = threadIdx.x;
-    // lane r loads row r of 16 columns at a time (16 loads in flight), clamped rows for r >= nb
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ lcx<T> lmul[2][NB];
+    __shared__ int spiv[2];
+    cx<T> v[16];
     const int64_t row = k0 + min(lane, nb - 1);
-    for (int c0 = 0; c0 < nb; c0 += 16) {
-        cx<T> v[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = A[(k0 + min(c0 + u, nb - 1)) * n + row];
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-            if (c0 + u < nb && lane < nb) blk[c0 + u][lane].set(v[u]);
+    for (int j = 0; j < 16; ++j) {
+        const int c = 16 * wave + j;
+        const cx<T> e = A[(k0 + min(c, nb - 1)) * n + row];
+        v[j] = (lane < nb && c < nb) ? e : cx<T>();
     }
-    __syncthreads();
-    for (int k = 0; k < nb; ++k) {
-        double best = (lane >= k && lane < nb) ? (double)abs2(blk[k][lane].get()) : -1.0;
-        int besti = lane;
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(best, off);
-            const int oi = __shfl_xor(besti, off);
-            if (ov > best || (ov == best && oi < besti)) {
-                best = ov;
-                besti = oi;
+    for (int ko = 0; ko < 4 && 16 * ko < nb; ++ko) {
+#pragma unroll
+        for (int kc = 0; kc < 16; ++kc) {
+            const int k = 16 * ko + kc;
+            if (k >= nb) break;
+            const int buf = k & 1;
+            if (wave == ko) {
+                double best = (lane >= k && lane < nb) ? (double)abs2(v[kc]) : -1.0;
+                int besti = lane;
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ov = __shfl_xor(best, off);
+                    const int oi = __shfl_xor(besti, off);
+                    if (ov > best || (ov == best && oi < besti)) {
+                        best = ov;
+                        besti = oi;
+                    }
+                }
+                const int p = __builtin_amdgcn_readfirstlane(besti);
+                if (!(best > 0.0)) {
+                    if (lane == 0) spiv[buf] = -1;
+                } else {
+                    const cx<T> pv = read_lane(v[kc], p), kv = read_lane(v[kc], k);
+                    const cx<T> inv = cx<T>((T)1) / pv;
+                    const cx<T> mine = lane == p ? kv : v[kc];  // row `lane` after the exchange
+                    lmul[buf][lane].set(lane > k && lane < nb ? mine * inv : cx<T>());
+                    if (lane == 0) {
+                        spiv[buf] = p;
+                        piv[k0 + k] = (int)(k0 + p);
+                    }
+                }
+            }
+            __syncthreads();
+            const int p = __builtin_amdgcn_readfirstlane(spiv[buf]);
+            if (p < 0) {
+                if (threadIdx.x == 0) status[z] = 1;
+                return;
+            }
+            const cx<T> l = lmul[buf][lane].get();
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int c = 16 * wave + j;
+                const cx<T> rk = read_lane(v[j], p), rp = read_lane(v[j], k);  // new rows k and p
+                cx<T> x = lane == k ? rk : (lane == p ? rp : v[j]);
+                if (c > k) {
+                    if (lane > k) x -= l * rk;
+                } else if (c == k) {
+                    if (lane > k) x = l;
+                }
+                v[j] = x;
             }
         }
-        if (!(best > 0.0)) {
-            if (lane == 0) status[z] = 1;
-            return;
-        }
-        const int p = besti;
-        if (lane == 0) piv[k0 + k] = (int)(k0 + p);
-        if (p != k && lane < nb) {
-            const lcx<T> tmp = blk[lane][k];
-            blk[lane][k] = blk[lane][p];
-            blk[lane][p] = tmp;
-        }
-        __syncthreads();
-        const cx<T> inv = cx<T>((T)1) / blk[k][k].get();
-        if (lane > k && lane < nb) {
-            const cx<T> lr = blk[k][lane].get() * inv;
-            blk[k][lane].set(lr);
-            for (int cc = k + 1; cc < nb; ++cc) blk[cc][lane].set(blk[cc][lane].get() - lr * blk[cc][k].get());
-        }
-        __syncthreads();
     }
-    for (int t = lane; t < nb * nb; t += 64) {
-        const int col = t / nb, row = t - col * nb;
-        A[(k0 + col) * n + k0 + row] = blk[col][row].get();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int c = 16 * wave + j;
+        if (lane < nb && c < nb) A[(k0 + c) * n + k0 + lane] = v[j];
     }
 }
 
@@ -151,8 +190,9 @@ __global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A0, Batch bt, i
     const int64_t r = k0 + nb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     // x U11 = a for this row.  c64: the row is held in registers (all 64 loads in
-    // flight; the launch has at most a few waves per SIMD anyway); c128 re-reads the
-    // solved entries, which stay in L1.
+    // flight; the launch has at most a few waves per SIMD anyway); c128 (whose row
+    // would not fit the 256 architectural registers) re-reads the solved entries,
+    // which stay in L1.
     if constexpr (sizeof(T) == 4) {
         cx<T> x[NB];
 #pragma unroll
@@ -174,13 +214,15 @@ __global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A0, Batch bt, i
     }
 }
 
-// U12 = L11^{-1} A12 (unit lower L11), in place: one wave per column, lane r holding
-// row r; each step broadcasts the solved entry and updates the rows below it.
+// U12 = L11^{-1} A12 (unit lower L11 of the block at k0) for the columns [cb, ce), in
+// place: one wave per column, lane r holding row r; each step broadcasts the solved
+// entry and updates the rows below it.
 template <class T>
-__global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A0, Batch bt, int64_t k0) {
+__global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A0, Batch bt, int64_t k0, int64_t cb, int64_t ce) {
     const int z = blockIdx.z;
     const int64_t n = bt.n[z];
-    if (k0 + NB >= n) return;
+    if (k0 + NB >= n || cb >= n) return;
+    const int64_t cend = min(ce, n);
     cx<T> *A = A0 + bt.off[z];
     const int nb = NB;
     __shared__ lcx<T> L[NB][NB];  // [col][row]
@@ -191,7 +233,7 @@ __global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A0, Batch bt, i
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t col = k0 + nb + (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); col < n;
+    for (int64_t col = cb + (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); col < cend;
          col += waves) {
         cx<T> *c = A + col * n + k0;
         cx<T> v = c[lane];
@@ -203,15 +245,19 @@ __global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A0, Batch bt, i
     }
 }
 
-// A22 -= L21 U12 with the 3M complex product on MFMA, one 64 x 64 tile of A22 per
-// workgroup; rows and columns beyond n are never written.
+// A[rb : re, cb : ce] -= A[rb : re, k0 : k0 + K] A[k0 : k0 + K, cb : ce] (rows and
+// columns clipped to n; K a multiple of 16 with k0 + K <= n wherever a tile is left)
+// with the 3M complex product on MFMA, one 64 x 64 tile per workgroup; rows and
+// columns beyond the range are never written.
 template <class T>
-__global__ __launch_bounds__(256, 2) void k_lu_update(cx<T> *A0, Batch bt, int64_t k0) {
+__global__ __launch_bounds__(256, 2) void k_lu_update(cx<T> *A0, Batch bt, int64_t k0, int K, int64_t rb, int64_t re,
+                                                      int64_t cb, int64_t ce) {
     const int z = blockIdx.z;
-    const int64_t n = bt.n[z];
-    const int nb = NB;
-    const int64_t r0 = k0 + nb + (int64_t)blockIdx.y * 64, c0 = k0 + nb + (int64_t)blockIdx.x * 64;
-    if (r0 >= n || c0 >= n) return;  // uniform over the workgroup
+    const int64_t nz = bt.n[z];
+    const int64_t rend = min(re, nz), cend = min(ce, nz);
+    const int64_t r0 = rb + (int64_t)blockIdx.y * 64, c0 = cb + (int64_t)blockIdx.x * 64;
+    if (r0 >= rend || c0 >= cend) return;  // uniform over the workgroup
+    const int64_t n = nz;
     cx<T> *A = A0 + bt.off[z];
     __shared__ TileLds<T> sm;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -227,15 +273,15 @@ __global__ __launch_bounds__(256, 2) void k_lu_update(cx<T> *A0, Batch bt, int64
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int64_t gr = min(r0 + wm + a * 16 + rr, n - 1);
-                    const int64_t gc = min(c0 + wn + b * 16 + mfma_out_row<T>(lane, v), n - 1);
+                    const int64_t gr = min(r0 + wm + a * 16 + rr, rend - 1);
+                    const int64_t gc = min(c0 + wn + b * 16 + mfma_out_row<T>(lane, v), cend - 1);
                     creg[a][b][v] = A[gc * n + gr];
                 }
     };
     if (prefetch) load_c();
     Tile3M<T> acc;
-    // L21 = A[:, k0 : k0 + nb] (rows absolute), U12 = A[k0 : k0 + nb, :] (columns absolute)
-    zgemm_tile_3m<T, false>(A + k0 * n, n, A + k0, n, r0, c0, n, n, nb, sm, acc);
+    // L21 = A[:, k0 : k0 + K] (rows absolute), U12 = A[k0 : k0 + K, :] (columns absolute)
+    zgemm_tile_3m<T, false>(A + k0 * n, n, A + k0, n, r0, c0, rend, cend, K, sm, acc);
     if (!prefetch) load_c();
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -248,7 +294,7 @@ __global__ __launch_bounds__(256, 2) void k_lu_update(cx<T> *A0, Batch bt, int64
                 cx<T> d = creg[a][b][v];
                 d.re -= t1 - t2;
                 d.im -= t3 - t1 - t2;
-                if (gr < n && gc < n) A[gc * n + gr] = d;
+                if (gr < rend && gc < cend) A[gc * n + gr] = d;
             }
 }
 
@@ -467,16 +513,39 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
     const unsigned z = (unsigned)ls.count;
     const int64_t nmax = ls.nmax;
     TG_HIP(hipMemsetAsync(ls.status, 0, sizeof(int) * ls.count, s));
-    for (int64_t k0 = 0; k0 < nmax; k0 += NB) {
-        k_lu_diag<T><<<dim3(1, 1, z), 64, 0, s>>>(A, bt, k0, ls.piv, ls.status);
-        k_lu_swap<T><<<dim3((unsigned)((nmax + 255) / 256), 1, z), 256, 0, s>>>(A, bt, k0, ls.piv);
-        const int64_t rest = nmax - k0 - NB;
-        if (rest > 0) {
-            k_trsm_lower_panel<T><<<dim3((unsigned)((rest + 255) / 256), 1, z), 256, 0, s>>>(A, bt, k0);
-            k_trsm_upper_panel<T>
-                <<<dim3((unsigned)std::min<int64_t>((rest + 3) / 4, 1024), 1, z), 256, 0, s>>>(A, bt, k0);
-            const unsigned tiles = (unsigned)((rest + 63) / 64);
-            k_lu_update<T><<<dim3(tiles, tiles, z), 256, 0, s>>>(A, bt, k0);
+    auto diag = [&](int64_t k) {
+        k_lu_diag<T><<<dim3(1, 1, z), 256, 0, s>>>(A, bt, k, ls.piv, ls.status);
+        k_lu_swap<T><<<dim3((unsigned)((nmax + 255) / 256), 1, z), 256, 0, s>>>(A, bt, k, ls.piv);
+    };
+    auto lower = [&](int64_t k) {  // rows below the block at k
+        k_trsm_lower_panel<T><<<dim3((unsigned)((nmax - k - NB + 255) / 256), 1, z), 256, 0, s>>>(A, bt, k);
+    };
+    auto upper = [&](int64_t k, int64_t cb, int64_t ce) {
+        k_trsm_upper_panel<T><<<dim3((unsigned)std::min<int64_t>((ce - cb + 3) / 4, 1024), 1, z), 256, 0, s>>>(
+            A, bt, k, cb, ce);
+    };
+    auto update = [&](int64_t k, int K, int64_t rb, int64_t re, int64_t cb, int64_t ce) {
+        k_lu_update<T><<<dim3((unsigned)((ce - cb + 63) / 64), (unsigned)((re - rb + 63) / 64), z), 256, 0, s>>>(
+            A, bt, k, K, rb, re, cb, ce);
+    };
+    // Pairs of 64-wide blocks a = [k0, k1), b = [k1, k2): the part of the trailing
+    // matrix right of b is updated once with K = 128 instead of twice with K = 64,
+    // halving its read-modify-write traffic (the update's HBM bound).
+    for (int64_t k0 = 0; k0 < nmax; k0 += 2 * NB) {
+        const int64_t k1 = k0 + NB, k2 = k1 + NB;
+        diag(k0);
+        if (k1 < nmax) {
+            lower(k0);
+            upper(k0, k1, k2);                    // U_ab
+            update(k0, NB, k1, nmax, k1, k2);     // block column b below block a
+            diag(k1);
+            if (k2 < nmax) {
+                lower(k1);
+                upper(k0, k2, nmax);              // U12 of block a
+                update(k0, NB, k1, k2, k2, nmax); // rows of b: A12_b -= L_ba U12_a
+                upper(k1, k2, nmax);              // U12 of block b
+                update(k0, 2 * NB, k2, nmax, k2, nmax);
+            }
         }
         TG_HIP(hipGetLastError());
     }
