@@ -151,23 +151,50 @@ __global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0
     }
 }
 
-// Apply the block's row exchanges to every column outside the diagonal block.
+// Apply the row exchanges of the block at k0 to the columns [cb, ce) outside the
+// block.  Pivots are clamped to the block's rows: after a zero pivot the rest of the
+// block's entries are never written, and the factorisation reports TG_SINGULAR.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_swap(cx<T> *A0, Batch bt, int64_t k0, const int *piv0) {
+__global__ __launch_bounds__(256) void k_lu_swap(cx<T> *A0, Batch bt, int64_t k0, const int *piv0, int64_t cb,
+                                                 int64_t ce) {
     const int z = blockIdx.z;
     const int64_t n = bt.n[z];
     if (k0 >= n) return;
     const int nb = (int)min((int64_t)NB, n - k0);
-    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= n || (col >= k0 && col < k0 + nb)) return;
+    const int64_t col = cb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= min(ce, n) || (col >= k0 && col < k0 + nb)) return;
     cx<T> *c = A0 + bt.off[z] + col * n;
     const int *piv = piv0 + bt.voff[z];
     for (int k = 0; k < nb; ++k) {
-        const int64_t p = piv[k0 + k];
+        const int64_t p = min(max((int64_t)piv[k0 + k], k0), k0 + nb - 1);
         if (p != k0 + k) {
             cx<T> tmp = c[k0 + k];
             c[k0 + k] = c[p];
             c[p] = tmp;
+        }
+    }
+}
+
+// The deferred exchanges of the L columns: column c takes the row exchanges of every
+// block of a later block pair (the blocks at and after the pair start following c),
+// in block order.  Blocks cover disjoint rows, so their exchanges commute.
+template <class T>
+__global__ __launch_bounds__(256) void k_lu_swap_left(cx<T> *A0, Batch bt, const int *piv0) {
+    const int z = blockIdx.z;
+    const int64_t n = bt.n[z];
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n) return;
+    cx<T> *c = A0 + bt.off[z] + col * n;
+    const int *piv = piv0 + bt.voff[z];
+    for (int64_t k0 = (col / (2 * NB) + 1) * (2 * NB); k0 < n; k0 += NB) {
+        const int nb = (int)min((int64_t)NB, n - k0);
+        for (int k = 0; k < nb; ++k) {
+            const int64_t p = min(max((int64_t)piv[k0 + k], k0), k0 + nb - 1);
+            if (p != k0 + k) {
+                cx<T> tmp = c[k0 + k];
+                c[k0 + k] = c[p];
+                c[p] = tmp;
+            }
         }
     }
 }
@@ -513,42 +540,89 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
     const unsigned z = (unsigned)ls.count;
     const int64_t nmax = ls.nmax;
     TG_HIP(hipMemsetAsync(ls.status, 0, sizeof(int) * ls.count, s));
-    auto diag = [&](int64_t k) {
-        k_lu_diag<T><<<dim3(1, 1, z), 256, 0, s>>>(A, bt, k, ls.piv, ls.status);
-        k_lu_swap<T><<<dim3((unsigned)((nmax + 255) / 256), 1, z), 256, 0, s>>>(A, bt, k, ls.piv);
+    // Block pairs a = [p0, p1), b = [p1, p2): the trailing matrix right of b is updated
+    // once with K = 128 instead of twice with K = 64 (halving its read-modify-write
+    // traffic, the update's HBM bound).  Look-ahead: the panel of pair j + 1 (its two
+    // diagonal blocks, their panel solves and the update between them) runs on a second
+    // stream while the main stream updates the columns right of that panel.  Row
+    // exchanges reach the panel's own columns at once, the columns right of the pair
+    // on the main stream before their panel solves, and the L columns left of the
+    // pair in one pass at the end.
+    hipStream_t ps = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+    int st = TG_OK;
+    auto check = [&](hipError_t e) {
+        if (e != hipSuccess && st == TG_OK) {
+            set_error("HIP error %s in the LU factorisation", hipGetErrorString(e));
+            st = TG_DEVICE_ERROR;
+        }
     };
-    auto lower = [&](int64_t k) {  // rows below the block at k
-        k_trsm_lower_panel<T><<<dim3((unsigned)((nmax - k - NB + 255) / 256), 1, z), 256, 0, s>>>(A, bt, k);
+    check(hipStreamCreateWithFlags(&ps, hipStreamNonBlocking));
+    check(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    check(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    auto swap = [&](hipStream_t q, int64_t k, int64_t cb, int64_t ce) {
+        if (ce > cb)
+            k_lu_swap<T><<<dim3((unsigned)((ce - cb + 255) / 256), 1, z), 256, 0, q>>>(A, bt, k, ls.piv, cb, ce);
     };
-    auto upper = [&](int64_t k, int64_t cb, int64_t ce) {
-        k_trsm_upper_panel<T><<<dim3((unsigned)std::min<int64_t>((ce - cb + 3) / 4, 1024), 1, z), 256, 0, s>>>(
+    auto diag = [&](hipStream_t q, int64_t k) {
+        k_lu_diag<T><<<dim3(1, 1, z), 256, 0, q>>>(A, bt, k, ls.piv, ls.status);
+    };
+    auto lower = [&](hipStream_t q, int64_t k) {  // rows below the block at k
+        k_trsm_lower_panel<T><<<dim3((unsigned)((nmax - k - NB + 255) / 256), 1, z), 256, 0, q>>>(A, bt, k);
+    };
+    auto upper = [&](hipStream_t q, int64_t k, int64_t cb, int64_t ce) {
+        k_trsm_upper_panel<T><<<dim3((unsigned)std::min<int64_t>((ce - cb + 3) / 4, 1024), 1, z), 256, 0, q>>>(
             A, bt, k, cb, ce);
     };
-    auto update = [&](int64_t k, int K, int64_t rb, int64_t re, int64_t cb, int64_t ce) {
-        k_lu_update<T><<<dim3((unsigned)((ce - cb + 63) / 64), (unsigned)((re - rb + 63) / 64), z), 256, 0, s>>>(
-            A, bt, k, K, rb, re, cb, ce);
+    auto update = [&](hipStream_t q, int64_t k, int K, int64_t rb, int64_t re, int64_t cb, int64_t ce) {
+        if (ce > cb && re > rb)
+            k_lu_update<T><<<dim3((unsigned)((ce - cb + 63) / 64), (unsigned)((re - rb + 63) / 64), z), 256, 0, q>>>(
+                A, bt, k, K, rb, re, cb, ce);
     };
-    // Pairs of 64-wide blocks a = [k0, k1), b = [k1, k2): the part of the trailing
-    // matrix right of b is updated once with K = 128 instead of twice with K = 64,
-    // halving its read-modify-write traffic (the update's HBM bound).
-    for (int64_t k0 = 0; k0 < nmax; k0 += 2 * NB) {
-        const int64_t k1 = k0 + NB, k2 = k1 + NB;
-        diag(k0);
-        if (k1 < nmax) {
-            lower(k0);
-            upper(k0, k1, k2);                    // U_ab
-            update(k0, NB, k1, nmax, k1, k2);     // block column b below block a
-            diag(k1);
-            if (k2 < nmax) {
-                lower(k1);
-                upper(k0, k2, nmax);              // U12 of block a
-                update(k0, NB, k1, k2, k2, nmax); // rows of b: A12_b -= L_ba U12_a
-                upper(k1, k2, nmax);              // U12 of block b
-                update(k0, 2 * NB, k2, nmax, k2, nmax);
-            }
-        }
-        TG_HIP(hipGetLastError());
+    if (st == TG_OK) {
+        check(hipEventRecord(ready, s));  // the panel stream starts after the work before
+        check(hipStreamWaitEvent(ps, ready, 0));
     }
+    for (int64_t p0 = 0; p0 < nmax && st == TG_OK; p0 += 2 * NB) {
+        const int64_t p1 = p0 + NB, p2 = p1 + NB, p4 = p2 + 2 * NB;
+        const int64_t pe = std::min(p2, nmax);
+        // the panel [p0, p2), on the panel stream
+        diag(ps, p0);
+        swap(ps, p0, p0, pe);
+        if (p1 < nmax) {
+            lower(ps, p0);
+            upper(ps, p0, p1, p2);                  // U_ab
+            update(ps, p0, NB, p1, nmax, p1, p2);   // block column b below block a
+            diag(ps, p1);
+            swap(ps, p1, p0, pe);                   // includes L_ba
+            if (p2 < nmax) lower(ps, p1);
+        }
+        check(hipGetLastError());
+        check(hipEventRecord(done, ps));
+        check(hipStreamWaitEvent(s, done, 0));
+        if (p2 >= nmax) continue;
+        // right of the panel, on the main stream
+        swap(s, p0, p2, nmax);
+        swap(s, p1, p2, nmax);
+        upper(s, p0, p2, nmax);                     // U12 of block a
+        update(s, p0, NB, p1, p2, p2, nmax);        // rows of b: A12_b -= L_ba U12_a
+        upper(s, p1, p2, nmax);                     // U12 of block b
+        update(s, p0, 2 * NB, p2, nmax, p2, std::min(p4, nmax));  // the next panel first
+        check(hipEventRecord(ready, s));
+        check(hipStreamWaitEvent(ps, ready, 0));
+        update(s, p0, 2 * NB, p2, nmax, p4, nmax);  // overlaps the next panel
+        check(hipGetLastError());
+    }
+    if (st == TG_OK && nmax > 2 * NB)
+        k_lu_swap_left<T><<<dim3((unsigned)((nmax + 255) / 256), 1, z), 256, 0, s>>>(A, bt, ls.piv);
+    check(hipGetLastError());
+    if (ps) {
+        (void)hipStreamSynchronize(ps);
+        (void)hipStreamDestroy(ps);
+    }
+    if (ready) (void)hipEventDestroy(ready);
+    if (done) (void)hipEventDestroy(done);
+    if (st != TG_OK) return st;
     std::vector<int> stat(ls.count), piv_h(ls.ntot), perm_h(ls.ntot);
     TG_HIP(hipMemcpyAsync(stat.data(), ls.status, sizeof(int) * ls.count, hipMemcpyDeviceToHost, s));
     TG_HIP(hipMemcpyAsync(piv_h.data(), ls.piv, sizeof(int) * ls.ntot, hipMemcpyDeviceToHost, s));
