@@ -242,9 +242,16 @@ typedef struct tg_factor tg_factor;
  * the factors take n^2 x 8 or 16 bytes.  Pivoting is partial within 64-row diagonal
  * blocks (the reference pivots over whole columns); the FP64 residual certifies every
  * solve.  TG_SINGULAR for a zero pivot.  The cluster must outlive the factor.
+ * An operator whose entries span more than a factor 10 while a diagonal entry is below
+ * a tenth of the largest is equilibrated first (linalg/mod.rs:137-236: rows and
+ * columns scaled by their largest magnitudes, rounded here to powers of two); its
+ * solves are then certified by the residual of the scaled system.
  */
 int tg_factor_create(tg_cluster *cluster, int precision, tg_factor **out);
 int tg_factor_destroy(tg_factor *factor);
+
+/* 1 when the factors are those of the equilibrated operator D_r A D_c, else 0. */
+int tg_factor_equilibrated(const tg_factor *factor);
 
 /*
  * adjoint == 0: X = (I - T C)^{-1} T A for `cols` incident columns (device c128,
@@ -257,6 +264,18 @@ int tg_factor_destroy(tg_factor *factor);
  */
 int tg_factor_solve(tg_factor *factor, const tg_c128 *incident_dev, int cols, double rtol, int adjoint,
                     tg_c128 *x_dev, tg_convergence *conv_host);
+
+/*
+ * The cluster's T-matrix in one spherical basis of the same lmax at the origin:
+ * T_g = sum_ij R(0 <- p_i) X_ij R(p_j <- 0), X = (I - T C)^{-1} T, with regular
+ * translations; out_dev is M x M c128 column-major (M = 2 lmax (lmax + 2), the
+ * sphere mode order).  Replaces TMatrix::expand(SphericalBasis.default(lmax)) of a
+ * solved sphere cluster (python/treams_rs/_tmatrix.py:243-257 with
+ * sw/expansion.rs:116-132, on sphere_cluster's X, cluster/spheres.rs:56-80).  Solves
+ * the M columns T R(p_j <- 0) with the factors (conv_host: M entries); rtol as in
+ * tg_factor_solve.
+ */
+int tg_factor_global_tmatrix(tg_factor *factor, double rtol, tg_c128 *out_dev, tg_convergence *conv_host);
 
 /*
  * tg_cluster_pullback with the adjoint solve by the LU factors: replaces

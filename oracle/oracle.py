@@ -420,6 +420,47 @@ class Cluster:
         """x = (I - T C)^{-1} T a (cluster/interaction.rs:263-266)."""
         return np.linalg.solve(self.operator(), apply_local(self.blocks, incident, False))
 
+    def interaction_tmatrix(self, equilibrate=True):
+        """X = (I - T C)^{-1} T, all n columns (cluster/spheres.rs:56-80), by LAPACK on the
+        operator equilibrated as in linalg/mod.rs:137-236: when its entries span more
+        than a factor 10 and a diagonal entry is below a tenth of the largest, rows
+        and columns are scaled by the reciprocal of their largest magnitude
+        (max(|re|, |im|)), the more disparate axis first."""
+        A = self.operator()
+        rhs = self.T
+        if not equilibrate:
+            return np.linalg.solve(A, rhs)
+        mag = np.maximum(np.abs(A.real), np.abs(A.imag))
+        big = mag.max()
+        nz = mag[mag > 0]
+        if nz.min() / big >= 0.1 or np.all(np.diag(mag) / big >= 0.1):
+            return np.linalg.solve(A, rhs)
+        rmax, cmax = mag.max(axis=1), mag.max(axis=0)
+        if rmax.min() / big >= 0.1 and cmax.min() / big >= 0.1:
+            return np.linalg.solve(A, rhs)
+        if cmax.min() < rmax.min():
+            c = 1.0 / cmax
+            r = 1.0 / (mag * c[None, :]).max(axis=1)
+        else:
+            r = 1.0 / rmax
+            c = 1.0 / (mag * r[:, None]).max(axis=0)
+        S = A * r[:, None] * c[None, :]
+        return c[:, None] * np.linalg.solve(S, r[:, None] * rhs)
+
+    def global_tmatrix(self, equilibrate=True):
+        """The cluster's T-matrix in one spherical basis at the origin with the same
+        lmax: T_g = sum_ij R(0 <- p_i) X_ij R(p_j <- 0) with regular translations
+        (python/treams_rs/_tmatrix.py:243-257, sw/expansion.rs:116-132)."""
+        X = self.interaction_tmatrix(equilibrate)
+        M = self.M
+        Tg = np.zeros((M, M), np.complex128)
+        rout = [translation_block(self.lmax, self.k0, -self.positions[i], True) for i in range(self.N)]
+        rin = [translation_block(self.lmax, self.k0, self.positions[j], True) for j in range(self.N)]
+        for i in range(self.N):
+            for j in range(self.N):
+                Tg += rout[i] @ X[i * M:(i + 1) * M, j * M:(j + 1) * M] @ rin[j]
+        return Tg
+
     def solve_gmres(self, incident, **options):
         """Matrix-free GMRES per column (cluster/iterative.rs:236-284)."""
         rhs = apply_local(self.blocks, incident, False)

@@ -178,3 +178,36 @@ def test_factor_invalid_input(pkg, oracle):
     f.close()
     with pytest.raises(ValueError):
         f.solve(np.zeros(cl.n, np.complex128))
+
+
+def test_global_tmatrix_matches_oracle(pkg, oracle):
+    """T-matrix of a 5-sphere cluster in one basis at the origin (regular translations,
+    all M columns solved with the factors) against the dense oracle; an operator
+    within a factor 10 of its diagonal is not equilibrated."""
+    cl, ref = _cluster(pkg, oracle, N=5, lmax=3, seed=12, side=2.4)
+    f = pkg.SphereClusterFactor(cl, "c128")
+    assert not f.equilibrated
+    Tg, conv = f.global_tmatrix()
+    assert len(conv) == cl.modes_per_sphere
+    assert _rel(Tg, ref.global_tmatrix()) <= 1e-10
+    f.close()
+
+
+def test_global_tmatrix_sic_chain_golden(pkg):
+    """The reference's golden (tests/tmatrix/test_particle_cluster.py:538-559): four
+    250 nm SiC spheres with 20 nm gaps at 3 THz, lmax 10, whose operator I - T C has
+    unit diagonal and entries above 1e22.  The factors are those of the equilibrated
+    operator; the absorption of the global T-matrix matches to rtol 2e-11 and distinct
+    azimuthal orders do not couple."""
+    k0 = 2 * np.pi * 3e12 / 299792458.0
+    pos = np.array([[0, 0, z * 520e-9] for z in (-1.5, -0.5, 0.5, 1.5)])
+    cl = pkg.IterativeSphereCluster(10, k0, np.full(4, 250e-9), np.full(4, 12.87707323 + 0.19225647000000007j), pos)
+    f = pkg.SphereClusterFactor(cl, "c128", rtol=1e-13)
+    assert f.equilibrated
+    Tg, conv = f.global_tmatrix()
+    T = Tg.cpu().numpy()
+    absorption = -2 * np.pi * (np.trace(T).real + np.vdot(T, T).real) / k0 ** 2
+    assert abs(absorption * 1e12 / 0.0001754433586134972 - 1) <= 2e-11, absorption * 1e12
+    m = np.array([mm for l in range(1, 11) for mm in range(-l, l + 1) for _ in (0, 1)])
+    assert np.abs(T[m[:, None] != m[None, :]]).max() <= 2e-20
+    f.close()

@@ -276,6 +276,32 @@ def test_reverse_gradient_example(oracle, golden_dir):
     assert np.abs(np.round(actual, 6) - expected).max() <= 1.5e-6
 
 
+def sic_chain(oracle):
+    """Four 250 nm SiC spheres with 20 nm gaps on the z axis at 3 THz, lmax 10: the
+    reference's test_small_wavenumber_high_order_chain_absorption_is_stable
+    (tests/tmatrix/test_particle_cluster.py:538-559)."""
+    k0 = 2 * np.pi * 3e12 / 299792458.0
+    pos = np.array([[0, 0, z * 520e-9] for z in (-1.5, -0.5, 0.5, 1.5)])
+    return oracle.Cluster(10, k0, [250e-9] * 4, 12.87707323 + 0.19225647000000007j, pos), k0
+
+
+def test_sic_chain_global_absorption_golden(oracle):
+    """The reference's golden absorption (rtol 2e-11) of the global T-matrix of a
+    cluster whose operator has unit diagonal and entries above 1e22: only the
+    equilibrated solve reaches it (the unscaled one misses by ~5e-4)."""
+    cl, k0 = sic_chain(oracle)
+    assert np.abs(cl.operator()).max() > 1e22
+    Tg = cl.global_tmatrix()
+    absorption = -2 * np.pi * (np.trace(Tg).real + np.vdot(Tg, Tg).real) / k0 ** 2
+    assert abs(absorption * 1e12 / 0.0001754433586134972 - 1) <= 2e-11
+    # axial symmetry: no coupling between distinct azimuthal orders
+    m = np.array([mm for l in range(1, 11) for mm in range(-l, l + 1) for _ in (0, 1)])
+    assert np.abs(Tg[m[:, None] != m[None, :]]).max() <= 2e-20
+    unscaled = cl.global_tmatrix(equilibrate=False)
+    a0 = -2 * np.pi * (np.trace(unscaled).real + np.vdot(unscaled, unscaled).real) / k0 ** 2
+    assert abs(a0 * 1e12 / 0.0001754433586134972 - 1) > 1e-6
+
+
 def test_gmres_matches_dense(oracle):
     rng = np.random.default_rng(1)
     N, lmax = 5, 2

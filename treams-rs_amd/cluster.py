@@ -322,6 +322,27 @@ class SphereClusterFactor:
         x, conv = self.solve(incident)
         return x, FactorContext(self, incident, x), conv
 
+    @property
+    def equilibrated(self) -> bool:
+        """Whether the factors are those of the row/column equilibrated operator
+        (linalg/mod.rs:137-236)."""
+        return bool(lib().tg_factor_equilibrated(self._h))
+
+    def global_tmatrix(self):
+        """The cluster's T-matrix in one spherical basis of the same lmax at the
+        origin, ``Cluster(...).solve().expand(SphericalBasis.default(lmax))``
+        (python/treams_rs/_tmatrix.py:243-257): M x M, sphere mode order.  Returns
+        (T, [Convergence] of the M solves)."""
+        if self._h is None:
+            raise ValueError("the factor was closed")
+        M = self.cluster.modes_per_sphere
+        out = torch.empty((M, M), dtype=torch.complex128, device=self.cluster._dev)
+        conv = (_Conv * M)()
+        check(lib().tg_factor_global_tmatrix(self._h, self.rtol, out.data_ptr(), conv))
+        reports = [Convergence(0, c.residual_norm, c.rhs_norm, c.refinements) for c in conv]
+        # column-major memory: out holds T_g transposed
+        return out.t().contiguous(), reports
+
 
 class FactorContext:
     """Pullback of a factored solve with respect to k0, radii, permittivities,

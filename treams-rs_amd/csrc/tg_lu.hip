@@ -33,6 +33,10 @@ struct tg_factor {
     int64_t n = 0;
     tg::DevBuf lu;   // the factors, n x n column-major
     tg::LuSet set;   // a batch of one matrix
+    // equilibration: the factors are those of D_r A D_c; eq holds D_r[n], D_c[n] (FP64
+    // powers of two, split order)
+    bool equilibrated = false;
+    tg::DevBuf eq;
 };
 
 namespace tg {
@@ -701,9 +705,164 @@ int luset_solve(LuSet &ls, hipStream_t s, cd *x, int cols, int adjoint) {
     return TG_OK;
 }
 
+// ------------------------------------------------------------ equilibration --
+//
+// linalg/mod.rs:137-236 (equilibrate): an operator whose entries span more than a
+// factor 10 while some diagonal entry is below a tenth of the largest gets its rows
+// and columns scaled by the reciprocals of their largest magnitudes (max(|re|, |im|)),
+// the axis with the smaller minimum first, the other axis measured after it.  Here the
+// scales are rounded to powers of two, so scaling is exact in either precision; the
+// reference's further power-of-two sum balancing (linalg/mod.rs:238-300) is not done.
+
+__device__ __forceinline__ double eq_mag(cd z) { return fmax(fabs(z.re), fabs(z.im)); }
+__device__ __forceinline__ double eq_mag(cf z) { return fmax(fabs((double)z.re), fabs((double)z.im)); }
+
+// Per row, over a chunk of columns: the largest magnitude (times colscale) and the
+// smallest nonzero one; the diagonal magnitude where the chunk holds it.
+template <class T>
+__global__ void k_eq_rows(const cx<T> *A, int64_t n, int chunk, const double *colscale, double *pmax, double *pmin,
+                          double *diag) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t c0 = (int64_t)blockIdx.y * chunk, c1 = min(c0 + chunk, n);
+    double mx = 0.0, mn = INFINITY;
+    for (int64_t c = c0; c < c1; ++c) {
+        const double m = eq_mag(A[c * n + r]) * (colscale ? colscale[c] : 1.0);
+        mx = fmax(mx, m);
+        if (m > 0.0) mn = fmin(mn, m);
+        if (c == r && diag) diag[r] = m;
+    }
+    pmax[(int64_t)blockIdx.y * n + r] = mx;
+    pmin[(int64_t)blockIdx.y * n + r] = mn;
+}
+
+__global__ void k_eq_reduce(const double *p, int parts, int64_t n, int take_min, double *out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double v = take_min ? INFINITY : 0.0;
+    for (int q = 0; q < parts; ++q) v = take_min ? fmin(v, p[(int64_t)q * n + r]) : fmax(v, p[(int64_t)q * n + r]);
+    out[r] = v;
+}
+
+// Largest magnitude (times rowscale) of each column: one workgroup per column.
+template <class T>
+__global__ __launch_bounds__(256) void k_eq_cols(const cx<T> *A, int64_t n, const double *rowscale, double *cmax) {
+    const int64_t c = blockIdx.x;
+    double mx = 0.0;
+    for (int64_t r = threadIdx.x; r < n; r += 256) mx = fmax(mx, eq_mag(A[c * n + r]) * (rowscale ? rowscale[r] : 1.0));
+    __shared__ double red[256];
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cmax[c] = red[0];
+}
+
+template <class T>
+__global__ void k_eq_apply(cx<T> *A, int64_t n, const double *r, const double *c) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * n) return;
+    const int64_t col = t / n, row = t - col * n;
+    const double s = r[row] * c[col];  // a power of two: exact (in double, then rounded once)
+    A[t] = cx<T>((T)((double)A[t].re * s), (T)((double)A[t].im * s));
+}
+
+// x[col][i] *= d[i] for `cols` packed columns of length n.
+__global__ void k_scale_rows(cd *x, int64_t n, int cols, const double *d) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * cols) return;
+    const double s = d[t % n];
+    x[t] = cd(x[t].re * s, x[t].im * s);
+}
+
+// 2^-e for v = m 2^e, m in [0.5, 1): the power of two that brings v into [0.5, 1).
+static double pow2_inverse(double v) {
+    int e = 0;
+    (void)std::frexp(v, &e);
+    return std::ldexp(1.0, -e);
+}
+
+template <class T>
+static int equilibrate_t(tg_factor *f) {
+    tg_cluster *c = f->cluster;
+    hipStream_t s = c->stream;
+    const int64_t n = f->n;
+    const cx<T> *A = reinterpret_cast<const cx<T> *>(f->lu.ptr);
+    const int chunk = 256;
+    const int parts = (int)((n + chunk - 1) / chunk);
+    DevBuf work;
+    TG_TRY(work.ensure(sizeof(double) * ((size_t)2 * parts * n + 6 * (size_t)n)));
+    double *pmax = work.as<double>(), *pmin = pmax + (size_t)parts * n, *rmax = pmin + (size_t)parts * n;
+    double *rmin = rmax + n, *diag = rmin + n, *cmax = diag + n, *scale = cmax + n, *second = scale + n;
+    const dim3 grows((unsigned)((n + 255) / 256), (unsigned)parts);
+    const unsigned g1 = (unsigned)((n + 255) / 256);
+    k_eq_rows<T><<<grows, 256, 0, s>>>(A, n, chunk, nullptr, pmax, pmin, diag);
+    k_eq_reduce<<<g1, 256, 0, s>>>(pmax, parts, n, 0, rmax);
+    k_eq_reduce<<<g1, 256, 0, s>>>(pmin, parts, n, 1, rmin);
+    k_eq_cols<T><<<(unsigned)n, 256, 0, s>>>(A, n, nullptr, cmax);
+    TG_HIP(hipGetLastError());
+    std::vector<double> h(4 * (size_t)n);
+    TG_HIP(hipMemcpyAsync(h.data(), rmax, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    const double *hr = h.data(), *hmin = hr + n, *hd = hmin + n, *hc = hd + n;
+    double maximum = 0.0, minimum = INFINITY, row_min = INFINITY, col_min = INFINITY;
+    bool diag_ok = true;
+    for (int64_t i = 0; i < n; ++i) {
+        if (!std::isfinite(hr[i]) || !std::isfinite(hc[i])) {
+            set_error("non-finite entry in the cluster operator");
+            return TG_INVALID_INPUT;
+        }
+        maximum = std::max(maximum, hr[i]);
+        minimum = std::min(minimum, hmin[i]);
+        row_min = std::min(row_min, hr[i]);
+        col_min = std::min(col_min, hc[i]);
+    }
+    for (int64_t i = 0; i < n; ++i) diag_ok = diag_ok && hd[i] / maximum >= 0.1;
+    f->equilibrated = false;
+    if (maximum > 0.0 && (minimum / maximum >= 0.1 || diag_ok)) return TG_OK;
+    if (row_min == 0.0 || col_min == 0.0) {
+        set_error("zero row or column in the cluster operator");
+        return TG_SINGULAR;
+    }
+    if (row_min / maximum >= 0.1 && col_min / maximum >= 0.1) return TG_OK;
+    // the axis with the smaller minimum first, then the other measured after it
+    const bool columns_first = col_min < row_min;
+    std::vector<double> first(n), other(n);
+    for (int64_t i = 0; i < n; ++i) first[i] = pow2_inverse(columns_first ? hc[i] : hr[i]);
+    TG_HIP(hipMemcpyAsync(scale, first.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (columns_first) {
+        k_eq_rows<T><<<grows, 256, 0, s>>>(A, n, chunk, scale, pmax, pmin, nullptr);
+        k_eq_reduce<<<g1, 256, 0, s>>>(pmax, parts, n, 0, second);
+    } else {
+        k_eq_cols<T><<<(unsigned)n, 256, 0, s>>>(A, n, scale, second);
+    }
+    TG_HIP(hipGetLastError());
+    TG_HIP(hipMemcpyAsync(other.data(), second, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < n; ++i) {
+        if (!(other[i] > 0.0)) {
+            set_error("zero row or column in the cluster operator");
+            return TG_SINGULAR;
+        }
+        other[i] = pow2_inverse(other[i]);
+    }
+    TG_TRY(f->eq.ensure(sizeof(double) * 2 * (size_t)n));
+    double *dr = f->eq.as<double>(), *dc = dr + n;
+    TG_HIP(hipMemcpyAsync(dr, (columns_first ? other : first).data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    TG_HIP(hipMemcpyAsync(dc, (columns_first ? first : other).data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    k_eq_apply<T><<<(unsigned)((n * n + 255) / 256), 256, 0, s>>>(reinterpret_cast<cx<T> *>(f->lu.ptr), n, dr, dc);
+    TG_HIP(hipGetLastError());
+    TG_HIP(hipStreamSynchronize(s));
+    f->equilibrated = true;
+    return TG_OK;
+}
+
 static int factor_cluster(tg_factor *f) {
     tg_cluster *c = f->cluster;
     TG_TRY(launch_assemble_operator(c, f->precision, f->lu.ptr));
+    TG_TRY(f->precision == TG_STORE_C64 ? equilibrate_t<float>(f) : equilibrate_t<double>(f));
     Timer::Rec rec;
     TG_TRY(c->timer.begin("lu_factor", c->stream, rec));
     const int st = luset_factor(f->set, c->stream);
@@ -711,11 +870,19 @@ static int factor_cluster(tg_factor *f) {
     return st;
 }
 
+// x <- A^{-1} x (A^{-H} x): with equilibration A = D_r^{-1} S D_c^{-1}, so
+// A^{-1} = D_c S^{-1} D_r and A^{-H} = D_r S^{-H} D_c.
 static int lu_solve(tg_factor *f, cd *x, int cols, int adjoint) {
     tg_cluster *c = f->cluster;
+    const int64_t n = f->n;
+    const double *dr = f->equilibrated ? f->eq.as<double>() : nullptr, *dc = dr ? dr + n : nullptr;
+    const unsigned grid = (unsigned)((n * cols + 255) / 256);
     Timer::Rec rec;
     TG_TRY(c->timer.begin("lu_solve", c->stream, rec));
+    if (dr) k_scale_rows<<<grid, 256, 0, c->stream>>>(x, n, cols, adjoint ? dc : dr);
     TG_TRY(luset_solve(f->set, c->stream, x, cols, adjoint));
+    if (dr) k_scale_rows<<<grid, 256, 0, c->stream>>>(x, n, cols, adjoint ? dr : dc);
+    TG_HIP(hipGetLastError());
     TG_TRY(c->timer.end("lu_solve", c->stream, rec));
     return TG_OK;
 }
@@ -732,8 +899,19 @@ int factor_solve_split(tg_factor *f, const cd *b, cd *x, int cols, int adjoint, 
     std::vector<double> bn(cols), rn(cols);
     int st = TG_OK;
     conv.assign(cols, tg_convergence{0, 0, 0.0, 0.0});
+    // with equilibration the certificate is the residual of the scaled system:
+    // D_r (b - A x) against D_r b (D_c for the adjoint)
+    const double *dnorm = f->equilibrated ? f->eq.as<double>() + (adjoint ? f->n : 0) : nullptr;
+    const unsigned sgrid = (unsigned)((count + 255) / 256);
+    auto norms = [&](const cd *v, double *out) -> int {
+        if (!dnorm) return vec_norms(c->stream, v, c->n, cols, c->n, out, c->ws[4]);
+        TG_HIP(hipMemcpyAsync(ax, v, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream));
+        k_scale_rows<<<sgrid, 256, 0, c->stream>>>(ax, c->n, cols, dnorm);
+        TG_HIP(hipGetLastError());
+        return vec_norms(c->stream, ax, c->n, cols, c->n, out, c->ws[4]);
+    };
     do {
-        if ((st = vec_norms(c->stream, b, c->n, cols, c->n, bn.data(), c->ws[4]))) break;
+        if ((st = norms(b, bn.data()))) break;
         if (hipMemcpyAsync(x, b, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
             set_error("copy of the right-hand side failed");
             st = TG_DEVICE_ERROR;
@@ -750,7 +928,7 @@ int factor_solve_split(tg_factor *f, const cd *b, cd *x, int cols, int adjoint, 
                 break;
             }
             if ((st = vec_axpby(c->stream, cd(-1.0), ax, cd(1.0), r, count))) break;
-            if ((st = vec_norms(c->stream, r, c->n, cols, c->n, rn.data(), c->ws[4]))) break;
+            if ((st = norms(r, rn.data()))) break;
             bool done = true;
             for (int k = 0; k < cols; ++k) {
                 conv[k].residual_norm = rn[k];
@@ -776,6 +954,29 @@ int factor_solve_split(tg_factor *f, const cd *b, cd *x, int cols, int adjoint, 
     } while (0);
     tmp.release();
     return st;
+}
+
+// C = A B for column-major complex matrices (M x K times K x N) on MFMA in the 3M
+// form, one 64 x 64 tile of C per workgroup.
+template <class T, bool KTAIL>
+__global__ __launch_bounds__(256, 2) void k_zgemm_plain(const cx<T> *A, int64_t lda, const cx<T> *B, int64_t ldb,
+                                                        cx<T> *C, int64_t ldc, int64_t M, int64_t N, int K) {
+    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    __shared__ TileLds<T> sm;
+    Tile3M<T> acc;
+    zgemm_tile_3m<T, KTAIL>(A, lda, B, ldb, r0, c0, M, N, K, sm, acc);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, rr = lane & 15;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<T>(lane, v);
+                const T t1 = acc.t1[a][b][v], t2 = acc.t2[a][b][v], t3 = acc.t3[a][b][v];
+                if (gr < M && gc < N) C[gc * ldc + gr] = cx<T>(t1 - t2, t3 - t1 - t2);
+            }
 }
 
 // Gradient body shared with the GMRES pullback (tg_adjoint.hip), here with the
@@ -859,6 +1060,65 @@ int tg_factor_solve(tg_factor *f, const tg_c128 *incident, int cols, double rtol
         for (int k = 0; k < cols && k < (int)conv.size(); ++k) conv_host[k] = conv[k];
     (void)hipStreamSynchronize(c->stream);
     io.release();
+    return st;
+}
+
+int tg_factor_equilibrated(const tg_factor *f) { return f && f->equilibrated ? 1 : 0; }
+
+int tg_factor_global_tmatrix(tg_factor *f, double rtol, tg_c128 *out, tg_convergence *conv_host) {
+    if (!f || !out || !(rtol > 0.0)) {
+        set_error("null argument or non-positive rtol");
+        return TG_INVALID_INPUT;
+    }
+    tg_cluster *c = f->cluster;
+    const int N = c->N, M = c->M;
+    const int64_t n = c->n, MM = (int64_t)M * M;
+    // regular blocks: R(p_j <- 0) (displacement p_j) for the incident side, then
+    // R(0 <- p_i) (displacement -p_i) for the outgoing side
+    std::vector<double> disp(6 * (size_t)N);
+    for (int j = 0; j < N; ++j)
+        for (int a = 0; a < 3; ++a) {
+            disp[3 * j + a] = c->positions[3 * j + a];
+            disp[3 * (N + j) + a] = -c->positions[3 * j + a];
+        }
+    DevBuf buf;
+    TG_TRY(buf.ensure(sizeof(cd) * (2 * N * MM + 2 * n * M) + sizeof(double) * disp.size()));
+    cd *blocks = buf.as<cd>(), *rin = blocks + 2 * N * MM, *z = rin + n * M;
+    double *ddisp = reinterpret_cast<double *>(z + n * M);
+    int st = TG_OK;
+    do {
+        if (hipMemcpyAsync(ddisp, disp.data(), sizeof(double) * disp.size(), hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess) {
+            set_error("copy of the displacements failed");
+            st = TG_DEVICE_ERROR;
+            break;
+        }
+        if ((st = launch_translation_blocks(c->lmax, c->plan, c->k0, 2 * N, ddisp, 1, blocks, c->stream))) break;
+        // Rin (n x M): rows (j, a), columns g; block j is R(p_j <- 0)[a][g] = blocks_j[g M + a]
+        for (int j = 0; j < N && !st; ++j)
+            if (hipMemcpy2DAsync(rin + (int64_t)j * M, sizeof(cd) * n, blocks + j * MM, sizeof(cd) * M, sizeof(cd) * M,
+                                 M, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+                set_error("copy of the translation blocks failed");
+                st = TG_DEVICE_ERROR;
+            }
+        if (st) break;
+        // Z = (I - T C)^{-1} T Rin, then T_g = Rout Z with Rout = [R(0 <- p_1) ... R(0 <- p_N)] (M x n)
+        if ((st = tg_factor_solve(f, reinterpret_cast<const tg_c128 *>(rin), M, rtol, 0,
+                                  reinterpret_cast<tg_c128 *>(z), conv_host)))
+            break;
+        const cd *rout = blocks + N * MM;
+        const dim3 grid((unsigned)((M + 63) / 64), (unsigned)((M + 63) / 64));
+        if (n % 16)
+            k_zgemm_plain<double, true><<<grid, 256, 0, c->stream>>>(rout, M, z, n, (cd *)out, M, M, M, (int)n);
+        else
+            k_zgemm_plain<double, false><<<grid, 256, 0, c->stream>>>(rout, M, z, n, (cd *)out, M, M, M, (int)n);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+            set_error("device failure in the global T-matrix");
+            st = TG_DEVICE_ERROR;
+        }
+    } while (0);
+    (void)hipStreamSynchronize(c->stream);
+    buf.release();
     return st;
 }
 
