@@ -147,6 +147,28 @@ struct GeoList {
     }
 };
 
+// Streaming (nontemporal) stores for data written once and read much later: the
+// stored coupling, the operator of the LU and the pair tables are all far larger than
+// the caches.
+typedef double tg_d2v __attribute__((ext_vector_type(2)));
+typedef float tg_f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void stream_store(double2 *p, double2 v) {
+    tg_d2v x = {v.x, v.y};
+    __builtin_nontemporal_store(x, reinterpret_cast<tg_d2v *>(p));
+}
+__device__ __forceinline__ void stream_store(float2 *p, float2 v) {
+    tg_f2v x = {v.x, v.y};
+    __builtin_nontemporal_store(x, reinterpret_cast<tg_f2v *>(p));
+}
+__device__ __forceinline__ void stream_store(cd *p, cd v) {
+    tg_d2v x = {v.re, v.im};
+    __builtin_nontemporal_store(x, reinterpret_cast<tg_d2v *>(p));
+}
+__device__ __forceinline__ void stream_store(cf *p, cf v) {
+    tg_f2v x = {v.re, v.im};
+    __builtin_nontemporal_store(x, reinterpret_cast<tg_f2v *>(p));
+}
+
 template <class S> __device__ __forceinline__ S cvt(cd v);
 template <> __device__ __forceinline__ float2 cvt<float2>(cd v) { return make_float2((float)v.re, (float)v.im); }
 template <> __device__ __forceinline__ double2 cvt<double2>(cd v) { return make_double2(v.re, v.im); }
@@ -158,8 +180,8 @@ struct OutSplit {
     int H;
     __device__ void store(int i, int j, int a, int b, cd vp, cd vm) const {
         int64_t idx = ((int64_t)j * H + a) * nh + (int64_t)i * H + b;
-        cp[idx] = cvt<S>(vp);
-        cm[idx] = cvt<S>(vm);
+        stream_store(cp + idx, cvt<S>(vp));
+        stream_store(cm + idx, cvt<S>(vm));
     }
 };
 
@@ -191,10 +213,10 @@ struct OutOperator {
         const cd *t = mie + (size_t)(i * lmax + l - 1) * 4;
         const int64_t r = (int64_t)i * H + b, col = (int64_t)j * H + a;
         const double d = (i == j && a == b) ? 1.0 : 0.0;
-        A[col * n + r] = cvt<S>(cd(d) - t[0] * vp);
-        A[(col + nh) * n + r] = cvt<S>(-(t[1] * vm));
-        A[col * n + r + nh] = cvt<S>(-(t[2] * vp));
-        A[(col + nh) * n + r + nh] = cvt<S>(cd(d) - t[3] * vm);
+        stream_store(A + col * n + r, cvt<S>(cd(d) - t[0] * vp));
+        stream_store(A + (col + nh) * n + r, cvt<S>(-(t[1] * vm)));
+        stream_store(A + col * n + r + nh, cvt<S>(-(t[2] * vp)));
+        stream_store(A + (col + nh) * n + r + nh, cvt<S>(cd(d) - t[3] * vm));
     }
 };
 
@@ -571,8 +593,8 @@ __global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd
     const int64_t o = (int64_t)j * N + i;
     // the FP64 entry and, when asked for, its c64 rounding (the FP32 MFMA tables)
     auto put = [&](int h, cd v) {
-        tab[(int64_t)h * NN + o] = v;
-        if (tab32) tab32[(int64_t)h * NN + o] = cf((float)v.re, (float)v.im);
+        stream_store(tab + (int64_t)h * NN + o, v);
+        if (tab32) stream_store(tab32 + (int64_t)h * NN + o, cf((float)v.re, (float)v.im));
     };
     double d[3], r, u[3];
     for (int a = 0; a < 3; ++a) d[a] = pos[3 * i + a] - pos[3 * j + a];
