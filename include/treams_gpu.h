@@ -288,6 +288,31 @@ int tg_factor_pullback(tg_factor *factor, const tg_c128 *incident_dev, const tg_
                        tg_convergence *conv_host);
 
 /* ---------------------------------------------------------------------------
+ * Fields
+ * ------------------------------------------------------------------------- */
+
+/*
+ * Electric field of outgoing (singular) helicity vector spherical waves about every
+ * sphere of the cluster, sum_i sum_modes x[i M + mode] wave_mode(point - p_i), at
+ * `npoints` points (device FP64, xyz per point) for `cols` coefficient columns
+ * (device c128, n x cols, reference layout): out_dev is (3 npoints) x cols
+ * column-major, rows 3 p + axis.  Replaces fields::field (fields.rs:947) with the
+ * cluster's multi-centre helicity basis (Wave.efield of a scattered wave,
+ * python/treams_rs/_waves.py).  Points at a sphere centre give NaN (the reference
+ * raises there).  regular != 0 (incoming waves) is not built: TG_INVALID_INPUT.
+ */
+int tg_cluster_field(tg_cluster *cluster, const tg_c128 *x_dev, int cols, int64_t npoints, const double *points_dev,
+                     int regular, tg_c128 *out_dev);
+
+/*
+ * Field of a unit helicity plane wave, (N + (2 pol - 1) M) / sqrt(2) exp(i k.r), at
+ * `npoints` points: out_dev (3 npoints) c128.  Replaces pw::field / field_value
+ * (pw/field.rs:267, pw/polarization.rs:138-203).
+ */
+int tg_plane_wave_field(double k0, const double *direction, int pol, int64_t npoints, const double *points_dev,
+                        tg_c128 *out_dev, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Preconditioners (no reference counterpart: the reference GMRES is
  * unpreconditioned).  Right preconditioning: iteration counts change, the answer
  * and its FP64 true-residual certificate do not.

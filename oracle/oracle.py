@@ -286,6 +286,94 @@ def plane_wave(lmax: int, positions, k0: float, direction, pol: int) -> np.ndarr
     return out
 
 
+# ------------------------------------------------------------------------ fields
+
+
+def spherical_waves(lmax: int, k: float, r, regular: bool = False) -> np.ndarray:
+    """(M, 3) Cartesian electric fields of one sphere's helicity vector spherical waves
+    (sphere mode order) at displacement r = point - centre (fields.rs:144-320):
+    psi = c S with c = z_l(k r) / r^l and the solid harmonic S = r^l P_l^m e^{i m phi};
+    n = d grad S + l e S r (d = (l + 1) c / k - r^2 e, e = z_{l+1} / r^{l+1}),
+    m = c (r x grad S); wave = i sqrt((2l+1) / (4 pi l (l+1))) sqrt((l-m)!/(l+m)!)
+    (n - (2 pol - 1) m) / sqrt(2).  Coordinates are scaled by the largest component
+    (and by 1/k for regular waves), which leaves the waves unchanged."""
+    r = np.asarray(r, np.float64)
+    scale = np.abs(r).max()
+    if regular or scale == 0.0:
+        scale = max(scale, 1.0 / abs(k))
+    ks, rs = k * scale, r / scale
+    r2 = float(rs @ rs)
+    rr = np.sqrt(r2)
+    val, grad = solid_table(lmax, rs, gradient=True)
+    if regular:
+        z = spherical_jn(lmax + 1, ks * rr) if rr > 0 else np.array([1.0] + [0.0] * (lmax + 1), np.complex128)
+        rad = np.array([z[l] / rr ** l if rr > 0 else 0.0 for l in range(lmax + 2)], np.complex128)
+        if rr == 0 or abs(ks * rr) < 0.5:  # the origin series (fields.rs:146-160)
+            rad = np.zeros(lmax + 2, np.complex128)
+            for l in range(lmax + 2):
+                den = np.prod([2.0 * i + 1 for i in range(l + 1)])
+                coef, tot = ks ** l / den, 0j
+                for q in range(12):
+                    tot += coef * (ks * rr) ** (2 * q)
+                    coef /= -2.0 * (q + 1) * (2 * l + 2 * q + 3)
+                rad[l] = tot
+    else:
+        h = hankels(lmax + 1, ks * rr)
+        rad = np.array([h[l] / rr ** l for l in range(lmax + 2)], np.complex128)
+    M = 2 * lmax * (lmax + 2)
+    out = np.zeros((M, 3), np.complex128)
+    for l in range(1, lmax + 1):
+        c, e = rad[l], rad[l + 1]
+        d = (l + 1) / ks * c - r2 * e
+        for m in range(-l, l + 1):
+            S, G = val[l * l + l + m], grad[l * l + l + m]
+            n = d * G + l * e * S * rs
+            mv = c * np.cross(rs, G)
+            norm = 1j * np.sqrt((2 * l + 1) / (4 * np.pi * l * (l + 1))) * np.sqrt(
+                float(mp_factorial_ratio(l, m)))
+            for pol in (1, 0):
+                out[2 * (l * l + l + m - 1) + (1 - pol)] = norm * (n - (2 * pol - 1) * mv) / np.sqrt(2)
+    return out
+
+
+def mp_factorial_ratio(l: int, m: int) -> float:
+    """(l - m)! / (l + m)!"""
+    from math import factorial
+    return factorial(l - m) / factorial(l + m)
+
+
+def cluster_field(lmax: int, k: float, positions, x, points, regular: bool = False) -> np.ndarray:
+    """(P, 3) field sum_i sum_modes x[i M + mode] wave_mode(point - p_i)
+    (fields.rs:947-1010 for a multi-centre helicity basis)."""
+    pos = np.asarray(positions, np.float64).reshape(-1, 3)
+    M = 2 * lmax * (lmax + 2)
+    out = np.zeros((len(points), 3), np.complex128)
+    for p, pt in enumerate(points):
+        for i in range(pos.shape[0]):
+            out[p] += x[i * M:(i + 1) * M] @ spherical_waves(lmax, k, np.asarray(pt) - pos[i], regular)
+    return out
+
+
+def plane_wave_field(k0: float, direction, pol: int, points) -> np.ndarray:
+    """(P, 3) field of a unit helicity plane wave: (N + (2 pol - 1) M) / sqrt(2)
+    exp(i k.r) with M = i (k_y, -k_x, 0) / k_t and N = (-k_x k_z, -k_y k_z, k_t^2) /
+    (k k_t); on the z axis M = (0, -i, 0) and N = (-sign k_z, 0, 0)
+    (pw/polarization.rs:138-203)."""
+    d = np.asarray(direction, np.float64)
+    kv = k0 * d / np.linalg.norm(d)
+    kt = np.hypot(kv[0], kv[1])
+    if kt == 0.0:
+        Mv = np.array([0, -1j, 0])
+        Nv = np.array([-np.sign(kv[2]), 0, 0], np.complex128)
+    else:
+        xy = kv[:2] / kt
+        Mv = np.array([1j * xy[1], -1j * xy[0], 0])
+        Nv = np.array([-xy[0] * kv[2] / k0, -xy[1] * kv[2] / k0, kt / k0], np.complex128)
+    e = (Nv + (2 * pol - 1) * Mv) / np.sqrt(2)
+    phase = np.exp(1j * (np.asarray(points, np.float64) @ kv))
+    return phase[:, None] * e[None, :]
+
+
 # ------------------------------------------------------------------------- GMRES
 
 

@@ -95,7 +95,30 @@ def mie_table():
     return out
 
 
+REFERENCE = "/root/reference"
+
+
+def cluster_example_fields():
+    """The 7 x 7 total-field intensity map (local expansion, y = 0, x and z on
+    linspace(-300, 300, 7) nm; NaN inside a sphere) of docs/examples/cluster.py, read
+    from docs/examples/output/cluster.txt into reference_examples.json."""
+    path = os.path.join(REFERENCE, "docs", "examples", "output", "cluster.txt")
+    with open(path) as f:
+        lines = f.read().splitlines()
+    start = lines.index("Intensity at y = 0, local expansion (rows: x, columns: z)") + 1
+    rows = [[None if v == "nan" else float(v) for v in line.split()] for line in lines[start:start + 7]]
+    out = os.path.join(HERE, "reference_examples.json")
+    with open(out) as f:
+        data = json.load(f)
+    data["cluster_example"]["intensity_local"] = {
+        "grid": "x, z = linspace(-300, 300, 7) nm, y = 0; rows x, columns z; null inside a sphere",
+        "values": rows, "significant_digits": 6}
+    with open(out, "w") as f:
+        json.dump(data, f, indent=1)
+
+
 def main():
+    cluster_example_fields()
     data = {
         "hankel": hankel_table(),
         "jn": jn_table(),

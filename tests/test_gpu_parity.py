@@ -332,3 +332,46 @@ def test_pullback_many_columns_chunked(pkg, oracle):
     assert np.abs(gp - r["positions"]).max() <= 1e-9 * np.abs(r["positions"]).max()
     assert np.abs(gr - r["radii"]).max() <= 1e-9 * np.abs(r["radii"]).max()
     assert abs(gk - r["k0"]) <= 1e-9 * abs(r["k0"])
+
+
+def _example(pkg, oracle, golden_dir):
+    g = json.load(open(os.path.join(golden_dir, "reference_examples.json")))["cluster_example"]
+    pos = (220 / np.sqrt(24)) * np.array(
+        [[np.sqrt(8), 0, -1], [-np.sqrt(2), np.sqrt(6), -1], [-np.sqrt(2), -np.sqrt(6), -1], [0, 0, 3]])
+    k0 = 2 * np.pi / g["k0_over_2pi_inverse"]
+    eps = np.full(4, complex(*g["epsilon"]))
+    cl = pkg.IterativeSphereCluster(g["lmax"], k0, np.array(g["radii"]), eps, pos)
+    return cl, oracle.Cluster(g["lmax"], k0, g["radii"], complex(*g["epsilon"]), pos), g
+
+
+def test_fields_match_oracle(pkg, oracle, golden_dir):
+    """Outgoing-wave field of a solved cluster (3 columns) and the plane-wave field at
+    points around and between the spheres, against the oracle (1e-12)."""
+    cl, ref, g = _example(pkg, oracle, golden_dir)
+    rng = np.random.default_rng(2)
+    pts = rng.uniform(-400, 400, size=(300, 3))
+    pts = pts[[all(np.linalg.norm(p - ref.positions[i]) > 1e-3 for i in range(4)) for p in pts]]
+    x = rng.normal(size=(ref.n, 3)) + 1j * rng.normal(size=(ref.n, 3))
+    got = cl.field(x, pts).cpu().numpy()
+    for c in range(3):
+        assert _rel(got[:, :, c], oracle.cluster_field(ref.lmax, ref.k0, ref.positions, x[:, c], pts)) <= 1e-12
+    for d, p in (([1, 0, 0], 0), ([0.3, -0.2, 1], 1), ([0, 0, -1], 1)):
+        assert _rel(pkg.plane_wave_field(ref.k0, d, p, pts), oracle.plane_wave_field(ref.k0, d, p, pts)) <= 1e-13
+
+
+def test_cluster_example_intensity_map_golden(pkg, oracle, golden_dir):
+    """docs/examples/cluster.py: total-field intensity 0.5 |E_inc + E_sca|^2 on the 7 x 7
+    grid at y = 0 from the GPU solve and field kernels, against
+    docs/examples/output/cluster.txt at its 6 printed digits (points inside a sphere
+    are NaN there and skipped)."""
+    cl, ref, g = _example(pkg, oracle, golden_dir)
+    a = cl.plane_wave(g["direction"], g["pol"])
+    x, _ = cl.solve(a, rtol=1e-12)
+    gx = np.linspace(-300, 300, 7)
+    pts = np.array([[u, 0.0, w] for u in gx for w in gx])
+    inside = np.array([any(np.linalg.norm(p - ref.positions[i]) <= ref.radii[i] for i in range(4)) for p in pts])
+    out = pts[~inside]
+    e = pkg.plane_wave_field(ref.k0, g["direction"], g["pol"], out) + cl.field(x, out)
+    got = 0.5 * torch.sum(torch.abs(e) ** 2, dim=1).cpu().numpy()
+    want = [v for row in g["intensity_local"]["values"] for v in row if v is not None]
+    assert [float(f"{v:.6g}") for v in got] == want

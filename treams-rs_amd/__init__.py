@@ -15,6 +15,7 @@ from .cluster import (  # noqa: F401
     IterativeSphereCluster,
     SphereClusterFactor,
     plan_terms,
+    plane_wave_field,
     scattering_gradient,
     sphere_cluster_factor,
     translation_blocks,

@@ -170,6 +170,19 @@ class IterativeSphereCluster:
         check(lib().tg_cluster_plane_wave(self._h, d.ctypes.data, int(pol), out.data_ptr()))
         return out
 
+    def field(self, x, points) -> torch.Tensor:
+        """Electric field of outgoing helicity vector spherical waves with coefficients
+        x (n or n x P, reference layout) about every sphere, at points (Q x 3):
+        (Q, 3) or (Q, 3, P) complex128 (fields.rs:947, the efield of a scattered
+        wave).  NaN at a sphere centre."""
+        cm, vector = _columns(x, self.n)
+        pts = torch.as_tensor(points, dtype=torch.float64).reshape(-1, 3).to(self._dev).contiguous()
+        q = pts.shape[0]
+        out = torch.empty((cm.shape[0], 3 * q), dtype=torch.complex128, device=self._dev)
+        check(lib().tg_cluster_field(self._h, cm.data_ptr(), cm.shape[0], q, pts.data_ptr(), 0, out.data_ptr()))
+        out = out.reshape(cm.shape[0], q, 3).permute(1, 2, 0)
+        return out[..., 0].contiguous() if vector else out.contiguous()
+
     # ------------------------------------------------------------------ solves
     @staticmethod
     def options(rtol=DEFAULT_RTOL, atol=DEFAULT_ATOL, restart=DEFAULT_RESTART,
@@ -407,6 +420,19 @@ def scattering_gradient(cluster, direction, pol, **options):
     phase = torch.sum(torch.conj(ab) * 1j * ar, dim=1).real.cpu().numpy()
     gp = gp + phase[:, None] * kvec[None, :]
     return sca, gr, gp, dict(k0=gk, epsilon=ge, convergence=conv)
+
+
+def plane_wave_field(k0: float, direction, pol, points) -> torch.Tensor:
+    """Field (Q, 3) of a unit helicity plane wave at points (Q x 3): (N + (2 pol - 1) M)
+    / sqrt(2) exp(i k.r) (pw/field.rs:267, pw/polarization.rs:138-203)."""
+    if isinstance(pol, str):
+        pol = {"positive_helicity": 1, "negative_helicity": 0}[pol]
+    d = np.ascontiguousarray(np.asarray(direction, np.float64).reshape(3))
+    pts = torch.as_tensor(points, dtype=torch.float64).reshape(-1, 3).to(_device()).contiguous()
+    out = torch.empty(3 * pts.shape[0], dtype=torch.complex128, device=_device())
+    check(lib().tg_plane_wave_field(float(k0), d.ctypes.data, int(pol), pts.shape[0], pts.data_ptr(),
+                                    out.data_ptr(), _stream_ptr()))
+    return out.reshape(-1, 3)
 
 
 def plan_terms(lmax: int):

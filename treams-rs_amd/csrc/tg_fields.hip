@@ -1,0 +1,209 @@
+// Electric fields of mode expansions at sample points: the scattered field of a
+// solved cluster (outgoing helicity vector spherical waves about every sphere) and
+// the field of a helicity plane wave.
+//
+// Vector spherical waves (fields.rs:144-320): with the scaled radial function
+// c = h_l(k r) / r^l, e = h_{l+1}(k r) / r^{l+1} and the solid harmonic
+// S = r^l P_l^m(cos theta) e^{i m phi},
+//   n = d grad S + l e S r,  d = (l + 1) c / k - r^2 e,     m = c (r x grad S),
+//   wave(l, m, pol) = i sqrt((2l+1) / (4 pi l (l+1))) sqrt((l-m)!/(l+m)!) (n - (2 pol - 1) m) / sqrt(2).
+// Coordinates are divided by their largest component (the waves do not change
+// under a common change of length unit), so r^l stays near one.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "tg_internal.h"
+
+namespace tg {
+
+constexpr int kFieldCols = 8;  // coefficient columns per pass
+
+// One thread per sample point: out[col][3 p + a] = sum_i sum_modes x[col][i M + mode]
+// wave_mode(point - pos_i).  Per sphere and order |m| the degree recurrences of the
+// solid harmonic (value and gradient) and of h_l run side by side, so no table
+// indexed by a run-time degree is kept.  The coefficients are read with wave-uniform
+// addresses.  A point at a sphere centre gets NaN.
+__global__ __launch_bounds__(128) void k_cluster_field(int N, int lmax, double k, const double *pos, const double *refl,
+                                                       const cd *x, int64_t n, int cols, int64_t npts,
+                                                       const double *points, cd *out) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    const int M = 2 * lmax * (lmax + 2);
+    cd E[kFieldCols][3];
+#pragma unroll
+    for (int c = 0; c < kFieldCols; ++c) E[c][0] = E[c][1] = E[c][2] = cd(0.0);
+    bool singular = false;
+    for (int i = 0; i < N; ++i) {
+        double rs[3];
+        double scale = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            rs[a] = points[3 * p + a] - pos[3 * i + a];
+            scale = fmax(scale, fabs(rs[a]));
+        }
+        if (!(scale > 0.0)) {
+            singular = true;
+            continue;
+        }
+        for (int a = 0; a < 3; ++a) rs[a] /= scale;
+        const double ks = k * scale;
+        const double r2 = rs[0] * rs[0] + rs[1] * rs[1] + rs[2] * rs[2], rr = sqrt(r2);
+        const double xarg = ks * rr, inv = 1.0 / xarg, sx = sin(xarg), cxv = cos(xarg);
+        const cd xy(rs[0], rs[1]);
+        const cd *xi = x + (int64_t)i * M;
+        // ladder over the order: P_q^q (-1)^q (2q-1)!! (x + i y)^q and its gradient
+        cd lp(1.0), lg[3] = {cd(0.0), cd(0.0), cd(0.0)};
+        for (int q = 0; q <= lmax; ++q) {
+            if (q > 0) {
+                const double f = -(double)(2 * q - 1);
+                const cd e0(1.0), e1(0.0, 1.0);
+                lg[0] = f * (lg[0] * xy + lp * e0);
+                lg[1] = f * (lg[1] * xy + lp * e1);
+                lg[2] = f * (lg[2] * xy);
+                lp = f * (lp * xy);
+            }
+            // degree recurrence at order q; h_l(ks rr) alongside
+            cd S = lp, G[3] = {lg[0], lg[1], lg[2]};
+            cd Sp(0.0), Gp[3] = {cd(0.0), cd(0.0), cd(0.0)};
+            cd hprev(cxv * inv, sx * inv), hcur(sx * inv, -cxv * inv);  // h_{-1}, h_0
+            double rpow = 1.0;                                            // rr^l
+            for (int l = 0; l <= lmax; ++l) {
+                const cd hnext = ((double)(2 * l + 1) * inv) * hcur - hprev;  // h_{l+1}
+                if (l > q) {
+                    const double a = (double)(2 * l - 1) / (double)(l - q), b = (double)(l + q - 1) / (double)(l - q);
+                    const cd Sn = (a * rs[2]) * S - (b * r2) * Sp;
+                    cd Gn[3];
+                    for (int c = 0; c < 3; ++c)
+                        Gn[c] = a * (rs[2] * G[c] + (c == 2 ? S : cd(0.0))) - b * (r2 * Gp[c] + (2.0 * rs[c]) * Sp);
+                    Sp = S;
+                    S = Sn;
+                    for (int c = 0; c < 3; ++c) {
+                        Gp[c] = G[c];
+                        G[c] = Gn[c];
+                    }
+                }
+                if (l >= q && l >= 1) {
+                    const cd cl = hcur / rpow, el = hnext / (rpow * rr);
+                    const cd d = ((double)(l + 1) / ks) * cl - r2 * el;
+                    const double base = sqrt((2.0 * l + 1.0) / (4.0 * M_PI * l * (l + 1.0)));
+                    // (l - q)! / (l + q)!: the plan's reflection factors hold it for q >= 1
+                    const double ratio = q == 0 ? 1.0 : fabs(refl[l * l + l - q]);
+                    for (int sgn = 0; sgn < (q > 0 ? 2 : 1); ++sgn) {
+                        // order m = +q, or m = -q from S^{-q} = (-1)^q (l-q)!/(l+q)! conj(S^q)
+                        const int m = sgn ? -q : q;
+                        cd Sm = S, Gm[3] = {G[0], G[1], G[2]};
+                        double nrm = base * sqrt(ratio);
+                        if (sgn) {
+                            const double f = (q & 1 ? -1.0 : 1.0) * ratio;
+                            Sm = f * conj(S);
+                            for (int c = 0; c < 3; ++c) Gm[c] = f * conj(G[c]);
+                            nrm = base / sqrt(ratio);
+                        }
+                        cd nv[3], mv[3];
+                        for (int c = 0; c < 3; ++c) nv[c] = d * Gm[c] + ((double)l * rs[c]) * (el * Sm);
+                        mv[0] = cl * (rs[1] * Gm[2] - rs[2] * Gm[1]);
+                        mv[1] = cl * (rs[2] * Gm[0] - rs[0] * Gm[2]);
+                        mv[2] = cl * (rs[0] * Gm[1] - rs[1] * Gm[0]);
+                        const cd norm(0.0, nrm * M_SQRT1_2);
+                        const int mode = 2 * (l * l + l + m - 1);
+                        for (int pol = 1; pol >= 0; --pol) {
+                            const double s = pol ? -1.0 : 1.0;  // -(2 pol - 1)
+                            cd w[3];
+                            for (int c = 0; c < 3; ++c) w[c] = norm * (nv[c] + s * mv[c]);
+                            const int64_t row = mode + (1 - pol);
+#pragma unroll
+                            for (int cc = 0; cc < kFieldCols; ++cc) {
+                                if (cc >= cols) break;
+                                const cd coef = xi[(int64_t)cc * n + row];
+                                for (int c = 0; c < 3; ++c) E[cc][c] += coef * w[c];
+                            }
+                        }
+                    }
+                }
+                hprev = hcur;
+                hcur = hnext;
+                rpow *= rr;
+            }
+        }
+    }
+#pragma unroll
+    for (int cc = 0; cc < kFieldCols; ++cc) {
+        if (cc >= cols) break;
+        for (int c = 0; c < 3; ++c) out[(int64_t)cc * 3 * npts + 3 * p + c] = singular ? cd(NAN, NAN) : E[cc][c];
+    }
+}
+
+// out[3 p + a] = e_a exp(i k.r_p) for the helicity polarization vector e.
+__global__ void k_plane_wave_field(double k0, double d0, double d1, double d2, int pol, int64_t npts,
+                                   const double *points, cd *out) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    const double kx = k0 * d0, ky = k0 * d1, kz = k0 * d2;
+    const double kt = hypot(kx, ky);
+    cd Mv[3], Nv[3];
+    if (kt == 0.0) {
+        Mv[0] = cd(0.0); Mv[1] = cd(0.0, -1.0); Mv[2] = cd(0.0);
+        Nv[0] = cd(kz > 0 ? -1.0 : 1.0); Nv[1] = cd(0.0); Nv[2] = cd(0.0);
+    } else {
+        const double ux = kx / kt, uy = ky / kt;
+        Mv[0] = cd(0.0, uy); Mv[1] = cd(0.0, -ux); Mv[2] = cd(0.0);
+        Nv[0] = cd(-ux * kz / k0); Nv[1] = cd(-uy * kz / k0); Nv[2] = cd(kt / k0);
+    }
+    const double s = 2.0 * pol - 1.0;
+    const double ph = kx * points[3 * p] + ky * points[3 * p + 1] + kz * points[3 * p + 2];
+    const cd phase(cos(ph), sin(ph));
+    for (int a = 0; a < 3; ++a) out[3 * p + a] = phase * ((Nv[a] + s * Mv[a]) * M_SQRT1_2);
+}
+
+}  // namespace tg
+
+using namespace tg;
+
+extern "C" {
+
+int tg_cluster_field(tg_cluster *c, const tg_c128 *x, int cols, int64_t npoints, const double *points, int regular,
+                     tg_c128 *out) {
+    if (!c || !x || !points || !out || cols < 1 || npoints < 0) {
+        set_error("null argument, no columns or negative point count");
+        return TG_INVALID_INPUT;
+    }
+    if (regular) {
+        set_error("fields of regular (incoming) wave expansions are not built; only outgoing waves");
+        return TG_INVALID_INPUT;
+    }
+    if (npoints == 0) return TG_OK;
+    const double *pos = c->b_pos.as<double>();
+    for (int c0 = 0; c0 < cols; c0 += kFieldCols) {
+        const int nc = std::min(kFieldCols, cols - c0);
+        k_cluster_field<<<(unsigned)((npoints + 127) / 128), 128, 0, c->stream>>>(
+            c->N, c->lmax, c->k0, pos, c->plan.refl, (const cd *)x + (int64_t)c0 * c->n, c->n, nc, npoints, points,
+            (cd *)out + (int64_t)c0 * 3 * npoints);
+        TG_HIP(hipGetLastError());
+    }
+    TG_HIP(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_plane_wave_field(double k0, const double *direction, int pol, int64_t npoints, const double *points,
+                        tg_c128 *out, void *stream) {
+    if (!direction || !(std::isfinite(k0) && k0 > 0) || (pol != 0 && pol != 1) || npoints < 0 ||
+        (npoints > 0 && (!points || !out))) {
+        set_error("invalid wavenumber, direction, polarization or points");
+        return TG_INVALID_INPUT;
+    }
+    const double nd = std::sqrt(direction[0] * direction[0] + direction[1] * direction[1] +
+                                direction[2] * direction[2]);
+    if (!(nd > 0) || !std::isfinite(nd)) {
+        set_error("plane wave needs a finite nonzero direction");
+        return TG_INVALID_INPUT;
+    }
+    if (npoints == 0) return TG_OK;
+    hipStream_t s = (hipStream_t)stream;
+    k_plane_wave_field<<<(unsigned)((npoints + 255) / 256), 256, 0, s>>>(
+        k0, direction[0] / nd, direction[1] / nd, direction[2] / nd, pol, npoints, points, (cd *)out);
+    TG_HIP(hipGetLastError());
+    TG_HIP(hipStreamSynchronize(s));
+    return TG_OK;
+}
+
+}  // extern "C"
