@@ -139,6 +139,16 @@ int tg_translation_blocks(int lmax, double k, int64_t count, const double *displ
                           tg_c128 *out_dev, void *stream);
 
 /*
+ * tg_translation_blocks between bases of different degree: destination degrees up to
+ * lmax_dst, source degrees up to lmax_src (lmax_dst + lmax_src <= 24); each block is
+ * (2 Hd) x (2 Hs) column-major, destination inner, Hd = lmax_dst (lmax_dst + 2).
+ * Replaces TranslationPlan::between on two spherical bases with TranslationPlan::evaluate
+ * (sw/plan.rs:71-170,282-289), as sw::expansion assembles them (sw/expansion.rs:116).
+ */
+int tg_translation_blocks_rect(int lmax_dst, int lmax_src, double k, int64_t count, const double *displacements_dev,
+                               int regular, tg_c128 *out_dev, void *stream);
+
+/*
  * Assemble and keep the coupling on the device in the helicity-split layout
  * (two N H x N H column-major matrices, H = M / 2), in c64 or c128.  No reference
  * counterpart for the storage precision; the values are those of coupling_matrix.
@@ -266,16 +276,17 @@ int tg_factor_solve(tg_factor *factor, const tg_c128 *incident_dev, int cols, do
                     tg_c128 *x_dev, tg_convergence *conv_host);
 
 /*
- * The cluster's T-matrix in one spherical basis of the same lmax at the origin:
- * T_g = sum_ij R(0 <- p_i) X_ij R(p_j <- 0), X = (I - T C)^{-1} T, with regular
- * translations; out_dev is M x M c128 column-major (M = 2 lmax (lmax + 2), the
- * sphere mode order).  Replaces TMatrix::expand(SphericalBasis.default(lmax)) of a
- * solved sphere cluster (python/treams_rs/_tmatrix.py:243-257 with
- * sw/expansion.rs:116-132, on sphere_cluster's X, cluster/spheres.rs:56-80).  Solves
- * the M columns T R(p_j <- 0) with the factors (conv_host: M entries); rtol as in
- * tg_factor_solve.
+ * The cluster's T-matrix in one spherical basis at the origin with degrees up to
+ * lmax_global: T_g = sum_ij R(0 <- p_i) X_ij R(p_j <- 0), X = (I - T C)^{-1} T, with
+ * regular translations between the spheres' lmax and lmax_global; out_dev is Mg x Mg
+ * c128 column-major (Mg = 2 lmax_global (lmax_global + 2), the sphere mode order).
+ * Replaces TMatrix::expand(SphericalBasis.default(lmax_global)) of a solved sphere
+ * cluster (python/treams_rs/_tmatrix.py:243-257 with sw/expansion.rs:116-132, on
+ * sphere_cluster's X, cluster/spheres.rs:56-80).  Solves the Mg columns T R(p_j <- 0)
+ * with the factors (conv_host: Mg entries); rtol as in tg_factor_solve.
  */
-int tg_factor_global_tmatrix(tg_factor *factor, double rtol, tg_c128 *out_dev, tg_convergence *conv_host);
+int tg_factor_global_tmatrix(tg_factor *factor, int lmax_global, double rtol, tg_c128 *out_dev,
+                             tg_convergence *conv_host);
 
 /*
  * tg_cluster_pullback with the adjoint solve by the LU factors: replaces

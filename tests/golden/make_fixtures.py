@@ -100,8 +100,9 @@ REFERENCE = "/root/reference"
 
 def cluster_example_fields():
     """The 7 x 7 total-field intensity map (local expansion, y = 0, x and z on
-    linspace(-300, 300, 7) nm; NaN inside a sphere) of docs/examples/cluster.py, read
-    from docs/examples/output/cluster.txt into reference_examples.json."""
+    linspace(-300, 300, 7) nm; NaN inside a sphere) and the cross sections of the
+    expansion about the origin with lmax 6 of docs/examples/cluster.py, read from
+    docs/examples/output/cluster.txt into reference_examples.json."""
     path = os.path.join(REFERENCE, "docs", "examples", "output", "cluster.txt")
     with open(path) as f:
         lines = f.read().splitlines()
@@ -110,6 +111,10 @@ def cluster_example_fields():
     out = os.path.join(HERE, "reference_examples.json")
     with open(out) as f:
         data = json.load(f)
+    glob = next(line.split() for line in lines if line.split()[:1] == ["global"])
+    data["cluster_example"]["expected_global"] = {
+        "lmax": 6, "scattering": float(glob[1]), "extinction": float(glob[2]), "absorption": float(glob[3]),
+        "significant_digits": 6}
     data["cluster_example"]["intensity_local"] = {
         "grid": "x, z = linspace(-300, 300, 7) nm, y = 0; rows x, columns z; null inside a sphere",
         "values": rows, "significant_digits": 6}

@@ -375,3 +375,48 @@ def test_cluster_example_intensity_map_golden(pkg, oracle, golden_dir):
     got = 0.5 * torch.sum(torch.abs(e) ** 2, dim=1).cpu().numpy()
     want = [v for row in g["intensity_local"]["values"] for v in row if v is not None]
     assert [float(f"{v:.6g}") for v in got] == want
+
+
+@pytest.mark.parametrize("ld,ls", [(3, 3), (6, 3), (3, 6), (2, 5)])
+@pytest.mark.parametrize("regular", [False, True])
+def test_translation_blocks_rect(pkg, ld, ls, regular):
+    """Blocks between bases of different degree: the modes of degree <= min(ld, ls)
+    come first in both bases, so that corner equals the square block of that degree
+    (the selection rule and weights do not depend on lmax); regular blocks at the
+    origin are [I 0; 0 0]."""
+    rng = np.random.default_rng(ld * 10 + ls)
+    d = rng.normal(size=(5, 3)) * 1.7
+    k = 1.3
+    rect = pkg.translation_blocks_rect(ld, ls, k, d, regular).cpu().numpy()
+    lo = min(ld, ls)
+    m = 2 * lo * (lo + 2)
+    sq = pkg.translation_blocks(lo, k, d, regular).cpu().numpy()
+    assert rect.shape == (5, 2 * ld * (ld + 2), 2 * ls * (ls + 2))
+    assert _rel(rect[:, :m, :m], sq) <= 1e-13
+    if ld == ls:
+        assert _rel(rect, sq) <= 1e-14
+    if regular:
+        o = pkg.translation_blocks_rect(ld, ls, k, np.zeros((1, 3)), True).cpu().numpy()[0]
+        eye = np.zeros_like(o)
+        eye[:m, :m] = np.eye(m)
+        assert np.abs(o - eye).max() <= 1e-15
+
+
+def test_cluster_example_global_cross_sections_golden(pkg, oracle, golden_dir):
+    """docs/examples/cluster.py: the solved cluster expanded about the origin with
+    lmax 6 (spheres at lmax 3; rectangular regular translations), its cross sections
+    for the same plane wave against the reference's printed "global" row."""
+    cl, ref, g = _example(pkg, oracle, golden_dir)
+    f = pkg.SphereClusterFactor(cl, "c128")
+    Tg, conv = f.global_tmatrix(6)
+    assert Tg.shape == (96, 96)
+    # a one-sphere basis at the origin gives the plane wave's coefficients and the
+    # cross sections of a single-origin T-matrix (its regular overlap is I)
+    one = pkg.IterativeSphereCluster(6, ref.k0, np.array([1.0]), np.array([2.0 + 0j]), np.zeros((1, 3)))
+    a = one.plane_wave(g["direction"], g["pol"])
+    sca, ext = one.cross_sections(a, Tg @ a)
+    e = g["expected_global"]
+    assert float(f"{sca:.6g}") == e["scattering"]
+    assert float(f"{ext:.6g}") == e["extinction"]
+    assert float(f"{ext - sca:.6g}") == e["absorption"]
+    f.close()

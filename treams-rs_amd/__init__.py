@@ -19,6 +19,7 @@ from .cluster import (  # noqa: F401
     scattering_gradient,
     sphere_cluster_factor,
     translation_blocks,
+    translation_blocks_rect,
     wigner3j,
 )
 from .geometry import WORKLOADS, random_cluster, workload  # noqa: F401

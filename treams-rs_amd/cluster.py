@@ -341,17 +341,19 @@ class SphereClusterFactor:
         (linalg/mod.rs:137-236)."""
         return bool(lib().tg_factor_equilibrated(self._h))
 
-    def global_tmatrix(self):
-        """The cluster's T-matrix in one spherical basis of the same lmax at the
-        origin, ``Cluster(...).solve().expand(SphericalBasis.default(lmax))``
-        (python/treams_rs/_tmatrix.py:243-257): M x M, sphere mode order.  Returns
-        (T, [Convergence] of the M solves)."""
+    def global_tmatrix(self, lmax: int | None = None):
+        """The cluster's T-matrix in one spherical basis at the origin with degrees up
+        to ``lmax`` (default: the spheres'),
+        ``Cluster(...).solve().expand(SphericalBasis.default(lmax))``
+        (python/treams_rs/_tmatrix.py:243-257): Mg x Mg, sphere mode order.  Returns
+        (T, [Convergence] of the Mg solves)."""
         if self._h is None:
             raise ValueError("the factor was closed")
-        M = self.cluster.modes_per_sphere
+        lg = self.cluster.lmax if lmax is None else int(lmax)
+        M = int(lib().tg_modes_per_sphere(lg))
         out = torch.empty((M, M), dtype=torch.complex128, device=self.cluster._dev)
         conv = (_Conv * M)()
-        check(lib().tg_factor_global_tmatrix(self._h, self.rtol, out.data_ptr(), conv))
+        check(lib().tg_factor_global_tmatrix(self._h, lg, self.rtol, out.data_ptr(), conv))
         reports = [Convergence(0, c.residual_norm, c.rhs_norm, c.refinements) for c in conv]
         # column-major memory: out holds T_g transposed
         return out.t().contiguous(), reports
@@ -451,6 +453,18 @@ def plan_terms(lmax: int):
 
 def wigner3j(j1, j2, j3, m1, m2, m3) -> float:
     return lib().tg_wigner3j(j1, j2, j3, m1, m2, m3)
+
+
+def translation_blocks_rect(lmax_dst: int, lmax_src: int, k: float, displacements,
+                            regular: bool = False) -> torch.Tensor:
+    """(count, Md, Ms) translation blocks from a basis of degree lmax_src to one of
+    degree lmax_dst (sw/plan.rs:71-170,282-289), [destination, source]."""
+    d = torch.as_tensor(displacements, dtype=torch.float64).reshape(-1, 3).to(_device()).contiguous()
+    md, ms = int(lib().tg_modes_per_sphere(lmax_dst)), int(lib().tg_modes_per_sphere(lmax_src))
+    out = torch.empty((d.shape[0], ms, md), dtype=torch.complex128, device=_device())
+    check(lib().tg_translation_blocks_rect(int(lmax_dst), int(lmax_src), float(k), d.shape[0], d.data_ptr(),
+                                           int(regular), out.data_ptr(), _stream_ptr()))
+    return out.transpose(1, 2).contiguous()
 
 
 def translation_blocks(lmax: int, k: float, displacements, regular: bool = False) -> torch.Tensor:

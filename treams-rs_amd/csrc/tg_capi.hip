@@ -249,6 +249,34 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp) {
     return TG_OK;
 }
 
+int upload_rect_plan(const HostPlan &hp, DevBuf &buf, RectPlan &rp, hipStream_t s) {
+    const size_t E = (size_t)hp.H * hp.Hs, nterms = hp.w.size();
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_st = 0, o_ns = al(sizeof(int32_t) * (E + 1)), o_w = al(o_ns + E);
+    const size_t o_h = al(o_w + sizeof(double) * nterms), o_r = al(o_h + sizeof(uint16_t) * nterms);
+    const size_t total = al(o_r + sizeof(double) * hp.table_len);
+    TG_TRY(buf.ensure(total));
+    std::vector<char> host(total, 0);
+    memcpy(host.data() + o_st, hp.starts.data(), sizeof(int32_t) * (E + 1));
+    memcpy(host.data() + o_ns, hp.nsame.data(), E);
+    memcpy(host.data() + o_w, hp.w.data(), sizeof(double) * nterms);
+    memcpy(host.data() + o_h, hp.hidx.data(), sizeof(uint16_t) * nterms);
+    memcpy(host.data() + o_r, hp.refl.data(), sizeof(double) * hp.table_len);
+    char *base = buf.as<char>();
+    TG_HIP(hipMemcpyAsync(base, host.data(), total, hipMemcpyHostToDevice, s));
+    TG_HIP(hipStreamSynchronize(s));
+    rp.Hd = hp.H;
+    rp.Hs = hp.Hs;
+    rp.L2 = hp.L2;
+    rp.T = hp.table_len;
+    rp.starts = (const int32_t *)(base + o_st);
+    rp.nsame = (const uint8_t *)(base + o_ns);
+    rp.w = (const double *)(base + o_w);
+    rp.hidx = (const uint16_t *)(base + o_h);
+    rp.refl = (const double *)(base + o_r);
+    return TG_OK;
+}
+
 // y = (I - T C) x or (I - C^H T^H) x, split layout.
 int apply_operator(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, int op) {
     const int64_t count = (int64_t)cols * c->n;
@@ -455,6 +483,26 @@ int tg_translation_blocks(int lmax, double k, int64_t count, const double *disp,
             set_error("%s", hipGetErrorString(e));
             st = TG_DEVICE_ERROR;
         }
+    }
+    buf.release();
+    return st;
+}
+
+int tg_translation_blocks_rect(int lmax_dst, int lmax_src, double k, int64_t count, const double *disp, int regular,
+                               tg_c128 *out, void *stream) {
+    HostPlan hp;
+    if (build_plan_rect(lmax_dst, lmax_src, hp) || !(std::isfinite(k)) || count < 0 || (count > 0 && (!disp || !out))) {
+        set_error("invalid lmax pair, wavenumber, count or buffers");
+        return TG_INVALID_INPUT;
+    }
+    DevBuf buf;
+    RectPlan rp{};
+    hipStream_t s = (hipStream_t)stream;
+    int st = upload_rect_plan(hp, buf, rp, s);
+    if (!st) st = launch_translation_rect(rp, k, count, disp, regular, (cd *)out, s);
+    if (!st && hipStreamSynchronize(s) != hipSuccess) {
+        set_error("device failure in the translation blocks");
+        st = TG_DEVICE_ERROR;
     }
     buf.release();
     return st;

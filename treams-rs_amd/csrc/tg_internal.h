@@ -80,6 +80,20 @@ struct Timer {
     int collect();
 };
 
+// Device view of a rectangular translation plan (destination degrees up to one lmax,
+// source degrees up to another); entry e = a Hd + b, source a outer.
+struct RectPlan {
+    int Hd, Hs, L2, T;
+    const int32_t *starts;
+    const uint8_t *nsame;
+    const double *w;
+    const uint16_t *hidx;
+    const double *refl;
+};
+
+// Upload HostPlan `hp` (square or rectangular) as a RectPlan view into `buf`.
+int upload_rect_plan(const HostPlan &hp, DevBuf &buf, RectPlan &rp, hipStream_t s);
+
 struct Preconditioner;  // defined in tg_precond.hip
 
 }  // namespace tg
@@ -123,6 +137,9 @@ int launch_assemble_ref(tg_cluster *c, int regular, cd *out);
 int launch_assemble_operator(tg_cluster *c, int precision, void *out);
 int launch_translation_blocks(int lmax, const DevPlan &plan, double k, int64_t count, const double *disp,
                               int regular, cd *out, hipStream_t s);
+// count blocks of (2 Hd) x (2 Hs), column-major, destination inner
+int launch_translation_rect(const RectPlan &pl, double k, int64_t count, const double *disp, int regular, cd *out,
+                            hipStream_t s);
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular);
 int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols);
 int coupling_stored(tg_cluster *c, const cd *x, cd *y, int cols);

@@ -427,6 +427,65 @@ int launch_translation_blocks(int lmax, const DevPlan &pl, double k, int64_t cou
     return TG_OK;
 }
 
+// ----------------------------------------- translation blocks, rectangular plans --
+
+// One workgroup per displacement: the harmonic table z_p(kd) P_p^q e^{i q phi},
+// p <= L2, in LDS (regular: identity at the origin), then every (source a,
+// destination b) entry of the Md x Ms block: column-major, destination inner, in the
+// reference mode order; opposite helicities stay at the caller's zero fill.
+template <bool REG>
+__global__ __launch_bounds__(256) void k_translation_rect(RectPlan pl, double k, const double *disp, cd *out) {
+    extern __shared__ cd rect_lds[];
+    cd *tab = rect_lds, *rad = rect_lds + pl.T;
+    __shared__ int ok_s;
+    __shared__ double u_s[3];
+    const int blk = blockIdx.x;
+    if (threadIdx.x == 0) {
+        double d[3] = {disp[3 * blk], disp[3 * blk + 1], disp[3 * blk + 2]}, r, u[3];
+        ok_s = unit_direction(d, r, u);
+        if (ok_s) {
+            radial_values(pl.L2, k * r, REG, rad);
+            u_s[0] = u[0]; u_s[1] = u[1]; u_s[2] = u[2];
+        }
+    }
+    __syncthreads();
+    const double u[3] = {u_s[0], u_s[1], u_s[2]};
+    for (int q = threadIdx.x; q <= pl.L2; q += blockDim.x) {
+        if (ok_s) {
+            table_column(q, pl.L2, u, rad, pl.refl, tab);
+        } else {
+            zero_column(q, pl.L2, tab);
+            if (REG && q == 0) tab[0] = cd(1.0);
+        }
+    }
+    __syncthreads();
+    const int Md = 2 * pl.Hd;
+    cd *o = out + (size_t)blk * Md * 2 * pl.Hs;
+    for (int e = threadIdx.x; e < pl.Hd * pl.Hs; e += blockDim.x) {
+        const int a = e / pl.Hd, b = e - a * pl.Hd;
+        const int s0 = pl.starts[e], s1 = pl.starts[e + 1], ns = pl.nsame[e];
+        cd same(0.0), cross(0.0);
+        for (int t = s0; t < s0 + ns; ++t) same += pl.w[t] * tab[pl.hidx[t]];
+        for (int t = s0 + ns; t < s1; ++t) cross += pl.w[t] * tab[pl.hidx[t]];
+        o[(size_t)(2 * a) * Md + 2 * b] = cd(same.re - cross.im, same.im + cross.re);          // pol 1
+        o[(size_t)(2 * a + 1) * Md + 2 * b + 1] = cd(same.re + cross.im, same.im - cross.re);  // pol 0
+    }
+}
+
+int launch_translation_rect(const RectPlan &pl, double k, int64_t count, const double *disp, int regular, cd *out,
+                            hipStream_t s) {
+    const size_t blk = (size_t)4 * pl.Hd * pl.Hs;
+    TG_HIP(hipMemsetAsync(out, 0, sizeof(cd) * blk * count, s));
+    if (count == 0) return TG_OK;
+    const size_t smem = sizeof(cd) * (pl.T + pl.L2 + 1);
+    if (regular)
+        k_translation_rect<true><<<(unsigned)count, 256, smem, s>>>(pl, k, disp, out);
+    else
+        k_translation_rect<false><<<(unsigned)count, 256, smem, s>>>(pl, k, disp, out);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
 // --------------------------------------------------- matrix-free coupling (K2) --
 
 // y_i = sum_{j != i} C_ij x_j (or the regular overlap R with identity self blocks),

@@ -1065,26 +1065,32 @@ int tg_factor_solve(tg_factor *f, const tg_c128 *incident, int cols, double rtol
 
 int tg_factor_equilibrated(const tg_factor *f) { return f && f->equilibrated ? 1 : 0; }
 
-int tg_factor_global_tmatrix(tg_factor *f, double rtol, tg_c128 *out, tg_convergence *conv_host) {
+int tg_factor_global_tmatrix(tg_factor *f, int lmax_global, double rtol, tg_c128 *out, tg_convergence *conv_host) {
     if (!f || !out || !(rtol > 0.0)) {
         set_error("null argument or non-positive rtol");
         return TG_INVALID_INPUT;
     }
     tg_cluster *c = f->cluster;
-    const int N = c->N, M = c->M;
-    const int64_t n = c->n, MM = (int64_t)M * M;
-    // regular blocks: R(p_j <- 0) (displacement p_j) for the incident side, then
-    // R(0 <- p_i) (displacement -p_i) for the outgoing side
+    HostPlan pin, pout;  // R(p_j <- 0): lmax <- lmax_global; R(0 <- p_i): lmax_global <- lmax
+    if (lmax_global < 1 || build_plan_rect(c->lmax, lmax_global, pin) || build_plan_rect(lmax_global, c->lmax, pout)) {
+        set_error("global lmax must be >= 1 with lmax + global lmax <= %d", 2 * kMaxLmax);
+        return TG_INVALID_INPUT;
+    }
+    const int N = c->N, M = c->M, Mg = 2 * lmax_global * (lmax_global + 2);
+    const int64_t n = c->n, MMg = (int64_t)M * Mg;
     std::vector<double> disp(6 * (size_t)N);
     for (int j = 0; j < N; ++j)
         for (int a = 0; a < 3; ++a) {
             disp[3 * j + a] = c->positions[3 * j + a];
             disp[3 * (N + j) + a] = -c->positions[3 * j + a];
         }
-    DevBuf buf;
-    TG_TRY(buf.ensure(sizeof(cd) * (2 * N * MM + 2 * n * M) + sizeof(double) * disp.size()));
-    cd *blocks = buf.as<cd>(), *rin = blocks + 2 * N * MM, *z = rin + n * M;
-    double *ddisp = reinterpret_cast<double *>(z + n * M);
+    DevBuf buf, pbin, pbout;
+    RectPlan rin_plan{}, rout_plan{};
+    TG_TRY(upload_rect_plan(pin, pbin, rin_plan, c->stream));
+    TG_TRY(upload_rect_plan(pout, pbout, rout_plan, c->stream));
+    TG_TRY(buf.ensure(sizeof(cd) * (2 * N * MMg + 2 * n * Mg) + sizeof(double) * disp.size()));
+    cd *blocks = buf.as<cd>(), *rin = blocks + 2 * N * MMg, *z = rin + n * Mg;
+    double *ddisp = reinterpret_cast<double *>(z + n * Mg);
     int st = TG_OK;
     do {
         if (hipMemcpyAsync(ddisp, disp.data(), sizeof(double) * disp.size(), hipMemcpyHostToDevice, c->stream) !=
@@ -1093,25 +1099,27 @@ int tg_factor_global_tmatrix(tg_factor *f, double rtol, tg_c128 *out, tg_converg
             st = TG_DEVICE_ERROR;
             break;
         }
-        if ((st = launch_translation_blocks(c->lmax, c->plan, c->k0, 2 * N, ddisp, 1, blocks, c->stream))) break;
-        // Rin (n x M): rows (j, a), columns g; block j is R(p_j <- 0)[a][g] = blocks_j[g M + a]
+        // incident side: N blocks M x Mg; outgoing side: N blocks Mg x M
+        if ((st = launch_translation_rect(rin_plan, c->k0, N, ddisp, 1, blocks, c->stream))) break;
+        if ((st = launch_translation_rect(rout_plan, c->k0, N, ddisp + 3 * N, 1, blocks + N * MMg, c->stream))) break;
+        // Rin (n x Mg): rows (j, a), columns g; block j holds R(p_j <- 0)[a][g] at g M + a
         for (int j = 0; j < N && !st; ++j)
-            if (hipMemcpy2DAsync(rin + (int64_t)j * M, sizeof(cd) * n, blocks + j * MM, sizeof(cd) * M, sizeof(cd) * M,
-                                 M, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+            if (hipMemcpy2DAsync(rin + (int64_t)j * M, sizeof(cd) * n, blocks + j * MMg, sizeof(cd) * M, sizeof(cd) * M,
+                                 Mg, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
                 set_error("copy of the translation blocks failed");
                 st = TG_DEVICE_ERROR;
             }
         if (st) break;
-        // Z = (I - T C)^{-1} T Rin, then T_g = Rout Z with Rout = [R(0 <- p_1) ... R(0 <- p_N)] (M x n)
-        if ((st = tg_factor_solve(f, reinterpret_cast<const tg_c128 *>(rin), M, rtol, 0,
+        // Z = (I - T C)^{-1} T Rin, then T_g = Rout Z, Rout = [R(0 <- p_1) ... R(0 <- p_N)] (Mg x n)
+        if ((st = tg_factor_solve(f, reinterpret_cast<const tg_c128 *>(rin), Mg, rtol, 0,
                                   reinterpret_cast<tg_c128 *>(z), conv_host)))
             break;
-        const cd *rout = blocks + N * MM;
-        const dim3 grid((unsigned)((M + 63) / 64), (unsigned)((M + 63) / 64));
+        const cd *rout = blocks + N * MMg;
+        const dim3 grid((unsigned)((Mg + 63) / 64), (unsigned)((Mg + 63) / 64));
         if (n % 16)
-            k_zgemm_plain<double, true><<<grid, 256, 0, c->stream>>>(rout, M, z, n, (cd *)out, M, M, M, (int)n);
+            k_zgemm_plain<double, true><<<grid, 256, 0, c->stream>>>(rout, Mg, z, n, (cd *)out, Mg, Mg, Mg, (int)n);
         else
-            k_zgemm_plain<double, false><<<grid, 256, 0, c->stream>>>(rout, M, z, n, (cd *)out, M, M, M, (int)n);
+            k_zgemm_plain<double, false><<<grid, 256, 0, c->stream>>>(rout, Mg, z, n, (cd *)out, Mg, Mg, Mg, (int)n);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
             set_error("device failure in the global T-matrix");
             st = TG_DEVICE_ERROR;
@@ -1119,6 +1127,8 @@ int tg_factor_global_tmatrix(tg_factor *f, double rtol, tg_c128 *out, tg_converg
     } while (0);
     (void)hipStreamSynchronize(c->stream);
     buf.release();
+    pbin.release();
+    pbout.release();
     return st;
 }
 

@@ -62,28 +62,36 @@ static double factorial_ratio_sqrt(int p, int q) {
     return (double)std::sqrt(factorial_ld(p - q) / factorial_ld(p + q));
 }
 
-int build_plan(int lmax, HostPlan &plan) {
-    if (lmax < 1 || lmax > kMaxLmax) return -1;
-    const int H = lmax * (lmax + 2);
-    plan.lmax = lmax;
-    plan.H = H;
-    plan.L2 = 2 * lmax;
-    plan.table_len = (2 * lmax + 1) * (2 * lmax + 1);
-    plan.starts.assign((size_t)H * H + 1, 0);
-    plan.nsame.assign((size_t)H * H, 0);
+int build_plan_rect(int lmax_dst, int lmax_src, HostPlan &plan) {
+    if (lmax_dst < 1 || lmax_src < 1 || lmax_dst > 2 * kMaxLmax || lmax_src > 2 * kMaxLmax ||
+        lmax_dst + lmax_src > 2 * kMaxLmax)
+        return -1;
+    const int Hd = lmax_dst * (lmax_dst + 2), Hs = lmax_src * (lmax_src + 2);
+    plan.lmax = lmax_dst;
+    plan.lmax_src = lmax_src;
+    plan.H = Hd;
+    plan.Hs = Hs;
+    plan.L2 = lmax_dst + lmax_src;
+    plan.table_len = (plan.L2 + 1) * (plan.L2 + 1);
+    plan.starts.assign((size_t)Hd * Hs + 1, 0);
+    plan.nsame.assign((size_t)Hd * Hs, 0);
     plan.w.clear();
     plan.hidx.clear();
-    std::vector<int> ls(H), ms(H);
-    for (int l = 1, a = 0; l <= lmax; ++l)
-        for (int m = -l; m <= l; ++m, ++a) {
-            ls[a] = l;
-            ms[a] = m;
-        }
-    for (int a = 0; a < H; ++a) {       // source (l, m): column
-        for (int b = 0; b < H; ++b) {   // destination (lambda, mu): row
-            const int e = a * H + b;
+    auto modes = [](int lmax, std::vector<int> &ls, std::vector<int> &ms) {
+        for (int l = 1; l <= lmax; ++l)
+            for (int m = -l; m <= l; ++m) {
+                ls.push_back(l);
+                ms.push_back(m);
+            }
+    };
+    std::vector<int> lsrc, msrc, ldst, mdst;
+    modes(lmax_src, lsrc, msrc);
+    modes(lmax_dst, ldst, mdst);
+    for (int a = 0; a < Hs; ++a) {       // source (l, m): column
+        for (int b = 0; b < Hd; ++b) {   // destination (lambda, mu): row
+            const int e = a * Hd + b;
             plan.starts[e] = (int32_t)plan.w.size();
-            const int l = ls[a], m = ms[a], lam = ls[b], mu = ms[b];
+            const int l = lsrc[a], m = msrc[a], lam = ldst[b], mu = mdst[b];
             const int q = m - mu;
             const double pref = 0.5 * ((m % 2 == 0) ? 1.0 : -1.0) *
                                 std::sqrt((double)((2 * l + 1) * (2 * lam + 1)) /
@@ -115,7 +123,7 @@ int build_plan(int lmax, HostPlan &plan) {
             plan.nsame[e] = (uint8_t)count_same;
         }
     }
-    plan.starts[(size_t)H * H] = (int32_t)plan.w.size();
+    plan.starts[(size_t)Hd * Hs] = (int32_t)plan.w.size();
     // reflection factors (-1)^q (p-q)!/(p+q)! for negative orders
     plan.refl.assign(plan.table_len, 0.0);
     for (int p = 0; p <= plan.L2; ++p)
@@ -123,6 +131,11 @@ int build_plan(int lmax, HostPlan &plan) {
             plan.refl[p * p + p - q] =
                 ((q % 2) ? -1.0 : 1.0) * (double)(factorial_ld(p - q) / factorial_ld(p + q));
     return 0;
+}
+
+int build_plan(int lmax, HostPlan &plan) {
+    if (lmax < 1 || lmax > kMaxLmax) return -1;
+    return build_plan_rect(lmax, lmax, plan);
 }
 
 }  // namespace tg
