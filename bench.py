@@ -159,6 +159,15 @@ def dense_lu(pkg, torch, seed):
                      "true_relative_residual": [c.residual_norm / c.rhs_norm for c in conv]}
         xs[prec] = x
         log(f"dense LU {prec}: factor {1e3 * (t1 - t0):.0f} ms ({tf:.1f} TFLOP/s), solve {1e3 * (t2 - t1):.0f} ms")
+        if prec == "c128":
+            # the cluster's T-matrix about the origin (lmax 4): 48 columns solved with the
+            # factors, then one GEMM
+            t3 = time.perf_counter()
+            Tg, gconv = f.global_tmatrix()
+            torch.cuda.synchronize()
+            out["global_tmatrix_ms"] = 1e3 * (time.perf_counter() - t3)
+            out["global_tmatrix_columns"] = len(gconv)
+            log(f"global T-matrix (lmax {lmax}, {len(gconv)} columns): {out['global_tmatrix_ms']:.1f} ms")
         f.close()
     out["c64_vs_c128_solution_rel_diff"] = float(torch.linalg.norm(xs["c64"] - xs["c128"]) /
                                                  torch.linalg.norm(xs["c128"]))
@@ -340,6 +349,23 @@ def main() -> int:
     }
     if "assembly" in roofs:
         result["assembly_roofline"] = roofs["assembly"]
+
+    # near fields of the headline solution: 128 x 128 points on the plane y = 0 across
+    # the cluster, 4 columns (one thread per point, N M waves each)
+    g = np.linspace(-10.0, 10.0, 128)
+    pts = torch.tensor([[u, 0.0, w] for u in g for w in g], dtype=torch.float64)
+    cl.field(x_mixed, pts[:16])  # warm
+    torch.cuda.synchronize()
+    tf0 = time.perf_counter()
+    E = cl.field(x_mixed, pts)
+    torch.cuda.synchronize()
+    tfield = time.perf_counter() - tf0
+    waves = pts.shape[0] * N * cl.modes_per_sphere
+    result["fields"] = {"points": int(pts.shape[0]), "columns": ncols, "ms": 1e3 * tfield,
+                        "waves_per_s": waves / tfield, "finite": bool(torch.isfinite(E).all()),
+                        "note": "outgoing-wave field of the headline solution on a 128 x 128 grid at y = 0"}
+    log(f"fields: {pts.shape[0]} points x {N} spheres x {cl.modes_per_sphere} modes x {ncols} columns in "
+        f"{1e3 * tfield:.1f} ms")
 
     ref_iterations = None
     x_dense = None

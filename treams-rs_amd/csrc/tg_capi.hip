@@ -28,7 +28,7 @@ void set_error(const char *fmt, ...) {
 }
 
 // Released device blocks are kept in a process-wide cache and handed to later requests
-// of at least half their size: hipFree synchronises the device and unmaps the pages,
+// of the same size, else of at least half their size: hipFree synchronises the device and unmaps the pages,
 // which costs milliseconds per large block, and every solve allocates its scratch
 // again.  Work queued before a release may still use the block, so taking one from
 // the cache synchronises the device first.  Blocks above kCacheBlockMax are freed at
@@ -44,7 +44,10 @@ int g_live_clusters = 0;
 
 void *cache_take(size_t need, size_t &got) {
     std::lock_guard<std::mutex> lock(g_cache_mu);
-    auto it = g_cache.lower_bound(need);
+    // a block of exactly this size first (repeated solves ask for the same sizes, and
+    // taking a larger block would push its own request to a fresh allocation)
+    auto it = g_cache.find(need);
+    if (it == g_cache.end()) it = g_cache.lower_bound(need);
     if (it == g_cache.end() || it->first / 2 > need) return nullptr;
     void *p = it->second;
     got = it->first;

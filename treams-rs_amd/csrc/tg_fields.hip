@@ -19,22 +19,24 @@ namespace tg {
 
 constexpr int kFieldCols = 8;  // coefficient columns per pass
 
-// One thread per sample point: out[col][3 p + a] = sum_i sum_modes x[col][i M + mode]
-// wave_mode(point - pos_i).  Per sphere and order |m| the degree recurrences of the
-// solid harmonic (value and gradient) and of h_l run side by side, so no table
-// indexed by a run-time degree is kept.  The coefficients are read with wave-uniform
-// addresses.  A point at a sphere centre gets NaN.
-__global__ __launch_bounds__(128) void k_cluster_field(int N, int lmax, double k, const double *pos, const double *refl,
-                                                       const cd *x, int64_t n, int cols, int64_t npts,
-                                                       const double *points, cd *out) {
+// One thread per (sample point, chunk of spheres): part[chunk][col][3 p + a] =
+// sum_{i in chunk} sum_modes x[col][i M + mode] wave_mode(point - pos_i); the chunks
+// are summed afterwards in a fixed order.  Per sphere and order |m| the degree
+// recurrences of the solid harmonic (value and gradient) and of h_l run side by
+// side, so no table indexed by a run-time degree is kept.  The coefficients are read
+// with wave-uniform addresses.  A point at a sphere centre gets NaN.
+__global__ __launch_bounds__(128) void k_cluster_field(int N, int chunk, int lmax, double k, const double *pos,
+                                                       const double *refl, const cd *x, int64_t n, int cols,
+                                                       int64_t npts, const double *points, cd *part) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npts) return;
+    const int i0 = blockIdx.y * chunk, i1 = min(i0 + chunk, N);
     const int M = 2 * lmax * (lmax + 2);
     cd E[kFieldCols][3];
 #pragma unroll
     for (int c = 0; c < kFieldCols; ++c) E[c][0] = E[c][1] = E[c][2] = cd(0.0);
     bool singular = false;
-    for (int i = 0; i < N; ++i) {
+    for (int i = i0; i < i1; ++i) {
         double rs[3];
         double scale = 0.0;
         for (int a = 0; a < 3; ++a) {
@@ -47,8 +49,8 @@ __global__ __launch_bounds__(128) void k_cluster_field(int N, int lmax, double k
         }
         for (int a = 0; a < 3; ++a) rs[a] /= scale;
         const double ks = k * scale;
-        const double r2 = rs[0] * rs[0] + rs[1] * rs[1] + rs[2] * rs[2], rr = sqrt(r2);
-        const double xarg = ks * rr, inv = 1.0 / xarg, sx = sin(xarg), cxv = cos(xarg);
+        const double r2 = rs[0] * rs[0] + rs[1] * rs[1] + rs[2] * rs[2], rr = sqrt(r2), irr = 1.0 / rr;
+        const double xarg = ks * rr, inv = 1.0 / xarg, sx = sin(xarg), cxv = cos(xarg), iks = 1.0 / ks;
         const cd xy(rs[0], rs[1]);
         const cd *xi = x + (int64_t)i * M;
         // ladder over the order: P_q^q (-1)^q (2q-1)!! (x + i y)^q and its gradient
@@ -66,7 +68,7 @@ __global__ __launch_bounds__(128) void k_cluster_field(int N, int lmax, double k
             cd S = lp, G[3] = {lg[0], lg[1], lg[2]};
             cd Sp(0.0), Gp[3] = {cd(0.0), cd(0.0), cd(0.0)};
             cd hprev(cxv * inv, sx * inv), hcur(sx * inv, -cxv * inv);  // h_{-1}, h_0
-            double rpow = 1.0;                                            // rr^l
+            double irpow = 1.0;                                           // rr^-l
             for (int l = 0; l <= lmax; ++l) {
                 const cd hnext = ((double)(2 * l + 1) * inv) * hcur - hprev;  // h_{l+1}
                 if (l > q) {
@@ -83,8 +85,8 @@ __global__ __launch_bounds__(128) void k_cluster_field(int N, int lmax, double k
                     }
                 }
                 if (l >= q && l >= 1) {
-                    const cd cl = hcur / rpow, el = hnext / (rpow * rr);
-                    const cd d = ((double)(l + 1) / ks) * cl - r2 * el;
+                    const cd cl = irpow * hcur, el = (irpow * irr) * hnext;
+                    const cd d = ((double)(l + 1) * iks) * cl - r2 * el;
                     const double base = sqrt((2.0 * l + 1.0) / (4.0 * M_PI * l * (l + 1.0)));
                     // (l - q)! / (l + q)!: the plan's reflection factors hold it for q >= 1
                     const double ratio = q == 0 ? 1.0 : fabs(refl[l * l + l - q]);
@@ -122,15 +124,25 @@ __global__ __launch_bounds__(128) void k_cluster_field(int N, int lmax, double k
                 }
                 hprev = hcur;
                 hcur = hnext;
-                rpow *= rr;
+                irpow *= irr;
             }
         }
     }
+    cd *out = part + (int64_t)blockIdx.y * cols * 3 * npts;
 #pragma unroll
     for (int cc = 0; cc < kFieldCols; ++cc) {
         if (cc >= cols) break;
         for (int c = 0; c < 3; ++c) out[(int64_t)cc * 3 * npts + 3 * p + c] = singular ? cd(NAN, NAN) : E[cc][c];
     }
+}
+
+// out[t] = sum over the chunks of part[chunk][t], in chunk order.
+__global__ void k_sum_parts(const cd *part, int parts, int64_t count, cd *out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    cd s(0.0);
+    for (int q = 0; q < parts; ++q) s += part[(int64_t)q * count + t];
+    out[t] = s;
 }
 
 // out[3 p + a] = e_a exp(i k.r_p) for the helicity polarization vector e.
@@ -173,14 +185,29 @@ int tg_cluster_field(tg_cluster *c, const tg_c128 *x, int cols, int64_t npoints,
     }
     if (npoints == 0) return TG_OK;
     const double *pos = c->b_pos.as<double>();
+    // enough (point, sphere chunk) threads to fill the chip: about 64 waves per CU
+    const int64_t target = (int64_t)256 * 64 * 64;
+    int parts = (int)std::min<int64_t>(c->N, std::max<int64_t>(1, target / std::max<int64_t>(npoints, 1)));
+    parts = std::min(parts, 64);
+    const int chunk = (c->N + parts - 1) / parts;
+    parts = (c->N + chunk - 1) / chunk;
+    DevBuf scratch;
+    TG_TRY(scratch.ensure(sizeof(cd) * (size_t)parts * kFieldCols * 3 * npoints));
     for (int c0 = 0; c0 < cols; c0 += kFieldCols) {
         const int nc = std::min(kFieldCols, cols - c0);
-        k_cluster_field<<<(unsigned)((npoints + 127) / 128), 128, 0, c->stream>>>(
-            c->N, c->lmax, c->k0, pos, c->plan.refl, (const cd *)x + (int64_t)c0 * c->n, c->n, nc, npoints, points,
-            (cd *)out + (int64_t)c0 * 3 * npoints);
+        cd *dst = (cd *)out + (int64_t)c0 * 3 * npoints;
+        k_cluster_field<<<dim3((unsigned)((npoints + 127) / 128), (unsigned)parts), 128, 0, c->stream>>>(
+            c->N, chunk, c->lmax, c->k0, pos, c->plan.refl, (const cd *)x + (int64_t)c0 * c->n, c->n, nc, npoints,
+            points, parts == 1 ? dst : scratch.as<cd>());
         TG_HIP(hipGetLastError());
+        if (parts > 1) {
+            const int64_t count = (int64_t)nc * 3 * npoints;
+            k_sum_parts<<<(unsigned)((count + 255) / 256), 256, 0, c->stream>>>(scratch.as<cd>(), parts, count, dst);
+            TG_HIP(hipGetLastError());
+        }
     }
     TG_HIP(hipStreamSynchronize(c->stream));
+    scratch.release();
     return TG_OK;
 }
 
