@@ -174,6 +174,37 @@ def dense_lu(pkg, torch, seed):
     return out
 
 
+def interacting_tmatrix(pkg, torch):
+    """The reference's dense benchmark (docs/performance/evidence.md:152-176,
+    scripts/benchmark_cluster.py:281-284): chains of spheres with radii linspace(0.15,
+    0.25), spacing 0.8, eps 4+0.1i, k0 1.3; particle coefficients, translations and the
+    full interacting T-matrix X = (I - T C)^{-1} T (all n columns), here c128 factors
+    and FP64-refined forward solves.  Published 4-thread CPU times beside."""
+    published = {(8, 3): 1.96, (16, 3): 9.35, (8, 5): 12.69, (32, 3): 43.41}
+    out = []
+    for (N, lmax), ref_ms in published.items():
+        radii = np.linspace(0.15, 0.25, N)
+        pos = np.column_stack([np.arange(N) * 0.8, np.zeros((N, 2))])
+        eps = np.full(N, 4 + 0.1j)
+        times = []
+        for rep in range(4):  # the first is the warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            cl = pkg.IterativeSphereCluster(lmax, 1.3, radii, eps, pos)
+            f = pkg.SphereClusterFactor(cl, "c128")
+            X, conv = f.solve(torch.eye(cl.n, dtype=torch.complex128, device="cuda"))
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            f.close()
+            del cl
+        ms = 1e3 * min(times[1:])
+        out.append({"spheres": N, "lmax": lmax, "n": int(X.shape[0]), "ms": ms, "reference_cpu_4_threads_ms": ref_ms,
+                    "speedup": ref_ms / ms})
+        log(f"interacting T-matrix N={N} lmax={lmax} (n={X.shape[0]}): {ms:.2f} ms (reference CPU, 4 threads: "
+            f"{ref_ms} ms)")
+    return out
+
+
 def gradient_lu(pkg, torch, seed):
     """sigma_sca of a +z positive-helicity plane wave and its gradient with respect to
     all 768 position components and 256 radii (N = 256, lmax = 4, 12,288 unknowns),
@@ -426,6 +457,7 @@ def main() -> int:
     if not args.no_lu:
         del cl
         result["dense_lu"] = dense_lu(pkg, torch, args.seed)
+        result["interacting_tmatrix"] = interacting_tmatrix(pkg, torch)
         result["gradient_lu"] = gradient_lu(pkg, torch, args.seed)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

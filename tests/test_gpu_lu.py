@@ -70,11 +70,14 @@ def test_factor_adjoint_solve(pkg, oracle, precision):
     assert _rel(f.pullback_incident(g), oracle.apply_local(ref.blocks, yr, True)) <= 1e-10
 
 
-def test_interacting_tmatrix_many_columns(pkg, oracle):
+@pytest.mark.parametrize("N,lmax,precision", [(4, 2, "c128"), (6, 3, "c64"), (9, 3, "c128")])
+def test_interacting_tmatrix_many_columns(pkg, oracle, N, lmax, precision):
     """All n columns (the interacting T-matrix X = (I - T C)^{-1} T of sphere_cluster,
-    cluster/spheres.rs:56-80), solved in chunks of 8 columns."""
-    cl, ref = _cluster(pkg, oracle, N=4, lmax=2, seed=2, side=1.6)
-    f = pkg.SphereClusterFactor(cl, "c128")
+    cluster/spheres.rs:56-80): forward solves of more than 8 columns go block by block
+    with MFMA GEMM updates (c64 factors widened to FP64), refined in FP64; n = 64, 180
+    (c64, ragged last block) and 270."""
+    cl, ref = _cluster(pkg, oracle, N=N, lmax=lmax, seed=2, side=1.6 * (N / 4) ** (1 / 3))
+    f = pkg.SphereClusterFactor(cl, precision)
     eye = np.eye(ref.n, dtype=np.complex128)
     X, conv = f.solve(eye)
     Xr = np.linalg.solve(ref.operator(), ref.T)

@@ -374,8 +374,10 @@ __global__ __launch_bounds__(512) void k_lu_block_solve(const cx<T> *A0, Batch b
             S[r][c].set(cx<T>(e.re, -e.im));  // conj(A(r, c)) at S[r][c]: (A^H)(c, r)
     }
     __syncthreads();
-    if (wave >= cols) return;
-    cd *v = x0 + (int64_t)wave * ld + bt.voff[z] + k0;
+    // 8 columns per workgroup: grid.x selects the chunk of columns
+    const int c0 = blockIdx.x * 8;
+    if (wave >= cols - c0) return;
+    cd *v = x0 + (int64_t)(c0 + wave) * ld + bt.voff[z] + k0;
     cd val = lane < nb ? v[lane] : cd();
     auto s = [&](int k) {
         const cx<T> e = S[k][lane].get();
@@ -695,7 +697,79 @@ static int luset_solve8(LuSet &ls, hipStream_t s, cd *x, int cols, int adjoint) 
     return TG_OK;
 }
 
+// X[rb : re, :] -= F[rb : re, k0 : k0 + K] X[k0 : k0 + K, :] for `cols` packed FP64
+// columns (matrix z's rows at voff[z]), F the factors in T (rows below the block: L;
+// rows above: U), on FP64 MFMA in the 3M form, one 64 x 64 tile of X per workgroup.
+template <class T>
+__global__ __launch_bounds__(256, 2) void k_lu_solve_update(const cx<T> *A0, Batch bt, int64_t k0, int64_t rb_,
+                                                            int64_t re_, cd *x0, int64_t ldx, int cols) {
+    const int z = blockIdx.z;
+    const int64_t n = bt.n[z];
+    if (k0 >= n) return;
+    const int K = (int)min((int64_t)NB, n - k0);
+    const int64_t re = min(re_, n), r0 = rb_ + (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    if (r0 >= re || c0 >= cols) return;
+    const cx<T> *F = A0 + bt.off[z];
+    cd *X = x0 + bt.voff[z];
+    __shared__ TileLds<double> sm;
+    Tile3M<double> acc;
+    const int64_t cmax = cols;
+    auto bcol = [=](int64_t c) { return (const cd *)(X + min(c, cmax - 1) * ldx + k0); };
+    zgemm_tile_3m_cols<double, true, 64, 64>(F + k0 * n, n, bcol, r0, c0, re, K, sm, acc);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, rr = lane & 15;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<double>(lane, v);
+                if (gr >= re || gc >= cols) continue;
+                const double t1 = acc.t1[a][b][v], t2 = acc.t2[a][b][v], t3 = acc.t3[a][b][v];
+                cd &y = X[gc * ldx + gr];
+                y = cd(y.re - (t1 - t2), y.im - (t3 - t1 - t2));
+            }
+}
+
+// Forward solve of many columns: x <- P x, L y = x, U x = y, each 64-row block solved
+// by 8-column workgroups and the rest of the rows updated by one MFMA GEMM per block.
+template <class T>
+static int luset_solve_many(LuSet &ls, hipStream_t s, cd *x, int cols) {
+    const cx<T> *A = reinterpret_cast<const cx<T> *>(ls.A);
+    const Batch bt{ls.off, ls.n, ls.voff};
+    const unsigned z = (unsigned)ls.count;
+    const int64_t nmax = ls.nmax, ld = ls.ntot;
+    DevBuf tmp;
+    TG_TRY(tmp.ensure(sizeof(cd) * (size_t)ld * cols));
+    const unsigned pgrid = (unsigned)((ld * cols + 255) / 256);
+    k_lu_permute<<<pgrid, 256, 0, s>>>(x, tmp.as<cd>(), ld, cols, ls.perm, 0);
+    TG_HIP(hipMemcpyAsync(x, tmp.as<cd>(), sizeof(cd) * ld * cols, hipMemcpyDeviceToDevice, s));
+    const unsigned cchunks = (unsigned)((cols + 7) / 8), ctiles = (unsigned)((cols + 63) / 64);
+    for (int64_t k0 = 0; k0 < nmax; k0 += NB) {
+        k_lu_block_solve<T><<<dim3(cchunks, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 0);
+        const int64_t below = nmax - k0 - NB;
+        if (below > 0)
+            k_lu_solve_update<T><<<dim3(ctiles, (unsigned)((below + 63) / 64), z), 256, 0, s>>>(
+                A, bt, k0, k0 + NB, nmax, x, ld, cols);
+    }
+    const int64_t last = ((nmax - 1) / NB) * NB;
+    for (int64_t k0 = last; k0 >= 0; k0 -= NB) {
+        k_lu_block_solve<T><<<dim3(cchunks, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 1);
+        if (k0 > 0)
+            k_lu_solve_update<T><<<dim3(ctiles, (unsigned)((k0 + 63) / 64), z), 256, 0, s>>>(A, bt, k0, 0, k0, x, ld,
+                                                                                            cols);
+    }
+    TG_HIP(hipGetLastError());
+    TG_HIP(hipStreamSynchronize(s));
+    tmp.release();
+    return TG_OK;
+}
+
 int luset_solve(LuSet &ls, hipStream_t s, cd *x, int cols, int adjoint) {
+    if (!adjoint && cols > 8)
+        return ls.precision == TG_STORE_C64 ? luset_solve_many<float>(ls, s, x, cols)
+                                            : luset_solve_many<double>(ls, s, x, cols);
     for (int c0 = 0; c0 < cols; c0 += 8) {
         const int w = std::min(8, cols - c0);
         cd *xc = x + (int64_t)c0 * ls.ntot;

@@ -52,14 +52,15 @@ struct Tile3M {
     typename mfma_acc<T>::V t1[2][2], t2[2][2], t3[2][2];
 };
 
-// t += A[r0 : r0 + 64, 0 : K] B[0 : K, cols] for column-major A (lda); column c of B
+// t += A[r0 : r0 + BM, 0 : K] B[0 : K, cols] for column-major A (lda, elements of type
+// TA converted to T on load); column c of B
 // (c = c0 .. c0 + 63) starts at bcol(c), which the caller keeps a valid address for
 // every c (clamped).  Rows >= M are read clamped: they only feed output rows that
 // the caller does not store.  With KTAIL, k >= K is read clamped and zeroed (K not a
 // multiple of 16).  No load is conditional; the next K slice is loaded while the
 // current one is multiplied.  256 threads.
-template <class T, bool KTAIL, int BM, int BN, class BCol>
-__device__ __forceinline__ void zgemm_tile_3m_cols(const cx<T> *A, int64_t lda, BCol bcol, int64_t r0, int64_t c0,
+template <class T, bool KTAIL, int BM, int BN, class TA, class BCol>
+__device__ __forceinline__ void zgemm_tile_3m_cols(const cx<TA> *A, int64_t lda, BCol bcol, int64_t r0, int64_t c0,
                                                    int64_t M, int K, TileLds<T, BM, BN> &sm, Tile3M<T> &acc) {
     static_assert(BM * BN == 64 * 64 && BM % 32 == 0 && BN % 32 == 0, "4 waves of 32 x 32");
     constexpr int WC = BN / 32, NA = BM / 16, NB = BN / 16, KA = 256 / BM;
@@ -84,11 +85,13 @@ __device__ __forceinline__ void zgemm_tile_3m_cols(const cx<T> *A, int64_t lda, 
 #pragma unroll
         for (int j = 0; j < NA; ++j) {
             const int ka = kk + lk + KA * j;
+            // A may be stored narrower than the compute type (c64 factors, FP64 vectors)
             if (KTAIL) {
-                const cx<T> av = A[(int64_t)min(ka, K - 1) * lda + arow];
-                areg[j] = ka < K ? av : cx<T>();
+                const cx<TA> av = A[(int64_t)min(ka, K - 1) * lda + arow];
+                areg[j] = ka < K ? cx<T>((T)av.re, (T)av.im) : cx<T>();
             } else {
-                areg[j] = A[(int64_t)ka * lda + arow];
+                const cx<TA> av = A[(int64_t)ka * lda + arow];
+                areg[j] = cx<T>((T)av.re, (T)av.im);
             }
         }
 #pragma unroll
