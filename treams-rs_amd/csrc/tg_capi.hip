@@ -157,7 +157,7 @@ int Timer::collect() {
 }
 
 // Upload a host plan into one device buffer and fill the device view.
-static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp) {
+static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp, std::vector<int> *ntiles = nullptr) {
     const size_t HH = (size_t)hp.H * hp.H;
     const size_t nterms = hp.w.size();
     // transposed plan: terms grouped by harmonic index, entry order kept
@@ -246,9 +246,12 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp) {
     dp.smode = (const int32_t *)(base + off_smode);
     dp.ucol = (const int32_t *)(base + off_ucol);
     dp.stiles = (const int32_t *)(base + off_stiles);
-    for (int w = 0; w < 2; ++w)
-        for (int cols = 1; cols <= kGemmCols; ++cols)
-            dp.ntiles[w][cols - 1] = stiles[(size_t)(w * kGemmCols + cols - 1) * (T + 1) + T];
+    if (ntiles) {
+        ntiles->assign((size_t)2 * kGemmCols, 0);
+        for (int w = 0; w < 2; ++w)
+            for (int cols = 1; cols <= kGemmCols; ++cols)
+                (*ntiles)[(size_t)w * kGemmCols + cols - 1] = stiles[(size_t)(w * kGemmCols + cols - 1) * (T + 1) + T];
+    }
     return TG_OK;
 }
 
@@ -391,7 +394,7 @@ int tg_cluster_create(int lmax, double k0, int n_spheres, const double *radii, c
             st = TG_INVALID_INPUT;
             break;
         }
-        if ((st = upload_plan(c->hplan, c->b_plan, c->plan))) break;
+        if ((st = upload_plan(c->hplan, c->b_plan, c->plan, &c->gemm_ntiles))) break;
         std::vector<double> geo(6 * (size_t)n_spheres);
         memcpy(geo.data(), positions, sizeof(double) * 3 * n_spheres);
         memcpy(geo.data() + 3 * n_spheres, radii, sizeof(double) * n_spheres);

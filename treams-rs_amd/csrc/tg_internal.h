@@ -31,7 +31,7 @@ void set_error(const char *fmt, ...);
         if (_s != TG_OK) return _s;    \
     } while (0)
 
-constexpr int kGemmCols = 8;  // right-hand sides per matrix-free GEMM pass
+constexpr int kGemmCols = 64;  // most right-hand sides per matrix-free GEMM pass
 // Column-tile widths of the matrix-free GEMMs: 64 x 64 output tiles in FP32 (the
 // fewest loads per MFMA), 128 x 32 in FP64 (less column padding).
 constexpr int kTileWidths[2] = {64, 32};
@@ -58,7 +58,6 @@ struct DevPlan {
     // GEMM column tiles of width kTileWidths[w] before harmonic t for G = 2 cols column
     // groups: stiles[((w kGemmCols) + cols - 1) (T + 1) + t], cols = 1 .. kGemmCols
     const int32_t *stiles;
-    int ntiles[2][kGemmCols];    // the totals (t = T): the column tiles of a pass
 };
 
 // Growable device buffer.
@@ -107,6 +106,7 @@ struct tg_cluster {
     std::vector<cd> eps;
     tg::HostPlan hplan;
     tg::DevPlan plan{};
+    std::vector<int> gemm_ntiles;      // column tiles of a GEMM pass: [w kGemmCols + cols - 1] (plan.stiles totals)
     tg::DevBuf b_plan, b_pos, b_mie;   // b_mie: value, d/dsize, d/deps blocks
     int storage = TG_STORE_NONE;
     tg::DevBuf b_C;                    // stored split coupling: [2][nh][nh] (c64 or c128)

@@ -819,9 +819,12 @@ template <class R>
 static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool regular) {
     TG_TRY(ensure_pair_tables(c, regular, sizeof(R) == 4));
     const int N = c->N, H = c->H, T = c->plan.T;  // T: harmonic indices
-    const int cmax = kGemmCols;  // right-hand sides per pass
-    const int64_t Qmax = (int64_t)std::min(cols, cmax) * 2 * H;
     const int S = c->plan.S;
+    // right-hand sides per pass: the packed sources and U of a pass within 512 MB
+    // (at least 8, at most kGemmCols)
+    const size_t per_col = sizeof(cx<R>) * ((size_t)2 * H * N + (size_t)2 * S * N);
+    const int cmax = (int)std::min<size_t>(kGemmCols, std::max<size_t>(8, ((size_t)512 << 20) / per_col));
+    const int64_t Qmax = (int64_t)std::min(cols, cmax) * 2 * H;
     TG_TRY(c->ws[0].ensure(sizeof(cx<R>) * ((size_t)Qmax * N + (size_t)std::min(cols, cmax) * 2 * S * N)));
     cx<R> *X = c->ws[0].as<cx<R>>();
     cx<R> *U = X + (size_t)Qmax * N;
@@ -836,7 +839,7 @@ static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool reg
         // FP32: 64 x 64 tiles (w = 0); FP64: 128 x 32 (w = 1)
         constexpr int w = sizeof(R) == 8 ? 1 : 0, BN = kTileWidths[w], BM = 64 * 64 / BN;
         const int32_t *tiles = c->plan.stiles + (size_t)(w * kGemmCols + nc - 1) * (T + 1);
-        const unsigned grid = (unsigned)c->plan.ntiles[w][nc - 1] * (unsigned)((N + BM - 1) / BM);
+        const unsigned grid = (unsigned)c->gemm_ntiles[(size_t)w * kGemmCols + nc - 1] * (unsigned)((N + BM - 1) / BM);
         Timer::Rec grec;
         const char *gname = regular ? "gemm_overlap" : sizeof(R) == 8 ? "gemm_f64" : "gemm_c64";
         TG_TRY(c->timer.begin(gname, c->stream, grec));
