@@ -35,6 +35,7 @@ sys.path.insert(0, ROOT)
 METRIC = "solves/s @N=1024,lmax=6 to 1e-8 resid: learned-PC vs plain; %HBM/MFMA peak"
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
+FP16_MFMA_PEAK_TFLOPS = 2500.0
 
 
 def subset_modes(pkg, lmax: int) -> int:
@@ -91,8 +92,8 @@ def run_method(cl, B, torch, *, mixed, pc, steps, warmup, rtol, restart, label, 
     elapsed = time.perf_counter() - t0
     cl.timing(False)
     kernels = {}
-    for name in ("assemble", "matvec_stored", "coupling_mf", "coupling_mf32", "gemm_c64", "gemm_f64", "pc_apply",
-                 "pc_factor"):
+    for name in ("assemble", "matvec_stored", "coupling_mf", "coupling_mf32", "gemm_c64", "gemm_split16",
+                 "gemm_f64", "pc_apply", "pc_factor"):
         ms, count = cl.timing_get(name)
         if count:
             kernels[name] = {"total_ms": ms, "launches": count, "avg_ms": ms / count}
@@ -248,6 +249,9 @@ def main() -> int:
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--operator", choices=("tables", "stored"), default="tables",
                     help="c64 coupling of the headline inner solves: pair tables (GEMMs) or the dense store")
+    ap.add_argument("--inner", choices=("split16", "fp32"), default="split16",
+                    help="arithmetic of the c64-precision table GEMMs: scaled FP16 hi + lo halves on FP16 MFMA, "
+                         "or c64 tables on FP32 MFMA")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="bytes of a plain device copy run first (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -277,6 +281,7 @@ def main() -> int:
     N = radii.size
     log(f"workload {args.workload}: N={N} lmax={lmax} n={N * 2 * lmax * (lmax + 2)}")
     cl = pkg.IterativeSphereCluster(lmax, k0, radii, eps, pos)
+    cl.set_inner_gemm(args.inner)
     cl.set_group_size(args.bj_group)
     if args.pc == "learned":
         import importlib
@@ -301,6 +306,23 @@ def main() -> int:
                "iterations": [c.iterations for c in cv], "refinements": [c.refinements for c in cv],
                "true_relative_residual": [c.residual_norm / c.rhs_norm for c in cv], "kernels": kern}
         roofs = {}
+        gemm = kern.get("gemm_split16", {})
+        if gemm:
+            cmacs = float(N) * N * 2 * ncols * subset_modes(pkg, lmax)
+            achieved = 8.0 * cmacs / (gemm["avg_ms"] * 1e-3) / 1e12
+            executed = 24.0 * cmacs / (gemm["avg_ms"] * 1e-3) / 1e12
+            roofs["gemm"] = {
+                "bound": "mfma",
+                "kernel": "k_zgemm_subsets_split16 (c64-precision pair-table GEMMs on FP16 MFMA, scaled hi + lo "
+                          "halves, 4 RHS, 4M form)",
+                "achieved": achieved, "peak": FP16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP16_MFMA_PEAK_TFLOPS, "traffic": None, "flops_per_launch": 8.0 * cmacs,
+                "avg_ms": gemm["avg_ms"], "executed_fp16_tflops": executed,
+                "executed_frac": executed / FP16_MFMA_PEAK_TFLOPS,
+                "flop_count": "8 real flops per complex multiply-add over the nonzero (harmonic, source mode) "
+                              "columns; the split 4M form executes 24 FP16 MFMA flops per complex multiply-add "
+                              "(4 real products x 3 half products)"}
+            add_traffic(roofs["gemm"], "gemm_split16_traffic_n1024_l6.json")
         gemm = kern.get("gemm_c64", {})
         if gemm:
             cmacs = float(N) * N * 2 * ncols * subset_modes(pkg, lmax)

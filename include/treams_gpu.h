@@ -174,6 +174,18 @@ int tg_cluster_release(tg_cluster *cluster);
 int tg_cluster_set_table_limit(tg_cluster *cluster, int64_t bytes);
 
 /*
+ * Arithmetic of the c64-precision coupling GEMMs of the inner mixed-precision solves
+ * (TG_OP_MATRIX_FREE_C64 and mixed_precision 2): TG_INNER_SPLIT16 (default) holds
+ * tables and sources as scaled FP16 hi + lo pairs and multiplies them on FP16 MFMA
+ * (22 significant bits per value, FP32 accumulation); TG_INNER_FP32 rounds the tables
+ * to c64 and multiplies on FP32 MFMA.  Frees the current c64-precision tables.  No
+ * reference counterpart (device arithmetic choice; the FP64 residual certifies every
+ * solve either way).
+ */
+typedef enum { TG_INNER_FP32 = 0, TG_INNER_SPLIT16 = 1 } tg_inner_gemm;
+int tg_cluster_set_inner_gemm(tg_cluster *cluster, int kind);
+
+/*
  * y = (I - T C) x, or (I - C^H T^H) x with adjoint != 0, for `cols` column-major
  * vectors of length n (device c128): replaces IterativeSphereCluster::apply
  * (cluster/iterative.rs:225-232).  op selects the matrix-free FP64 or stored path.

@@ -123,6 +123,31 @@ def test_operator_apply_matrix_free_c64(pkg, oracle):
         assert _rel(cl.coupling(x, c64=True), ref.C @ x) <= 1e-5
 
 
+@pytest.mark.parametrize("N,lmax,cols", [(6, 3, 3), (45, 4, 5), (96, 6, 4), (130, 3, 12)])
+def test_inner_gemm_split16_vs_fp32(pkg, N, lmax, cols):
+    """The split-FP16 inner coupling (scaled hi + lo halves on FP16 MFMA) against the
+    FP64 operator, beside the FP32 form: single-precision level for both, on ragged
+    sizes (N not a multiple of 32, partial row and column tiles), several right-hand
+    sides, and a source vector whose sphere amplitudes span 1e-9 .. 1 (the per-column
+    shifts and the 2^17 full-precision window)."""
+    radii, e, pos = pkg.random_cluster(N, 1.9 * N ** (1 / 3), 0.05, (0.15, 0.35), None, seed=N)
+    cl = pkg.IterativeSphereCluster(lmax, 1.3, radii, e, pos)
+    rng = np.random.default_rng(N)
+    x = rng.normal(size=(cl.n, cols)) + 1j * rng.normal(size=(cl.n, cols))
+    amp = 10.0 ** rng.uniform(-9, 0, size=N)
+    xs = x * np.repeat(amp, cl.n // N)[:, None]
+    errs = {}
+    for kind in ("split16", "fp32"):
+        cl.set_inner_gemm(kind)
+        for name, v in (("plain", x), ("spread", xs)):
+            ref = cl.apply(v).cpu().numpy()
+            errs[kind, name] = _rel(cl.apply(v, c64=True), ref)
+            assert errs[kind, name] <= 2e-6, (kind, name, errs[kind, name])
+        errs[kind, "coupling"] = _rel(cl.coupling(x, c64=True), cl.coupling(x))
+        assert errs[kind, "coupling"] <= 1e-5, (kind, errs[kind, "coupling"])
+    print(N, lmax, {k: f"{v:.2e}" for k, v in errs.items()})
+
+
 def test_solve_mixed_matrix_free_c64(pkg, oracle):
     cl, ref = _cluster(pkg, oracle, N=12, lmax=4, seed=8, side=3.5)
     a = cl.plane_wave([0, 0, 1], 1)

@@ -32,7 +32,7 @@ EXPORTED = [
     "tg_cluster_local", "tg_cluster_overlap", "tg_cluster_plane_wave", "tg_cluster_solve", "tg_cluster_cross_sections",
     "tg_cluster_pullback", "tg_cluster_overlap_pullback", "tg_cluster_timing_enable",
     "tg_cluster_timing_get", "tg_cluster_timing_reset", "tg_cluster_set_group_size",
-    "tg_cluster_set_table_limit", "tg_cluster_near_pairs",
+    "tg_cluster_set_table_limit", "tg_cluster_set_inner_gemm", "tg_cluster_near_pairs",
     "tg_cluster_pair_list", "tg_cluster_pair_features", "tg_cluster_set_pair_output", "tg_cluster_learned_apply",
     "tg_cluster_learned_loss", "tg_cluster_set_mlp", "tg_mlp_create", "tg_mlp_destroy", "tg_mlp_param_count",
     "tg_mlp_get_params", "tg_mlp_set_params", "tg_mlp_get_grads", "tg_mlp_zero_grad", "tg_mlp_forward",
@@ -97,6 +97,7 @@ def lib():
         "tg_cluster_assemble": (_i, [_vp, _i]),
         "tg_cluster_release": (_i, [_vp]),
         "tg_cluster_set_table_limit": (_i, [_vp, _i64]),
+        "tg_cluster_set_inner_gemm": (_i, [_vp, _i]),
         "tg_cluster_apply": (_i, [_vp, _vp, _vp, _i, _i, _i]),
         "tg_cluster_coupling": (_i, [_vp, _vp, _vp, _i, _i, _i]),
         "tg_cluster_local": (_i, [_vp, _vp, _vp, _i, _i]),

@@ -130,6 +130,15 @@ class IterativeSphereCluster:
         clusters recompute every pair's translation coefficients per apply."""
         check(lib().tg_cluster_set_table_limit(self._h, int(nbytes)))
 
+    def set_inner_gemm(self, kind: str) -> None:
+        """Arithmetic of the c64-precision coupling GEMMs of the inner solves:
+        "split16" (default; scaled FP16 hi + lo pairs on FP16 MFMA) or "fp32" (c64
+        tables on FP32 MFMA)."""
+        kinds = {"fp32": 0, "split16": 1}
+        if kind not in kinds:
+            raise ValueError(f"inner GEMM kind must be one of {sorted(kinds)}, got {kind!r}")
+        check(lib().tg_cluster_set_inner_gemm(self._h, kinds[kind]))
+
     def release(self) -> None:
         """Free the stored coupling, the pair tables and the preconditioner (rebuilt on
         demand)."""

@@ -446,6 +446,9 @@ int tg_cluster_destroy(tg_cluster *c) {
     c->have_tables = false;
     c->b_tab32.release();
     c->have_tables32 = false;
+    c->b_tabh.release();
+    c->b_texp.release();
+    c->have_tablesh = false;
     c->b_tabR.release();
     c->have_tablesR = false;
     learned_release(c);
@@ -539,11 +542,30 @@ int tg_cluster_release(tg_cluster *c) {
     c->have_tables = false;
     c->b_tab32.release();
     c->have_tables32 = false;
+    c->b_tabh.release();
+    c->b_texp.release();
+    c->have_tablesh = false;
     c->b_tabR.release();
     c->have_tablesR = false;
     c->storage = TG_STORE_NONE;
     learned_release(c);
     for (auto &w : c->ws) w.release();
+    return TG_OK;
+}
+
+int tg_cluster_set_inner_gemm(tg_cluster *c, int kind) {
+    TG_TRY(check_cluster(c));
+    if (kind != TG_INNER_FP32 && kind != TG_INNER_SPLIT16) {
+        set_error("inner GEMM kind must be TG_INNER_FP32 or TG_INNER_SPLIT16, got %d", kind);
+        return TG_INVALID_INPUT;
+    }
+    TG_HIP(hipStreamSynchronize(c->stream));
+    c->inner_gemm = kind;
+    c->b_tab32.release();
+    c->have_tables32 = false;
+    c->b_tabh.release();
+    c->b_texp.release();
+    c->have_tablesh = false;
     return TG_OK;
 }
 
@@ -559,6 +581,9 @@ int tg_cluster_set_table_limit(tg_cluster *c, int64_t bytes) {
     c->have_tables = false;
     c->b_tab32.release();
     c->have_tables32 = false;
+    c->b_tabh.release();
+    c->b_texp.release();
+    c->have_tablesh = false;
     c->b_tabR.release();
     c->have_tablesR = false;
     return TG_OK;

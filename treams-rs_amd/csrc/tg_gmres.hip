@@ -163,7 +163,11 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
             }
             if (!more) break;
         }
-        // back substitution and x += M^{-1} V y
+        // back substitution and x += M^{-1} V y, over the basis vectors written in this
+        // cycle only (the rest of V holds whatever the buffer held before, and a zero
+        // coefficient times a NaN there is NaN)
+        int nuse = 0;
+        for (int k = 0; k < cols; ++k) nuse = std::max(nuse, cycle[k] ? used[k] : 0);
         for (int k = 0; k < cols; ++k) {
             zc *gk = &g[(size_t)k * (restart + 1)];
             for (int i = used[k] - 1; i >= 0; --i) {
@@ -171,14 +175,16 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
                 for (int jj = i + 1; jj < used[k]; ++jj) tail += H[((size_t)k * restart + jj) * (restart + 1) + i] * gk[jj];
                 gk[i] = (gk[i] - tail) / H[((size_t)k * restart + i) * (restart + 1) + i];
             }
-            for (int i = 0; i < restart; ++i) {
+            for (int i = 0; i < nuse; ++i) {
                 zc v = (i < used[k] && cycle[k]) ? gk[i] : zc(0.0);
-                hbuf[(size_t)k * restart + i] = cd(v.real(), v.imag());
+                hbuf[(size_t)k * nuse + i] = cd(v.real(), v.imag());
             }
         }
-        TG_HIP(hipMemcpyAsync(coef, hbuf.data(), sizeof(cd) * cols * restart, hipMemcpyHostToDevice, s));
         TG_HIP(hipMemsetAsync(X, 0, sizeof(cd) * cols * n, s));
-        TG_TRY(vec_update(s, V, n, restart, coef, X, n, cols, csv, n, 1.0));
+        if (nuse > 0) {
+            TG_HIP(hipMemcpyAsync(coef, hbuf.data(), sizeof(cd) * cols * nuse, hipMemcpyHostToDevice, s));
+            TG_TRY(vec_update(s, V, n, nuse, coef, X, n, cols, csv, n, 1.0));
+        }
         if (Minv) {
             TG_TRY((*Minv)(X, W, cols));
             TG_TRY(vec_axpby(s, cd(1.0), W, cd(1.0), x, (int64_t)cols * n));

@@ -114,6 +114,9 @@ struct tg_cluster {
     bool have_tables = false;
     tg::DevBuf b_tab32;                // the same tables rounded to c64 (FP32 MFMA coupling)
     bool have_tables32 = false;
+    tg::DevBuf b_tabh, b_texp;         // split FP16 tables and their row exponents (FP16 MFMA coupling)
+    bool have_tablesh = false;
+    int inner_gemm = TG_INNER_SPLIT16; // c64-precision GEMMs of the inner solves (tg_inner_gemm)
     tg::DevBuf b_tabR;                 // FP64 regular harmonic tables (overlap for cross sections)
     bool have_tablesR = false;
     int64_t table_limit = (int64_t)64 << 30;  // largest pair-table set kept (bytes)
@@ -143,6 +146,11 @@ int launch_translation_rect(const RectPlan &pl, double k, int64_t count, const d
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular);
 int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols);
 int coupling_stored(tg_cluster *c, const cd *x, cd *y, int cols);
+// split FP16 coupling (tg_split16.hip) from the FP64 singular tables tab64
+int ensure_split_tables(tg_cluster *c, const cd *tab64);
+int coupling_tables_split16(tg_cluster *c, const cd *tab64, const cd *x, cd *y, int cols);
+// the FP64 plan contraction of a compact c64 U for nc columns into y (tg_kernels.hip)
+int launch_tab_contract_c64(tg_cluster *c, const cf *U, int nc, cd *y);
 int local_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *subtract_from);
 int ref_to_split(tg_cluster *c, const cd *ref, cd *split, int cols);
 int split_to_ref(tg_cluster *c, const cd *split, cd *ref, int cols);

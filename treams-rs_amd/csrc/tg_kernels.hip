@@ -863,7 +863,19 @@ static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool reg
 // inner operator of mixed-precision solves without a stored coupling.
 int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols) {
     if (!tables_fit(c)) return coupling_matrix_free(c, x, y, cols, 0);
+    if (c->inner_gemm == TG_INNER_SPLIT16) {
+        TG_TRY(ensure_pair_tables(c, false, false));
+        return coupling_tables_split16(c, c->b_tab.as<cd>(), x, y, cols);
+    }
     return coupling_tables<float>(c, x, y, cols, false);
+}
+
+int launch_tab_contract_c64(tg_cluster *c, const cf *U, int nc, cd *y) {
+    const int N = c->N, H = c->H;
+    k_tab_contract<float><<<dim3((unsigned)((N + 63) / 64), (unsigned)(nc * 2 * H)), 64, 0, c->stream>>>(
+        c->plan, U, N, H, c->n, c->nh, y);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
 }
 
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular) {
@@ -1310,7 +1322,8 @@ int vec_norms(hipStream_t s, const cd *w, int64_t n, int cols, int64_t colstride
     std::vector<cd> d(cols);
     // |w|^2 as the dot of w with itself (nvec = 1, V = w)
     TG_TRY(vec_dots(s, w, 0, 1, w, n, cols, colstride, colstride, d.data(), scratch));
-    for (int i = 0; i < cols; ++i) out_host[i] = sqrt(fmax(d[i].re, 0.0));
+    // a NaN stays NaN (fmax would turn it into 0 and pass a non-finite vector as zero)
+    for (int i = 0; i < cols; ++i) out_host[i] = d[i].re > 0.0 ? sqrt(d[i].re) : d[i].re == d[i].re ? 0.0 : d[i].re;
     return TG_OK;
 }
 

@@ -1,0 +1,314 @@
+// Matrix-free coupling C x on FP16 MFMA with two-term splits: the c64-precision
+// operator of the inner solves.
+//
+// The pair-table GEMMs U_t = TAB_t X (harmonic index t, subset columns as in the FP32
+// and FP64 forms) run on v_mfma_f32_16x16x32_f16.  Every real operand value v is held
+// as hi + lo, hi = fp16(v 2^s), lo = fp16(v 2^s - hi), with a power-of-two shift s per
+// table row (t, i) and per source column q that brings the row's or column's largest
+// magnitude to [2^14, 2^15).  Each real product takes three MFMAs (hi hi, hi lo,
+// lo hi), so a value keeps 22 significant bits while it lies within 2^17 of its row's
+// or column's largest (beyond that the error is below 2^-38 of the largest); the
+// products are exact in FP32 and accumulate in FP32, as in the c64 form.  The shifts
+// are undone exactly (ldexp) when a tile is stored.
+//
+// The complex product is the 4M form (Re = Tr Xr - Ti Xi, Im = Tr Xi + Ti Xr): four
+// operand fragments per side and twelve MFMAs per 16 x 16 x 32 step, against six
+// fragments and nine MFMAs for 3M, which would also need split sums Tr + Ti.
+//
+// Layouts (K = sources j padded to Np = 32 ceil(N / 32) with zeros):
+//   tables  TABH[t][i / 16][j / 32][c][lane][8]: one MFMA operand fragment per (c,
+//           lane), lane = (i % 16) + 16 ((j % 32) / 8), element j % 8; components c =
+//           re hi, re lo, im hi, im lo.  T Np^2 8 bytes (the size of the c64 tables).
+//   shifts  row exponents E[t][i] (|TAB_t[i][j]| < 2^E), shift 15 - E.
+//   sources XH[q][c][Np] halves with the column shift cshift[q].
+#include "tg_internal.h"
+#include "tg_mfma.h"
+
+namespace tg {
+
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+
+constexpr int kNoExp = -100000;  // row or column of zeros
+
+__device__ __forceinline__ int mag_exp(double m) {
+    int e = 0;
+    (void)frexp(m, &e);
+    return m > 0.0 ? e : kNoExp;
+}
+__device__ __forceinline__ int shift_of(int e) { return e == kNoExp ? 0 : 15 - e; }
+
+// hi and lo halves of v 2^s
+struct half2s { _Float16 hi, lo; };
+__device__ __forceinline__ half2s split16(double v, int s) {
+    const double w = ldexp(v, s);
+    const _Float16 hi = (_Float16)(float)w;
+    return {hi, (_Float16)(float)(w - (double)(float)hi)};
+}
+
+__global__ void k_fill_int(int *p, int64_t count, int v) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) p[t] = v;
+}
+
+// E[t][i] = max over j in chunk blockIdx.z of the exponent of max(|re|, |im|) of
+// TAB[t][j][i]; grid (ceil(N / 256), T, chunks).
+__global__ __launch_bounds__(256) void k_split_rowexp(const cd *TAB, int N, int Np, int jchunk, int *E) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.y;
+    if (i >= N) return;
+    const int j0 = blockIdx.z * jchunk, j1 = min(N, j0 + jchunk);
+    const cd *p = TAB + ((int64_t)t * N) * N + i;
+    double m = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const cd v = p[(int64_t)j * N];
+        m = fmax(m, fmax(fabs(v.re), fabs(v.im)));
+    }
+    const int e = mag_exp(m);
+    if (e != kNoExp) atomicMax(E + (int64_t)t * Np + i, e);
+}
+
+// TABH from the FP64 tables: one thread per (t, i block, j block, lane).
+__global__ __launch_bounds__(256) void k_split_tables(const cd *TAB, const int *E, int N, int Np, int T,
+                                                      h8v *TABH) {
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int NB16 = Np / 16, NB32 = Np / 32;
+    const int64_t nfrag = (int64_t)T * NB16 * NB32;
+    const int64_t frag = gid >> 6;
+    if (frag >= nfrag) return;
+    const int lane = (int)(gid & 63);
+    const int jb = (int)(frag % NB32);
+    const int64_t rest = frag / NB32;
+    const int ib = (int)(rest % NB16), t = (int)(rest / NB16);
+    const int i = ib * 16 + (lane & 15), j0 = jb * 32 + 8 * (lane >> 4);
+    const int s = i < N ? shift_of(E[(int64_t)t * Np + i]) : 0;
+    h8v rh, rl, ih, il;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int j = j0 + e;
+        cd v(0.0);
+        if (i < N && j < N) v = TAB[((int64_t)t * N + j) * N + i];
+        const half2s pr = split16(v.re, s), pi = split16(v.im, s);
+        rh[e] = pr.hi;
+        rl[e] = pr.lo;
+        ih[e] = pi.hi;
+        il[e] = pi.lo;
+    }
+    h8v *out = TABH + frag * 256 + lane;
+    out[0] = rh;
+    out[64] = rl;
+    out[128] = ih;
+    out[192] = il;
+}
+
+// XH[q][c][j] and cshift[q] for q = (column cc, helicity h, mode a) of x (split
+// layout); one workgroup per q.
+__global__ __launch_bounds__(256) void k_split_sources(const cd *x, int64_t n, int64_t nh, int N, int Np, int H,
+                                                       _Float16 *XH, int *cshift) {
+    const int q = blockIdx.x;
+    const int cc = q / (2 * H), rem = q - cc * 2 * H, h = rem / H, a = rem - h * H;
+    const cd *xq = x + (int64_t)cc * n + (int64_t)h * nh + a;
+    double m = 0.0;
+    for (int j = threadIdx.x; j < N; j += 256) {
+        const cd v = xq[(int64_t)j * H];
+        m = fmax(m, fmax(fabs(v.re), fabs(v.im)));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    __shared__ double wm[4];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+    const int s = shift_of(mag_exp(m));
+    if (threadIdx.x == 0) cshift[q] = s;
+    _Float16 *o = XH + (int64_t)q * 4 * Np;
+    for (int j = threadIdx.x; j < Np; j += 256) {
+        const cd v = j < N ? xq[(int64_t)j * H] : cd(0.0);
+        const half2s pr = split16(v.re, s), pi = split16(v.im, s);
+        o[j] = pr.hi;
+        o[Np + j] = pr.lo;
+        o[2 * Np + j] = pi.hi;
+        o[3 * Np + j] = pi.lo;
+    }
+}
+
+__device__ __forceinline__ f32x4v mfma_h(h8v a, h8v b, f32x4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// U[g][soff[t] + r][i] = sum_j TAB_t[i][j] X[g H + smode[soff[t] + r]][j] (c64 out).
+// One 128 x 64 tile (rows i, subset columns) per workgroup; wave w owns rows
+// 32 w .. 32 w + 31 of the tile and all 64 columns (2 x 4 MFMA tiles), computed
+// transposed (columns as the MFMA's M, rows i as its N) so that lanes run along i in
+// the stores.  Operand fragments are loaded straight from global memory (the layouts
+// above are fragment-native); the table fragments of the next K step are in flight
+// during the current one.  Tile order and the XCD remap as in the FP32 form, with
+// 64-wide column tiles (tiles[t]) and 128-row tiles.
+__global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TABH, const _Float16 *XH,
+                                                                  const int *E, const int *cshift, cf *U, int N,
+                                                                  int Np, int H, int S, int G, int T_,
+                                                                  const int32_t *tiles, const int32_t *soff,
+                                                                  const int32_t *smode) {
+    const int nb = gridDim.x, bid = blockIdx.x;
+    const int xcd = bid & 7, q8 = nb >> 3, r8 = nb & 7;
+    const int L = xcd * q8 + min(xcd, r8) + (bid >> 3);
+    const int R = (N + 127) / 128;  // row tiles
+    int lo = 0, hi = T_ - 1;        // last t with tiles[t] R <= L
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tiles[mid] * R <= L) lo = mid; else hi = mid - 1;
+    }
+    const int t = lo;
+    const int nct = tiles[t + 1] - tiles[t], local = L - tiles[t] * R;
+    const int rt = local / nct, ct = local - rt * nct;
+    const int s0 = soff[t], nt = soff[t + 1] - s0, ncol = G * nt;
+    const int c0 = ct * 64, r0 = rt * 128;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int NB16 = Np / 16, NB32 = Np / 32;
+
+    // table fragments of the wave's two 16-row blocks (clamped: rows >= N are zero
+    // fragments or, past Np, a repeated valid block whose results are not stored)
+    const h8v *tp[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int ib = min((r0 + 32 * wave) / 16 + a, NB16 - 1);
+        tp[a] = TABH + ((int64_t)t * NB16 + ib) * NB32 * 256 + lane;
+    }
+    // source fragments of the four 16-column blocks (columns clamped to ncol - 1)
+    const _Float16 *xp[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int col = min(c0 + 16 * b + (lane & 15), ncol - 1);
+        const int g = col / nt, r = col - g * nt;
+        xp[b] = XH + ((int64_t)g * H + smode[s0 + r]) * 4 * Np + 8 * (lane >> 4);
+    }
+    f32x4v acc[3][2][4];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[p][a][b] = f32x4v{0.f, 0.f, 0.f, 0.f};
+
+    h8v ta[2][4], tn[2][4], xb[4][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ta[a][c] = tp[a][c * 64];
+    for (int jb = 0; jb < NB32; ++jb) {
+        if (jb + 1 < NB32) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) tn[a][c] = tp[a][(int64_t)(jb + 1) * 256 + c * 64];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                xb[b][c] = *reinterpret_cast<const h8v *>(xp[b] + (int64_t)c * Np + 32 * jb);
+        // acc0 += Xr Tr, acc1 += Xi Ti, acc2 += Xr Ti + Xi Tr (components 0, 1 = re hi,
+        // lo; 2, 3 = im hi, lo); consecutive MFMAs write different tiles
+#define TG_SPLIT_STEP(P, XC, TC)                                               \
+    _Pragma("unroll") for (int a = 0; a < 2; ++a)                              \
+        _Pragma("unroll") for (int b = 0; b < 4; ++b)                          \
+            acc[P][a][b] = mfma_h(xb[b][XC], ta[a][TC], acc[P][a][b]);
+        TG_SPLIT_STEP(0, 0, 0)
+        TG_SPLIT_STEP(1, 2, 2)
+        TG_SPLIT_STEP(2, 0, 2)
+        TG_SPLIT_STEP(2, 2, 0)
+        TG_SPLIT_STEP(0, 0, 1)
+        TG_SPLIT_STEP(1, 2, 3)
+        TG_SPLIT_STEP(2, 0, 3)
+        TG_SPLIT_STEP(2, 2, 1)
+        TG_SPLIT_STEP(0, 1, 0)
+        TG_SPLIT_STEP(1, 3, 2)
+        TG_SPLIT_STEP(2, 1, 2)
+        TG_SPLIT_STEP(2, 3, 0)
+#undef TG_SPLIT_STEP
+        if (jb + 1 < NB32) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) ta[a][c] = tn[a][c];
+        }
+    }
+    // D[m = column 4 (lane / 16) + v][n = row lane % 16]
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int i = r0 + 32 * wave + 16 * a + (lane & 15);
+        if (i >= N) continue;
+        const int rs = shift_of(E[(int64_t)t * Np + i]);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int col = c0 + 16 * b + 4 * (lane >> 4) + v;
+                if (col >= ncol) continue;
+                const int g = col / nt, r = col - g * nt;
+                const int sh = -(rs + cshift[g * H + smode[s0 + r]]);
+                U[((int64_t)g * S + s0 + r) * N + i] =
+                    cf(ldexpf(acc[0][a][b][v] - acc[1][a][b][v], sh), ldexpf(acc[2][a][b][v], sh));
+            }
+    }
+}
+
+// Split tables of the singular FP64 pair tables (built first when missing).
+int ensure_split_tables(tg_cluster *c, const cd *tab64) {
+    if (c->have_tablesh) return TG_OK;
+    const int N = c->N, T = c->plan.T, Np = (N + 31) & ~31;
+    TG_TRY(c->b_texp.ensure(sizeof(int) * (size_t)T * Np));
+    TG_TRY(c->b_tabh.ensure((size_t)T * Np * Np * 8));
+    int *E = c->b_texp.as<int>();
+    const int64_t ne = (int64_t)T * Np;
+    k_fill_int<<<(unsigned)((ne + 255) / 256), 256, 0, c->stream>>>(E, ne, kNoExp);
+    const int jchunk = 128;
+    k_split_rowexp<<<dim3((unsigned)((N + 255) / 256), (unsigned)T, (unsigned)((N + jchunk - 1) / jchunk)), 256, 0,
+                     c->stream>>>(tab64, N, Np, jchunk, E);
+    TG_HIP(hipGetLastError());
+    const int64_t threads = (int64_t)T * (Np / 16) * (Np / 32) * 64;
+    k_split_tables<<<(unsigned)((threads + 255) / 256), 256, 0, c->stream>>>(tab64, E, N, Np, T,
+                                                                             c->b_tabh.as<h8v>());
+    TG_HIP(hipGetLastError());
+    c->have_tablesh = true;
+    return TG_OK;
+}
+
+// C x for `cols` columns on the split FP16 tables; U in c64, then the FP64 plan
+// contraction of the FP32 form.
+int coupling_tables_split16(tg_cluster *c, const cd *tab64, const cd *x, cd *y, int cols) {
+    TG_TRY(ensure_split_tables(c, tab64));
+    const int N = c->N, H = c->H, T = c->plan.T, S = c->plan.S, Np = (N + 31) & ~31;
+    // right-hand sides per pass: the split sources and U of a pass within 512 MB
+    const size_t per_col = (size_t)2 * H * 4 * Np * sizeof(_Float16) + sizeof(cf) * (size_t)2 * S * N;
+    const int cmax = (int)std::min<size_t>(kGemmCols, std::max<size_t>(8, ((size_t)512 << 20) / per_col));
+    const int ncmax = std::min(cols, cmax);
+    const size_t xh_bytes = (((size_t)ncmax * 2 * H * 4 * Np * sizeof(_Float16)) + 255) & ~(size_t)255;
+    const size_t sh_bytes = (((size_t)ncmax * 2 * H * sizeof(int)) + 255) & ~(size_t)255;
+    TG_TRY(c->ws[0].ensure(xh_bytes + sh_bytes + sizeof(cf) * (size_t)ncmax * 2 * S * N));
+    char *base = c->ws[0].as<char>();
+    _Float16 *XH = reinterpret_cast<_Float16 *>(base);
+    int *cshift = reinterpret_cast<int *>(base + xh_bytes);
+    cf *U = reinterpret_cast<cf *>(base + xh_bytes + sh_bytes);
+    Timer::Rec rec;
+    TG_TRY(c->timer.begin("coupling_mf32", c->stream, rec));
+    for (int c0 = 0; c0 < cols; c0 += cmax) {
+        const int nc = std::min(cmax, cols - c0);
+        const int Q = nc * 2 * H;
+        k_split_sources<<<(unsigned)Q, 256, 0, c->stream>>>(x + (int64_t)c0 * c->n, c->n, c->nh, N, Np, H, XH,
+                                                            cshift);
+        TG_HIP(hipGetLastError());
+        const int32_t *tiles = c->plan.stiles + (size_t)(nc - 1) * (T + 1);  // 64-wide column tiles
+        const unsigned grid = (unsigned)c->gemm_ntiles[(size_t)nc - 1] * (unsigned)((N + 127) / 128);
+        Timer::Rec grec;
+        TG_TRY(c->timer.begin("gemm_split16", c->stream, grec));
+        k_zgemm_subsets_split16<<<grid, 256, 0, c->stream>>>(c->b_tabh.as<h8v>(), XH, c->b_texp.as<int>(), cshift,
+                                                             U, N, Np, H, S, 2 * nc, T, tiles, c->plan.soff,
+                                                             c->plan.smode);
+        TG_HIP(hipGetLastError());
+        TG_TRY(c->timer.end("gemm_split16", c->stream, grec));
+        TG_TRY(launch_tab_contract_c64(c, U, nc, y + (int64_t)c0 * c->n));
+    }
+    TG_TRY(c->timer.end("coupling_mf32", c->stream, rec));
+    return TG_OK;
+}
+
+}  // namespace tg
