@@ -135,6 +135,11 @@ def test_learned_preconditioned_solve_same_answer(pkg, oracle, learned):
     # mixed precision with the learned preconditioner certifies the FP64 residual too
     x2, c2 = cl.solve(a, rtol=1e-9, mixed_precision=True, preconditioner="learned", restart=50)
     assert c2[0][1] <= 1e-9 * c2[0][2]
+    # inner solves on the split-FP16 matrix-free coupling with the near blocks streamed
+    # as c64; the FP64 residual certifies, and the answer is the FP64 one
+    x3, c3 = cl.solve(a, rtol=1e-9, mixed_precision="matrix_free", preconditioner="learned", restart=50)
+    assert c3[0][1] <= 1e-9 * c3[0][2]
+    assert np.linalg.norm((x3 - x0).cpu().numpy()) <= 5e-8 * np.linalg.norm(x0.cpu().numpy())
 
 
 def test_training_reduces_error_radius(pkg, learned):

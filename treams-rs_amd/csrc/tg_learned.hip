@@ -27,6 +27,9 @@ struct LearnedState {
     int64_t npairs = 0;
     tg::DevBuf meta;    // pi[np], pj[np], row_start[N+1], col_start[N+1], col_list[np]
     tg::DevBuf blocks;  // [np][M x M] c128, reference layout, column-major
+    tg::DevBuf blocks32;  // the same rounded to c64 (forward applies of mixed-precision solves)
+    tg::DevBuf part;      // forward apply: weighted pair products [np][4][M]
+    bool have_blocks32 = false;
     tg::DevBuf feats;   // [np][64] fp32
     tg::DevBuf out16;   // [np][16] fp32 network output
     tg::DevBuf alpha;   // [np][lmax] c128
@@ -79,78 +82,77 @@ __global__ void k_alpha_from_output(int64_t np, int lmax, const float *out, cd *
     alpha[t] = cd(1.0 + 0.5 * tanh((double)re), 0.5 * tanh((double)im));
 }
 
-// y_i = x_i + T_i sum_{pairs p of i} D(alpha_p) C_p x_{j_p} (split layout).  One
-// workgroup per sphere i reads each of its pair blocks once for 4 columns at a time:
-// wave w takes the pairs row_start[i] + w, + 4, ..., lane l the block rows l + 64 k
-// (k < RS, reference mode order); the waves' partial sums meet in LDS ([4][M]).
-template <int RS>
-__global__ __launch_bounds__(256) void k_near_apply(int M, int H, int lmax, int64_t n, int64_t nh, int cols,
-                                                    const int *row_start, const int *pj, const cd *blocks,
-                                                    const cd *alpha, const cd *mie, const cd *x, cd *y) {
+__device__ __forceinline__ cd block_value(cd v) { return v; }
+__device__ __forceinline__ cd block_value(cf v) { return cd((double)v.re, (double)v.im); }
+
+// y_i = x_i + T_i sum_{pairs p of i} D(alpha_p) C_p x_{j_p} (split layout), in two
+// steps.  k_near_pair_products: one wave per near pair p (many short independent
+// streams keep enough block loads in flight), lane l the block rows l + 64 k (k < RS,
+// reference mode order), the source values x_j wave-uniform (scalar loads);
+// part[p][cc][r] = alpha_p(l_r) (C_p x_j)[r] for up to 4 columns.  k_near_finish: one
+// workgroup per sphere sums its pairs' parts in pair order (deterministic), applies
+// T_i and adds x_i.
+template <int RS, class B>
+__global__ __launch_bounds__(64) void k_near_pair_products(int M, int H, int lmax, int64_t n, int64_t nh, int nc,
+                                                           const int *__restrict__ pj, const B *__restrict__ blocks,
+                                                           const cd *__restrict__ alpha, const cd *__restrict__ x,
+                                                           cd *__restrict__ part) {
     constexpr int NC = 4;
-    extern __shared__ cd red[];
-    const int i = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int p0 = row_start[i], p1 = row_start[i + 1];
-    for (int col0 = 0; col0 < cols; col0 += NC) {
-        const int nc = min(NC, cols - col0);
-        cd acc[RS][NC];
+    const int p = blockIdx.x, lane = threadIdx.x;
+    const int j = __builtin_amdgcn_readfirstlane(pj[p]);
+    const B *blk = blocks + (int64_t)p * M * M;
+    cd sum[RS][NC];
 #pragma unroll
-        for (int k = 0; k < RS; ++k)
+    for (int k = 0; k < RS; ++k)
 #pragma unroll
-            for (int cc = 0; cc < NC; ++cc) acc[k][cc] = cd(0.0);
-        for (int p = p0 + wave; p < p1; p += 4) {
-            const int j = pj[p];
-            const cd *blk = blocks + (int64_t)p * M * M;
-            cd sum[RS][NC];
-#pragma unroll
-            for (int k = 0; k < RS; ++k)
-#pragma unroll
-                for (int cc = 0; cc < NC; ++cc) sum[k][cc] = cd(0.0);
+        for (int cc = 0; cc < NC; ++cc) sum[k][cc] = cd(0.0);
 #pragma unroll 4
-            for (int c = 0; c < M; ++c) {
-                const int64_t xo = (int64_t)(c & 1) * nh + (int64_t)j * H + (c >> 1);
-                cd xv[NC];
+    for (int c = 0; c < M; ++c) {
+        const int64_t xo = (int64_t)(c & 1) * nh + (int64_t)j * H + (c >> 1);
+        cd xv[NC];
 #pragma unroll
-                for (int cc = 0; cc < NC; ++cc) xv[cc] = cc < nc ? x[(int64_t)(col0 + cc) * n + xo] : cd(0.0);
+        for (int cc = 0; cc < NC; ++cc) xv[cc] = cc < nc ? x[(int64_t)cc * n + xo] : cd(0.0);
 #pragma unroll
-                for (int k = 0; k < RS; ++k) {
-                    const cd b = blk[(int64_t)c * M + min(lane + 64 * k, M - 1)];  // rows >= M unused
+        for (int k = 0; k < RS; ++k) {
+            const cd b = block_value(blk[(int64_t)c * M + min(lane + 64 * k, M - 1)]);  // rows >= M unused
 #pragma unroll
-                    for (int cc = 0; cc < NC; ++cc) sum[k][cc] += b * xv[cc];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < RS; ++k) {
-                const int r = lane + 64 * k;
-                if (r < M) {
-                    const cd a = alpha[(int64_t)p * lmax + row_degree(r) - 1];
-#pragma unroll
-                    for (int cc = 0; cc < NC; ++cc) acc[k][cc] += a * sum[k][cc];
-                }
-            }
+            for (int cc = 0; cc < NC; ++cc) sum[k][cc] += b * xv[cc];
         }
-        for (int w = 0; w < 4; ++w) {
-            if (wave == w) {
+    }
 #pragma unroll
-                for (int k = 0; k < RS; ++k) {
-                    const int r = lane + 64 * k;
-                    if (r < M)
+    for (int k = 0; k < RS; ++k) {
+        const int r = lane + 64 * k;
+        if (r < M) {
+            const cd a = alpha[(int64_t)p * lmax + row_degree(r) - 1];
 #pragma unroll
-                        for (int cc = 0; cc < NC; ++cc)
-                            red[cc * M + r] = (w == 0 ? cd(0.0) : red[cc * M + r]) + acc[k][cc];
-                }
-            }
-            __syncthreads();
+            for (int cc = 0; cc < NC; ++cc)
+                if (cc < nc) part[((int64_t)p * NC + cc) * M + r] = a * sum[k][cc];
         }
-        for (int t = threadIdx.x; t < nc * M; t += blockDim.x) {
-            const int cc = t / M, r = t - cc * M;
-            const int b = r >> 1, h = r & 1, l = row_degree(r);
-            const cd *t2 = mie + (size_t)(i * lmax + l - 1) * 4;
-            const cd v = t2[h * 2 + 0] * red[cc * M + 2 * b] + t2[h * 2 + 1] * red[cc * M + 2 * b + 1];
-            const int64_t idx = (int64_t)(col0 + cc) * n + (int64_t)h * nh + (int64_t)i * H + b;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_near_finish(int M, int H, int lmax, int64_t n, int64_t nh, int nc,
+                                                     const int *__restrict__ row_start, const cd *__restrict__ part,
+                                                     const cd *__restrict__ mie, const cd *__restrict__ x,
+                                                     cd *__restrict__ y) {
+    constexpr int NC = 4;
+    const int i = blockIdx.x;
+    const int p0 = row_start[i], p1 = row_start[i + 1];
+    for (int t = threadIdx.x; t < nc * (M / 2); t += blockDim.x) {
+        const int cc = t / (M / 2), b = t - cc * (M / 2);
+        cd s0(0.0), s1(0.0);  // rows 2 b (helicity 0) and 2 b + 1 (helicity 1)
+        for (int p = p0; p < p1; ++p) {
+            const cd *pp = part + ((int64_t)p * NC + cc) * M;
+            s0 += pp[2 * b];
+            s1 += pp[2 * b + 1];
+        }
+        const int l = row_degree(2 * b);
+        const cd *t2 = mie + (size_t)(i * lmax + l - 1) * 4;
+        for (int h = 0; h < 2; ++h) {
+            const cd v = t2[h * 2 + 0] * s0 + t2[h * 2 + 1] * s1;
+            const int64_t idx = (int64_t)cc * n + (int64_t)h * nh + (int64_t)i * H + b;
             y[idx] = x[idx] + v;
         }
-        __syncthreads();
     }
 }
 
@@ -286,6 +288,7 @@ int learned_pairs(tg_cluster *c, int K) {
     TG_HIP(hipMemcpy(ddisp.ptr, disp.data(), sizeof(double) * disp.size(), hipMemcpyHostToDevice));
     const int M = c->M;
     TG_TRY(s->blocks.ensure(sizeof(cd) * (size_t)np * M * M));
+    s->have_blocks32 = false;
     TG_TRY(launch_translation_blocks(c->lmax, c->plan, c->k0, np, ddisp.as<double>(), 0, s->blocks.as<cd>(),
                                      c->stream));
     TG_TRY(s->feats.ensure(sizeof(float) * (size_t)np * kFeat));
@@ -316,14 +319,24 @@ int learned_set_output(tg_cluster *c, const float *out16_dev) {
     return TG_OK;
 }
 
-int learned_near_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint) {
+__global__ void k_blocks_to_c64(const cd *in, int64_t count, cf *out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) out[t] = cf((float)in[t].re, (float)in[t].im);
+}
+
+template <class B>
+static int near_apply_forward(tg_cluster *c, const B *blocks, const cd *x, cd *y, int cols) {
     LearnedState *s = c->learned;
     const int M = c->M;
-    if (!adjoint) {
-        const size_t smem = sizeof(cd) * 4 * M;
-#define TG_NEAR(RS)                                                                                             \
-    k_near_apply<RS><<<c->N, 256, smem, c->stream>>>(M, c->H, c->lmax, c->n, c->nh, cols, s->row_start, s->pj, \
-                                                     s->blocks.as<cd>(), s->alpha.as<cd>(), c->b_mie.as<cd>(), x, y)
+    TG_TRY(s->part.ensure(sizeof(cd) * (size_t)s->npairs * 4 * M));
+    cd *part = s->part.as<cd>();
+    for (int c0 = 0; c0 < cols; c0 += 4) {
+        const int nc = std::min(4, cols - c0);
+        const cd *xc = x + (int64_t)c0 * c->n;
+#define TG_NEAR(RS)                                                                                              \
+    k_near_pair_products<RS, B><<<(unsigned)s->npairs, 64, 0, c->stream>>>(M, c->H, c->lmax, c->n, c->nh, nc,   \
+                                                                             s->pj, blocks, s->alpha.as<cd>(), xc, \
+                                                                             part)
         const int rs = (M + 63) / 64;
         if (rs <= 1) TG_NEAR(1);
         else if (rs == 2) TG_NEAR(2);
@@ -331,8 +344,32 @@ int learned_near_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint)
         else TG_NEAR(6);
 #undef TG_NEAR
         TG_HIP(hipGetLastError());
-        return TG_OK;
+        k_near_finish<<<c->N, 256, 0, c->stream>>>(M, c->H, c->lmax, c->n, c->nh, nc, s->row_start, part,
+                                                   c->b_mie.as<cd>(), xc, y + (int64_t)c0 * c->n);
+        TG_HIP(hipGetLastError());
     }
+    return TG_OK;
+}
+
+// forward apply with the near blocks rounded to c64 (half the bytes streamed; the
+// products and sums stay FP64), for the inner solves of mixed precision
+static int learned_near_apply_c64(tg_cluster *c, const cd *x, cd *y, int cols) {
+    LearnedState *s = c->learned;
+    if (!s->have_blocks32) {
+        const int64_t count = s->npairs * (int64_t)c->M * c->M;
+        TG_TRY(s->blocks32.ensure(sizeof(cf) * (size_t)count));
+        k_blocks_to_c64<<<(unsigned)((count + 255) / 256), 256, 0, c->stream>>>(s->blocks.as<cd>(), count,
+                                                                                s->blocks32.as<cf>());
+        TG_HIP(hipGetLastError());
+        s->have_blocks32 = true;
+    }
+    return near_apply_forward(c, s->blocks32.as<cf>(), x, y, cols);
+}
+
+int learned_near_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint) {
+    LearnedState *s = c->learned;
+    const int M = c->M;
+    if (!adjoint) return near_apply_forward(c, s->blocks.as<cd>(), x, y, cols);
     TG_TRY(s->work.ensure(sizeof(cd) * (size_t)cols * c->n));
     cd *z = s->work.as<cd>();
     TG_TRY(local_apply(c, x, z, cols, 1, nullptr));
@@ -356,7 +393,8 @@ int learned_setup(tg_cluster *c, Preconditioner *) {
     return TG_INVALID_INPUT;
 }
 
-int learned_apply(tg_cluster *c, Preconditioner *, const cd *x, cd *y, int cols, int adjoint) {
+int learned_apply(tg_cluster *c, Preconditioner *, const cd *x, cd *y, int cols, int adjoint, int storage) {
+    if (!adjoint && storage == TG_STORE_C64) return learned_near_apply_c64(c, x, y, cols);
     return learned_near_apply(c, x, y, cols, adjoint);
 }
 
@@ -365,7 +403,8 @@ void learned_destroy(Preconditioner *) {}
 void learned_release(tg_cluster *c) {
     if (!c->learned) return;
     LearnedState *s = c->learned;
-    for (DevBuf *b : {&s->meta, &s->blocks, &s->feats, &s->out16, &s->alpha, &s->work}) b->release();
+    for (DevBuf *b : {&s->meta, &s->blocks, &s->blocks32, &s->part, &s->feats, &s->out16, &s->alpha, &s->work})
+        b->release();
     delete s;
     c->learned = nullptr;
 }

@@ -31,6 +31,9 @@ struct Preconditioner {
     const int *d_size = nullptr, *d_moff = nullptr, *d_members = nullptr;
     // learned near-field correction (tg_learned.hip)
     void *learned = nullptr;
+    // precision the current solve asked for: TG_STORE_C64 (inner solves of mixed
+    // precision) lets the learned correction stream c64 copies of its near blocks
+    int storage = TG_STORE_C128;
 };
 
 std::vector<std::vector<int>> spatial_groups(const std::vector<double> &pos, int N, int size) {
@@ -120,7 +123,7 @@ __global__ void k_pc_scatter(const int *members, int M, int H, int64_t n, int64_
 }
 
 int learned_setup(tg_cluster *c, Preconditioner *pc);
-int learned_apply(tg_cluster *c, Preconditioner *pc, const cd *x, cd *y, int cols, int adjoint);
+int learned_apply(tg_cluster *c, Preconditioner *pc, const cd *x, cd *y, int cols, int adjoint, int storage);
 void learned_destroy(Preconditioner *pc);
 
 static int build_block_jacobi(tg_cluster *c, Preconditioner *pc) {
@@ -202,8 +205,10 @@ static int build_block_jacobi(tg_cluster *c, Preconditioner *pc) {
 }
 
 int ensure_preconditioner(tg_cluster *c, int kind, int storage) {
-    (void)storage;
-    if (c->pc && c->pc_kind == kind) return TG_OK;
+    if (c->pc && c->pc_kind == kind) {
+        c->pc->storage = storage;
+        return TG_OK;
+    }
     destroy_preconditioner(c);
     if (kind != TG_PC_BLOCK_JACOBI && kind != TG_PC_LEARNED) {
         set_error("unknown preconditioner %d", kind);
@@ -212,6 +217,7 @@ int ensure_preconditioner(tg_cluster *c, int kind, int storage) {
     Preconditioner *pc = new Preconditioner();
     pc->kind = kind;
     pc->group_size = c->group_size;
+    pc->storage = storage;
     int st = kind == TG_PC_BLOCK_JACOBI ? build_block_jacobi(c, pc) : learned_setup(c, pc);
     if (st) {
         pc->mats.release();
@@ -236,7 +242,7 @@ int apply_preconditioner(tg_cluster *c, const cd *x, cd *y, int cols, int adjoin
     Timer::Rec rec;
     TG_TRY(c->timer.begin("pc_apply", c->stream, rec));
     if (pc->kind == TG_PC_LEARNED) {
-        TG_TRY(learned_apply(c, pc, x, y, cols, adjoint));
+        TG_TRY(learned_apply(c, pc, x, y, cols, adjoint, pc->storage));
     } else {
         const int64_t ld = pc->lu.ntot;  // = N M: every sphere sits in exactly one group
         TG_TRY(pc->packed.ensure(sizeof(cd) * (size_t)ld * cols));

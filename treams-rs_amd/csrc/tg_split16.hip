@@ -138,9 +138,10 @@ __device__ __forceinline__ f32x4v mfma_h(h8v a, h8v b, f32x4v c) {
 // One 128 x 64 tile (rows i, subset columns) per workgroup; wave w owns rows
 // 32 w .. 32 w + 31 of the tile and all 64 columns (2 x 4 MFMA tiles), computed
 // transposed (columns as the MFMA's M, rows i as its N) so that lanes run along i in
-// the stores.  Operand fragments are loaded straight from global memory (the layouts
-// above are fragment-native); the table fragments of the next K step are in flight
-// during the current one.  Tile order and the XCD remap as in the FP32 form, with
+// the stores.  The table fragments are loaded straight from global memory (the layout
+// above is fragment-native), the next K step's while the current one multiplies; the
+// source tile, shared by the four waves, goes through LDS (double-buffered, one
+// barrier per K step).  Tile order and the XCD remap as in the FP32 form, with
 // 64-wide column tiles (tiles[t]) and 128-row tiles.
 __global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TABH, const _Float16 *XH,
                                                                   const int *E, const int *cshift, cf *U, int N,
@@ -172,13 +173,16 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TAB
         const int ib = min((r0 + 32 * wave) / 16 + a, NB16 - 1);
         tp[a] = TABH + ((int64_t)t * NB16 + ib) * NB32 * 256 + lane;
     }
-    // source fragments of the four 16-column blocks (columns clamped to ncol - 1)
-    const _Float16 *xp[4];
+    // source tile of a K step in LDS, double-buffered, in fragment order
+    // xs[stage][b * 4 + c][lane] (column block b, component c): wave w copies
+    // component w of the four column blocks (columns clamped to ncol - 1)
+    __shared__ h8v xs[2][16][64];
+    const _Float16 *sp[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int col = min(c0 + 16 * b + (lane & 15), ncol - 1);
+    for (int u = 0; u < 4; ++u) {
+        const int col = min(c0 + 16 * u + (lane & 15), ncol - 1);
         const int g = col / nt, r = col - g * nt;
-        xp[b] = XH + ((int64_t)g * H + smode[s0 + r]) * 4 * Np + 8 * (lane >> 4);
+        sp[u] = XH + (((int64_t)g * H + smode[s0 + r]) * 4 + wave) * Np + 8 * (lane >> 4);
     }
     f32x4v acc[3][2][4];
 #pragma unroll
@@ -188,48 +192,69 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TAB
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[p][a][b] = f32x4v{0.f, 0.f, 0.f, 0.f};
 
-    h8v ta[2][4], tn[2][4], xb[4][4];
+    h8v ta[2][4], tb[2][4], xr[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xs[0][u * 4 + wave][lane] = *reinterpret_cast<const h8v *>(sp[u]);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int c = 0; c < 4; ++c) ta[a][c] = tp[a][c * 64];
-    for (int jb = 0; jb < NB32; ++jb) {
-        if (jb + 1 < NB32) {
+    // wait for the first step's loads here: left pending into the loop, they would make
+    // every step wait for all its own prefetches before its first MFMA (vmcnt counts
+    // loads in issue order)
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    // one K step on the table fragments `cur` while the next step's go to `nxt` (the
+    // two register sets alternate, so no copy waits for the prefetch before the
+    // barrier)
+    auto step = [&](int jb, h8v (&cur)[2][4], h8v (&nxt)[2][4]) {
+        const int st = jb & 1;
+        // the next step's loads, issued on every step (the last one reloads its own
+        // block, unused) so that every path has the same loads in flight and each wait
+        // is exact; sources first: the LDS store at the end of the step then waits for
+        // them only, not for the table prefetch behind them (loads complete in order)
+        const int jn = min(jb + 1, NB32 - 1);
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+        for (int u = 0; u < 4; ++u) xr[u] = *reinterpret_cast<const h8v *>(sp[u] + 32 * jn);
 #pragma unroll
-                for (int c = 0; c < 4; ++c) tn[a][c] = tp[a][(int64_t)(jb + 1) * 256 + c * 64];
-        }
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                xb[b][c] = *reinterpret_cast<const h8v *>(xp[b] + (int64_t)c * Np + 32 * jb);
+            for (int c = 0; c < 4; ++c) nxt[a][c] = tp[a][(int64_t)jn * 256 + c * 64];
         // acc0 += Xr Tr, acc1 += Xi Ti, acc2 += Xr Ti + Xi Tr (components 0, 1 = re hi,
-        // lo; 2, 3 = im hi, lo); consecutive MFMAs write different tiles
-#define TG_SPLIT_STEP(P, XC, TC)                                               \
-    _Pragma("unroll") for (int a = 0; a < 2; ++a)                              \
-        _Pragma("unroll") for (int b = 0; b < 4; ++b)                          \
-            acc[P][a][b] = mfma_h(xb[b][XC], ta[a][TC], acc[P][a][b]);
-        TG_SPLIT_STEP(0, 0, 0)
-        TG_SPLIT_STEP(1, 2, 2)
-        TG_SPLIT_STEP(2, 0, 2)
-        TG_SPLIT_STEP(2, 2, 0)
-        TG_SPLIT_STEP(0, 0, 1)
-        TG_SPLIT_STEP(1, 2, 3)
-        TG_SPLIT_STEP(2, 0, 3)
-        TG_SPLIT_STEP(2, 2, 1)
-        TG_SPLIT_STEP(0, 1, 0)
-        TG_SPLIT_STEP(1, 3, 2)
-        TG_SPLIT_STEP(2, 1, 2)
-        TG_SPLIT_STEP(2, 3, 0)
+        // lo; 2, 3 = im hi, lo), two column blocks at a time: consecutive MFMAs go to
+        // four different accumulators, so none waits for the previous one's result
+#pragma unroll
+        for (int bp = 0; bp < 4; bp += 2) {
+            h8v xb[2][4];
+#pragma unroll
+            for (int d = 0; d < 2; ++d)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) xb[d][c] = xs[st][(bp + d) * 4 + c][lane];
+#define TG_SPLIT_STEP(P, XC, TC)                                                         \
+    _Pragma("unroll") for (int a = 0; a < 2; ++a)                                        \
+        _Pragma("unroll") for (int d = 0; d < 2; ++d)                                    \
+            acc[P][a][bp + d] = mfma_h(xb[d][XC], cur[a][TC], acc[P][a][bp + d]);
+            TG_SPLIT_STEP(0, 0, 0)
+            TG_SPLIT_STEP(1, 2, 2)
+            TG_SPLIT_STEP(2, 0, 2)
+            TG_SPLIT_STEP(0, 0, 1)
+            TG_SPLIT_STEP(1, 2, 3)
+            TG_SPLIT_STEP(2, 2, 0)
+            TG_SPLIT_STEP(0, 1, 0)
+            TG_SPLIT_STEP(1, 3, 2)
+            TG_SPLIT_STEP(2, 0, 3)
+            TG_SPLIT_STEP(2, 2, 1)
+            TG_SPLIT_STEP(2, 1, 2)
+            TG_SPLIT_STEP(2, 3, 0)
 #undef TG_SPLIT_STEP
-        if (jb + 1 < NB32) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) ta[a][c] = tn[a][c];
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xs[st ^ 1][u * 4 + wave][lane] = xr[u];
+        __syncthreads();
+    };
+    for (int jb = 0; jb < NB32; jb += 2) {
+        step(jb, ta, tb);
+        if (jb + 1 < NB32) step(jb + 1, tb, ta);
     }
     // D[m = column 4 (lane / 16) + v][n = row lane % 16]
 #pragma unroll
