@@ -295,9 +295,9 @@ def main() -> int:
     ncols = B.shape[1]
 
     def measure(operator, label):
-        """Learned-PC (or --pc) mixed-precision job on the c64 coupling held as pair
-        tables (FP32 MFMA GEMMs) or as the dense helicity-split store (streaming
-        matvec); with its rooflines."""
+        """Learned-PC (or --pc) mixed-precision job on the c64-precision coupling held
+        as pair tables (split-FP16 or FP32 MFMA GEMMs, --inner) or as the dense
+        helicity-split store (streaming matvec); with its rooflines."""
         cl.release()
         xm, cv, el, kern = run_method(cl, B, torch, mixed="matrix_free" if operator == "tables" else True,
                                       pc=args.pc, steps=args.steps, warmup=args.warmup, rtol=args.rtol,
@@ -361,9 +361,9 @@ def main() -> int:
             roof["traffic_source"] = t.get("source", f"profiles/{name}")
 
     c64_bytes = 2 * nh * nh * 8
-    # the headline: inner solves on the c64 coupling held as pair harmonic tables
-    # (FP32 MFMA GEMMs) unless --operator stored; FP64 refinement and the true-residual
-    # certificate on the FP64 tables
+    # the headline: inner solves on the c64-precision coupling held as pair harmonic
+    # tables (split-FP16 MFMA GEMMs by default) unless --operator stored; FP64
+    # refinement and the true-residual certificate on the FP64 tables
     x, conv, head, roofs = measure(args.operator, "mixed")
     elapsed = head["ms_per_step"] * 1e-3 * args.steps
     value = head["value"]
@@ -383,14 +383,19 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("c64 coupling (pair tables" if tables else "c64 coupling (dense store") +
+        "dtype": (("c64-precision coupling (pair tables as scaled FP16 hi + lo halves on FP16 MFMA"
+                   if args.inner == "split16" else "c64 coupling (pair tables on FP32 MFMA")
+                  if tables else "c64 coupling (dense store") +
                  ") with FP32 accumulate for the inner solves; c128 tables, Krylov vectors, residuals and refinement",
         "data": "synthetic",
         "config": {"workload": f"N={N} spheres, lmax={lmax} ({n} unknowns), radii U[0.15,0.35], eps U[2,6]+iU[0,0.2], "
                                f"cube side 16, gap 0.05, k0 1.3, 4 plane waves (+-z x +-helicity), "
                                f"GMRES({args.restart}) to {args.rtol:g} true FP64 relative residual",
-                   "coupling": ("c64 pair harmonic tables (N^2 (2 lmax + 1)^2 values, 1.4 GB) applied by FP32 MFMA "
-                                "GEMMs; the dense c64 store (38.65 GB, helicity-split) is measured in "
+                   "coupling": (("pair harmonic tables (N^2 (2 lmax + 1)^2 values) as scaled FP16 hi + lo "
+                                 "halves (1.4 GB) applied by FP16 MFMA GEMMs (4M form, 3 half products per real "
+                                 "product)" if args.inner == "split16" else
+                                 "c64 pair harmonic tables (N^2 (2 lmax + 1)^2 values, 1.4 GB) applied by FP32 MFMA "
+                                 "GEMMs") + "; the dense c64 store (38.65 GB, helicity-split) is measured in "
                                 "stored_dense_c64") if tables else
                                "dense c64 store (38.65 GB, helicity-split), streaming matvec",
                    "rhs": ncols, "seed": args.seed, "preconditioner": args.pc, "restart": args.restart},
