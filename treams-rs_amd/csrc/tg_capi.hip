@@ -191,6 +191,28 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp, std::vector
         for (int32_t t = hp.starts[e]; t < hp.starts[e + 1]; ++t)
             ucol[t] = soff[hp.hidx[t]] + rank[hp.hidx[t] * hp.H + e / hp.H];
     const int S = soff[T];
+    // destination-major term list for the contraction
+    std::vector<int32_t> cstarts(hp.H + 1, 0), cnsame(hp.H, 0), cucol(nterms);
+    std::vector<double> cw(nterms);
+    {
+        size_t k = 0;
+        for (int b = 0; b < hp.H; ++b) {
+            cstarts[b] = (int32_t)k;
+            for (int kind = 0; kind < 2; ++kind) {
+                for (int a = 0; a < hp.H; ++a) {
+                    const size_t e = (size_t)a * hp.H + b;
+                    const int32_t s0 = hp.starts[e], ns = hp.nsame[e], s1 = hp.starts[e + 1];
+                    for (int32_t t = kind ? s0 + ns : s0; t < (kind ? s1 : s0 + ns); ++t) {
+                        cucol[k] = ucol[t];
+                        cw[k] = hp.w[t];
+                        ++k;
+                    }
+                }
+                if (kind == 0) cnsame[b] = (int32_t)(k - cstarts[b]);
+            }
+        }
+        cstarts[hp.H] = (int32_t)k;
+    }
     std::vector<int32_t> stiles((size_t)2 * kGemmCols * (T + 1), 0);
     for (int w = 0; w < 2; ++w)
         for (int cols = 1; cols <= kGemmCols; ++cols) {
@@ -213,7 +235,11 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp, std::vector
     size_t off_smode = al(off_soff + sizeof(int32_t) * (T + 1));
     size_t off_ucol = al(off_smode + sizeof(int32_t) * std::max(S, 1));
     size_t off_stiles = al(off_ucol + sizeof(int32_t) * nterms);
-    size_t total = al(off_stiles + sizeof(int32_t) * stiles.size());
+    size_t off_cst = al(off_stiles + sizeof(int32_t) * stiles.size());
+    size_t off_cns = al(off_cst + sizeof(int32_t) * (hp.H + 1));
+    size_t off_cuc = al(off_cns + sizeof(int32_t) * hp.H);
+    size_t off_cw = al(off_cuc + sizeof(int32_t) * nterms);
+    size_t total = al(off_cw + sizeof(double) * nterms);
     TG_TRY(buf.ensure(total));
     char *base = buf.as<char>();
     std::vector<char> host(total, 0);
@@ -229,6 +255,10 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp, std::vector
     if (S) memcpy(host.data() + off_smode, smode.data(), sizeof(int32_t) * S);
     memcpy(host.data() + off_ucol, ucol.data(), sizeof(int32_t) * nterms);
     memcpy(host.data() + off_stiles, stiles.data(), sizeof(int32_t) * stiles.size());
+    memcpy(host.data() + off_cst, cstarts.data(), sizeof(int32_t) * (hp.H + 1));
+    memcpy(host.data() + off_cns, cnsame.data(), sizeof(int32_t) * hp.H);
+    memcpy(host.data() + off_cuc, cucol.data(), sizeof(int32_t) * nterms);
+    memcpy(host.data() + off_cw, cw.data(), sizeof(double) * nterms);
     TG_HIP(hipMemcpy(base, host.data(), total, hipMemcpyHostToDevice));
     dp.H = hp.H;
     dp.L2 = hp.L2;
@@ -246,6 +276,10 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp, std::vector
     dp.smode = (const int32_t *)(base + off_smode);
     dp.ucol = (const int32_t *)(base + off_ucol);
     dp.stiles = (const int32_t *)(base + off_stiles);
+    dp.cstarts = (const int32_t *)(base + off_cst);
+    dp.cnsame = (const int32_t *)(base + off_cns);
+    dp.cucol = (const int32_t *)(base + off_cuc);
+    dp.cw = (const double *)(base + off_cw);
     if (ntiles) {
         ntiles->assign((size_t)2 * kGemmCols, 0);
         for (int w = 0; w < 2; ++w)

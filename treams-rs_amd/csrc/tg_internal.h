@@ -58,6 +58,11 @@ struct DevPlan {
     // GEMM column tiles of width kTileWidths[w] before harmonic t for G = 2 cols column
     // groups: stiles[((w kGemmCols) + cols - 1) (T + 1) + t], cols = 1 .. kGemmCols
     const int32_t *stiles;
+    // the terms by destination mode b (for the contraction): cstarts[b] .. cstarts[b+1],
+    // same-kind terms first (cnsame[b] of them), each with its subset column and weight,
+    // source modes ascending
+    const int32_t *cstarts, *cnsame, *cucol;   // H + 1, H, terms
+    const double *cw;                          // terms
 };
 
 // Growable device buffer.
@@ -138,6 +143,9 @@ int launch_assemble_split(tg_cluster *c, int storage, void *out);
 int launch_assemble_ref(tg_cluster *c, int regular, cd *out);
 // A = I - T C in the split order, column-major n x n, c64 or c128 (precision: tg_storage)
 int launch_assemble_operator(tg_cluster *c, int precision, void *out);
+// singular blocks as their two same-helicity H x H parts in c64, [count][h][a][b]
+int launch_translation_blocks_helicity_c64(int lmax, const DevPlan &plan, double k, int64_t count,
+                                          const double *disp, cf *out, hipStream_t s);
 int launch_translation_blocks(int lmax, const DevPlan &plan, double k, int64_t count, const double *disp,
                               int regular, cd *out, hipStream_t s);
 // count blocks of (2 Hd) x (2 Hs), column-major, destination inner

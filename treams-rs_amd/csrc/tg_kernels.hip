@@ -231,6 +231,19 @@ struct OutBlocks {
     }
 };
 
+// The two same-helicity H x H blocks of each pair in c64, [pair][h][a][b] (destination
+// b inner; h = 0 the reference's first polarisation index): the opposite-helicity
+// entries are zero and not stored.
+struct OutBlocksHelicity {
+    float2 *out;
+    int H;
+    __device__ void store(int, int j, int a, int b, cd vp, cd vm) const {
+        float2 *blk = out + (size_t)j * 2 * H * H;
+        blk[(size_t)a * H + b] = make_float2((float)vp.re, (float)vp.im);
+        blk[(size_t)H * H + (size_t)a * H + b] = make_float2((float)vm.re, (float)vm.im);
+    }
+};
+
 // Assembly: one workgroup per (source unit, group of P destination units).  The
 // harmonic tables of the P pairs sit in LDS; each thread owns block entries and
 // reuses every plan term it loads across the P pairs, accumulating in FP64.  The
@@ -423,6 +436,20 @@ int launch_translation_blocks(int lmax, const DevPlan &pl, double k, int64_t cou
         k_assemble<1, GeoList, OutBlocks, true><<<grid, 256, smem, s>>>(pl, k, geo, out);
     else
         k_assemble<1, GeoList, OutBlocks, false><<<grid, 256, smem, s>>>(pl, k, geo, out);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+int launch_translation_blocks_helicity_c64(int lmax, const DevPlan &pl, double k, int64_t count,
+                                          const double *disp, cf *outp, hipStream_t s) {
+    const int H = lmax * (lmax + 2);
+    TG_HIP(hipMemsetAsync(outp, 0, sizeof(cf) * 2 * H * H * count, s));
+    if (count == 0) return TG_OK;
+    GeoList geo{disp};
+    OutBlocksHelicity out{(float2 *)outp, H};
+    size_t smem = assemble_smem(pl, 1);
+    TG_TRY(check_smem(smem, lmax));
+    k_assemble<1, GeoList, OutBlocksHelicity, false><<<dim3((unsigned)count, 1), 256, smem, s>>>(pl, k, geo, out);
     TG_HIP(hipGetLastError());
     return TG_OK;
 }
@@ -780,28 +807,31 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets(const cx<T> *TAB, cons
 
 // y[cc][h][i H + b] = sum_{a, t in (a, b)} w_t U[(cc, h)][ucol_t][i], same kind plus
 // or minus i times cross kind (h = 0: pol 1, +; h = 1: pol 0, -).  One thread per
-// destination sphere i (coalesced over i); the plan is read with wave-uniform
-// addresses.  grid.y = (cc, h, b).
+// destination sphere i (coalesced over i), grid.y = (cc, h, b); the terms of b are one
+// contiguous list (same kind, then cross kind; source modes ascending, as the
+// per-entry plan), read with wave-uniform (scalar) loads, so the loop is flat and
+// keeps many U loads in flight.
 template <class T>
-__global__ __launch_bounds__(64) void k_tab_contract(DevPlan pl, const cx<T> *U, int N, int H, int64_t n,
-                                                     int64_t nh, cd *y) {
+__global__ __launch_bounds__(64) void k_tab_contract(DevPlan pl, const cx<T> *__restrict__ U, int N, int H,
+                                                     int64_t n, int64_t nh, cd *__restrict__ y) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int yb = blockIdx.y;
     const int cc = yb / (2 * H), rem = yb - cc * 2 * H, h = rem / H, b = rem - h * H;
     if (i >= N) return;
     const cx<T> *Ug = U + ((int64_t)cc * 2 + h) * pl.S * N + i;
+    const int32_t *__restrict__ cu = pl.cucol;
+    const double *__restrict__ cw = pl.cw;
+    const int s0 = pl.cstarts[b], s1 = pl.cstarts[b + 1], sm = s0 + pl.cnsame[b];
     cd same(0.0), cross(0.0);
-    for (int a = 0; a < H; ++a) {
-        const int e = a * H + b;
-        const int s0 = pl.starts[e], s1 = pl.starts[e + 1], ns = pl.nsame[e];
-        for (int t = s0; t < s0 + ns; ++t) {
-            const cx<T> u = Ug[(int64_t)pl.ucol[t] * N];
-            same += pl.w[t] * cd((double)u.re, (double)u.im);
-        }
-        for (int t = s0 + ns; t < s1; ++t) {
-            const cx<T> u = Ug[(int64_t)pl.ucol[t] * N];
-            cross += pl.w[t] * cd((double)u.re, (double)u.im);
-        }
+#pragma unroll 8
+    for (int t = s0; t < sm; ++t) {
+        const cx<T> u = Ug[(int64_t)cu[t] * N];
+        same += cw[t] * cd((double)u.re, (double)u.im);
+    }
+#pragma unroll 8
+    for (int t = sm; t < s1; ++t) {
+        const cx<T> u = Ug[(int64_t)cu[t] * N];
+        cross += cw[t] * cd((double)u.re, (double)u.im);
     }
     const cd v = h == 0 ? cd(same.re - cross.im, same.im + cross.re) : cd(same.re + cross.im, same.im - cross.re);
     y[(int64_t)cc * n + (int64_t)h * nh + (int64_t)i * H + b] = v;

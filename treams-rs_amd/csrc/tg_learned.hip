@@ -26,10 +26,12 @@ struct LearnedState {
     int K = 0;
     int64_t npairs = 0;
     tg::DevBuf meta;    // pi[np], pj[np], row_start[N+1], col_start[N+1], col_list[np]
-    tg::DevBuf blocks;  // [np][M x M] c128, reference layout, column-major
-    tg::DevBuf blocks32;  // the same rounded to c64 (forward applies of mixed-precision solves)
+    tg::DevBuf disp;      // [np][3] pos_i - pos_j of the near pairs
+    tg::DevBuf blocks;    // [np][M x M] c128, reference layout, column-major (built on first use)
+    bool have_blocks = false;
+    tg::DevBuf blocksh;   // [np][2][H][H] c64 same-helicity blocks (forward applies of mixed-precision solves)
+    bool have_blocksh = false;
     tg::DevBuf part;      // forward apply: weighted pair products [np][4][M]
-    bool have_blocks32 = false;
     tg::DevBuf feats;   // [np][64] fp32
     tg::DevBuf out16;   // [np][16] fp32 network output
     tg::DevBuf alpha;   // [np][lmax] c128
@@ -127,6 +129,64 @@ __global__ __launch_bounds__(64) void k_near_pair_products(int M, int H, int lma
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc)
                 if (cc < nc) part[((int64_t)p * NC + cc) * M + r] = a * sum[k][cc];
+        }
+    }
+}
+
+// part[p][cc][r] as k_near_pair_products, from the same-helicity c64 blocks
+// [p][h][a][b]: one workgroup per pair, wave h takes the helicity-h rows, lane l the
+// destinations b = l + 64 k (reference row 2 b + h); the sum runs over the H sources a
+// of that helicity.  The source values x_j (both helicities, up to 4 columns, zero
+// past nc) are staged in LDS first, so the loop is branch-free: one block load and
+// four broadcast LDS reads per source.
+template <int RS>
+__global__ __launch_bounds__(128) void k_near_pair_products_helicity(int H, int lmax, int64_t n, int64_t nh, int nc,
+                                                                     const int *__restrict__ pj,
+                                                                     const cf *__restrict__ blocks,
+                                                                     const cd *__restrict__ alpha,
+                                                                     const cd *__restrict__ x, cd *__restrict__ part) {
+    constexpr int NC = 4;
+    extern __shared__ cd xs[];  // [h][a][NC]
+    const int p = blockIdx.x, lane = threadIdx.x & 63, M = 2 * H;
+    const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j = pj[p];
+    for (int t = threadIdx.x; t < 2 * H * NC; t += blockDim.x) {
+        const int cc = t % NC, ha = t / NC, hh = ha / H, a = ha - hh * H;
+        xs[t] = cc < nc ? x[(int64_t)cc * n + (int64_t)hh * nh + (int64_t)j * H + a] : cd(0.0);
+    }
+    __syncthreads();
+    const cf *blk = blocks + ((int64_t)p * 2 + h) * H * H;
+    const cd *xh = xs + h * H * NC;
+    int dst[RS];
+#pragma unroll
+    for (int k = 0; k < RS; ++k) dst[k] = min(lane + 64 * k, H - 1);  // destinations >= H unused
+    cd sum[RS][NC];
+#pragma unroll
+    for (int k = 0; k < RS; ++k)
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) sum[k][cc] = cd(0.0);
+#pragma unroll 4
+    for (int a = 0; a < H; ++a) {
+        cd xv[NC];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) xv[cc] = xh[a * NC + cc];
+#pragma unroll
+        for (int k = 0; k < RS; ++k) {
+            const cf bv = blk[a * H + dst[k]];
+            const cd bb((double)bv.re, (double)bv.im);
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) sum[k][cc] += bb * xv[cc];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RS; ++k) {
+        const int bd = lane + 64 * k;
+        if (bd < H) {
+            const int r = 2 * bd + h;
+            const cd al = alpha[(int64_t)p * lmax + row_degree(r) - 1];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc)
+                if (cc < nc) part[((int64_t)p * NC + cc) * M + r] = al * sum[k][cc];
         }
     }
 }
@@ -283,14 +343,11 @@ int learned_pairs(tg_cluster *c, int K) {
     std::vector<double> disp(3 * np);
     for (int64_t p = 0; p < np; ++p)
         for (int a = 0; a < 3; ++a) disp[3 * p + a] = c->positions[3 * pi[p] + a] - c->positions[3 * pj[p] + a];
-    DevBuf ddisp;
-    TG_TRY(ddisp.ensure(sizeof(double) * disp.size()));
-    TG_HIP(hipMemcpy(ddisp.ptr, disp.data(), sizeof(double) * disp.size(), hipMemcpyHostToDevice));
-    const int M = c->M;
-    TG_TRY(s->blocks.ensure(sizeof(cd) * (size_t)np * M * M));
-    s->have_blocks32 = false;
-    TG_TRY(launch_translation_blocks(c->lmax, c->plan, c->k0, np, ddisp.as<double>(), 0, s->blocks.as<cd>(),
-                                     c->stream));
+    TG_TRY(s->disp.ensure(sizeof(double) * disp.size()));
+    TG_HIP(hipMemcpy(s->disp.ptr, disp.data(), sizeof(double) * disp.size(), hipMemcpyHostToDevice));
+    // the blocks themselves are built on first use, in the form that use needs
+    s->have_blocks = false;
+    s->have_blocksh = false;
     TG_TRY(s->feats.ensure(sizeof(float) * (size_t)np * kFeat));
     k_pair_features<<<(unsigned)((np + 127) / 128), 128, 0, c->stream>>>(
         np, s->pi, s->pj, c->b_pos.as<double>(), c->b_mie.as<cd>(), c->lmax, c->k0, s->feats.as<float>());
@@ -298,7 +355,6 @@ int learned_pairs(tg_cluster *c, int K) {
     TG_TRY(s->alpha.ensure(sizeof(cd) * (size_t)np * c->lmax));
     TG_TRY(s->out16.ensure(sizeof(float) * (size_t)np * kOut));
     TG_HIP(hipStreamSynchronize(c->stream));
-    ddisp.release();
     return TG_OK;
 }
 
@@ -319,9 +375,16 @@ int learned_set_output(tg_cluster *c, const float *out16_dev) {
     return TG_OK;
 }
 
-__global__ void k_blocks_to_c64(const cd *in, int64_t count, cf *out) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < count) out[t] = cf((float)in[t].re, (float)in[t].im);
+// c128 reference-layout blocks (adjoint, training, FP64 solves)
+static int ensure_blocks128(tg_cluster *c) {
+    LearnedState *s = c->learned;
+    if (s->have_blocks) return TG_OK;
+    const int M = c->M;
+    TG_TRY(s->blocks.ensure(sizeof(cd) * (size_t)s->npairs * M * M));
+    TG_TRY(launch_translation_blocks(c->lmax, c->plan, c->k0, s->npairs, s->disp.as<double>(), 0,
+                                     s->blocks.as<cd>(), c->stream));
+    s->have_blocks = true;
+    return TG_OK;
 }
 
 template <class B>
@@ -351,24 +414,44 @@ static int near_apply_forward(tg_cluster *c, const B *blocks, const cd *x, cd *y
     return TG_OK;
 }
 
-// forward apply with the near blocks rounded to c64 (half the bytes streamed; the
-// products and sums stay FP64), for the inner solves of mixed precision
+// forward apply with the same-helicity near blocks in c64 (a quarter of the c128
+// blocks' bytes; the products and sums stay FP64), for the inner solves of mixed
+// precision
 static int learned_near_apply_c64(tg_cluster *c, const cd *x, cd *y, int cols) {
     LearnedState *s = c->learned;
-    if (!s->have_blocks32) {
-        const int64_t count = s->npairs * (int64_t)c->M * c->M;
-        TG_TRY(s->blocks32.ensure(sizeof(cf) * (size_t)count));
-        k_blocks_to_c64<<<(unsigned)((count + 255) / 256), 256, 0, c->stream>>>(s->blocks.as<cd>(), count,
-                                                                                s->blocks32.as<cf>());
-        TG_HIP(hipGetLastError());
-        s->have_blocks32 = true;
+    const int H = c->H, M = c->M;
+    if (!s->have_blocksh) {
+        TG_TRY(s->blocksh.ensure(sizeof(cf) * (size_t)s->npairs * 2 * H * H));
+        TG_TRY(launch_translation_blocks_helicity_c64(c->lmax, c->plan, c->k0, s->npairs, s->disp.as<double>(),
+                                                      s->blocksh.as<cf>(), c->stream));
+        s->have_blocksh = true;
     }
-    return near_apply_forward(c, s->blocks32.as<cf>(), x, y, cols);
+    TG_TRY(s->part.ensure(sizeof(cd) * (size_t)s->npairs * 4 * M));
+    cd *part = s->part.as<cd>();
+    for (int c0 = 0; c0 < cols; c0 += 4) {
+        const int nc = std::min(4, cols - c0);
+        const cd *xc = x + (int64_t)c0 * c->n;
+        const int rs = (H + 63) / 64;
+#define TG_NEARH(RS)                                                                                               \
+    k_near_pair_products_helicity<RS><<<(unsigned)s->npairs, 128, sizeof(cd) * 2 * H * 4, c->stream>>>(           \
+        H, c->lmax, c->n, c->nh, nc, s->pj, s->blocksh.as<cf>(), s->alpha.as<cd>(), xc, part)
+        if (rs <= 1) TG_NEARH(1);
+        else if (rs == 2) TG_NEARH(2);
+        else if (rs <= 4) TG_NEARH(4);
+        else TG_NEARH(6);
+#undef TG_NEARH
+        TG_HIP(hipGetLastError());
+        k_near_finish<<<c->N, 256, 0, c->stream>>>(M, H, c->lmax, c->n, c->nh, nc, s->row_start, part,
+                                                   c->b_mie.as<cd>(), xc, y + (int64_t)c0 * c->n);
+        TG_HIP(hipGetLastError());
+    }
+    return TG_OK;
 }
 
 int learned_near_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint) {
     LearnedState *s = c->learned;
     const int M = c->M;
+    TG_TRY(ensure_blocks128(c));
     if (!adjoint) return near_apply_forward(c, s->blocks.as<cd>(), x, y, cols);
     TG_TRY(s->work.ensure(sizeof(cd) * (size_t)cols * c->n));
     cd *z = s->work.as<cd>();
@@ -403,7 +486,8 @@ void learned_destroy(Preconditioner *) {}
 void learned_release(tg_cluster *c) {
     if (!c->learned) return;
     LearnedState *s = c->learned;
-    for (DevBuf *b : {&s->meta, &s->blocks, &s->blocks32, &s->part, &s->feats, &s->out16, &s->alpha, &s->work})
+    for (DevBuf *b : {&s->meta, &s->disp, &s->blocks, &s->blocksh, &s->part, &s->feats, &s->out16, &s->alpha,
+                      &s->work})
         b->release();
     delete s;
     c->learned = nullptr;
@@ -523,6 +607,7 @@ int tg_cluster_learned_loss(tg_cluster *c, const tg_c128 *probes, int cols, doub
         if (!dout_dev) break;
         if ((st = apply_operator(c, R, U, cols, 1, TG_OP_MATRIX_FREE_F64))) break;  // A^H R
         if ((st = local_apply(c, U, Z, cols, 1, nullptr))) break;                     // Z = T^H A^H R
+        if ((st = ensure_blocks128(c))) break;
         k_pair_products<<<(unsigned)s->npairs, ((M + 63) / 64) * 64, 0, c->stream>>>(
             s->npairs, M, c->H, c->n, c->nh, cols, s->pj, s->blocks.as<cd>(), V, P);
         if (hipGetLastError() != hipSuccess) { st = TG_DEVICE_ERROR; break; }
