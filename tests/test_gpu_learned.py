@@ -159,3 +159,35 @@ def test_training_reduces_error_radius(pkg, learned):
         learned.set_pair_output(cl, torch.zeros_like(out))
         plain.append(learned.error_radius(cl, seed=c))
     assert np.mean(trained) < np.mean(plain), (trained, plain, log)
+
+
+def _knn_reference(pos, k):
+    """k nearest other spheres by (squared distance, index), the distances rounded
+    as ((t0^2 + t1^2) + t2^2) without fused multiply-adds; neighbours listed by index."""
+    out = []
+    for i in range(len(pos)):
+        t = pos[i] - pos
+        d = (t[:, 0] * t[:, 0] + t[:, 1] * t[:, 1]) + t[:, 2] * t[:, 2]
+        j = np.arange(len(pos))
+        keep = j != i
+        order = np.lexsort((j[keep], d[keep]))[:k]
+        out.extend((i, int(x)) for x in sorted(j[keep][order]))
+    return np.array(out, np.int32)
+
+
+@pytest.mark.parametrize("case", ["random", "lattice"])
+@pytest.mark.parametrize("k", [8, 5])
+def test_near_pairs_match_brute_force(pkg, learned, case, k):
+    """Near-pair lists (k = 8 on the device, other k on the host) equal a brute-force
+    selection, including exact distance ties broken by sphere index (cubic lattice)."""
+    if case == "random":
+        radii, e, pos = pkg.random_cluster(300, 9.0, 0.05, (0.15, 0.35), None, seed=3)
+    else:
+        g = np.arange(6, dtype=float)
+        pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) * 1.0
+        radii = np.full(len(pos), 0.3)
+        e = np.full(len(pos), 4 + 0.1j)
+    cl = pkg.IterativeSphereCluster(2, 1.3, radii, e, pos)
+    npairs = learned.near_pairs(cl, k)
+    got = learned.pair_list(cl, npairs)
+    assert np.array_equal(got, _knn_reference(np.asarray(pos, float), k))

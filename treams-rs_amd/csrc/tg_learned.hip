@@ -283,31 +283,92 @@ static LearnedState *state(tg_cluster *c) {
 }
 
 // K nearest neighbours of every sphere (host), near blocks and features (device).
+// The KT nearest other spheres of every sphere i, ordered by (squared distance,
+// index) as the host selection (std::pair order): one thread per sphere keeps the
+// best KT in registers, each candidate carried down a fully unrolled insertion chain.
+// The squared distances are rounded as on the host ((0 + t0^2) + t1^2) + t2^2, no
+// fused multiply-adds, so equal distances tie the same way.
+template <int KT>
+__global__ __launch_bounds__(256) void k_nearest(int N, const double *__restrict__ pos, int *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double bd[KT];
+    int bj[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+        bd[k] = __builtin_inf();
+        bj[k] = 0x7fffffff;
+    }
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    for (int j = 0; j < N; ++j) {
+        if (j == i) continue;
+        const double t0 = __dsub_rn(xi, pos[3 * j]), t1 = __dsub_rn(yi, pos[3 * j + 1]),
+                     t2 = __dsub_rn(zi, pos[3 * j + 2]);
+        double d = __dadd_rn(__dadd_rn(__dadd_rn(0.0, __dmul_rn(t0, t0)), __dmul_rn(t1, t1)), __dmul_rn(t2, t2));
+        if (d < bd[KT - 1] || (d == bd[KT - 1] && j < bj[KT - 1])) {
+            int jj = j;
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                if (d < bd[k] || (d == bd[k] && jj < bj[k])) {
+                    const double td = bd[k];
+                    const int tj = bj[k];
+                    bd[k] = d;
+                    bj[k] = jj;
+                    d = td;
+                    jj = tj;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KT; ++k) out[(int64_t)i * KT + k] = bj[k];
+}
+
 int learned_pairs(tg_cluster *c, int K) {
     LearnedState *s = state(c);
     if (s->npairs > 0 && s->K == K) return TG_OK;
     const int N = c->N;
     K = std::min(K, N - 1);
     std::vector<int> pi, pj;
-    std::vector<std::pair<double, int>> dist(N);
-    for (int i = 0; i < N; ++i) {
-        int m = 0;
-        for (int j = 0; j < N; ++j) {
-            if (j == i) continue;
-            double d = 0;
-            for (int a = 0; a < 3; ++a) {
-                double t = c->positions[3 * i + a] - c->positions[3 * j + a];
-                d += t * t;
+    if (K == 8) {
+        // on the device (the host selection is O(N^2) and idles the GPU for ms)
+        DevBuf dnb;
+        TG_TRY(dnb.ensure(sizeof(int) * (size_t)N * 8));
+        k_nearest<8><<<(unsigned)((N + 255) / 256), 256, 0, c->stream>>>(N, c->b_pos.as<double>(), dnb.as<int>());
+        TG_HIP(hipGetLastError());
+        std::vector<int> nbs((size_t)N * 8);
+        TG_HIP(hipMemcpyAsync(nbs.data(), dnb.ptr, sizeof(int) * nbs.size(), hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+        dnb.release();
+        for (int i = 0; i < N; ++i) {
+            int *nb = nbs.data() + (size_t)i * 8;
+            std::sort(nb, nb + 8);
+            for (int t = 0; t < 8; ++t) {
+                pi.push_back(i);
+                pj.push_back(nb[t]);
             }
-            dist[m++] = {d, j};
         }
-        std::partial_sort(dist.begin(), dist.begin() + K, dist.begin() + m);
-        std::vector<int> nb;
-        for (int t = 0; t < K; ++t) nb.push_back(dist[t].second);
-        std::sort(nb.begin(), nb.end());
-        for (int j : nb) {
-            pi.push_back(i);
-            pj.push_back(j);
+    } else {
+        std::vector<std::pair<double, int>> dist(N);
+        for (int i = 0; i < N; ++i) {
+            int m = 0;
+            for (int j = 0; j < N; ++j) {
+                if (j == i) continue;
+                double d = 0;
+                for (int a = 0; a < 3; ++a) {
+                    double t = c->positions[3 * i + a] - c->positions[3 * j + a];
+                    d += t * t;
+                }
+                dist[m++] = {d, j};
+            }
+            std::partial_sort(dist.begin(), dist.begin() + K, dist.begin() + m);
+            std::vector<int> nb;
+            for (int t = 0; t < K; ++t) nb.push_back(dist[t].second);
+            std::sort(nb.begin(), nb.end());
+            for (int j : nb) {
+                pi.push_back(i);
+                pj.push_back(j);
+            }
         }
     }
     const int64_t np = (int64_t)pi.size();
