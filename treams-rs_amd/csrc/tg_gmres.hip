@@ -32,13 +32,19 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
     const int64_t csv = (int64_t)(restart + 1) * n;
     TG_TRY(c->ws[2].ensure(sizeof(cd) * (size_t)cols * csv));
     TG_TRY(c->ws[3].ensure(sizeof(cd) * (size_t)cols * n * 3));
-    TG_TRY(c->ws[5].ensure(sizeof(cd) * (size_t)cols * (restart + 2) + sizeof(double) * cols));
+    // device scalars: solution coefficients, both Gram-Schmidt passes' projections and
+    // |w|^2 of an iteration, then the column scales and the extending mask
+    const size_t nproj = (size_t)cols * (2 * (restart + 1) + 1);
+    TG_TRY(c->ws[5].ensure(sizeof(cd) * ((size_t)cols * (restart + 2) + nproj) + sizeof(double) * 2 * cols));
     cd *V = c->ws[2].as<cd>();
     cd *W = c->ws[3].as<cd>();
     cd *X = W + (size_t)cols * n;
     cd *R = X + (size_t)cols * n;
     cd *coef = c->ws[5].as<cd>();
-    double *scale = reinterpret_cast<double *>(coef + (size_t)cols * (restart + 2));
+    cd *proj = coef + (size_t)cols * (restart + 2);
+    double *scale = reinterpret_cast<double *>(proj + nproj);
+    double *mask = scale + cols;
+    std::vector<cd> proj_h(nproj);
 
     std::vector<double> bnorm(cols), tol(cols), rn(cols);
     TG_TRY(vec_norms(s, b, n, cols, n, bnorm.data(), c->ws[4]));
@@ -55,7 +61,7 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
     std::vector<zc> sn((size_t)cols * restart);
     std::vector<int> used(cols), extending(cols), cycle(cols);
     std::vector<cd> hbuf((size_t)cols * (restart + 1));
-    std::vector<double> nxt(cols), scale_h(cols);
+    std::vector<double> nxt(cols), scale_h(2 * cols);
     while (true) {
         TG_TRY(vec_norms(s, R, n, cols, n, rn.data(), c->ws[4]));
         int any = 0;
@@ -77,8 +83,11 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
         }
         if (!any) break;
         // V[k][0] = r_k / |r_k| for cycling columns, 0 otherwise
-        for (int k = 0; k < cols; ++k) scale_h[k] = cycle[k] ? 1.0 / rn[k] : 0.0;
-        TG_HIP(hipMemcpyAsync(scale, scale_h.data(), sizeof(double) * cols, hipMemcpyHostToDevice, s));
+        for (int k = 0; k < cols; ++k) {
+            scale_h[k] = cycle[k] ? 1.0 / rn[k] : 0.0;
+            scale_h[cols + k] = cycle[k] ? 1.0 : 0.0;  // the extending mask
+        }
+        TG_HIP(hipMemcpyAsync(scale, scale_h.data(), sizeof(double) * 2 * cols, hipMemcpyHostToDevice, s));
         for (int k = 0; k < cols; ++k) TG_TRY(vec_scale_cols(s, scale + k, R + (int64_t)k * n, V + k * csv, n, 1));
         std::fill(H.begin(), H.end(), zc(0.0));
         std::fill(g.begin(), g.end(), zc(0.0));
@@ -98,19 +107,30 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
             } else {
                 TG_TRY(A(X, W, cols));
             }
-            // two classical Gram-Schmidt passes against V[:, 0..j]
-            for (int pass = 0; pass < 2; ++pass) {
-                TG_TRY(vec_dots(s, V, n, j + 1, W, n, cols, csv, n, hbuf.data(), c->ws[4]));
-                for (int k = 0; k < cols; ++k)
-                    for (int i = 0; i <= j; ++i) {
-                        cd h = extending[k] ? hbuf[(size_t)k * (j + 1) + i] : cd(0.0);
-                        hbuf[(size_t)k * (j + 1) + i] = h;
-                        H[((size_t)k * restart + j) * (restart + 1) + i] += zc(h.re, h.im);
-                    }
-                TG_HIP(hipMemcpyAsync(coef, hbuf.data(), sizeof(cd) * cols * (j + 1), hipMemcpyHostToDevice, s));
-                TG_TRY(vec_update(s, V, n, j + 1, coef, W, n, cols, csv, n, -1.0));
+            // two classical Gram-Schmidt passes against V[:, 0..j], their projections
+            // (zero for columns no longer extending) kept on the device; one transfer
+            // brings both passes' projections and |w|^2 to the host
+            const size_t nj = (size_t)cols * (j + 1);
+            cd *p0 = proj, *p1 = proj + nj, *pn = proj + 2 * nj;
+            TG_TRY(vec_dots_dev(s, V, n, j + 1, W, n, cols, csv, n, p0, mask, c->ws[4]));
+            TG_TRY(vec_update(s, V, n, j + 1, p0, W, n, cols, csv, n, -1.0));
+            TG_TRY(vec_dots_dev(s, V, n, j + 1, W, n, cols, csv, n, p1, mask, c->ws[4]));
+            TG_TRY(vec_update(s, V, n, j + 1, p1, W, n, cols, csv, n, -1.0));
+            TG_TRY(vec_dots_dev(s, W, 0, 1, W, n, cols, n, n, pn, nullptr, c->ws[4]));
+            TG_HIP(hipMemcpyAsync(proj_h.data(), proj, sizeof(cd) * (2 * nj + cols), hipMemcpyDeviceToHost, s));
+            TG_HIP(hipStreamSynchronize(s));
+            for (int k = 0; k < cols; ++k)
+                for (int i = 0; i <= j; ++i) {
+                    zc &hh = H[((size_t)k * restart + j) * (restart + 1) + i];
+                    const cd a0 = proj_h[(size_t)k * (j + 1) + i], a1 = proj_h[nj + (size_t)k * (j + 1) + i];
+                    hh += zc(a0.re, a0.im);
+                    hh += zc(a1.re, a1.im);
+                }
+            for (int k = 0; k < cols; ++k) {
+                // a NaN stays NaN, as in vec_norms
+                const double q = proj_h[2 * nj + k].re;
+                nxt[k] = q > 0.0 ? sqrt(q) : q == q ? 0.0 : q;
             }
-            TG_TRY(vec_norms(s, W, n, cols, n, nxt.data(), c->ws[4]));
             int more = 0;
             for (int k = 0; k < cols; ++k) {
                 double &sc = scale_h[k];
@@ -153,7 +173,8 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
                     }
                 }
             }
-            TG_HIP(hipMemcpyAsync(scale, scale_h.data(), sizeof(double) * cols, hipMemcpyHostToDevice, s));
+            for (int k = 0; k < cols; ++k) scale_h[cols + k] = extending[k] ? 1.0 : 0.0;
+            TG_HIP(hipMemcpyAsync(scale, scale_h.data(), sizeof(double) * 2 * cols, hipMemcpyHostToDevice, s));
             ++iterations;
             if (j + 1 < restart || more) {
                 if (j + 1 < restart)

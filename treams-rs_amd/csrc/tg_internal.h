@@ -174,6 +174,10 @@ int apply_operator(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, int
 int vec_dots(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols,
              int64_t colstride_v, int64_t colstride_w, cd *out_host, DevBuf &scratch);
 int vec_norms(hipStream_t s, const cd *w, int64_t n, int cols, int64_t colstride, double *out_host, DevBuf &scratch);
+// vec_dots into device memory, out_dev[col nvec + i], zeroed for columns whose mask_dev
+// entry is 0 (mask_dev may be null); no host synchronisation
+int vec_dots_dev(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols,
+                 int64_t csv, int64_t csw, cd *out_dev, const double *mask_dev, DevBuf &scratch);
 int vec_update(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *coef_dev, cd *w, int64_t n, int cols,
                int64_t colstride_v, int64_t colstride_w, double sign);
 int vec_axpby(hipStream_t s, cd alpha, const cd *x, cd beta, cd *y, int64_t count);

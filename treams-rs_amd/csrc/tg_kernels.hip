@@ -1348,6 +1348,29 @@ int vec_dots(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *w,
     return TG_OK;
 }
 
+// out[r] = sum of partial row r, times mask[r / nvec] when a mask is given
+__global__ void k_sum_rows_masked(const cd *partial, int width, int rows, int nvec, const double *mask, cd *out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    cd s(0.0);
+    for (int b = 0; b < width; ++b) s += partial[(int64_t)r * width + b];
+    out[r] = mask ? (mask[r / nvec] != 0.0 ? s : cd(0.0)) : s;
+}
+
+int vec_dots_dev(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols,
+                 int64_t csv, int64_t csw, cd *out_dev, const double *mask_dev, DevBuf &scratch) {
+    const int nb = dot_blocks(n);
+    const int rows = cols * nvec;
+    const size_t cap = (size_t)cols * ((nvec + 63) / 64 * 64);
+    TG_TRY(scratch.ensure(sizeof(cd) * (cap * nb + cap)));
+    cd *partial = scratch.as<cd>();
+    k_dots<<<dim3(nb, cols), 256, 0, s>>>(V, vstride, nvec, w, n, csv, csw, partial);
+    TG_HIP(hipGetLastError());
+    k_sum_rows_masked<<<(rows + 63) / 64, 64, 0, s>>>(partial, nb, rows, nvec, mask_dev, out_dev);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
 int vec_norms(hipStream_t s, const cd *w, int64_t n, int cols, int64_t colstride, double *out_host, DevBuf &scratch) {
     std::vector<cd> d(cols);
     // |w|^2 as the dot of w with itself (nvec = 1, V = w)
