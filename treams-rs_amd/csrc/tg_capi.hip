@@ -483,6 +483,8 @@ int tg_cluster_destroy(tg_cluster *c) {
     c->b_tabh.release();
     c->b_texp.release();
     c->have_tablesh = false;
+    c->b_texpo.release();
+    c->have_texpo = false;
     c->b_tabR.release();
     c->have_tablesR = false;
     learned_release(c);
@@ -579,6 +581,8 @@ int tg_cluster_release(tg_cluster *c) {
     c->b_tabh.release();
     c->b_texp.release();
     c->have_tablesh = false;
+    c->b_texpo.release();
+    c->have_texpo = false;
     c->b_tabR.release();
     c->have_tablesR = false;
     c->storage = TG_STORE_NONE;
@@ -600,6 +604,8 @@ int tg_cluster_set_inner_gemm(tg_cluster *c, int kind) {
     c->b_tabh.release();
     c->b_texp.release();
     c->have_tablesh = false;
+    c->b_texpo.release();
+    c->have_texpo = false;
     return TG_OK;
 }
 
@@ -618,6 +624,8 @@ int tg_cluster_set_table_limit(tg_cluster *c, int64_t bytes) {
     c->b_tabh.release();
     c->b_texp.release();
     c->have_tablesh = false;
+    c->b_texpo.release();
+    c->have_texpo = false;
     c->b_tabR.release();
     c->have_tablesR = false;
     return TG_OK;

@@ -121,6 +121,8 @@ struct tg_cluster {
     bool have_tables32 = false;
     tg::DevBuf b_tabh, b_texp;         // split FP16 tables and their row exponents (FP16 MFMA coupling)
     bool have_tablesh = false;
+    tg::DevBuf b_texpo;                // int16 exponents of the FP64 table entries, until the split tables exist
+    bool have_texpo = false;
     int inner_gemm = TG_INNER_SPLIT16; // c64-precision GEMMs of the inner solves (tg_inner_gemm)
     tg::DevBuf b_tabR;                 // FP64 regular harmonic tables (overlap for cross sections)
     bool have_tablesR = false;

@@ -669,8 +669,16 @@ __device__ __forceinline__ void singular_table_put(int L2, double x, const doubl
     }
 }
 
+// exponent e with max(|re|, |im|) < 2^e (frexp), or -32768 for zero
+__device__ __forceinline__ int16_t entry_exponent(cd v) {
+    const double m = fmax(fabs(v.re), fabs(v.im));
+    int e = 0;
+    (void)frexp(m, &e);
+    return m > 0.0 ? (int16_t)e : (int16_t)-32768;
+}
+
 template <bool REG>
-__global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd *tab, cf *tab32) {
+__global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd *tab, cf *tab32, int16_t *expo) {
     const int L2 = pl.L2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)N * N) return;
@@ -678,9 +686,11 @@ __global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd
     const int64_t NN = (int64_t)N * N;
     const int64_t o = (int64_t)j * N + i;
     // the FP64 entry and, when asked for, its c64 rounding (the FP32 MFMA tables)
+    // and, when asked for, the entry's exponent (row maxima of the split FP16 tables)
     auto put = [&](int h, cd v) {
         stream_store(tab + (int64_t)h * NN + o, v);
         if (tab32) stream_store(tab32 + (int64_t)h * NN + o, cf((float)v.re, (float)v.im));
+        if (expo) expo[(int64_t)h * NN + o] = entry_exponent(v);
     };
     double d[3], r, u[3];
     for (int a = 0; a < 3; ++a) d[a] = pos[3 * i + a] - pos[3 * j + a];
@@ -719,16 +729,23 @@ static int ensure_pair_tables(tg_cluster *c, bool regular, bool c64) {
     const int64_t NN = (int64_t)c->N * c->N, count = NN * c->plan.T;
     if (need32) TG_TRY(c->b_tab32.ensure(sizeof(cf) * (size_t)count));
     cf *t32 = need32 ? c->b_tab32.as<cf>() : nullptr;
+    // the entries' exponents too when split FP16 tables will be built from these
+    int16_t *expo = nullptr;
+    if (!regular && !have && c->inner_gemm == TG_INNER_SPLIT16 && !c->have_tablesh) {
+        TG_TRY(c->b_texpo.ensure(sizeof(int16_t) * (size_t)count));
+        expo = c->b_texpo.as<int16_t>();
+        c->have_texpo = true;
+    }
     if (have) {
         k_tables_to_c64<<<grid1(count), 256, 0, c->stream>>>(buf.as<cd>(), count, t32);
     } else {
         TG_TRY(buf.ensure(sizeof(cd) * (size_t)count));
         if (regular)
             k_pair_tables<true><<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N,
-                                                                  buf.as<cd>(), nullptr);
+                                                                  buf.as<cd>(), nullptr, nullptr);
         else
             k_pair_tables<false><<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N,
-                                                                   buf.as<cd>(), t32);
+                                                                   buf.as<cd>(), t32, expo);
     }
     TG_HIP(hipGetLastError());
     have = true;

@@ -67,6 +67,20 @@ __global__ __launch_bounds__(256) void k_split_rowexp(const cd *TAB, int N, int 
     if (e != kNoExp) atomicMax(E + (int64_t)t * Np + i, e);
 }
 
+// The same row maxima from the int16 exponents of the table entries (written with the
+// FP64 tables): an eighth of the bytes.
+__global__ __launch_bounds__(256) void k_split_rowexp16(const int16_t *ex, int N, int Np, int jchunk, int *E) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.y;
+    if (i >= N) return;
+    const int j0 = blockIdx.z * jchunk, j1 = min(N, j0 + jchunk);
+    const int16_t *p = ex + ((int64_t)t * N) * N + i;
+    int m = -32768;
+#pragma unroll 8
+    for (int j = j0; j < j1; ++j) m = max(m, (int)p[(int64_t)j * N]);
+    if (m != -32768) atomicMax(E + (int64_t)t * Np + i, m);
+}
+
 // TABH from the FP64 tables: one thread per (t, i block, j block, lane).
 __global__ __launch_bounds__(256) void k_split_tables(const cd *TAB, const int *E, int N, int Np, int T,
                                                       h8v *TABH) {
@@ -286,14 +300,19 @@ int ensure_split_tables(tg_cluster *c, const cd *tab64) {
     const int64_t ne = (int64_t)T * Np;
     k_fill_int<<<(unsigned)((ne + 255) / 256), 256, 0, c->stream>>>(E, ne, kNoExp);
     const int jchunk = 128;
-    k_split_rowexp<<<dim3((unsigned)((N + 255) / 256), (unsigned)T, (unsigned)((N + jchunk - 1) / jchunk)), 256, 0,
-                     c->stream>>>(tab64, N, Np, jchunk, E);
+    const dim3 grid((unsigned)((N + 255) / 256), (unsigned)T, (unsigned)((N + jchunk - 1) / jchunk));
+    if (c->have_texpo)
+        k_split_rowexp16<<<grid, 256, 0, c->stream>>>(c->b_texpo.as<int16_t>(), N, Np, jchunk, E);
+    else
+        k_split_rowexp<<<grid, 256, 0, c->stream>>>(tab64, N, Np, jchunk, E);
     TG_HIP(hipGetLastError());
     const int64_t threads = (int64_t)T * (Np / 16) * (Np / 32) * 64;
     k_split_tables<<<(unsigned)((threads + 255) / 256), 256, 0, c->stream>>>(tab64, E, N, Np, T,
                                                                              c->b_tabh.as<h8v>());
     TG_HIP(hipGetLastError());
     c->have_tablesh = true;
+    c->b_texpo.release();
+    c->have_texpo = false;
     return TG_OK;
 }
 
