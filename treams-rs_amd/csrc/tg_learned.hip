@@ -284,14 +284,15 @@ static LearnedState *state(tg_cluster *c) {
 
 // K nearest neighbours of every sphere (host), near blocks and features (device).
 // The KT nearest other spheres of every sphere i, ordered by (squared distance,
-// index) as the host selection (std::pair order): one thread per sphere keeps the
-// best KT in registers, each candidate carried down a fully unrolled insertion chain.
-// The squared distances are rounded as on the host ((0 + t0^2) + t1^2) + t2^2, no
-// fused multiply-adds, so equal distances tie the same way.
+// index) as the host selection (std::pair order).  One wave per sphere: lane l keeps
+// the best KT of the candidates j = l, l + 64, ... in registers (each candidate carried
+// down a fully unrolled insertion chain), then KT rounds of a wave-wide arg-min over
+// the lanes' heads merge them.  The squared distances are rounded as on the host
+// ((0 + t0^2) + t1^2) + t2^2, no fused multiply-adds, so equal distances tie the same
+// way.
 template <int KT>
-__global__ __launch_bounds__(256) void k_nearest(int N, const double *__restrict__ pos, int *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+__global__ __launch_bounds__(64) void k_nearest(int N, const double *__restrict__ pos, int *__restrict__ out) {
+    const int i = blockIdx.x, lane = threadIdx.x;
     double bd[KT];
     int bj[KT];
 #pragma unroll
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(256) void k_nearest(int N, const double *__restrict
         bj[k] = 0x7fffffff;
     }
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-    for (int j = 0; j < N; ++j) {
+    for (int j = lane; j < N; j += 64) {
         if (j == i) continue;
         const double t0 = __dsub_rn(xi, pos[3 * j]), t1 = __dsub_rn(yi, pos[3 * j + 1]),
                      t2 = __dsub_rn(zi, pos[3 * j + 2]);
@@ -320,8 +321,28 @@ __global__ __launch_bounds__(256) void k_nearest(int N, const double *__restrict
             }
         }
     }
+    for (int k = 0; k < KT; ++k) {
+        double d = bd[0];
+        int j = bj[0];
+        for (int off = 32; off > 0; off >>= 1) {
+            const double od = __shfl_xor(d, off);
+            const int oj = __shfl_xor(j, off);
+            if (od < d || (od == d && oj < j)) {
+                d = od;
+                j = oj;
+            }
+        }
+        if (bj[0] == j) {  // this lane held the winner: drop it from its list
 #pragma unroll
-    for (int k = 0; k < KT; ++k) out[(int64_t)i * KT + k] = bj[k];
+            for (int m = 0; m + 1 < KT; ++m) {
+                bd[m] = bd[m + 1];
+                bj[m] = bj[m + 1];
+            }
+            bd[KT - 1] = __builtin_inf();
+            bj[KT - 1] = 0x7fffffff;
+        }
+        if (lane == 0) out[(int64_t)i * KT + k] = j;
+    }
 }
 
 int learned_pairs(tg_cluster *c, int K) {
@@ -334,7 +355,7 @@ int learned_pairs(tg_cluster *c, int K) {
         // on the device (the host selection is O(N^2) and idles the GPU for ms)
         DevBuf dnb;
         TG_TRY(dnb.ensure(sizeof(int) * (size_t)N * 8));
-        k_nearest<8><<<(unsigned)((N + 255) / 256), 256, 0, c->stream>>>(N, c->b_pos.as<double>(), dnb.as<int>());
+        k_nearest<8><<<(unsigned)N, 64, 0, c->stream>>>(N, c->b_pos.as<double>(), dnb.as<int>());
         TG_HIP(hipGetLastError());
         std::vector<int> nbs((size_t)N * 8);
         TG_HIP(hipMemcpyAsync(nbs.data(), dnb.ptr, sizeof(int) * nbs.size(), hipMemcpyDeviceToHost, c->stream));
