@@ -1300,33 +1300,53 @@ int plane_wave(tg_cluster *c, const double *dir, int pol, cd *out) {
 // ------------------------------------------------------------ vector ops --
 
 // partial[((col * nvec + i) * gridDim.x + block] = sum_k conj(V[col][i][k]) w[col][k]
-__global__ __launch_bounds__(256) void k_dots(const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n,
-                                               int64_t csv, int64_t csw, cd *partial) {
-    __shared__ double sre[256], sim[256];
-    const int col = blockIdx.y;
+// partial[(col nvec + i) gridDim.x + block] = sum over the block's slice of
+// conj(V[col][i]) w[col]: up to 8 vectors per pass over the slice (w read once per
+// pass), sums reduced across the wave by shuffles and across the 4 waves in LDS.
+__global__ __launch_bounds__(256) void k_dots(const cd *__restrict__ V, int64_t vstride, int nvec,
+                                               const cd *__restrict__ w, int64_t n, int64_t csv, int64_t csw,
+                                               cd *__restrict__ partial) {
+    constexpr int CH = 8;
+    __shared__ double sred[4][CH][2];
+    const int col = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const cd *wc = w + col * csw;
     const cd *Vc = V + col * csv;
     const int64_t per = (n + gridDim.x - 1) / gridDim.x;
     const int64_t k0 = blockIdx.x * per, k1 = min(n, k0 + per);
-    for (int i = 0; i < nvec; ++i) {
-        double re = 0.0, im = 0.0;
-        const cd *v = Vc + (int64_t)i * vstride;
+    for (int i0 = 0; i0 < nvec; i0 += CH) {
+        double re[CH], im[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) re[c] = im[c] = 0.0;
         for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-            cd a = v[k], b = wc[k];
-            re += a.re * b.re + a.im * b.im;
-            im += a.re * b.im - a.im * b.re;
-        }
-        sre[threadIdx.x] = re;
-        sim[threadIdx.x] = im;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) {
-                sre[threadIdx.x] += sre[threadIdx.x + s];
-                sim[threadIdx.x] += sim[threadIdx.x + s];
+            const cd bv = wc[k];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                if (i0 + c < nvec) {
+                    const cd a = Vc[(int64_t)(i0 + c) * vstride + k];
+                    re[c] += a.re * bv.re + a.im * bv.im;
+                    im[c] += a.re * bv.im - a.im * bv.re;
+                }
             }
-            __syncthreads();
         }
-        if (threadIdx.x == 0) partial[((int64_t)col * nvec + i) * gridDim.x + blockIdx.x] = cd(sre[0], sim[0]);
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            for (int off = 32; off > 0; off >>= 1) {
+                re[c] += __shfl_xor(re[c], off);
+                im[c] += __shfl_xor(im[c], off);
+            }
+        if (lane == 0)
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                sred[wave][c][0] = re[c];
+                sred[wave][c][1] = im[c];
+            }
+        __syncthreads();
+        if (threadIdx.x < CH && i0 + (int)threadIdx.x < nvec) {
+            const int c = threadIdx.x;
+            const double r = (sred[0][c][0] + sred[1][c][0]) + (sred[2][c][0] + sred[3][c][0]);
+            const double m = (sred[0][c][1] + sred[1][c][1]) + (sred[2][c][1] + sred[3][c][1]);
+            partial[((int64_t)col * nvec + i0 + c) * gridDim.x + blockIdx.x] = cd(r, m);
+        }
         __syncthreads();
     }
 }
@@ -1340,7 +1360,7 @@ __global__ void k_sum_rows(const cd *partial, int width, int rows, cd *out) {
 }
 
 static int dot_blocks(int64_t n) {
-    int64_t b = (n + 4095) / 4096;
+    int64_t b = (n + 2047) / 2048;
     if (b > 256) b = 256;
     if (b < 1) b = 1;
     return (int)b;
