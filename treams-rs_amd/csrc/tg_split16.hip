@@ -176,6 +176,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TAB
     const int rt = local / nct, ct = local - rt * nct;
     const int s0 = soff[t], nt = soff[t + 1] - s0, ncol = G * nt;
     const int c0 = ct * 64, r0 = rt * 128;
+    const bool half = c0 + 32 >= ncol;  // uniform: only the first 32 columns carry work
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int NB16 = Np / 16, NB32 = Np / 32;
 
@@ -239,6 +240,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TAB
         // four different accumulators, so none waits for the previous one's result
 #pragma unroll
         for (int bp = 0; bp < 4; bp += 2) {
+            if (bp == 2 && half) break;  // the tile's last 32 columns are all padding
             h8v xb[2][4];
 #pragma unroll
             for (int d = 0; d < 2; ++d)
