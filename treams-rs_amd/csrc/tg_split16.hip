@@ -149,15 +149,15 @@ __device__ __forceinline__ f32x4v mfma_h(h8v a, h8v b, f32x4v c) {
 }
 
 // U[g][soff[t] + r][i] = sum_j TAB_t[i][j] X[g H + smode[soff[t] + r]][j] (c64 out).
-// One 128 x 64 tile (rows i, subset columns) per workgroup; wave w owns rows
-// 32 w .. 32 w + 31 of the tile and all 64 columns (2 x 4 MFMA tiles), computed
+// One 256 x 64 tile (rows i, subset columns) per workgroup of 8 waves; wave w owns
+// rows 32 w .. 32 w + 31 of the tile and all 64 columns (2 x 4 MFMA tiles), computed
 // transposed (columns as the MFMA's M, rows i as its N) so that lanes run along i in
 // the stores.  The table fragments are loaded straight from global memory (the layout
 // above is fragment-native), the next K step's while the current one multiplies; the
-// source tile, shared by the four waves, goes through LDS (double-buffered, one
+// source tile, shared by the eight waves, goes through LDS (double-buffered, one
 // barrier per K step).  Tile order and the XCD remap as in the FP32 form, with
-// 64-wide column tiles (tiles[t]) and 128-row tiles.
-__global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TABH, const _Float16 *XH,
+// 64-wide column tiles (tiles[t]) and 256-row tiles.
+__global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TABH, const _Float16 *XH,
                                                                   const int *E, const int *cshift, cf *U, int N,
                                                                   int Np, int H, int S, int G, int T_,
                                                                   const int32_t *tiles, const int32_t *soff,
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TAB
     const int nb = gridDim.x, bid = blockIdx.x;
     const int xcd = bid & 7, q8 = nb >> 3, r8 = nb & 7;
     const int L = xcd * q8 + min(xcd, r8) + (bid >> 3);
-    const int R = (N + 127) / 128;  // row tiles
+    const int R = (N + 255) / 256;  // row tiles
     int lo = 0, hi = T_ - 1;        // last t with tiles[t] R <= L
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TAB
     const int nct = tiles[t + 1] - tiles[t], local = L - tiles[t] * R;
     const int rt = local / nct, ct = local - rt * nct;
     const int s0 = soff[t], nt = soff[t + 1] - s0, ncol = G * nt;
-    const int c0 = ct * 64, r0 = rt * 128;
+    const int c0 = ct * 64, r0 = rt * 256;
     const bool half = c0 + 32 >= ncol;  // uniform: only the first 32 columns carry work
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int NB16 = Np / 16, NB32 = Np / 32;
@@ -192,12 +192,14 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TAB
     // xs[stage][b * 4 + c][lane] (column block b, component c): wave w copies
     // component w of the four column blocks (columns clamped to ncol - 1)
     __shared__ h8v xs[2][16][64];
-    const _Float16 *sp[4];
+    // wave w copies component w % 4 of column blocks 2 (w / 4) and 2 (w / 4) + 1
+    const _Float16 *sp[2];
+    const int xc = wave & 3, xb0 = 2 * (wave >> 2);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int col = min(c0 + 16 * u + (lane & 15), ncol - 1);
+    for (int u = 0; u < 2; ++u) {
+        const int col = min(c0 + 16 * (xb0 + u) + (lane & 15), ncol - 1);
         const int g = col / nt, r = col - g * nt;
-        sp[u] = XH + (((int64_t)g * H + smode[s0 + r]) * 4 + wave) * Np + 8 * (lane >> 4);
+        sp[u] = XH + (((int64_t)g * H + smode[s0 + r]) * 4 + xc) * Np + 8 * (lane >> 4);
     }
     f32x4v acc[3][2][4];
 #pragma unroll
@@ -207,9 +209,9 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TAB
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[p][a][b] = f32x4v{0.f, 0.f, 0.f, 0.f};
 
-    h8v ta[2][4], tb[2][4], xr[4];
+    h8v ta[2][4], tb[2][4], xr[2];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) xs[0][u * 4 + wave][lane] = *reinterpret_cast<const h8v *>(sp[u]);
+    for (int u = 0; u < 2; ++u) xs[0][(xb0 + u) * 4 + xc][lane] = *reinterpret_cast<const h8v *>(sp[u]);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -230,7 +232,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TAB
         // them only, not for the table prefetch behind them (loads complete in order)
         const int jn = min(jb + 1, NB32 - 1);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xr[u] = *reinterpret_cast<const h8v *>(sp[u] + 32 * jn);
+        for (int u = 0; u < 2; ++u) xr[u] = *reinterpret_cast<const h8v *>(sp[u] + 32 * jn);
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -265,7 +267,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets_split16(const h8v *TAB
 #undef TG_SPLIT_STEP
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xs[st ^ 1][u * 4 + wave][lane] = xr[u];
+        for (int u = 0; u < 2; ++u) xs[st ^ 1][(xb0 + u) * 4 + xc][lane] = xr[u];
         __syncthreads();
     };
     for (int jb = 0; jb < NB32; jb += 2) {
@@ -343,10 +345,10 @@ int coupling_tables_split16(tg_cluster *c, const cd *tab64, const cd *x, cd *y, 
                                                             cshift);
         TG_HIP(hipGetLastError());
         const int32_t *tiles = c->plan.stiles + (size_t)(nc - 1) * (T + 1);  // 64-wide column tiles
-        const unsigned grid = (unsigned)c->gemm_ntiles[(size_t)nc - 1] * (unsigned)((N + 127) / 128);
+        const unsigned grid = (unsigned)c->gemm_ntiles[(size_t)nc - 1] * (unsigned)((N + 255) / 256);
         Timer::Rec grec;
         TG_TRY(c->timer.begin("gemm_split16", c->stream, grec));
-        k_zgemm_subsets_split16<<<grid, 256, 0, c->stream>>>(c->b_tabh.as<h8v>(), XH, c->b_texp.as<int>(), cshift,
+        k_zgemm_subsets_split16<<<grid, 512, 0, c->stream>>>(c->b_tabh.as<h8v>(), XH, c->b_texp.as<int>(), cshift,
                                                              U, N, Np, H, S, 2 * nc, T, tiles, c->plan.soff,
                                                              c->plan.smode);
         TG_HIP(hipGetLastError());
