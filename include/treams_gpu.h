@@ -113,6 +113,20 @@ double tg_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3);
  */
 int tg_cluster_create(int lmax, double k0, int n_spheres, const double *radii, const tg_c128 *epsilon,
                       const double *positions, void *stream, tg_cluster **out);
+/*
+ * Create a particle cluster: arbitrary local T-matrices at distinct positions coupled
+ * through the singular expansion, replacing particle_cluster with spherical bases in
+ * the helicity basis (cluster/particles.rs:65-123; python/treams_rs/diff.py:1473).
+ * tmatrices: device buffer of n_particles column-major M x M blocks, M = 2 lmax
+ * (lmax + 2), in the reference mode order (tg_modes_per_sphere).  Particles with
+ * smaller bases or mode subsets are embedded with zero rows and columns (their modes
+ * solve to zero; the other modes solve the reduced system).  One real wavenumber k0
+ * for both helicities.  Solves, factors, the interacting and global T-matrices and
+ * cross sections work as for spheres; the preconditioners and the pullbacks to radii
+ * and permittivities need spheres and return TG_INVALID_INPUT.
+ */
+int tg_cluster_create_particles(int lmax, double k0, int n_particles, const double *positions,
+                                const tg_c128 *tmatrices, void *stream, tg_cluster **out);
 int tg_cluster_destroy(tg_cluster *cluster);
 /* Unknowns n = N M. */
 int64_t tg_cluster_dimension(const tg_cluster *cluster);

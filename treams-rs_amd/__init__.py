@@ -14,6 +14,8 @@ from .cluster import (  # noqa: F401
     IterativeContext,
     IterativeSphereCluster,
     SphereClusterFactor,
+    mode_index,
+    particle_cluster,
     plan_terms,
     plane_wave_field,
     scattering_gradient,

@@ -98,6 +98,33 @@ class IterativeSphereCluster:
             self._destroy(h)
             self._h = None
 
+    @classmethod
+    def from_tmatrices(cls, lmax: int, k0: float, positions, tmatrices) -> "IterativeSphereCluster":
+        """Particle cluster: arbitrary local T-matrices (N, M, M) in the reference mode
+        order at distinct positions (cluster/particles.rs:65-85).  Solves, factors,
+        the interacting and global T-matrices and cross sections work as for spheres;
+        the preconditioners and the radius / permittivity pullbacks need spheres."""
+        pos = np.ascontiguousarray(np.asarray(positions, np.float64).reshape(-1, 3))
+        N, M = pos.shape[0], int(lib().tg_modes_per_sphere(int(lmax)))
+        t = torch.as_tensor(np.asarray(tmatrices) if not torch.is_tensor(tmatrices) else tmatrices)
+        t = t.to(_device(), torch.complex128).reshape(N, M, M)
+        tcol = t.transpose(1, 2).contiguous()  # column-major blocks
+        self = cls.__new__(cls)
+        self.lmax, self.k0 = int(lmax), float(k0)
+        self.radii = self.epsilon = None
+        self.positions = pos.copy()
+        self._dev = _device()
+        h = ctypes.c_void_p()
+        check(lib().tg_cluster_create_particles(self.lmax, self.k0, N, pos.ctypes.data, tcol.data_ptr(),
+                                                ctypes.c_void_p(_stream_ptr()), ctypes.byref(h)))
+        self._h = h
+        self._destroy = lib().tg_cluster_destroy
+        self.N = N
+        self.modes_per_sphere = M
+        self.n = int(lib().tg_cluster_dimension(h))
+        self.tmatrices = t
+        return self
+
     # ---------------------------------------------------------------- queries
     @property
     def dimension(self) -> int:
@@ -396,6 +423,58 @@ class FactorContext:
                                        inc.data_ptr(), conv))
         reports = [(v.refinements, v.residual_norm, v.rhs_norm) for v in conv]
         return gk.value, gr, ge, gp.reshape(c.N, 3), _back(inc, vector), reports
+
+
+def mode_index(l: int, m: int, pol: int) -> int:
+    """Index of mode (l, m, pol) in one particle's block of the reference mode order
+    (l = 1.., m = -l..l, polarisation 1 before 0; sw/mod.rs:114-125)."""
+    return 2 * (l * l - 1 + m + l) + (1 - pol)
+
+
+def particle_cluster(local, positions, k0: float, *, lmax=None, modes=None, precision: str = "c128",
+                     rtol: float = 1e-12):
+    """Coupled T-matrix (I - T C)^{-1} T of particles with arbitrary local T-matrices,
+    over the concatenated particle bases (python/treams_rs/diff.py:1473-1523,
+    cluster/particles.rs:65-85; spherical waves, helicity basis, one real wavenumber).
+
+    local: one (n_i, n_i) T-matrix per particle; its basis is either the full basis
+    up to lmax[i] (reference mode order) or the mode list modes[i] of (l, m, pol).
+    The particles are embedded in the basis up to the largest degree with zero rows and
+    columns for modes outside their own basis (those solve to zero, so the result is
+    the reduced system's); the dense LU of the embedded system (refined in FP64 to
+    rtol) gives the columns of the own modes."""
+    pos = np.asarray(positions, np.float64).reshape(-1, 3)
+    N = pos.shape[0]
+    if len(local) != N or N == 0:
+        raise ValueError("one local matrix and position required per particle")
+    if (lmax is None) == (modes is None):
+        raise ValueError("give either lmax (one per particle) or modes (one list per particle)")
+    if modes is None:
+        lm = np.broadcast_to(np.asarray(lmax, int), (N,))
+        modes = []
+        for li in lm:
+            modes.append([(l, m, p) for l in range(1, int(li) + 1) for m in range(-l, l + 1) for p in (1, 0)])
+    L = max(l for ms in modes for (l, _, _) in ms)
+    M = int(lib().tg_modes_per_sphere(L))
+    full = np.zeros((N, M, M), np.complex128)
+    keep = []
+    for i, (t, ms) in enumerate(zip(local, modes)):
+        t = np.asarray(t, np.complex128)
+        idx = np.array([mode_index(l, m, p) for (l, m, p) in ms], int)
+        if t.shape != (len(idx), len(idx)) or len(set(idx.tolist())) != len(idx):
+            raise ValueError("local matrix shape must match its basis, modes distinct")
+        full[i][np.ix_(idx, idx)] = t
+        keep.extend((i * M + idx).tolist())
+    keep = np.asarray(keep, int)
+    cl = IterativeSphereCluster.from_tmatrices(L, k0, pos, full)
+    f = SphereClusterFactor(cl, precision, rtol)
+    try:
+        inc = torch.zeros((cl.n, keep.size), dtype=torch.complex128, device=cl._dev)
+        inc[torch.as_tensor(keep, device=cl._dev), torch.arange(keep.size, device=cl._dev)] = 1.0
+        x, _ = f.solve(inc)
+        return x[torch.as_tensor(keep, device=cl._dev)]
+    finally:
+        f.close()
 
 
 def sphere_cluster_factor(lmax: int, k0: float, radii, epsilon, positions, precision: str = "c128",

@@ -262,6 +262,7 @@ namespace tg {
 int pullback_with(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent, int cols,
                   const AdjointSolve &adjoint_solve, double *grad_k0, double *grad_radii, tg_c128 *grad_eps,
                   double *grad_positions, tg_c128 *incident_grad, tg_convergence *conv_host) {
+    TG_TRY(require_spheres(c, "the pullback to radii, permittivities and positions"));
     if (!c || !incident || !x || !cotangent || cols < 1) {
         set_error("null argument or no columns");
         return TG_INVALID_INPUT;

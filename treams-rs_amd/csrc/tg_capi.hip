@@ -487,6 +487,7 @@ int tg_cluster_destroy(tg_cluster *c) {
     c->have_texpo = false;
     c->b_tabR.release();
     c->have_tablesR = false;
+    c->b_tloc.release();
     learned_release(c);
     for (auto &w : c->ws) w.release();
     c->timer.collect();
@@ -499,6 +500,7 @@ int64_t tg_cluster_dimension(const tg_cluster *c) { return c ? c->n : 0; }
 
 int tg_cluster_mie_blocks(tg_cluster *c, tg_c128 *out) {
     TG_TRY(check_cluster(c));
+    TG_TRY(require_spheres(c, "the Mie blocks"));
     TG_HIP(hipMemcpyAsync(out, c->b_mie.ptr, sizeof(cd) * 4 * c->N * c->lmax, hipMemcpyDeviceToHost, c->stream));
     TG_HIP(hipStreamSynchronize(c->stream));
     return TG_OK;

@@ -124,6 +124,11 @@ struct tg_cluster {
     tg::DevBuf b_texpo;                // int16 exponents of the FP64 table entries, until the split tables exist
     bool have_texpo = false;
     int inner_gemm = TG_INNER_SPLIT16; // c64-precision GEMMs of the inner solves (tg_inner_gemm)
+    // particle clusters: arbitrary per-particle T-matrices in place of the Mie blocks,
+    // b_tloc[p][h'][a][h][b] = T_p[(h, b)][(h', a)] (per (p, h') a 2H x H column-major
+    // panel; h = 0 the reference's polarisation 1)
+    bool particles = false;
+    tg::DevBuf b_tloc;
     tg::DevBuf b_tabR;                 // FP64 regular harmonic tables (overlap for cross sections)
     bool have_tablesR = false;
     int64_t table_limit = (int64_t)64 << 30;  // largest pair-table set kept (bytes)
@@ -145,6 +150,12 @@ int launch_assemble_split(tg_cluster *c, int storage, void *out);
 int launch_assemble_ref(tg_cluster *c, int regular, cd *out);
 // A = I - T C in the split order, column-major n x n, c64 or c128 (precision: tg_storage)
 int launch_assemble_operator(tg_cluster *c, int precision, void *out);
+// the same for a particle cluster (arbitrary T blocks): C assembled in the split order,
+// then A = I - T C block row by block row on MFMA (tg_particles.hip)
+int assemble_operator_particles(tg_cluster *c, int precision, void *out);
+int local_apply_particles(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *subtract_from);
+// error for operations that need the spheres' Mie blocks (radii, permittivities)
+int require_spheres(const tg_cluster *c, const char *what);
 // singular blocks as their two same-helicity H x H parts in c64, [count][h][a][b]
 int launch_translation_blocks_helicity_c64(int lmax, const DevPlan &plan, double k, int64_t count,
                                           const double *disp, cf *out, hipStream_t s);

@@ -380,6 +380,7 @@ int launch_assemble_split(tg_cluster *c, int storage, void *outp) {
 }
 
 int launch_assemble_operator(tg_cluster *c, int precision, void *outp) {
+    if (c->particles) return assemble_operator_particles(c, precision, outp);
     constexpr int P = 8;
     dim3 grid(c->N, (c->N + P - 1) / P);
     GeoCluster geo{c->b_pos.as<double>(), c->N};
@@ -1156,6 +1157,7 @@ __global__ void k_local(const cd *mie, int lmax, int N, int H, int64_t n, int64_
 }
 
 int local_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *sub) {
+    if (c->particles) return local_apply_particles(c, x, y, cols, adjoint, sub);
     int64_t count = (int64_t)cols * c->nh;
     k_local<<<grid1(count), 256, 0, c->stream>>>(c->b_mie.as<cd>(), c->lmax, c->N, c->H, c->n, c->nh, cols, x, y,
                                                  adjoint, sub);

@@ -346,6 +346,7 @@ __global__ __launch_bounds__(64) void k_nearest(int N, const double *__restrict_
 }
 
 int learned_pairs(tg_cluster *c, int K) {
+    TG_TRY(require_spheres(c, "the learned preconditioner"));
     LearnedState *s = state(c);
     if (s->npairs > 0 && s->K == K) return TG_OK;
     const int N = c->N;

@@ -205,6 +205,7 @@ static int build_block_jacobi(tg_cluster *c, Preconditioner *pc) {
 }
 
 int ensure_preconditioner(tg_cluster *c, int kind, int storage) {
+    TG_TRY(require_spheres(c, "the block-Jacobi and learned preconditioners"));
     if (c->pc && c->pc_kind == kind) {
         c->pc->storage = storage;
         return TG_OK;
