@@ -206,6 +206,67 @@ def interacting_tmatrix(pkg, torch):
     return out
 
 
+def illumination_case(n):
+    """scripts/benchmark_illumination.py:55-86 ("weak"): spheres on a cube grid of
+    spacing 1.3, radii linspace(0.12, 0.18), eps 2.2+0.02i, k0 1.3, one plane wave of
+    direction (0.13, -0.11, 1) / norm and polarisation 0."""
+    side = int(np.ceil(n ** (1 / 3)))
+    grid = np.column_stack(np.unravel_index(np.arange(n), (side, side, side)))
+    direction = np.array([0.13, -0.11, 1.0])
+    return np.linspace(0.12, 0.18, n), np.full(n, 2.2 + 0.02j), grid * 1.3, direction / np.linalg.norm(direction)
+
+
+def reference_illumination(pkg, torch):
+    """The reference's requested-illumination benchmark (docs/performance/
+    large-problems.md:27-50): lmax 1, one incident column, GMRES tolerance 1e-10
+    (restart 30).  "Fresh" = cluster set-up (Mie, coupling or tables, factor) plus the
+    solve; "pullback" = gradients with respect to k0, radii, permittivities, positions
+    and the incident field from a recorded solve.  Here: c128 LU ("selected LU": the
+    requested column; "full T": all n columns, then X a), or FP64 matrix-free GMRES.
+    Published 4-thread CPU seconds beside (9950X)."""
+    published = {(512, "full T"): (0.940, 1.885), (512, "selected LU"): (0.381, 0.764),
+                 (512, "matrix-free"): (0.544, 0.711), (1024, "matrix-free"): (2.192, 2.826)}
+    out = []
+    for (n, method), (ref_fresh, ref_pull) in published.items():
+        radii, eps, pos, direction = illumination_case(n)
+        cl0 = pkg.IterativeSphereCluster(1, 1.3, radii, eps, pos)
+        a = cl0.plane_wave(direction, 0)  # input construction, excluded as in the reference
+        del cl0
+        fresh, pull = [], []
+        for rep in range(3):  # the first is the warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            cl = pkg.IterativeSphereCluster(1, 1.3, radii, eps, pos)
+            if method == "matrix-free":
+                x, ctx, conv = cl.record(a, rtol=1e-10, restart=30)
+            else:
+                f = pkg.SphereClusterFactor(cl, "c128", rtol=1e-10)
+                if method == "full T":
+                    X, _ = f.solve(torch.eye(cl.n, dtype=torch.complex128, device="cuda"))
+                    x = X @ a
+                x2, ctx, conv = f.record(a)
+                if method == "selected LU":
+                    x = x2
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            ctx.pullback(torch.ones_like(x))
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if rep:
+                fresh.append(t1 - t0)
+                pull.append(t2 - t1)
+            if method != "matrix-free":
+                f.close()
+            del cl
+        fs, ps = min(fresh), min(pull)
+        out.append({"spheres": n, "lmax": 1, "method": method, "fresh_s": fs, "pullback_s": ps,
+                    "reference_cpu_4_threads_fresh_s": ref_fresh, "reference_cpu_4_threads_pullback_s": ref_pull,
+                    "speedup_fresh": ref_fresh / fs, "speedup_pullback": ref_pull / ps})
+        log(f"illumination N={n} lmax=1 {method}: fresh {1e3 * fs:.2f} ms, pullback {1e3 * ps:.2f} ms "
+            f"(reference CPU, 4 threads: {ref_fresh} s, {ref_pull} s)")
+    return out
+
+
 def gradient_lu(pkg, torch, seed):
     """sigma_sca of a +z positive-helicity plane wave and its gradient with respect to
     all 768 position components and 256 radii (N = 256, lmax = 4, 12,288 unknowns),
@@ -485,6 +546,7 @@ def main() -> int:
         del cl
         result["dense_lu"] = dense_lu(pkg, torch, args.seed)
         result["interacting_tmatrix"] = interacting_tmatrix(pkg, torch)
+        result["reference_illumination"] = reference_illumination(pkg, torch)
         result["gradient_lu"] = gradient_lu(pkg, torch, args.seed)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
