@@ -70,13 +70,31 @@ __device__ __forceinline__ cx<T> read_lane(cx<T> x, int lane) {
     return cx<T>(read_lane(x.re, lane), read_lane(x.im, lane));
 }
 
+// Largest value over the wave (wave-uniform): DPP steps inside each row of 16 lanes,
+// then the four row maxima by readlane.  Every lane of the wave must be active.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    auto mx = [](uint32_t a, uint32_t b) { return a > b ? a : b; };
+    v = mx(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v = mx(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v = mx(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = mx(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
+    const uint32_t a = __builtin_amdgcn_readlane(v, 15), b = __builtin_amdgcn_readlane(v, 31);
+    const uint32_t c = __builtin_amdgcn_readlane(v, 47), d = __builtin_amdgcn_readlane(v, 63);
+    return mx(mx(a, b), mx(c, d));
+}
+
 // Factor the nb x nb diagonal block at k0 with partial pivoting inside the block;
 // piv[k0 + k] = row exchanged with row k0 + k.  Four waves: lane r holds row r, wave w
-// its columns 16 w .. 16 w + 15 in registers.  Step k: the wave owning column k finds
-// the pivot (wave argmax) and the multipliers and publishes them in LDS (double
-// buffered: one barrier per step); every wave then exchanges rows k and p and updates
-// its columns right of k, reading the pivot row with readlane.  Each step is a
-// latency chain (argmax, division, barrier): about 2 us whatever the layout.
+// its columns 16 w .. 16 w + 15 in registers.  Rows never move during the
+// factorisation (implicit pivoting): ord = the step at which a row became the pivot
+// (-1 before), and each row lands at position ord when the block is stored, which is
+// the arrangement of the equivalent exchange sequence piv (tracked in cur = the row at
+// each position).  Step k: the wave owning column k finds the pivot among the rows
+// not yet pivoted (one 32-bit wave max over keys = the high word of |a|^2 with the
+// low 6 bits replaced by 63 - row: magnitudes within 2^-14 of the largest tie, and
+// the lowest row wins) and the multipliers, and publishes them in LDS (double
+// buffered: one barrier per step); every wave then updates its columns right of k
+// on the rows not yet pivoted, reading the pivot row with readlane.
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0, int *piv0, int *status) {
     const int z = blockIdx.z;
@@ -96,6 +114,8 @@ __global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0
         const cx<T> e = A[(k0 + min(c, nb - 1)) * n + row];
         v[j] = (lane < nb && c < nb) ? e : cx<T>();
     }
+    int ord = lane < nb ? -1 : NB;  // rows beyond the block never pivot
+    int cur = lane;
     for (int ko = 0; ko < 4 && 16 * ko < nb; ++ko) {
 #pragma unroll
         for (int kc = 0; kc < 16; ++kc) {
@@ -103,28 +123,20 @@ __global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0
             if (k >= nb) break;
             const int buf = k & 1;
             if (wave == ko) {
-                double best = (lane >= k && lane < nb) ? (double)abs2(v[kc]) : -1.0;
-                int besti = lane;
-                for (int off = 32; off > 0; off >>= 1) {
-                    const double ov = __shfl_xor(best, off);
-                    const int oi = __shfl_xor(besti, off);
-                    if (ov > best || (ov == best && oi < besti)) {
-                        best = ov;
-                        besti = oi;
-                    }
-                }
-                const int p = __builtin_amdgcn_readfirstlane(besti);
-                if (!(best > 0.0)) {
+                const double m = (double)abs2(v[kc]);
+                const uint32_t key =
+                    ord < 0 ? ((uint32_t)((uint64_t)__double_as_longlong(m) >> 32) & ~63u) | (uint32_t)(63 - lane) : 0u;
+                const int p = 63 - (int)(wave_max_u32(key) & 63u);
+                const cx<T> pv = read_lane(v[kc], p);
+                if (!((double)abs2(pv) > 0.0)) {
                     if (lane == 0) spiv[buf] = -1;
                 } else {
-                    const cx<T> pv = read_lane(v[kc], p), kv = read_lane(v[kc], k);
                     const cx<T> inv = cx<T>((T)1) / pv;
-                    const cx<T> mine = lane == p ? kv : v[kc];  // row `lane` after the exchange
-                    lmul[buf][lane].set(lane > k && lane < nb ? mine * inv : cx<T>());
-                    if (lane == 0) {
-                        spiv[buf] = p;
-                        piv[k0 + k] = (int)(k0 + p);
-                    }
+                    const bool below = ord < 0 && lane != p;
+                    const cx<T> l = below ? v[kc] * inv : cx<T>();
+                    lmul[buf][lane].set(l);
+                    if (below) v[kc] = l;  // the stored L entry
+                    if (lane == 0) spiv[buf] = p;
                 }
             }
             __syncthreads();
@@ -133,31 +145,54 @@ __global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0
                 if (threadIdx.x == 0) status[z] = 1;
                 return;
             }
-            const cx<T> l = lmul[buf][lane].get();
+            const bool below = ord < 0 && lane != p;
+            if (lane == p) ord = k;
+            // the exchange sequence: position j of row p swaps with position k
+            const int j = __ffsll((long long)__ballot(cur == p)) - 1;
+            const int ck = __builtin_amdgcn_readlane(cur, k);
+            cur = lane == j ? ck : (lane == k ? p : cur);
+            if (threadIdx.x == 0) piv[k0 + k] = (int)(k0 + j);
+            if (16 * wave + 15 > k) {
+                const cx<T> l = lmul[buf][lane].get();
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int c = 16 * wave + j;
-                const cx<T> rk = read_lane(v[j], p), rp = read_lane(v[j], k);  // new rows k and p
-                cx<T> x = lane == k ? rk : (lane == p ? rp : v[j]);
-                if (c > k) {
-                    if (lane > k) x -= l * rk;
-                } else if (c == k) {
-                    if (lane > k) x = l;
+                for (int jj = 0; jj < 16; ++jj) {
+                    if (16 * wave + jj > k) {
+                        const cx<T> rp = read_lane(v[jj], p);
+                        if (below) v[jj] -= l * rp;
+                    }
                 }
-                v[j] = x;
             }
         }
     }
+    if (lane < nb) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int c = 16 * wave + j;
-        if (lane < nb && c < nb) A[(k0 + c) * n + k0 + lane] = v[j];
+        for (int j = 0; j < 16; ++j) {
+            const int c = 16 * wave + j;
+            if (c < nb) A[(k0 + c) * n + k0 + ord] = v[j];
+        }
     }
 }
 
+// Local source row of row k0 + lane after the block's exchanges (row k0 + k with
+// piv[k0 + k], clamped to the block, for k = 0 .. nb - 1, in order), composed into one
+// gather: lane i of the wave holds the composition's entry i, each exchange swaps two
+// lanes' entries.
+__device__ inline int block_perm(const int *piv, int64_t k0, int nb, int lane) {
+    const int pv = lane < nb ? (int)(min(max((int64_t)piv[k0 + lane], k0), k0 + nb - 1) - k0) : lane;
+    int v = lane;
+    for (int k = 0; k < nb; ++k) {
+        const int p = __builtin_amdgcn_readlane(pv, k);
+        const int a = __builtin_amdgcn_readlane(v, k), b = __builtin_amdgcn_readlane(v, p);
+        v = lane == k ? b : (lane == p ? a : v);
+    }
+    return v;
+}
+
 // Apply the row exchanges of the block at k0 to the columns [cb, ce) outside the
-// block.  Pivots are clamped to the block's rows: after a zero pivot the rest of the
-// block's entries are never written, and the factorisation reports TG_SINGULAR.
+// block: one wave per column, lane i gathering row k0 + perm(i) of the column (one
+// load for the wave, then one store: the column is read whole before it is written).
+// Pivots are clamped to the block's rows: after a zero pivot the rest of the block's
+// entries are never written, and the factorisation reports TG_SINGULAR.
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_swap(cx<T> *A0, Batch bt, int64_t k0, const int *piv0, int64_t cb,
                                                  int64_t ce) {
@@ -165,40 +200,40 @@ __global__ __launch_bounds__(256) void k_lu_swap(cx<T> *A0, Batch bt, int64_t k0
     const int64_t n = bt.n[z];
     if (k0 >= n) return;
     const int nb = (int)min((int64_t)NB, n - k0);
-    const int64_t col = cb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= min(ce, n) || (col >= k0 && col < k0 + nb)) return;
-    cx<T> *c = A0 + bt.off[z] + col * n;
-    const int *piv = piv0 + bt.voff[z];
-    for (int k = 0; k < nb; ++k) {
-        const int64_t p = min(max((int64_t)piv[k0 + k], k0), k0 + nb - 1);
-        if (p != k0 + k) {
-            cx<T> tmp = c[k0 + k];
-            c[k0 + k] = c[p];
-            c[p] = tmp;
+    const int lane = threadIdx.x & 63;
+    const int src = block_perm(piv0 + bt.voff[z], k0, nb, lane);
+    const int64_t cend = min(ce, n), waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    cx<T> *A = A0 + bt.off[z];
+    for (int64_t col = cb + (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); col < cend;
+         col += waves) {
+        if (col >= k0 && col < k0 + nb) continue;  // uniform over the wave
+        cx<T> *c = A + col * n + k0;
+        if (src != lane) {
+            const cx<T> x = c[src];
+            c[lane] = x;
         }
     }
 }
 
-// The deferred exchanges of the L columns: column c takes the row exchanges of every
-// block of a later block pair (the blocks at and after the pair start following c),
-// in block order.  Blocks cover disjoint rows, so their exchanges commute.
+// The deferred exchanges of the L columns: block k0 (from the second block pair on;
+// blockIdx.y selects it) exchanges rows of the columns of every earlier block pair.
+// Blocks cover disjoint rows, so their exchanges commute and run concurrently.
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_swap_left(cx<T> *A0, Batch bt, const int *piv0) {
     const int z = blockIdx.z;
     const int64_t n = bt.n[z];
-    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= n) return;
-    cx<T> *c = A0 + bt.off[z] + col * n;
-    const int *piv = piv0 + bt.voff[z];
-    for (int64_t k0 = (col / (2 * NB) + 1) * (2 * NB); k0 < n; k0 += NB) {
-        const int nb = (int)min((int64_t)NB, n - k0);
-        for (int k = 0; k < nb; ++k) {
-            const int64_t p = min(max((int64_t)piv[k0 + k], k0), k0 + nb - 1);
-            if (p != k0 + k) {
-                cx<T> tmp = c[k0 + k];
-                c[k0 + k] = c[p];
-                c[p] = tmp;
-            }
+    const int64_t k0 = (int64_t)(2 + blockIdx.y) * NB;
+    if (k0 >= n) return;
+    const int nb = (int)min((int64_t)NB, n - k0);
+    const int lane = threadIdx.x & 63;
+    const int src = block_perm(piv0 + bt.voff[z], k0, nb, lane);
+    const int64_t cend = (k0 / (2 * NB)) * (2 * NB), waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    cx<T> *A = A0 + bt.off[z];
+    for (int64_t col = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); col < cend; col += waves) {
+        cx<T> *c = A + col * n + k0;
+        if (src != lane) {
+            const cx<T> x = c[src];
+            c[lane] = x;
         }
     }
 }
@@ -222,8 +257,9 @@ __global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A0, Batch bt, i
     if (r >= n) return;
     // x U11 = a for this row.  c64: the row is held in registers (all 64 loads in
     // flight; the launch has at most a few waves per SIMD anyway); c128 (whose row
-    // would not fit the 256 architectural registers) re-reads the solved entries,
-    // which stay in L1.
+    // would not fit the 256 architectural registers) solves it in groups of 16
+    // columns, re-reading the solved groups (L1 hits, 16 loads in flight) for the
+    // later ones; the sums run over k in ascending order either way.
     if constexpr (sizeof(T) == 4) {
         cx<T> x[NB];
 #pragma unroll
@@ -237,10 +273,28 @@ __global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A0, Batch bt, i
             A[(k0 + c) * n + r] = x[c];
         }
     } else {
-        for (int c = 0; c < nb; ++c) {
-            cx<T> s = A[(k0 + c) * n + r];
-            for (int k = 0; k < c; ++k) s -= A[(k0 + k) * n + r] * U[c][k].get();
-            A[(k0 + c) * n + r] = s / U[c][c].get();
+        constexpr int G = 16;
+        static_assert(NB % G == 0, "groups of 16 columns");
+        for (int g = 0; g < NB; g += G) {
+            cx<T> s[G];
+#pragma unroll
+            for (int c = 0; c < G; ++c) s[c] = A[(k0 + g + c) * n + r];
+            for (int h = 0; h < g; h += G) {
+                cx<T> x[G];
+#pragma unroll
+                for (int k = 0; k < G; ++k) x[k] = A[(k0 + h + k) * n + r];
+#pragma unroll
+                for (int c = 0; c < G; ++c)
+#pragma unroll
+                    for (int k = 0; k < G; ++k) s[c] -= x[k] * U[g + c][h + k].get();
+            }
+#pragma unroll
+            for (int c = 0; c < G; ++c) {
+#pragma unroll
+                for (int k = 0; k < c; ++k) s[c] -= s[k] * U[g + c][g + k].get();
+                s[c] = s[c] / U[g + c][g + c].get();
+                A[(k0 + g + c) * n + r] = s[c];
+            }
         }
     }
 }
@@ -268,8 +322,9 @@ __global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A0, Batch bt, i
          col += waves) {
         cx<T> *c = A + col * n + k0;
         cx<T> v = c[lane];
+#pragma unroll 4
         for (int k = 0; k < nb - 1; ++k) {
-            const cx<T> ck(__shfl(v.re, k), __shfl(v.im, k));
+            const cx<T> ck = read_lane(v, k);
             if (lane > k) v -= L[k][lane].get() * ck;
         }
         c[lane] = v;
@@ -383,23 +438,25 @@ __global__ __launch_bounds__(512) void k_lu_block_solve(const cx<T> *A0, Batch b
         const cx<T> e = S[k][lane].get();
         return cd((double)e.re, (double)e.im);
     };
-    auto diag = [&](int k) {
-        const cx<T> e = S[k][k].get();
-        return cd((double)e.re, (double)e.im);
-    };
-    auto bcast = [&](int k) { return cd(__shfl(val.re, k), __shfl(val.im, k)); };
+    // the reciprocal of this lane's diagonal entry, for the non-unit triangles (every
+    // lane at once, off the substitution's latency chain)
+    cd rdiag(0.0);
+    if ((which == 1 || which == 2) && lane < nb) {
+        const cx<T> e = S[lane][lane].get();
+        rdiag = cd(1.0) / cd((double)e.re, (double)e.im);
+    }
     if (which == 0 || which == 2) {
         // lower triangular (L, or U^H with S[k][r] = conj(U(k, r)))
-        for (int k = 0; k < nb; ++k) {
-            if (which == 2 && lane == k) val = val / diag(k);
-            const cd vk = bcast(k);
+for (int k = 0; k < nb; ++k) {
+            if (which == 2 && lane == k) val = val * rdiag;
+            const cd vk = read_lane(val, k);
             if (lane > k && lane < nb) val -= s(k) * vk;
         }
     } else {
         // upper triangular (U, or L^H with S[k][r] = conj(L(k, r)))
-        for (int k = nb - 1; k >= 0; --k) {
-            if (which == 1 && lane == k) val = val / diag(k);
-            const cd vk = bcast(k);
+for (int k = nb - 1; k >= 0; --k) {
+            if (which == 1 && lane == k) val = val * rdiag;
+            const cd vk = read_lane(val, k);
             if (lane < k) val -= s(k) * vk;
         }
     }
@@ -566,9 +623,10 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
     check(hipStreamCreateWithFlags(&ps, hipStreamNonBlocking));
     check(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
     check(hipEventCreateWithFlags(&done, hipEventDisableTiming));
-    auto swap = [&](hipStream_t q, int64_t k, int64_t cb, int64_t ce) {
+    auto swap = [&](hipStream_t q, int64_t k, int64_t cb, int64_t ce) {  // one wave per column
         if (ce > cb)
-            k_lu_swap<T><<<dim3((unsigned)((ce - cb + 255) / 256), 1, z), 256, 0, q>>>(A, bt, k, ls.piv, cb, ce);
+            k_lu_swap<T><<<dim3((unsigned)std::min<int64_t>((ce - cb + 3) / 4, 2048), 1, z), 256, 0, q>>>(
+                A, bt, k, ls.piv, cb, ce);
     };
     auto diag = [&](hipStream_t q, int64_t k) {
         k_lu_diag<T><<<dim3(1, 1, z), 256, 0, q>>>(A, bt, k, ls.piv, ls.status);
@@ -619,8 +677,12 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
         update(s, p0, 2 * NB, p2, nmax, p4, nmax);  // overlaps the next panel
         check(hipGetLastError());
     }
-    if (st == TG_OK && nmax > 2 * NB)
-        k_lu_swap_left<T><<<dim3((unsigned)((nmax + 255) / 256), 1, z), 256, 0, s>>>(A, bt, ls.piv);
+    if (st == TG_OK && nmax > 2 * NB) {
+        // blocks from the second pair on (y), a wave per column of the earlier pairs (x)
+        const unsigned blocks = (unsigned)((nmax + NB - 1) / NB - 2);
+        k_lu_swap_left<T><<<dim3((unsigned)std::min<int64_t>((nmax + 3) / 4, 64), blocks, z), 256, 0, s>>>(A, bt,
+                                                                                                         ls.piv);
+    }
     check(hipGetLastError());
     if (ps) {
         (void)hipStreamSynchronize(ps);
@@ -800,6 +862,7 @@ __global__ void k_eq_rows(const cx<T> *A, int64_t n, int chunk, const double *co
     if (r >= n) return;
     const int64_t c0 = (int64_t)blockIdx.y * chunk, c1 = min(c0 + chunk, n);
     double mx = 0.0, mn = INFINITY;
+#pragma unroll 4
     for (int64_t c = c0; c < c1; ++c) {
         const double m = eq_mag(A[c * n + r]) * (colscale ? colscale[c] : 1.0);
         mx = fmax(mx, m);
@@ -814,6 +877,7 @@ __global__ void k_eq_reduce(const double *p, int parts, int64_t n, int take_min,
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     double v = take_min ? INFINITY : 0.0;
+#pragma unroll 8
     for (int q = 0; q < parts; ++q) v = take_min ? fmin(v, p[(int64_t)q * n + r]) : fmax(v, p[(int64_t)q * n + r]);
     out[r] = v;
 }
@@ -864,7 +928,10 @@ static int equilibrate_t(tg_factor *f) {
     hipStream_t s = c->stream;
     const int64_t n = f->n;
     const cx<T> *A = reinterpret_cast<const cx<T> *>(f->lu.ptr);
-    const int chunk = 256;
+    // columns per row-partial: enough partials for about 2048 workgroups on small
+    // operators (a 240-row operator otherwise runs one workgroup over all columns)
+    const int64_t rblocks = (n + 255) / 256;
+    const int chunk = (int)std::min<int64_t>(256, std::max<int64_t>(4, (n * rblocks + 2047) / 2048));
     const int parts = (int)((n + chunk - 1) / chunk);
     DevBuf work;
     TG_TRY(work.ensure(sizeof(double) * ((size_t)2 * parts * n + 6 * (size_t)n)));
