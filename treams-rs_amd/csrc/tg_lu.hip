@@ -37,11 +37,16 @@ struct tg_factor {
     // powers of two, split order)
     bool equilibrated = false;
     tg::DevBuf eq;
+    // c128 factors of at most kDenseResidual unknowns keep the FP64 operator as
+    // assembled (before equilibration): the refinement residual of forward solves is
+    // then one GEMM instead of a matrix-free apply
+    tg::DevBuf op;
 };
 
 namespace tg {
 
 constexpr int NB = 64;
+constexpr int64_t kDenseResidual = 2048;  // largest operator copied for the residual (64 MiB)
 
 // ------------------------------------------------------------ factor kernels --
 //
@@ -1003,6 +1008,11 @@ static int equilibrate_t(tg_factor *f) {
 static int factor_cluster(tg_factor *f) {
     tg_cluster *c = f->cluster;
     TG_TRY(launch_assemble_operator(c, f->precision, f->lu.ptr));
+    if (f->precision == TG_STORE_C128 && f->n <= kDenseResidual) {
+        const size_t bytes = sizeof(cd) * (size_t)f->n * (size_t)f->n;
+        TG_TRY(f->op.ensure(bytes));
+        TG_HIP(hipMemcpyAsync(f->op.ptr, f->lu.ptr, bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
     TG_TRY(f->precision == TG_STORE_C64 ? equilibrate_t<float>(f) : equilibrate_t<double>(f));
     Timer::Rec rec;
     TG_TRY(c->timer.begin("lu_factor", c->stream, rec));
@@ -1026,6 +1036,29 @@ static int lu_solve(tg_factor *f, cd *x, int cols, int adjoint) {
     TG_HIP(hipGetLastError());
     TG_TRY(c->timer.end("lu_solve", c->stream, rec));
     return TG_OK;
+}
+
+// C = A B for column-major complex matrices (M x K times K x N) on MFMA in the 3M
+// form, one 64 x 64 tile of C per workgroup.
+template <class T, bool KTAIL>
+__global__ __launch_bounds__(256, 2) void k_zgemm_plain(const cx<T> *A, int64_t lda, const cx<T> *B, int64_t ldb,
+                                                        cx<T> *C, int64_t ldc, int64_t M, int64_t N, int K) {
+    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    __shared__ TileLds<T> sm;
+    Tile3M<T> acc;
+    zgemm_tile_3m<T, KTAIL>(A, lda, B, ldb, r0, c0, M, N, K, sm, acc);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, rr = lane & 15;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<T>(lane, v);
+                const T t1 = acc.t1[a][b][v], t2 = acc.t2[a][b][v], t3 = acc.t3[a][b][v];
+                if (gr < M && gc < N) C[gc * ldc + gr] = cx<T>(t1 - t2, t3 - t1 - t2);
+            }
 }
 
 // x = A^{-1} b (or A^{-H} b) in the split order with FP64 iterative refinement on the
@@ -1062,7 +1095,23 @@ int factor_solve_split(tg_factor *f, const cd *b, cd *x, int cols, int adjoint, 
         int refinements = 0;
         const int max_refine = f->precision == TG_STORE_C64 ? 12 : 4;
         while (true) {
-            if ((st = apply_operator(c, x, ax, cols, adjoint, TG_OP_MATRIX_FREE_F64))) break;
+            if (f->op.ptr && !adjoint) {
+                // ax = A x with the stored FP64 operator (n x n times n x cols)
+                const int64_t n = f->n;
+                const dim3 grid((unsigned)((cols + 63) / 64), (unsigned)((n + 63) / 64));
+                const cd *op = f->op.as<cd>();
+                if (n % 16)
+                    k_zgemm_plain<double, true><<<grid, 256, 0, c->stream>>>(op, n, x, n, ax, n, n, cols, (int)n);
+                else
+                    k_zgemm_plain<double, false><<<grid, 256, 0, c->stream>>>(op, n, x, n, ax, n, n, cols, (int)n);
+                if (hipGetLastError() != hipSuccess) {
+                    set_error("launch of the residual GEMM failed");
+                    st = TG_DEVICE_ERROR;
+                    break;
+                }
+            } else if ((st = apply_operator(c, x, ax, cols, adjoint, TG_OP_MATRIX_FREE_F64))) {
+                break;
+            }
             if (hipMemcpyAsync(r, b, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
                 set_error("copy of the right-hand side failed");
                 st = TG_DEVICE_ERROR;
@@ -1097,29 +1146,6 @@ int factor_solve_split(tg_factor *f, const cd *b, cd *x, int cols, int adjoint, 
     return st;
 }
 
-// C = A B for column-major complex matrices (M x K times K x N) on MFMA in the 3M
-// form, one 64 x 64 tile of C per workgroup.
-template <class T, bool KTAIL>
-__global__ __launch_bounds__(256, 2) void k_zgemm_plain(const cx<T> *A, int64_t lda, const cx<T> *B, int64_t ldb,
-                                                        cx<T> *C, int64_t ldc, int64_t M, int64_t N, int K) {
-    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
-    __shared__ TileLds<T> sm;
-    Tile3M<T> acc;
-    zgemm_tile_3m<T, KTAIL>(A, lda, B, ldb, r0, c0, M, N, K, sm, acc);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, rr = lane & 15;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<T>(lane, v);
-                const T t1 = acc.t1[a][b][v], t2 = acc.t2[a][b][v], t3 = acc.t3[a][b][v];
-                if (gr < M && gc < N) C[gc * ldc + gr] = cx<T>(t1 - t2, t3 - t1 - t2);
-            }
-}
-
 // Gradient body shared with the GMRES pullback (tg_adjoint.hip), here with the
 // adjoint solve by the factors.
 int pullback_with(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent, int cols,
@@ -1149,6 +1175,8 @@ int tg_factor_create(tg_cluster *c, int precision, tg_factor **out) {
     if (st) {
         (void)hipStreamSynchronize(c->stream);
         f->lu.release();
+        f->eq.release();
+        f->op.release();
         luset_release(f->set);
         delete f;
         return st;
@@ -1161,6 +1189,8 @@ int tg_factor_destroy(tg_factor *f) {
     if (!f) return TG_OK;
     (void)hipStreamSynchronize(f->cluster->stream);
     f->lu.release();
+    f->eq.release();
+    f->op.release();
     luset_release(f->set);
     delete f;
     return TG_OK;
