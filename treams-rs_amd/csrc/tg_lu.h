@@ -21,6 +21,10 @@ struct LuSet {
     const int *n = nullptr;
     const int64_t *voff = nullptr;
     int *piv = nullptr, *perm = nullptr, *status = nullptr;
+    // the look-ahead panel stream and its two events, created at the first
+    // factorisation and kept until luset_release
+    hipStream_t panel = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
 };
 
 // Matrix z: dimension n[z], elements from A + off[z] (column-major, leading dim n[z]).

@@ -597,8 +597,29 @@ int luset_init(LuSet &ls, int precision, void *A, const std::vector<int64_t> &of
 }
 
 void luset_release(LuSet &ls) {
+    if (ls.panel) {
+        (void)hipStreamSynchronize(ls.panel);
+        (void)hipStreamDestroy(ls.panel);
+        ls.panel = nullptr;
+    }
+    if (ls.ready) (void)hipEventDestroy(ls.ready);
+    if (ls.done) (void)hipEventDestroy(ls.done);
+    ls.ready = ls.done = nullptr;
     ls.meta.release();
     ls.work.release();
+}
+
+// The packed row permutation of the solves, perm[voff + k0 + i] = voff + k0 + (the
+// block's composed exchanges)(i): pivots stay inside their 64-row block, so each block
+// composes its own; one wave per block (grid.x), matrix grid.z.
+__global__ __launch_bounds__(64) void k_lu_perm(Batch bt, const int *piv0, int *perm0) {
+    const int z = blockIdx.z;
+    const int64_t n = bt.n[z], k0 = (int64_t)blockIdx.x * NB;
+    if (k0 >= n) return;
+    const int nb = (int)min((int64_t)NB, n - k0);
+    const int lane = threadIdx.x;
+    const int src = block_perm(piv0 + bt.voff[z], k0, nb, lane);
+    if (lane < nb) perm0[bt.voff[z] + k0 + lane] = (int)(bt.voff[z] + k0 + src);
 }
 
 template <class T>
@@ -616,8 +637,6 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
     // exchanges reach the panel's own columns at once, the columns right of the pair
     // on the main stream before their panel solves, and the L columns left of the
     // pair in one pass at the end.
-    hipStream_t ps = nullptr;
-    hipEvent_t ready = nullptr, done = nullptr;
     int st = TG_OK;
     auto check = [&](hipError_t e) {
         if (e != hipSuccess && st == TG_OK) {
@@ -625,9 +644,11 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
             st = TG_DEVICE_ERROR;
         }
     };
-    check(hipStreamCreateWithFlags(&ps, hipStreamNonBlocking));
-    check(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-    check(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    if (!ls.panel) check(hipStreamCreateWithFlags(&ls.panel, hipStreamNonBlocking));
+    if (!ls.ready) check(hipEventCreateWithFlags(&ls.ready, hipEventDisableTiming));
+    if (!ls.done) check(hipEventCreateWithFlags(&ls.done, hipEventDisableTiming));
+    hipStream_t ps = ls.panel;
+    hipEvent_t ready = ls.ready, done = ls.done;
     auto swap = [&](hipStream_t q, int64_t k, int64_t cb, int64_t ce) {  // one wave per column
         if (ce > cb)
             k_lu_swap<T><<<dim3((unsigned)std::min<int64_t>((ce - cb + 3) / 4, 2048), 1, z), 256, 0, q>>>(
@@ -689,31 +710,27 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
                                                                                                          ls.piv);
     }
     check(hipGetLastError());
-    if (ps) {
+    if (st == TG_OK) {
+        // the panel stream's last work joins the main stream (the last pair's panel
+        // has no main-stream work after it)
+        check(hipEventRecord(done, ps));
+        check(hipStreamWaitEvent(s, done, 0));
+    } else {
         (void)hipStreamSynchronize(ps);
-        (void)hipStreamDestroy(ps);
     }
-    if (ready) (void)hipEventDestroy(ready);
-    if (done) (void)hipEventDestroy(done);
     if (st != TG_OK) return st;
-    std::vector<int> stat(ls.count), piv_h(ls.ntot), perm_h(ls.ntot);
+    // each matrix's exchanges composed into one gather over the packed rows:
+    // (P x)[voff + i] = x[voff + perm_local[i]]
+    k_lu_perm<<<dim3((unsigned)((nmax + NB - 1) / NB), 1, z), 64, 0, s>>>(bt, ls.piv, ls.perm);
+    TG_HIP(hipGetLastError());
+    std::vector<int> stat(ls.count);
     TG_HIP(hipMemcpyAsync(stat.data(), ls.status, sizeof(int) * ls.count, hipMemcpyDeviceToHost, s));
-    TG_HIP(hipMemcpyAsync(piv_h.data(), ls.piv, sizeof(int) * ls.ntot, hipMemcpyDeviceToHost, s));
     TG_HIP(hipStreamSynchronize(s));
     for (int v : stat)
         if (v) {
             set_error("zero pivot in the LU factorisation");
             return TG_SINGULAR;
         }
-    // compose each matrix's exchanges into one gather over the packed rows:
-    // (P x)[voff + i] = x[voff + perm_local[i]]
-    for (int z2 = 0; z2 < ls.count; ++z2) {
-        const int64_t v0 = ls.voff_h[z2], nz = ls.n_h[z2];
-        for (int64_t i = 0; i < nz; ++i) perm_h[v0 + i] = (int)(v0 + i);
-        for (int64_t k = 0; k < nz; ++k) std::swap(perm_h[v0 + k], perm_h[v0 + piv_h[v0 + k]]);
-    }
-    TG_HIP(hipMemcpyAsync(ls.perm, perm_h.data(), sizeof(int) * ls.ntot, hipMemcpyHostToDevice, s));
-    TG_HIP(hipStreamSynchronize(s));
     return TG_OK;
 }
 
