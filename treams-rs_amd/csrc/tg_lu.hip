@@ -243,65 +243,47 @@ __global__ __launch_bounds__(256) void k_lu_swap_left(cx<T> *A0, Batch bt, const
     }
 }
 
-// L21 = A21 U11^{-1}, in place: one thread per row below the block (coalesced
-// column reads), U11 in LDS.
+// L21 = A21 U11^{-1}, in place.  Each wave solves 8 consecutive rows below the block
+// with lane c holding column c of every row (for c128 one 128-byte segment of each
+// column); step k scales entry k by 1 / u_kk and broadcasts it (readlane), and the
+// columns right of k subtract x_k u_kc, with U11 staged in LDS by rows (lane c reads
+// u_kc).  A chain of 64 short steps per wave, all rows in parallel; the sums run over
+// k in ascending order.
 template <class T>
 __global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A0, Batch bt, int64_t k0) {
+    constexpr int R = 8;
     const int z = blockIdx.z;
     const int64_t n = bt.n[z];
-    if (k0 + NB >= n) return;  // no rows below a full block
+    const int64_t rw = k0 + NB + (int64_t)blockIdx.x * 4 * R;  // first row of the workgroup
+    if (k0 + NB >= n || rw >= n) return;                         // uniform over the workgroup
     cx<T> *A = A0 + bt.off[z];
-    const int nb = NB;
-    __shared__ lcx<T> U[NB][NB];  // [col][row]
-    for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
-        const int col = t / nb, row = t - col * nb;
-        U[col][row].set(A[(k0 + col) * n + k0 + row]);
+    __shared__ lcx<T> U[NB][NB];  // U[k][c] = u_kc
+    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+        const int col = t / NB, row = t - col * NB;
+        U[row][col].set(A[(k0 + col) * n + k0 + row]);
     }
     __syncthreads();
-    const int64_t r = k0 + nb + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    // x U11 = a for this row.  c64: the row is held in registers (all 64 loads in
-    // flight; the launch has at most a few waves per SIMD anyway); c128 (whose row
-    // would not fit the 256 architectural registers) solves it in groups of 16
-    // columns, re-reading the solved groups (L1 hits, 16 loads in flight) for the
-    // later ones; the sums run over k in ascending order either way.
-    if constexpr (sizeof(T) == 4) {
-        cx<T> x[NB];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t r0 = rw + wave * R;
+    const cx<T> rd = cx<T>((T)1) / U[lane][lane].get();
+    cx<T> a[R];
 #pragma unroll
-        for (int c = 0; c < NB; ++c) x[c] = A[(k0 + c) * n + r];
+    for (int r = 0; r < R; ++r) a[r] = A[(k0 + lane) * n + min(r0 + r, n - 1)];
+    for (int k = 0; k < NB; ++k) {
+        const cx<T> u = U[k][lane].get();
+        const cx<T> rk = read_lane(rd, k);
 #pragma unroll
-        for (int c = 0; c < NB; ++c) {
-            cx<T> s = x[c];
-#pragma unroll
-            for (int k = 0; k < c; ++k) s -= x[k] * U[c][k].get();
-            x[c] = s / U[c][c].get();
-            A[(k0 + c) * n + r] = x[c];
-        }
-    } else {
-        constexpr int G = 16;
-        static_assert(NB % G == 0, "groups of 16 columns");
-        for (int g = 0; g < NB; g += G) {
-            cx<T> s[G];
-#pragma unroll
-            for (int c = 0; c < G; ++c) s[c] = A[(k0 + g + c) * n + r];
-            for (int h = 0; h < g; h += G) {
-                cx<T> x[G];
-#pragma unroll
-                for (int k = 0; k < G; ++k) x[k] = A[(k0 + h + k) * n + r];
-#pragma unroll
-                for (int c = 0; c < G; ++c)
-#pragma unroll
-                    for (int k = 0; k < G; ++k) s[c] -= x[k] * U[g + c][h + k].get();
-            }
-#pragma unroll
-            for (int c = 0; c < G; ++c) {
-#pragma unroll
-                for (int k = 0; k < c; ++k) s[c] -= s[k] * U[g + c][g + k].get();
-                s[c] = s[c] / U[g + c][g + c].get();
-                A[(k0 + g + c) * n + r] = s[c];
-            }
+        for (int r = 0; r < R; ++r) {
+            const cx<T> xk = read_lane(a[r], k) * rk;
+            if (lane == k)
+                a[r] = xk;
+            else if (lane > k)
+                a[r] -= xk * u;
         }
     }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (r0 + r < n) A[(k0 + lane) * n + r0 + r] = a[r];
 }
 
 // U12 = L11^{-1} A12 (unit lower L11 of the block at k0) for the columns [cb, ce), in
@@ -658,7 +640,7 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
         k_lu_diag<T><<<dim3(1, 1, z), 256, 0, q>>>(A, bt, k, ls.piv, ls.status);
     };
     auto lower = [&](hipStream_t q, int64_t k) {  // rows below the block at k
-        k_trsm_lower_panel<T><<<dim3((unsigned)((nmax - k - NB + 255) / 256), 1, z), 256, 0, q>>>(A, bt, k);
+        k_trsm_lower_panel<T><<<dim3((unsigned)((nmax - k - NB + 31) / 32), 1, z), 256, 0, q>>>(A, bt, k);
     };
     auto upper = [&](hipStream_t q, int64_t k, int64_t cb, int64_t ce) {
         k_trsm_upper_panel<T><<<dim3((unsigned)std::min<int64_t>((ce - cb + 3) / 4, 1024), 1, z), 256, 0, q>>>(
