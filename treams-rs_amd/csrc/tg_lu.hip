@@ -75,6 +75,24 @@ __device__ __forceinline__ cx<T> read_lane(cx<T> x, int lane) {
     return cx<T>(read_lane(x.re, lane), read_lane(x.im, lane));
 }
 
+// 1 / z with one real division: z scaled exactly by the power of two 2^-e that brings
+// its larger component into [0.5, 1), so |z 2^-e|^2 lies in [0.25, 2] whatever |z|.
+template <class T>
+__device__ __forceinline__ cx<T> recip_scaled(cx<T> z) {
+    int e;
+    (void)frexp(fmax(fabs(z.re), fabs(z.im)), &e);
+    const T a = ldexp(z.re, -e), b = ldexp(z.im, -e);
+    const T r = (T)1 / (a * a + b * b);
+    return cx<T>(ldexp(a * r, -e), ldexp(-b * r, -e));
+}
+
+// v -= l r as four FMAs.
+template <class T>
+__device__ __forceinline__ void sub_mul(cx<T> &v, cx<T> l, cx<T> r) {
+    v.re = fma(-l.im, -r.im, fma(-l.re, r.re, v.re));
+    v.im = fma(-l.im, r.re, fma(-l.re, r.im, v.im));
+}
+
 // Largest value over the wave (wave-uniform): DPP steps inside each row of 16 lanes,
 // then the four row maxima by readlane.  Every lane of the wave must be active.
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
@@ -99,7 +117,7 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // low 6 bits replaced by 63 - row: magnitudes within 2^-14 of the largest tie, and
 // the lowest row wins) and the multipliers, and publishes them in LDS (double
 // buffered: one barrier per step); every wave then updates its columns right of k
-// on the rows not yet pivoted, reading the pivot row with readlane.
+// (multipliers zero on the rows already pivoted), reading the pivot row with readlane.
 template <class T>
 __global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0, int *piv0, int *status) {
     const int z = blockIdx.z;
@@ -108,7 +126,8 @@ __global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0
     cx<T> *A = A0 + bt.off[z];
     int *piv = piv0 + bt.voff[z];
     const int nb = (int)min((int64_t)NB, n - k0);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // wave-uniform (a scalar register): the column tests below are scalar branches
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     __shared__ lcx<T> lmul[2][NB];
     __shared__ int spiv[2];
     cx<T> v[16];
@@ -136,7 +155,7 @@ __global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0
                 if (!((double)abs2(pv) > 0.0)) {
                     if (lane == 0) spiv[buf] = -1;
                 } else {
-                    const cx<T> inv = cx<T>((T)1) / pv;
+                    const cx<T> inv = recip_scaled(pv);
                     const bool below = ord < 0 && lane != p;
                     const cx<T> l = below ? v[kc] * inv : cx<T>();
                     lmul[buf][lane].set(l);
@@ -145,27 +164,30 @@ __global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0
                 }
             }
             __syncthreads();
-            const int p = __builtin_amdgcn_readfirstlane(spiv[buf]);
+            // both LDS reads issued together (the multipliers are unused after a zero pivot)
+            const int praw = spiv[buf];
+            const cx<T> l = lmul[buf][lane].get();
+            const int p = __builtin_amdgcn_readfirstlane(praw);
             if (p < 0) {
                 if (threadIdx.x == 0) status[z] = 1;
                 return;
             }
-            const bool below = ord < 0 && lane != p;
             if (lane == p) ord = k;
             // the exchange sequence: position j of row p swaps with position k
             const int j = __ffsll((long long)__ballot(cur == p)) - 1;
             const int ck = __builtin_amdgcn_readlane(cur, k);
             cur = lane == j ? ck : (lane == k ? p : cur);
             if (threadIdx.x == 0) piv[k0 + k] = (int)(k0 + j);
-            if (16 * wave + 15 > k) {
-                const cx<T> l = lmul[buf][lane].get();
+            // l = 0 on the rows already pivoted and on the pivot row: they keep their
+            // values (v - 0 rp = v for finite pivot rows).  A wave right of column k
+            // updates all its columns, the wave of column k those right of kc (known at
+            // compile time), one left of it none: three straight-line paths.
+            if (wave > ko) {
 #pragma unroll
-                for (int jj = 0; jj < 16; ++jj) {
-                    if (16 * wave + jj > k) {
-                        const cx<T> rp = read_lane(v[jj], p);
-                        if (below) v[jj] -= l * rp;
-                    }
-                }
+                for (int jj = 0; jj < 16; ++jj) sub_mul(v[jj], l, read_lane(v[jj], p));
+            } else if (wave == ko && kc < 15) {
+#pragma unroll
+                for (int jj = kc + 1; jj < 16; ++jj) sub_mul(v[jj], l, read_lane(v[jj], p));
             }
         }
     }
