@@ -135,3 +135,34 @@ def test_particle_cluster_rejects_sphere_only_operations(pkg):
         par.solve(a, preconditioner="block_jacobi")
     with pytest.raises(ValueError, match="distinct"):
         pkg.IterativeSphereCluster.from_tmatrices(2, 1.0, np.zeros((2, 3)), t)
+
+
+@pytest.mark.parametrize("N,side,scale", [(12, 5.0, 1.0), (24, 8.0, 0.5)])
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_particle_cluster_strong_coupling_exchanges_rows(pkg, oracle, N, side, scale, precision):
+    """T-matrices of norm ~0.5-1 on close particles: A = I - T C has off-diagonal
+    entries up to ~100-150 against its unit diagonal, so the factorisation exchanges
+    rows in many pivot steps of the 64-row diagonal blocks (121 of 192 steps at N = 12,
+    61 of 384 at N = 24, in a partial-pivoting LU of the reference-order blocks) and
+    equilibrates.  n = 192 is a block pair and a single block, whose exchanges reach
+    the pair's columns at the end; n = 384 is three pairs with look-ahead.  Checked
+    with the oracle's operator: the residual |A X - T| and the distance to its dense
+    solve (cond(A) ~ 3e4 bounds the latter)."""
+    rng = np.random.default_rng(5)
+    radii, e, pos = pkg.random_cluster(N, side, 0.1, (0.2, 0.3), 4 + 0.1j, seed=7)
+    lmax = 2
+    M = 2 * lmax * (lmax + 2)
+    local = [_random_local(rng, M, scale=scale) for _ in range(N)]
+    C = oracle.Cluster(lmax, 1.1, radii, e, pos).C
+    T = np.zeros_like(C)
+    for i, t in enumerate(local):
+        T[i * M:(i + 1) * M, i * M:(i + 1) * M] = t
+    A = np.eye(len(C)) - T @ C
+    expected = np.linalg.solve(A, T)
+    got = pkg.particle_cluster(local, pos, 1.1, lmax=lmax, precision=precision)
+    X = got.detach().cpu().numpy() if torch.is_tensor(got) else np.asarray(got)
+    resid = np.linalg.norm(A @ X - T) / np.linalg.norm(T)
+    err = _rel(X, expected)
+    print(f"N={N} {precision}: residual {resid:.2e}, distance to the dense solve {err:.2e}")
+    assert resid <= 1e-11  # observed 4e-14 - 1.3e-13
+    assert err <= 1e-10    # observed 8e-14 - 2.9e-13
