@@ -355,6 +355,22 @@ static int check_cluster(const tg_cluster *c) {
 
 using namespace tg;
 
+// The translation plan depends on lmax alone (0.2 ms to build at lmax 3, 3.7 ms at
+// lmax 6): built once per process and copied into each cluster.
+static int cached_plan(int lmax, HostPlan &out) {
+    static std::mutex mu;
+    static std::map<int, HostPlan> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(lmax);
+    if (it == cache.end()) {
+        HostPlan hp;
+        if (build_plan(lmax, hp)) return 1;
+        it = cache.emplace(lmax, std::move(hp)).first;
+    }
+    out = it->second;
+    return 0;
+}
+
 extern "C" {
 
 const char *tg_last_error(void) { return g_error.c_str(); }
@@ -423,7 +439,7 @@ int tg_cluster_create(int lmax, double k0, int n_spheres, const double *radii, c
     for (int i = 0; i < n_spheres; ++i) c->eps.push_back(cd(epsilon[i].re, epsilon[i].im));
     int st = TG_OK;
     do {
-        if (build_plan(lmax, c->hplan)) {
+        if (cached_plan(lmax, c->hplan)) {
             set_error("plan construction failed");
             st = TG_INVALID_INPUT;
             break;
