@@ -67,9 +67,12 @@ TG_HD tg_dual operator*(tg_dual a, tg_dual b) {
     return {a.v * b.v, a.dx * b.v + a.v * b.dx, a.de * b.v + a.v * b.de};
 }
 TG_HD tg_dual operator*(double s, tg_dual a) { return {s * a.v, s * a.dx, s * a.de}; }
+// One complex reciprocal for the value and both derivatives (the recurrences below
+// are chains of these divisions).
 TG_HD tg_dual operator/(tg_dual a, tg_dual b) {
-    cd q = a.v / b.v;
-    return {q, (a.dx - q * b.dx) / b.v, (a.de - q * b.de) / b.v};
+    const cd r = cd(1.0) / b.v;
+    const cd q = a.v * r;
+    return {q, (a.dx - q * b.dx) * r, (a.de - q * b.de) * r};
 }
 TG_HD tg_dual operator/(double s, tg_dual b) { return tg_dual(cd(s)) / b; }
 
