@@ -180,29 +180,43 @@ def interacting_tmatrix(pkg, torch):
     scripts/benchmark_cluster.py:281-284): chains of spheres with radii linspace(0.15,
     0.25), spacing 0.8, eps 4+0.1i, k0 1.3; particle coefficients, translations and the
     full interacting T-matrix X = (I - T C)^{-1} T (all n columns), here c128 factors
-    and FP64-refined forward solves.  Published 4-thread CPU times beside."""
+    and FP64-refined forward solves.  Timed the reference's way
+    (scripts/benchmark_cluster.py:1395-1405): one warm-up call, a batch of calls per
+    sample sized to about 20 ms, 7 samples, the median per-call time.  Published
+    4-thread CPU medians beside."""
+    import statistics
+
     published = {(8, 3): 1.96, (16, 3): 9.35, (8, 5): 12.69, (32, 3): 43.41}
     out = []
     for (N, lmax), ref_ms in published.items():
         radii = np.linspace(0.15, 0.25, N)
         pos = np.column_stack([np.arange(N) * 0.8, np.zeros((N, 2))])
         eps = np.full(N, 4 + 0.1j)
-        times = []
-        for rep in range(4):  # the first is the warm-up
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
+
+        def call():
             cl = pkg.IterativeSphereCluster(lmax, 1.3, radii, eps, pos)
             f = pkg.SphereClusterFactor(cl, "c128")
-            X, conv = f.solve(torch.eye(cl.n, dtype=torch.complex128, device="cuda"))
+            X, _ = f.solve(torch.eye(cl.n, dtype=torch.complex128, device="cuda"))
             torch.cuda.synchronize()
-            times.append(time.perf_counter() - t0)
             f.close()
             del cl
-        ms = 1e3 * min(times[1:])
-        out.append({"spheres": N, "lmax": lmax, "n": int(X.shape[0]), "ms": ms, "reference_cpu_4_threads_ms": ref_ms,
-                    "speedup": ref_ms / ms})
-        log(f"interacting T-matrix N={N} lmax={lmax} (n={X.shape[0]}): {ms:.2f} ms (reference CPU, 4 threads: "
-            f"{ref_ms} ms)")
+            return X
+
+        call()  # warm-up
+        t0 = time.perf_counter()
+        X = call()
+        batch = max(1, int(np.ceil(0.02 / (time.perf_counter() - t0))))
+        samples = []
+        for _ in range(7):
+            t0 = time.perf_counter()
+            for _ in range(batch):
+                X = call()
+            samples.append((time.perf_counter() - t0) / batch)
+        ms = 1e3 * statistics.median(samples)
+        out.append({"spheres": N, "lmax": lmax, "n": int(X.shape[0]), "ms": ms, "min_ms": 1e3 * min(samples),
+                    "calls_per_sample": batch, "reference_cpu_4_threads_ms": ref_ms, "speedup": ref_ms / ms})
+        log(f"interacting T-matrix N={N} lmax={lmax} (n={X.shape[0]}): {ms:.2f} ms median of 7 x {batch} calls "
+            f"(reference CPU, 4 threads: {ref_ms} ms)")
     return out
 
 
@@ -307,6 +321,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=0, help="destination spheres per CPU apply (0: all)")
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--no-lu", action="store_true")
+    ap.add_argument("--dense-benchmark-only", action="store_true",
+                    help="run only the reference's dense benchmark (interacting T-matrices of sphere chains, "
+                         "n = 240 to 960) and print its results as one JSON line (for profiling)")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--operator", choices=("tables", "stored"), default="tables",
                     help="c64 coupling of the headline inner solves: pair tables (GEMMs) or the dense store")
@@ -337,6 +354,9 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"copy calibration: {src.numel() * 4} bytes x 3")
         del src, dst
+    if args.dense_benchmark_only:
+        print(json.dumps({"interacting_tmatrix": interacting_tmatrix(pkg, torch)}))
+        return 0
     lmax, radii, eps, pos = pkg.workload(args.workload, seed=args.seed)
     k0 = 1.3
     N = radii.size
