@@ -46,7 +46,7 @@ struct tg_factor {
 namespace tg {
 
 constexpr int NB = 64;
-constexpr int64_t kDenseResidual = 2048;  // largest operator copied for the residual (64 MiB)
+constexpr int64_t kDenseResidual = 4096;  // largest operator copied for the residual (256 MiB)
 
 // ------------------------------------------------------------ factor kernels --
 //
