@@ -328,8 +328,9 @@ EXPORT void or_hankels(int last, double x, cplx *out) {
  * Regular spherical Bessel j_p(z), p = 0..last, complex z.  Below |z| = 0.5 the
  * 16-term even power series of special/bessel.rs:425-431; elsewhere (AMOS in the
  * reference, un-vendored complex-bessel 0.3.0) a backward ratio recurrence
- * r_p = j_p / j_{p-1} = z / (2p + 1 - z r_{p+1}) normalised by j_0 = sin z / z.
- * Pinned by mpmath fixtures.
+ * r_p = j_p / j_{p-1} = z / (2p + 1 - z r_{p+1}) normalised by the larger of
+ * j_0 = sin z / z and j_1.  Pinned by mpmath fixtures, including the zeros of j_0
+ * and j_1 (z = pi, 2 pi, 3 pi, 4.4934...).
  */
 static void sph_j_series(int l, cplx z, cplx *value, cplx *first, cplx *second);
 
@@ -349,8 +350,13 @@ EXPORT void or_spherical_jn(int last, cplx z, cplx *out) {
         r = z / ((double)(2 * p + 1) - z * r);
         if (p <= last) ratios[p] = r;
     }
-    out[0] = csin(z) / z;
-    for (int p = 1; p <= last; ++p) out[p] = out[p - 1] * ratios[p];
+    /* one scale for the whole sequence, from whichever of j_0 = sin z / z and
+     * j_1 = (sin z / z - cos z) / z is larger: their zeros interlace, so the larger
+     * is never near a zero (scaling by j_0 alone loses j_1.. near z = k pi) */
+    cplx j0 = csin(z) / z, j1 = (j0 - ccos(z)) / z;
+    out[0] = j0;
+    if (last >= 1) out[1] = cabs(j0) >= cabs(j1) ? j0 * ratios[1] : j1;
+    for (int p = 2; p <= last; ++p) out[p] = out[p - 1] * ratios[p];
     free(ratios);
 }
 

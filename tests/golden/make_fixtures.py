@@ -42,6 +42,10 @@ def hankel_table():
 
 def jn_table():
     zs = [0.3, complex(0.45, 0.1), 0.7, 1.3, complex(3.0, 0.5), complex(5.66, 22.6), 12.0, complex(0.2, -0.3)]
+    # zeros of j_0 (k pi) and of j_1 (4.4934..., 7.7253...), where a sequence scaled by
+    # one of them alone loses its accuracy, and a complex point next to pi
+    zs += [float(mp.pi), float(2 * mp.pi), float(3 * mp.pi), 4.493409457909064, 7.725251836937707,
+           complex(float(mp.pi), 1e-3)]
     return {"z": [[z.real, z.imag] if isinstance(z, complex) else [z, 0.0] for z in zs], "pmax": 13,
             "values": [[c2(sph_j(p, z)) for p in range(14)] for z in zs]}
 
@@ -86,7 +90,7 @@ def mie_bh(n, x, eps):
 
 def mie_table():
     cases = [(0.3, complex(4, 0.1)), (0.455, complex(6, 0.2)), (1.1, complex(2.5, 0)), (2.0, complex(4, 0.1)),
-             (0.26, complex(16, 0.5))]
+             (0.26, complex(16, 0.5)), (float(mp.pi), complex(4, 0.1)), (4.493409457909064, complex(2.5, 0))]
     out = []
     for x, eps in cases:
         for n in range(1, 7):

@@ -66,7 +66,10 @@ def test_spherical_jn_against_mpmath(oracle, golden_dir):
     for z, row in zip(d["z"], d["values"]):
         actual = oracle.spherical_jn(d["pmax"], complex(*z))
         expected = np.array([complex(*v) for v in row])
-        assert np.all(np.abs(actual - expected) <= 1e-13 * np.abs(expected) + 1e-300), z
+        # relative 1e-13, except for a value at a zero of j_0 or j_1 (z = pi, 4.4934...),
+        # which FP64 resolves only to the rounding of the sequence's scale
+        scale = np.abs(expected[:2]).max()
+        assert np.all(np.abs(actual - expected) <= 1e-13 * np.abs(expected) + 1e-15 * scale), z
 
 
 def test_wigner3j_against_sympy(oracle, golden_dir):

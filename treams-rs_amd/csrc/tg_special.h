@@ -35,7 +35,11 @@ TG_HD double tg_sph_j_series(int p, double x) {
     return sum;
 }
 
-// j_p(x), p = 0..last (last <= 63), real x >= 0.
+// j_p(x), p = 0..last (last <= 63), real x >= 0.  The ratios r_p = j_p / j_{p-1} of
+// the backward recurrence fix the sequence up to one factor, taken from whichever of
+// j_0 = sin x / x and j_1 = (sin x / x - cos x) / x is larger in magnitude: their
+// zeros interlace, so the larger one is never near a zero (at x = pi, j_0 = 0 while
+// j_1 = 1 / pi), and every j_p keeps its full relative accuracy.
 TG_HD void tg_bessel_j_seq(int last, double x, double *out) {
     if (x < 0.5) {
         for (int p = 0; p <= last; ++p) out[p] = tg_sph_j_series(p, x);
@@ -48,8 +52,11 @@ TG_HD void tg_bessel_j_seq(int last, double x, double *out) {
         r = x / ((double)(2 * p + 1) - x * r);
         if (p <= last) ratios[p] = r;
     }
-    out[0] = sin(x) / x;
-    for (int p = 1; p <= last; ++p) out[p] = out[p - 1] * ratios[p];
+    const double s = sin(x), j0 = s / x, j1 = (j0 - cos(x)) / x;
+    out[0] = j0;
+    if (last < 1) return;
+    out[1] = fabs(j0) >= fabs(j1) ? j0 * ratios[1] : j1;
+    for (int p = 2; p <= last; ++p) out[p] = out[p - 1] * ratios[p];
 }
 
 // ------------------------------------------------------------------ dual numbers --
@@ -106,8 +113,13 @@ TG_HD int tg_mie_bh(int lmax, double xv, cd eps, tg_dual *a, tg_dual *b) {
     tg_dual psi_prev(cd(s), cd(c), cd(0.0));                   // psi_0 = sin x
     tg_dual xi_prev(cd(s, -c), cd(c, s), cd(0.0));             // xi_0 = -i e^{ix}
     tg_dual xi_prev2(cd(c, s), cd(-s, c), cd(0.0));            // xi_-1 = e^{ix}
+    // psi_1 = sin x / x - cos x in closed form where it exceeds psi_0 (near x = k pi
+    // the ratio chain would scale a vanishing psi_0)
+    const double p1 = s / xv - c;
+    const bool psi1_closed = fabs(p1) > fabs(s);
+    const tg_dual psi1(cd(p1), cd(c / xv - s / (xv * xv) + s), cd(0.0));
     for (int n = 1; n <= lmax; ++n) {
-        tg_dual psi = psi_prev * ratio[n];
+        tg_dual psi = (n == 1 && psi1_closed) ? psi1 : psi_prev * ratio[n];
         tg_dual xi = ((double)(2 * n - 1) / x) * xi_prev - xi_prev2;
         tg_dual nx = (double)n / x;
         tg_dual ga = D[n] / m + nx, gb = m * D[n] + nx;
