@@ -153,6 +153,18 @@ int tg_translation_blocks(int lmax, double k, int64_t count, const double *displ
                           tg_c128 *out_dev, void *stream);
 
 /*
+ * tg_translation_blocks without any device allocation: the same kernel and output,
+ * with the plan staged in a fixed 4 MiB device array of the library (every lmax up
+ * to 12 fits; a plan that would not is rejected with TG_INVALID_INPUT) instead of a
+ * hipMalloc'ed buffer.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Every call on one GPU shares that GPU's array,
+ * so calls that overlap on one GPU race; the call returns after its work is done.
+ */
+int tg_translation_blocks_noalloc(int lmax, double k, int64_t count, const double *displacements_dev, int regular,
+                                  tg_c128 *out_dev, void *stream);
+
+/*
  * tg_translation_blocks between bases of different degree: destination degrees up to
  * lmax_dst, source degrees up to lmax_src (lmax_dst + lmax_src <= 24); each block is
  * (2 Hd) x (2 Hs) column-major, destination inner, Hd = lmax_dst (lmax_dst + 2).

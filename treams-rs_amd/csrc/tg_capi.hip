@@ -551,6 +551,96 @@ int tg_translation_blocks(int lmax, double k, int64_t count, const double *disp,
     return st;
 }
 
+}  // extern "C"
+
+// ----------------------------------------------------- allocation-free blocks --
+// tg_translation_blocks_noalloc keeps its plan in `scratch`, a fixed device array of
+// every GPU the library runs on, instead of a device allocation: the compact plan
+// (starts, nsame, weights, harmonic indices, reflection factors) of lmax <= 12 needs
+// at most 2.95 MiB.  In host code `scratch` names a host-side stand-in, never device
+// memory: its device address comes from hipGetSymbolAddress, once per device, and
+// only that address is used.  Every call on a GPU overwrites that GPU's copy, so
+// calls that overlap on one GPU race.
+static constexpr size_t kScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of the current device's `scratch`, looked up on first use.
+int scratch_address(void **out) {
+    int dev = 0;
+    TG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices) {
+        set_error("device ordinal %d beyond the %d devices the scratch table holds", dev, kScratchDevices);
+        return TG_INVALID_INPUT;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+            return TG_DEVICE_ERROR;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int tg_translation_blocks_noalloc(int lmax, double k, int64_t count, const double *disp, int regular, tg_c128 *out,
+                                  void *stream) {
+    HostPlan hp;
+    if (lmax < 1 || lmax > kMaxLmax || cached_plan(lmax, hp) || !std::isfinite(k) || count < 0 ||
+        (count > 0 && (!disp || !out))) {
+        set_error("invalid lmax (1..%d), wavenumber, count or buffers", kMaxLmax);
+        return TG_INVALID_INPUT;
+    }
+    const size_t HH = (size_t)hp.H * hp.H, nterms = hp.w.size();
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_st = 0, o_ns = al(sizeof(int32_t) * (HH + 1)), o_w = al(o_ns + HH);
+    const size_t o_h = al(o_w + sizeof(double) * nterms), o_r = al(o_h + sizeof(uint16_t) * nterms);
+    const size_t total = al(o_r + sizeof(double) * hp.table_len);
+    if (total > kScratchBytes) {
+        set_error("the plan of lmax %d needs %zu bytes of scratch; the allocation-free path holds %zu", lmax, total,
+                  kScratchBytes);
+        return TG_INVALID_INPUT;
+    }
+    void *base = nullptr;
+    TG_TRY(scratch_address(&base));
+    std::vector<char> host(total, 0);
+    memcpy(host.data() + o_st, hp.starts.data(), sizeof(int32_t) * (HH + 1));
+    memcpy(host.data() + o_ns, hp.nsame.data(), HH);
+    memcpy(host.data() + o_w, hp.w.data(), sizeof(double) * nterms);
+    memcpy(host.data() + o_h, hp.hidx.data(), sizeof(uint16_t) * nterms);
+    memcpy(host.data() + o_r, hp.refl.data(), sizeof(double) * hp.table_len);
+    hipStream_t s = (hipStream_t)stream;
+    char *b = (char *)base;
+    TG_HIP(hipMemcpyAsync(b, host.data(), total, hipMemcpyHostToDevice, s));
+    DevPlan dp{};
+    dp.H = hp.H;
+    dp.L2 = hp.L2;
+    dp.T = hp.table_len;
+    dp.starts = (const int32_t *)(b + o_st);
+    dp.nsame = (const uint8_t *)(b + o_ns);
+    dp.w = (const double *)(b + o_w);
+    dp.hidx = (const uint16_t *)(b + o_h);
+    dp.refl = (const double *)(b + o_r);
+    int st = launch_translation_blocks(lmax, dp, k, count, disp, regular, (cd *)out, s);
+    // the staging vector must outlive the copy
+    hipError_t e = hipStreamSynchronize(s);
+    if (!st && e != hipSuccess) {
+        set_error("%s", hipGetErrorString(e));
+        st = TG_DEVICE_ERROR;
+    }
+    return st;
+}
+
 int tg_translation_blocks_rect(int lmax_dst, int lmax_src, double k, int64_t count, const double *disp, int regular,
                                tg_c128 *out, void *stream) {
     HostPlan hp;

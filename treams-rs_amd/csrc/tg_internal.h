@@ -161,6 +161,8 @@ int launch_translation_blocks_helicity_c64(int lmax, const DevPlan &plan, double
                                           const double *disp, cf *out, hipStream_t s);
 int launch_translation_blocks(int lmax, const DevPlan &plan, double k, int64_t count, const double *disp,
                               int regular, cd *out, hipStream_t s);
+// p[0 .. count) = 0 by a kernel (no memset node)
+int launch_zero_f64(double *p, int64_t count, hipStream_t s);
 // count blocks of (2 Hd) x (2 Hs), column-major, destination inner
 int launch_translation_rect(const RectPlan &pl, double k, int64_t count, const double *disp, int regular, cd *out,
                             hipStream_t s);

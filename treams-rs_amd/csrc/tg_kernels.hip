@@ -423,10 +423,25 @@ int launch_assemble_ref(tg_cluster *c, int regular, cd *outp) {
     return TG_OK;
 }
 
+// Zero `count` doubles with vector stores (a kernel, so code that a caller captures
+// into a graph records no memset node).
+__global__ void k_zero_f64(double *p, int64_t count) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = 0.0;
+}
+
+int launch_zero_f64(double *p, int64_t count, hipStream_t s) {
+    if (count <= 0) return TG_OK;
+    const int64_t blocks = std::min<int64_t>((count + 255) / 256, 4096);
+    k_zero_f64<<<(unsigned)blocks, 256, 0, s>>>(p, count);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
 int launch_translation_blocks(int lmax, const DevPlan &pl, double k, int64_t count, const double *disp, int regular,
                               cd *outp, hipStream_t s) {
     const int M = 2 * lmax * (lmax + 2);
-    TG_HIP(hipMemsetAsync(outp, 0, sizeof(cd) * M * M * count, s));
+    TG_TRY(launch_zero_f64((double *)outp, (int64_t)2 * M * M * count, s));
     if (count == 0) return TG_OK;
     dim3 grid((unsigned)count, 1);
     GeoList geo{disp};
