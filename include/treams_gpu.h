@@ -190,6 +190,15 @@ int tg_cluster_assemble(tg_cluster *cluster, int storage);
 int tg_cluster_release(tg_cluster *cluster);
 
 /*
+ * Mark the stored coupling, the pair tables and the preconditioners stale, keeping
+ * their device memory: the next call that needs them recomputes them in place
+ * (tg_cluster_release frees the memory instead).  For repeated whole jobs on one
+ * cluster without re-allocating a stored coupling of tens of GB each time.  No
+ * reference counterpart (device memory management).
+ */
+int tg_cluster_reset(tg_cluster *cluster);
+
+/*
  * Largest pair-table set the matrix-free operators keep, in bytes (default 64 GiB).
  * One set holds N^2 (2 lmax + 1)^2 c128 values (2.8 GB at N = 1024, lmax = 6); the
  * singular coupling, its c64 copy and the regular overlap each keep one.  Clusters

@@ -59,6 +59,8 @@ def lib():
                                               _c128p, _f64p, _f64p]
         L.or_coupling_apply.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _f64p, ctypes.c_int, _c128p,
                                         _c128p, ctypes.c_int, ctypes.c_int]
+        L.or_pair_apply.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _f64p, ctypes.c_int, _c128p,
+                                    _c128p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_mie.restype = ctypes.c_int
         L.or_mie.argtypes = [ctypes.c_int, ctypes.c_double, _C128, _c128p]
         L.or_mie_pullback.restype = ctypes.c_int
@@ -188,9 +190,10 @@ def pair_matrix_pullback(lmax: int, k0: float, positions, regular: bool, gbar: n
     return grad.reshape(N, 3), float(gk[0])
 
 
-def coupling_apply(lmax: int, k0: float, positions, x: np.ndarray, rows=None) -> np.ndarray:
+def coupling_apply(lmax: int, k0: float, positions, x: np.ndarray, rows=None, regular: bool = False) -> np.ndarray:
     """Matrix-free C x (cluster/iterative.rs:187-221); x is (n,) or (n, cols).  With
-    rows = (i0, i1) only destination particles i0..i1-1 are computed (others zero)."""
+    rows = (i0, i1) only destination particles i0..i1-1 are computed (others zero).
+    regular: the regular overlap R x instead (identity self blocks)."""
     pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
     N = pos.shape[0]
     vec = x.ndim == 1
@@ -198,7 +201,8 @@ def coupling_apply(lmax: int, k0: float, positions, x: np.ndarray, rows=None) ->
     cols = X.shape[0]
     Y = np.zeros_like(X)
     i0, i1 = rows if rows is not None else (0, N)
-    lib().or_coupling_apply(lmax, float(k0), N, pos.reshape(-1), cols, X.reshape(-1), Y.reshape(-1), i0, i1)
+    lib().or_pair_apply(lmax, float(k0), N, pos.reshape(-1), cols, X.reshape(-1), Y.reshape(-1), i0, i1,
+                        int(regular))
     Y = Y.T
     return Y[:, 0] if vec else Y
 
@@ -447,6 +451,153 @@ def gmres(apply, rhs: np.ndarray, rtol=1e-10, atol=0.0, restart=30, max_iteratio
             g[i] = (g[i] - tail) / H[i, i]
             x = x + g[i] * V[i]
         residual = rhs - apply(x)
+
+
+class _Krylov:
+    """One column's restarted Arnoldi state (linalg/gmres.rs:200-347): two MGS passes,
+    complex Givens rotations, convergence only on the true residual."""
+
+    def __init__(self, rhs, rtol, atol, max_iterations):
+        self.rhs = rhs
+        self.rhs_norm = float(np.linalg.norm(rhs))
+        self.tol = max(atol, rtol * self.rhs_norm)
+        self.max_iterations = max_iterations
+        self.x = np.zeros_like(rhs)
+        self.residual = rhs.copy()
+        self.iterations = 0
+        self.rn = self.rhs_norm
+
+    def converged(self):
+        self.rn = float(np.linalg.norm(self.residual))
+        if np.isfinite(self.rn) and self.rn <= self.tol:
+            return True
+        if not np.isfinite(self.rn) or self.iterations >= self.max_iterations:
+            raise NotConverged(f"GMRES did not converge after {self.iterations} iterations: residual {self.rn:e}")
+        return False
+
+    def start(self, restart):
+        self.V = [self.residual / self.rn]
+        self.H = np.zeros((restart + 1, restart), np.complex128)
+        self.rot = []
+        self.g = np.zeros(restart + 1, np.complex128)
+        self.g[0] = self.rn
+        self.used = 0
+
+    def step(self, w, j):
+        """Extend the basis with w = A V[j]; True while the cycle should go on."""
+        H = self.H
+        for _ in range(2):
+            for i in range(j + 1):
+                dot = np.vdot(self.V[i], w)
+                H[i, j] += dot
+                w = w - dot * self.V[i]
+        nxt = float(np.linalg.norm(w))
+        H[j + 1, j] = nxt
+        for i, (c, s) in enumerate(self.rot):
+            a, b = H[i, j], H[i + 1, j]
+            H[i, j] = c * a + s * b
+            H[i + 1, j] = -np.conj(s) * a + c * b
+        a, b = H[j, j], H[j + 1, j]
+        length = float(np.hypot(abs(a), abs(b)))
+        if not np.isfinite(length) or length == 0.0:
+            raise NotConverged("GMRES Arnoldi breakdown before convergence")
+        phase = 1.0 if abs(a) == 0 else a / abs(a)
+        c = abs(a) / length
+        s = phase * np.conj(b) / length
+        H[j, j] = phase * length
+        H[j + 1, j] = 0.0
+        self.rot.append((c, s))
+        self.g[j + 1] = -np.conj(s) * self.g[j]
+        self.g[j] *= c
+        self.used = j + 1
+        self.iterations += 1
+        if abs(self.g[j + 1]) <= self.tol or nxt == 0.0 or self.iterations == self.max_iterations:
+            return False
+        self.V.append(w / nxt)
+        return True
+
+    def update(self):
+        g, H = self.g, self.H
+        for i in range(self.used - 1, -1, -1):
+            tail = sum(H[i, k] * g[k] for k in range(i + 1, self.used))
+            g[i] = (g[i] - tail) / H[i, i]
+            self.x = self.x + g[i] * self.V[i]
+
+
+def gmres_batch(apply_columns, rhs: np.ndarray, rtol=1e-10, atol=0.0, restart=30, max_iterations=300):
+    """Independent GMRES solves of the columns of rhs (n x cols) that share operator
+    applications, restating linalg/gmres.rs:119-185: each column keeps its own Krylov
+    basis, rotations and true-residual certificate; apply_columns(X) multiplies the
+    n x k block of the columns that still need that application."""
+    n, cols = rhs.shape
+    restart = min(restart, n, max_iterations)
+    states = [_Krylov(rhs[:, c].copy(), rtol, atol, max_iterations) for c in range(cols)]
+    while True:
+        cycle = [c for c, s in enumerate(states) if not s.converged()]
+        if not cycle:
+            break
+        for c in cycle:
+            states[c].start(restart)
+        extending = list(cycle)
+        for j in range(restart):
+            W = apply_columns(np.stack([states[c].V[j] for c in extending], axis=1))
+            extending = [c for k, c in enumerate(extending) if states[c].step(W[:, k].copy(), j)]
+            if not extending:
+                break
+        for c in cycle:
+            states[c].update()
+        AX = apply_columns(np.stack([states[c].x for c in cycle], axis=1))
+        for k, c in enumerate(cycle):
+            states[c].residual = states[c].rhs - AX[:, k]
+    X = np.stack([s.x for s in states], axis=1)
+    return X, [Convergence(s.iterations, s.rn, s.rhs_norm) for s in states]
+
+
+class MatrixFreeCluster:
+    """Matrix-free FP64 oracle of a sphere cluster too large for dense matrices
+    (cluster/iterative.rs:53-300): (I - T C) x with the per-pair plan application
+    (or_pair_apply), T from the Mie blocks, batched GMRES, and the cross sections with
+    the regular overlap applied matrix-free (python/treams_rs/_framework_tmatrix.py:
+    231-289)."""
+
+    def __init__(self, lmax, k0, radii, eps, positions):
+        self.lmax = int(lmax)
+        self.k0 = float(k0)
+        self.radii = np.asarray(radii, np.float64).copy()
+        self.eps = np.broadcast_to(np.asarray(eps, np.complex128), self.radii.shape).copy()
+        self.positions = np.asarray(positions, np.float64).reshape(-1, 3).copy()
+        self.N = self.radii.size
+        self.M = 2 * self.lmax * (self.lmax + 2)
+        self.n = self.N * self.M
+        self.blocks = tmatrix_blocks(self.lmax, self.k0, self.radii, self.eps)
+
+    def coupling(self, x, rows=None):
+        return coupling_apply(self.lmax, self.k0, self.positions, x, rows=rows)
+
+    def overlap(self, x, rows=None):
+        return coupling_apply(self.lmax, self.k0, self.positions, x, rows=rows, regular=True)
+
+    def apply(self, x):
+        """(I - T C) x, x of shape (n,) or (n, cols)."""
+        return x - apply_local(self.blocks, self.coupling(x), False)
+
+    def rhs(self, incident):
+        return apply_local(self.blocks, incident, False)
+
+    def solve(self, incident, rtol=1e-10, atol=0.0, restart=30, max_iterations=300):
+        """X = (I - T C)^{-1} T A by batched GMRES (cluster/iterative.rs:264-284)."""
+        A = incident.reshape(self.n, -1)
+        return gmres_batch(self.apply, self.rhs(A), rtol, atol, restart, max_iterations)
+
+    def cross_sections(self, incident, x, flux=0.5):
+        """(sca, ext) per column, with weights 1/k^2 and R x matrix-free."""
+        X = x.reshape(self.n, -1)
+        A = incident.reshape(self.n, -1)
+        w = 1.0 / self.k0 ** 2
+        RX = self.overlap(X)
+        sca = np.real(np.sum(np.conj(X) * RX, axis=0)) * w * 0.5 / flux
+        ext = -np.real(np.sum(np.conj(A) * X, axis=0)) * w * 0.5 / flux
+        return sca, ext
 
 
 # ----------------------------------------------------------------------- cluster

@@ -665,12 +665,15 @@ EXPORT void or_pair_matrix_pullback(int lmax, double k0, int N, const double *po
 /*
  * Matrix-free y_i = sum_{j != i} C_ij x_j for `cols` column-major vectors, the
  * per-pair plan application of cluster/iterative.rs:187-221 with sw/plan.rs:327-360
- * (harmonic table once per pair, shared by the columns).  Only destination
- * particles in [i0, i1) are computed (a bounded sample for timing); OpenMP over
- * destinations, each thread owning its rows.
+ * (harmonic table once per pair, shared by the columns).  With regular != 0 the
+ * regular overlap R instead (j_p radial functions, identity self blocks:
+ * sw/expansion.rs:116-130 with Radial::Regular), the R x of the cross sections
+ * (python/treams_rs/_tmatrix.py:404-411).  Only destination particles in [i0, i1)
+ * are computed (a bounded sample for timing); OpenMP over destinations, each thread
+ * owning its rows.
  */
-EXPORT void or_coupling_apply(int lmax, double k0, int N, const double *positions, int cols, const cplx *x, cplx *y,
-                              int i0, int i1) {
+EXPORT void or_pair_apply(int lmax, double k0, int N, const double *positions, int cols, const cplx *x, cplx *y,
+                          int i0, int i1, int regular) {
     Plan plan = plan_new(lmax);
     int M = plan.M;
     int64_t n = (int64_t)N * M;
@@ -680,12 +683,13 @@ EXPORT void or_coupling_apply(int lmax, double k0, int N, const double *position
 #pragma omp for schedule(dynamic, 1)
         for (int i = i0; i < i1; ++i) {
             for (int c = 0; c < cols; ++c)
-                for (int r = 0; r < M; ++r) y[(int64_t)c * n + (int64_t)i * M + r] = 0.0;
+                for (int r = 0; r < M; ++r)
+                    y[(int64_t)c * n + (int64_t)i * M + r] = regular ? x[(int64_t)c * n + (int64_t)i * M + r] : 0.0;
             for (int j = 0; j < N; ++j) {
                 if (j == i) continue;
                 double d[3];
                 for (int a = 0; a < 3; ++a) d[a] = positions[3 * i + a] - positions[3 * j + a];
-                plan_values(&plan, k0, d, 0, table);
+                plan_values(&plan, k0, d, regular, table);
                 for (int s = 0; s < M; ++s)
                     for (int dd = 0; dd < M; ++dd) {
                         int e = s * M + dd;
@@ -701,6 +705,12 @@ EXPORT void or_coupling_apply(int lmax, double k0, int N, const double *position
         free(table);
     }
     plan_free(&plan);
+}
+
+/* Matrix-free C x, cluster/iterative.rs:187-221 (self blocks zero). */
+EXPORT void or_coupling_apply(int lmax, double k0, int N, const double *positions, int cols, const cplx *x, cplx *y,
+                              int i0, int i1) {
+    or_pair_apply(lmax, k0, N, positions, cols, x, y, i0, i1, 0);
 }
 
 /* --------------------------------------------------------------------- Mie -- */

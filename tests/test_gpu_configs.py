@@ -1,0 +1,164 @@
+"""The benchmark configurations at their full sizes, checked by the FP64 oracle.
+
+* N = 1024, lmax = 6 (98,304 unknowns; the headline): FP64 and c64-precision coupling
+  applies on destination-sphere rows against the oracle's per-pair plan application;
+  the headline solve (learned preconditioner, mixed precision, GMRES(50) to 1e-8)
+  certified by a full oracle FP64 apply; its cross sections against an FP64 solve to
+  1e-12 whose residual the oracle certifies as well.
+* N = 512, lmax = 4 (24,576 unknowns): c64 LU with FP64 refinement against c128 LU,
+  both residuals measured by the oracle.
+* N = 256, lmax = 4 (12,288 unknowns): the scattering cross section's gradient by the
+  LU adjoint against central differences (step 1e-6), and the pullback's translation
+  and scale identities (the reference's cluster/iterative.rs:832-844).
+
+Seed-0 inputs of the benchmark's workload generator.  Tolerances are stated per test.
+"""
+import numpy as np
+import pytest
+import torch
+
+from __graft_entry__ import load_package
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    p = load_package()
+    assert torch.cuda.is_available()
+    return p
+
+
+def _rhs(cl):
+    return torch.stack([cl.plane_wave([0, 0, 1], 1), cl.plane_wave([0, 0, 1], 0), cl.plane_wave([0, 0, -1], 1),
+                        cl.plane_wave([0, 0, -1], 0)], dim=1)
+
+
+@pytest.fixture(scope="module")
+def config5(pkg, oracle):
+    lmax, radii, eps, pos = pkg.workload("n1024_l6", seed=0)
+    cl = pkg.IterativeSphereCluster(lmax, 1.3, radii, eps, pos)
+    mf = oracle.MatrixFreeCluster(lmax, 1.3, radii, eps, pos)
+    yield cl, mf
+    cl.release()
+
+
+def test_config5_coupling_rows_against_oracle(config5):
+    cl, mf = config5
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(cl.n, 4)) + 1j * rng.normal(size=(cl.n, 4))
+    got64 = cl.coupling(x).cpu().numpy()
+    got16 = cl.coupling(x, c64=True).cpu().numpy()
+    M = cl.modes_per_sphere
+    for i0, i1 in ((0, 2), (511, 513), (1022, 1024)):
+        print(f"config5 rows {i0}..{i1 - 1}", flush=True)
+        ref = mf.coupling(x, rows=(i0, i1))[i0 * M:i1 * M]
+        scale = np.linalg.norm(ref)
+        assert np.linalg.norm(got64[i0 * M:i1 * M] - ref) <= 1e-12 * scale
+        # split-FP16 inner operator: 22 significant bits per value, FP32 accumulation
+        assert np.linalg.norm(got16[i0 * M:i1 * M] - ref) <= 1e-6 * scale
+
+
+def test_config5_headline_solve_certified_by_oracle(pkg, config5):
+    cl, mf = config5
+    import treams_rs_amd.learned as learned
+
+    learned.attach(cl, learned.MLP.load(learned.DEFAULT_WEIGHTS))
+    B = _rhs(cl)
+    x, conv = cl.solve(B, rtol=1e-8, restart=50, max_iterations=2000, mixed_precision="matrix_free",
+                       preconditioner="learned", convergence="full")
+    print(f"config5 headline: iterations {[c.iterations for c in conv]}", flush=True)
+    Bh, xh = B.cpu().numpy(), x.cpu().numpy()
+    b = mf.rhs(Bh)
+    r = b - mf.apply(xh)  # one full FP64 oracle apply (4 columns)
+    for c in range(4):
+        rel = np.linalg.norm(r[:, c]) / np.linalg.norm(b[:, c])
+        assert rel <= 1e-8, (c, rel)
+        assert abs(rel - conv[c].residual_norm / conv[c].rhs_norm) <= 1e-12, c
+    # an FP64 solve to 1e-12, certified by the oracle, as the reference answer
+    x64, conv64 = cl.solve(B, rtol=1e-12, restart=50, max_iterations=2000, preconditioner="block_jacobi")
+    x64h = x64.cpu().numpy()
+    r64 = b - mf.apply(x64h)
+    for c in range(4):
+        assert np.linalg.norm(r64[:, c]) <= 1.1e-12 * np.linalg.norm(b[:, c]), c
+    assert np.linalg.norm(xh - x64h) <= 1e-7 * np.linalg.norm(x64h)
+    for c in range(4):
+        s1, e1 = cl.cross_sections(B[:, c], x[:, c])
+        s2, e2 = cl.cross_sections(B[:, c], x64[:, c])
+        assert abs(s1 - s2) <= 1e-8 * abs(s2) and abs(e1 - e2) <= 1e-8 * abs(e2), c
+    # the GPU's cross sections of the FP64 answer against the oracle's overlap apply
+    sca, ext = mf.cross_sections(Bh[:, :1], x64h[:, :1])
+    s2, e2 = cl.cross_sections(B[:, 0], x64[:, 0])
+    assert abs(s2 - sca[0]) <= 1e-12 * abs(sca[0]) and abs(e2 - ext[0]) <= 1e-12 * abs(ext[0])
+
+
+def test_config2_c64_lu_against_c128_lu(pkg, oracle):
+    lmax, radii, eps, pos = pkg.workload("n512_l4", seed=0)
+    cl = pkg.IterativeSphereCluster(lmax, 1.3, radii, eps, pos)
+    mf = oracle.MatrixFreeCluster(lmax, 1.3, radii, eps, pos)
+    B = _rhs(cl)
+    b = mf.rhs(B.cpu().numpy())
+    xs = {}
+    for prec in ("c64", "c128"):
+        print(f"config2 LU {prec}", flush=True)
+        f = pkg.SphereClusterFactor(cl, prec, rtol=1e-12)
+        x, conv = f.solve(B)
+        f.close()
+        xs[prec] = x.cpu().numpy()
+        r = b - mf.apply(xs[prec])
+        for c in range(4):
+            assert np.linalg.norm(r[:, c]) <= 1.1e-12 * np.linalg.norm(b[:, c]), (prec, c)
+    assert np.linalg.norm(xs["c64"] - xs["c128"]) <= 1e-10 * np.linalg.norm(xs["c128"])
+
+
+def _sigma(pkg, lmax, radii, eps, pos):
+    f = pkg.sphere_cluster_factor(lmax, 1.3, radii, eps, pos)
+    cl = f.cluster
+    a = cl.plane_wave([0, 0, 1], 1)
+    x, _ = f.solve(a)
+    s, _ = cl.cross_sections(a, x)
+    f.close()
+    return s
+
+
+def test_config3_gradient_against_central_differences(pkg):
+    lmax, radii, eps, pos = pkg.workload("n256_l4_grad", seed=0)
+    f = pkg.sphere_cluster_factor(lmax, 1.3, radii, eps, pos)
+    sca, gr, gp, info = pkg.scattering_gradient(f, [0, 0, 1], 1)
+    f.close()
+    # sigma_sca here is 2 Re(x^H R x) / k^2 with flux 0.5 (the pullback's objective)
+    s0 = _sigma(pkg, lmax, radii, eps, pos)
+    assert abs(s0 - sca) <= 1e-12 * abs(sca)
+    h = 1e-6
+    for i in (0, 97, 255):
+        print(f"config3 FD radius {i}", flush=True)
+        rp, rm = radii.copy(), radii.copy()
+        rp[i] += h
+        rm[i] -= h
+        fd = (_sigma(pkg, lmax, rp, eps, pos) - _sigma(pkg, lmax, rm, eps, pos)) / (2 * h)
+        assert abs(fd - gr[i]) <= 1e-6 * np.abs(gr).max(), (i, fd, gr[i])
+    for i, axis in ((3, 0), (128, 1), (200, 2)):
+        print(f"config3 FD position {i} axis {axis}", flush=True)
+        pp, pm = pos.copy(), pos.copy()
+        pp[i, axis] += h
+        pm[i, axis] -= h
+        fd = (_sigma(pkg, lmax, radii, eps, pp) - _sigma(pkg, lmax, radii, eps, pm)) / (2 * h)
+        assert abs(fd - gp[i, axis]) <= 1e-6 * np.abs(gp).max(), (i, axis, fd, gp[i, axis])
+
+
+def test_config3_pullback_translation_and_scale_identities(pkg):
+    lmax, radii, eps, pos = pkg.workload("n256_l4_grad", seed=0)
+    f = pkg.sphere_cluster_factor(lmax, 1.3, radii, eps, pos)
+    cl = f.cluster
+    a = _rhs(cl)
+    x, ctx, _ = f.record(a)
+    rng = np.random.default_rng(3)
+    G = torch.tensor(rng.normal(size=(cl.n, 4)) + 1j * rng.normal(size=(cl.n, 4)), device=x.device)
+    gk, gr, ge, gp, gi, _ = ctx.pullback(G)
+    f.close()
+    # loss Re<G, X> with the incident field held fixed: invariant under a rigid
+    # translation, and under r -> s r, pos -> s pos, k0 -> k0 / s
+    scale = np.abs(gp).max()
+    assert np.abs(gp.sum(axis=0)).max() <= 1e-10 * scale * np.sqrt(cl.N)
+    terms = np.array([radii @ gr, np.sum(pos * gp), -1.3 * gk])
+    assert abs(terms.sum()) <= 1e-10 * np.abs(terms).max()

@@ -20,6 +20,8 @@ forms, and exact symmetries.
 | Cluster.solve / cross_sections  | docs/examples/output/cluster.txt |
 | Cluster.scattering_gradient     | docs/examples/output/reverse_cluster_gradient.txt |
 | gmres                           | dense LAPACK solve |
+| gmres_batch / MatrixFreeCluster | dense LAPACK solve (4 columns), dense R and cross sections |
+| or_pair_apply (regular)         | dense pair matrix R (or_pair_matrix), identity self blocks |
 """
 import json
 import math
@@ -336,6 +338,32 @@ def test_gmres_matches_dense(oracle):
     x_it, conv = cl.solve_gmres(a, rtol=1e-12)
     assert conv.residual_norm <= 1e-12 * conv.rhs_norm
     assert np.linalg.norm(x_it - x_dense) <= 1e-10 * np.linalg.norm(x_dense)
+
+
+def test_batched_gmres_and_matrix_free_cluster_match_dense(oracle):
+    rng = np.random.default_rng(4)
+    N, lmax = 9, 3
+    pos = rng.uniform(-2.0, 2.0, size=(N, 3))
+    pos[:, 0] += 1.6 * np.arange(N)
+    radii = rng.uniform(0.2, 0.3, N)
+    eps = 4 + 0.1j
+    dense = oracle.Cluster(lmax, 1.3, radii, eps, pos)
+    mf = oracle.MatrixFreeCluster(lmax, 1.3, radii, eps, pos)
+    a = np.stack([dense.plane_wave(d, p) for d, p in (([0, 0, 1], 1), ([0, 0, -1], 0), ([1, 0, 0], 1),
+                                                        ([0, 1, 1], 0))], axis=1)
+    x_dense = np.linalg.solve(dense.operator(), oracle.apply_local(dense.blocks, a, False))
+    X, conv = mf.solve(a, rtol=1e-12, restart=10)
+    for c in conv:
+        assert c.residual_norm <= 1e-12 * c.rhs_norm
+    assert np.linalg.norm(X - x_dense) <= 1e-10 * np.linalg.norm(x_dense)
+    # the batch changes when products run, not the arithmetic of a column
+    x0, c0 = dense.solve_gmres(a[:, 1], rtol=1e-12, restart=10)
+    assert conv[1].iterations == c0.iterations
+    assert np.linalg.norm(mf.overlap(X) - dense.R @ X) <= 1e-14 * np.linalg.norm(dense.R @ X)
+    sca, ext = mf.cross_sections(a, X)
+    for c in range(4):
+        s_ref, e_ref = dense.cross_sections(a[:, c], x_dense[:, c])
+        assert abs(sca[c] - s_ref) <= 1e-11 * abs(s_ref) and abs(ext[c] - e_ref) <= 1e-11 * abs(e_ref)
 
 
 def test_gradient_planted_mistake_is_caught(oracle, golden_dir):

@@ -28,7 +28,7 @@ TG_PC_NONE, TG_PC_BLOCK_JACOBI, TG_PC_LEARNED = 0, 1, 2
 EXPORTED = [
     "tg_last_error", "tg_modes_per_sphere", "tg_plan_terms", "tg_wigner3j", "tg_cluster_create", "tg_cluster_create_particles",
     "tg_cluster_destroy", "tg_cluster_dimension", "tg_cluster_mie_blocks", "tg_pair_matrix",
-    "tg_translation_blocks", "tg_translation_blocks_noalloc", "tg_cluster_assemble", "tg_cluster_release", "tg_cluster_apply", "tg_cluster_coupling",
+    "tg_translation_blocks", "tg_translation_blocks_noalloc", "tg_cluster_assemble", "tg_cluster_release", "tg_cluster_reset", "tg_cluster_apply", "tg_cluster_coupling",
     "tg_cluster_local", "tg_cluster_overlap", "tg_cluster_plane_wave", "tg_cluster_solve", "tg_cluster_cross_sections",
     "tg_cluster_pullback", "tg_cluster_overlap_pullback", "tg_cluster_timing_enable",
     "tg_cluster_timing_get", "tg_cluster_timing_reset", "tg_cluster_set_group_size",
@@ -98,6 +98,7 @@ def lib():
         "tg_translation_blocks_noalloc": (_i, [_i, _d, _i64, _vp, _i, _vp, _vp]),
         "tg_cluster_assemble": (_i, [_vp, _i]),
         "tg_cluster_release": (_i, [_vp]),
+        "tg_cluster_reset": (_i, [_vp]),
         "tg_cluster_set_table_limit": (_i, [_vp, _i64]),
         "tg_cluster_set_inner_gemm": (_i, [_vp, _i]),
         "tg_cluster_apply": (_i, [_vp, _vp, _vp, _i, _i, _i]),

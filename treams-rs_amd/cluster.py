@@ -171,6 +171,11 @@ class IterativeSphereCluster:
         demand)."""
         check(lib().tg_cluster_release(self._h))
 
+    def reset(self) -> None:
+        """Mark the stored coupling, the pair tables and the preconditioner stale while
+        keeping their device memory: the next solve recomputes them in place."""
+        check(lib().tg_cluster_reset(self._h))
+
     # -------------------------------------------------------------- operators
     def _apply(self, fn, x, *args):
         cm, vector = _columns(x, self.n)

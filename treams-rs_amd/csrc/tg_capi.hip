@@ -699,6 +699,20 @@ int tg_cluster_release(tg_cluster *c) {
     return TG_OK;
 }
 
+int tg_cluster_reset(tg_cluster *c) {
+    TG_TRY(check_cluster(c));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    destroy_preconditioner(c);
+    learned_release(c);
+    c->storage = TG_STORE_NONE;
+    c->have_tables = false;
+    c->have_tables32 = false;
+    c->have_tablesh = false;
+    c->have_texpo = false;
+    c->have_tablesR = false;
+    return TG_OK;
+}
+
 int tg_cluster_set_inner_gemm(tg_cluster *c, int kind) {
     TG_TRY(check_cluster(c));
     if (kind != TG_INNER_FP32 && kind != TG_INNER_SPLIT16) {
