@@ -153,6 +153,17 @@ int tg_translation_blocks(int lmax, double k, int64_t count, const double *displ
                           tg_c128 *out_dev, void *stream);
 
 /*
+ * Directional derivatives of tg_translation_blocks: for each displacement d
+ * (device double[3 count]) with tangent dd (device double[3 count]) and a wavenumber
+ * tangent dk, the M x M block sum_a dB/dd_a dd_a + dB/dk dk (column-major c128).
+ * Replaces TranslationPlan::pushforward / tangent_table (sw/plan.rs:552-595); the
+ * wavenumber and its tangent are real here.
+ */
+int tg_translation_blocks_tangent(int lmax, double k, int64_t count, const double *displacements_dev,
+                                  const double *tangent_displacements_dev, double tangent_k, int regular,
+                                  tg_c128 *out_dev, void *stream);
+
+/*
  * tg_translation_blocks without any device allocation: the same kernel and output,
  * with the plan staged in a fixed 4 MiB device array of the library (every lmax up
  * to 12 fits; a plan that would not is rejected with TG_INVALID_INPUT) instead of a
@@ -278,6 +289,21 @@ int tg_cluster_pullback(tg_cluster *cluster, const tg_c128 *incident_dev, const 
                         tg_convergence *conv_host);
 
 /*
+ * Directional derivative of a solved X = (I - T C)^{-1} T B (device, n x cols,
+ * reference layout) along tangents of k0, the radii[N], permittivities[N] and
+ * positions[3N] (host arrays; NULL means zero) and of the incident columns B (device
+ * n x cols; NULL means zero): solves (I - T C) dX = dT (B + C X) + T (dB + dC X) with
+ * GMRES under `options` (relative tolerance only, 1e-10 when rtol is 0).  dC X runs as
+ * the pair-table GEMMs on the tables' tangents, so one FP64 table set must fit the
+ * table limit.  Replaces IterativeResidual::pushforward (cluster/iterative.rs:362-447).
+ * dx_dev: n x cols; conv_host: cols entries (the tangent solves' certificates).
+ */
+int tg_cluster_pushforward(tg_cluster *cluster, const tg_c128 *incident_dev, const tg_c128 *x_dev, int cols,
+                           const tg_gmres_options *options, double tangent_k0, const double *tangent_radii,
+                           const tg_c128 *tangent_epsilon, const double *tangent_positions,
+                           const tg_c128 *tangent_incident_dev, tg_c128 *dx_dev, tg_convergence *conv_host);
+
+/*
  * Position gradients of Re<Gbar, R> for the regular overlap with cotangent
  * Gbar = sum_c u_c v_c^H (device c128 n x cols each): the R-dependence of the
  * scattering cross section (sw/expansion.rs ExpansionResidual::pullback).
@@ -344,6 +370,18 @@ int tg_factor_pullback(tg_factor *factor, const tg_c128 *incident_dev, const tg_
                        const tg_c128 *cotangent_dev, int cols, double rtol, double *grad_k0, double *grad_radii,
                        tg_c128 *grad_eps, double *grad_positions, tg_c128 *incident_grad_dev,
                        tg_convergence *conv_host);
+
+/*
+ * tg_cluster_pushforward with the tangent solve by the LU factors (one FP64-refined
+ * solve to rtol): replaces SphereClusterResidual::pushforward (cluster/spheres.rs:263-300)
+ * and IlluminateResidual::pushforward (cluster/interaction.rs:464-478) for physical
+ * tangents; with B = I (n columns) it is the tangent of the interacting T-matrix,
+ * InteractionResidual::pushforward_blocks (cluster/interaction.rs:359-368).
+ */
+int tg_factor_pushforward(tg_factor *factor, const tg_c128 *incident_dev, const tg_c128 *x_dev, int cols,
+                          double rtol, double tangent_k0, const double *tangent_radii, const tg_c128 *tangent_epsilon,
+                          const double *tangent_positions, const tg_c128 *tangent_incident_dev, tg_c128 *dx_dev,
+                          tg_convergence *conv_host);
 
 /* ---------------------------------------------------------------------------
  * Fields

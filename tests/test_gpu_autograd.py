@@ -97,3 +97,34 @@ def test_adapter_matches_library_pullback(pkg, ag):
     g1 = r.grad.clone()
     sca.backward()
     assert torch.allclose(r.grad, 2 * g1, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("method", ["lu", "gmres"])
+def test_forward_mode_matches_reverse_mode(pkg, ag, method):
+    """torch forward mode (the record's pushforward) against reverse mode (its
+    pullback): Re<G, J t> = <J^T G, t> through the plane wave and the solve."""
+    import torch.autograd.forward_ad as fwAD
+
+    radii, eps, pos = _inputs(pkg, seed=11, N=5)
+    lmax, k0 = 3, 1.3
+    rng = np.random.default_rng(0)
+    tr = torch.tensor(rng.normal(size=5) * 0.05)
+    tp = torch.tensor(rng.normal(size=(5, 3)))
+    opts = {"rtol": 1e-13}
+    with fwAD.dual_level():
+        r = fwAD.make_dual(torch.tensor(radii), tr)
+        p = fwAD.make_dual(torch.tensor(pos), tp)
+        c = ag.TorchCluster(lmax, k0, r, torch.tensor(eps), p)
+        a = c.plane_wave([0, 0, 1], 1)
+        x = c.solve(a, method=method, **opts)
+        dx = fwAD.unpack_dual(x).tangent
+    assert dx is not None
+    G = torch.tensor(rng.normal(size=dx.shape[0]) + 1j * rng.normal(size=dx.shape[0]), device=dx.device)
+    r2 = torch.tensor(radii, requires_grad=True)
+    p2 = torch.tensor(pos, requires_grad=True)
+    c2 = ag.TorchCluster(lmax, k0, r2, torch.tensor(eps), p2)
+    x2 = c2.solve(c2.plane_wave([0, 0, 1], 1), method=method, **opts)
+    torch.real(torch.vdot(G, x2)).backward()
+    lhs = float(torch.real(torch.vdot(G, dx)))
+    rhs = float(r2.grad @ tr + torch.sum(p2.grad * tp))
+    assert abs(lhs - rhs) <= 1e-9 * float(torch.linalg.norm(G) * torch.linalg.norm(dx))

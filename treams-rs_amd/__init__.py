@@ -21,6 +21,7 @@ from .cluster import (  # noqa: F401
     scattering_gradient,
     sphere_cluster_factor,
     translation_blocks,
+    translation_blocks_tangent,
     translation_blocks_rect,
     wigner3j,
 )

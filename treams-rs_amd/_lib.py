@@ -38,7 +38,8 @@ EXPORTED = [
     "tg_mlp_get_params", "tg_mlp_set_params", "tg_mlp_get_grads", "tg_mlp_zero_grad", "tg_mlp_forward",
     "tg_mlp_backward", "tg_mlp_adam_step", "tg_factor_create", "tg_factor_destroy", "tg_factor_solve",
     "tg_factor_pullback", "tg_factor_equilibrated", "tg_factor_global_tmatrix", "tg_cluster_field",
-    "tg_plane_wave_field", "tg_translation_blocks_rect",
+    "tg_plane_wave_field", "tg_translation_blocks_rect", "tg_translation_blocks_tangent", "tg_cluster_pushforward",
+    "tg_factor_pushforward",
 ]
 
 
@@ -141,6 +142,10 @@ def lib():
         "tg_cluster_field": (_i, [_vp, _vp, _i, _i64, _vp, _i, _vp]),
         "tg_plane_wave_field": (_i, [_d, _vp, _i, _i64, _vp, _vp, _vp]),
         "tg_factor_pullback": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "tg_translation_blocks_tangent": (_i, [_i, _d, _i64, _vp, _vp, _d, _i, _vp, _vp]),
+        "tg_cluster_pushforward": (_i, [_vp, _vp, _vp, _i, ctypes.POINTER(GmresOptions), _d, _vp, _vp, _vp, _vp,
+                                        _vp, _vp]),
+        "tg_factor_pushforward": (_i, [_vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

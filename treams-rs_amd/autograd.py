@@ -16,7 +16,8 @@ Differentiate a scattering cross section::
     sca.backward()          # radii.grad, positions.grad, epsilon.grad
 
 The saved record keeps the LU factors (or the GMRES options) until the graph is
-freed; a second backward (``retain_graph=True``) reuses them.
+freed; a second backward (``retain_graph=True``) reuses them.  Forward mode
+(``torch.autograd.forward_ad``) runs the record's pushforward: one tangent solve.
 """
 from __future__ import annotations
 
@@ -62,6 +63,18 @@ class _Solve(torch.autograd.Function):
         return x
 
     @staticmethod
+    def jvp(ctx, tr, tp, te, ti, *_):
+        """Forward mode: one tangent solve (the record's pushforward,
+        cluster/iterative.rs:362-447 or cluster/spheres.rs:263-300)."""
+        def host(t, dtype):
+            return None if t is None else _np(t, dtype)
+
+        ti = None if ti is None else ti.detach().to(torch.complex128)
+        dx, _ = ctx.record.pushforward(0.0, host(tr, np.float64), host(te, np.complex128),
+                                       host(tp, np.float64), ti)
+        return dx
+
+    @staticmethod
     def backward(ctx, gx):
         mr, mp, me, mi = ctx.metas
         gk, gr, ge, gp, ginc, _ = ctx.record.pullback(gx.detach().to(torch.complex128))
@@ -82,9 +95,19 @@ class _PlaneWave(torch.autograd.Function):
         d = np.asarray(direction, np.float64)
         ctx.kvec = cluster.handle.k0 * d / np.linalg.norm(d)
         ctx.save_for_backward(a)
+        ctx.primal = a  # for jvp (forward mode reads no saved tensors)
         ctx.N = cluster.handle.N
         ctx.pos_meta = _meta(positions)
         return a
+
+    @staticmethod
+    def jvp(ctx, tp, *_):
+        a = ctx.primal
+        if tp is None:
+            return torch.zeros_like(a)
+        # da_j = i (k . dr_j) a_j
+        kd = tp.detach().to(torch.float64).reshape(ctx.N, 3) @ torch.as_tensor(ctx.kvec, device=tp.device)
+        return (1j * kd.to(a.device)[:, None] * a.reshape(ctx.N, -1)).reshape(a.shape)
 
     @staticmethod
     def backward(ctx, ga):

@@ -295,6 +295,27 @@ class IterativeContext:
         self.x = x
         self.options = dict(options)
 
+    def pushforward(self, k0=0.0, radii=None, epsilon=None, positions=None, incident=None):
+        """Directional derivative of the converged solution along tangents of k0, the
+        radii, permittivities, positions and incident columns (None: zero), by one
+        tangent GMRES solve (cluster/iterative.rs:362-447; python/treams_rs/
+        iterative.py:152-180).  Returns (dX, [(iterations, residual, rhs_norm)])."""
+        c = self.cluster
+        a, vector = _columns(self.incident, c.n)
+        xc, _ = _columns(self.x, c.n)
+        dk, tr, te, tp = _tangents(c, k0, radii, epsilon, positions)
+        ti = _incident_tangent(incident, a, c.n)
+        cols = a.shape[0]
+        out = torch.empty_like(a)
+        conv = (_Conv * cols)()
+        opts = c.options(**{k: v for k, v in self.options.items() if k in (
+            "rtol", "atol", "restart", "max_iterations", "mixed_precision", "preconditioner")})
+        check(lib().tg_cluster_pushforward(c._h, a.data_ptr(), xc.data_ptr(), cols, ctypes.byref(opts), dk,
+                                           tr.ctypes.data, te.ctypes.data, tp.ctypes.data,
+                                           None if ti is None else ti.data_ptr(), out.data_ptr(), conv))
+        reports = [(v.iterations, v.residual_norm, v.rhs_norm) for v in conv]
+        return _back(out, vector), reports
+
     def pullback(self, cotangent):
         """Gradients from the cotangent of X: returns
         (k0, radii[N], eps[N], positions[N, 3], incident, convergence)."""
@@ -316,6 +337,26 @@ class IterativeContext:
                                         inc.data_ptr(), conv))
         reports = [(v.iterations, v.residual_norm, v.rhs_norm) for v in conv]
         return gk.value, gr, ge, gp.reshape(c.N, 3), _back(inc, vector), reports
+
+
+def _tangents(cluster, k0, radii, epsilon, positions):
+    """Host arrays of the physical tangents (zeros where None)."""
+    N = cluster.N
+    tr = np.zeros(N) if radii is None else np.ascontiguousarray(np.broadcast_to(np.asarray(radii, np.float64), (N,)))
+    te = (np.zeros(N, np.complex128) if epsilon is None else
+          np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, np.complex128), (N,))))
+    tp = (np.zeros((N, 3)) if positions is None else
+          np.ascontiguousarray(np.asarray(positions, np.float64).reshape(N, 3)))
+    return float(k0 or 0.0), tr, te, tp
+
+
+def _incident_tangent(incident, like: torch.Tensor, n: int):
+    if incident is None:
+        return None
+    t, _ = _columns(incident, n)
+    if t.shape != like.shape:
+        raise ValueError("incident tangent shape does not match the recorded incident fields")
+    return t
 
 
 class SphereClusterFactor:
@@ -409,6 +450,25 @@ class FactorContext:
         self.factor = factor
         self.incident = torch.as_tensor(incident).to(factor.cluster._dev, torch.complex128)
         self.x = x
+
+    def pushforward(self, k0=0.0, radii=None, epsilon=None, positions=None, incident=None):
+        """Directional derivative of the solution along tangents of k0, the radii,
+        permittivities, positions and incident columns (None: zero), the tangent
+        solve reusing the LU factors (cluster/spheres.rs:263-300,
+        cluster/interaction.rs:464-478).  Returns (dX, [Convergence])."""
+        f, c = self.factor, self.factor.cluster
+        a, vector = _columns(self.incident, c.n)
+        xc, _ = _columns(self.x, c.n)
+        dk, tr, te, tp = _tangents(c, k0, radii, epsilon, positions)
+        ti = _incident_tangent(incident, a, c.n)
+        cols = a.shape[0]
+        out = torch.empty_like(a)
+        conv = (_Conv * cols)()
+        check(lib().tg_factor_pushforward(f._h, a.data_ptr(), xc.data_ptr(), cols, f.rtol, dk, tr.ctypes.data,
+                                          te.ctypes.data, tp.ctypes.data, None if ti is None else ti.data_ptr(),
+                                          out.data_ptr(), conv))
+        reports = [Convergence(0, v.residual_norm, v.rhs_norm, v.refinements) for v in conv]
+        return _back(out, vector), reports
 
     def pullback(self, cotangent):
         """Returns (k0, radii[N], eps[N], positions[N, 3], incident, convergence)."""
@@ -558,6 +618,24 @@ def translation_blocks_rect(lmax_dst: int, lmax_src: int, k: float, displacement
     check(lib().tg_translation_blocks_rect(int(lmax_dst), int(lmax_src), float(k), d.shape[0], d.data_ptr(),
                                            int(regular), out.data_ptr(), _stream_ptr()))
     return out.transpose(1, 2).contiguous()
+
+
+def translation_blocks_tangent(lmax: int, k: float, displacements, tangents, k_tangent: float = 0.0,
+                               regular: bool = False) -> torch.Tensor:
+    """Directional derivatives of the M x M translation blocks along displacement
+    tangents and a wavenumber tangent (TranslationPlan::pushforward, sw/plan.rs:552-595);
+    returns (count, M, M) indexed [destination, source]."""
+    d = torch.as_tensor(np.ascontiguousarray(np.asarray(displacements, np.float64).reshape(-1, 3)),
+                        device=_device())
+    t = torch.as_tensor(np.ascontiguousarray(np.asarray(tangents, np.float64).reshape(-1, 3)), device=_device())
+    if t.shape != d.shape:
+        raise ValueError("one displacement tangent per displacement")
+    M = lib().tg_modes_per_sphere(lmax)
+    out = torch.empty((d.shape[0], M, M), dtype=torch.complex128, device=d.device)
+    check(lib().tg_translation_blocks_tangent(lmax, float(k), d.shape[0], d.data_ptr(), t.data_ptr(),
+                                              float(k_tangent), int(regular), out.data_ptr(),
+                                              ctypes.c_void_p(_stream_ptr())))
+    return out.transpose(1, 2)
 
 
 def translation_blocks(lmax: int, k: float, displacements, regular: bool = False) -> torch.Tensor:

@@ -551,6 +551,32 @@ int tg_translation_blocks(int lmax, double k, int64_t count, const double *disp,
     return st;
 }
 
+int tg_translation_blocks_tangent(int lmax, double k, int64_t count, const double *disp,
+                                  const double *tangent_disp, double tangent_k, int regular, tg_c128 *out,
+                                  void *stream) {
+    HostPlan hp;
+    if (lmax < 1 || lmax > kMaxLmax || cached_plan(lmax, hp) || !std::isfinite(k) || !(k > 0) ||
+        !std::isfinite(tangent_k) || count < 0 || (count > 0 && (!disp || !tangent_disp || !out))) {
+        set_error("invalid lmax, wavenumber, wavenumber tangent, count or buffers");
+        return TG_INVALID_INPUT;
+    }
+    DevBuf buf, tables;
+    DevPlan dp;
+    hipStream_t s = (hipStream_t)stream;
+    int st = upload_plan(hp, buf, dp);
+    if (!st) st = tables.ensure(sizeof(cd) * (size_t)std::max<int64_t>(count, 1) * hp.table_len);
+    if (!st)
+        st = launch_translation_tangent(lmax, dp, k, count, disp, tangent_disp, tangent_k, regular, tables.as<cd>(),
+                                        (cd *)out, s);
+    if (!st && hipStreamSynchronize(s) != hipSuccess) {
+        set_error("device failure in the translation tangents");
+        st = TG_DEVICE_ERROR;
+    }
+    tables.release();
+    buf.release();
+    return st;
+}
+
 }  // extern "C"
 
 // ----------------------------------------------------- allocation-free blocks --

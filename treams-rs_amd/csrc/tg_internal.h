@@ -161,12 +161,25 @@ int launch_translation_blocks_helicity_c64(int lmax, const DevPlan &plan, double
                                           const double *disp, cf *out, hipStream_t s);
 int launch_translation_blocks(int lmax, const DevPlan &plan, double k, int64_t count, const double *disp,
                               int regular, cd *out, hipStream_t s);
+// tangent tables of every ordered pair (FP64, the layout of the singular pair tables)
+// along position tangents dpos_dev (3 N) and a wavenumber tangent dk
+int launch_pair_tangent_tables(tg_cluster *c, const double *dpos_dev, double dk, cd *tab);
+// tangent blocks of a displacement list (tables: scratch of count x T values)
+int launch_translation_tangent(int lmax, const DevPlan &pl, double k, int64_t count, const double *disp,
+                               const double *tdisp, double dk, int regular, cd *tables, cd *out, hipStream_t s);
 // p[0 .. count) = 0 by a kernel (no memset node)
 int launch_zero_f64(double *p, int64_t count, hipStream_t s);
 // count blocks of (2 Hd) x (2 Hs), column-major, destination inner
 int launch_translation_rect(const RectPlan &pl, double k, int64_t count, const double *disp, int regular, cd *out,
                             hipStream_t s);
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular);
+// whether one FP64 pair-table set fits the cluster's table limit
+bool tables_fit(const tg_cluster *c);
+// y = C' x for a caller's FP64 table set TAB[h][j][i] (pair-table GEMMs and the plan
+// contraction of the coupling, e.g. on tangent tables)
+int coupling_tables_f64(tg_cluster *c, const cd *TAB, const cd *x, cd *y, int cols);
+// y = T' x (or T'^H x) with 2x2 blocks [sphere][degree][4] in place of the Mie blocks
+int local_apply_blocks(tg_cluster *c, const cd *blocks, const cd *x, cd *y, int cols, int adjoint, const cd *sub);
 int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols);
 int coupling_stored(tg_cluster *c, const cd *x, cd *y, int cols);
 // split FP16 coupling (tg_split16.hip) from the FP64 singular tables tab64

@@ -724,6 +724,185 @@ __global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd
     for (int q = 0; q <= L2; ++q) table_column_put(q, L2, u, rad, pl.refl, put);
 }
 
+// ------------------------------------------------------------ tangent tables --
+// Directional derivative of the harmonic table z_p(k r) S_p^q(d / r) along a
+// displacement tangent dd and a wavenumber tangent dk (sw/plan.rs:552-595,
+// TranslationPlan::tangent_table): with S the solid harmonic at the unit vector u and
+// g its Cartesian gradient there,
+//   d/dd_a = z_p'(k r) k u_a S + z_p(k r) / r (g_a - p u_a S),   d/dk = z_p'(k r) r S,
+// contracted with (dd, dk).  Negative orders: reflection factor times the conjugate
+// of S and g (z_p is not conjugated).  One order q at a time: the ladder P_q^q, then
+// the degree recurrence of S and of its gradient (r^2 = 1 on the unit sphere, with
+// gradient 2 u), as in the pullback's derivative table.
+struct SolidGrad {
+    cd s, g0, g1, g2;
+};
+
+// z(p) returns z_p and z_{p+1} at x = k r for p = q..L2 (called in ascending p).
+template <class Radial, class Put>
+__device__ __forceinline__ void tangent_column_put(int q, int L2, double k, double r, const double u[3],
+                                                   const double dd[3], double dk, const double *refl,
+                                                   const SolidGrad &ladder, Radial radial, Put put) {
+    const double ud = u[0] * dd[0] + u[1] * dd[1] + u[2] * dd[2];
+    const double inv_x = 1.0 / (k * r);
+    cd s = ladder.s, g0 = ladder.g0, g1 = ladder.g1, g2 = ladder.g2;
+    cd ps(0.0), pg0(0.0), pg1(0.0), pg2(0.0);
+    for (int p = q; p <= L2; ++p) {
+        if (p > q) {
+            const double a = (double)(2 * p - 1) / (double)(p - q);
+            const double b = (double)(p + q - 1) / (double)(p - q);
+            const cd ns = (a * u[2]) * s - b * ps;
+            const cd n0 = a * (u[2] * g0) - b * (pg0 + (2.0 * u[0]) * ps);
+            const cd n1 = a * (u[2] * g1) - b * (pg1 + (2.0 * u[1]) * ps);
+            const cd n2 = a * (u[2] * g2 + s) - b * (pg2 + (2.0 * u[2]) * ps);
+            ps = s; pg0 = g0; pg1 = g1; pg2 = g2;
+            s = ns; g0 = n0; g1 = n1; g2 = n2;
+        }
+        cd zp, zn;
+        radial(p, zp, zn);
+        const cd zd = ((double)p * inv_x) * zp - zn;  // z_p'(x)
+        const cd radial_part = zd * (k * ud + r * dk);
+        const cd ang = zp * (1.0 / r);
+        const cd gd = dd[0] * g0 + dd[1] * g1 + dd[2] * g2;
+        put(p * p + p + q, radial_part * s + ang * (gd - ((double)p * ud) * s));
+        if (q > 0) {
+            const double f = refl[p * p + p - q];
+            const cd sn = f * conj(s), gn = f * conj(gd);
+            put(p * p + p - q, radial_part * sn + ang * (gn - ((double)p * ud) * sn));
+        }
+    }
+}
+
+__device__ __forceinline__ void ladder_step(int q, const double u[3], SolidGrad &l) {
+    const cd xy(u[0], u[1]);
+    const double f = -(double)(2 * q - 1);
+    l.g0 = f * (l.g0 * xy + l.s);
+    l.g1 = f * (l.g1 * xy + l.s * cd(0.0, 1.0));
+    l.g2 = f * (l.g2 * xy);
+    l.s = f * (l.s * xy);
+}
+
+// TAN[h][j][i] = tangent of TAB[h][j][i] for the position tangents dpos (3 N) and dk:
+// dd = dpos_i - dpos_j.  Singular tables (zero for i = j); one thread per (i, j), i
+// fastest; the Hankel upward recurrence runs alongside each order (no indexed array).
+__global__ void k_pair_tangent_tables(DevPlan pl, double k, const double *pos, const double *dpos, double dk, int N,
+                                      cd *tab) {
+    const int L2 = pl.L2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * N) return;
+    const int j = (int)(t / N), i = (int)(t - (int64_t)j * N);
+    const int64_t NN = (int64_t)N * N;
+    const int64_t o = (int64_t)j * N + i;
+    auto put = [&](int h, cd v) { stream_store(tab + (int64_t)h * NN + o, v); };
+    double d[3], r, u[3], dd[3];
+    for (int a = 0; a < 3; ++a) {
+        d[a] = pos[3 * i + a] - pos[3 * j + a];
+        dd[a] = dpos[3 * i + a] - dpos[3 * j + a];
+    }
+    if (i == j || !unit_direction(d, r, u)) {
+        for (int q = 0; q <= L2; ++q) zero_column_put(q, L2, put);
+        return;
+    }
+    const double x = k * r, inv = 1.0 / x, sx = sin(x), cx_ = cos(x);
+    SolidGrad ladder{cd(1.0), cd(0.0), cd(0.0), cd(0.0)};
+    for (int q = 0; q <= L2; ++q) {
+        if (q > 0) ladder_step(q, u, ladder);
+        // h_{p} and h_{p+1} from h_{-1}, h_0 upward, advanced to p = q first
+        cd hprev(cx_ * inv, sx * inv), hcur(sx * inv, -cx_ * inv);
+        for (int p = 0; p < q; ++p) {
+            const cd hn = ((double)(2 * p + 1) * inv) * hcur - hprev;
+            hprev = hcur;
+            hcur = hn;
+        }
+        int at = q;
+        cd hnext = ((double)(2 * q + 1) * inv) * hcur - hprev;
+        auto radial = [&](int p, cd &zp, cd &zn) {
+            while (at < p) {
+                const cd hn = ((double)(2 * at + 3) * inv) * hnext - hcur;
+                hcur = hnext;
+                hnext = hn;
+                ++at;
+            }
+            zp = hcur;
+            zn = hnext;
+        };
+        tangent_column_put(q, L2, k, r, u, dd, dk, pl.refl, ladder, radial, put);
+    }
+}
+
+int launch_pair_tangent_tables(tg_cluster *c, const double *dpos_dev, double dk, cd *tab) {
+    const int64_t NN = (int64_t)c->N * c->N;
+    k_pair_tangent_tables<<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), dpos_dev, dk, c->N,
+                                                            tab);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+// Tangent tables of a displacement list (one thread per displacement, [count][T]):
+// singular or regular radial functions from an array of z_0 .. z_{L2+1}.  A regular
+// table at the origin is constant (identity block), so its tangent is zero there.
+template <bool REG>
+__global__ void k_tangent_tables_list(DevPlan pl, double k, int64_t count, const double *disp, const double *tdisp,
+                                      double dk, cd *tab) {
+    const int L2 = pl.L2, T = pl.T;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    cd *out = tab + e * T;
+    auto put = [&](int h, cd v) { out[h] = v; };
+    double d[3], r, u[3], dd[3];
+    for (int a = 0; a < 3; ++a) {
+        d[a] = disp[3 * e + a];
+        dd[a] = tdisp[3 * e + a];
+    }
+    if (!unit_direction(d, r, u)) {
+        for (int q = 0; q <= L2; ++q) zero_column_put(q, L2, put);
+        return;
+    }
+    cd rad[2 * kMaxLmax + 2];
+    radial_values(L2 + 1, k * r, REG, rad);
+    SolidGrad ladder{cd(1.0), cd(0.0), cd(0.0), cd(0.0)};
+    for (int q = 0; q <= L2; ++q) {
+        if (q > 0) ladder_step(q, u, ladder);
+        auto radial = [&](int p, cd &zp, cd &zn) {
+            zp = rad[p];
+            zn = rad[p + 1];
+        };
+        tangent_column_put(q, L2, k, r, u, dd, dk, pl.refl, ladder, radial, put);
+    }
+}
+
+// M x M blocks from per-displacement tables [count][T] (TranslationPlan::evaluate_table,
+// sw/plan.rs:224-251), one workgroup per block, the table in LDS.
+__global__ __launch_bounds__(256) void k_blocks_from_tables(DevPlan pl, const cd *tab, double2 *out) {
+    extern __shared__ cd tl[];
+    const int T = pl.T, H = pl.H, M = 2 * H;
+    const int64_t e = blockIdx.x;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) tl[t] = tab[e * T + t];
+    __syncthreads();
+    OutBlocks ob{out, M};
+    for (int en = threadIdx.x; en < H * H; en += blockDim.x) {
+        const int a = en / H, b = en - a * H;
+        cd vp, vm;
+        entry_values(pl, en, tl, vp, vm);
+        ob.store(0, (int)e, a, b, vp, vm);
+    }
+}
+
+int launch_translation_tangent(int lmax, const DevPlan &pl, double k, int64_t count, const double *disp,
+                               const double *tdisp, double dk, int regular, cd *tables, cd *outp, hipStream_t s) {
+    const int M = 2 * lmax * (lmax + 2);
+    TG_TRY(launch_zero_f64((double *)outp, (int64_t)2 * M * M * count, s));
+    if (count == 0) return TG_OK;
+    if (regular)
+        k_tangent_tables_list<true><<<grid1(count, 64), 64, 0, s>>>(pl, k, count, disp, tdisp, dk, tables);
+    else
+        k_tangent_tables_list<false><<<grid1(count, 64), 64, 0, s>>>(pl, k, count, disp, tdisp, dk, tables);
+    TG_HIP(hipGetLastError());
+    k_blocks_from_tables<<<(unsigned)count, 256, sizeof(cd) * pl.T, s>>>(pl, tables, (double2 *)outp);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
 // FP64 pair tables rounded to c64, for the FP32 MFMA coupling.
 __global__ void k_tables_to_c64(const cd *tab, int64_t count, cf *out) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -731,7 +910,7 @@ __global__ void k_tables_to_c64(const cd *tab, int64_t count, cf *out) {
 }
 
 // Pair tables are kept when one set takes at most c->table_limit bytes.
-static bool tables_fit(const tg_cluster *c) {
+bool tables_fit(const tg_cluster *c) {
     return (int64_t)sizeof(cd) * c->N * c->N * c->plan.T <= c->table_limit;
 }
 
@@ -878,9 +1057,27 @@ template <> const cd *pair_tables<double>(tg_cluster *c, bool regular) {
 }
 template <> const cf *pair_tables<float>(tg_cluster *c, bool) { return c->b_tab32.as<cf>(); }
 
+// C x over a given table set TAB (GEMM precision R): pack the sources, one subset GEMM
+// per harmonic index, the FP64 plan contraction.
+template <class R>
+static int coupling_on_tables(tg_cluster *c, const cx<R> *TAB, const cd *x, cd *y, int cols, const char *name,
+                              const char *gname);
+
 template <class R>
 static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool regular) {
     TG_TRY(ensure_pair_tables(c, regular, sizeof(R) == 4));
+    const char *name = regular ? "overlap_mf" : sizeof(R) == 8 ? "coupling_mf" : "coupling_mf32";
+    const char *gname = regular ? "gemm_overlap" : sizeof(R) == 8 ? "gemm_f64" : "gemm_c64";
+    return coupling_on_tables<R>(c, pair_tables<R>(c, regular), x, y, cols, name, gname);
+}
+
+int coupling_tables_f64(tg_cluster *c, const cd *TAB, const cd *x, cd *y, int cols) {
+    return coupling_on_tables<double>(c, TAB, x, y, cols, "coupling_tangent", "gemm_tangent");
+}
+
+template <class R>
+static int coupling_on_tables(tg_cluster *c, const cx<R> *TAB, const cd *x, cd *y, int cols, const char *name,
+                              const char *gname) {
     const int N = c->N, H = c->H, T = c->plan.T;  // T: harmonic indices
     const int S = c->plan.S;
     // right-hand sides per pass: the packed sources and U of a pass within 512 MB
@@ -891,9 +1088,7 @@ static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool reg
     TG_TRY(c->ws[0].ensure(sizeof(cx<R>) * ((size_t)Qmax * N + (size_t)std::min(cols, cmax) * 2 * S * N)));
     cx<R> *X = c->ws[0].as<cx<R>>();
     cx<R> *U = X + (size_t)Qmax * N;
-    const cx<R> *TAB = pair_tables<R>(c, regular);
     Timer::Rec rec;
-    const char *name = regular ? "overlap_mf" : sizeof(R) == 8 ? "coupling_mf" : "coupling_mf32";
     TG_TRY(c->timer.begin(name, c->stream, rec));
     for (int c0 = 0; c0 < cols; c0 += cmax) {
         const int nc = std::min(cmax, cols - c0);
@@ -904,7 +1099,6 @@ static int coupling_tables(tg_cluster *c, const cd *x, cd *y, int cols, bool reg
         const int32_t *tiles = c->plan.stiles + (size_t)(w * kGemmCols + nc - 1) * (T + 1);
         const unsigned grid = (unsigned)c->gemm_ntiles[(size_t)w * kGemmCols + nc - 1] * (unsigned)((N + BM - 1) / BM);
         Timer::Rec grec;
-        const char *gname = regular ? "gemm_overlap" : sizeof(R) == 8 ? "gemm_f64" : "gemm_c64";
         TG_TRY(c->timer.begin(gname, c->stream, grec));
         if (N % 16)
             k_zgemm_subsets<R, true, BM, BN><<<grid, 256, 0, c->stream>>>(TAB, X, U, N, H, S, 2 * nc, T, tiles,
@@ -1169,6 +1363,13 @@ __global__ void k_local(const cd *mie, int lmax, int N, int H, int64_t n, int64_
     }
     y[ip] = yp;
     y[im] = ym;
+}
+
+int local_apply_blocks(tg_cluster *c, const cd *blocks, const cd *x, cd *y, int cols, int adjoint, const cd *sub) {
+    int64_t count = (int64_t)cols * c->nh;
+    k_local<<<grid1(count), 256, 0, c->stream>>>(blocks, c->lmax, c->N, c->H, c->n, c->nh, cols, x, y, adjoint, sub);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
 }
 
 int local_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *sub) {

@@ -23,6 +23,7 @@
 
 #include "tg_gmres.h"
 #include "tg_internal.h"
+#include "tg_tangent.h"
 
 #include "tg_lu.h"
 #include "tg_mfma.h"
@@ -1336,6 +1337,21 @@ int tg_factor_pullback(tg_factor *f, const tg_c128 *incident, const tg_c128 *x, 
     };
     return pullback_with(f->cluster, incident, x, cotangent, cols, solve, grad_k0, grad_radii, grad_eps,
                          grad_positions, incident_grad, conv_host);
+}
+
+int tg_factor_pushforward(tg_factor *f, const tg_c128 *incident, const tg_c128 *x, int cols, double rtol,
+                          double tangent_k0, const double *tangent_radii, const tg_c128 *tangent_eps,
+                          const double *tangent_positions, const tg_c128 *tangent_incident, tg_c128 *dx,
+                          tg_convergence *conv_host) {
+    if (!f || !(rtol > 0.0)) {
+        set_error("null factor or non-positive rtol");
+        return TG_INVALID_INPUT;
+    }
+    TangentSolve solve = [f, rtol](const cd *b, cd *y, int k, std::vector<tg_convergence> &conv) {
+        return factor_solve_split(f, b, y, k, 0, rtol, conv);
+    };
+    return pushforward_with(f->cluster, incident, x, cols, tangent_k0, tangent_radii, tangent_eps,
+                            tangent_positions, tangent_incident, solve, dx, conv_host);
 }
 
 }  // extern "C"
