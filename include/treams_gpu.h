@@ -323,15 +323,43 @@ typedef struct tg_factor tg_factor;
  * (cluster/interaction.rs:155-205, linalg/mod.rs:398-428).  precision is TG_STORE_C64
  * (complex FP32 factors on MFMA, every solve refined in FP64) or TG_STORE_C128;
  * the factors take n^2 x 8 or 16 bytes.  Pivoting is partial within 64-row diagonal
- * blocks (the reference pivots over whole columns); the FP64 residual certifies every
- * solve.  TG_SINGULAR for a zero pivot.  The cluster must outlive the factor.
+ * blocks (every trailing update an MFMA GEMM); a zero pivot there refactors with
+ * partial pivoting over whole columns, the reference's (faer) order, and a solve whose
+ * FP64 refinement stalls or turns non-finite refactors once in c128 with whole-column
+ * pivoting and repeats (tg_factor_info reports both).  TG_SINGULAR when whole-column
+ * pivoting meets a zero pivot.  The cluster must outlive the factor.
  * An operator whose entries span more than a factor 10 while a diagonal entry is below
- * a tenth of the largest is equilibrated first (linalg/mod.rs:137-236: rows and
- * columns scaled by their largest magnitudes, rounded here to powers of two); its
- * solves are then certified by the residual of the scaled system.
+ * a tenth of the largest is equilibrated first (linalg/mod.rs:137-310: rows and
+ * columns scaled by their largest magnitudes, rounded here to powers of two, then
+ * balanced by power-of-two sweeps that even out the magnitude sums); its solves are
+ * then certified by the residual of the scaled system.
  */
 int tg_factor_create(tg_cluster *cluster, int precision, tg_factor **out);
 int tg_factor_destroy(tg_factor *factor);
+
+/*
+ * The factor's state: precision (TG_STORE_C64 / TG_STORE_C128, after any fallback),
+ * pivoting (0 within 64-row blocks, 1 over whole columns) and fallback (1 when a
+ * stalled or singular solve refactored the operator in c128 with whole-column
+ * pivoting).  No reference counterpart (reports this library's fallbacks).
+ */
+int tg_factor_info(const tg_factor *factor, int *precision, int *pivoting, int *fallback);
+
+/*
+ * LU factors of a caller's dense n x n matrix A (device, column-major c128), with the
+ * same kernels, equilibration and fallbacks as tg_factor_create; A is copied (FP64,
+ * the refinement residual) and factored in c64 or c128.  Replaces Lu::new
+ * (linalg/mod.rs:398-428) for the dense interaction of python/treams_rs/diff.py:1638
+ * (InteractionFactor::from_blocks with dense T and C, cluster/interaction.rs:180-205).
+ */
+int tg_dense_factor_create(int64_t n, const tg_c128 *a_dev, int precision, void *stream, tg_factor **out);
+/*
+ * x = A^{-1} b (adjoint != 0: A^{-H} b) for `cols` columns (device, n x cols), refined
+ * in FP64 until |b - A x| <= rtol |b|: Lu::solve_in_place / solve_adjoint_in_place
+ * (linalg/mod.rs:431-438).
+ */
+int tg_dense_factor_solve(tg_factor *factor, const tg_c128 *b_dev, int cols, double rtol, int adjoint,
+                          tg_c128 *x_dev, tg_convergence *conv_host);
 
 /* 1 when the factors are those of the equilibrated operator D_r A D_c, else 0. */
 int tg_factor_equilibrated(const tg_factor *factor);
@@ -382,6 +410,36 @@ int tg_factor_pushforward(tg_factor *factor, const tg_c128 *incident_dev, const 
                           double rtol, double tangent_k0, const double *tangent_radii, const tg_c128 *tangent_epsilon,
                           const double *tangent_positions, const tg_c128 *tangent_incident_dev, tg_c128 *dx_dev,
                           tg_convergence *conv_host);
+
+/*
+ * Derivatives with the local T-matrices as independent inputs (dense M x M blocks per
+ * particle, device, reference layout, column-major, N blocks): the k0 tangent or
+ * gradient covers the coupling only.  This is the differentiation of a particle
+ * cluster, ParticleClusterResidual::{pushforward, pullback} (cluster/particles.rs:
+ * 206-250) for spherical bases, helicity and one real wavenumber, and of the dense
+ * interaction, IlluminateResidual::{pushforward, pullback} (cluster/interaction.rs:
+ * 464-558) with C given by the positions; it works for sphere clusters too.
+ * pushforward: dX from tangents of k0, positions[3N] (host; NULL = 0), the local blocks
+ * (NULL = 0) and the incident columns (NULL = 0).  pullback: the gradients of k0 and
+ * positions (host), the local blocks T-bar = Y (B + C X)^H (device, N M x M blocks;
+ * NULL skips them) and B-bar = T^H Y (device; C-bar = B-bar X^H).
+ */
+int tg_cluster_pushforward_local(tg_cluster *cluster, const tg_c128 *incident_dev, const tg_c128 *x_dev, int cols,
+                                 const tg_gmres_options *options, double tangent_k0, const double *tangent_positions,
+                                 const tg_c128 *tangent_local_dev, const tg_c128 *tangent_incident_dev, tg_c128 *dx_dev,
+                                 tg_convergence *conv_host);
+int tg_factor_pushforward_local(tg_factor *factor, const tg_c128 *incident_dev, const tg_c128 *x_dev, int cols,
+                                double rtol, double tangent_k0, const double *tangent_positions,
+                                const tg_c128 *tangent_local_dev, const tg_c128 *tangent_incident_dev, tg_c128 *dx_dev,
+                                tg_convergence *conv_host);
+int tg_cluster_pullback_local(tg_cluster *cluster, const tg_c128 *incident_dev, const tg_c128 *x_dev,
+                              const tg_c128 *cotangent_dev, int cols, const tg_gmres_options *options, double *grad_k0,
+                              double *grad_positions, tg_c128 *grad_local_dev, tg_c128 *incident_grad_dev,
+                              tg_convergence *conv_host);
+int tg_factor_pullback_local(tg_factor *factor, const tg_c128 *incident_dev, const tg_c128 *x_dev,
+                             const tg_c128 *cotangent_dev, int cols, double rtol, double *grad_k0,
+                             double *grad_positions, tg_c128 *grad_local_dev, tg_c128 *incident_grad_dev,
+                             tg_convergence *conv_host);
 
 /* ---------------------------------------------------------------------------
  * Fields

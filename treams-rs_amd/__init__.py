@@ -10,6 +10,7 @@ from .cluster import (  # noqa: F401
     DEFAULT_RESTART,
     DEFAULT_RTOL,
     Convergence,
+    DenseFactor,
     FactorContext,
     IterativeContext,
     IterativeSphereCluster,
@@ -26,3 +27,4 @@ from .cluster import (  # noqa: F401
     wigner3j,
 )
 from .geometry import WORKLOADS, random_cluster, workload  # noqa: F401
+from . import diff  # noqa: F401,E402

@@ -39,7 +39,9 @@ EXPORTED = [
     "tg_mlp_backward", "tg_mlp_adam_step", "tg_factor_create", "tg_factor_destroy", "tg_factor_solve",
     "tg_factor_pullback", "tg_factor_equilibrated", "tg_factor_global_tmatrix", "tg_cluster_field",
     "tg_plane_wave_field", "tg_translation_blocks_rect", "tg_translation_blocks_tangent", "tg_cluster_pushforward",
-    "tg_factor_pushforward",
+    "tg_factor_pushforward", "tg_factor_info", "tg_dense_factor_create", "tg_dense_factor_solve",
+    "tg_cluster_pushforward_local", "tg_factor_pushforward_local", "tg_cluster_pullback_local",
+    "tg_factor_pullback_local",
 ]
 
 
@@ -146,6 +148,15 @@ def lib():
         "tg_cluster_pushforward": (_i, [_vp, _vp, _vp, _i, ctypes.POINTER(GmresOptions), _d, _vp, _vp, _vp, _vp,
                                         _vp, _vp]),
         "tg_factor_pushforward": (_i, [_vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "tg_factor_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+        "tg_dense_factor_create": (_i, [_i64, _vp, _i, _vp, ctypes.POINTER(_vp)]),
+        "tg_dense_factor_solve": (_i, [_vp, _vp, _i, _d, _i, _vp, _vp]),
+        "tg_cluster_pushforward_local": (_i, [_vp, _vp, _vp, _i, ctypes.POINTER(GmresOptions), _d, _vp, _vp, _vp,
+                                              _vp, _vp]),
+        "tg_factor_pushforward_local": (_i, [_vp, _vp, _vp, _i, _d, _d, _vp, _vp, _vp, _vp, _vp]),
+        "tg_cluster_pullback_local": (_i, [_vp, _vp, _vp, _vp, _i, ctypes.POINTER(GmresOptions), _vp, _vp, _vp,
+                                           _vp, _vp]),
+        "tg_factor_pullback_local": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

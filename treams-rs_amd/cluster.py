@@ -316,6 +316,27 @@ class IterativeContext:
         reports = [(v.iterations, v.residual_norm, v.rhs_norm) for v in conv]
         return _back(out, vector), reports
 
+    def _opts(self):
+        return self.cluster.options(**{k: v for k, v in self.options.items() if k in (
+            "rtol", "atol", "restart", "max_iterations", "mixed_precision", "preconditioner")})
+
+    def pushforward_local(self, k0=0.0, positions=None, local=None, incident=None):
+        """dX with the local T-matrices as inputs: tangents of k0 (coupling only), the
+        positions, the (N, M, M) local blocks and the incident columns
+        (cluster/particles.rs:206-220, cluster/interaction.rs:464-478)."""
+        c = self.cluster
+        dx, conv = _pushforward_local(lib().tg_cluster_pushforward_local, c._h, c, self.incident, self.x,
+                                      ctypes.byref(self._opts()), k0, positions, local, incident)
+        return dx, [(v.iterations, v.residual_norm, v.rhs_norm) for v in conv]
+
+    def pullback_local(self, cotangent):
+        """(k0, positions[N, 3], local blocks (N, M, M), incident, convergence) with the
+        local T-matrices as inputs (cluster/particles.rs:231-250)."""
+        c = self.cluster
+        gk, gp, gl, gi, conv = _pullback_local(lib().tg_cluster_pullback_local, c._h, c, self.incident, self.x,
+                                               cotangent, ctypes.byref(self._opts()))
+        return gk, gp, gl, gi, [(v.iterations, v.residual_norm, v.rhs_norm) for v in conv]
+
     def pullback(self, cotangent):
         """Gradients from the cotangent of X: returns
         (k0, radii[N], eps[N], positions[N, 3], incident, convergence)."""
@@ -357,6 +378,98 @@ def _incident_tangent(incident, like: torch.Tensor, n: int):
     if t.shape != like.shape:
         raise ValueError("incident tangent shape does not match the recorded incident fields")
     return t
+
+
+def _factor_info(h) -> dict:
+    p, v, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(lib().tg_factor_info(h, ctypes.byref(p), ctypes.byref(v), ctypes.byref(f)))
+    return {"precision": "c64" if p.value == _lib.TG_STORE_C64 else "c128",
+            "pivoting": "column" if v.value else "block", "fallback": bool(f.value)}
+
+
+def _local_blocks_in(local, N: int, M: int, dev):
+    """(N, M, M) local tangent blocks as column-major device memory (None stays None)."""
+    if local is None:
+        return None
+    t = torch.as_tensor(local).to(dev, torch.complex128).reshape(N, M, M)
+    return t.transpose(1, 2).contiguous()
+
+
+def _pushforward_local(fn, handle, c, incident, x, cols_opts, k0, positions, local, incident_tangent):
+    a, vector = _columns(incident, c.n)
+    xc, _ = _columns(x, c.n)
+    ti = _incident_tangent(incident_tangent, a, c.n)
+    tp = (np.zeros((c.N, 3)) if positions is None else
+          np.ascontiguousarray(np.asarray(positions, np.float64).reshape(c.N, 3)))
+    tl = _local_blocks_in(local, c.N, c.modes_per_sphere, c._dev)
+    cols = a.shape[0]
+    out = torch.empty_like(a)
+    conv = (_Conv * cols)()
+    check(fn(handle, a.data_ptr(), xc.data_ptr(), cols, cols_opts, float(k0 or 0.0), tp.ctypes.data,
+             None if tl is None else tl.data_ptr(), None if ti is None else ti.data_ptr(), out.data_ptr(), conv))
+    return _back(out, vector), conv
+
+
+def _pullback_local(fn, handle, c, incident, x, cotangent, opts):
+    a, vector = _columns(incident, c.n)
+    xc, _ = _columns(x, c.n)
+    g, _ = _columns(cotangent, c.n)
+    cols = a.shape[0]
+    inc = torch.empty_like(a)
+    M = c.modes_per_sphere
+    gl = torch.empty((c.N, M, M), dtype=torch.complex128, device=c._dev)  # column-major blocks
+    gk = ctypes.c_double()
+    gp = np.zeros(3 * c.N)
+    conv = (_Conv * cols)()
+    check(fn(handle, a.data_ptr(), xc.data_ptr(), g.data_ptr(), cols, opts, ctypes.byref(gk), gp.ctypes.data,
+             gl.data_ptr(), inc.data_ptr(), conv))
+    return gk.value, gp.reshape(c.N, 3), gl.transpose(1, 2), _back(inc, vector), conv
+
+
+class DenseFactor:
+    """LU factors of a caller's dense n x n matrix on the device (Lu::new,
+    linalg/mod.rs:398-428), with the cluster factor's kernels, equilibration and
+    fallbacks; solves are refined in FP64 against the matrix to ``rtol``."""
+
+    def __init__(self, matrix, precision: str = "c128", rtol: float = 1e-12):
+        a = torch.as_tensor(matrix).to(_device(), torch.complex128)
+        if a.ndim != 2 or a.shape[0] != a.shape[1]:
+            raise ValueError("require a square matrix")
+        self.n = int(a.shape[0])
+        self.rtol = float(rtol)
+        p = {"c64": _lib.TG_STORE_C64, "c128": _lib.TG_STORE_C128}[precision]
+        acol = a.t().contiguous()  # column-major
+        h = ctypes.c_void_p()
+        check(lib().tg_dense_factor_create(self.n, acol.data_ptr(), p, ctypes.c_void_p(_stream_ptr()),
+                                           ctypes.byref(h)))
+        self._h = h
+        self._destroy = lib().tg_factor_destroy
+        self._dev = a.device
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def info(self) -> dict:
+        return _factor_info(self._h)
+
+    def solve(self, b, adjoint: bool = False):
+        """A^{-1} b (A^{-H} b); returns (x, [Convergence])."""
+        if self._h is None:
+            raise ValueError("the factor was closed")
+        cm, vector = _columns(b, self.n)
+        cols = cm.shape[0]
+        out = torch.empty_like(cm)
+        conv = (_Conv * cols)()
+        check(lib().tg_dense_factor_solve(self._h, cm.data_ptr(), cols, self.rtol, int(adjoint), out.data_ptr(),
+                                          conv))
+        return _back(out, vector), [Convergence(0, c.residual_norm, c.rhs_norm, c.refinements) for c in conv]
 
 
 class SphereClusterFactor:
@@ -418,6 +531,13 @@ class SphereClusterFactor:
         return x, FactorContext(self, incident, x), conv
 
     @property
+    def info(self) -> dict:
+        """The factor's precision ("c64"/"c128", after any fallback), pivoting ("block":
+        inside 64-row diagonal blocks, "column": over whole columns) and whether a
+        stalled or singular solve refactored it (fallback)."""
+        return _factor_info(self._h)
+
+    @property
     def equilibrated(self) -> bool:
         """Whether the factors are those of the row/column equilibrated operator
         (linalg/mod.rs:137-236)."""
@@ -469,6 +589,24 @@ class FactorContext:
                                           out.data_ptr(), conv))
         reports = [Convergence(0, v.residual_norm, v.rhs_norm, v.refinements) for v in conv]
         return _back(out, vector), reports
+
+    def pushforward_local(self, k0=0.0, positions=None, local=None, incident=None):
+        """dX with the local T-matrices as inputs (tangents of k0 through the coupling,
+        positions, (N, M, M) local blocks, incident columns), the tangent solve by the
+        factors (cluster/particles.rs:206-220, cluster/interaction.rs:464-478)."""
+        f, c = self.factor, self.factor.cluster
+        dx, conv = _pushforward_local(lib().tg_factor_pushforward_local, f._h, c, self.incident, self.x,
+                                      ctypes.c_double(f.rtol), k0, positions, local, incident)
+        return dx, [Convergence(0, v.residual_norm, v.rhs_norm, v.refinements) for v in conv]
+
+    def pullback_local(self, cotangent):
+        """(k0, positions[N, 3], local blocks (N, M, M), incident, convergence) with the
+        local T-matrices as inputs (cluster/particles.rs:231-250,
+        cluster/interaction.rs:533-558)."""
+        f, c = self.factor, self.factor.cluster
+        gk, gp, gl, gi, conv = _pullback_local(lib().tg_factor_pullback_local, f._h, c, self.incident, self.x,
+                                               cotangent, ctypes.c_double(f.rtol))
+        return gk, gp, gl, gi, [Convergence(0, v.residual_norm, v.rhs_norm, v.refinements) for v in conv]
 
     def pullback(self, cotangent):
         """Returns (k0, radii[N], eps[N], positions[N, 3], incident, convergence)."""

@@ -14,6 +14,7 @@
 #include "tg_gmres.h"
 #include "tg_internal.h"
 #include "tg_special.h"
+#include "tg_tangent.h"
 
 namespace tg {
 
@@ -327,9 +328,78 @@ int pullback_with(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, cons
     return st;
 }
 
+// Gradients with T an independent input (cluster/interaction.rs:379-440 through the
+// expansion pullback of cluster/particles.rs:231-250): Y = A^{-H} G, B-bar = T^H Y,
+// the pair gradients of C-bar = T^H Y X^H (positions, and k0 through the coupling),
+// and the local blocks T-bar = Y (B + C X)^H.
+int pullback_local_with(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent, int cols,
+                        const TangentSolve &adjoint_solve, double *grad_k0, double *grad_positions,
+                        tg_c128 *grad_local, tg_c128 *incident_grad, tg_convergence *conv_host) {
+    if (!c || !incident || !x || !cotangent || cols < 1) {
+        set_error("null argument or no columns");
+        return TG_INVALID_INPUT;
+    }
+    const int64_t count = (int64_t)cols * c->n;
+    DevBuf io;
+    TG_TRY(io.ensure(sizeof(cd) * count * 6 + sizeof(double) * (3 * (size_t)c->N + 2)));
+    cd *a = io.as<cd>(), *X = a + count, *Gs = X + count, *Y = Gs + count, *U = Y + count, *Rsp = U + count;
+    double *gpos = reinterpret_cast<double *>(Rsp + count);
+    double *gk = gpos + 3 * c->N;
+    int st = TG_OK;
+    std::vector<tg_convergence> conv;
+    do {
+        if ((st = ref_to_split(c, (const cd *)incident, a, cols))) break;
+        if ((st = ref_to_split(c, (const cd *)x, X, cols))) break;
+        if ((st = ref_to_split(c, (const cd *)cotangent, Gs, cols))) break;
+        if ((st = adjoint_solve(Gs, Y, cols, conv))) break;
+        if ((st = local_apply(c, Y, U, cols, 1, nullptr))) break;  // T^H Y
+        if (incident_grad && (st = split_to_ref(c, U, (cd *)incident_grad, cols))) break;
+        if ((st = pair_gradients(c, U, X, cols, 0, gpos, gk))) break;
+        if (grad_local) {
+            if ((st = coupling_matrix_free(c, X, Rsp, cols, 0))) break;
+            if ((st = vec_axpby(c->stream, cd(1.0), a, cd(1.0), Rsp, count))) break;
+            if ((st = local_cotangent(c, Y, Rsp, cols, (cd *)grad_local))) break;
+        }
+        std::vector<double> h_pos(3 * c->N);
+        double h_gk = 0.0;
+        if (hipMemcpyAsync(h_pos.data(), gpos, sizeof(double) * 3 * c->N, hipMemcpyDeviceToHost, c->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(&h_gk, gk, sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            set_error("pullback copy failed");
+            st = TG_DEVICE_ERROR;
+            break;
+        }
+        if (grad_k0) *grad_k0 = h_gk;
+        if (grad_positions) memcpy(grad_positions, h_pos.data(), sizeof(double) * 3 * c->N);
+    } while (0);
+    if (conv_host)
+        for (int k = 0; k < cols && k < (int)conv.size(); ++k) conv_host[k] = conv[k];
+    (void)hipStreamSynchronize(c->stream);
+    io.release();
+    return st;
+}
+
 }  // namespace tg
 
 extern "C" {
+
+int tg_cluster_pullback_local(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent,
+                              int cols, const tg_gmres_options *o, double *grad_k0, double *grad_positions,
+                              tg_c128 *grad_local, tg_c128 *incident_grad, tg_convergence *conv_host) {
+    if (!o) {
+        set_error("null options");
+        return TG_INVALID_INPUT;
+    }
+    tg_gmres_options od = *o;
+    od.atol = 0.0;
+    if (!(od.rtol > 0)) od.rtol = 1e-10;
+    TangentSolve solve = [c, od](const cd *g, cd *y, int k, std::vector<tg_convergence> &conv) {
+        return solve_split(c, g, y, k, od, 1, conv);
+    };
+    return pullback_local_with(c, incident, x, cotangent, cols, solve, grad_k0, grad_positions, grad_local,
+                               incident_grad, conv_host);
+}
 
 int tg_cluster_pullback(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent, int cols,
                         const tg_gmres_options *o, double *grad_k0, double *grad_radii, tg_c128 *grad_eps,

@@ -154,6 +154,13 @@ int launch_assemble_operator(tg_cluster *c, int precision, void *out);
 // then A = I - T C block row by block row on MFMA (tg_particles.hip)
 int assemble_operator_particles(tg_cluster *c, int precision, void *out);
 int local_apply_particles(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *subtract_from);
+// dense block-diagonal T' x with T' as split panels (the layout of b_tloc)
+int local_apply_dense(tg_cluster *c, const cd *panels, const cd *x, cd *y, int cols, int adjoint,
+                      const cd *subtract_from);
+// N reference-layout M x M blocks (device) -> split panels
+int tloc_from_ref(tg_cluster *c, const cd *ref, cd *panels);
+// per-particle sum_c Y R^H (split-layout Y, R) as reference-layout M x M blocks
+int local_cotangent(tg_cluster *c, const cd *Y, const cd *R, int cols, cd *out_ref);
 // error for operations that need the spheres' Mie blocks (radii, permittivities)
 int require_spheres(const tg_cluster *c, const char *what);
 // singular blocks as their two same-helicity H x H parts in c64, [count][h][a][b]

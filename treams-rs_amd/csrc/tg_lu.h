@@ -32,6 +32,8 @@ int luset_init(LuSet &ls, int precision, void *A, const std::vector<int64_t> &of
 void luset_release(LuSet &ls);
 // Factor in place; TG_SINGULAR on a zero pivot.
 int luset_factor(LuSet &ls, hipStream_t s);
+// The same for a batch of one with partial pivoting over whole columns (GETRF order).
+int luset_factor_gepp(LuSet &ls, hipStream_t s);
 // x: `cols` packed columns of length ntot (matrix z's rows at voff[z]); overwritten by
 // A^{-1} x (adjoint = 0) or A^{-H} x (adjoint = 1), blockwise.
 int luset_solve(LuSet &ls, hipStream_t s, cd *x, int cols, int adjoint);

@@ -42,6 +42,14 @@ struct tg_factor {
     // assembled (before equilibration): the refinement residual of forward solves is
     // then one GEMM instead of a matrix-free apply
     tg::DevBuf op;
+    // pivoting: 0 inside 64-row diagonal blocks, 1 over whole columns (GETRF order)
+    int pivoting = 0;
+    // fallbacks taken: 1 when a stalled or singular solve refactored the operator (in
+    // c128 with whole-column pivoting)
+    int fallback = 0;
+    // a factor of a caller's dense matrix (no cluster): `cluster` is a bare handle
+    // carrying the stream, owned by the factor; `op` always holds the matrix
+    bool dense = false;
 };
 
 namespace tg {
@@ -743,6 +751,184 @@ int luset_factor(LuSet &ls, hipStream_t s) {
     return ls.precision == TG_STORE_C64 ? luset_factor_t<float>(ls, s) : luset_factor_t<double>(ls, s);
 }
 
+// ------------------------------------------ whole-column pivoting (GETRF order) --
+// The factorisation of one matrix with partial pivoting over whole columns
+// (linalg/mod.rs:398-428 through faer's partial-pivot LU): a 64-column panel is
+// factored column by column, each pivot searched over all remaining rows, the rows
+// exchanged across the panel; then the panel's exchanges reach the other columns, the
+// block row right of the panel is solved with the panel's unit lower triangle, and
+// the trailing matrix is updated by the MFMA tile of the main LU (k_lu_update).  The
+// factors land in the same storage and the exchanges in the same gather permutation
+// as the main LU's, so every solve kernel serves both.  It is the fallback when
+// pivoting inside 64-row diagonal blocks meets a zero pivot or its solves stall: three
+// launches per column, so it is a latency-bound path, not the fast one.
+
+// Pivot of column j: the row of largest |a_ij|^2 over i in [j, n), the lowest row on
+// ties; one workgroup.  A zero (or NaN) pivot sets *status.
+template <class T>
+__global__ __launch_bounds__(1024) void k_gepp_pivot(const cx<T> *A, int64_t n, int64_t j, int *pivot, int *status) {
+    __shared__ double sv[1024];
+    __shared__ int si[1024];
+    const int tid = threadIdx.x;
+    double best = -1.0;
+    int bi = (int)j;
+    const cx<T> *col = A + j * n;
+    for (int64_t i = j + tid; i < n; i += 1024) {
+        const double re = (double)col[i].re, im = (double)col[i].im;
+        const double m = re * re + im * im;
+        if (m > best) {  // rows ascend per thread: the first of equal maxima stays
+            best = m;
+            bi = (int)i;
+        }
+    }
+    sv[tid] = best;
+    si[tid] = bi;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (tid < w) {
+            const double a = sv[tid], b = sv[tid + w];
+            if (b > a || (b == a && si[tid + w] < si[tid])) {
+                sv[tid] = b;
+                si[tid] = si[tid + w];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        pivot[j] = si[0];
+        if (!(sv[0] > 0.0)) *status = 1;
+    }
+}
+
+// Exchange rows j and pivot[j] across the panel columns [k0, k1) (k1 - k0 <= 64).
+template <class T>
+__global__ __launch_bounds__(64) void k_gepp_swap_panel(cx<T> *A, int64_t n, int64_t j, int64_t k0, int64_t k1,
+                                                        const int *pivot) {
+    const int64_t c = k0 + threadIdx.x, p = pivot[j];
+    if (c >= k1 || p == j) return;
+    cx<T> *col = A + c * n;
+    const cx<T> t = col[j];
+    col[j] = col[p];
+    col[p] = t;
+}
+
+// Column j of the panel: l_i = a_ij / a_jj for i > j, then a_ic -= l_i a_jc for the
+// panel columns j < c < k1; one thread per row.
+template <class T>
+__global__ __launch_bounds__(256) void k_gepp_eliminate(cx<T> *A, int64_t n, int64_t j, int64_t k1) {
+    __shared__ T ur[64][2];
+    __shared__ T rpv[2];
+    const int nc = (int)(k1 - j - 1);
+    for (int q = threadIdx.x; q < nc; q += blockDim.x) {
+        const cx<T> u = A[(j + 1 + q) * n + j];
+        ur[q][0] = u.re;
+        ur[q][1] = u.im;
+    }
+    if (threadIdx.x == 0) {
+        const cx<T> r = recip_scaled(A[j * n + j]);
+        rpv[0] = r.re;
+        rpv[1] = r.im;
+    }
+    __syncthreads();
+    const cx<T> rp(rpv[0], rpv[1]);
+    const int64_t i = j + 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const cx<T> l = A[j * n + i] * rp;
+    A[j * n + i] = l;
+    for (int q = 0; q < nc; ++q) sub_mul(A[(j + 1 + q) * n + i], l, cx<T>(ur[q][0], ur[q][1]));
+}
+
+// The panel's exchanges (pivot[k0 .. k1)) in order on every column outside the panel,
+// [0, k0) and [k1, n); one thread per column.
+template <class T>
+__global__ void k_gepp_laswp(cx<T> *A, int64_t n, int64_t k0, int64_t k1, const int *pivot) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n - (k1 - k0)) return;
+    const int64_t c = t < k0 ? t : t + (k1 - k0);
+    cx<T> *col = A + c * n;
+    for (int64_t j = k0; j < k1; ++j) {
+        const int64_t p = pivot[j];
+        if (p != j) {
+            const cx<T> v = col[j];
+            col[j] = col[p];
+            col[p] = v;
+        }
+    }
+}
+
+// U12 = L11^{-1} A12 for the rows [k0, k1) of the columns [k1, n), L11 unit lower;
+// forward substitution, one thread per column (L11 read through the caches).
+template <class T>
+__global__ __launch_bounds__(64) void k_gepp_trsm(cx<T> *A, int64_t n, int64_t k0, int64_t k1) {
+    const int64_t c = k1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    cx<T> *col = A + c * n;
+    const int kb = (int)(k1 - k0);
+    for (int r = 1; r < kb; ++r) {
+        cx<T> v = col[k0 + r];
+        for (int q = 0; q < r; ++q) sub_mul(v, A[(k0 + q) * n + k0 + r], col[k0 + q]);
+        col[k0 + r] = v;
+    }
+}
+
+// perm[i] = the original row at position i after the exchanges pivot[0 .. n), in order.
+__global__ void k_gepp_perm(const int *pivot, int64_t n, int *perm) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int64_t i = 0; i < n; ++i) perm[i] = (int)i;
+    for (int64_t j = 0; j < n; ++j) {
+        const int p = pivot[j];
+        const int t = perm[j];
+        perm[j] = perm[p];
+        perm[p] = t;
+    }
+}
+
+template <class T>
+static int luset_factor_gepp_t(LuSet &ls, hipStream_t s) {
+    if (ls.count != 1) {
+        set_error("whole-column pivoting factors one matrix at a time");
+        return TG_INVALID_INPUT;
+    }
+    cx<T> *A = reinterpret_cast<cx<T> *>(ls.A);
+    const int64_t n = ls.n_h[0];
+    DevBuf pv;
+    TG_TRY(pv.ensure(sizeof(int) * (size_t)n));
+    int *pivot = pv.as<int>();
+    TG_HIP(hipMemsetAsync(ls.status, 0, sizeof(int), s));
+    const Batch bt{ls.off, ls.n, ls.voff};
+    auto blocks = [](int64_t w, int b) { return (unsigned)std::max<int64_t>(1, (w + b - 1) / b); };
+    for (int64_t k0 = 0; k0 < n; k0 += NB) {
+        const int64_t k1 = std::min<int64_t>(k0 + NB, n);
+        for (int64_t j = k0; j < k1; ++j) {
+            k_gepp_pivot<T><<<1, 1024, 0, s>>>(A, n, j, pivot, ls.status);
+            k_gepp_swap_panel<T><<<1, 64, 0, s>>>(A, n, j, k0, k1, pivot);
+            if (j + 1 < n) k_gepp_eliminate<T><<<blocks(n - j - 1, 256), 256, 0, s>>>(A, n, j, k1);
+        }
+        if (n > k1 - k0) k_gepp_laswp<T><<<blocks(n - (k1 - k0), 256), 256, 0, s>>>(A, n, k0, k1, pivot);
+        if (k1 < n) {
+            k_gepp_trsm<T><<<blocks(n - k1, 64), 64, 0, s>>>(A, n, k0, k1);
+            k_lu_update<T><<<dim3(blocks(n - k1, 64), blocks(n - k1, 64), 1), 256, 0, s>>>(A, bt, k0, (int)NB, k1, n,
+                                                                                       k1, n);
+        }
+        TG_HIP(hipGetLastError());
+    }
+    k_gepp_perm<<<1, 64, 0, s>>>(pivot, n, ls.perm);
+    TG_HIP(hipGetLastError());
+    int stat = 0;
+    TG_HIP(hipMemcpyAsync(&stat, ls.status, sizeof(int), hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    pv.release();
+    if (stat) {
+        set_error("zero pivot in the LU factorisation with whole-column pivoting (singular operator)");
+        return TG_SINGULAR;
+    }
+    return TG_OK;
+}
+
+int luset_factor_gepp(LuSet &ls, hipStream_t s) {
+    return ls.precision == TG_STORE_C64 ? luset_factor_gepp_t<float>(ls, s) : luset_factor_gepp_t<double>(ls, s);
+}
+
 // x (packed rows, FP64, `cols` <= 8 columns of stride ld = ntot) <- A^{-1} x or A^{-H} x.
 template <class T>
 static int luset_solve8(LuSet &ls, hipStream_t s, cd *x, int cols, int adjoint) {
@@ -942,6 +1128,111 @@ __global__ void k_scale_rows(cd *x, int64_t n, int cols, const double *d) {
     x[t] = cd(x[t].re * s, x[t].im * s);
 }
 
+// ---- balancing (linalg/mod.rs:247-310): power-of-two row and column factors that
+// even out the magnitude sums of the max-equilibrated operator S, one sweep moving
+// every column factor, then every row factor, to the power of two that minimises
+// sum |s_ij| x_i y_j - sum ln x_i - sum ln y_j along it.  flags[0]: a sum was not a
+// normal number (keep the max scales); flags[1]: some factor moved beyond 2.
+
+// factor *= the power of two that puts sum * factor (given as value) / ln 2 in [1, 2).
+__device__ __forceinline__ void bal_advance(double &factor, double value, int *flags) {
+    const double ratio = value / 0.69314718055994530942;
+    if (!(ratio >= 2.2250738585072014e-308 && ratio <= 1.7976931348623157e308)) {
+        flags[0] = 1;
+        return;
+    }
+    int e = 0;
+    (void)frexp(ratio, &e);                // ratio = m 2^e, m in [0.5, 1)
+    const double step = ldexp(1.0, 1 - e);  // 1 / (largest power of two <= ratio)
+    factor *= step;
+    if (step < 0.5 || step > 2.0) flags[1] = 1;
+}
+
+// Column pass: y_j from sum_i x_i |s_ij|; one workgroup per column.
+template <class T>
+__global__ __launch_bounds__(256) void k_bal_cols(const cx<T> *S, int64_t n, const double *x, double *y,
+                                                  int *flags) {
+    const int64_t c = blockIdx.x;
+    double sum = 0.0;
+    for (int64_t r = threadIdx.x; r < n; r += 256) sum += x[r] * eq_mag(S[c * n + r]);
+    __shared__ double red[256];
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bal_advance(y[c], red[0] * y[c], flags);
+}
+
+// Row pass: x_i from sum_j |s_ij| y_j; one thread per row (coalesced over rows).
+template <class T>
+__global__ void k_bal_rows(const cx<T> *S, int64_t n, double *x, const double *y, int *flags) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double sum = 0.0;
+    for (int64_t c = 0; c < n; ++c) sum += eq_mag(S[c * n + r]) * y[c];
+    bal_advance(x[r], sum * x[r], flags);
+}
+
+__global__ void k_fill_f64(double *p, int64_t count, double v) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) p[t] = v;
+}
+
+__global__ void k_fill_i32(int *p, int count, int v) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) p[t] = v;
+}
+
+constexpr int kBalanceSweeps = 64;  // dense operators take at most three (linalg/mod.rs:240)
+
+// Balance the equilibrated S in place and fold the factors into dr, dc (device): the
+// sweeps run until none moves a factor beyond 2, at most kBalanceSweeps; a non-normal
+// sum or a factor beyond 2^+-256 keeps the max scales.
+template <class T>
+static int balance_t(cx<T> *S, int64_t n, double *dr, double *dc, hipStream_t s) {
+    DevBuf work;
+    TG_TRY(work.ensure(sizeof(double) * 2 * (size_t)n + 256));
+    double *x = work.as<double>(), *y = x + n;
+    int *flags = reinterpret_cast<int *>(y + n);
+    const unsigned g = (unsigned)((n + 255) / 256);
+    k_fill_f64<<<(unsigned)((2 * n + 255) / 256), 256, 0, s>>>(x, 2 * n, 1.0);
+    int hf[2] = {0, 0};
+    for (int sweep = 0; sweep < kBalanceSweeps; ++sweep) {
+        k_fill_i32<<<1, 64, 0, s>>>(flags, 2, 0);
+        k_bal_cols<T><<<(unsigned)n, 256, 0, s>>>(S, n, x, y, flags);
+        k_bal_rows<T><<<g, 256, 0, s>>>(S, n, x, y, flags);
+        TG_HIP(hipGetLastError());
+        TG_HIP(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, s));
+        TG_HIP(hipStreamSynchronize(s));
+        if (hf[0] || !hf[1]) break;
+    }
+    if (hf[0]) return TG_OK;  // not normal: the max scales stay
+    std::vector<double> h(4 * (size_t)n);
+    TG_HIP(hipMemcpyAsync(h.data(), x, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipMemcpyAsync(h.data() + 2 * n, dr, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipMemcpyAsync(h.data() + 3 * n, dc, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    TG_HIP(hipStreamSynchronize(s));
+    const double bound = std::ldexp(1.0, 256);
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < 2; ++k) {
+            const double fct = h[k * n + i], scale = h[(2 + k) * n + i] * fct;
+            if (!(fct >= 1.0 / bound && fct <= bound) || !std::isnormal(scale)) return TG_OK;
+        }
+    k_eq_apply<T><<<(unsigned)((n * n + 255) / 256), 256, 0, s>>>(S, n, x, y);
+    for (int64_t i = 0; i < n; ++i) {
+        h[2 * n + i] *= h[i];
+        h[3 * n + i] *= h[n + i];
+    }
+    TG_HIP(hipMemcpyAsync(dr, h.data() + 2 * n, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    TG_HIP(hipMemcpyAsync(dc, h.data() + 3 * n, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    TG_HIP(hipGetLastError());
+    TG_HIP(hipStreamSynchronize(s));
+    work.release();
+    return TG_OK;
+}
+
 // 2^-e for v = m 2^e, m in [0.5, 1): the power of two that brings v into [0.5, 1).
 static double pow2_inverse(double v) {
     int e = 0;
@@ -1022,25 +1313,71 @@ static int equilibrate_t(tg_factor *f) {
     TG_HIP(hipMemcpyAsync(dc, (columns_first ? first : other).data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     k_eq_apply<T><<<(unsigned)((n * n + 255) / 256), 256, 0, s>>>(reinterpret_cast<cx<T> *>(f->lu.ptr), n, dr, dc);
     TG_HIP(hipGetLastError());
+    TG_TRY(balance_t<T>(reinterpret_cast<cx<T> *>(f->lu.ptr), n, dr, dc, s));
     TG_HIP(hipStreamSynchronize(s));
     f->equilibrated = true;
     return TG_OK;
 }
 
-static int factor_cluster(tg_factor *f) {
+__global__ void k_c128_to_c64(const cd *in, int64_t count, cf *out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) out[t] = cf((float)in[t].re, (float)in[t].im);
+}
+
+// Assemble (or, for a dense factor, copy) the operator into the factor storage,
+// equilibrate and factor it, pivoting inside 64-row blocks or over whole columns.
+static int dense_factor_error() {
+    set_error("a factor of a caller's dense matrix has no cluster: use tg_dense_factor_solve");
+    return TG_INVALID_INPUT;
+}
+
+static int factor_cluster(tg_factor *f, bool whole_column) {
     tg_cluster *c = f->cluster;
-    TG_TRY(launch_assemble_operator(c, f->precision, f->lu.ptr));
-    if (f->precision == TG_STORE_C128 && f->n <= kDenseResidual) {
-        const size_t bytes = sizeof(cd) * (size_t)f->n * (size_t)f->n;
-        TG_TRY(f->op.ensure(bytes));
-        TG_HIP(hipMemcpyAsync(f->op.ptr, f->lu.ptr, bytes, hipMemcpyDeviceToDevice, c->stream));
+    const int64_t nn = f->n * f->n;
+    if (f->dense) {
+        if (f->precision == TG_STORE_C64)
+            k_c128_to_c64<<<(unsigned)((nn + 255) / 256), 256, 0, c->stream>>>(f->op.as<cd>(), nn, f->lu.as<cf>());
+        else
+            TG_HIP(hipMemcpyAsync(f->lu.ptr, f->op.ptr, sizeof(cd) * (size_t)nn, hipMemcpyDeviceToDevice, c->stream));
+        TG_HIP(hipGetLastError());
+    } else {
+        TG_TRY(launch_assemble_operator(c, f->precision, f->lu.ptr));
+        if (f->precision == TG_STORE_C128 && f->n <= kDenseResidual) {
+            const size_t bytes = sizeof(cd) * (size_t)nn;
+            TG_TRY(f->op.ensure(bytes));
+            TG_HIP(hipMemcpyAsync(f->op.ptr, f->lu.ptr, bytes, hipMemcpyDeviceToDevice, c->stream));
+        }
     }
     TG_TRY(f->precision == TG_STORE_C64 ? equilibrate_t<float>(f) : equilibrate_t<double>(f));
     Timer::Rec rec;
     TG_TRY(c->timer.begin("lu_factor", c->stream, rec));
-    const int st = luset_factor(f->set, c->stream);
+    const int st = whole_column ? luset_factor_gepp(f->set, c->stream) : luset_factor(f->set, c->stream);
     TG_TRY(c->timer.end("lu_factor", c->stream, rec));
+    f->pivoting = whole_column ? 1 : 0;
     return st;
+}
+
+// Factor, and on a zero pivot inside the 64-row blocks factor again with whole-column
+// pivoting.
+static int factor_with_fallback(tg_factor *f) {
+    const int st = factor_cluster(f, false);
+    if (st != TG_SINGULAR) return st;
+    return factor_cluster(f, true);
+}
+
+// The fallback of a stalled or singular solve: the operator factored again in c128
+// with whole-column pivoting (once per factor).
+static int refactor_robust(tg_factor *f) {
+    tg_cluster *c = f->cluster;
+    if (f->precision == TG_STORE_C64) {
+        TG_HIP(hipStreamSynchronize(c->stream));
+        luset_release(f->set);
+        TG_TRY(f->lu.ensure(sizeof(cd) * (size_t)f->n * (size_t)f->n));
+        f->precision = TG_STORE_C128;
+        TG_TRY(luset_init(f->set, TG_STORE_C128, f->lu.ptr, {0}, {(int)f->n}));
+    }
+    f->fallback = 1;
+    return factor_cluster(f, true);
 }
 
 // x <- A^{-1} x (A^{-H} x): with equilibration A = D_r^{-1} S D_c^{-1}, so
@@ -1083,10 +1420,48 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_plain(const cx<T> *A, int64_t 
             }
 }
 
+// y = A^H x for the stored FP64 operator (n x n, column-major): one workgroup per
+// output row j = column j of A, reduced over its contiguous entries, `cols` columns.
+__global__ __launch_bounds__(256) void k_gemv_adjoint(const cd *A, int64_t n, const cd *x, cd *y, int cols) {
+    const int64_t j = blockIdx.x;
+    const cd *a = A + j * n;
+    __shared__ double red[2][256];
+    for (int cc = 0; cc < cols; ++cc) {
+        cd acc(0.0);
+        const cd *xc = x + (int64_t)cc * n;
+        for (int64_t i = threadIdx.x; i < n; i += 256) acc += conj(a[i]) * xc[i];
+        red[0][threadIdx.x] = acc.re;
+        red[1][threadIdx.x] = acc.im;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) {
+                red[0][threadIdx.x] += red[0][threadIdx.x + w];
+                red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) y[(int64_t)cc * n + j] = cd(red[0][0], red[1][0]);
+        __syncthreads();
+    }
+}
+
+static int factor_solve_once(tg_factor *f, const cd *b, cd *x, int cols, int adjoint, double rtol,
+                             std::vector<tg_convergence> &conv);
+
 // x = A^{-1} b (or A^{-H} b) in the split order with FP64 iterative refinement on the
-// matrix-free FP64 operator, until |b - A x| <= rtol |b| for every column.
+// matrix-free FP64 operator, until |b - A x| <= rtol |b| for every column.  A solve
+// that stalls or meets a non-finite residual refactors the operator once in c128 with
+// whole-column pivoting (reported by tg_factor_info) and is repeated.
 int factor_solve_split(tg_factor *f, const cd *b, cd *x, int cols, int adjoint, double rtol,
                        std::vector<tg_convergence> &conv) {
+    int st = factor_solve_once(f, b, x, cols, adjoint, rtol, conv);
+    if ((st != TG_NOT_CONVERGED && st != TG_SINGULAR) || f->fallback) return st;
+    TG_TRY(refactor_robust(f));
+    return factor_solve_once(f, b, x, cols, adjoint, rtol, conv);
+}
+
+static int factor_solve_once(tg_factor *f, const cd *b, cd *x, int cols, int adjoint, double rtol,
+                             std::vector<tg_convergence> &conv) {
     tg_cluster *c = f->cluster;
     const int64_t count = (int64_t)cols * c->n;
     DevBuf tmp;
@@ -1128,6 +1503,13 @@ int factor_solve_split(tg_factor *f, const cd *b, cd *x, int cols, int adjoint, 
                     k_zgemm_plain<double, false><<<grid, 256, 0, c->stream>>>(op, n, x, n, ax, n, n, cols, (int)n);
                 if (hipGetLastError() != hipSuccess) {
                     set_error("launch of the residual GEMM failed");
+                    st = TG_DEVICE_ERROR;
+                    break;
+                }
+            } else if (f->op.ptr && adjoint) {
+                k_gemv_adjoint<<<(unsigned)f->n, 256, 0, c->stream>>>(f->op.as<cd>(), f->n, x, ax, cols);
+                if (hipGetLastError() != hipSuccess) {
+                    set_error("launch of the adjoint residual failed");
                     st = TG_DEVICE_ERROR;
                     break;
                 }
@@ -1193,7 +1575,7 @@ int tg_factor_create(tg_cluster *c, int precision, tg_factor **out) {
     const size_t esz = precision == TG_STORE_C64 ? sizeof(cf) : sizeof(cd);
     int st = f->lu.ensure(esz * (size_t)c->n * (size_t)c->n);
     if (!st) st = luset_init(f->set, precision, f->lu.ptr, {0}, {(int)c->n});
-    if (!st) st = factor_cluster(f);
+    if (!st) st = factor_with_fallback(f);
     if (st) {
         (void)hipStreamSynchronize(c->stream);
         f->lu.release();
@@ -1214,12 +1596,76 @@ int tg_factor_destroy(tg_factor *f) {
     f->eq.release();
     f->op.release();
     luset_release(f->set);
+    if (f->dense) {
+        for (auto &w : f->cluster->ws) w.release();
+        delete f->cluster;
+    }
     delete f;
+    return TG_OK;
+}
+
+int tg_dense_factor_create(int64_t n, const tg_c128 *a, int precision, void *stream, tg_factor **out) {
+    if (!out || !a || n < 1 || n > INT32_MAX || (precision != TG_STORE_C64 && precision != TG_STORE_C128)) {
+        set_error("null argument, n outside 1..2^31-1 or precision other than TG_STORE_C64 / TG_STORE_C128");
+        return TG_INVALID_INPUT;
+    }
+    *out = nullptr;
+    tg_factor *f = new tg_factor;
+    f->cluster = new tg_cluster();
+    f->cluster->n = n;
+    f->cluster->stream = (hipStream_t)stream;
+    f->dense = true;
+    f->precision = precision;
+    f->n = n;
+    const size_t nn = (size_t)n * (size_t)n;
+    const size_t esz = precision == TG_STORE_C64 ? sizeof(cf) : sizeof(cd);
+    int st = f->op.ensure(sizeof(cd) * nn);
+    if (!st && hipMemcpyAsync(f->op.ptr, a, sizeof(cd) * nn, hipMemcpyDeviceToDevice, f->cluster->stream) != hipSuccess) {
+        set_error("copy of the matrix failed");
+        st = TG_DEVICE_ERROR;
+    }
+    if (!st) st = f->lu.ensure(esz * nn);
+    if (!st) st = luset_init(f->set, precision, f->lu.ptr, {0}, {(int)n});
+    if (!st) st = factor_with_fallback(f);
+    if (st) {
+        tg_factor_destroy(f);
+        return st;
+    }
+    *out = f;
+    return TG_OK;
+}
+
+int tg_dense_factor_solve(tg_factor *f, const tg_c128 *b, int cols, double rtol, int adjoint, tg_c128 *x,
+                          tg_convergence *conv_host) {
+    if (!f || !b || !x || cols < 1 || !(rtol > 0.0)) {
+        set_error("null argument, no columns or non-positive rtol");
+        return TG_INVALID_INPUT;
+    }
+    std::vector<tg_convergence> conv;
+    int st = factor_solve_split(f, (const cd *)b, (cd *)x, cols, adjoint, rtol, conv);
+    if (hipStreamSynchronize(f->cluster->stream) != hipSuccess && !st) {
+        set_error("device failure in the LU solve");
+        st = TG_DEVICE_ERROR;
+    }
+    if (conv_host)
+        for (int k = 0; k < cols && k < (int)conv.size(); ++k) conv_host[k] = conv[k];
+    return st;
+}
+
+int tg_factor_info(const tg_factor *f, int *precision, int *pivoting, int *fallback) {
+    if (!f) {
+        set_error("null factor");
+        return TG_INVALID_INPUT;
+    }
+    if (precision) *precision = f->precision;
+    if (pivoting) *pivoting = f->pivoting;
+    if (fallback) *fallback = f->fallback;
     return TG_OK;
 }
 
 int tg_factor_solve(tg_factor *f, const tg_c128 *incident, int cols, double rtol, int adjoint, tg_c128 *x,
                     tg_convergence *conv_host) {
+    if (f && f->dense) return dense_factor_error();
     if (!f || !incident || !x || cols < 1 || !(rtol > 0.0)) {
         set_error("null argument, no columns or non-positive rtol");
         return TG_INVALID_INPUT;
@@ -1259,6 +1705,7 @@ int tg_factor_solve(tg_factor *f, const tg_c128 *incident, int cols, double rtol
 int tg_factor_equilibrated(const tg_factor *f) { return f && f->equilibrated ? 1 : 0; }
 
 int tg_factor_global_tmatrix(tg_factor *f, int lmax_global, double rtol, tg_c128 *out, tg_convergence *conv_host) {
+    if (f && f->dense) return dense_factor_error();
     if (!f || !out || !(rtol > 0.0)) {
         set_error("null argument or non-positive rtol");
         return TG_INVALID_INPUT;
@@ -1328,6 +1775,7 @@ int tg_factor_global_tmatrix(tg_factor *f, int lmax_global, double rtol, tg_c128
 int tg_factor_pullback(tg_factor *f, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent, int cols,
                        double rtol, double *grad_k0, double *grad_radii, tg_c128 *grad_eps, double *grad_positions,
                        tg_c128 *incident_grad, tg_convergence *conv_host) {
+    if (f && f->dense) return dense_factor_error();
     if (!f || !(rtol > 0.0)) {
         set_error("null factor or non-positive rtol");
         return TG_INVALID_INPUT;
@@ -1343,6 +1791,7 @@ int tg_factor_pushforward(tg_factor *f, const tg_c128 *incident, const tg_c128 *
                           double tangent_k0, const double *tangent_radii, const tg_c128 *tangent_eps,
                           const double *tangent_positions, const tg_c128 *tangent_incident, tg_c128 *dx,
                           tg_convergence *conv_host) {
+    if (f && f->dense) return dense_factor_error();
     if (!f || !(rtol > 0.0)) {
         set_error("null factor or non-positive rtol");
         return TG_INVALID_INPUT;
@@ -1351,7 +1800,37 @@ int tg_factor_pushforward(tg_factor *f, const tg_c128 *incident, const tg_c128 *
         return factor_solve_split(f, b, y, k, 0, rtol, conv);
     };
     return pushforward_with(f->cluster, incident, x, cols, tangent_k0, tangent_radii, tangent_eps,
-                            tangent_positions, tangent_incident, solve, dx, conv_host);
+                            tangent_positions, nullptr, true, tangent_incident, solve, dx, conv_host);
+}
+
+int tg_factor_pushforward_local(tg_factor *f, const tg_c128 *incident, const tg_c128 *x, int cols, double rtol,
+                                double tangent_k0, const double *tangent_positions, const tg_c128 *tangent_local,
+                                const tg_c128 *tangent_incident, tg_c128 *dx, tg_convergence *conv_host) {
+    if (f && f->dense) return dense_factor_error();
+    if (!f || !(rtol > 0.0)) {
+        set_error("null factor or non-positive rtol");
+        return TG_INVALID_INPUT;
+    }
+    TangentSolve solve = [f, rtol](const cd *b, cd *y, int k, std::vector<tg_convergence> &conv) {
+        return factor_solve_split(f, b, y, k, 0, rtol, conv);
+    };
+    return pushforward_with(f->cluster, incident, x, cols, tangent_k0, nullptr, nullptr, tangent_positions,
+                            tangent_local, false, tangent_incident, solve, dx, conv_host);
+}
+
+int tg_factor_pullback_local(tg_factor *f, const tg_c128 *incident, const tg_c128 *x, const tg_c128 *cotangent,
+                             int cols, double rtol, double *grad_k0, double *grad_positions, tg_c128 *grad_local,
+                             tg_c128 *incident_grad, tg_convergence *conv_host) {
+    if (f && f->dense) return dense_factor_error();
+    if (!f || !(rtol > 0.0)) {
+        set_error("null factor or non-positive rtol");
+        return TG_INVALID_INPUT;
+    }
+    TangentSolve solve = [f, rtol](const cd *g, cd *y, int k, std::vector<tg_convergence> &conv) {
+        return factor_solve_split(f, g, y, k, 1, rtol, conv);
+    };
+    return pullback_local_with(f->cluster, incident, x, cotangent, cols, solve, grad_k0, grad_positions, grad_local,
+                               incident_grad, conv_host);
 }
 
 }  // extern "C"

@@ -45,10 +45,44 @@ __global__ void k_local_dense(const cd *__restrict__ tloc, int H, int64_t n, int
     y[t] = sub ? sub[t] - acc : acc;
 }
 
-int local_apply_particles(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *sub) {
+int local_apply_dense(tg_cluster *c, const cd *panels, const cd *x, cd *y, int cols, int adjoint, const cd *sub) {
     const int64_t count = (int64_t)cols * c->n;
-    k_local_dense<<<(unsigned)((count + 255) / 256), 256, 0, c->stream>>>(c->b_tloc.as<cd>(), c->H, c->n, c->nh,
-                                                                          cols, x, y, adjoint, sub);
+    k_local_dense<<<(unsigned)((count + 255) / 256), 256, 0, c->stream>>>(panels, c->H, c->n, c->nh, cols, x, y,
+                                                                          adjoint, sub);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+int local_apply_particles(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *sub) {
+    return local_apply_dense(c, c->b_tloc.as<cd>(), x, y, cols, adjoint, sub);
+}
+
+int tloc_from_ref(tg_cluster *c, const cd *ref, cd *panels) {
+    const int64_t count = (int64_t)c->N * c->M * c->M;
+    k_tloc_from_ref<<<(unsigned)((count + 255) / 256), 256, 0, c->stream>>>(ref, count, c->M, c->H, panels);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+// T-bar_p = sum_c Y_p[:, c] R_p[:, c]^H per particle, in the reference mode order
+// (column-major M x M per particle), from split-layout Y and R; one thread per entry.
+__global__ void k_local_cotangent(const cd *__restrict__ Y, const cd *__restrict__ R, int N, int M, int H,
+                                  int64_t n, int64_t nh, int cols, cd *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t MM = (int64_t)M * M;
+    if (t >= (int64_t)N * MM) return;
+    const int64_t p = t / MM;
+    const int rem = (int)(t - p * MM), s = rem / M, r = rem - s * M;
+    const int64_t ir = (int64_t)(r & 1) * nh + p * H + (r >> 1), is = (int64_t)(s & 1) * nh + p * H + (s >> 1);
+    cd acc(0.0);
+    for (int cc = 0; cc < cols; ++cc) acc += Y[(int64_t)cc * n + ir] * conj(R[(int64_t)cc * n + is]);
+    out[t] = acc;
+}
+
+int local_cotangent(tg_cluster *c, const cd *Y, const cd *R, int cols, cd *out_ref) {
+    const int64_t count = (int64_t)c->N * c->M * c->M;
+    k_local_cotangent<<<(unsigned)((count + 255) / 256), 256, 0, c->stream>>>(Y, R, c->N, c->M, c->H, c->n, c->nh,
+                                                                              cols, out_ref);
     TG_HIP(hipGetLastError());
     return TG_OK;
 }

@@ -32,13 +32,13 @@ __global__ void k_mie_tangent(const cd *mie, int N, int lmax, double k0, const d
 
 int pushforward_with(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, int cols, double tangent_k0,
                      const double *tangent_radii, const tg_c128 *tangent_eps, const double *tangent_positions,
-                     const tg_c128 *tangent_incident, const TangentSolve &solve, tg_c128 *dx,
-                     tg_convergence *conv_host) {
+                     const tg_c128 *tangent_local, bool physical, const tg_c128 *tangent_incident,
+                     const TangentSolve &solve, tg_c128 *dx, tg_convergence *conv_host) {
     if (!c || !incident || !x || !dx || cols < 1) {
         set_error("null argument or no columns");
         return TG_INVALID_INPUT;
     }
-    TG_TRY(require_spheres(c, "the pushforward from radii, permittivities and positions"));
+    if (physical) TG_TRY(require_spheres(c, "the pushforward from radii and permittivities"));
     const int N = c->N;
     // cluster/iterative.rs:378-394: finite tangents of the recorded shapes
     bool ok = std::isfinite(tangent_k0);
@@ -58,7 +58,7 @@ int pushforward_with(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, i
         return TG_INVALID_INPUT;
     }
     const int64_t count = (int64_t)cols * c->n;
-    const size_t nblk = (size_t)N * c->lmax * 4;
+    const size_t nblk = physical ? (size_t)N * c->lmax * 4 : tangent_local ? (size_t)N * c->M * c->M : 0;
     DevBuf io, tan;
     // vectors a, X, dB, R (response), W (work), rhs, dX; then dT blocks, dpos[3N], dr[N], deps[2N]
     TG_TRY(io.ensure(sizeof(cd) * (count * 7 + nblk) + sizeof(double) * 6 * (size_t)N));
@@ -101,16 +101,22 @@ int pushforward_with(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, i
         if ((st = vec_axpby(c->stream, cd(1.0), dB, cd(1.0), W, count))) break;
         // rhs = T (dB + dC X) + dT (B + C X)
         if ((st = local_apply(c, W, rhs, cols, 0, nullptr))) break;
-        const double *radii = c->b_pos.as<double>() + 3 * N;
-        k_mie_tangent<<<(unsigned)((nblk + 255) / 256), 256, 0, c->stream>>>(c->b_mie.as<cd>(), N, c->lmax, c->k0,
-                                                                             radii, tangent_k0, dr, deps, dT);
-        if (hipGetLastError() != hipSuccess) {
-            set_error("Mie tangent launch failed");
-            st = TG_DEVICE_ERROR;
-            break;
+        if (physical) {
+            const double *radii = c->b_pos.as<double>() + 3 * N;
+            k_mie_tangent<<<(unsigned)((nblk + 255) / 256), 256, 0, c->stream>>>(c->b_mie.as<cd>(), N, c->lmax,
+                                                                                 c->k0, radii, tangent_k0, dr, deps,
+                                                                                 dT);
+            if (hipGetLastError() != hipSuccess) {
+                set_error("Mie tangent launch failed");
+                st = TG_DEVICE_ERROR;
+                break;
+            }
+            if ((st = local_apply_blocks(c, dT, R, W, cols, 0, nullptr))) break;
+        } else if (tangent_local) {
+            if ((st = tloc_from_ref(c, (const cd *)tangent_local, dT))) break;
+            if ((st = local_apply_dense(c, dT, R, W, cols, 0, nullptr))) break;
         }
-        if ((st = local_apply_blocks(c, dT, R, W, cols, 0, nullptr))) break;
-        if ((st = vec_axpby(c->stream, cd(1.0), W, cd(1.0), rhs, count))) break;
+        if ((physical || tangent_local) && (st = vec_axpby(c->stream, cd(1.0), W, cd(1.0), rhs, count))) break;
         st = solve(rhs, dXs, cols, conv);
         if (st && st != TG_NOT_CONVERGED) break;
         int s2 = split_to_ref(c, dXs, (cd *)dx, cols);
@@ -147,8 +153,26 @@ int tg_cluster_pushforward(tg_cluster *c, const tg_c128 *incident, const tg_c128
     TangentSolve solve = [c, od](const cd *b, cd *y, int k, std::vector<tg_convergence> &conv) {
         return solve_split(c, b, y, k, od, 0, conv);
     };
-    return pushforward_with(c, incident, x, cols, tangent_k0, tangent_radii, tangent_eps, tangent_positions,
-                            tangent_incident, solve, dx, conv_host);
+    return pushforward_with(c, incident, x, cols, tangent_k0, tangent_radii, tangent_eps, tangent_positions, nullptr,
+                            true, tangent_incident, solve, dx, conv_host);
+}
+
+int tg_cluster_pushforward_local(tg_cluster *c, const tg_c128 *incident, const tg_c128 *x, int cols,
+                                 const tg_gmres_options *o, double tangent_k0, const double *tangent_positions,
+                                 const tg_c128 *tangent_local, const tg_c128 *tangent_incident, tg_c128 *dx,
+                                 tg_convergence *conv_host) {
+    if (!o) {
+        set_error("null options");
+        return TG_INVALID_INPUT;
+    }
+    tg_gmres_options od = *o;
+    od.atol = 0.0;
+    if (!(od.rtol > 0)) od.rtol = 1e-10;
+    TangentSolve solve = [c, od](const cd *b, cd *y, int k, std::vector<tg_convergence> &conv) {
+        return solve_split(c, b, y, k, od, 0, conv);
+    };
+    return pushforward_with(c, incident, x, cols, tangent_k0, nullptr, nullptr, tangent_positions, tangent_local,
+                            false, tangent_incident, solve, dx, conv_host);
 }
 
 }  // extern "C"
