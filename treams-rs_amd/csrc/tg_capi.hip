@@ -109,6 +109,9 @@ int DevBuf::ensure(size_t need) {
         e = hipMalloc(&ptr, need);
     }
     if (e != hipSuccess) {
+        // a failed hipMalloc leaves its error as the runtime's last error, which the next
+        // launch check (hipGetLastError) would report as its own: clear it here
+        (void)hipGetLastError();
         ptr = nullptr;
         set_error("device allocation of %zu bytes failed: %s", need, hipGetErrorString(e));
         return e == hipErrorOutOfMemory ? TG_OUT_OF_MEMORY : TG_DEVICE_ERROR;
