@@ -141,11 +141,12 @@ def attach(cluster: IterativeSphereCluster, mlp: MLP | None, k_nearest: int = K_
     cluster._mlp = mlp  # keep the network alive as long as the cluster uses it
 
 
-def training_cluster(seed: int, rng: np.random.Generator | None = None):
-    """One training cluster: N in {64..256}, lmax in {3..6}, k0 U[0.8, 2.0], radii
-    U[0.15, 0.35], eps U[2,6] + i U[0,0.2], volume fill fraction U[0.05, 0.25]."""
+def training_cluster(seed: int, rng: np.random.Generator | None = None, n_range=(64, 1024)):
+    """One training cluster of the training-set distribution: N uniform in
+    {64..1024}, lmax in {3..6}, k0 U[0.8, 2.0], radii U[0.15, 0.35], eps U[2,6] +
+    i U[0,0.2], volume fill fraction U[0.05, 0.25]."""
     rng = rng or np.random.default_rng(seed)
-    N = int(rng.integers(64, 257))
+    N = int(rng.integers(n_range[0], n_range[1] + 1))
     lmax = int(rng.integers(3, 7))
     k0 = float(rng.uniform(0.8, 2.0))
     fill = float(rng.uniform(0.05, 0.25))
@@ -184,30 +185,45 @@ def error_radius(cluster: IterativeSphereCluster, probes: int = 4, seed: int = 0
     return float(torch.linalg.norm(W, dim=0).max())
 
 
-def train(n_clusters=48, epochs=6, probes=4, lr=1e-3, seed=1000, log=print, mlp: MLP | None = None) -> MLP:
-    """Train on clusters the method generates (seeds seed..seed+n_clusters-1), each
-    with a fixed block of Gaussian probe columns."""
+def train(n_clusters=48, epochs=6, probes=4, lr=1e-3, seed=1000, log=print, mlp: MLP | None = None,
+          n_range=(64, 1024), lr_final=None, log_every=256) -> MLP:
+    """Train on clusters the method generates (seeds seed..seed+n_clusters-1, drawn
+    from the training-set distribution; a new order every epoch), one Adam step per
+    cluster on the dominant error modes of the current preconditioner.  The learning
+    rate decays on a cosine from `lr` to `lr_final` (default lr / 10) over all steps."""
     mlp = mlp or MLP(seed=seed)
-    clusters = [training_cluster(seed + c) for c in range(n_clusters)]
+    clusters = [training_cluster(seed + c, n_range=n_range) for c in range(n_clusters)]
+    lr_final = lr / 10 if lr_final is None else lr_final
+    total = max(1, epochs * n_clusters)
+    order_rng = np.random.default_rng(seed)
+    done = 0
     for epoch in range(epochs):
         t0 = time.perf_counter()
         losses, base = [], []
-        for c, (lmax, k0, radii, eps, pos) in enumerate(clusters):
+        order = order_rng.permutation(n_clusters) if epoch else np.arange(n_clusters)
+        for k, c in enumerate(order):
+            lmax, k0, radii, eps, pos = clusters[c]
             cl = IterativeSphereCluster(lmax, k0, radii, eps, pos)
             npairs = near_pairs(cl)
             feats = pair_features(cl, npairs)
             out = mlp.forward(feats)
             set_pair_output(cl, out)
-            V = dominant_probes(cl, probes, seed * 7919 + c)
+            V = dominant_probes(cl, probes, seed * 7919 + int(c) + 104729 * epoch)
             loss, dout = learned_loss(cl, V, npairs)
             mlp.zero_grad()
             mlp.backward(dout)
-            mlp.adam(lr=lr)
+            rate = lr_final + 0.5 * (lr - lr_final) * (1.0 + np.cos(np.pi * done / total))
+            mlp.adam(lr=float(rate))
+            done += 1
             losses.append(loss)
             if epoch == 0:
                 set_pair_output(cl, torch.zeros_like(out))
                 base.append(learned_loss(cl, V, npairs, grad=False)[0])
             del cl
+            if log_every and (k + 1) % log_every == 0:
+                log(f"epoch {epoch + 1} cluster {k + 1}/{n_clusters}: mean loss {np.mean(losses[-log_every:]):.4e}"
+                    + (f", alpha = 1: {np.mean(base[-log_every:]):.4e}" if base else "")
+                    + f" ({time.perf_counter() - t0:.1f} s)")
         msg = f"epoch {epoch + 1}/{epochs}: mean loss {np.mean(losses):.4e} ({time.perf_counter() - t0:.1f} s)"
         if base:
             msg += f"; plain Neumann (alpha = 1) {np.mean(base):.4e}"
