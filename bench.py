@@ -542,6 +542,31 @@ def main() -> int:
     gpu_check["cx64"] = cl.coupling(x_mixed)[sl].cpu().numpy()
     gpu_check["cx16"] = cl.coupling(x_mixed, c64=True)[sl].cpu().numpy()
 
+    # iterations of the headline's mixed solve with each preconditioner (outside the
+    # timed steps): none, block-Jacobi, the near-field form without the network (alpha
+    # = 1, an MLP with a zero last layer) and with the trained network
+    if args.pc == "learned":
+        pcs = {}
+        for name, pc, net in (("none", "none", None), ("block_jacobi", "block_jacobi", None),
+                              ("near_field_alpha1", "learned", learned.MLP(seed=1)), ("learned", "learned", mlp)):
+            cl.reset()
+            if net is not None:
+                learned.attach(cl, net)
+            torch.cuda.synchronize()
+            tp0 = time.perf_counter()
+            _, cv = cl.solve(B, rtol=args.rtol, restart=args.restart, max_iterations=2000,
+                             mixed_precision="matrix_free" if args.operator == "tables" else True, preconditioner=pc,
+                             convergence="full")
+            torch.cuda.synchronize()
+            pcs[name] = {"iterations": max(c.iterations for c in cv), "refinements": max(c.refinements for c in cv),
+                         "ms": 1e3 * (time.perf_counter() - tp0)}
+        learned.attach(cl, mlp)
+        result["preconditioner_iterations"] = pcs
+        result["preconditioner_iterations_note"] = (
+            "one mixed-precision job each, same operator, tolerance and restart; ms includes the "
+            "preconditioner's set-up (block-Jacobi factors, or near pairs, features and network)")
+        log(f"preconditioner iterations: {pcs}")
+
     # near fields of the headline solution: 128 x 128 points on the plane y = 0 across
     # the cluster, 4 columns (one thread per point, N M waves each)
     g = np.linspace(-10.0, 10.0, 128)

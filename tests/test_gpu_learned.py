@@ -121,6 +121,31 @@ def test_learned_apply_many_columns(pkg, oracle, learned, lmax, cols):
     assert np.linalg.norm(y - Minv @ x) <= 1e-12 * np.linalg.norm(Minv @ x)
 
 
+@pytest.mark.parametrize("lmax,cols", [(2, 5), (6, 6)])
+def test_learned_apply_c64_inner_path(pkg, oracle, learned, lmax, cols):
+    """The mixed-precision inner solves' apply (c64 same-helicity near blocks, FP32
+    products and sums, 8 block loads in flight) against the dense oracle: c64 storage
+    bounds the error near 1e-7 of the result; 5e-6 leaves room for FP32 sums over
+    the near pairs of one sphere."""
+    import mlp_oracle
+
+    cl, ref = _small_cluster(pkg, oracle, seed=12, N=9, lmax=lmax)
+    npairs = learned.near_pairs(cl, 8)
+    pairs = learned.pair_list(cl, npairs)
+    rng = np.random.default_rng(20 + lmax)
+    out = (0.3 * rng.normal(size=(npairs, 16))).astype(np.float32)
+    learned.set_pair_output(cl, torch.from_numpy(out))
+    Minv = mlp_oracle.learned_inverse(ref, pairs, out)
+    x = rng.normal(size=(ref.n, cols)) + 1j * rng.normal(size=(ref.n, cols))
+    want = Minv @ x
+    y = learned.learned_apply(cl, x, c64=True).cpu().numpy()
+    assert np.linalg.norm(y - want) <= 5e-6 * np.linalg.norm(want)
+    # and not merely the c128 path again: the two differ at c64's precision
+    y128 = learned.learned_apply(cl, x).cpu().numpy()
+    assert np.linalg.norm(y128 - want) <= 1e-12 * np.linalg.norm(want)
+    assert np.linalg.norm(y - y128) > 1e-12 * np.linalg.norm(want)
+
+
 def test_learned_preconditioned_solve_same_answer(pkg, oracle, learned):
     cl, ref = _small_cluster(pkg, oracle, seed=6, N=24, lmax=4)
     mlp = learned.MLP(seed=2)

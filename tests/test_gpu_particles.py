@@ -2,6 +2,8 @@
 FP64 oracle's dense systems and the reference's invariances
 (cluster/particles.rs:65-123, python/treams_rs/diff.py:1473-1523,
 tests/tmatrix/test_particle_cluster.py:133-143, 343-366)."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -166,3 +168,38 @@ def test_particle_cluster_strong_coupling_exchanges_rows(pkg, oracle, N, side, s
     print(f"N={N} {precision}: residual {resid:.2e}, distance to the dense solve {err:.2e}")
     assert resid <= 1e-11  # observed 4e-14 - 1.3e-13
     assert err <= 1e-10    # observed 8e-14 - 2.9e-13
+
+
+def test_particle_cluster_parity_basis_matches_mie_spheres(pkg, golden_dir):
+    """Spheres given in the parity basis as diag(-a_l (N, pol 1), -b_l (M, pol 0)) with
+    the Bohren-Huffman a_l, b_l of the mpmath fixtures (treams' T = -a (N), -b (M)) give
+    the sphere cluster's interacting T-matrix in the parity basis: pins the helicity /
+    parity convention (H = (N +- M) / sqrt(2)) through the Mie physics."""
+    import json
+
+    rows = json.load(open(os.path.join(golden_dir, "special_functions.json")))["mie_bohren_huffman"]
+    cases = [(0.3, complex(4, 0.1)), (0.455, complex(6, 0.2)), (1.1, complex(2.5, 0))]
+    lmax = 2
+    modes = [(l, m, p) for l in range(1, lmax + 1) for m in range(-l, l + 1) for p in (1, 0)]
+    local = []
+    for x, eps in cases:
+        ab = {r["n"]: (complex(*r["a"]), complex(*r["b"])) for r in rows
+              if abs(r["x"] - x) < 1e-12 and complex(*r["eps"]) == eps}
+        local.append(np.diag([-(ab[l][0] if p == 1 else ab[l][1]) for (l, m, p) in modes]))
+    pos = np.array([[0.0, 0.0, 0.0], [3.0, 0.5, 0.0], [0.4, -0.3, 3.2]])
+    Xp, ctx = pkg.diff.particle_cluster(local, pos, 1.0, modes=[modes] * 3, poltype="parity")
+    Xh, _ = pkg.diff.sphere_cluster(lmax, 1.0, [c[0] for c in cases], [c[1] for c in cases], pos)
+    U = pkg.diff.parity_change(modes)
+    big = np.kron(np.eye(3), U)
+    expected = big @ Xh.cpu().numpy() @ big
+    assert np.linalg.norm(Xp.cpu().numpy() - expected) <= 1e-11 * np.linalg.norm(expected)
+    # derivatives in the parity basis: adjoint identity of pushforward and pullback
+    rng = np.random.default_rng(2)
+    dl = [rng.normal(size=(16, 16)) + 1j * rng.normal(size=(16, 16)) for _ in range(3)]
+    dp = rng.normal(size=(3, 3))
+    dX = ctx.pushforward(dl, dp, 0.07).cpu().numpy()
+    G = rng.normal(size=dX.shape) + 1j * rng.normal(size=dX.shape)
+    gl, gp, gk = ctx.pullback(G)
+    lhs = np.real(np.vdot(G, dX))
+    rhs = sum(np.real(np.vdot(g.cpu().numpy(), d)) for g, d in zip(gl, dl)) + np.sum(gp * dp) + gk * 0.07
+    assert abs(lhs - rhs) <= 1e-9 * np.linalg.norm(G) * np.linalg.norm(dX)

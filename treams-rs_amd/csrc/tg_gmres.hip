@@ -28,9 +28,11 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
     hipStream_t s = c->stream;
     const int64_t n = c->n;
     const int restart = (int)std::min<int64_t>(std::min(prm.restart, prm.max_iterations), n);
-    // buffers: V [cols][restart+1][n], W, X packed [cols][n], R residual [cols][n]
-    const int64_t csv = (int64_t)(restart + 1) * n;
-    TG_TRY(c->ws[2].ensure(sizeof(cd) * (size_t)cols * csv));
+    // buffers: V [restart+1][cols][n] (basis vector j of every column contiguous, so
+    // a step's operator input is V[j] itself and its normalisation one launch), W, X
+    // packed [cols][n], R residual [cols][n]
+    const int64_t csv = n, vstep = (int64_t)cols * n;
+    TG_TRY(c->ws[2].ensure(sizeof(cd) * (size_t)(restart + 1) * vstep));
     TG_TRY(c->ws[3].ensure(sizeof(cd) * (size_t)cols * n * 3));
     // device scalars: solution coefficients, both Gram-Schmidt passes' projections and
     // |w|^2 of an iteration, then the column scales and the extending mask
@@ -88,7 +90,7 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
             scale_h[cols + k] = cycle[k] ? 1.0 : 0.0;  // the extending mask
         }
         TG_HIP(hipMemcpyAsync(scale, scale_h.data(), sizeof(double) * 2 * cols, hipMemcpyHostToDevice, s));
-        for (int k = 0; k < cols; ++k) TG_TRY(vec_scale_cols(s, scale + k, R + (int64_t)k * n, V + k * csv, n, 1));
+        TG_TRY(vec_scale_cols(s, scale, R, V, n, cols));
         std::fill(H.begin(), H.end(), zc(0.0));
         std::fill(g.begin(), g.end(), zc(0.0));
         for (int k = 0; k < cols; ++k) {
@@ -97,25 +99,23 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
             extending[k] = cycle[k];
         }
         for (int j = 0; j < restart; ++j) {
-            // X = V[:, j]; W = A M^{-1} X
-            for (int k = 0; k < cols; ++k)
-                TG_HIP(hipMemcpyAsync(X + (int64_t)k * n, V + k * csv + (int64_t)j * n, sizeof(cd) * n,
-                                      hipMemcpyDeviceToDevice, s));
+            // W = A M^{-1} V[j]
+            const cd *Vj = V + (int64_t)j * vstep;
             if (Minv) {
-                TG_TRY((*Minv)(X, R, cols));
+                TG_TRY((*Minv)(Vj, R, cols));
                 TG_TRY(A(R, W, cols));
             } else {
-                TG_TRY(A(X, W, cols));
+                TG_TRY(A(Vj, W, cols));
             }
             // two classical Gram-Schmidt passes against V[:, 0..j], their projections
             // (zero for columns no longer extending) kept on the device; one transfer
             // brings both passes' projections and |w|^2 to the host
             const size_t nj = (size_t)cols * (j + 1);
             cd *p0 = proj, *p1 = proj + nj, *pn = proj + 2 * nj;
-            TG_TRY(vec_dots_dev(s, V, n, j + 1, W, n, cols, csv, n, p0, mask, c->ws[4]));
-            TG_TRY(vec_update(s, V, n, j + 1, p0, W, n, cols, csv, n, -1.0));
-            TG_TRY(vec_dots_dev(s, V, n, j + 1, W, n, cols, csv, n, p1, mask, c->ws[4]));
-            TG_TRY(vec_update(s, V, n, j + 1, p1, W, n, cols, csv, n, -1.0));
+            TG_TRY(vec_dots_dev(s, V, vstep, j + 1, W, n, cols, csv, n, p0, mask, c->ws[4]));
+            TG_TRY(vec_update(s, V, vstep, j + 1, p0, W, n, cols, csv, n, -1.0));
+            TG_TRY(vec_dots_dev(s, V, vstep, j + 1, W, n, cols, csv, n, p1, mask, c->ws[4]));
+            TG_TRY(vec_update(s, V, vstep, j + 1, p1, W, n, cols, csv, n, -1.0));
             TG_TRY(vec_dots_dev(s, W, 0, 1, W, n, cols, n, n, pn, nullptr, c->ws[4]));
             TG_HIP(hipMemcpyAsync(proj_h.data(), proj, sizeof(cd) * (2 * nj + cols), hipMemcpyDeviceToHost, s));
             TG_HIP(hipStreamSynchronize(s));
@@ -176,12 +176,7 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
             for (int k = 0; k < cols; ++k) scale_h[cols + k] = extending[k] ? 1.0 : 0.0;
             TG_HIP(hipMemcpyAsync(scale, scale_h.data(), sizeof(double) * 2 * cols, hipMemcpyHostToDevice, s));
             ++iterations;
-            if (j + 1 < restart || more) {
-                if (j + 1 < restart)
-                    for (int k = 0; k < cols; ++k)
-                        TG_TRY(vec_scale_cols(s, scale + k, W + (int64_t)k * n, V + k * csv + (int64_t)(j + 1) * n, n,
-                                              1));
-            }
+            if (j + 1 < restart && more) TG_TRY(vec_scale_cols(s, scale, W, V + (int64_t)(j + 1) * vstep, n, cols));
             if (!more) break;
         }
         // back substitution and x += M^{-1} V y, over the basis vectors written in this
@@ -204,7 +199,7 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
         TG_HIP(hipMemsetAsync(X, 0, sizeof(cd) * cols * n, s));
         if (nuse > 0) {
             TG_HIP(hipMemcpyAsync(coef, hbuf.data(), sizeof(cd) * cols * nuse, hipMemcpyHostToDevice, s));
-            TG_TRY(vec_update(s, V, n, nuse, coef, X, n, cols, csv, n, 1.0));
+            TG_TRY(vec_update(s, V, vstep, nuse, coef, X, n, cols, csv, n, 1.0));
         }
         if (Minv) {
             TG_TRY((*Minv)(X, W, cols));

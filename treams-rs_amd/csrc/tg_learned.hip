@@ -137,46 +137,63 @@ __global__ __launch_bounds__(64) void k_near_pair_products(int M, int H, int lma
 // [p][h][a][b]: one workgroup per pair, wave h takes the helicity-h rows, lane l the
 // destinations b = l + 64 k (reference row 2 b + h); the sum runs over the H sources a
 // of that helicity.  The source values x_j (both helicities, up to 4 columns, zero
-// past nc) are staged in LDS first, so the loop is branch-free: one block load and
-// four broadcast LDS reads per source.
+// past nc) are staged in LDS as FP32 first, so the loop is branch-free: the block
+// values of 8 sources are loaded before any is used (8 loads in flight per lane),
+// then multiplied with FP32 accumulation (the blocks are c64; the inner solves this
+// preconditioner serves run at c64 precision, and the FP64 residual certifies them).
 template <int RS>
 __global__ __launch_bounds__(128) void k_near_pair_products_helicity(int H, int lmax, int64_t n, int64_t nh, int nc,
                                                                      const int *__restrict__ pj,
                                                                      const cf *__restrict__ blocks,
                                                                      const cd *__restrict__ alpha,
                                                                      const cd *__restrict__ x, cd *__restrict__ part) {
-    constexpr int NC = 4;
-    extern __shared__ cd xs[];  // [h][a][NC]
+    constexpr int NC = 4, U = 8;
+    extern __shared__ float2 xsf[];  // [h][a][NC]
     const int p = blockIdx.x, lane = threadIdx.x & 63, M = 2 * H;
     const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int j = pj[p];
     for (int t = threadIdx.x; t < 2 * H * NC; t += blockDim.x) {
         const int cc = t % NC, ha = t / NC, hh = ha / H, a = ha - hh * H;
-        xs[t] = cc < nc ? x[(int64_t)cc * n + (int64_t)hh * nh + (int64_t)j * H + a] : cd(0.0);
+        const cd v = cc < nc ? x[(int64_t)cc * n + (int64_t)hh * nh + (int64_t)j * H + a] : cd(0.0);
+        xsf[t] = make_float2((float)v.re, (float)v.im);
     }
     __syncthreads();
     const cf *blk = blocks + ((int64_t)p * 2 + h) * H * H;
-    const cd *xh = xs + h * H * NC;
+    const float2 *xh = xsf + h * H * NC;
     int dst[RS];
 #pragma unroll
     for (int k = 0; k < RS; ++k) dst[k] = min(lane + 64 * k, H - 1);  // destinations >= H unused
-    cd sum[RS][NC];
+    float2 sum[RS][NC];
 #pragma unroll
     for (int k = 0; k < RS; ++k)
 #pragma unroll
-        for (int cc = 0; cc < NC; ++cc) sum[k][cc] = cd(0.0);
-#pragma unroll 4
-    for (int a = 0; a < H; ++a) {
-        cd xv[NC];
+        for (int cc = 0; cc < NC; ++cc) sum[k][cc] = make_float2(0.f, 0.f);
+    auto mac = [&](int a, const cf (&bv)[RS]) {
 #pragma unroll
-        for (int cc = 0; cc < NC; ++cc) xv[cc] = xh[a * NC + cc];
+        for (int cc = 0; cc < NC; ++cc) {
+            const float2 xv = xh[a * NC + cc];
 #pragma unroll
-        for (int k = 0; k < RS; ++k) {
-            const cf bv = blk[a * H + dst[k]];
-            const cd bb((double)bv.re, (double)bv.im);
-#pragma unroll
-            for (int cc = 0; cc < NC; ++cc) sum[k][cc] += bb * xv[cc];
+            for (int k = 0; k < RS; ++k) {
+                sum[k][cc].x = fmaf(bv[k].re, xv.x, fmaf(-bv[k].im, xv.y, sum[k][cc].x));
+                sum[k][cc].y = fmaf(bv[k].re, xv.y, fmaf(bv[k].im, xv.x, sum[k][cc].y));
+            }
         }
+    };
+    int a0 = 0;
+    for (; a0 + U <= H; a0 += U) {
+        cf bv[U][RS];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < RS; ++k) bv[u][k] = blk[(a0 + u) * H + dst[k]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) mac(a0 + u, bv[u]);
+    }
+    for (; a0 < H; ++a0) {
+        cf bv[RS];
+#pragma unroll
+        for (int k = 0; k < RS; ++k) bv[k] = blk[a0 * H + dst[k]];
+        mac(a0, bv);
     }
 #pragma unroll
     for (int k = 0; k < RS; ++k) {
@@ -186,7 +203,7 @@ __global__ __launch_bounds__(128) void k_near_pair_products_helicity(int H, int 
             const cd al = alpha[(int64_t)p * lmax + row_degree(r) - 1];
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc)
-                if (cc < nc) part[((int64_t)p * NC + cc) * M + r] = al * sum[k][cc];
+                if (cc < nc) part[((int64_t)p * NC + cc) * M + r] = al * cd((double)sum[k][cc].x, (double)sum[k][cc].y);
         }
     }
 }
@@ -516,7 +533,7 @@ static int learned_near_apply_c64(tg_cluster *c, const cd *x, cd *y, int cols) {
         const cd *xc = x + (int64_t)c0 * c->n;
         const int rs = (H + 63) / 64;
 #define TG_NEARH(RS)                                                                                               \
-    k_near_pair_products_helicity<RS><<<(unsigned)s->npairs, 128, sizeof(cd) * 2 * H * 4, c->stream>>>(           \
+    k_near_pair_products_helicity<RS><<<(unsigned)s->npairs, 128, sizeof(float2) * 2 * H * 4, c->stream>>>(           \
         H, c->lmax, c->n, c->nh, nc, s->pj, s->blocksh.as<cf>(), s->alpha.as<cd>(), xc, part)
         if (rs <= 1) TG_NEARH(1);
         else if (rs == 2) TG_NEARH(2);
@@ -654,7 +671,7 @@ int tg_cluster_learned_apply(tg_cluster *c, const tg_c128 *x, tg_c128 *y, int co
     TG_TRY(io.ensure(sizeof(cd) * count * 2));
     cd *xs = io.as<cd>(), *ys = xs + count;
     int st = ref_to_split(c, (const cd *)x, xs, cols);
-    if (!st) st = learned_near_apply(c, xs, ys, cols, adjoint);
+    if (!st) st = adjoint == 2 ? learned_near_apply_c64(c, xs, ys, cols) : learned_near_apply(c, xs, ys, cols, adjoint);
     if (!st) st = split_to_ref(c, ys, (cd *)y, cols);
     if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = TG_DEVICE_ERROR;
     io.release();

@@ -115,13 +115,75 @@ class ParticleClusterContext:
         return local, gp, gk
 
 
-def particle_cluster(local, positions, k0: float, *, lmax=None, modes=None, precision: str = "c128",
-                     rtol: float = 1e-12):
+def parity_change(modes) -> np.ndarray:
+    """The real orthogonal, symmetric change between helicity and parity coefficients of
+    one basis (special/polarization.rs:1-60): helicity waves are (N +- M) / sqrt(2)
+    (pol 1 positive, pol 0 negative), parity waves M (pol 0) and N (pol 1), so
+    b_N = (a_+ + a_-) / sqrt(2), b_M = (a_+ - a_-) / sqrt(2) for each (l, m).  The same
+    matrix maps parity to helicity; every (l, m) needs both polarisations."""
+    index = {tuple(int(v) for v in md): k for k, md in enumerate(modes)}
+    U = np.zeros((len(modes), len(modes)))
+    r = 1.0 / np.sqrt(2.0)
+    for (l, m, pol), k in index.items():
+        other = index.get((l, m, 1 - pol))
+        if other is None:
+            raise ValueError(f"the parity basis needs both polarisations of (l, m) = ({l}, {m})")
+        # row k: parity pol (1: N, 0: M) from helicity pols (1: +, 0: -)
+        plus = k if pol == 1 else other
+        minus = other if pol == 1 else k
+        U[k, plus] = r
+        U[k, minus] = r if pol == 1 else -r
+    return U
+
+
+class _ParityContext:
+    """A particle-cluster context in the parity basis: the helicity context's value,
+    tangents and cotangents conjugated by the per-particle basis change."""
+
+    def __init__(self, ctx, changes):
+        self._ctx = ctx
+        self._U = changes
+        big = torch.block_diag(*[torch.as_tensor(u, dtype=torch.complex128) for u in changes])
+        self._big = big.to(ctx.value.device)
+        self.value = self._big @ ctx.value @ self._big
+
+    def pushforward(self, local=None, positions=None, k0: float = 0.0) -> torch.Tensor:
+        lh = None if local is None else [u @ np.asarray(t.cpu() if torch.is_tensor(t) else t) @ u
+                                         for t, u in zip(local, self._U)]
+        return self._big @ self._ctx.pushforward(lh, positions, k0) @ self._big
+
+    def pullback(self, cotangent):
+        g = torch.as_tensor(cotangent).to(self._big.device, torch.complex128)
+        local, gp, gk = self._ctx.pullback(self._big @ g @ self._big)
+        out = []
+        for t, u in zip(local, self._U):
+            ut = torch.as_tensor(u, dtype=torch.complex128, device=t.device)
+            out.append(ut @ t @ ut)
+        return out, gp, gk
+
+
+def particle_cluster(local, positions, k0: float, *, lmax=None, modes=None, poltype: str = "helicity",
+                     precision: str = "c128", rtol: float = 1e-12):
     """Coupled T-matrix of particles with arbitrary local T-matrices and its derivative
     context.  ``local``: one (n_i, n_i) T-matrix per particle in its basis, either the
     full basis up to lmax[i] (reference mode order) or the mode list modes[i] of
     (l, m, pol); the particles are embedded in the basis up to the largest degree with
-    zero rows and columns (exactly the reduced system)."""
+    zero rows and columns (exactly the reduced system).  ``poltype`` "parity" takes and
+    returns every matrix in the parity basis (diff.py:1473-1523); the device computes in
+    the helicity basis."""
+    if poltype not in ("helicity", "parity"):
+        raise ValueError("poltype must be 'helicity' or 'parity'")
+    if poltype == "parity":
+        pos = np.asarray(positions, np.float64).reshape(-1, 3)
+        if modes is None:
+            lm = np.broadcast_to(np.asarray(lmax, int), (pos.shape[0],))
+            modes = [[(l, m, p) for l in range(1, int(li) + 1) for m in range(-l, l + 1) for p in (1, 0)]
+                     for li in lm]
+        changes = [parity_change(ms) for ms in modes]
+        hel = [u @ np.asarray(t.cpu() if torch.is_tensor(t) else t, np.complex128) @ u for t, u in zip(local, changes)]
+        _, ctx = particle_cluster(hel, pos, k0, modes=modes, precision=precision, rtol=rtol)
+        pctx = _ParityContext(ctx, changes)
+        return pctx.value, pctx
     pos = np.asarray(positions, np.float64).reshape(-1, 3)
     N = pos.shape[0]
     if len(local) != N or N == 0:

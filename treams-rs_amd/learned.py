@@ -117,10 +117,13 @@ def set_pair_output(cluster: IterativeSphereCluster, out: torch.Tensor) -> None:
     check(lib().tg_cluster_set_pair_output(cluster._h, out.data_ptr()))
 
 
-def learned_apply(cluster: IterativeSphereCluster, x, adjoint=False):
+def learned_apply(cluster: IterativeSphereCluster, x, adjoint=False, c64=False):
+    """M^{-1} x (M^{-H} x with adjoint); c64: as the mixed-precision inner solves apply
+    it (c64 same-helicity blocks, FP32 products)."""
     cm, vector = _columns(x, cluster.n)
     y = torch.empty_like(cm)
-    check(lib().tg_cluster_learned_apply(cluster._h, cm.data_ptr(), y.data_ptr(), cm.shape[0], int(adjoint)))
+    mode = 2 if c64 else int(adjoint)
+    check(lib().tg_cluster_learned_apply(cluster._h, cm.data_ptr(), y.data_ptr(), cm.shape[0], mode))
     return y[0] if vector else y.T.contiguous()
 
 
