@@ -71,11 +71,14 @@ def _small_cluster(pkg, oracle, seed=4, N=10, lmax=3):
     return pkg.IterativeSphereCluster(lmax, 1.3, radii, eps, pos), oracle.Cluster(lmax, 1.3, radii, eps, pos)
 
 
-def test_learned_apply_and_loss_against_dense_oracle(pkg, oracle, learned):
+@pytest.mark.parametrize("k", [4, 8])
+def test_learned_apply_and_loss_against_dense_oracle(pkg, oracle, learned, k):
+    """k = 8: pair lists built on the device, their column lists (the adjoint's) on
+    first use; k = 4: both on the host."""
     import mlp_oracle
 
     cl, ref = _small_cluster(pkg, oracle)
-    npairs = learned.near_pairs(cl, 4)
+    npairs = learned.near_pairs(cl, k)
     pairs = learned.pair_list(cl, npairs)
     rng = np.random.default_rng(1)
     out = (0.3 * rng.normal(size=(npairs, 16))).astype(np.float32)

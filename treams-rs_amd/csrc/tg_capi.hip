@@ -86,6 +86,27 @@ void cluster_died() {
 }
 }  // namespace
 
+int HostBuf::ensure(size_t need) {
+    if (need <= bytes && ptr) return TG_OK;
+    release();
+    if (need == 0) return TG_OK;
+    hipError_t e = hipHostMalloc(&ptr, need, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ptr = nullptr;
+        set_error("hipHostMalloc(%zu bytes): %s", need, hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? TG_OUT_OF_MEMORY : TG_DEVICE_ERROR;
+    }
+    bytes = need;
+    return TG_OK;
+}
+
+void HostBuf::release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
 int DevBuf::ensure(size_t need) {
     if (need <= bytes && ptr) return TG_OK;
     release();
@@ -509,6 +530,7 @@ int tg_cluster_destroy(tg_cluster *c) {
     c->b_tloc.release();
     learned_release(c);
     for (auto &w : c->ws) w.release();
+    c->pin.release();
     c->timer.collect();
     delete c;
     cluster_died();
@@ -908,14 +930,14 @@ int solve_split(tg_cluster *c, const cd *b, cd *x, int cols, const tg_gmres_opti
     TG_TRY(outer.ensure(sizeof(cd) * count * 3));
     cd *r = outer.as<cd>(), *d = r + count, *ax = d + count;
     std::vector<double> bn(cols), rn(cols);
-    TG_TRY(vec_norms(c->stream, b, c->n, cols, c->n, bn.data(), c->ws[4]));
+    TG_TRY(vec_norms(c, b, c->n, cols, c->n, bn.data()));
     TG_HIP(hipMemsetAsync(x, 0, sizeof(cd) * count, c->stream));
     TG_HIP(hipMemcpyAsync(r, b, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream));
     conv.assign(cols, tg_convergence{0, 0, 0.0, 0.0});
     int iterations = 0, refinements = 0;
     int st = TG_OK;
     while (true) {
-        TG_TRY(vec_norms(c->stream, r, c->n, cols, c->n, rn.data(), c->ws[4]));
+        TG_TRY(vec_norms(c, r, c->n, cols, c->n, rn.data()));
         double worst_ratio = 0.0;
         bool done = true;
         for (int k = 0; k < cols; ++k) {
@@ -1030,8 +1052,8 @@ int tg_cluster_cross_sections(tg_cluster *c, const tg_c128 *incident, const tg_c
     if (!st) st = ref_to_split(c, (const cd *)incident, as, 1);
     if (!st) st = coupling_matrix_free(c, xs, rx, 1, 1);  // R x (self blocks identity)
     cd d1, d2;
-    if (!st) st = vec_dots(c->stream, xs, 0, 1, rx, n, 1, n, n, &d1, c->ws[4]);
-    if (!st) st = vec_dots(c->stream, as, 0, 1, xs, n, 1, n, n, &d2, c->ws[4]);
+    if (!st) st = vec_dots(c, xs, 0, 1, rx, n, 1, n, n, &d1);
+    if (!st) st = vec_dots(c, as, 0, 1, xs, n, 1, n, n, &d2);
     io.release();
     if (st) return st;
     const double w = 1.0 / (c->k0 * c->k0);

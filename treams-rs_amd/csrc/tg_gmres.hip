@@ -46,10 +46,13 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
     cd *proj = coef + (size_t)cols * (restart + 2);
     double *scale = reinterpret_cast<double *>(proj + nproj);
     double *mask = scale + cols;
-    std::vector<cd> proj_h(nproj);
+    // pinned host side: the projections and |w|^2 of a step (written there by the last
+    // reduction kernel), then the column scales and the extending mask (sent up with one
+    // asynchronous copy)
+    TG_TRY(c->pin.ensure(sizeof(cd) * nproj + sizeof(double) * 2 * cols));
 
     std::vector<double> bnorm(cols), tol(cols), rn(cols);
-    TG_TRY(vec_norms(s, b, n, cols, n, bnorm.data(), c->ws[4]));
+    TG_TRY(vec_norms(c, b, n, cols, n, bnorm.data()));
     conv.assign(cols, tg_convergence{0, 0, 0.0, 0.0});
     for (int k = 0; k < cols; ++k) {
         tol[k] = std::max(prm.atol, prm.rtol * bnorm[k]);
@@ -63,9 +66,9 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
     std::vector<zc> sn((size_t)cols * restart);
     std::vector<int> used(cols), extending(cols), cycle(cols);
     std::vector<cd> hbuf((size_t)cols * (restart + 1));
-    std::vector<double> nxt(cols), scale_h(2 * cols);
+    std::vector<double> nxt(cols);
     while (true) {
-        TG_TRY(vec_norms(s, R, n, cols, n, rn.data(), c->ws[4]));
+        TG_TRY(vec_norms(c, R, n, cols, n, rn.data()));
         int any = 0;
         for (int k = 0; k < cols; ++k) {
             conv[k].residual_norm = rn[k];
@@ -84,12 +87,15 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
             any = 1;
         }
         if (!any) break;
+        // the pinned buffer (vec_norms above writes its first values only)
+        cd *proj_h = c->pin.as<cd>();
+        double *scale_h = reinterpret_cast<double *>(proj_h + nproj);
         // V[k][0] = r_k / |r_k| for cycling columns, 0 otherwise
         for (int k = 0; k < cols; ++k) {
             scale_h[k] = cycle[k] ? 1.0 / rn[k] : 0.0;
             scale_h[cols + k] = cycle[k] ? 1.0 : 0.0;  // the extending mask
         }
-        TG_HIP(hipMemcpyAsync(scale, scale_h.data(), sizeof(double) * 2 * cols, hipMemcpyHostToDevice, s));
+        TG_HIP(hipMemcpyAsync(scale, scale_h, sizeof(double) * 2 * cols, hipMemcpyHostToDevice, s));
         TG_TRY(vec_scale_cols(s, scale, R, V, n, cols));
         std::fill(H.begin(), H.end(), zc(0.0));
         std::fill(g.begin(), g.end(), zc(0.0));
@@ -111,13 +117,12 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
             // (zero for columns no longer extending) kept on the device; one transfer
             // brings both passes' projections and |w|^2 to the host
             const size_t nj = (size_t)cols * (j + 1);
-            cd *p0 = proj, *p1 = proj + nj, *pn = proj + 2 * nj;
-            TG_TRY(vec_dots_dev(s, V, vstep, j + 1, W, n, cols, csv, n, p0, mask, c->ws[4]));
+            cd *p0 = proj, *p1 = proj + nj;
+            TG_TRY(vec_dots_dev(c, V, vstep, j + 1, W, n, cols, csv, n, p0, mask));
             TG_TRY(vec_update(s, V, vstep, j + 1, p0, W, n, cols, csv, n, -1.0));
-            TG_TRY(vec_dots_dev(s, V, vstep, j + 1, W, n, cols, csv, n, p1, mask, c->ws[4]));
+            TG_TRY(vec_dots_dev(c, V, vstep, j + 1, W, n, cols, csv, n, p1, mask));
             TG_TRY(vec_update(s, V, vstep, j + 1, p1, W, n, cols, csv, n, -1.0));
-            TG_TRY(vec_dots_dev(s, W, 0, 1, W, n, cols, n, n, pn, nullptr, c->ws[4]));
-            TG_HIP(hipMemcpyAsync(proj_h.data(), proj, sizeof(cd) * (2 * nj + cols), hipMemcpyDeviceToHost, s));
+            TG_TRY(vec_dots_dev(c, W, 0, 1, W, n, cols, n, n, nullptr, nullptr, proj_h, proj, (int64_t)(2 * nj)));
             TG_HIP(hipStreamSynchronize(s));
             for (int k = 0; k < cols; ++k)
                 for (int i = 0; i <= j; ++i) {
@@ -174,7 +179,7 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
                 }
             }
             for (int k = 0; k < cols; ++k) scale_h[cols + k] = extending[k] ? 1.0 : 0.0;
-            TG_HIP(hipMemcpyAsync(scale, scale_h.data(), sizeof(double) * 2 * cols, hipMemcpyHostToDevice, s));
+            TG_HIP(hipMemcpyAsync(scale, scale_h, sizeof(double) * 2 * cols, hipMemcpyHostToDevice, s));
             ++iterations;
             if (j + 1 < restart && more) TG_TRY(vec_scale_cols(s, scale, W, V + (int64_t)(j + 1) * vstep, n, cols));
             if (!more) break;

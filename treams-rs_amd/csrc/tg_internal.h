@@ -74,6 +74,16 @@ struct DevBuf {
     template <class T> T *as() const { return reinterpret_cast<T *>(ptr); }
 };
 
+// Pinned host memory (hipHostMalloc): kernels write it, the host reads it after the
+// stream is synchronised, with no copy in between.
+struct HostBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <class T> T *as() const { return reinterpret_cast<T *>(ptr); }
+};
+
 struct Timer {
     bool enabled = false;
     struct Rec { hipEvent_t a, b; };
@@ -133,6 +143,7 @@ struct tg_cluster {
     bool have_tablesR = false;
     int64_t table_limit = (int64_t)64 << 30;  // largest pair-table set kept (bytes)
     tg::DevBuf ws[8];                  // scratch
+    tg::HostBuf pin;                   // pinned host results of the Krylov reductions
     tg::Timer timer;
     tg::Preconditioner *pc = nullptr;
     int pc_kind = TG_PC_NONE;
@@ -206,13 +217,17 @@ int pair_gradients(tg_cluster *c, const cd *u, const cd *v, int cols, int regula
 int apply_operator(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, int op);
 
 // Vector helpers (tg_kernels.hip)
-int vec_dots(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols,
-             int64_t colstride_v, int64_t colstride_w, cd *out_host, DevBuf &scratch);
-int vec_norms(hipStream_t s, const cd *w, int64_t n, int cols, int64_t colstride, double *out_host, DevBuf &scratch);
-// vec_dots into device memory, out_dev[col nvec + i], zeroed for columns whose mask_dev
-// entry is 0 (mask_dev may be null); no host synchronisation
-int vec_dots_dev(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols,
-                 int64_t csv, int64_t csw, cd *out_dev, const double *mask_dev, DevBuf &scratch);
+// out[col nvec + i] = sum_k conj(V[col][i][k]) w[col][k] (block partial sums, then one
+// workgroup adds them in a fixed order), on the cluster's stream with its scratch
+int vec_dots(tg_cluster *c, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols,
+             int64_t colstride_v, int64_t colstride_w, cd *out_host);
+int vec_norms(tg_cluster *c, const cd *w, int64_t n, int cols, int64_t colstride, double *out_host);
+// vec_dots into device memory out_dev (may be null) and/or pinned host memory host_out
+// (after copy_len values copied from copy_src there first), zeroed for columns whose
+// mask_dev entry is 0 (mask_dev may be null); no host synchronisation
+int vec_dots_dev(tg_cluster *c, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols,
+                 int64_t csv, int64_t csw, cd *out_dev, const double *mask_dev, cd *host_out = nullptr,
+                 const cd *copy_src = nullptr, int64_t copy_len = 0);
 int vec_update(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *coef_dev, cd *w, int64_t n, int cols,
                int64_t colstride_v, int64_t colstride_w, double sign);
 int vec_axpby(hipStream_t s, cd alpha, const cd *x, cd beta, cd *y, int64_t count);

@@ -1019,16 +1019,23 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets(const cx<T> *TAB, cons
 
 // y[cc][h][i H + b] = sum_{a, t in (a, b)} w_t U[(cc, h)][ucol_t][i], same kind plus
 // or minus i times cross kind (h = 0: pol 1, +; h = 1: pol 0, -).  One thread per
-// destination sphere i (coalesced over i), grid.y = (cc, h, b); the terms of b are one
-// contiguous list (same kind, then cross kind; source modes ascending, as the
-// per-entry plan), read with wave-uniform (scalar) loads, so the loop is flat and
-// keeps many U loads in flight.
+// destination sphere i (coalesced over i), one workgroup per (cc, h, 64 spheres, b);
+// the terms of b are one contiguous list (same kind, then cross kind; source modes
+// ascending, as the per-entry plan), read with wave-uniform (scalar) loads, so the loop
+// is flat and keeps many U loads in flight.  Each U value is read by about three
+// destination modes: the workgroups are ordered (cc, h), sphere block, b fastest, and
+// each XCD takes a contiguous range of that order, so the H workgroups that share one
+// U slice (S x 64 values) run together on one L2.
 template <class T>
 __global__ __launch_bounds__(64) void k_tab_contract(DevPlan pl, const cx<T> *__restrict__ U, int N, int H,
                                                      int64_t n, int64_t nh, cd *__restrict__ y) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int yb = blockIdx.y;
-    const int cc = yb / (2 * H), rem = yb - cc * 2 * H, h = rem / H, b = rem - h * H;
+    const int nb = gridDim.x, bid = blockIdx.x;
+    const int xcd = bid & 7, q8 = nb >> 3, r8 = nb & 7;
+    const int L = xcd * q8 + min(xcd, r8) + (bid >> 3);
+    const int NI = (N + 63) / 64;
+    const int b = L % H, rest = L / H, ib = rest % NI, g = rest / NI;
+    const int cc = g >> 1, h = g & 1;
+    const int i = ib * 64 + threadIdx.x;
     if (i >= N) return;
     const cx<T> *Ug = U + ((int64_t)cc * 2 + h) * pl.S * N + i;
     const int32_t *__restrict__ cu = pl.cucol;
@@ -1108,8 +1115,8 @@ static int coupling_on_tables(tg_cluster *c, const cx<R> *TAB, const cd *x, cd *
                                                                           c->plan.soff, c->plan.smode);
         TG_HIP(hipGetLastError());
         TG_TRY(c->timer.end(gname, c->stream, grec));
-        k_tab_contract<R><<<dim3((unsigned)((N + 63) / 64), (unsigned)Q), 64, 0, c->stream>>>(
-            c->plan, U, N, H, c->n, c->nh, y + (int64_t)c0 * c->n);
+        k_tab_contract<R><<<(unsigned)(((N + 63) / 64) * Q), 64, 0, c->stream>>>(c->plan, U, N, H, c->n, c->nh,
+                                                                                y + (int64_t)c0 * c->n);
         TG_HIP(hipGetLastError());
     }
     TG_TRY(c->timer.end(name, c->stream, rec));
@@ -1129,8 +1136,8 @@ int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols) {
 
 int launch_tab_contract_c64(tg_cluster *c, const cf *U, int nc, cd *y) {
     const int N = c->N, H = c->H;
-    k_tab_contract<float><<<dim3((unsigned)((N + 63) / 64), (unsigned)(nc * 2 * H)), 64, 0, c->stream>>>(
-        c->plan, U, N, H, c->n, c->nh, y);
+    k_tab_contract<float><<<(unsigned)(((N + 63) / 64) * nc * 2 * H), 64, 0, c->stream>>>(c->plan, U, N, H, c->n,
+                                                                                          c->nh, y);
     TG_HIP(hipGetLastError());
     return TG_OK;
 }
@@ -1517,10 +1524,10 @@ int plane_wave(tg_cluster *c, const double *dir, int pol, cd *out) {
 
 // ------------------------------------------------------------ vector ops --
 
-// partial[((col * nvec + i) * gridDim.x + block] = sum_k conj(V[col][i][k]) w[col][k]
-// partial[(col nvec + i) gridDim.x + block] = sum over the block's slice of
-// conj(V[col][i]) w[col]: up to 8 vectors per pass over the slice (w read once per
-// pass), sums reduced across the wave by shuffles and across the 4 waves in LDS.
+// conj(V[col][i]) . w[col] for nvec vectors per column: block (x, col) sums its slice
+// of k (up to 8 vectors per pass over the slice, w read once per pass; reduced across
+// the wave by shuffles and across the 4 waves in LDS) into
+// partial[(col nvec + i) gridDim.x + x].
 __global__ __launch_bounds__(256) void k_dots(const cd *__restrict__ V, int64_t vstride, int nvec,
                                                const cd *__restrict__ w, int64_t n, int64_t csv, int64_t csw,
                                                cd *__restrict__ partial) {
@@ -1569,12 +1576,34 @@ __global__ __launch_bounds__(256) void k_dots(const cd *__restrict__ V, int64_t 
     }
 }
 
-__global__ void k_sum_rows(const cd *partial, int width, int rows, cd *out) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    cd s(0.0);
-    for (int b = 0; b < width; ++b) s += partial[(int64_t)r * width + b];
-    out[r] = s;
+// out[r] = sum of partial row r (width values; one wave per row: lane l adds values l,
+// l + 64, ..., then a fixed shuffle tree), zero where mask[r / nvec] is 0, to out_dev
+// and/or host_out[copy_len + r], after copy_src[0 .. copy_len) is copied to host_out.
+// One workgroup.
+__global__ __launch_bounds__(256) void k_dots_finish(const cd *__restrict__ partial, int width, int rows, int nvec,
+                                                      const double *mask, cd *out_dev, cd *host_out,
+                                                      const cd *copy_src, int64_t copy_len) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t k = threadIdx.x; k < copy_len; k += blockDim.x) host_out[k] = copy_src[k];
+    for (int r = wave; r < rows; r += 4) {
+        const cd *pr = partial + (int64_t)r * width;
+        double sre = 0.0, sim = 0.0;
+        for (int b = lane; b < width; b += 64) {
+            const cd v = pr[b];
+            sre += v.re;
+            sim += v.im;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            sre += __shfl_xor(sre, off);
+            sim += __shfl_xor(sim, off);
+        }
+        if (lane == 0) {
+            cd s(sre, sim);
+            if (mask && mask[r / nvec] == 0.0) s = cd(0.0);
+            if (out_dev) out_dev[r] = s;
+            if (host_out) host_out[copy_len + r] = s;
+        }
+    }
 }
 
 static int dot_blocks(int64_t n) {
@@ -1584,52 +1613,44 @@ static int dot_blocks(int64_t n) {
     return (int)b;
 }
 
-int vec_dots(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols, int64_t csv,
-             int64_t csw, cd *out_host, DevBuf &scratch) {
+static int dots_launch(tg_cluster *c, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols,
+                       int64_t csv, int64_t csw, const double *mask, cd *out_dev, cd *host_out, const cd *copy_src,
+                       int64_t copy_len) {
     const int nb = dot_blocks(n);
-    const int rows = cols * nvec;
     // room for 64 vectors per column at once: the Arnoldi basis grows by one vector per
     // iteration, and the scratch should not be reallocated every time
     const size_t cap = (size_t)cols * ((nvec + 63) / 64 * 64);
-    TG_TRY(scratch.ensure(sizeof(cd) * (cap * nb + cap)));
-    cd *partial = scratch.as<cd>();
-    cd *res = partial + (size_t)rows * nb;
-    k_dots<<<dim3(nb, cols), 256, 0, s>>>(V, vstride, nvec, w, n, csv, csw, partial);
+    TG_TRY(c->ws[4].ensure(sizeof(cd) * cap * nb));
+    cd *partial = c->ws[4].as<cd>();
+    k_dots<<<dim3(nb, cols), 256, 0, c->stream>>>(V, vstride, nvec, w, n, csv, csw, partial);
     TG_HIP(hipGetLastError());
-    k_sum_rows<<<(rows + 63) / 64, 64, 0, s>>>(partial, nb, rows, res);
+    k_dots_finish<<<1, 256, 0, c->stream>>>(partial, nb, cols * nvec, nvec, mask, out_dev, host_out, copy_src,
+                                            copy_len);
     TG_HIP(hipGetLastError());
-    TG_HIP(hipMemcpyAsync(out_host, res, sizeof(cd) * rows, hipMemcpyDeviceToHost, s));
-    TG_HIP(hipStreamSynchronize(s));
     return TG_OK;
 }
 
-// out[r] = sum of partial row r, times mask[r / nvec] when a mask is given
-__global__ void k_sum_rows_masked(const cd *partial, int width, int rows, int nvec, const double *mask, cd *out) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    cd s(0.0);
-    for (int b = 0; b < width; ++b) s += partial[(int64_t)r * width + b];
-    out[r] = mask ? (mask[r / nvec] != 0.0 ? s : cd(0.0)) : s;
-}
-
-int vec_dots_dev(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols,
-                 int64_t csv, int64_t csw, cd *out_dev, const double *mask_dev, DevBuf &scratch) {
-    const int nb = dot_blocks(n);
+int vec_dots(tg_cluster *c, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols, int64_t csv,
+             int64_t csw, cd *out_host) {
     const int rows = cols * nvec;
-    const size_t cap = (size_t)cols * ((nvec + 63) / 64 * 64);
-    TG_TRY(scratch.ensure(sizeof(cd) * (cap * nb + cap)));
-    cd *partial = scratch.as<cd>();
-    k_dots<<<dim3(nb, cols), 256, 0, s>>>(V, vstride, nvec, w, n, csv, csw, partial);
-    TG_HIP(hipGetLastError());
-    k_sum_rows_masked<<<(rows + 63) / 64, 64, 0, s>>>(partial, nb, rows, nvec, mask_dev, out_dev);
-    TG_HIP(hipGetLastError());
+    TG_TRY(c->pin.ensure(sizeof(cd) * (size_t)rows));
+    cd *res = c->pin.as<cd>();
+    TG_TRY(dots_launch(c, V, vstride, nvec, w, n, cols, csv, csw, nullptr, nullptr, res, nullptr, 0));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    std::copy(res, res + rows, out_host);
     return TG_OK;
 }
 
-int vec_norms(hipStream_t s, const cd *w, int64_t n, int cols, int64_t colstride, double *out_host, DevBuf &scratch) {
+int vec_dots_dev(tg_cluster *c, const cd *V, int64_t vstride, int nvec, const cd *w, int64_t n, int cols,
+                 int64_t csv, int64_t csw, cd *out_dev, const double *mask_dev, cd *host_out, const cd *copy_src,
+                 int64_t copy_len) {
+    return dots_launch(c, V, vstride, nvec, w, n, cols, csv, csw, mask_dev, out_dev, host_out, copy_src, copy_len);
+}
+
+int vec_norms(tg_cluster *c, const cd *w, int64_t n, int cols, int64_t colstride, double *out_host) {
     std::vector<cd> d(cols);
     // |w|^2 as the dot of w with itself (nvec = 1, V = w)
-    TG_TRY(vec_dots(s, w, 0, 1, w, n, cols, colstride, colstride, d.data(), scratch));
+    TG_TRY(vec_dots(c, w, 0, 1, w, n, cols, colstride, colstride, d.data()));
     // a NaN stays NaN (fmax would turn it into 0 and pass a non-finite vector as zero)
     for (int i = 0; i < cols; ++i) out_host[i] = d[i].re > 0.0 ? sqrt(d[i].re) : d[i].re == d[i].re ? 0.0 : d[i].re;
     return TG_OK;

@@ -37,6 +37,7 @@ struct LearnedState {
     tg::DevBuf alpha;   // [np][lmax] c128
     tg::DevBuf work;    // training scratch
     const int *pi = nullptr, *pj = nullptr, *row_start = nullptr, *col_start = nullptr, *col_list = nullptr;
+    bool have_cols = false;  // col_start and col_list filled
     bool have_alpha = false;
 };
 
@@ -299,16 +300,20 @@ static LearnedState *state(tg_cluster *c) {
     return c->learned;
 }
 
-// K nearest neighbours of every sphere (host), near blocks and features (device).
-// The KT nearest other spheres of every sphere i, ordered by (squared distance,
-// index) as the host selection (std::pair order).  One wave per sphere: lane l keeps
-// the best KT of the candidates j = l, l + 64, ... in registers (each candidate carried
-// down a fully unrolled insertion chain), then KT rounds of a wave-wide arg-min over
-// the lanes' heads merge them.  The squared distances are rounded as on the host
+// K nearest neighbours of every sphere, near blocks and features (device).
+// The KT nearest other spheres of every sphere i, selected by (squared distance,
+// index) as the host selection (std::pair order), then listed by ascending index: pair
+// p = i KT + k is (pi, pj) = (i, out[p]), with row_start[i] = i KT and the
+// displacement disp[p] = pos_i - pos_j.  One wave per sphere: lane l keeps the best KT
+// of the candidates j = l, l + 64, ... in registers (each candidate carried down a
+// fully unrolled insertion chain), then KT rounds of a wave-wide arg-min over the
+// lanes' heads merge them.  The squared distances are rounded as on the host
 // ((0 + t0^2) + t1^2) + t2^2, no fused multiply-adds, so equal distances tie the same
 // way.
 template <int KT>
-__global__ __launch_bounds__(64) void k_nearest(int N, const double *__restrict__ pos, int *__restrict__ out) {
+__global__ __launch_bounds__(64) void k_nearest(int N, const double *__restrict__ pos, int *__restrict__ out,
+                                                 int *__restrict__ pi, int *__restrict__ row_start,
+                                                 double *__restrict__ disp) {
     const int i = blockIdx.x, lane = threadIdx.x;
     double bd[KT];
     int bj[KT];
@@ -338,6 +343,8 @@ __global__ __launch_bounds__(64) void k_nearest(int N, const double *__restrict_
             }
         }
     }
+    int win[KT];  // the winners, the same in every lane
+#pragma unroll
     for (int k = 0; k < KT; ++k) {
         double d = bd[0];
         int j = bj[0];
@@ -358,8 +365,69 @@ __global__ __launch_bounds__(64) void k_nearest(int N, const double *__restrict_
             bd[KT - 1] = __builtin_inf();
             bj[KT - 1] = 0x7fffffff;
         }
-        if (lane == 0) out[(int64_t)i * KT + k] = j;
+        win[k] = j;
     }
+    // ascending index (distinct values: a fixed compare-exchange network)
+#pragma unroll
+    for (int a = 1; a < KT; ++a)
+#pragma unroll
+        for (int b = a; b > 0; --b)
+            if (win[b] < win[b - 1]) {
+                const int t = win[b];
+                win[b] = win[b - 1];
+                win[b - 1] = t;
+            }
+    if (lane < KT) {
+        int j = win[0];
+#pragma unroll
+        for (int k = 1; k < KT; ++k)
+            if (lane == k) j = win[k];
+        const int64_t p = (int64_t)i * KT + lane;
+        out[p] = j;
+        if (pi) pi[p] = i;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (disp) disp[3 * p + a] = __dsub_rn(pos[3 * i + a], pos[3 * j + a]);
+    }
+    if (row_start && lane == 0) {
+        row_start[i] = i * KT;
+        if (i == N - 1) row_start[N] = N * KT;
+    }
+}
+
+// column lists of the near pairs (the adjoint apply's), built on the host from pj the
+// first time they are needed
+static int ensure_pair_columns(tg_cluster *c) {
+    LearnedState *s = c->learned;
+    if (s->have_cols) return TG_OK;
+    const int N = c->N;
+    const int64_t np = s->npairs;
+    std::vector<int> pj(np), col_start(N + 1, 0), col_list(np);
+    TG_HIP(hipMemcpyAsync(pj.data(), s->pj, sizeof(int) * np, hipMemcpyDeviceToHost, c->stream));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t p = 0; p < np; ++p) col_start[pj[p] + 1]++;
+    for (int i = 0; i < N; ++i) col_start[i + 1] += col_start[i];
+    std::vector<int> fill(col_start.begin(), col_start.end() - 1);
+    for (int64_t p = 0; p < np; ++p) col_list[fill[pj[p]]++] = (int)p;
+    TG_HIP(hipMemcpy(const_cast<int *>(s->col_start), col_start.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+    TG_HIP(hipMemcpy(const_cast<int *>(s->col_list), col_list.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+    s->have_cols = true;
+    return TG_OK;
+}
+
+// features of the near pairs and room for the network's output; the blocks themselves
+// are built on first use, in the form that use needs
+static int pair_setup_tail(tg_cluster *c, LearnedState *s) {
+    const int64_t np = s->npairs;
+    s->have_blocks = false;
+    s->have_blocksh = false;
+    TG_TRY(s->feats.ensure(sizeof(float) * (size_t)np * kFeat));
+    k_pair_features<<<(unsigned)((np + 127) / 128), 128, 0, c->stream>>>(
+        np, s->pi, s->pj, c->b_pos.as<double>(), c->b_mie.as<cd>(), c->lmax, c->k0, s->feats.as<float>());
+    TG_HIP(hipGetLastError());
+    TG_TRY(s->alpha.ensure(sizeof(cd) * (size_t)np * c->lmax));
+    TG_TRY(s->out16.ensure(sizeof(float) * (size_t)np * kOut));
+    return TG_OK;
 }
 
 int learned_pairs(tg_cluster *c, int K) {
@@ -370,23 +438,26 @@ int learned_pairs(tg_cluster *c, int K) {
     K = std::min(K, N - 1);
     std::vector<int> pi, pj;
     if (K == 8) {
-        // on the device (the host selection is O(N^2) and idles the GPU for ms)
-        DevBuf dnb;
-        TG_TRY(dnb.ensure(sizeof(int) * (size_t)N * 8));
-        k_nearest<8><<<(unsigned)N, 64, 0, c->stream>>>(N, c->b_pos.as<double>(), dnb.as<int>());
+        // on the device, pair lists, row starts and displacements included (the host
+        // selection is O(N^2) and idles the GPU for ms); the column lists wait for the
+        // first adjoint apply
+        const int64_t np = (int64_t)N * 8;
+        TG_TRY(s->meta.ensure(sizeof(int) * (size_t)(3 * np + 2 * (N + 1))));
+        int *base = s->meta.as<int>();
+        s->pi = base;
+        s->pj = base + np;
+        s->row_start = base + 2 * np;
+        s->col_start = s->row_start + N + 1;
+        s->col_list = s->col_start + N + 1;
+        s->npairs = np;
+        s->K = K;
+        s->have_alpha = false;
+        s->have_cols = false;
+        TG_TRY(s->disp.ensure(sizeof(double) * 3 * (size_t)np));
+        k_nearest<8><<<(unsigned)N, 64, 0, c->stream>>>(N, c->b_pos.as<double>(), base + np, base,
+                                                        base + 2 * np, s->disp.as<double>());
         TG_HIP(hipGetLastError());
-        std::vector<int> nbs((size_t)N * 8);
-        TG_HIP(hipMemcpyAsync(nbs.data(), dnb.ptr, sizeof(int) * nbs.size(), hipMemcpyDeviceToHost, c->stream));
-        TG_HIP(hipStreamSynchronize(c->stream));
-        dnb.release();
-        for (int i = 0; i < N; ++i) {
-            int *nb = nbs.data() + (size_t)i * 8;
-            std::sort(nb, nb + 8);
-            for (int t = 0; t < 8; ++t) {
-                pi.push_back(i);
-                pj.push_back(nb[t]);
-            }
-        }
+        return pair_setup_tail(c, s);
     } else {
         std::vector<std::pair<double, int>> dist(N);
         for (int i = 0; i < N; ++i) {
@@ -439,23 +510,14 @@ int learned_pairs(tg_cluster *c, int K) {
     s->npairs = np;
     s->K = K;
     s->have_alpha = false;
+    s->have_cols = true;
     // near blocks C_ij = translation at pos_i - pos_j
     std::vector<double> disp(3 * np);
     for (int64_t p = 0; p < np; ++p)
         for (int a = 0; a < 3; ++a) disp[3 * p + a] = c->positions[3 * pi[p] + a] - c->positions[3 * pj[p] + a];
     TG_TRY(s->disp.ensure(sizeof(double) * disp.size()));
     TG_HIP(hipMemcpy(s->disp.ptr, disp.data(), sizeof(double) * disp.size(), hipMemcpyHostToDevice));
-    // the blocks themselves are built on first use, in the form that use needs
-    s->have_blocks = false;
-    s->have_blocksh = false;
-    TG_TRY(s->feats.ensure(sizeof(float) * (size_t)np * kFeat));
-    k_pair_features<<<(unsigned)((np + 127) / 128), 128, 0, c->stream>>>(
-        np, s->pi, s->pj, c->b_pos.as<double>(), c->b_mie.as<cd>(), c->lmax, c->k0, s->feats.as<float>());
-    TG_HIP(hipGetLastError());
-    TG_TRY(s->alpha.ensure(sizeof(cd) * (size_t)np * c->lmax));
-    TG_TRY(s->out16.ensure(sizeof(float) * (size_t)np * kOut));
-    TG_HIP(hipStreamSynchronize(c->stream));
-    return TG_OK;
+    return pair_setup_tail(c, s);
 }
 
 int learned_set_output(tg_cluster *c, const float *out16_dev) {
@@ -553,6 +615,7 @@ int learned_near_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint)
     const int M = c->M;
     TG_TRY(ensure_blocks128(c));
     if (!adjoint) return near_apply_forward(c, s->blocks.as<cd>(), x, y, cols);
+    TG_TRY(ensure_pair_columns(c));
     TG_TRY(s->work.ensure(sizeof(cd) * (size_t)cols * c->n));
     cd *z = s->work.as<cd>();
     TG_TRY(local_apply(c, x, z, cols, 1, nullptr));
@@ -696,8 +759,8 @@ int tg_cluster_learned_loss(tg_cluster *c, const tg_c128 *probes, int cols, doub
         if ((st = apply_operator(c, U, R, cols, 0, TG_OP_MATRIX_FREE_F64))) break;
         if ((st = vec_axpby(c->stream, cd(-1.0), V, cd(1.0), R, count))) break;  // R = A U - V
         std::vector<double> vn(cols), rn(cols);
-        if ((st = vec_norms(c->stream, V, c->n, cols, c->n, vn.data(), c->ws[4]))) break;
-        if ((st = vec_norms(c->stream, R, c->n, cols, c->n, rn.data(), c->ws[4]))) break;
+        if ((st = vec_norms(c, V, c->n, cols, c->n, vn.data()))) break;
+        if ((st = vec_norms(c, R, c->n, cols, c->n, rn.data()))) break;
         double v2 = 0, r2 = 0;
         for (int k = 0; k < cols; ++k) {
             v2 += vn[k] * vn[k];

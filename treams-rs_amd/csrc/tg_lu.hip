@@ -1475,11 +1475,11 @@ static int factor_solve_once(tg_factor *f, const cd *b, cd *x, int cols, int adj
     const double *dnorm = f->equilibrated ? f->eq.as<double>() + (adjoint ? f->n : 0) : nullptr;
     const unsigned sgrid = (unsigned)((count + 255) / 256);
     auto norms = [&](const cd *v, double *out) -> int {
-        if (!dnorm) return vec_norms(c->stream, v, c->n, cols, c->n, out, c->ws[4]);
+        if (!dnorm) return vec_norms(c, v, c->n, cols, c->n, out);
         TG_HIP(hipMemcpyAsync(ax, v, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream));
         k_scale_rows<<<sgrid, 256, 0, c->stream>>>(ax, c->n, cols, dnorm);
         TG_HIP(hipGetLastError());
-        return vec_norms(c->stream, ax, c->n, cols, c->n, out, c->ws[4]);
+        return vec_norms(c, ax, c->n, cols, c->n, out);
     };
     do {
         if ((st = norms(b, bn.data()))) break;
