@@ -215,27 +215,45 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp, std::vector
         for (int32_t t = hp.starts[e]; t < hp.starts[e + 1]; ++t)
             ucol[t] = soff[hp.hidx[t]] + rank[hp.hidx[t] * hp.H + e / hp.H];
     const int S = soff[T];
-    // destination-major term list for the contraction
-    std::vector<int32_t> cstarts(hp.H + 1, 0), cnsame(hp.H, 0), cucol(nterms);
-    std::vector<double> cw(nterms);
-    {
-        size_t k = 0;
-        for (int b = 0; b < hp.H; ++b) {
-            cstarts[b] = (int32_t)k;
-            for (int kind = 0; kind < 2; ++kind) {
-                for (int a = 0; a < hp.H; ++a) {
-                    const size_t e = (size_t)a * hp.H + b;
-                    const int32_t s0 = hp.starts[e], ns = hp.nsame[e], s1 = hp.starts[e + 1];
-                    for (int32_t t = kind ? s0 + ns : s0; t < (kind ? s1 : s0 + ns); ++t) {
-                        cucol[k] = ucol[t];
-                        cw[k] = hp.w[t];
-                        ++k;
-                    }
-                }
-                if (kind == 0) cnsame[b] = (int32_t)(k - cstarts[b]);
+    // subset columns grouped by destination order mu = m - q, with per-lambda weights
+    const int lmx = hp.lmax, ngrp = 2 * lmx + 1;
+    auto isqrt = [](int v) {
+        int r = (int)std::sqrt((double)v);
+        while (r * r > v) --r;
+        while ((r + 1) * (r + 1) <= v) ++r;
+        return r;
+    };
+    std::vector<int32_t> gstart(ngrp + 1, 0), gpos(std::max(S, 1)), mu_of(std::max(S, 1));
+    for (int t = 0; t < T; ++t) {
+        const int p = isqrt(t), q = t - p * p - p;
+        for (int s = soff[t]; s < soff[t + 1]; ++s) {
+            const int a1 = smode[s] + 1, la = isqrt(a1), ma = a1 - la * la - la;
+            mu_of[s] = ma - q;
+            if (std::abs(mu_of[s]) > lmx) {
+                set_error("plan: subset column feeds no destination order");
+                return TG_INVALID_INPUT;
             }
+            gstart[mu_of[s] + lmx + 1]++;
         }
-        cstarts[hp.H] = (int32_t)k;
+    }
+    for (int g = 0; g < ngrp; ++g) gstart[g + 1] += gstart[g];
+    {
+        std::vector<int32_t> fillg(gstart.begin(), gstart.end() - 1);
+        for (int s = 0; s < S; ++s) {
+            gpos[s] = fillg[mu_of[s] + lmx]++;
+        }
+    }
+    std::vector<double> gw((size_t)std::max(S, 1) * lmx * 2, 0.0);
+    for (size_t e = 0; e < HH; ++e) {
+        const int b1 = (int)(e % hp.H) + 1, lam = isqrt(b1), mu = b1 - lam * lam - lam;
+        for (int32_t t = hp.starts[e]; t < hp.starts[e + 1]; ++t) {
+            const int s = ucol[t], kind = (t - hp.starts[e]) >= hp.nsame[e];
+            if (mu_of[s] != mu) {
+                set_error("plan: term order mismatch");
+                return TG_INVALID_INPUT;
+            }
+            gw[((size_t)gpos[s] * lmx + lam - 1) * 2 + kind] += hp.w[t];
+        }
     }
     std::vector<int32_t> stiles((size_t)2 * kGemmCols * (T + 1), 0);
     for (int w = 0; w < 2; ++w)
@@ -259,11 +277,10 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp, std::vector
     size_t off_smode = al(off_soff + sizeof(int32_t) * (T + 1));
     size_t off_ucol = al(off_smode + sizeof(int32_t) * std::max(S, 1));
     size_t off_stiles = al(off_ucol + sizeof(int32_t) * nterms);
-    size_t off_cst = al(off_stiles + sizeof(int32_t) * stiles.size());
-    size_t off_cns = al(off_cst + sizeof(int32_t) * (hp.H + 1));
-    size_t off_cuc = al(off_cns + sizeof(int32_t) * hp.H);
-    size_t off_cw = al(off_cuc + sizeof(int32_t) * nterms);
-    size_t total = al(off_cw + sizeof(double) * nterms);
+    size_t off_gst = al(off_stiles + sizeof(int32_t) * stiles.size());
+    size_t off_gpos = al(off_gst + sizeof(int32_t) * gstart.size());
+    size_t off_gw = al(off_gpos + sizeof(int32_t) * gpos.size());
+    size_t total = al(off_gw + sizeof(double) * gw.size());
     TG_TRY(buf.ensure(total));
     char *base = buf.as<char>();
     std::vector<char> host(total, 0);
@@ -279,10 +296,9 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp, std::vector
     if (S) memcpy(host.data() + off_smode, smode.data(), sizeof(int32_t) * S);
     memcpy(host.data() + off_ucol, ucol.data(), sizeof(int32_t) * nterms);
     memcpy(host.data() + off_stiles, stiles.data(), sizeof(int32_t) * stiles.size());
-    memcpy(host.data() + off_cst, cstarts.data(), sizeof(int32_t) * (hp.H + 1));
-    memcpy(host.data() + off_cns, cnsame.data(), sizeof(int32_t) * hp.H);
-    memcpy(host.data() + off_cuc, cucol.data(), sizeof(int32_t) * nterms);
-    memcpy(host.data() + off_cw, cw.data(), sizeof(double) * nterms);
+    memcpy(host.data() + off_gst, gstart.data(), sizeof(int32_t) * gstart.size());
+    memcpy(host.data() + off_gpos, gpos.data(), sizeof(int32_t) * gpos.size());
+    memcpy(host.data() + off_gw, gw.data(), sizeof(double) * gw.size());
     TG_HIP(hipMemcpy(base, host.data(), total, hipMemcpyHostToDevice));
     dp.H = hp.H;
     dp.L2 = hp.L2;
@@ -300,10 +316,10 @@ static int upload_plan(const HostPlan &hp, DevBuf &buf, DevPlan &dp, std::vector
     dp.smode = (const int32_t *)(base + off_smode);
     dp.ucol = (const int32_t *)(base + off_ucol);
     dp.stiles = (const int32_t *)(base + off_stiles);
-    dp.cstarts = (const int32_t *)(base + off_cst);
-    dp.cnsame = (const int32_t *)(base + off_cns);
-    dp.cucol = (const int32_t *)(base + off_cuc);
-    dp.cw = (const double *)(base + off_cw);
+    dp.lmax = lmx;
+    dp.gstart = (const int32_t *)(base + off_gst);
+    dp.gpos = (const int32_t *)(base + off_gpos);
+    dp.gw = (const double *)(base + off_gw);
     if (ntiles) {
         ntiles->assign((size_t)2 * kGemmCols, 0);
         for (int w = 0; w < 2; ++w)

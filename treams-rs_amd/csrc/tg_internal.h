@@ -58,11 +58,14 @@ struct DevPlan {
     // GEMM column tiles of width kTileWidths[w] before harmonic t for G = 2 cols column
     // groups: stiles[((w kGemmCols) + cols - 1) (T + 1) + t], cols = 1 .. kGemmCols
     const int32_t *stiles;
-    // the terms by destination mode b (for the contraction): cstarts[b] .. cstarts[b+1],
-    // same-kind terms first (cnsame[b] of them), each with its subset column and weight,
-    // source modes ascending
-    const int32_t *cstarts, *cnsame, *cucol;   // H + 1, H, terms
-    const double *cw;                          // terms
+    // the subset columns by destination order: column s (harmonic (p, q), source mode
+    // (l, m)) feeds only destination modes (lambda, mu = m - q).  The GEMMs store column
+    // s at position k = gpos[s]; group mu + lmax holds positions gstart[.] ..
+    // gstart[. + 1) (columns ascending), and position k carries the weights
+    // gw[(k lmax + lambda - 1) 2 + kind] (kind 0 same, 1 cross; 0 where no term)
+    int lmax;
+    const int32_t *gstart, *gpos;              // 2 lmax + 2, S
+    const double *gw;                          // S lmax 2
 };
 
 // Growable device buffer.

@@ -963,8 +963,9 @@ __global__ void k_pack_sources(const cd *x, int64_t n, int64_t nh, int N, int H,
 }
 
 
-// U[g][soff[t] + r][i] = sum_j TAB_t[i][j] X[g H + smode[soff[t] + r]][j] for every
-// harmonic index t, column group g = (column, helicity) < G and subset rank r: one
+// U[g][gpos[soff[t] + r]][i] = sum_j TAB_t[i][j] X[g H + smode[soff[t] + r]][j] for
+// every harmonic index t, column group g = (column, helicity) < G and subset rank r
+// (columns stored in the contraction's group order): one
 // BM x BN tile per workgroup (64 x 64, or 128 x 32: a harmonic's subset is 13-48
 // modes per column group, and 32-wide column tiles pad less).  Tiles are ordered
 // (t, row tile, column tile), column
@@ -975,7 +976,8 @@ __global__ void k_pack_sources(const cd *x, int64_t n, int64_t nh, int N, int H,
 template <class T, bool KTAIL, int BM, int BN>
 __global__ __launch_bounds__(256, 2) void k_zgemm_subsets(const cx<T> *TAB, const cx<T> *X, cx<T> *U, int N, int H,
                                                           int S, int G, int T_, const int32_t *tiles,
-                                                          const int32_t *soff, const int32_t *smode) {
+                                                          const int32_t *soff, const int32_t *smode,
+                                                          const int32_t *gpos) {
     constexpr int WC = BN / 32;
     const int nb = gridDim.x, bid = blockIdx.x;
     const int xcd = bid & 7, q8 = nb >> 3, r8 = nb & 7;
@@ -1007,7 +1009,7 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets(const cx<T> *TAB, cons
             const int q = (int)c0 + wn + b * 16 + mfma_out_row<T>(lane, v);
             if (q >= ncol) continue;
             const int g = q / nt, r = q - g * nt;
-            cx<T> *Uc = U + ((int64_t)g * S + s0 + r) * N;
+            cx<T> *Uc = U + ((int64_t)g * S + gpos[s0 + r]) * N;
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
                 const int64_t gr = r0 + wm + a * 16 + rr;
@@ -1017,43 +1019,105 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets(const cx<T> *TAB, cons
         }
 }
 
-// y[cc][h][i H + b] = sum_{a, t in (a, b)} w_t U[(cc, h)][ucol_t][i], same kind plus
-// or minus i times cross kind (h = 0: pol 1, +; h = 1: pol 0, -).  One thread per
-// destination sphere i (coalesced over i), one workgroup per (cc, h, 64 spheres, b);
-// the terms of b are one contiguous list (same kind, then cross kind; source modes
-// ascending, as the per-entry plan), read with wave-uniform (scalar) loads, so the loop
-// is flat and keeps many U loads in flight.  Each U value is read by about three
-// destination modes: the workgroups are ordered (cc, h), sphere block, b fastest, and
-// each XCD takes a contiguous range of that order, so the H workgroups that share one
-// U slice (S x 64 values) run together on one L2.
-template <class T>
-__global__ __launch_bounds__(64) void k_tab_contract(DevPlan pl, const cx<T> *__restrict__ U, int N, int H,
-                                                     int64_t n, int64_t nh, cd *__restrict__ y) {
+// The plan contraction y[cc][h][i H + b] = sum over the terms (a, t) of destination
+// mode b of w_t U[(cc, h)][column of (t, a)][i], same kind plus or minus i times cross
+// kind (h = 0: pol 1, +; h = 1: pol 0, -).  Column s of U (harmonic (p, q), source mode
+// (l, m)) feeds only the modes (lambda, mu = m - q), so a workgroup of 4 waves per
+// (column group g, 64 spheres, order mu) reads each of its columns once per sphere (U
+// is read once in all, against about three times per value when each destination mode
+// reads its own terms) and keeps the same- and cross-kind sums of every lambda in
+// registers.  U stores the columns in group order (column s at
+// position gpos[s]), so an order's columns are contiguous; wave w takes columns w,
+// w + 4, ... of them (wave-uniform, so the weights are scalar loads); the four waves'
+// sums are added in LDS in wave order.  LB >= lmax bounds the register arrays.
+template <class T, int LB>
+__global__ __launch_bounds__(256) void k_tab_contract_mu(DevPlan pl, const cx<T> *__restrict__ U, int N, int H,
+                                                         int64_t n, int64_t nh, cd *__restrict__ y) {
+    constexpr int UN = 8;  // loads in flight per wave
+    const int lmax = pl.lmax, ng = 2 * lmax + 1, NI = (N + 63) / 64;
     const int nb = gridDim.x, bid = blockIdx.x;
     const int xcd = bid & 7, q8 = nb >> 3, r8 = nb & 7;
     const int L = xcd * q8 + min(xcd, r8) + (bid >> 3);
-    const int NI = (N + 63) / 64;
-    const int b = L % H, rest = L / H, ib = rest % NI, g = rest / NI;
-    const int cc = g >> 1, h = g & 1;
-    const int i = ib * 64 + threadIdx.x;
-    if (i >= N) return;
-    const cx<T> *Ug = U + ((int64_t)cc * 2 + h) * pl.S * N + i;
-    const int32_t *__restrict__ cu = pl.cucol;
-    const double *__restrict__ cw = pl.cw;
-    const int s0 = pl.cstarts[b], s1 = pl.cstarts[b + 1], sm = s0 + pl.cnsame[b];
-    cd same(0.0), cross(0.0);
-#pragma unroll 8
-    for (int t = s0; t < sm; ++t) {
-        const cx<T> u = Ug[(int64_t)cu[t] * N];
-        same += cw[t] * cd((double)u.re, (double)u.im);
+    // sphere blocks fastest: the workgroups reading one column's N values run together
+    const int ib = L % NI, rest = L / NI, mg = rest % ng, g = rest / ng;
+    // wave index made provably uniform, so that the column list and weights are scalar
+    // loads
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = ib * 64 + lane, ic = min(i, N - 1);
+    const int k0 = pl.gstart[mg], k1 = pl.gstart[mg + 1];
+    const cx<T> *Ug = U + (int64_t)g * pl.S * N + ic;
+    double sr[LB], si[LB], cr[LB], ci[LB];
+#pragma unroll
+    for (int l = 0; l < LB; ++l) sr[l] = si[l] = cr[l] = ci[l] = 0.0;
+    // U holds the columns in group order (the GEMMs store column s at gpos[s]), so the
+    // order's columns are one contiguous block of k1 - k0 columns
+    for (int k = k0 + wave; k < k1; k += 4 * UN) {
+        cx<T> u[UN];
+#pragma unroll
+        for (int v = 0; v < UN; ++v) u[v] = Ug[(int64_t)min(k + 4 * v, k1 - 1) * N];
+#pragma unroll
+        for (int v = 0; v < UN; ++v) {
+            if (k + 4 * v >= k1) break;
+            const double *wk = pl.gw + (int64_t)(k + 4 * v) * lmax * 2;
+            const double ur = (double)u[v].re, ui = (double)u[v].im;
+#pragma unroll
+            for (int l = 0; l < LB; ++l) {
+                if (l >= lmax) break;
+                const double ws = wk[2 * l], wc = wk[2 * l + 1];
+                sr[l] = fma(ws, ur, sr[l]);
+                si[l] = fma(ws, ui, si[l]);
+                cr[l] = fma(wc, ur, cr[l]);
+                ci[l] = fma(wc, ui, ci[l]);
+            }
+        }
     }
-#pragma unroll 8
-    for (int t = sm; t < s1; ++t) {
-        const cx<T> u = Ug[(int64_t)cu[t] * N];
-        cross += cw[t] * cd((double)u.re, (double)u.im);
+    // one lambda at a time through LDS (every wave reaches every barrier: lmax and mu are
+    // uniform)
+    __shared__ double red[3][4][64];
+    const int mu = mg - lmax, cc = g >> 1, h = g & 1;
+    cd *yo = y + (int64_t)cc * n + (int64_t)h * nh + (int64_t)ic * H;
+#pragma unroll
+    for (int l = 0; l < LB; ++l) {
+        const int lam = l + 1;
+        if (lam > lmax) break;
+        if (lam < (mu < 0 ? -mu : mu)) continue;
+        if (wave > 0) {
+            red[wave - 1][0][lane] = sr[l];
+            red[wave - 1][1][lane] = si[l];
+            red[wave - 1][2][lane] = cr[l];
+            red[wave - 1][3][lane] = ci[l];
+        }
+        __syncthreads();
+        if (wave == 0 && i < N) {
+            double a0 = sr[l], a1 = si[l], a2 = cr[l], a3 = ci[l];
+#pragma unroll
+            for (int w = 0; w < 3; ++w) {
+                a0 += red[w][0][lane];
+                a1 += red[w][1][lane];
+                a2 += red[w][2][lane];
+                a3 += red[w][3][lane];
+            }
+            yo[lam * lam - 1 + mu + lam] = h == 0 ? cd(a0 - a3, a1 + a2) : cd(a0 + a3, a1 - a2);
+        }
+        __syncthreads();
     }
-    const cd v = h == 0 ? cd(same.re - cross.im, same.im + cross.re) : cd(same.re + cross.im, same.im - cross.re);
-    y[(int64_t)cc * n + (int64_t)h * nh + (int64_t)i * H + b] = v;
+}
+
+// y = the plan contraction of U for nc right-hand sides (2 nc column groups)
+template <class T>
+static int launch_contract(tg_cluster *c, const cx<T> *U, int nc, cd *y) {
+    const int N = c->N, H = c->H, lmax = c->plan.lmax;
+    const unsigned grid = (unsigned)(((N + 63) / 64) * (2 * lmax + 1) * 2 * nc);
+#define TG_CONTRACT(LB)                                                                                           \
+    k_tab_contract_mu<T, LB><<<grid, 256, 0, c->stream>>>(c->plan, U, N, H, c->n, c->nh, y)
+    if (lmax <= 2) TG_CONTRACT(2);
+    else if (lmax <= 4) TG_CONTRACT(4);
+    else if (lmax <= 6) TG_CONTRACT(6);
+    else if (lmax <= 8) TG_CONTRACT(8);
+    else TG_CONTRACT(12);
+#undef TG_CONTRACT
+    TG_HIP(hipGetLastError());
+    return TG_OK;
 }
 
 // The tables in the GEMM precision: FP64 pair tables, or their c64 rounding (built
@@ -1109,15 +1173,13 @@ static int coupling_on_tables(tg_cluster *c, const cx<R> *TAB, const cd *x, cd *
         TG_TRY(c->timer.begin(gname, c->stream, grec));
         if (N % 16)
             k_zgemm_subsets<R, true, BM, BN><<<grid, 256, 0, c->stream>>>(TAB, X, U, N, H, S, 2 * nc, T, tiles,
-                                                                         c->plan.soff, c->plan.smode);
+                                                                         c->plan.soff, c->plan.smode, c->plan.gpos);
         else
             k_zgemm_subsets<R, false, BM, BN><<<grid, 256, 0, c->stream>>>(TAB, X, U, N, H, S, 2 * nc, T, tiles,
-                                                                          c->plan.soff, c->plan.smode);
+                                                                          c->plan.soff, c->plan.smode, c->plan.gpos);
         TG_HIP(hipGetLastError());
         TG_TRY(c->timer.end(gname, c->stream, grec));
-        k_tab_contract<R><<<(unsigned)(((N + 63) / 64) * Q), 64, 0, c->stream>>>(c->plan, U, N, H, c->n, c->nh,
-                                                                                y + (int64_t)c0 * c->n);
-        TG_HIP(hipGetLastError());
+        TG_TRY(launch_contract<R>(c, U, nc, y + (int64_t)c0 * c->n));
     }
     TG_TRY(c->timer.end(name, c->stream, rec));
     return TG_OK;
@@ -1134,13 +1196,7 @@ int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols) {
     return coupling_tables<float>(c, x, y, cols, false);
 }
 
-int launch_tab_contract_c64(tg_cluster *c, const cf *U, int nc, cd *y) {
-    const int N = c->N, H = c->H;
-    k_tab_contract<float><<<(unsigned)(((N + 63) / 64) * nc * 2 * H), 64, 0, c->stream>>>(c->plan, U, N, H, c->n,
-                                                                                          c->nh, y);
-    TG_HIP(hipGetLastError());
-    return TG_OK;
-}
+int launch_tab_contract_c64(tg_cluster *c, const cf *U, int nc, cd *y) { return launch_contract<float>(c, U, nc, y); }
 
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular) {
     // pair tables and GEMMs while the tables fit the cluster's limit; beyond it the

@@ -148,7 +148,8 @@ __device__ __forceinline__ f32x4v mfma_h(h8v a, h8v b, f32x4v c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
-// U[g][soff[t] + r][i] = sum_j TAB_t[i][j] X[g H + smode[soff[t] + r]][j] (c64 out).
+// U[g][gpos[soff[t] + r]][i] = sum_j TAB_t[i][j] X[g H + smode[soff[t] + r]][j] (c64
+// out; columns in the contraction's group order).
 // One 256 x 64 tile (rows i, subset columns) per workgroup of 8 waves; wave w owns
 // rows 32 w .. 32 w + 31 of the tile and all 64 columns (2 x 4 MFMA tiles), computed
 // transposed (columns as the MFMA's M, rows i as its N) so that lanes run along i in
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TAB
                                                                   const int *E, const int *cshift, cf *U, int N,
                                                                   int Np, int H, int S, int G, int T_,
                                                                   const int32_t *tiles, const int32_t *soff,
-                                                                  const int32_t *smode) {
+                                                                  const int32_t *smode, const int32_t *gpos) {
     const int nb = gridDim.x, bid = blockIdx.x;
     const int xcd = bid & 7, q8 = nb >> 3, r8 = nb & 7;
     const int L = xcd * q8 + min(xcd, r8) + (bid >> 3);
@@ -288,7 +289,7 @@ __global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TAB
                 if (col >= ncol) continue;
                 const int g = col / nt, r = col - g * nt;
                 const int sh = -(rs + cshift[g * H + smode[s0 + r]]);
-                U[((int64_t)g * S + s0 + r) * N + i] =
+                U[((int64_t)g * S + gpos[s0 + r]) * N + i] =
                     cf(ldexpf(acc[0][a][b][v] - acc[1][a][b][v], sh), ldexpf(acc[2][a][b][v], sh));
             }
     }
@@ -350,7 +351,7 @@ int coupling_tables_split16(tg_cluster *c, const cd *tab64, const cd *x, cd *y, 
         TG_TRY(c->timer.begin("gemm_split16", c->stream, grec));
         k_zgemm_subsets_split16<<<grid, 512, 0, c->stream>>>(c->b_tabh.as<h8v>(), XH, c->b_texp.as<int>(), cshift,
                                                              U, N, Np, H, S, 2 * nc, T, tiles, c->plan.soff,
-                                                             c->plan.smode);
+                                                             c->plan.smode, c->plan.gpos);
         TG_HIP(hipGetLastError());
         TG_TRY(c->timer.end("gemm_split16", c->stream, grec));
         TG_TRY(launch_tab_contract_c64(c, U, nc, y + (int64_t)c0 * c->n));
