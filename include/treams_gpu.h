@@ -492,7 +492,8 @@ int tg_cluster_pair_features(tg_cluster *cluster, float *features_dev);
 /* Set the pair weights from network outputs (device float[npairs * 16]). */
 int tg_cluster_set_pair_output(tg_cluster *cluster, const float *out_dev);
 /* y = M^{-1} x (adjoint 1: M^{-H} x), reference layout, device vectors; adjoint 2: M^{-1} x
- * as the mixed-precision inner solves apply it (c64 same-helicity blocks, FP32 products). */
+ * as the mixed-precision inner solves apply it (same-helicity near blocks stored as
+ * bfloat16 pairs, FP32 products). */
 int tg_cluster_learned_apply(tg_cluster *cluster, const tg_c128 *x_dev, tg_c128 *y_dev, int cols, int adjoint);
 /*
  * Training objective of the pair weights for probe columns V (device, n x cols):

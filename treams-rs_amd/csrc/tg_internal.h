@@ -180,6 +180,9 @@ int require_spheres(const tg_cluster *c, const char *what);
 // singular blocks as their two same-helicity H x H parts in c64, [count][h][a][b]
 int launch_translation_blocks_helicity_c64(int lmax, const DevPlan &plan, double k, int64_t count,
                                           const double *disp, cf *out, hipStream_t s);
+// the same as pairs of bfloat16 [count][h][a][b] (real part in the low half)
+int launch_translation_blocks_helicity_bf16(int lmax, const DevPlan &plan, double k, int64_t count,
+                                           const double *disp, uint32_t *out, hipStream_t s);
 int launch_translation_blocks(int lmax, const DevPlan &plan, double k, int64_t count, const double *disp,
                               int regular, cd *out, hipStream_t s);
 // tangent tables of every ordered pair (FP64, the layout of the singular pair tables)

@@ -244,6 +244,23 @@ struct OutBlocksHelicity {
     }
 };
 
+// The same with each complex value stored as two bfloat16 (real part in the low half,
+// rounded to nearest even): the near-field blocks of the preconditioner applied inside
+// mixed-precision solves, where an approximate operator only costs iterations.
+__device__ __forceinline__ uint32_t bf16_bits(float v) {
+    const uint32_t u = __float_as_uint(v);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;  // finite values
+}
+struct OutBlocksHelicityBF16 {
+    uint32_t *out;
+    int H;
+    __device__ void store(int, int j, int a, int b, cd vp, cd vm) const {
+        uint32_t *blk = out + (size_t)j * 2 * H * H;
+        blk[(size_t)a * H + b] = bf16_bits((float)vp.re) | (bf16_bits((float)vp.im) << 16);
+        blk[(size_t)H * H + (size_t)a * H + b] = bf16_bits((float)vm.re) | (bf16_bits((float)vm.im) << 16);
+    }
+};
+
 // Assembly: one workgroup per (source unit, group of P destination units).  The
 // harmonic tables of the P pairs sit in LDS; each thread owns block entries and
 // reuses every plan term it loads across the P pairs, accumulating in FP64.  The
@@ -466,6 +483,21 @@ int launch_translation_blocks_helicity_c64(int lmax, const DevPlan &pl, double k
     size_t smem = assemble_smem(pl, 1);
     TG_TRY(check_smem(smem, lmax));
     k_assemble<1, GeoList, OutBlocksHelicity, false><<<dim3((unsigned)count, 1), 256, smem, s>>>(pl, k, geo, out);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
+int launch_translation_blocks_helicity_bf16(int lmax, const DevPlan &pl, double k, int64_t count,
+                                           const double *disp, uint32_t *outp, hipStream_t s) {
+    const int H = lmax * (lmax + 2);
+    TG_HIP(hipMemsetAsync(outp, 0, sizeof(uint32_t) * 2 * H * H * count, s));
+    if (count == 0) return TG_OK;
+    GeoList geo{disp};
+    OutBlocksHelicityBF16 out{outp, H};
+    size_t smem = assemble_smem(pl, 1);
+    TG_TRY(check_smem(smem, lmax));
+    k_assemble<1, GeoList, OutBlocksHelicityBF16, false><<<dim3((unsigned)count, 1), 256, smem, s>>>(pl, k, geo,
+                                                                                                     out);
     TG_HIP(hipGetLastError());
     return TG_OK;
 }

@@ -29,7 +29,7 @@ struct LearnedState {
     tg::DevBuf disp;      // [np][3] pos_i - pos_j of the near pairs
     tg::DevBuf blocks;    // [np][M x M] c128, reference layout, column-major (built on first use)
     bool have_blocks = false;
-    tg::DevBuf blocksh;   // [np][2][H][H] c64 same-helicity blocks (forward applies of mixed-precision solves)
+    tg::DevBuf blocksh;   // [np][2][H][H] bfloat16-pair same-helicity blocks (forward applies of mixed-precision solves)
     bool have_blocksh = false;
     tg::DevBuf part;      // forward apply: weighted pair products [np][4][M]
     tg::DevBuf feats;   // [np][64] fp32
@@ -134,18 +134,19 @@ __global__ __launch_bounds__(64) void k_near_pair_products(int M, int H, int lma
     }
 }
 
-// part[p][cc][r] as k_near_pair_products, from the same-helicity c64 blocks
-// [p][h][a][b]: one workgroup per pair, wave h takes the helicity-h rows, lane l the
-// destinations b = l + 64 k (reference row 2 b + h); the sum runs over the H sources a
-// of that helicity.  The source values x_j (both helicities, up to 4 columns, zero
-// past nc) are staged in LDS as FP32 first, so the loop is branch-free: the block
-// values of 8 sources are loaded before any is used (8 loads in flight per lane),
-// then multiplied with FP32 accumulation (the blocks are c64; the inner solves this
-// preconditioner serves run at c64 precision, and the FP64 residual certifies them).
+// part[p][cc][r] as k_near_pair_products, from the same-helicity blocks [p][h][a][b]
+// stored as bfloat16 pairs: one workgroup per pair, wave h takes the helicity-h rows,
+// lane l the destinations b = l + 64 k (reference row 2 b + h); the sum runs over the H
+// sources a of that helicity.  The source values x_j (both helicities, up to 4
+// columns, zero past nc) are staged in LDS as FP32 first, so the loop is branch-free:
+// the block values of 8 sources are loaded before any is used (8 loads in flight per
+// lane), then multiplied with FP32 accumulation.  This preconditioner serves the
+// inner solves of mixed precision: a rounded M changes iteration counts at most,
+// never the answer, which the FP64 residual certifies.
 template <int RS>
 __global__ __launch_bounds__(128) void k_near_pair_products_helicity(int H, int lmax, int64_t n, int64_t nh, int nc,
                                                                      const int *__restrict__ pj,
-                                                                     const cf *__restrict__ blocks,
+                                                                     const uint32_t *__restrict__ blocks,
                                                                      const cd *__restrict__ alpha,
                                                                      const cd *__restrict__ x, cd *__restrict__ part) {
     constexpr int NC = 4, U = 8;
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(128) void k_near_pair_products_helicity(int H, int 
         xsf[t] = make_float2((float)v.re, (float)v.im);
     }
     __syncthreads();
-    const cf *blk = blocks + ((int64_t)p * 2 + h) * H * H;
+    const uint32_t *blk = blocks + ((int64_t)p * 2 + h) * H * H;
     const float2 *xh = xsf + h * H * NC;
     int dst[RS];
 #pragma unroll
@@ -169,20 +170,26 @@ __global__ __launch_bounds__(128) void k_near_pair_products_helicity(int H, int 
     for (int k = 0; k < RS; ++k)
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) sum[k][cc] = make_float2(0.f, 0.f);
-    auto mac = [&](int a, const cf (&bv)[RS]) {
+    auto mac = [&](int a, const uint32_t (&bw)[RS]) {
+        float br[RS], bi[RS];
+#pragma unroll
+        for (int k = 0; k < RS; ++k) {
+            br[k] = __uint_as_float(bw[k] << 16);
+            bi[k] = __uint_as_float(bw[k] & 0xffff0000u);
+        }
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) {
             const float2 xv = xh[a * NC + cc];
 #pragma unroll
             for (int k = 0; k < RS; ++k) {
-                sum[k][cc].x = fmaf(bv[k].re, xv.x, fmaf(-bv[k].im, xv.y, sum[k][cc].x));
-                sum[k][cc].y = fmaf(bv[k].re, xv.y, fmaf(bv[k].im, xv.x, sum[k][cc].y));
+                sum[k][cc].x = fmaf(br[k], xv.x, fmaf(-bi[k], xv.y, sum[k][cc].x));
+                sum[k][cc].y = fmaf(br[k], xv.y, fmaf(bi[k], xv.x, sum[k][cc].y));
             }
         }
     };
     int a0 = 0;
     for (; a0 + U <= H; a0 += U) {
-        cf bv[U][RS];
+        uint32_t bv[U][RS];
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -191,7 +198,7 @@ __global__ __launch_bounds__(128) void k_near_pair_products_helicity(int H, int 
         for (int u = 0; u < U; ++u) mac(a0 + u, bv[u]);
     }
     for (; a0 < H; ++a0) {
-        cf bv[RS];
+        uint32_t bv[RS];
 #pragma unroll
         for (int k = 0; k < RS; ++k) bv[k] = blk[a0 * H + dst[k]];
         mac(a0, bv);
@@ -576,16 +583,15 @@ static int near_apply_forward(tg_cluster *c, const B *blocks, const cd *x, cd *y
     return TG_OK;
 }
 
-// forward apply with the same-helicity near blocks in c64 (a quarter of the c128
-// blocks' bytes; the products and sums stay FP64), for the inner solves of mixed
-// precision
-static int learned_near_apply_c64(tg_cluster *c, const cd *x, cd *y, int cols) {
+// forward apply with the same-helicity near blocks as bfloat16 pairs (an eighth of the
+// c128 blocks' bytes; FP32 products and sums), for the inner solves of mixed precision
+static int learned_near_apply_inner(tg_cluster *c, const cd *x, cd *y, int cols) {
     LearnedState *s = c->learned;
     const int H = c->H, M = c->M;
     if (!s->have_blocksh) {
-        TG_TRY(s->blocksh.ensure(sizeof(cf) * (size_t)s->npairs * 2 * H * H));
-        TG_TRY(launch_translation_blocks_helicity_c64(c->lmax, c->plan, c->k0, s->npairs, s->disp.as<double>(),
-                                                      s->blocksh.as<cf>(), c->stream));
+        TG_TRY(s->blocksh.ensure(sizeof(uint32_t) * (size_t)s->npairs * 2 * H * H));
+        TG_TRY(launch_translation_blocks_helicity_bf16(c->lmax, c->plan, c->k0, s->npairs, s->disp.as<double>(),
+                                                       s->blocksh.as<uint32_t>(), c->stream));
         s->have_blocksh = true;
     }
     TG_TRY(s->part.ensure(sizeof(cd) * (size_t)s->npairs * 4 * M));
@@ -596,7 +602,7 @@ static int learned_near_apply_c64(tg_cluster *c, const cd *x, cd *y, int cols) {
         const int rs = (H + 63) / 64;
 #define TG_NEARH(RS)                                                                                               \
     k_near_pair_products_helicity<RS><<<(unsigned)s->npairs, 128, sizeof(float2) * 2 * H * 4, c->stream>>>(           \
-        H, c->lmax, c->n, c->nh, nc, s->pj, s->blocksh.as<cf>(), s->alpha.as<cd>(), xc, part)
+        H, c->lmax, c->n, c->nh, nc, s->pj, s->blocksh.as<uint32_t>(), s->alpha.as<cd>(), xc, part)
         if (rs <= 1) TG_NEARH(1);
         else if (rs == 2) TG_NEARH(2);
         else if (rs <= 4) TG_NEARH(4);
@@ -640,7 +646,7 @@ int learned_setup(tg_cluster *c, Preconditioner *) {
 }
 
 int learned_apply(tg_cluster *c, Preconditioner *, const cd *x, cd *y, int cols, int adjoint, int storage) {
-    if (!adjoint && storage == TG_STORE_C64) return learned_near_apply_c64(c, x, y, cols);
+    if (!adjoint && storage == TG_STORE_C64) return learned_near_apply_inner(c, x, y, cols);
     return learned_near_apply(c, x, y, cols, adjoint);
 }
 
@@ -734,7 +740,7 @@ int tg_cluster_learned_apply(tg_cluster *c, const tg_c128 *x, tg_c128 *y, int co
     TG_TRY(io.ensure(sizeof(cd) * count * 2));
     cd *xs = io.as<cd>(), *ys = xs + count;
     int st = ref_to_split(c, (const cd *)x, xs, cols);
-    if (!st) st = adjoint == 2 ? learned_near_apply_c64(c, xs, ys, cols) : learned_near_apply(c, xs, ys, cols, adjoint);
+    if (!st) st = adjoint == 2 ? learned_near_apply_inner(c, xs, ys, cols) : learned_near_apply(c, xs, ys, cols, adjoint);
     if (!st) st = split_to_ref(c, ys, (cd *)y, cols);
     if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = TG_DEVICE_ERROR;
     io.release();

@@ -117,12 +117,12 @@ def set_pair_output(cluster: IterativeSphereCluster, out: torch.Tensor) -> None:
     check(lib().tg_cluster_set_pair_output(cluster._h, out.data_ptr()))
 
 
-def learned_apply(cluster: IterativeSphereCluster, x, adjoint=False, c64=False):
-    """M^{-1} x (M^{-H} x with adjoint); c64: as the mixed-precision inner solves apply
-    it (c64 same-helicity blocks, FP32 products)."""
+def learned_apply(cluster: IterativeSphereCluster, x, adjoint=False, inner=False):
+    """M^{-1} x (M^{-H} x with adjoint); inner: as the mixed-precision inner solves apply
+    it (same-helicity near blocks stored as bfloat16 pairs, FP32 products)."""
     cm, vector = _columns(x, cluster.n)
     y = torch.empty_like(cm)
-    mode = 2 if c64 else int(adjoint)
+    mode = 2 if inner else int(adjoint)
     check(lib().tg_cluster_learned_apply(cluster._h, cm.data_ptr(), y.data_ptr(), cm.shape[0], mode))
     return y[0] if vector else y.T.contiguous()
 
