@@ -1062,10 +1062,9 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_subsets(const cx<T> *TAB, cons
 // position gpos[s]), so an order's columns are contiguous; wave w takes columns w,
 // w + 4, ... of them (wave-uniform, so the weights are scalar loads); the four waves'
 // sums are added in LDS in wave order.  LB >= lmax bounds the register arrays.
-template <class T, int LB>
-__global__ __launch_bounds__(256) void k_tab_contract_mu(DevPlan pl, const cx<T> *__restrict__ U, int N, int H,
-                                                         int64_t n, int64_t nh, cd *__restrict__ y) {
-    constexpr int UN = 8;  // loads in flight per wave
+template <class T, int LB, int NW = 4, int UN = 8>
+__global__ __launch_bounds__(NW * 64) void k_tab_contract_mu(DevPlan pl, const cx<T> *__restrict__ U, int N, int H,
+                                                             int64_t n, int64_t nh, cd *__restrict__ y) {
     const int lmax = pl.lmax, ng = 2 * lmax + 1, NI = (N + 63) / 64;
     const int nb = gridDim.x, bid = blockIdx.x;
     const int xcd = bid & 7, q8 = nb >> 3, r8 = nb & 7;
@@ -1083,14 +1082,14 @@ __global__ __launch_bounds__(256) void k_tab_contract_mu(DevPlan pl, const cx<T>
     for (int l = 0; l < LB; ++l) sr[l] = si[l] = cr[l] = ci[l] = 0.0;
     // U holds the columns in group order (the GEMMs store column s at gpos[s]), so the
     // order's columns are one contiguous block of k1 - k0 columns
-    for (int k = k0 + wave; k < k1; k += 4 * UN) {
+    for (int k = k0 + wave; k < k1; k += NW * UN) {
         cx<T> u[UN];
 #pragma unroll
-        for (int v = 0; v < UN; ++v) u[v] = Ug[(int64_t)min(k + 4 * v, k1 - 1) * N];
+        for (int v = 0; v < UN; ++v) u[v] = Ug[(int64_t)min(k + NW * v, k1 - 1) * N];
 #pragma unroll
         for (int v = 0; v < UN; ++v) {
-            if (k + 4 * v >= k1) break;
-            const double *wk = pl.gw + (int64_t)(k + 4 * v) * lmax * 2;
+            if (k + NW * v >= k1) break;
+            const double *wk = pl.gw + (int64_t)(k + NW * v) * lmax * 2;
             const double ur = (double)u[v].re, ui = (double)u[v].im;
 #pragma unroll
             for (int l = 0; l < LB; ++l) {
@@ -1105,7 +1104,7 @@ __global__ __launch_bounds__(256) void k_tab_contract_mu(DevPlan pl, const cx<T>
     }
     // one lambda at a time through LDS (every wave reaches every barrier: lmax and mu are
     // uniform)
-    __shared__ double red[3][4][64];
+    __shared__ double red[NW - 1][4][64];
     const int mu = mg - lmax, cc = g >> 1, h = g & 1;
     cd *yo = y + (int64_t)cc * n + (int64_t)h * nh + (int64_t)ic * H;
 #pragma unroll
@@ -1123,7 +1122,7 @@ __global__ __launch_bounds__(256) void k_tab_contract_mu(DevPlan pl, const cx<T>
         if (wave == 0 && i < N) {
             double a0 = sr[l], a1 = si[l], a2 = cr[l], a3 = ci[l];
 #pragma unroll
-            for (int w = 0; w < 3; ++w) {
+            for (int w = 0; w < NW - 1; ++w) {
                 a0 += red[w][0][lane];
                 a1 += red[w][1][lane];
                 a2 += red[w][2][lane];
@@ -1135,13 +1134,16 @@ __global__ __launch_bounds__(256) void k_tab_contract_mu(DevPlan pl, const cx<T>
     }
 }
 
+constexpr int kContractWaves = 4, kContractLoads = 8;
+
 // y = the plan contraction of U for nc right-hand sides (2 nc column groups)
 template <class T>
 static int launch_contract(tg_cluster *c, const cx<T> *U, int nc, cd *y) {
     const int N = c->N, H = c->H, lmax = c->plan.lmax;
     const unsigned grid = (unsigned)(((N + 63) / 64) * (2 * lmax + 1) * 2 * nc);
 #define TG_CONTRACT(LB)                                                                                           \
-    k_tab_contract_mu<T, LB><<<grid, 256, 0, c->stream>>>(c->plan, U, N, H, c->n, c->nh, y)
+    k_tab_contract_mu<T, LB, kContractWaves, kContractLoads><<<grid, kContractWaves * 64, 0, c->stream>>>(        \
+        c->plan, U, N, H, c->n, c->nh, y)
     if (lmax <= 2) TG_CONTRACT(2);
     else if (lmax <= 4) TG_CONTRACT(4);
     else if (lmax <= 6) TG_CONTRACT(6);
@@ -1664,16 +1666,29 @@ __global__ __launch_bounds__(256) void k_dots(const cd *__restrict__ V, int64_t 
     }
 }
 
-// out[r] = sum of partial row r (width values; one wave per row: lane l adds values l,
-// l + 64, ..., then a fixed shuffle tree), zero where mask[r / nvec] is 0, to out_dev
-// and/or host_out[copy_len + r], after copy_src[0 .. copy_len) is copied to host_out.
-// One workgroup.
+// out[r] = sum of partial row r (width values), zero where mask[r / nvec] is 0, to
+// out_dev and/or host_out[copy_len + r], after copy_src[0 .. copy_len) is copied to
+// host_out (workgroup 0).  Rows of at most 16 values: one thread per row, values in
+// order; wider rows: one wave per row, lane l adding values l, l + 64, ..., then a
+// fixed shuffle tree.
 __global__ __launch_bounds__(256) void k_dots_finish(const cd *__restrict__ partial, int width, int rows, int nvec,
                                                       const double *mask, cd *out_dev, cd *host_out,
                                                       const cd *copy_src, int64_t copy_len) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int64_t k = threadIdx.x; k < copy_len; k += blockDim.x) host_out[k] = copy_src[k];
-    for (int r = wave; r < rows; r += 4) {
+    if (blockIdx.x == 0)
+        for (int64_t k = threadIdx.x; k < copy_len; k += blockDim.x) host_out[k] = copy_src[k];
+    if (width <= 16) {
+        const int r = blockIdx.x * blockDim.x + threadIdx.x;
+        if (r >= rows) return;
+        const cd *pr = partial + (int64_t)r * width;
+        cd s(0.0);
+        for (int b = 0; b < width; ++b) s += pr[b];
+        if (mask && mask[r / nvec] == 0.0) s = cd(0.0);
+        if (out_dev) out_dev[r] = s;
+        if (host_out) host_out[copy_len + r] = s;
+        return;
+    }
+    for (int r = blockIdx.x * 4 + wave; r < rows; r += 4 * gridDim.x) {
         const cd *pr = partial + (int64_t)r * width;
         double sre = 0.0, sim = 0.0;
         for (int b = lane; b < width; b += 64) {
@@ -1712,8 +1727,10 @@ static int dots_launch(tg_cluster *c, const cd *V, int64_t vstride, int nvec, co
     cd *partial = c->ws[4].as<cd>();
     k_dots<<<dim3(nb, cols), 256, 0, c->stream>>>(V, vstride, nvec, w, n, csv, csw, partial);
     TG_HIP(hipGetLastError());
-    k_dots_finish<<<1, 256, 0, c->stream>>>(partial, nb, cols * nvec, nvec, mask, out_dev, host_out, copy_src,
-                                            copy_len);
+    const int rows = cols * nvec;
+    const unsigned fblocks = nb <= 16 ? (unsigned)((rows + 255) / 256) : (unsigned)std::min(64, (rows + 3) / 4);
+    k_dots_finish<<<fblocks, 256, 0, c->stream>>>(partial, nb, rows, nvec, mask, out_dev, host_out, copy_src,
+                                                  copy_len);
     TG_HIP(hipGetLastError());
     return TG_OK;
 }
