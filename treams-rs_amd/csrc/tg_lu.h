@@ -21,10 +21,11 @@ struct LuSet {
     const int *n = nullptr;
     const int64_t *voff = nullptr;
     int *piv = nullptr, *perm = nullptr, *status = nullptr;
-    // the look-ahead panel stream and its two events, created at the first
-    // factorisation and kept until luset_release
+    // the look-ahead panel stream and its two events, taken at the first factorisation
+    // (from the device's pool, or created) and returned to the pool by luset_release
     hipStream_t panel = nullptr;
     hipEvent_t ready = nullptr, done = nullptr;
+    int panel_dev = 0;
 };
 
 // Matrix z: dimension n[z], elements from A + off[z] (column-major, leading dim n[z]).

@@ -19,6 +19,8 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "tg_gmres.h"
@@ -609,15 +611,53 @@ int luset_init(LuSet &ls, int precision, void *A, const std::vector<int64_t> &of
     return TG_OK;
 }
 
+// Look-ahead streams and their event pairs, kept per device for the next factor (a
+// stream's creation costs about as much as a small operator's whole factorisation).
+namespace {
+struct PanelSet {
+    hipStream_t stream;
+    hipEvent_t ready, done;
+};
+std::mutex g_panel_mu;
+std::map<int, std::vector<PanelSet>> g_panels;
+}  // namespace
+
+static int panel_acquire(LuSet &ls) {
+    if (ls.panel) return TG_OK;
+    int dev = 0;
+    TG_HIP(hipGetDevice(&dev));
+    ls.panel_dev = dev;
+    {
+        std::lock_guard<std::mutex> lock(g_panel_mu);
+        auto &v = g_panels[dev];
+        if (!v.empty()) {
+            ls.panel = v.back().stream;
+            ls.ready = v.back().ready;
+            ls.done = v.back().done;
+            v.pop_back();
+            return TG_OK;
+        }
+    }
+    TG_HIP(hipStreamCreateWithFlags(&ls.panel, hipStreamNonBlocking));
+    TG_HIP(hipEventCreateWithFlags(&ls.ready, hipEventDisableTiming));
+    TG_HIP(hipEventCreateWithFlags(&ls.done, hipEventDisableTiming));
+    return TG_OK;
+}
+
 void luset_release(LuSet &ls) {
     if (ls.panel) {
-        (void)hipStreamSynchronize(ls.panel);
-        (void)hipStreamDestroy(ls.panel);
+        const bool ok = hipStreamSynchronize(ls.panel) == hipSuccess && ls.ready && ls.done;
+        if (ok) {
+            std::lock_guard<std::mutex> lock(g_panel_mu);
+            g_panels[ls.panel_dev].push_back(PanelSet{ls.panel, ls.ready, ls.done});
+        } else {
+            (void)hipStreamDestroy(ls.panel);
+            if (ls.ready) (void)hipEventDestroy(ls.ready);
+            if (ls.done) (void)hipEventDestroy(ls.done);
+        }
         ls.panel = nullptr;
+        ls.ready = ls.done = nullptr;
     }
-    if (ls.ready) (void)hipEventDestroy(ls.ready);
-    if (ls.done) (void)hipEventDestroy(ls.done);
-    ls.ready = ls.done = nullptr;
     ls.meta.release();
     ls.work.release();
 }
@@ -657,9 +697,7 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
             st = TG_DEVICE_ERROR;
         }
     };
-    if (!ls.panel) check(hipStreamCreateWithFlags(&ls.panel, hipStreamNonBlocking));
-    if (!ls.ready) check(hipEventCreateWithFlags(&ls.ready, hipEventDisableTiming));
-    if (!ls.done) check(hipEventCreateWithFlags(&ls.done, hipEventDisableTiming));
+    if (!ls.panel && (st = panel_acquire(ls))) return st;
     hipStream_t ps = ls.panel;
     hipEvent_t ready = ls.ready, done = ls.done;
     auto swap = [&](hipStream_t q, int64_t k, int64_t cb, int64_t ce) {  // one wave per column
