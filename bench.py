@@ -105,6 +105,33 @@ def run_method(cl, B, torch, *, mixed, pc, steps, warmup, rtol, restart, label, 
     return x, conv, elapsed, kernels
 
 
+_WRITE_CEILING = None
+
+
+def write_ceiling(torch, gib=16):
+    """The write-only HBM rate this device reaches: torch's fill kernel over a 16 GiB
+    buffer (stores only, 16 bytes per thread), best of 5 after a warm-up.  The dense
+    assembly is a write stream, so this, not the 8 TB/s read-and-write peak, is what
+    it can approach."""
+    global _WRITE_CEILING
+    if _WRITE_CEILING is None:
+        buf = torch.empty((gib << 30) // 16, dtype=torch.complex128, device="cuda")
+        buf.zero_()
+        best = float("inf")
+        for _ in range(5):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            buf.zero_()
+            b.record()
+            b.synchronize()
+            best = min(best, a.elapsed_time(b))
+        del buf
+        torch.cuda.empty_cache()
+        _WRITE_CEILING = {"gbs": (gib << 30) / (best * 1e-3) / 1e9, "bytes": gib << 30, "ms": best,
+                          "kernel": "torch fill (zero_) of a 16 GiB buffer, best of 5"}
+    return _WRITE_CEILING
+
+
 def cpu_info():
     """Host CPU model, logical CPUs and the threads the OpenMP oracle runs with."""
     model = "unknown"
@@ -469,10 +496,12 @@ def main() -> int:
         asm = kern.get("assemble", {})
         if asm:
             achieved = c64_bytes / (asm["avg_ms"] * 1e-3) / 1e9
+            wc = write_ceiling(torch)
             roofs["assembly"] = {"bound": "hbm", "kernel": "k_assemble (dense c64 coupling, written once)",
                                  "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": achieved / HBM_PEAK_GBS, "bytes_per_launch": c64_bytes,
-                                 "avg_ms": asm["avg_ms"]}
+                                 "avg_ms": asm["avg_ms"], "write_only_ceiling": wc,
+                                 "frac_of_write_only_ceiling": achieved / wc["gbs"]}
         return xm, cv, out, roofs
 
     def add_traffic(roof, name):
