@@ -527,6 +527,13 @@ int tg_cluster_create(int lmax, double k0, int n_spheres, const double *radii, c
 int tg_cluster_destroy(tg_cluster *c) {
     if (!c) return TG_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)side_sync(c);
+    if (c->side) {
+        (void)hipStreamDestroy(c->side);
+        (void)hipEventDestroy(c->side_start);
+        (void)hipEventDestroy(c->side_done);
+        c->side = nullptr;
+    }
     destroy_preconditioner(c);
     c->b_plan.release();
     c->b_pos.release();
@@ -546,6 +553,7 @@ int tg_cluster_destroy(tg_cluster *c) {
     c->b_tloc.release();
     learned_release(c);
     for (auto &w : c->ws) w.release();
+    c->outer.release();
     c->pin.release();
     c->timer.collect();
     delete c;
@@ -747,6 +755,7 @@ int tg_cluster_assemble(tg_cluster *c, int storage) {
 int tg_cluster_release(tg_cluster *c) {
     TG_TRY(check_cluster(c));
     TG_HIP(hipStreamSynchronize(c->stream));
+    TG_TRY(side_sync(c));
     destroy_preconditioner(c);
     c->b_C.release();
     c->b_tab.release();
@@ -763,14 +772,16 @@ int tg_cluster_release(tg_cluster *c) {
     c->storage = TG_STORE_NONE;
     learned_release(c);
     for (auto &w : c->ws) w.release();
+    c->outer.release();
     return TG_OK;
 }
 
 int tg_cluster_reset(tg_cluster *c) {
     TG_TRY(check_cluster(c));
     TG_HIP(hipStreamSynchronize(c->stream));
+    TG_TRY(side_sync(c));
     destroy_preconditioner(c);
-    learned_release(c);
+    learned_invalidate(c);
     c->storage = TG_STORE_NONE;
     c->have_tables = false;
     c->have_tables32 = false;
@@ -787,6 +798,7 @@ int tg_cluster_set_inner_gemm(tg_cluster *c, int kind) {
         return TG_INVALID_INPUT;
     }
     TG_HIP(hipStreamSynchronize(c->stream));
+    TG_TRY(side_sync(c));
     c->inner_gemm = kind;
     c->b_tab32.release();
     c->have_tables32 = false;
@@ -805,6 +817,7 @@ int tg_cluster_set_table_limit(tg_cluster *c, int64_t bytes) {
         return TG_INVALID_INPUT;
     }
     TG_HIP(hipStreamSynchronize(c->stream));
+    TG_TRY(side_sync(c));
     c->table_limit = bytes;
     c->b_tab.release();
     c->have_tables = false;
@@ -921,6 +934,8 @@ int solve_split(tg_cluster *c, const cd *b, cd *x, int cols, const tg_gmres_opti
     prm.max_iterations = o.max_iterations;
     OpFn pc_fn;
     const OpFn *pc = nullptr;
+    // the inner coupling's tables build on the side stream meanwhile
+    if (o.mixed_precision == 2 && o.preconditioner != TG_PC_NONE) TG_TRY(prefetch_inner_tables(c));
     if (o.preconditioner != TG_PC_NONE) {
         TG_TRY(ensure_preconditioner(c, o.preconditioner, o.mixed_precision ? TG_STORE_C64 : TG_STORE_C128));
         pc_fn = [c, adjoint](const cd *in, cd *out, int nc) -> int { return apply_preconditioner(c, in, out, nc, adjoint); };
@@ -942,7 +957,9 @@ int solve_split(tg_cluster *c, const cd *b, cd *x, int cols, const tg_gmres_opti
     OpFn A32 = [c, adjoint, op32](const cd *in, cd *out, int nc) -> int {
         return apply_operator(c, in, out, nc, adjoint, op32);
     };
-    DevBuf outer;
+    // kept with the cluster: a fresh block from the buffer cache would synchronise the
+    // device, and with it the side stream's table build
+    DevBuf &outer = c->outer;
     TG_TRY(outer.ensure(sizeof(cd) * count * 3));
     cd *r = outer.as<cd>(), *d = r + count, *ax = d + count;
     std::vector<double> bn(cols), rn(cols);
@@ -964,7 +981,6 @@ int solve_split(tg_cluster *c, const cd *b, cd *x, int cols, const tg_gmres_opti
             conv[k].refinements = refinements;
             if (!std::isfinite(rn[k])) {
                 set_error("non-finite residual in mixed-precision refinement");
-                outer.release();
                 return TG_NOT_CONVERGED;
             }
             if (rn[k] > tol) done = false;
@@ -974,7 +990,6 @@ int solve_split(tg_cluster *c, const cd *b, cd *x, int cols, const tg_gmres_opti
         if (iterations >= o.max_iterations || refinements >= 20) {
             set_error("mixed-precision GMRES did not converge after %d iterations (%d refinements)", iterations,
                       refinements);
-            outer.release();
             return TG_NOT_CONVERGED;
         }
         // inner relative tolerance: half of what the outer still needs, but not below
@@ -1010,7 +1025,6 @@ int solve_split(tg_cluster *c, const cd *b, cd *x, int cols, const tg_gmres_opti
             break;
         }
     }
-    outer.release();
     return st;
 }
 

@@ -146,7 +146,13 @@ struct tg_cluster {
     bool have_tablesR = false;
     int64_t table_limit = (int64_t)64 << 30;  // largest pair-table set kept (bytes)
     tg::DevBuf ws[8];                  // scratch
+    tg::DevBuf outer;                  // residual, correction and A x of mixed-precision refinement
     tg::HostBuf pin;                   // pinned host results of the Krylov reductions
+    // side stream that builds the inner coupling's tables while the preconditioner is
+    // set up; table users wait for side_done while tables_pending
+    hipStream_t side = nullptr;
+    hipEvent_t side_start = nullptr, side_done = nullptr;
+    bool tables_pending = false;
     tg::Timer timer;
     tg::Preconditioner *pc = nullptr;
     int pc_kind = TG_PC_NONE;
@@ -199,6 +205,12 @@ int launch_translation_rect(const RectPlan &pl, double k, int64_t count, const d
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular);
 // whether one FP64 pair-table set fits the cluster's table limit
 bool tables_fit(const tg_cluster *c);
+// start the FP64 and split tables of the c64-precision inner coupling on the side
+// stream (no-op when built, not fitting or not split); tables_wait makes the
+// cluster's stream wait for them, side_sync waits on the host (before freeing them)
+int prefetch_inner_tables(tg_cluster *c);
+int tables_wait(tg_cluster *c);
+int side_sync(tg_cluster *c);
 // y = C' x for a caller's FP64 table set TAB[h][j][i] (pair-table GEMMs and the plan
 // contraction of the coupling, e.g. on tangent tables)
 int coupling_tables_f64(tg_cluster *c, const cd *TAB, const cd *x, cd *y, int cols);

@@ -949,6 +949,7 @@ bool tables_fit(const tg_cluster *c) {
 // The FP64 singular (or regular) tables, and with c64 also their c64 rounding: built
 // together in one pass when neither exists yet.
 static int ensure_pair_tables(tg_cluster *c, bool regular, bool c64) {
+    TG_TRY(tables_wait(c));
     bool &have = regular ? c->have_tablesR : c->have_tables;
     DevBuf &buf = regular ? c->b_tabR : c->b_tab;
     const bool need32 = c64 && !regular && !c->have_tables32;
@@ -978,6 +979,44 @@ static int ensure_pair_tables(tg_cluster *c, bool regular, bool c64) {
     have = true;
     if (need32) c->have_tables32 = true;
     return TG_OK;
+}
+
+int tables_wait(tg_cluster *c) {
+    if (!c->tables_pending) return TG_OK;
+    c->tables_pending = false;
+    TG_HIP(hipStreamWaitEvent(c->stream, c->side_done, 0));
+    return TG_OK;
+}
+
+int side_sync(tg_cluster *c) {
+    c->tables_pending = false;
+    if (c->side) TG_HIP(hipStreamSynchronize(c->side));
+    return TG_OK;
+}
+
+// The inner coupling's tables are built on the side stream, after the work already on
+// the cluster's stream, while the cluster's stream sets up the preconditioner (its
+// kernels and host round trips); the first table user waits for them.
+int prefetch_inner_tables(tg_cluster *c) {
+    if (c->tables_pending || c->have_tablesh || c->inner_gemm != TG_INNER_SPLIT16 || !tables_fit(c)) return TG_OK;
+    if (!c->side) {
+        TG_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        TG_HIP(hipEventCreateWithFlags(&c->side_start, hipEventDisableTiming));
+        TG_HIP(hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming));
+    }
+    TG_HIP(hipEventRecord(c->side_start, c->stream));
+    TG_HIP(hipStreamWaitEvent(c->side, c->side_start, 0));
+    hipStream_t main_stream = c->stream;
+    c->stream = c->side;
+    int st = ensure_pair_tables(c, false, false);
+    if (!st) st = ensure_split_tables(c, c->b_tab.as<cd>());
+    c->stream = main_stream;
+    if (hipEventRecord(c->side_done, c->side) != hipSuccess && !st) {
+        set_error("hipEventRecord failed on the side stream");
+        st = TG_DEVICE_ERROR;
+    }
+    c->tables_pending = true;  // even after an error: the cluster's stream still waits
+    return st;
 }
 
 // X[q][j] with q = (column cc, helicity h, mode a): the GEMM's right operand, in the

@@ -652,6 +652,18 @@ int learned_apply(tg_cluster *c, Preconditioner *, const cd *x, cd *y, int cols,
 
 void learned_destroy(Preconditioner *) {}
 
+// near pairs, features, weights and blocks to be rebuilt, their buffers kept (a fresh
+// block from the buffer cache synchronises the device)
+void learned_invalidate(tg_cluster *c) {
+    if (!c->learned) return;
+    LearnedState *s = c->learned;
+    s->npairs = 0;
+    s->have_alpha = false;
+    s->have_blocks = false;
+    s->have_blocksh = false;
+    s->have_cols = false;
+}
+
 void learned_release(tg_cluster *c) {
     if (!c->learned) return;
     LearnedState *s = c->learned;

@@ -11,6 +11,8 @@ int apply_preconditioner(tg_cluster *c, const cd *x, cd *y, int cols, int adjoin
 void destroy_preconditioner(tg_cluster *c);
 // free the near pairs, blocks and weights of the learned preconditioner
 void learned_release(tg_cluster *c);
+// rebuild near pairs, features, weights and blocks at the next use; buffers kept
+void learned_invalidate(tg_cluster *c);
 
 // Spatial groups of at most `size` spheres by recursive coordinate bisection.
 std::vector<std::vector<int>> spatial_groups(const std::vector<double> &positions, int N, int size);

@@ -297,6 +297,7 @@ __global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TAB
 
 // Split tables of the singular FP64 pair tables (built first when missing).
 int ensure_split_tables(tg_cluster *c, const cd *tab64) {
+    TG_TRY(tables_wait(c));
     if (c->have_tablesh) return TG_OK;
     const int N = c->N, T = c->plan.T, Np = (N + 31) & ~31;
     TG_TRY(c->b_texp.ensure(sizeof(int) * (size_t)T * Np));
