@@ -202,6 +202,23 @@ __global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TAB
         const int g = col / nt, r = col - g * nt;
         sp[u] = XH + (((int64_t)g * H + smode[s0 + r]) * 4 + xc) * Np + 8 * (lane >> 4);
     }
+    // the store's shifts, loaded before the K loop (in the store they would be a chain
+    // of dependent loads after it): column shifts of the lane's 16 output columns
+    // (clamped), row exponents of its two rows
+    int csh[4][4], rsh[2];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int col = min(c0 + 16 * b + 4 * (lane >> 4) + v, ncol - 1);
+            const int g = col / nt, r = col - g * nt;
+            csh[b][v] = cshift[g * H + smode[s0 + r]];
+        }
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int i = min(r0 + 32 * wave + 16 * a + (lane & 15), N - 1);
+        rsh[a] = shift_of(E[(int64_t)t * Np + i]);
+    }
     f32x4v acc[3][2][4];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
@@ -280,7 +297,6 @@ __global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TAB
     for (int a = 0; a < 2; ++a) {
         const int i = r0 + 32 * wave + 16 * a + (lane & 15);
         if (i >= N) continue;
-        const int rs = shift_of(E[(int64_t)t * Np + i]);
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
@@ -288,7 +304,7 @@ __global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TAB
                 const int col = c0 + 16 * b + 4 * (lane >> 4) + v;
                 if (col >= ncol) continue;
                 const int g = col / nt, r = col - g * nt;
-                const int sh = -(rs + cshift[g * H + smode[s0 + r]]);
+                const int sh = -(rsh[a] + csh[b][v]);
                 U[((int64_t)g * S + gpos[s0 + r]) * N + i] =
                     cf(ldexpf(acc[0][a][b][v] - acc[1][a][b][v], sh), ldexpf(acc[2][a][b][v], sh));
             }
