@@ -53,7 +53,9 @@ typedef enum { TG_STORE_NONE = 0, TG_STORE_C64 = 1, TG_STORE_C128 = 2 } tg_stora
 typedef enum {
     TG_OP_MATRIX_FREE_F64 = 0, /* FP64 matrix-free (cluster/iterative.rs:187-221): pair-table GEMMs */
     TG_OP_STORED = 1,          /* stream the stored coupling (c64 or c128) */
-    TG_OP_MATRIX_FREE_C64 = 2  /* matrix-free from c64 pair tables on FP32 MFMA (no reference counterpart) */
+    TG_OP_MATRIX_FREE_C64 = 2, /* matrix-free from c64 pair tables on FP32 MFMA (no reference counterpart) */
+    TG_OP_MATRIX_FREE_C64_HI = 3 /* the same with split-FP16 tables multiplying the hi halves only: the relaxed
+                                    applies of tg_cluster_set_inner_relaxation (no reference counterpart) */
 } tg_operator;
 
 /* Preconditioner of the GMRES solves (right preconditioning; no reference counterpart). */
@@ -230,6 +232,21 @@ int tg_cluster_set_table_limit(tg_cluster *cluster, int64_t bytes);
  */
 typedef enum { TG_INNER_FP32 = 0, TG_INNER_SPLIT16 = 1 } tg_inner_gemm;
 int tg_cluster_set_inner_gemm(tg_cluster *cluster, int kind);
+
+/*
+ * Relaxed inner applies of mixed-precision solves on split-FP16 tables (inexact
+ * Krylov: Simoncini and Szyld 2003, Bouras and Fraysse 2005).  Before each Arnoldi
+ * step of an inner solve the operator may err by tol / |r_j| relative (tol the inner
+ * tolerance, |r_j| the step's residual estimate, the smallest bound over the columns
+ * still extending); when that bound is at least `threshold` the step multiplies the
+ * FP16 hi halves only (11 significant bits per value, a third of the MFMAs), otherwise
+ * hi and lo.  threshold 0 switches relaxation off; the default is 1e-3 (the hi-only
+ * coupling errs by about 3e-4 relative).  The FP64
+ * residual of the refinement certifies every solve either way, so the threshold moves
+ * iteration counts, never the accepted answer.  No reference counterpart (device
+ * arithmetic choice).
+ */
+int tg_cluster_set_inner_relaxation(tg_cluster *cluster, double threshold);
 
 /*
  * y = (I - T C) x, or (I - C^H T^H) x with adjoint != 0, for `cols` column-major

@@ -156,6 +156,63 @@ def test_inner_gemm_split16_vs_fp32(pkg, N, lmax, cols):
     assert np.array_equal(cl.apply(x, c64=True).cpu().numpy(), first)
 
 
+@pytest.mark.parametrize("N,lmax,cols", [(6, 3, 3), (45, 4, 5), (96, 6, 4)])
+def test_inner_relaxed_hi_only_coupling(pkg, oracle, N, lmax, cols):
+    """The relaxed inner coupling (split-FP16 tables, hi halves only: 11 significant bits
+    per value, FP32 accumulation) against the oracle's FP64 coupling on a row subset and
+    against the library's FP64 coupling on every row: half-precision level (2^-11 per
+    operand), forward and adjoint; and measurably coarser than the hi + lo form, so the
+    hi-only kernel is the one that ran."""
+    radii, e, pos = pkg.random_cluster(N, 1.9 * N ** (1 / 3), 0.05, (0.15, 0.35), None, seed=N + 1)
+    cl = pkg.IterativeSphereCluster(lmax, 1.3, radii, e, pos)
+    rng = np.random.default_rng(N)
+    x = rng.normal(size=(cl.n, cols)) + 1j * rng.normal(size=(cl.n, cols))
+    ref = cl.coupling(x).cpu().numpy()
+    M = cl.n // N
+    rows = (0, min(N, 3))
+    yr = oracle.coupling_apply(lmax, 1.3, pos, x, rows=rows)[rows[0] * M:rows[1] * M]
+    assert _rel(ref[rows[0] * M:rows[1] * M], yr) <= 1e-12
+    hi = cl.coupling(x, c64="hi")
+    full = cl.coupling(x, c64=True)
+    err_hi, err_full = _rel(hi, ref), _rel(full, ref)
+    assert _rel(hi.cpu().numpy()[rows[0] * M:rows[1] * M], yr) <= 2e-3
+    assert err_hi <= 2e-3, err_hi
+    assert err_hi >= 10 * err_full, (err_hi, err_full)
+    assert _rel(cl.coupling(x, adjoint=True, c64="hi"), cl.coupling(x, adjoint=True)) <= 2e-3
+    print(N, lmax, f"hi-only {err_hi:.2e}, hi + lo {err_full:.2e}")
+
+
+def test_relaxed_inner_solves_certified(pkg, oracle):
+    """Mixed solves on split-FP16 tables with relaxed inner applies give the answer of
+    the unrelaxed solve: relaxation off, the default threshold, and every inner apply
+    hi-only (threshold 1e-300: more FP64 refinements, same certified tolerance).  The
+    timers show which GEMM ran."""
+    cl, ref = _cluster(pkg, oracle, N=24, lmax=4, seed=8, side=4.5)
+    B = torch.stack([cl.plane_wave([0, 0, 1], 1), cl.plane_wave([0, 0, -1], 0)], dim=1)
+    xr = ref.solve(B.cpu().numpy())
+    out = {}
+    for th in (0.0, 1e-3, 1e-300):
+        cl.set_inner_relaxation(th)
+        cl.timing_reset()
+        cl.timing(True)
+        x, conv = cl.solve(B, rtol=1e-10, restart=50, mixed_precision="matrix_free", convergence="full")
+        cl.timing(False)
+        for c in conv:
+            assert c.residual_norm <= 1e-10 * c.rhs_norm
+        assert _rel(x, xr) <= 1e-8
+        out[th] = (cl.timing_get("gemm_split16")[1], cl.timing_get("gemm_split16_hi")[1],
+                   [c.iterations for c in conv], [c.refinements for c in conv])
+    print(out)
+    assert out[0.0][1] == 0 and out[0.0][0] > 0
+    assert out[1e-300][0] == 0 and out[1e-300][1] > 0
+    assert out[1e-3][1] > 0
+    # the default relaxation leaves the iteration and refinement counts unchanged
+    assert out[1e-3][2:] == out[0.0][2:]
+    cl.set_inner_relaxation(1e-3)
+    with pytest.raises(ValueError):
+        cl.set_inner_relaxation(-1.0)
+
+
 def test_solve_mixed_matrix_free_c64(pkg, oracle):
     cl, ref = _cluster(pkg, oracle, N=12, lmax=4, seed=8, side=3.5)
     a = cl.plane_wave([0, 0, 1], 1)

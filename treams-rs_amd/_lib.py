@@ -21,7 +21,7 @@ TG_OUT_OF_MEMORY = 6
 TG_DEVICE_ERROR = 7
 
 TG_STORE_NONE, TG_STORE_C64, TG_STORE_C128 = 0, 1, 2
-TG_OP_MATRIX_FREE_F64, TG_OP_STORED, TG_OP_MATRIX_FREE_C64 = 0, 1, 2
+TG_OP_MATRIX_FREE_F64, TG_OP_STORED, TG_OP_MATRIX_FREE_C64, TG_OP_MATRIX_FREE_C64_HI = 0, 1, 2, 3
 TG_PC_NONE, TG_PC_BLOCK_JACOBI, TG_PC_LEARNED = 0, 1, 2
 
 #: every symbol include/treams_gpu.h declares
@@ -32,7 +32,8 @@ EXPORTED = [
     "tg_cluster_local", "tg_cluster_overlap", "tg_cluster_plane_wave", "tg_cluster_solve", "tg_cluster_cross_sections",
     "tg_cluster_pullback", "tg_cluster_overlap_pullback", "tg_cluster_timing_enable",
     "tg_cluster_timing_get", "tg_cluster_timing_reset", "tg_cluster_set_group_size",
-    "tg_cluster_set_table_limit", "tg_cluster_set_inner_gemm", "tg_cluster_near_pairs",
+    "tg_cluster_set_table_limit", "tg_cluster_set_inner_gemm", "tg_cluster_set_inner_relaxation",
+    "tg_cluster_near_pairs",
     "tg_cluster_pair_list", "tg_cluster_pair_features", "tg_cluster_set_pair_output", "tg_cluster_learned_apply",
     "tg_cluster_learned_loss", "tg_cluster_set_mlp", "tg_mlp_create", "tg_mlp_destroy", "tg_mlp_param_count",
     "tg_mlp_get_params", "tg_mlp_set_params", "tg_mlp_get_grads", "tg_mlp_zero_grad", "tg_mlp_forward",
@@ -104,6 +105,7 @@ def lib():
         "tg_cluster_reset": (_i, [_vp]),
         "tg_cluster_set_table_limit": (_i, [_vp, _i64]),
         "tg_cluster_set_inner_gemm": (_i, [_vp, _i]),
+        "tg_cluster_set_inner_relaxation": (_i, [_vp, _d]),
         "tg_cluster_apply": (_i, [_vp, _vp, _vp, _i, _i, _i]),
         "tg_cluster_coupling": (_i, [_vp, _vp, _vp, _i, _i, _i]),
         "tg_cluster_local": (_i, [_vp, _vp, _vp, _i, _i]),

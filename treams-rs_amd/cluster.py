@@ -166,6 +166,14 @@ class IterativeSphereCluster:
             raise ValueError(f"inner GEMM kind must be one of {sorted(kinds)}, got {kind!r}")
         check(lib().tg_cluster_set_inner_gemm(self._h, kinds[kind]))
 
+    def set_inner_relaxation(self, threshold: float) -> None:
+        """Relaxed inner applies of mixed-precision solves (inexact Krylov): an inner
+        Arnoldi step whose allowed relative operator error (inner tolerance over the
+        step's residual estimate) is at least `threshold` multiplies the FP16 hi halves
+        only.  0 switches it off; default 1e-3.  The FP64 residual certifies the answer
+        either way."""
+        check(lib().tg_cluster_set_inner_relaxation(self._h, float(threshold)))
+
     def release(self) -> None:
         """Free the stored coupling, the pair tables and the preconditioner (rebuilt on
         demand)."""
@@ -184,9 +192,14 @@ class IterativeSphereCluster:
         return _back(out, vector)
 
     @staticmethod
-    def _op(stored: bool, c64: bool) -> int:
-        """tg_operator: FP64 matrix-free, stored coupling, or matrix-free c64 (FP32 MFMA)."""
-        return _lib.TG_OP_STORED if stored else (_lib.TG_OP_MATRIX_FREE_C64 if c64 else _lib.TG_OP_MATRIX_FREE_F64)
+    def _op(stored: bool, c64) -> int:
+        """tg_operator: FP64 matrix-free, stored coupling, or matrix-free c64 (c64=True;
+        c64="hi" the relaxed arithmetic of split-FP16 tables, hi halves only)."""
+        if stored:
+            return _lib.TG_OP_STORED
+        if c64 == "hi":
+            return _lib.TG_OP_MATRIX_FREE_C64_HI
+        return _lib.TG_OP_MATRIX_FREE_C64 if c64 else _lib.TG_OP_MATRIX_FREE_F64
 
     def apply(self, x, adjoint: bool = False, stored: bool = False, c64: bool = False):
         """(I - T C) x or (I - C^H T^H) x (cluster/iterative.rs:225-232)."""

@@ -365,7 +365,8 @@ int apply_operator(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, int
     cd *t2 = t1 + count;
     auto coupling = [&](const cd *in, cd *out) -> int {
         if (op == TG_OP_STORED) return coupling_stored(c, in, out, cols);
-        if (op == TG_OP_MATRIX_FREE_C64) return coupling_matrix_free32(c, in, out, cols);
+        if (op == TG_OP_MATRIX_FREE_C64 || op == TG_OP_MATRIX_FREE_C64_HI)
+            return coupling_matrix_free32(c, in, out, cols, op == TG_OP_MATRIX_FREE_C64_HI);
         return coupling_matrix_free(c, in, out, cols, 0);
     };
     if (!adjoint) {
@@ -791,6 +792,16 @@ int tg_cluster_reset(tg_cluster *c) {
     return TG_OK;
 }
 
+int tg_cluster_set_inner_relaxation(tg_cluster *c, double threshold) {
+    TG_TRY(check_cluster(c));
+    if (!(threshold >= 0.0) || !std::isfinite(threshold)) {
+        set_error("relaxation threshold must be finite and >= 0, got %g", threshold);
+        return TG_INVALID_INPUT;
+    }
+    c->relax_threshold = threshold;
+    return TG_OK;
+}
+
 int tg_cluster_set_inner_gemm(tg_cluster *c, int kind) {
     TG_TRY(check_cluster(c));
     if (kind != TG_INNER_FP32 && kind != TG_INNER_SPLIT16) {
@@ -866,7 +877,8 @@ int tg_cluster_coupling(tg_cluster *c, const tg_c128 *x, tg_c128 *y, int cols, i
     return with_split(c, x, y, cols, [&](const cd *xs, cd *ys) -> int {
         auto cp = [&](const cd *in, cd *out) -> int {
             if (op == TG_OP_STORED) return coupling_stored(c, in, out, cols);
-            if (op == TG_OP_MATRIX_FREE_C64) return coupling_matrix_free32(c, in, out, cols);
+            if (op == TG_OP_MATRIX_FREE_C64 || op == TG_OP_MATRIX_FREE_C64_HI)
+                return coupling_matrix_free32(c, in, out, cols, op == TG_OP_MATRIX_FREE_C64_HI);
             return coupling_matrix_free(c, in, out, cols, 0);
         };
         if (!adjoint) return cp(xs, ys);

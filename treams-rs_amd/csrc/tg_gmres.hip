@@ -67,6 +67,26 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
     std::vector<int> used(cols), extending(cols), cycle(cols);
     std::vector<cd> hbuf((size_t)cols * (restart + 1));
     std::vector<double> nxt(cols);
+    // Relaxed inner applies (inexact Krylov: Simoncini and Szyld, SIAM J. Sci. Comput.
+    // 25 (2003) 454; Bouras and Fraysse, SIAM J. Matrix Anal. Appl. 26 (2005) 660): once
+    // the residual estimate |g_j| of every extending column is small, step j's operator
+    // may err by tol / |g_j| relative without moving the attainable residual above tol.
+    // The bound goes to the cluster before each apply of an inner solve; the operator
+    // picks its arithmetic from it, and every other apply sees 0 (exact).
+    struct RelaxReset {
+        tg_cluster *c;
+        ~RelaxReset() { c->relax_allowed = 0.0; }
+    } relax_reset{c};
+    c->relax_allowed = 0.0;
+    auto relax_bound = [&](int j) {
+        double allowed = INFINITY;
+        for (int k = 0; k < cols; ++k) {
+            if (!extending[k]) continue;
+            const double gj = std::abs(g[(size_t)k * (restart + 1) + j]);
+            allowed = std::min(allowed, gj > 0.0 ? tol[k] / gj : INFINITY);
+        }
+        return std::isfinite(allowed) ? allowed : 0.0;
+    };
     while (true) {
         TG_TRY(vec_norms(c, R, n, cols, n, rn.data()));
         int any = 0;
@@ -107,6 +127,7 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
         for (int j = 0; j < restart; ++j) {
             // W = A M^{-1} V[j]
             const cd *Vj = V + (int64_t)j * vstep;
+            c->relax_allowed = prm.inner ? relax_bound(j) : 0.0;
             if (Minv) {
                 TG_TRY((*Minv)(Vj, R, cols));
                 TG_TRY(A(R, W, cols));
@@ -223,6 +244,7 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
             if (all || iterations >= prm.max_iterations) break;
         }
         // true residual R = b - A x
+        c->relax_allowed = 0.0;
         TG_TRY(A(x, R, cols));
         TG_TRY(vec_axpby(s, cd(1.0), b, cd(-1.0), R, (int64_t)cols * n));
     }

@@ -32,6 +32,10 @@ void set_error(const char *fmt, ...);
     } while (0)
 
 constexpr int kGemmCols = 64;  // most right-hand sides per matrix-free GEMM pass
+// smallest allowed relative operator error at which an inner split-FP16 apply drops the
+// lo halves (tg_cluster_set_inner_relaxation): about three times the hi-only coupling's
+// measured relative error (2.8e-4 to 3.0e-4)
+constexpr double kRelaxThresholdDefault = 1e-3;
 // Column-tile widths of the matrix-free GEMMs: 64 x 64 output tiles in FP32 (the
 // fewest loads per MFMA), 128 x 32 in FP64 (less column padding).
 constexpr int kTileWidths[2] = {64, 32};
@@ -137,6 +141,11 @@ struct tg_cluster {
     tg::DevBuf b_texpo;                // int16 exponents of the FP64 table entries, until the split tables exist
     bool have_texpo = false;
     int inner_gemm = TG_INNER_SPLIT16; // c64-precision GEMMs of the inner solves (tg_inner_gemm)
+    // relaxed inner applies: a split-FP16 apply multiplies the hi halves only when the
+    // error it may make (relax_allowed, relative, set per Arnoldi step by an inner
+    // GMRES) is at least relax_threshold; 0 switches relaxation off
+    double relax_threshold = tg::kRelaxThresholdDefault;
+    double relax_allowed = 0.0;
     // particle clusters: arbitrary per-particle T-matrices in place of the Mie blocks,
     // b_tloc[p][h'][a][h][b] = T_p[(h, b)][(h', a)] (per (p, h') a 2H x H column-major
     // panel; h = 0 the reference's polarisation 1)
@@ -216,11 +225,11 @@ int side_sync(tg_cluster *c);
 int coupling_tables_f64(tg_cluster *c, const cd *TAB, const cd *x, cd *y, int cols);
 // y = T' x (or T'^H x) with 2x2 blocks [sphere][degree][4] in place of the Mie blocks
 int local_apply_blocks(tg_cluster *c, const cd *blocks, const cd *x, cd *y, int cols, int adjoint, const cd *sub);
-int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols);
+int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols, bool hi_only = false);
 int coupling_stored(tg_cluster *c, const cd *x, cd *y, int cols);
 // split FP16 coupling (tg_split16.hip) from the FP64 singular tables tab64
 int ensure_split_tables(tg_cluster *c, const cd *tab64);
-int coupling_tables_split16(tg_cluster *c, const cd *tab64, const cd *x, cd *y, int cols);
+int coupling_tables_split16(tg_cluster *c, const cd *tab64, const cd *x, cd *y, int cols, bool hi_only = false);
 // the FP64 plan contraction of a compact c64 U for nc columns into y (tg_kernels.hip)
 int launch_tab_contract_c64(tg_cluster *c, const cf *U, int nc, cd *y);
 int local_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *subtract_from);

@@ -1339,12 +1339,15 @@ static int coupling_on_tables(tg_cluster *c, const cx<R> *TAB, const cd *x, cd *
 }
 
 // C x with the c64 tables on FP32 MFMA (accumulated in FP32, contracted in FP64): the
-// inner operator of mixed-precision solves without a stored coupling.
-int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols) {
+// inner operator of mixed-precision solves without a stored coupling.  On split-FP16
+// tables the hi halves alone are multiplied when hi_only is set or the current inner
+// step allows the error (relaxation).
+int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols, bool hi_only) {
     if (!tables_fit(c)) return coupling_matrix_free(c, x, y, cols, 0);
     if (c->inner_gemm == TG_INNER_SPLIT16) {
         TG_TRY(ensure_pair_tables(c, false, false));
-        return coupling_tables_split16(c, c->b_tab.as<cd>(), x, y, cols);
+        hi_only = hi_only || (c->relax_threshold > 0.0 && c->relax_allowed >= c->relax_threshold);
+        return coupling_tables_split16(c, c->b_tab.as<cd>(), x, y, cols, hi_only);
     }
     return coupling_tables<float>(c, x, y, cols, false);
 }

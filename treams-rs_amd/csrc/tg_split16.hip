@@ -158,6 +158,10 @@ __device__ __forceinline__ f32x4v mfma_h(h8v a, h8v b, f32x4v c) {
 // source tile, shared by the eight waves, goes through LDS (double-buffered, one
 // barrier per K step).  Tile order and the XCD remap as in the FP32 form, with
 // 64-wide column tiles (tiles[t]) and 256-row tiles.
+// LO = false multiplies the hi halves only (11 significant bits per value, four MFMAs
+// per step instead of twelve, half the table fragments): the relaxed applies of an
+// inner solve whose residual is already small (coupling_tables_split16).
+template <bool LO>
 __global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TABH, const _Float16 *XH,
                                                                   const int *E, const int *cshift, cf *U, int N,
                                                                   int Np, int H, int S, int G, int T_,
@@ -227,34 +231,47 @@ __global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TAB
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[p][a][b] = f32x4v{0.f, 0.f, 0.f, 0.f};
 
-    h8v ta[2][4], tb[2][4], xr[2];
+    // table fragments run PD K steps ahead in PD + 1 register sets: one step with the
+    // lo halves (the register budget of two waves per SIMD), two without them (a step
+    // then has a third of the MFMAs to cover the load latency with)
+    constexpr int PD = LO ? 1 : 2;
+    h8v tf[PD + 1][2][4], xr[2];
+    // without the lo halves the waves of components 1 and 3 copy nothing (wave-uniform)
+    const bool xcopy = LO || (xc & 1) == 0;
+    if (xcopy) {
 #pragma unroll
-    for (int u = 0; u < 2; ++u) xs[0][(xb0 + u) * 4 + xc][lane] = *reinterpret_cast<const h8v *>(sp[u]);
+        for (int u = 0; u < 2; ++u) xs[0][(xb0 + u) * 4 + xc][lane] = *reinterpret_cast<const h8v *>(sp[u]);
+    }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int p = 0; p < PD; ++p) {
+        const int jp = min(p, NB32 - 1);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) ta[a][c] = tp[a][c * 64];
-    // wait for the first step's loads here: left pending into the loop, they would make
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 4; c += LO ? 1 : 2) tf[p][a][c] = tp[a][(int64_t)jp * 256 + c * 64];
+    }
+    // wait for the first steps' loads here: left pending into the loop, they would make
     // every step wait for all its own prefetches before its first MFMA (vmcnt counts
     // loads in issue order)
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    // one K step on the table fragments `cur` while the next step's go to `nxt` (the
-    // two register sets alternate, so no copy waits for the prefetch before the
-    // barrier)
+    // one K step on the table fragments `cur` while step jb + PD's go to `nxt` (the
+    // register sets rotate, so no copy waits for the prefetch before the barrier)
     auto step = [&](int jb, h8v (&cur)[2][4], h8v (&nxt)[2][4]) {
         const int st = jb & 1;
-        // the next step's loads, issued on every step (the last one reloads its own
+        // the later steps' loads, issued on every step (the last ones reload the last
         // block, unused) so that every path has the same loads in flight and each wait
         // is exact; sources first: the LDS store at the end of the step then waits for
         // them only, not for the table prefetch behind them (loads complete in order)
-        const int jn = min(jb + 1, NB32 - 1);
+        const int jn = min(jb + 1, NB32 - 1), jt = min(jb + PD, NB32 - 1);
+        if (xcopy) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) xr[u] = *reinterpret_cast<const h8v *>(sp[u] + 32 * jn);
+            for (int u = 0; u < 2; ++u) xr[u] = *reinterpret_cast<const h8v *>(sp[u] + 32 * jn);
+        }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) nxt[a][c] = tp[a][(int64_t)jn * 256 + c * 64];
+            for (int c = 0; c < 4; c += LO ? 1 : 2) nxt[a][c] = tp[a][(int64_t)jt * 256 + c * 64];
         // acc0 += Xr Tr, acc1 += Xi Ti, acc2 += Xr Ti + Xi Tr (components 0, 1 = re hi,
         // lo; 2, 3 = im hi, lo), two column blocks at a time: consecutive MFMAs go to
         // four different accumulators, so none waits for the previous one's result
@@ -265,7 +282,7 @@ __global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TAB
 #pragma unroll
             for (int d = 0; d < 2; ++d)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) xb[d][c] = xs[st][(bp + d) * 4 + c][lane];
+                for (int c = 0; c < 4; c += LO ? 1 : 2) xb[d][c] = xs[st][(bp + d) * 4 + c][lane];
 #define TG_SPLIT_STEP(P, XC, TC)                                                         \
     _Pragma("unroll") for (int a = 0; a < 2; ++a)                                        \
         _Pragma("unroll") for (int d = 0; d < 2; ++d)                                    \
@@ -273,24 +290,31 @@ __global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TAB
             TG_SPLIT_STEP(0, 0, 0)
             TG_SPLIT_STEP(1, 2, 2)
             TG_SPLIT_STEP(2, 0, 2)
-            TG_SPLIT_STEP(0, 0, 1)
-            TG_SPLIT_STEP(1, 2, 3)
-            TG_SPLIT_STEP(2, 2, 0)
-            TG_SPLIT_STEP(0, 1, 0)
-            TG_SPLIT_STEP(1, 3, 2)
-            TG_SPLIT_STEP(2, 0, 3)
-            TG_SPLIT_STEP(2, 2, 1)
-            TG_SPLIT_STEP(2, 1, 2)
-            TG_SPLIT_STEP(2, 3, 0)
+            if (LO) {
+                TG_SPLIT_STEP(0, 0, 1)
+                TG_SPLIT_STEP(1, 2, 3)
+                TG_SPLIT_STEP(2, 2, 0)
+                TG_SPLIT_STEP(0, 1, 0)
+                TG_SPLIT_STEP(1, 3, 2)
+                TG_SPLIT_STEP(2, 0, 3)
+                TG_SPLIT_STEP(2, 2, 1)
+                TG_SPLIT_STEP(2, 1, 2)
+                TG_SPLIT_STEP(2, 3, 0)
+            } else {
+                TG_SPLIT_STEP(2, 2, 0)
+            }
 #undef TG_SPLIT_STEP
         }
+        if (xcopy) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) xs[st ^ 1][(xb0 + u) * 4 + xc][lane] = xr[u];
+            for (int u = 0; u < 2; ++u) xs[st ^ 1][(xb0 + u) * 4 + xc][lane] = xr[u];
+        }
         __syncthreads();
     };
-    for (int jb = 0; jb < NB32; jb += 2) {
-        step(jb, ta, tb);
-        if (jb + 1 < NB32) step(jb + 1, tb, ta);
+    for (int jb = 0; jb < NB32; jb += PD + 1) {
+        step(jb, tf[0], tf[PD]);
+        if (jb + 1 < NB32) step(jb + 1, tf[1], tf[(1 + PD) % (PD + 1)]);
+        if (PD == 2 && jb + 2 < NB32) step(jb + 2, tf[PD], tf[(2 + PD) % (PD + 1)]);
     }
     // D[m = column 4 (lane / 16) + v][n = row lane % 16]
 #pragma unroll
@@ -339,8 +363,8 @@ int ensure_split_tables(tg_cluster *c, const cd *tab64) {
 }
 
 // C x for `cols` columns on the split FP16 tables; U in c64, then the FP64 plan
-// contraction of the FP32 form.
-int coupling_tables_split16(tg_cluster *c, const cd *tab64, const cd *x, cd *y, int cols) {
+// contraction of the FP32 form.  hi_only multiplies the hi halves alone.
+int coupling_tables_split16(tg_cluster *c, const cd *tab64, const cd *x, cd *y, int cols, bool hi_only) {
     TG_TRY(ensure_split_tables(c, tab64));
     const int N = c->N, H = c->H, T = c->plan.T, S = c->plan.S, Np = (N + 31) & ~31;
     // right-hand sides per pass: the split sources and U of a pass within 512 MB
@@ -365,12 +389,18 @@ int coupling_tables_split16(tg_cluster *c, const cd *tab64, const cd *x, cd *y, 
         const int32_t *tiles = c->plan.stiles + (size_t)(nc - 1) * (T + 1);  // 64-wide column tiles
         const unsigned grid = (unsigned)c->gemm_ntiles[(size_t)nc - 1] * (unsigned)((N + 255) / 256);
         Timer::Rec grec;
-        TG_TRY(c->timer.begin("gemm_split16", c->stream, grec));
-        k_zgemm_subsets_split16<<<grid, 512, 0, c->stream>>>(c->b_tabh.as<h8v>(), XH, c->b_texp.as<int>(), cshift,
-                                                             U, N, Np, H, S, 2 * nc, T, tiles, c->plan.soff,
-                                                             c->plan.smode, c->plan.gpos);
+        const char *gname = hi_only ? "gemm_split16_hi" : "gemm_split16";
+        TG_TRY(c->timer.begin(gname, c->stream, grec));
+        if (hi_only)
+            k_zgemm_subsets_split16<false><<<grid, 512, 0, c->stream>>>(
+                c->b_tabh.as<h8v>(), XH, c->b_texp.as<int>(), cshift, U, N, Np, H, S, 2 * nc, T, tiles, c->plan.soff,
+                c->plan.smode, c->plan.gpos);
+        else
+            k_zgemm_subsets_split16<true><<<grid, 512, 0, c->stream>>>(
+                c->b_tabh.as<h8v>(), XH, c->b_texp.as<int>(), cshift, U, N, Np, H, S, 2 * nc, T, tiles, c->plan.soff,
+                c->plan.smode, c->plan.gpos);
         TG_HIP(hipGetLastError());
-        TG_TRY(c->timer.end("gemm_split16", c->stream, grec));
+        TG_TRY(c->timer.end(gname, c->stream, grec));
         TG_TRY(launch_tab_contract_c64(c, U, nc, y + (int64_t)c0 * c->n));
     }
     TG_TRY(c->timer.end("coupling_mf32", c->stream, rec));
