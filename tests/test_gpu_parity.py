@@ -150,8 +150,8 @@ def test_inner_gemm_split16_vs_fp32(pkg, N, lmax, cols):
         errs[kind, "coupling"] = _rel(cl.coupling(x, c64=True), cl.coupling(x))
         assert errs[kind, "coupling"] <= 1e-5, (kind, errs[kind, "coupling"])
     print(N, lmax, {k: f"{v:.2e}" for k, v in errs.items()})
-    # split tables rebuilt from the kept FP64 tables (row maxima read from them, not from
-    # the exponents written with them): the same tables, the same result bit for bit
+    # split tables rebuilt (row exponents and fragments again from the recurrences): the
+    # same tables, the same result bit for bit
     cl.set_inner_gemm("split16")
     assert np.array_equal(cl.apply(x, c64=True).cpu().numpy(), first)
 

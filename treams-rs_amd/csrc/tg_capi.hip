@@ -532,6 +532,7 @@ int tg_cluster_destroy(tg_cluster *c) {
     if (c->side) {
         (void)hipStreamDestroy(c->side);
         (void)hipEventDestroy(c->side_start);
+        (void)hipEventDestroy(c->side_split);
         (void)hipEventDestroy(c->side_done);
         c->side = nullptr;
     }
@@ -547,8 +548,6 @@ int tg_cluster_destroy(tg_cluster *c) {
     c->b_tabh.release();
     c->b_texp.release();
     c->have_tablesh = false;
-    c->b_texpo.release();
-    c->have_texpo = false;
     c->b_tabR.release();
     c->have_tablesR = false;
     c->b_tloc.release();
@@ -766,8 +765,6 @@ int tg_cluster_release(tg_cluster *c) {
     c->b_tabh.release();
     c->b_texp.release();
     c->have_tablesh = false;
-    c->b_texpo.release();
-    c->have_texpo = false;
     c->b_tabR.release();
     c->have_tablesR = false;
     c->storage = TG_STORE_NONE;
@@ -787,7 +784,6 @@ int tg_cluster_reset(tg_cluster *c) {
     c->have_tables = false;
     c->have_tables32 = false;
     c->have_tablesh = false;
-    c->have_texpo = false;
     c->have_tablesR = false;
     return TG_OK;
 }
@@ -816,8 +812,6 @@ int tg_cluster_set_inner_gemm(tg_cluster *c, int kind) {
     c->b_tabh.release();
     c->b_texp.release();
     c->have_tablesh = false;
-    c->b_texpo.release();
-    c->have_texpo = false;
     return TG_OK;
 }
 
@@ -837,8 +831,6 @@ int tg_cluster_set_table_limit(tg_cluster *c, int64_t bytes) {
     c->b_tabh.release();
     c->b_texp.release();
     c->have_tablesh = false;
-    c->b_texpo.release();
-    c->have_texpo = false;
     c->b_tabR.release();
     c->have_tablesR = false;
     return TG_OK;

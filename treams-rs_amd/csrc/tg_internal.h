@@ -138,8 +138,6 @@ struct tg_cluster {
     bool have_tables32 = false;
     tg::DevBuf b_tabh, b_texp;         // split FP16 tables and their row exponents (FP16 MFMA coupling)
     bool have_tablesh = false;
-    tg::DevBuf b_texpo;                // int16 exponents of the FP64 table entries, until the split tables exist
-    bool have_texpo = false;
     int inner_gemm = TG_INNER_SPLIT16; // c64-precision GEMMs of the inner solves (tg_inner_gemm)
     // relaxed inner applies: a split-FP16 apply multiplies the hi halves only when the
     // error it may make (relax_allowed, relative, set per Arnoldi step by an inner
@@ -160,8 +158,9 @@ struct tg_cluster {
     // side stream that builds the inner coupling's tables while the preconditioner is
     // set up; table users wait for side_done while tables_pending
     hipStream_t side = nullptr;
-    hipEvent_t side_start = nullptr, side_done = nullptr;
+    hipEvent_t side_start = nullptr, side_split = nullptr, side_done = nullptr;
     bool tables_pending = false;
+    bool split_pending = false;        // the split FP16 tables (side_split) are not waited for yet
     tg::Timer timer;
     tg::Preconditioner *pc = nullptr;
     int pc_kind = TG_PC_NONE;
@@ -228,8 +227,11 @@ int local_apply_blocks(tg_cluster *c, const cd *blocks, const cd *x, cd *y, int 
 int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols, bool hi_only = false);
 int coupling_stored(tg_cluster *c, const cd *x, cd *y, int cols);
 // split FP16 coupling (tg_split16.hip) from the FP64 singular tables tab64
-int ensure_split_tables(tg_cluster *c, const cd *tab64);
-int coupling_tables_split16(tg_cluster *c, const cd *tab64, const cd *x, cd *y, int cols, bool hi_only = false);
+int ensure_split_tables(tg_cluster *c);
+int coupling_tables_split16(tg_cluster *c, const cd *x, cd *y, int cols, bool hi_only = false);
+// the cluster's stream waits for the split tables the side stream builds (tables_wait
+// waits for everything it builds)
+int split_wait(tg_cluster *c);
 // the FP64 plan contraction of a compact c64 U for nc columns into y (tg_kernels.hip)
 int launch_tab_contract_c64(tg_cluster *c, const cf *U, int nc, cd *y);
 int local_apply(tg_cluster *c, const cd *x, cd *y, int cols, int adjoint, const cd *subtract_from);

@@ -8,25 +8,12 @@
 #include <algorithm>
 #include "tg_internal.h"
 #include "tg_mfma.h"
+#include "tg_pairtab.h"
 #include "tg_special.h"
 
 namespace tg {
 
 // ---------------------------------------------------------------- geometry --
-
-// Length and unit direction of d, scaling by the largest component first;
-// returns 0 where every component is below DBL_MIN (the origin).
-__device__ __forceinline__ int unit_direction(const double d[3], double &r, double u[3]) {
-    double s = fmax(fabs(d[0]), fmax(fabs(d[1]), fabs(d[2])));
-    if (s < 2.2250738585072014e-308) return 0;
-    double a = d[0] / s, b = d[1] / s, c = d[2] / s;
-    double nrm = sqrt(a * a + b * b + c * c);
-    r = s * nrm;
-    u[0] = a / nrm;
-    u[1] = b / nrm;
-    u[2] = c / nrm;
-    return 1;
-}
 
 // Radial values z_p(kr), p = 0..L2: outgoing Hankel or regular Bessel.
 __device__ void radial_values(int L2, double kr, bool regular, cd *rad) {
@@ -764,49 +751,11 @@ static int grid1(int64_t count, int block = 256) { return (int)((count + block -
 // z_p = h_p (singular coupling) or j_p (REG: regular overlap, whose self blocks are the
 // identity: the harmonic table at the origin is 1 at p = q = 0).  One thread per
 // (i, j) with i fastest, so each table entry is a coalesced store across the wave.
-// The singular table h_p(x) P_p^q e^{i q phi}: the table_column recurrences with the
-// Hankel upward recurrence run alongside each order q instead of from an array of
-// h_p (an array indexed by a run-time degree lives in scratch memory).  Same
-// operations in the same order, so the same values as table_column.
-template <class Put>
-__device__ __forceinline__ void singular_table_put(int L2, double x, const double u[3], const double *refl,
-                                                   Put put) {
-    const double inv = 1.0 / x, sx = sin(x), cx_ = cos(x);
-    const cd xy(u[0], u[1]);
-    cd ladder(1.0);
-    for (int q = 0; q <= L2; ++q) {
-        if (q > 0) ladder = (-(double)(2 * q - 1)) * (ladder * xy);
-        cd hprev(cx_ * inv, sx * inv), hcur(sx * inv, -cx_ * inv);  // h_{-1}, h_0
-        cd cur = ladder, prev(0.0);
-        for (int p = 0; p <= L2; ++p) {
-            if (p > q) {
-                const double a = (double)(2 * p - 1) / (double)(p - q);
-                const double b = (double)(p + q - 1) / (double)(p - q);
-                const cd next = (a * u[2]) * cur - b * prev;
-                prev = cur;
-                cur = next;
-            }
-            if (p >= q) {
-                put(p * p + p + q, hcur * cur);
-                if (q > 0) put(p * p + p - q, hcur * (refl[p * p + p - q] * conj(cur)));
-            }
-            const cd hnext = ((double)(2 * p + 1) * inv) * hcur - hprev;
-            hprev = hcur;
-            hcur = hnext;
-        }
-    }
-}
-
-// exponent e with max(|re|, |im|) < 2^e (frexp), or -32768 for zero
-__device__ __forceinline__ int16_t entry_exponent(cd v) {
-    const double m = fmax(fabs(v.re), fabs(v.im));
-    int e = 0;
-    (void)frexp(m, &e);
-    return m > 0.0 ? (int16_t)e : (int16_t)-32768;
-}
-
+// The singular table h_p(x) P_p^q e^{i q phi} comes from singular_table_put
+// (tg_pairtab.h): the table_column recurrences with the Hankel upward recurrence run
+// alongside each order q, the same operations in the same order, so the same values.
 template <bool REG>
-__global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd *tab, cf *tab32, int16_t *expo) {
+__global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd *tab, cf *tab32) {
     const int L2 = pl.L2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)N * N) return;
@@ -814,11 +763,9 @@ __global__ void k_pair_tables(DevPlan pl, double k, const double *pos, int N, cd
     const int64_t NN = (int64_t)N * N;
     const int64_t o = (int64_t)j * N + i;
     // the FP64 entry and, when asked for, its c64 rounding (the FP32 MFMA tables)
-    // and, when asked for, the entry's exponent (row maxima of the split FP16 tables)
     auto put = [&](int h, cd v) {
         stream_store(tab + (int64_t)h * NN + o, v);
         if (tab32) stream_store(tab32 + (int64_t)h * NN + o, cf((float)v.re, (float)v.im));
-        if (expo) expo[(int64_t)h * NN + o] = entry_exponent(v);
     };
     double d[3], r, u[3];
     for (int a = 0; a < 3; ++a) d[a] = pos[3 * i + a] - pos[3 * j + a];
@@ -1037,23 +984,16 @@ static int ensure_pair_tables(tg_cluster *c, bool regular, bool c64) {
     const int64_t NN = (int64_t)c->N * c->N, count = NN * c->plan.T;
     if (need32) TG_TRY(c->b_tab32.ensure(sizeof(cf) * (size_t)count));
     cf *t32 = need32 ? c->b_tab32.as<cf>() : nullptr;
-    // the entries' exponents too when split FP16 tables will be built from these
-    int16_t *expo = nullptr;
-    if (!regular && !have && c->inner_gemm == TG_INNER_SPLIT16 && !c->have_tablesh) {
-        TG_TRY(c->b_texpo.ensure(sizeof(int16_t) * (size_t)count));
-        expo = c->b_texpo.as<int16_t>();
-        c->have_texpo = true;
-    }
     if (have) {
         k_tables_to_c64<<<grid1(count), 256, 0, c->stream>>>(buf.as<cd>(), count, t32);
     } else {
         TG_TRY(buf.ensure(sizeof(cd) * (size_t)count));
         if (regular)
             k_pair_tables<true><<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N,
-                                                                  buf.as<cd>(), nullptr, nullptr);
+                                                                  buf.as<cd>(), nullptr);
         else
             k_pair_tables<false><<<grid1(NN), 256, 0, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), c->N,
-                                                                   buf.as<cd>(), t32, expo);
+                                                                   buf.as<cd>(), t32);
     }
     TG_HIP(hipGetLastError());
     have = true;
@@ -1062,39 +1002,56 @@ static int ensure_pair_tables(tg_cluster *c, bool regular, bool c64) {
 }
 
 int tables_wait(tg_cluster *c) {
+    c->split_pending = false;
     if (!c->tables_pending) return TG_OK;
     c->tables_pending = false;
     TG_HIP(hipStreamWaitEvent(c->stream, c->side_done, 0));
     return TG_OK;
 }
 
+int split_wait(tg_cluster *c) {
+    if (!c->split_pending) return TG_OK;
+    c->split_pending = false;
+    TG_HIP(hipStreamWaitEvent(c->stream, c->side_split, 0));
+    return TG_OK;
+}
+
 int side_sync(tg_cluster *c) {
     c->tables_pending = false;
+    c->split_pending = false;
     if (c->side) TG_HIP(hipStreamSynchronize(c->side));
     return TG_OK;
 }
 
 // The inner coupling's tables are built on the side stream, after the work already on
 // the cluster's stream, while the cluster's stream sets up the preconditioner (its
-// kernels and host round trips); the first table user waits for them.
+// kernels and host round trips): the split FP16 tables first, straight from the
+// recurrences (the first inner apply waits for them only), then the FP64 tables of the
+// refinement residuals (their first user waits for them).
 int prefetch_inner_tables(tg_cluster *c) {
     if (c->tables_pending || c->have_tablesh || c->inner_gemm != TG_INNER_SPLIT16 || !tables_fit(c)) return TG_OK;
     if (!c->side) {
         TG_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         TG_HIP(hipEventCreateWithFlags(&c->side_start, hipEventDisableTiming));
+        TG_HIP(hipEventCreateWithFlags(&c->side_split, hipEventDisableTiming));
         TG_HIP(hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming));
     }
     TG_HIP(hipEventRecord(c->side_start, c->stream));
     TG_HIP(hipStreamWaitEvent(c->side, c->side_start, 0));
     hipStream_t main_stream = c->stream;
     c->stream = c->side;
-    int st = ensure_pair_tables(c, false, false);
-    if (!st) st = ensure_split_tables(c, c->b_tab.as<cd>());
+    int st = ensure_split_tables(c);
+    if (hipEventRecord(c->side_split, c->side) != hipSuccess && !st) {
+        set_error("hipEventRecord failed on the side stream");
+        st = TG_DEVICE_ERROR;
+    }
+    if (!st) st = ensure_pair_tables(c, false, false);
     c->stream = main_stream;
     if (hipEventRecord(c->side_done, c->side) != hipSuccess && !st) {
         set_error("hipEventRecord failed on the side stream");
         st = TG_DEVICE_ERROR;
     }
+    c->split_pending = true;
     c->tables_pending = true;  // even after an error: the cluster's stream still waits
     return st;
 }
@@ -1345,9 +1302,8 @@ static int coupling_on_tables(tg_cluster *c, const cx<R> *TAB, const cd *x, cd *
 int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols, bool hi_only) {
     if (!tables_fit(c)) return coupling_matrix_free(c, x, y, cols, 0);
     if (c->inner_gemm == TG_INNER_SPLIT16) {
-        TG_TRY(ensure_pair_tables(c, false, false));
         hi_only = hi_only || (c->relax_threshold > 0.0 && c->relax_allowed >= c->relax_threshold);
-        return coupling_tables_split16(c, c->b_tab.as<cd>(), x, y, cols, hi_only);
+        return coupling_tables_split16(c, x, y, cols, hi_only);
     }
     return coupling_tables<float>(c, x, y, cols, false);
 }

@@ -23,6 +23,7 @@
 //   sources XH[q][c][Np] halves with the column shift cshift[q].
 #include "tg_internal.h"
 #include "tg_mfma.h"
+#include "tg_pairtab.h"
 
 namespace tg {
 
@@ -45,73 +46,168 @@ __device__ __forceinline__ half2s split16(double v, int s) {
     return {hi, (_Float16)(float)(w - (double)(float)hi)};
 }
 
-__global__ void k_fill_int(int *p, int64_t count, int v) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < count) p[t] = v;
-}
-
-// E[t][i] = max over j in chunk blockIdx.z of the exponent of max(|re|, |im|) of
-// TAB[t][j][i]; grid (ceil(N / 256), T, chunks).
-__global__ __launch_bounds__(256) void k_split_rowexp(const cd *TAB, int N, int Np, int jchunk, int *E) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int t = blockIdx.y;
-    if (i >= N) return;
-    const int j0 = blockIdx.z * jchunk, j1 = min(N, j0 + jchunk);
-    const cd *p = TAB + ((int64_t)t * N) * N + i;
-    double m = 0.0;
-    for (int j = j0; j < j1; ++j) {
-        const cd v = p[(int64_t)j * N];
-        m = fmax(m, fmax(fabs(v.re), fabs(v.im)));
+// Row exponents from an upper bound of each table row instead of its maximum (no pass
+// over the entries): |h_p(k d)| decreases with d (|h_p(x)|^2 is a sum of positive
+// powers of 1 / x), so over the sources j it is largest at the nearest other centre
+// d_min(i); and the addition theorem (sum over q of |Y_p^q|^2 = (2p + 1) / (4 pi))
+// bounds the unnormalised harmonic by |P_p^q| <= sqrt((p + q)! / (p - q)!), and its
+// negative order (the reflection factor (p - q)! / (p + q)! times the conjugate) by
+// the reciprocal square root.  E[t][i] is the exponent of that bound (times 1 + 1e-10
+// against rounding in the entries), kNoExp where no other centre is apart from i
+// (rows of zeros) and for the padding rows i >= N.  One wave per row.
+__global__ __launch_bounds__(64) void k_split_rowexp_bound(int L2, double k, const double *pos, int N, int Np,
+                                                            int *E) {
+    // one wave per row: the lanes take every 64th source, then a wave minimum
+    const int i = blockIdx.x, lane = threadIdx.x;
+    const int T = (L2 + 1) * (L2 + 1);
+    double dmin = INFINITY;
+    if (i < N) {
+        const double p0 = pos[3 * i], p1 = pos[3 * i + 1], p2 = pos[3 * i + 2];
+        for (int j = lane; j < N; j += 64) {
+            double d[3] = {p0 - pos[3 * j], p1 - pos[3 * j + 1], p2 - pos[3 * j + 2]};
+            double r, u[3];
+            if (j != i && unit_direction(d, r, u)) dmin = fmin(dmin, r);
+        }
     }
-    const int e = mag_exp(m);
-    if (e != kNoExp) atomicMax(E + (int64_t)t * Np + i, e);
-}
-
-// The same row maxima from the int16 exponents of the table entries (written with the
-// FP64 tables): an eighth of the bytes.
-__global__ __launch_bounds__(256) void k_split_rowexp16(const int16_t *ex, int N, int Np, int jchunk, int *E) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int t = blockIdx.y;
-    if (i >= N) return;
-    const int j0 = blockIdx.z * jchunk, j1 = min(N, j0 + jchunk);
-    const int16_t *p = ex + ((int64_t)t * N) * N + i;
-    int m = -32768;
-#pragma unroll 8
-    for (int j = j0; j < j1; ++j) m = max(m, (int)p[(int64_t)j * N]);
-    if (m != -32768) atomicMax(E + (int64_t)t * Np + i, m);
-}
-
-// TABH from the FP64 tables: one thread per (t, i block, j block, lane).
-__global__ __launch_bounds__(256) void k_split_tables(const cd *TAB, const int *E, int N, int Np, int T,
-                                                      h8v *TABH) {
-    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int NB16 = Np / 16, NB32 = Np / 32;
-    const int64_t nfrag = (int64_t)T * NB16 * NB32;
-    const int64_t frag = gid >> 6;
-    if (frag >= nfrag) return;
-    const int lane = (int)(gid & 63);
-    const int jb = (int)(frag % NB32);
-    const int64_t rest = frag / NB32;
-    const int ib = (int)(rest % NB16), t = (int)(rest / NB16);
-    const int i = ib * 16 + (lane & 15), j0 = jb * 32 + 8 * (lane >> 4);
-    const int s = i < N ? shift_of(E[(int64_t)t * Np + i]) : 0;
-    h8v rh, rl, ih, il;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int j = j0 + e;
-        cd v(0.0);
-        if (i < N && j < N) v = TAB[((int64_t)t * N + j) * N + i];
-        const half2s pr = split16(v.re, s), pi = split16(v.im, s);
-        rh[e] = pr.hi;
-        rl[e] = pr.lo;
-        ih[e] = pi.hi;
-        il[e] = pi.lo;
+    for (int o = 32; o > 0; o >>= 1) dmin = fmin(dmin, __shfl_xor(dmin, o));
+    if (!(dmin < INFINITY)) {
+        for (int t = lane; t < T; t += 64) E[(int64_t)t * Np + i] = kNoExp;
+        return;
     }
-    h8v *out = TABH + frag * 256 + lane;
-    out[0] = rh;
-    out[64] = rl;
-    out[128] = ih;
-    out[192] = il;
+    // every lane runs the Hankel recurrence up to the degrees of its harmonics
+    const double x = k * dmin, inv = 1.0 / x;
+    cd hprev(cos(x) * inv, sin(x) * inv), hcur(sin(x) * inv, -cos(x) * inv);  // h_{-1}, h_0
+    int t = lane;
+    for (int p = 0; p <= L2 && t < T; ++p) {
+        const double hm = cabs_(hcur) * (1.0 + 1e-10);
+        for (; t < T && t <= p * p + 2 * p; t += 64) {  // this lane's harmonics of degree p
+            const int q = t - p * p - p, aq = q < 0 ? -q : q;
+            double fr = 1.0;  // (p + |q|)! / (p - |q|)!
+            for (int m = 1; m <= aq; ++m) fr *= (double)(p + m) * (double)(p - m + 1);
+            E[(int64_t)t * Np + i] = mag_exp(q >= 0 ? hm * sqrt(fr) : hm / sqrt(fr));
+        }
+        const cd hnext = hankel_next((double)(2 * p + 1) * inv, hcur, hprev);
+        hprev = hcur;
+        hcur = hnext;
+    }
+}
+
+// The FP64 pair tables TAB[t][j][i] (the matrix-free FP64 coupling's operands) and their
+// split FP16 fragments TABH in one pass over the recurrences.  One workgroup per 16
+// rows i x 16 sources j (half of each fragment's 32 sources: lanes 0..31 or 32..63 of
+// every fragment, 512 contiguous bytes per component); thread (ii = tid % 16, jj =
+// tid / 16) evaluates its pair one order q at a time with the operations of
+// singular_column_put (the same values as the FP64 tables of k_pair_tables), but with
+// h_0 .. h_L2 computed once into registers (LB2 >= L2 bounds the unrolled degree
+// loop) and the degree-recurrence coefficients read from LDS.  Each value is stored in
+// FP64 (16 consecutive rows per source: 256-byte runs) when write64, and split with
+// its row's shift into LDS in fragment order; after each order the workgroup writes
+// the order's fragments (up to 2 L2 harmonics x 4 components x 512 bytes) with 16-byte
+// stores.  Pairs with i or j beyond N give zero fragments (and no FP64 entries), self
+// pairs and coincident centres zeros.  Dynamic LDS: the staging [nstage][4][256]
+// halves, the row shifts [T][16] (int16) and the coefficients [T] (a, b).
+template <int LB2>
+__global__ __launch_bounds__(256) void k_pair_tables_split(DevPlan pl, double k, const double *pos, int N, int Np,
+                                                           const int *E, cd *tab, h8v *TABH, int write64) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int T = pl.T, L2 = pl.L2;
+    const int nstage = 2 * L2 > L2 + 1 ? 2 * L2 : L2 + 1;  // most harmonics of one order
+    _Float16 *stage = reinterpret_cast<_Float16 *>(lds);     // [nstage][4][256]
+    double2 *coef = reinterpret_cast<double2 *>(lds + (size_t)nstage * 4 * 256 * sizeof(_Float16));  // [T]
+    int16_t *shift = reinterpret_cast<int16_t *>(coef + T);  // [T][16]
+    const int tid = threadIdx.x, ii = tid & 15, jj = tid >> 4;
+    const int j16 = blockIdx.x, ib = blockIdx.y;
+    const int i = ib * 16 + ii, j = j16 * 16 + jj;
+    const int NB16 = Np / 16, NB32 = Np / 32, jb = j16 >> 1, jhalf = j16 & 1;
+    const int64_t NN = (int64_t)N * N;
+    for (int e = tid; e < T * 16; e += 256) {
+        const int t = e >> 4, row = ib * 16 + (e & 15);
+        shift[e] = (int16_t)(row < N ? shift_of(E[(int64_t)t * Np + row]) : 0);
+    }
+    for (int h = tid; h < T; h += 256) {
+        const int p = (int)sqrt((double)h + 0.5), q = h - p * p - p;
+        double2 ab = make_double2(0.0, 0.0);
+        if (q >= 0 && p > q) ab = make_double2((double)(2 * p - 1) / (double)(p - q),
+                                               (double)(p + q - 1) / (double)(p - q));
+        coef[h] = ab;
+    }
+    // this thread's place in the fragments: lane (ii + 16 (jj / 8)) of the half, element
+    // jj % 8
+    const int slot = (ii + 16 * (jj >> 3)) * 8 + (jj & 7);
+    const bool valid = i < N && j < N;  // an entry of the FP64 tables
+    double d[3], r = 1.0, u[3] = {0.0, 0.0, 1.0};
+    bool live = valid && i != j;
+    if (live) {
+        for (int a = 0; a < 3; ++a) d[a] = pos[3 * i + a] - pos[3 * j + a];
+        live = unit_direction(d, r, u);
+    }
+    cd hv[LB2 + 1];
+    cd xy(u[0], u[1]), ladder(1.0);
+    {
+        const double x = k * r, inv = 1.0 / x, sx = sin(x), cx_ = cos(x);
+        cd hprev(cx_ * inv, sx * inv), hcur(sx * inv, -cx_ * inv);  // h_{-1}, h_0
+#pragma clang loop unroll(full)
+        for (int p = 0; p <= LB2; ++p) {
+            hv[p] = hcur;
+            const cd hnext = hankel_next((double)(2 * p + 1) * inv, hcur, hprev);
+            hprev = hcur;
+            hcur = hnext;
+        }
+    }
+    cd *tij = tab + (int64_t)j * N + i;
+    __syncthreads();
+    for (int q = 0; q <= L2; ++q) {
+        // (p, +q) and then (p, -q) for p = q, q + 1, ...: the staging index is the
+        // call count
+        int s = 0;
+        auto put = [&](int h, cd v) {
+            if (write64 && valid) {
+                typedef double d2v __attribute__((ext_vector_type(2)));
+                const d2v x = {v.re, v.im};
+                __builtin_nontemporal_store(x, reinterpret_cast<d2v *>(tij + (int64_t)h * NN));
+            }
+            const int sh = shift[h * 16 + ii];
+            const half2s pr = split16(v.re, sh), pi = split16(v.im, sh);
+            _Float16 *o = stage + (size_t)s * 1024 + slot;
+            o[0] = pr.hi;
+            o[256] = pr.lo;
+            o[512] = pi.hi;
+            o[768] = pi.lo;
+            ++s;
+        };
+        if (q > 0) ladder = (-(double)(2 * q - 1)) * cmul_fma(ladder, xy);
+        cd cur = ladder, prev(0.0);
+        // fully unrolled, so hv[p] is a register, never scratch
+#pragma clang loop unroll(full)
+        for (int p = 0; p <= LB2; ++p) {
+            if (p > L2 || p < q) continue;
+            if (p > q) {
+                const double2 ab = coef[p * p + p + q];
+                const cd next = legendre_next(ab.x * u[2], ab.y, cur, prev);
+                prev = cur;
+                cur = next;
+            }
+            if (live) {
+                put(p * p + p + q, cmul_fma(hv[p], cur));
+                if (q > 0) put(p * p + p - q, cmul_fma(hv[p], reflect(pl.refl[p * p + p - q], cur)));
+            } else {
+                put(p * p + p + q, cd(0.0));
+                if (q > 0) put(p * p + p - q, cd(0.0));
+            }
+        }
+        __syncthreads();
+        // write the order's staged fragments: chunk = (staging index, component), 32
+        // lanes of 16 bytes each
+        const int chunks = (L2 + 1 - q) * (q ? 2 : 1) * 4;
+        const h8v *sv = reinterpret_cast<const h8v *>(stage);
+        for (int w = tid; w < chunks * 32; w += 256) {
+            const int ch = w >> 5, l32 = w & 31, sidx = ch >> 2, comp = ch & 3;
+            const int p = q + (q ? sidx >> 1 : sidx);
+            const int h = p * p + p + ((q && (sidx & 1)) ? -q : q);
+            TABH[(((int64_t)h * NB16 + ib) * NB32 + jb) * 256 + comp * 64 + jhalf * 32 + l32] = sv[ch * 32 + l32];
+        }
+        __syncthreads();
+    }
 }
 
 // XH[q][c][j] and cshift[q] for q = (column cc, helicity h, mode a) of x (split
@@ -335,37 +431,47 @@ __global__ __launch_bounds__(512, 1) void k_zgemm_subsets_split16(const h8v *TAB
     }
 }
 
-// Split tables of the singular FP64 pair tables (built first when missing).
-int ensure_split_tables(tg_cluster *c, const cd *tab64) {
-    TG_TRY(tables_wait(c));
+// Split tables (and the FP64 tables when they are missing) in one pass over the
+// recurrences, with row exponents from the bound of k_split_rowexp_bound.
+int ensure_split_tables(tg_cluster *c) {
+    TG_TRY(split_wait(c));
     if (c->have_tablesh) return TG_OK;
-    const int N = c->N, T = c->plan.T, Np = (N + 31) & ~31;
+    const int N = c->N, T = c->plan.T, L2 = c->plan.L2, Np = (N + 31) & ~31;
+    const bool write64 = !c->have_tables;
+    if (write64) {
+        TG_TRY(tables_wait(c));
+        TG_TRY(c->b_tab.ensure(sizeof(cd) * (size_t)N * N * T));
+    }
     TG_TRY(c->b_texp.ensure(sizeof(int) * (size_t)T * Np));
     TG_TRY(c->b_tabh.ensure((size_t)T * Np * Np * 8));
     int *E = c->b_texp.as<int>();
-    const int64_t ne = (int64_t)T * Np;
-    k_fill_int<<<(unsigned)((ne + 255) / 256), 256, 0, c->stream>>>(E, ne, kNoExp);
-    const int jchunk = 128;
-    const dim3 grid((unsigned)((N + 255) / 256), (unsigned)T, (unsigned)((N + jchunk - 1) / jchunk));
-    if (c->have_texpo)
-        k_split_rowexp16<<<grid, 256, 0, c->stream>>>(c->b_texpo.as<int16_t>(), N, Np, jchunk, E);
-    else
-        k_split_rowexp<<<grid, 256, 0, c->stream>>>(tab64, N, Np, jchunk, E);
+    const int nstage = std::max(2 * L2, L2 + 1);
+    const size_t lds = sizeof(_Float16) * (size_t)nstage * 4 * 256 + sizeof(double2) * (size_t)T +
+                       sizeof(int16_t) * (size_t)T * 16;
+    if (lds > 160 * 1024) {
+        set_error("split tables: lmax %d needs more LDS than a workgroup has", c->plan.lmax);
+        return TG_INVALID_INPUT;
+    }
+    k_split_rowexp_bound<<<(unsigned)Np, 64, 0, c->stream>>>(L2, c->k0, c->b_pos.as<double>(), N, Np, E);
     TG_HIP(hipGetLastError());
-    const int64_t threads = (int64_t)T * (Np / 16) * (Np / 32) * 64;
-    k_split_tables<<<(unsigned)((threads + 255) / 256), 256, 0, c->stream>>>(tab64, E, N, Np, T,
-                                                                             c->b_tabh.as<h8v>());
+    const dim3 grid((unsigned)(Np / 16), (unsigned)(Np / 16));
+#define TG_TABLES_SPLIT(LB)                                                                                     \
+    k_pair_tables_split<LB><<<grid, 256, lds, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), N, Np, E,    \
+                                                          c->b_tab.as<cd>(), c->b_tabh.as<h8v>(), write64 ? 1 : 0)
+    if (L2 <= 6) TG_TABLES_SPLIT(6);
+    else if (L2 <= 12) TG_TABLES_SPLIT(12);
+    else TG_TABLES_SPLIT(24);
+#undef TG_TABLES_SPLIT
     TG_HIP(hipGetLastError());
     c->have_tablesh = true;
-    c->b_texpo.release();
-    c->have_texpo = false;
+    if (write64) c->have_tables = true;
     return TG_OK;
 }
 
 // C x for `cols` columns on the split FP16 tables; U in c64, then the FP64 plan
 // contraction of the FP32 form.  hi_only multiplies the hi halves alone.
-int coupling_tables_split16(tg_cluster *c, const cd *tab64, const cd *x, cd *y, int cols, bool hi_only) {
-    TG_TRY(ensure_split_tables(c, tab64));
+int coupling_tables_split16(tg_cluster *c, const cd *x, cd *y, int cols, bool hi_only) {
+    TG_TRY(ensure_split_tables(c));
     const int N = c->N, H = c->H, T = c->plan.T, S = c->plan.S, Np = (N + 31) & ~31;
     // right-hand sides per pass: the split sources and U of a pass within 512 MB
     const size_t per_col = (size_t)2 * H * 4 * Np * sizeof(_Float16) + sizeof(cf) * (size_t)2 * S * N;
