@@ -28,48 +28,79 @@ void set_error(const char *fmt, ...) {
 }
 
 // Released device blocks are kept in a process-wide cache and handed to later requests
-// of the same size, else of at least half their size: hipFree synchronises the device and unmaps the pages,
-// which costs milliseconds per large block, and every solve allocates its scratch
-// again.  Work queued before a release may still use the block, so taking one from
-// the cache synchronises the device first.  Blocks above kCacheBlockMax are freed at
-// once, the cache holds at most kCacheBytes, it is emptied when an allocation fails
-// (and the allocation retried) and when the last cluster is destroyed.
+// of the same size, else of at least half their size: hipFree synchronises the device
+// and unmaps the pages, which costs milliseconds per large block, and every solve
+// allocates its scratch again.  Work queued before a release may still use the block,
+// so taking one from the cache synchronises the device first, unless the block was
+// released clean (after the streams that used it were synchronised: a cluster or
+// factor being destroyed).  Blocks above kCacheBlockMax are freed at once, the cache
+// holds at most kCacheBytes, it is emptied when an allocation fails (and the
+// allocation retried); when the last cluster is destroyed the blocks above
+// kKeepBlockMax are freed, and the small ones (up to kKeepBytes) stay for the next
+// cluster, so that creating and destroying small clusters allocates nothing.  Pinned
+// host blocks (HostBuf) are pooled the same way, up to kPinnedKeep blocks.
 namespace {
 constexpr size_t kCacheBytes = (size_t)32 << 30;
 constexpr size_t kCacheBlockMax = (size_t)8 << 30;
+constexpr size_t kKeepBlockMax = (size_t)256 << 20;
+constexpr size_t kKeepBytes = (size_t)1 << 30;
+constexpr size_t kPinnedKeep = 16;
 std::mutex g_cache_mu;
-std::multimap<size_t, void *> g_cache;  // block size -> block
+struct CachedBlock {
+    void *p;
+    bool clean;
+};
+std::multimap<size_t, CachedBlock> g_cache;  // block size -> block
 size_t g_cached = 0;
 int g_live_clusters = 0;
+std::multimap<size_t, void *> g_pinned;  // pinned host blocks by size
+thread_local bool t_clean_release = false;
 
-void *cache_take(size_t need, size_t &got) {
+void *cache_take(size_t need, size_t &got, bool &clean) {
     std::lock_guard<std::mutex> lock(g_cache_mu);
     // a block of exactly this size first (repeated solves ask for the same sizes, and
     // taking a larger block would push its own request to a fresh allocation)
     auto it = g_cache.find(need);
     if (it == g_cache.end()) it = g_cache.lower_bound(need);
     if (it == g_cache.end() || it->first / 2 > need) return nullptr;
-    void *p = it->second;
+    void *p = it->second.p;
+    clean = it->second.clean;
     got = it->first;
     g_cached -= got;
     g_cache.erase(it);
     return p;
 }
 
-bool cache_put(void *p, size_t bytes) {
+bool cache_put(void *p, size_t bytes, bool clean) {
     std::lock_guard<std::mutex> lock(g_cache_mu);
     if (bytes > kCacheBlockMax || g_cached + bytes > kCacheBytes) return false;
-    g_cache.emplace(bytes, p);
+    g_cache.emplace(bytes, CachedBlock{p, clean});
     g_cached += bytes;
     return true;
 }
 
-void cache_clear() {
+// free the cached blocks larger than `keep_block`, then the largest others until at
+// most `keep_total` bytes remain
+void cache_trim(size_t keep_block, size_t keep_total) {
     std::lock_guard<std::mutex> lock(g_cache_mu);
-    for (auto &kv : g_cache) (void)hipFree(kv.second);
-    g_cache.clear();
-    g_cached = 0;
+    for (auto it = g_cache.begin(); it != g_cache.end();) {
+        if (it->first > keep_block) {
+            (void)hipFree(it->second.p);
+            g_cached -= it->first;
+            it = g_cache.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    while (g_cached > keep_total && !g_cache.empty()) {
+        auto it = std::prev(g_cache.end());
+        (void)hipFree(it->second.p);
+        g_cached -= it->first;
+        g_cache.erase(it);
+    }
 }
+
+void cache_clear() { cache_trim(0, 0); }
 
 void cluster_born() {
     std::lock_guard<std::mutex> lock(g_cache_mu);
@@ -82,14 +113,42 @@ void cluster_died() {
         std::lock_guard<std::mutex> lock(g_cache_mu);
         last = --g_live_clusters == 0;
     }
-    if (last) cache_clear();
+    if (last) cache_trim(kKeepBlockMax, kKeepBytes);
+}
+
+void *pinned_take(size_t need, size_t &got) {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    auto it = g_pinned.lower_bound(need);
+    if (it == g_pinned.end() || it->first / 2 > need) return nullptr;
+    void *p = it->second;
+    got = it->first;
+    g_pinned.erase(it);
+    return p;
+}
+
+bool pinned_put(void *p, size_t bytes) {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    if (g_pinned.size() >= kPinnedKeep || bytes > ((size_t)16 << 20)) return false;
+    g_pinned.emplace(bytes, p);
+    return true;
 }
 }  // namespace
+
+CleanReleases::CleanReleases() : prev(t_clean_release) { t_clean_release = true; }
+CleanReleases::~CleanReleases() { t_clean_release = prev; }
 
 int HostBuf::ensure(size_t need) {
     if (need <= bytes && ptr) return TG_OK;
     release();
     if (need == 0) return TG_OK;
+    size_t got = 0;
+    if (void *p = pinned_take(need, got)) {
+        // a pinned block is released by its owner only after the stream that wrote it
+        // was synchronised (the host has read it)
+        ptr = p;
+        bytes = got;
+        return TG_OK;
+    }
     hipError_t e = hipHostMalloc(&ptr, need, hipHostMallocDefault);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -102,7 +161,7 @@ int HostBuf::ensure(size_t need) {
 }
 
 void HostBuf::release() {
-    if (ptr) (void)hipHostFree(ptr);
+    if (ptr && !pinned_put(ptr, bytes)) (void)hipHostFree(ptr);
     ptr = nullptr;
     bytes = 0;
 }
@@ -112,8 +171,9 @@ int DevBuf::ensure(size_t need) {
     release();
     if (need == 0) return TG_OK;
     size_t got = 0;
-    if (void *p = cache_take(need, got)) {
-        hipError_t e = hipDeviceSynchronize();
+    bool clean = false;
+    if (void *p = cache_take(need, got, clean)) {
+        hipError_t e = clean ? hipSuccess : hipDeviceSynchronize();
         if (e != hipSuccess) {
             (void)hipFree(p);
             set_error("hipDeviceSynchronize: %s", hipGetErrorString(e));
@@ -142,7 +202,7 @@ int DevBuf::ensure(size_t need) {
 }
 
 void DevBuf::release() {
-    if (ptr && !cache_put(ptr, bytes)) (void)hipFree(ptr);
+    if (ptr && !cache_put(ptr, bytes, t_clean_release)) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
 }
@@ -529,6 +589,8 @@ int tg_cluster_destroy(tg_cluster *c) {
     if (!c) return TG_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     (void)side_sync(c);
+    // both streams are idle: the cluster's blocks go back to the cache clean
+    CleanReleases clean;
     if (c->side) {
         (void)hipStreamDestroy(c->side);
         (void)hipEventDestroy(c->side_start);

@@ -81,6 +81,15 @@ struct DevBuf {
     template <class T> T *as() const { return reinterpret_cast<T *>(ptr); }
 };
 
+// While one of these is alive on a thread, DevBuf::release marks the blocks it returns
+// to the cache clean (no work queued on them): the releasing code has synchronised
+// every stream that used them.
+struct CleanReleases {
+    bool prev;
+    CleanReleases();
+    ~CleanReleases();
+};
+
 // Pinned host memory (hipHostMalloc): kernels write it, the host reads it after the
 // stream is synchronised, with no copy in between.
 struct HostBuf {

@@ -40,6 +40,7 @@ struct tg_factor {
     // powers of two, split order)
     bool equilibrated = false;
     tg::DevBuf eq;
+    tg::DevBuf eqwork;  // scratch of the equilibration statistics
     // c128 factors of at most kDenseResidual unknowns keep the FP64 operator as
     // assembled (before equilibration): the refinement residual of forward solves is
     // then one GEMM instead of a matrix-free apply
@@ -1289,7 +1290,7 @@ static int equilibrate_t(tg_factor *f) {
     const int64_t rblocks = (n + 255) / 256;
     const int chunk = (int)std::min<int64_t>(256, std::max<int64_t>(4, (n * rblocks + 2047) / 2048));
     const int parts = (int)((n + chunk - 1) / chunk);
-    DevBuf work;
+    DevBuf &work = f->eqwork;
     TG_TRY(work.ensure(sizeof(double) * ((size_t)2 * parts * n + 6 * (size_t)n)));
     double *pmax = work.as<double>(), *pmin = pmax + (size_t)parts * n, *rmax = pmin + (size_t)parts * n;
     double *rmin = rmax + n, *diag = rmin + n, *cmax = diag + n, *scale = cmax + n, *second = scale + n;
@@ -1630,6 +1631,9 @@ int tg_factor_create(tg_cluster *c, int precision, tg_factor **out) {
 int tg_factor_destroy(tg_factor *f) {
     if (!f) return TG_OK;
     (void)hipStreamSynchronize(f->cluster->stream);
+    // the panel stream's work joined the cluster's stream: the blocks are idle
+    CleanReleases clean;
+    f->eqwork.release();
     f->lu.release();
     f->eq.release();
     f->op.release();
