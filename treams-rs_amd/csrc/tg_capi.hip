@@ -472,6 +472,33 @@ static int cached_plan(int lmax, HostPlan &out) {
     return 0;
 }
 
+// The device plan (with its transposed plan, GEMM subsets and tile counts) depends on
+// lmax alone too: uploaded once per (device, lmax) and shared by every cluster on
+// that device, never freed (a few MB at lmax 12).
+static int shared_device_plan(int lmax, const HostPlan &hp, DevPlan &out, std::vector<int> &ntiles) {
+    struct Shared {
+        DevPlan plan;
+        std::vector<int> ntiles;
+    };
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, Shared> cache;
+    int dev = 0;
+    TG_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({dev, lmax});
+    if (it == cache.end()) {
+        DevBuf buf;
+        Shared sh;
+        TG_TRY(upload_plan(hp, buf, sh.plan, &sh.ntiles));
+        buf.ptr = nullptr;  // owned by the plan from now on (never returned to the cache)
+        buf.bytes = 0;
+        it = cache.emplace(std::make_pair(dev, lmax), std::move(sh)).first;
+    }
+    out = it->second.plan;
+    ntiles = it->second.ntiles;
+    return TG_OK;
+}
+
 extern "C" {
 
 const char *tg_last_error(void) { return g_error.c_str(); }
@@ -545,7 +572,7 @@ int tg_cluster_create(int lmax, double k0, int n_spheres, const double *radii, c
             st = TG_INVALID_INPUT;
             break;
         }
-        if ((st = upload_plan(c->hplan, c->b_plan, c->plan, &c->gemm_ntiles))) break;
+        if ((st = shared_device_plan(lmax, c->hplan, c->plan, c->gemm_ntiles))) break;
         std::vector<double> geo(6 * (size_t)n_spheres);
         memcpy(geo.data(), positions, sizeof(double) * 3 * n_spheres);
         memcpy(geo.data() + 3 * n_spheres, radii, sizeof(double) * n_spheres);
@@ -554,7 +581,9 @@ int tg_cluster_create(int lmax, double k0, int n_spheres, const double *radii, c
             geo[4 * (size_t)n_spheres + 2 * i + 1] = epsilon[i].im;
         }
         if ((st = c->b_pos.ensure(sizeof(double) * geo.size()))) break;
-        hipError_t e = hipMemcpy(c->b_pos.ptr, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice);
+        // (geo outlives the stream synchronisation below)
+        hipError_t e = hipMemcpyAsync(c->b_pos.ptr, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice,
+                                      c->stream);
         if (e != hipSuccess) {
             set_error("upload: %s", hipGetErrorString(e));
             st = TG_DEVICE_ERROR;
