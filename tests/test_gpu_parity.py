@@ -156,6 +156,25 @@ def test_inner_gemm_split16_vs_fp32(pkg, N, lmax, cols):
     assert np.array_equal(cl.apply(x, c64=True).cpu().numpy(), first)
 
 
+@pytest.mark.parametrize("N,lmax", [(45, 4), (96, 6), (130, 3)])
+def test_one_pass_tables_equal_fp64_tables(pkg, N, lmax):
+    """The one-pass table build (split FP16 and FP64 tables together, the path of a
+    cluster whose first coupling is c64-precision) writes the same FP64 tables, bit for
+    bit, as the FP64 table kernel (the path of a cluster whose first coupling is FP64):
+    both evaluate the recurrences with the same explicit FMAs."""
+    radii, e, pos = pkg.random_cluster(N, 1.9 * N ** (1 / 3), 0.05, (0.15, 0.35), None, seed=N)
+    rng = np.random.default_rng(1)
+    n = N * 2 * lmax * (lmax + 2)
+    x = rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))
+    a = pkg.IterativeSphereCluster(lmax, 1.3, radii, e, pos)
+    ya = a.coupling(x)
+    b = pkg.IterativeSphereCluster(lmax, 1.3, radii, e, pos)
+    yb16 = b.coupling(x, c64=True)
+    yb = b.coupling(x)
+    assert torch.equal(ya, yb)
+    assert _rel(yb16, yb) <= 1e-6
+
+
 @pytest.mark.parametrize("N,lmax,cols", [(6, 3, 3), (45, 4, 5), (96, 6, 4)])
 def test_inner_relaxed_hi_only_coupling(pkg, oracle, N, lmax, cols):
     """The relaxed inner coupling (split-FP16 tables, hi halves only: 11 significant bits

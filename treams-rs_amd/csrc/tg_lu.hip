@@ -1585,7 +1585,13 @@ static int factor_solve_once(tg_factor *f, const cd *b, cd *x, int cols, int adj
             ++refinements;
         }
     } while (0);
-    tmp.release();
+    if (st == TG_OK) {
+        // the last residual norms were read on the host: the stream is idle
+        CleanReleases clean;
+        tmp.release();
+    } else {
+        tmp.release();
+    }
     return st;
 }
 
@@ -1739,8 +1745,13 @@ int tg_factor_solve(tg_factor *f, const tg_c128 *incident, int cols, double rtol
     }
     if (conv_host)
         for (int k = 0; k < cols && k < (int)conv.size(); ++k) conv_host[k] = conv[k];
-    (void)hipStreamSynchronize(c->stream);
-    io.release();
+    const bool idle = hipStreamSynchronize(c->stream) == hipSuccess;
+    if (idle) {
+        CleanReleases clean;
+        io.release();
+    } else {
+        io.release();
+    }
     return st;
 }
 
