@@ -209,8 +209,16 @@ void DevBuf::release() {
 
 int Timer::begin(const char *name, hipStream_t s, Rec &r) {
     if (!enabled) return TG_OK;
-    TG_HIP(hipEventCreate(&r.a));
-    TG_HIP(hipEventCreate(&r.b));
+    // events are reused from collected records (creating two per region cost about as
+    // much host time as recording them)
+    for (hipEvent_t *e : {&r.a, &r.b}) {
+        if (!spare.empty()) {
+            *e = spare.back();
+            spare.pop_back();
+        } else {
+            TG_HIP(hipEventCreate(e));
+        }
+    }
     TG_HIP(hipEventRecord(r.a, s));
     (void)name;
     return TG_OK;
@@ -232,12 +240,17 @@ int Timer::collect() {
             TG_HIP(hipEventElapsedTime(&ms, r.a, r.b));
             tot.first += ms;
             tot.second += 1;
-            (void)hipEventDestroy(r.a);
-            (void)hipEventDestroy(r.b);
+            spare.push_back(r.a);
+            spare.push_back(r.b);
         }
     }
     pending.clear();
     return TG_OK;
+}
+
+void Timer::destroy_events() {
+    for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+    spare.clear();
 }
 
 // Upload a host plan into one device buffer and fill the device view.
@@ -647,6 +660,7 @@ int tg_cluster_destroy(tg_cluster *c) {
     c->outer.release();
     c->pin.release();
     c->timer.collect();
+    c->timer.destroy_events();
     delete c;
     cluster_died();
     return TG_OK;

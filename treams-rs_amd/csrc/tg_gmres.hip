@@ -87,6 +87,25 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
         }
         return std::isfinite(allowed) ? allowed : 0.0;
     };
+    // the column scales and extending masks of the next step (host side: scale_h[0 ..
+    // 2 cols) in the pinned buffer) to the device array `scale`, scaling src into dst
+    // when apply: by value in the scaling kernel for up to ColScales::kMax columns, else
+    // by an asynchronous copy first.  Without apply nothing reads the masks before the
+    // next cycle writes them, so the few-column form sends nothing.
+    auto send_scales = [&](const cd *src, cd *dst, bool apply) -> int {
+        const double *scale_h = reinterpret_cast<const double *>(c->pin.as<cd>() + nproj);
+        if (cols <= ColScales::kMax) {
+            if (!apply) return TG_OK;
+            ColScales v;
+            for (int k = 0; k < ColScales::kMax; ++k) {
+                v.s[k] = k < cols ? scale_h[k] : 0.0;
+                v.m[k] = k < cols ? scale_h[cols + k] : 0.0;
+            }
+            return vec_scale_cols_v(s, v, scale, src, dst, n, cols);
+        }
+        TG_HIP(hipMemcpyAsync(scale, scale_h, sizeof(double) * 2 * cols, hipMemcpyHostToDevice, s));
+        return apply ? vec_scale_cols(s, scale, src, dst, n, cols) : TG_OK;
+    };
     while (true) {
         TG_TRY(vec_norms(c, R, n, cols, n, rn.data()));
         int any = 0;
@@ -115,8 +134,7 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
             scale_h[k] = cycle[k] ? 1.0 / rn[k] : 0.0;
             scale_h[cols + k] = cycle[k] ? 1.0 : 0.0;  // the extending mask
         }
-        TG_HIP(hipMemcpyAsync(scale, scale_h, sizeof(double) * 2 * cols, hipMemcpyHostToDevice, s));
-        TG_TRY(vec_scale_cols(s, scale, R, V, n, cols));
+        TG_TRY(send_scales(R, V, true));
         std::fill(H.begin(), H.end(), zc(0.0));
         std::fill(g.begin(), g.end(), zc(0.0));
         for (int k = 0; k < cols; ++k) {
@@ -200,9 +218,8 @@ int gmres_run(tg_cluster *c, const OpFn &A, const OpFn *Minv, const cd *b, cd *x
                 }
             }
             for (int k = 0; k < cols; ++k) scale_h[cols + k] = extending[k] ? 1.0 : 0.0;
-            TG_HIP(hipMemcpyAsync(scale, scale_h, sizeof(double) * 2 * cols, hipMemcpyHostToDevice, s));
             ++iterations;
-            if (j + 1 < restart && more) TG_TRY(vec_scale_cols(s, scale, W, V + (int64_t)(j + 1) * vstep, n, cols));
+            TG_TRY(send_scales(W, V + (int64_t)(j + 1) * vstep, j + 1 < restart && more));
             if (!more) break;
         }
         // back substitution and x += M^{-1} V y, over the basis vectors written in this

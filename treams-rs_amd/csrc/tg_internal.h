@@ -105,9 +105,11 @@ struct Timer {
     struct Rec { hipEvent_t a, b; };
     std::map<std::string, std::vector<Rec>> pending;
     std::map<std::string, std::pair<double, int64_t>> totals;
+    std::vector<hipEvent_t> spare;  // events of collected records, reused by begin
     int begin(const char *name, hipStream_t s, Rec &r);
     int end(const char *name, hipStream_t s, Rec &r);
     int collect();
+    void destroy_events();  // after collect: the spare events
 };
 
 // Device view of a rectangular translation plan (destination degrees up to one lmax,
@@ -270,5 +272,12 @@ int vec_update(hipStream_t s, const cd *V, int64_t vstride, int nvec, const cd *
                int64_t colstride_v, int64_t colstride_w, double sign);
 int vec_axpby(hipStream_t s, cd alpha, const cd *x, cd beta, cd *y, int64_t count);
 int vec_scale_cols(hipStream_t s, const double *scale_dev, const cd *x, cd *y, int64_t n, int cols);
+// per-column scales and masks of a few columns, passed to a kernel by value
+struct ColScales {
+    static constexpr int kMax = 8;
+    double s[kMax], m[kMax];
+};
+// y = s x per column with s by value; s and m also written to store[0 .. 2 cols)
+int vec_scale_cols_v(hipStream_t s, const ColScales &v, double *store, const cd *x, cd *y, int64_t n, int cols);
 
 }  // namespace tg

@@ -1882,4 +1882,26 @@ int vec_scale_cols(hipStream_t s, const double *scale_dev, const cd *x, cd *y, i
     return TG_OK;
 }
 
+// The same with the scales and masks passed by value (no host-to-device copy):
+// y[col] = v.s[col] x[col], and v (the scales, then the masks) written to `store` for
+// the kernels after it.
+__global__ void k_scale_cols_v(ColScales v, double *store, const cd *x, cd *y, int64_t n, int cols) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2 * cols)
+        store[threadIdx.x] = threadIdx.x < cols ? v.s[threadIdx.x] : v.m[threadIdx.x - cols];
+    const int col = blockIdx.y;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    y[col * n + k] = v.s[col] * x[col * n + k];
+}
+
+int vec_scale_cols_v(hipStream_t s, const ColScales &v, double *store, const cd *x, cd *y, int64_t n, int cols) {
+    if (cols > ColScales::kMax) {
+        set_error("vec_scale_cols_v: at most %d columns", ColScales::kMax);
+        return TG_INVALID_INPUT;
+    }
+    k_scale_cols_v<<<dim3(grid1(n), cols), 256, 0, s>>>(v, store, x, y, n, cols);
+    TG_HIP(hipGetLastError());
+    return TG_OK;
+}
+
 }  // namespace tg
