@@ -70,12 +70,13 @@ def test_factor_adjoint_solve(pkg, oracle, precision):
     assert _rel(f.pullback_incident(g), oracle.apply_local(ref.blocks, yr, True)) <= 1e-10
 
 
-@pytest.mark.parametrize("N,lmax,precision", [(4, 2, "c128"), (6, 3, "c64"), (9, 3, "c128")])
+@pytest.mark.parametrize("N,lmax,precision", [(4, 2, "c128"), (6, 3, "c64"), (9, 3, "c128"), (16, 3, "c128"),
+                                             (16, 3, "c64"), (19, 3, "c128")])
 def test_interacting_tmatrix_many_columns(pkg, oracle, N, lmax, precision):
     """All n columns (the interacting T-matrix X = (I - T C)^{-1} T of sphere_cluster,
     cluster/spheres.rs:56-80): forward solves of more than 8 columns go block by block
     with MFMA GEMM updates (c64 factors widened to FP64), refined in FP64; n = 64, 180
-    (c64, ragged last block) and 270."""
+    (c64, ragged last block), 270, 480 (c128 and c64, 8 blocks) and 570 (ragged)."""
     cl, ref = _cluster(pkg, oracle, N=N, lmax=lmax, seed=2, side=1.6 * (N / 4) ** (1 / 3))
     f = pkg.SphereClusterFactor(cl, precision)
     eye = np.eye(ref.n, dtype=np.complex128)

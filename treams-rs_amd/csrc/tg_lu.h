@@ -17,6 +17,7 @@ struct LuSet {
     std::vector<int64_t> voff_h;
     DevBuf meta;                    // off[count], n[count], voff[count], piv[ntot], perm[ntot], status[count]
     DevBuf work;                    // 8 ntot FP64 complex values
+    DevBuf many;                    // the permuted right-hand sides of many-column solves
     const int64_t *off = nullptr;
     const int *n = nullptr;
     const int64_t *voff = nullptr;

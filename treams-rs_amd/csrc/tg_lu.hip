@@ -661,6 +661,7 @@ void luset_release(LuSet &ls) {
     }
     ls.meta.release();
     ls.work.release();
+    ls.many.release();
 }
 
 // The packed row permutation of the solves, perm[voff + k0 + i] = voff + k0 + (the
@@ -1054,7 +1055,8 @@ static int luset_solve_many(LuSet &ls, hipStream_t s, cd *x, int cols) {
     const Batch bt{ls.off, ls.n, ls.voff};
     const unsigned z = (unsigned)ls.count;
     const int64_t nmax = ls.nmax, ld = ls.ntot;
-    DevBuf tmp;
+    // kept with the set (a block released here would make the next solve synchronise)
+    DevBuf &tmp = ls.many;
     TG_TRY(tmp.ensure(sizeof(cd) * (size_t)ld * cols));
     const unsigned pgrid = (unsigned)((ld * cols + 255) / 256);
     k_lu_permute<<<pgrid, 256, 0, s>>>(x, tmp.as<cd>(), ld, cols, ls.perm, 0);
@@ -1075,8 +1077,6 @@ static int luset_solve_many(LuSet &ls, hipStream_t s, cd *x, int cols) {
                                                                                             cols);
     }
     TG_HIP(hipGetLastError());
-    TG_HIP(hipStreamSynchronize(s));
-    tmp.release();
     return TG_OK;
 }
 
