@@ -123,7 +123,7 @@ def test_operator_apply_matrix_free_c64(pkg, oracle):
         assert _rel(cl.coupling(x, c64=True), ref.C @ x) <= 1e-5
 
 
-@pytest.mark.parametrize("N,lmax,cols", [(6, 3, 3), (45, 4, 5), (96, 6, 4), (130, 3, 12)])
+@pytest.mark.parametrize("N,lmax,cols", [(6, 3, 3), (45, 4, 5), (96, 6, 4), (130, 3, 12), (12, 10, 2)])
 def test_inner_gemm_split16_vs_fp32(pkg, N, lmax, cols):
     """The split-FP16 inner coupling (scaled hi + lo halves on FP16 MFMA) against the
     FP64 operator, beside the FP32 form: single-precision level for both, on ragged

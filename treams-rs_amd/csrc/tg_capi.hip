@@ -556,6 +556,7 @@ int tg_cluster_create(int lmax, double k0, int n_spheres, const double *radii, c
         set_error("require positive finite k0 and radii, and matching finite permittivities and positions");
         return TG_INVALID_INPUT;
     }
+    double dmin = INFINITY;  // the closest pair of centres
     for (int i = 0; i < n_spheres; ++i)
         for (int j = 0; j < i; ++j) {
             double d = 0.0;
@@ -564,9 +565,11 @@ int tg_cluster_create(int lmax, double k0, int n_spheres, const double *radii, c
                 set_error("spherical particles must have finite separation and must not touch or overlap");
                 return TG_INVALID_INPUT;
             }
+            dmin = std::min(dmin, d);
         }
     tg_cluster *c = new tg_cluster();
     cluster_born();
+    c->dmin = dmin;
     c->lmax = lmax;
     c->N = n_spheres;
     c->M = 2 * lmax * (lmax + 2);

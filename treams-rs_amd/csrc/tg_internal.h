@@ -150,6 +150,7 @@ struct tg_cluster {
     tg::DevBuf b_tabh, b_texp;         // split FP16 tables and their row exponents (FP16 MFMA coupling)
     bool have_tablesh = false;
     int inner_gemm = TG_INNER_SPLIT16; // c64-precision GEMMs of the inner solves (tg_inner_gemm)
+    double dmin = INFINITY;            // the closest pair of centres (c64_tables_in_range)
     // relaxed inner applies: a split-FP16 apply multiplies the hi halves only when the
     // error it may make (relax_allowed, relative, set per Arnoldi step by an inner
     // GMRES) is at least relax_threshold; 0 switches relaxation off
@@ -202,12 +203,11 @@ int tloc_from_ref(tg_cluster *c, const cd *ref, cd *panels);
 int local_cotangent(tg_cluster *c, const cd *Y, const cd *R, int cols, cd *out_ref);
 // error for operations that need the spheres' Mie blocks (radii, permittivities)
 int require_spheres(const tg_cluster *c, const char *what);
-// singular blocks as their two same-helicity H x H parts in c64, [count][h][a][b]
-int launch_translation_blocks_helicity_c64(int lmax, const DevPlan &plan, double k, int64_t count,
-                                          const double *disp, cf *out, hipStream_t s);
-// the same as pairs of bfloat16 [count][h][a][b] (real part in the low half)
+// singular blocks as their two same-helicity H x H parts, each entry a pair of bfloat16
+// (real part in the low half), [count][h][a][b]
+// (fp32_ok: the tables of these pairs fit FP32; fp32_tables_in_range of their cluster)
 int launch_translation_blocks_helicity_bf16(int lmax, const DevPlan &plan, double k, int64_t count,
-                                           const double *disp, uint32_t *out, hipStream_t s);
+                                           const double *disp, uint32_t *out, hipStream_t s, bool fp32_ok);
 int launch_translation_blocks(int lmax, const DevPlan &plan, double k, int64_t count, const double *disp,
                               int regular, cd *out, hipStream_t s);
 // tangent tables of every ordered pair (FP64, the layout of the singular pair tables)
@@ -224,6 +224,16 @@ int launch_translation_rect(const RectPlan &pl, double k, int64_t count, const d
 int coupling_matrix_free(tg_cluster *c, const cd *x, cd *y, int cols, int regular);
 // whether one FP64 pair-table set fits the cluster's table limit
 bool tables_fit(const tg_cluster *c);
+// whether the c64-precision inner couplings stay within FP32 range: the largest pair
+// table entry is bounded by |h_p(k0 dmin)| sqrt((2p)!) over the degrees p (the bound
+// of the split tables' row shifts at the closest pair), and that bound times N must
+// leave the FP32 intermediates (c64 tables, the GEMM outputs) below 2^110; beyond it
+// the inner applies run in FP64
+bool c64_tables_in_range(const tg_cluster *c);
+// whether that bound stays below 2^110: FP32 copies of the tables (the LDS tables of the
+// c64 assemblies), and the sums of at most 2 lmax + 1 of them times coefficients below 2^5
+// that form an assembled entry, cannot overflow
+bool fp32_tables_in_range(const tg_cluster *c);
 // start the FP64 and split tables of the c64-precision inner coupling on the side
 // stream (no-op when built, not fitting or not split); tables_wait makes the
 // cluster's stream wait for them, side_sync waits on the host (before freeing them)

@@ -134,6 +134,20 @@ struct GeoList {
     }
 };
 
+// A displacement list taken P at a time: workgroup `col` holds displacements col P ..
+// col P + P - 1 (its rows), none past `count`.
+template <int P>
+struct GeoListP {
+    const double *disp;
+    int64_t count;
+    __device__ int get(int col, int row, double d[3]) const {
+        const int64_t idx = (int64_t)col * P + row;
+        if (idx >= count) return 0;
+        for (int a = 0; a < 3; ++a) d[a] = disp[3 * idx + a];
+        return 2;
+    }
+};
+
 // Streaming (nontemporal) stores for data written once and read much later: the
 // stored coupling, the operator of the LU and the pair tables are all far larger than
 // the caches.
@@ -218,21 +232,10 @@ struct OutBlocks {
     }
 };
 
-// The two same-helicity H x H blocks of each pair in c64, [pair][h][a][b] (destination
-// b inner; h = 0 the reference's first polarisation index): the opposite-helicity
-// entries are zero and not stored.
-struct OutBlocksHelicity {
-    float2 *out;
-    int H;
-    __device__ void store(int, int j, int a, int b, cd vp, cd vm) const {
-        float2 *blk = out + (size_t)j * 2 * H * H;
-        blk[(size_t)a * H + b] = make_float2((float)vp.re, (float)vp.im);
-        blk[(size_t)H * H + (size_t)a * H + b] = make_float2((float)vm.re, (float)vm.im);
-    }
-};
-
-// The same with each complex value stored as two bfloat16 (real part in the low half,
-// rounded to nearest even): the near-field blocks of the preconditioner applied inside
+// The two same-helicity H x H blocks of each pair, [pair][h][a][b] (destination b inner;
+// h = 0 the reference's first polarisation index; the opposite-helicity entries are zero
+// and not stored), each complex value as two bfloat16 (real part in the low half, rounded
+// to nearest even): the near-field blocks of the preconditioner applied inside
 // mixed-precision solves, where an approximate operator only costs iterations.
 __device__ __forceinline__ uint32_t bf16_bits(float v) {
     const uint32_t u = __float_as_uint(v);
@@ -245,6 +248,16 @@ struct OutBlocksHelicityBF16 {
         uint32_t *blk = out + (size_t)j * 2 * H * H;
         blk[(size_t)a * H + b] = bf16_bits((float)vp.re) | (bf16_bits((float)vp.im) << 16);
         blk[(size_t)H * H + (size_t)a * H + b] = bf16_bits((float)vm.re) | (bf16_bits((float)vm.im) << 16);
+    }
+};
+
+// The same for GeoListP: pair col P + row.
+template <int P>
+struct OutBlocksHelicityBF16P {
+    uint32_t *out;
+    int H;
+    __device__ void store(int row, int col, int a, int b, cd vp, cd vm) const {
+        OutBlocksHelicityBF16{out, H}.store(0, col * P + row, a, b, vp, vm);
     }
 };
 
@@ -443,14 +456,18 @@ int launch_assemble_split(tg_cluster *c, int storage, void *outp) {
     Timer::Rec rec;
     TG_TRY(c->timer.begin("assemble", c->stream, rec));
     TG_TRY(check_smem(smem, c->lmax));
-    if (storage == TG_STORE_C64) {
-        // 16 destination spheres per workgroup: measured 7.3 ms at N = 1024, lmax 6
-        // against 7.7 ms for 8 and 7.4 ms for 24 or 32 (fewer waves per SIMD)
-        constexpr int PC = 16;
+    // the c64 kernel: 16 destination spheres per workgroup, measured 7.3 ms at N = 1024,
+    // lmax 6 against 7.7 ms for 8 and 7.4 ms for 24 or 32 (fewer waves per SIMD)
+    constexpr int PC = 16;
+    const size_t smemf = (size_t)((PC * c->plan.T + 1) & ~1) * sizeof(float2) +
+                         (size_t)PC * (c->plan.L2 + 1) * sizeof(cd);
+    if (storage == TG_STORE_C64 && (smemf > 65536 || !fp32_tables_in_range(c))) {
+        // high degrees, or tables beyond FP32 range (close spheres): FP64 tables in LDS
         float2 *base = (float2 *)outp;
-        const size_t smemf = (size_t)((PC * c->plan.T + 1) & ~1) * sizeof(float2) +
-                             (size_t)PC * (c->plan.L2 + 1) * sizeof(cd);
-        TG_TRY(check_smem(smemf, c->lmax));
+        OutSplit<float2> out{base, base + c->nh * c->nh, c->nh, c->H};
+        k_assemble<P, GeoCluster, OutSplit<float2>, false><<<grid, 256, smem, c->stream>>>(c->plan, c->k0, geo, out);
+    } else if (storage == TG_STORE_C64) {
+        float2 *base = (float2 *)outp;
         k_assemble_c64<PC><<<dim3(c->N, (c->N + PC - 1) / PC), 256, smemf, c->stream>>>(c->plan, c->k0, geo, base,
                                                                                         base + c->nh * c->nh, c->nh);
     } else {
@@ -475,7 +492,7 @@ int launch_assemble_operator(tg_cluster *c, int precision, void *outp) {
     if (precision == TG_STORE_C64) {
         OutOperator<float2> out{(float2 *)outp, c->n, c->nh, c->H, c->lmax, c->b_mie.as<cd>()};
         const size_t smem32 = assemble_smem<float2>(c->plan, P);
-        if (smem32 <= 65536)
+        if (smem32 <= 65536 && fp32_tables_in_range(c))
             k_assemble<P, GeoCluster, OutOperator<float2>, false, float2>
                 <<<grid, 256, smem32, c->stream>>>(c->plan, c->k0, geo, out);
         else
@@ -540,31 +557,27 @@ int launch_translation_blocks(int lmax, const DevPlan &pl, double k, int64_t cou
     return TG_OK;
 }
 
-int launch_translation_blocks_helicity_c64(int lmax, const DevPlan &pl, double k, int64_t count,
-                                          const double *disp, cf *outp, hipStream_t s) {
-    const int H = lmax * (lmax + 2);
-    TG_HIP(hipMemsetAsync(outp, 0, sizeof(cf) * 2 * H * H * count, s));
-    if (count == 0) return TG_OK;
-    GeoList geo{disp};
-    OutBlocksHelicity out{(float2 *)outp, H};
-    size_t smem = assemble_smem(pl, 1);
-    TG_TRY(check_smem(smem, lmax));
-    k_assemble<1, GeoList, OutBlocksHelicity, false><<<dim3((unsigned)count, 1), 256, smem, s>>>(pl, k, geo, out);
-    TG_HIP(hipGetLastError());
-    return TG_OK;
-}
-
 int launch_translation_blocks_helicity_bf16(int lmax, const DevPlan &pl, double k, int64_t count,
-                                           const double *disp, uint32_t *outp, hipStream_t s) {
+                                           const double *disp, uint32_t *outp, hipStream_t s, bool fp32_ok) {
+    // 8 pairs per workgroup (each plan term loaded once for the 8), their tables in
+    // LDS rounded to FP32 (the blocks are stored as bfloat16); every entry of every
+    // block is written, so the output needs no clearing
+    constexpr int P = 8;
     const int H = lmax * (lmax + 2);
-    TG_HIP(hipMemsetAsync(outp, 0, sizeof(uint32_t) * 2 * H * H * count, s));
     if (count == 0) return TG_OK;
-    GeoList geo{disp};
-    OutBlocksHelicityBF16 out{outp, H};
-    size_t smem = assemble_smem(pl, 1);
-    TG_TRY(check_smem(smem, lmax));
-    k_assemble<1, GeoList, OutBlocksHelicityBF16, false><<<dim3((unsigned)count, 1), 256, smem, s>>>(pl, k, geo,
-                                                                                                     out);
+    const size_t smem = assemble_smem<float2>(pl, P);
+    if (smem <= 65536 && fp32_ok) {
+        GeoListP<P> geo{disp, count};
+        OutBlocksHelicityBF16P<P> out{outp, H};
+        k_assemble<P, GeoListP<P>, OutBlocksHelicityBF16P<P>, false, float2>
+            <<<dim3((unsigned)((count + P - 1) / P), 1), 256, smem, s>>>(pl, k, geo, out);
+    } else {
+        // high degrees, or tables beyond FP32 range: one pair per workgroup, FP64 tables
+        const size_t smem1 = assemble_smem(pl, 1);
+        TG_TRY(check_smem(smem1, lmax));
+        k_assemble<1, GeoList, OutBlocksHelicityBF16, false><<<dim3((unsigned)count, 1), 256, smem1, s>>>(
+            pl, k, GeoList{disp}, OutBlocksHelicityBF16{outp, H});
+    }
     TG_HIP(hipGetLastError());
     return TG_OK;
 }
@@ -973,6 +986,33 @@ bool tables_fit(const tg_cluster *c) {
     return (int64_t)sizeof(cd) * c->N * c->N * c->plan.T <= c->table_limit;
 }
 
+// log2 of the bound |h_p(k0 dmin)| sqrt((2p)!), maximised over p, of every pair table
+// entry (-inf for a single sphere)
+static double log2_table_bound(const tg_cluster *c) {
+    if (!(c->dmin < INFINITY)) return -INFINITY;  // a single sphere: no pair tables
+    const double x = c->k0 * c->dmin, inv = 1.0 / x;
+    std::complex<double> hprev(std::cos(x) * inv, std::sin(x) * inv), hcur(std::sin(x) * inv, -std::cos(x) * inv);
+    double log2max = -INFINITY, lfact = 0.0;  // lfact = ln (2p)!
+    for (int p = 0; p <= c->plan.L2; ++p) {
+        if (p > 0) lfact += std::log((double)(2 * p - 1)) + std::log((double)(2 * p));
+        log2max = std::max(log2max, std::log2(std::abs(hcur)) + 0.5 * lfact / std::log(2.0));
+        const std::complex<double> hnext = ((double)(2 * p + 1) * inv) * hcur - hprev;
+        hprev = hcur;
+        hcur = hnext;
+    }
+    return log2max;
+}
+
+bool c64_tables_in_range(const tg_cluster *c) {
+    const double b = log2_table_bound(c);
+    return b == -INFINITY || (std::isfinite(b) && b + std::log2((double)c->N) < 110.0);
+}
+
+bool fp32_tables_in_range(const tg_cluster *c) {
+    const double b = log2_table_bound(c);
+    return b == -INFINITY || (std::isfinite(b) && b < 110.0);
+}
+
 // The FP64 singular (or regular) tables, and with c64 also their c64 rounding: built
 // together in one pass when neither exists yet.
 static int ensure_pair_tables(tg_cluster *c, bool regular, bool c64) {
@@ -1029,7 +1069,9 @@ int side_sync(tg_cluster *c) {
 // recurrences (the first inner apply waits for them only), then the FP64 tables of the
 // refinement residuals (their first user waits for them).
 int prefetch_inner_tables(tg_cluster *c) {
-    if (c->tables_pending || c->have_tablesh || c->inner_gemm != TG_INNER_SPLIT16 || !tables_fit(c)) return TG_OK;
+    if (c->tables_pending || c->have_tablesh || c->inner_gemm != TG_INNER_SPLIT16 || !tables_fit(c) ||
+        !c64_tables_in_range(c))
+        return TG_OK;
     if (!c->side) {
         TG_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         TG_HIP(hipEventCreateWithFlags(&c->side_start, hipEventDisableTiming));
@@ -1300,7 +1342,7 @@ static int coupling_on_tables(tg_cluster *c, const cx<R> *TAB, const cd *x, cd *
 // tables the hi halves alone are multiplied when hi_only is set or the current inner
 // step allows the error (relaxation).
 int coupling_matrix_free32(tg_cluster *c, const cd *x, cd *y, int cols, bool hi_only) {
-    if (!tables_fit(c)) return coupling_matrix_free(c, x, y, cols, 0);
+    if (!tables_fit(c) || !c64_tables_in_range(c)) return coupling_matrix_free(c, x, y, cols, 0);
     if (c->inner_gemm == TG_INNER_SPLIT16) {
         hi_only = hi_only || (c->relax_threshold > 0.0 && c->relax_allowed >= c->relax_threshold);
         return coupling_tables_split16(c, x, y, cols, hi_only);

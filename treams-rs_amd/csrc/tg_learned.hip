@@ -591,7 +591,8 @@ static int learned_near_apply_inner(tg_cluster *c, const cd *x, cd *y, int cols)
     if (!s->have_blocksh) {
         TG_TRY(s->blocksh.ensure(sizeof(uint32_t) * (size_t)s->npairs * 2 * H * H));
         TG_TRY(launch_translation_blocks_helicity_bf16(c->lmax, c->plan, c->k0, s->npairs, s->disp.as<double>(),
-                                                       s->blocksh.as<uint32_t>(), c->stream));
+                                                       s->blocksh.as<uint32_t>(), c->stream,
+                                                       fp32_tables_in_range(c)));
         s->have_blocksh = true;
     }
     TG_TRY(s->part.ensure(sizeof(cd) * (size_t)s->npairs * 4 * M));

@@ -103,17 +103,19 @@ __global__ __launch_bounds__(64) void k_split_rowexp_bound(int L2, double k, con
 // FP64 (16 consecutive rows per source: 256-byte runs) when write64, and split with
 // its row's shift into LDS in fragment order; after each order the workgroup writes
 // the order's fragments (up to 2 L2 harmonics x 4 components x 512 bytes) with 16-byte
-// stores.  Pairs with i or j beyond N give zero fragments (and no FP64 entries), self
-// pairs and coincident centres zeros.  Dynamic LDS: the staging [nstage][4][256]
-// halves, the row shifts [T][16] (int16) and the coefficients [T] (a, b).
+// stores; an order with more harmonics than the staging holds (nsub, sized so that
+// the dynamic LDS stays within the 64 KB a workgroup may allocate) is written in
+// rounds.  Pairs with i or j beyond N give zero fragments (and no FP64 entries), self
+// pairs and coincident centres zeros.  Dynamic LDS: the staging [nsub][4][256] halves,
+// the coefficients [T] (a, b) and the row shifts [T][16] (int16).
 template <int LB2>
 __global__ __launch_bounds__(256) void k_pair_tables_split(DevPlan pl, double k, const double *pos, int N, int Np,
-                                                           const int *E, cd *tab, h8v *TABH, int write64) {
+                                                           const int *E, cd *tab, h8v *TABH, int write64,
+                                                           int nsub) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int T = pl.T, L2 = pl.L2;
-    const int nstage = 2 * L2 > L2 + 1 ? 2 * L2 : L2 + 1;  // most harmonics of one order
-    _Float16 *stage = reinterpret_cast<_Float16 *>(lds);     // [nstage][4][256]
-    double2 *coef = reinterpret_cast<double2 *>(lds + (size_t)nstage * 4 * 256 * sizeof(_Float16));  // [T]
+    _Float16 *stage = reinterpret_cast<_Float16 *>(lds);     // [nsub][4][256]
+    double2 *coef = reinterpret_cast<double2 *>(lds + (size_t)nsub * 4 * 256 * sizeof(_Float16));  // [T]
     int16_t *shift = reinterpret_cast<int16_t *>(coef + T);  // [T][16]
     const int tid = threadIdx.x, ii = tid & 15, jj = tid >> 4;
     const int j16 = blockIdx.x, ib = blockIdx.y;
@@ -155,11 +157,26 @@ __global__ __launch_bounds__(256) void k_pair_tables_split(DevPlan pl, double k,
         }
     }
     cd *tij = tab + (int64_t)j * N + i;
+    // write the staged fragments of the order's staging indices base .. base + count - 1
+    // (held at staging slots 0 .. count - 1): chunk = (slot, component), 32 lanes of 16
+    // bytes each.  Every thread calls it at the same points (the put sequence of an
+    // order is the same for all), so its barriers are uniform.
+    auto flush = [&](int q, int base, int count) {
+        __syncthreads();
+        const h8v *sv = reinterpret_cast<const h8v *>(stage);
+        for (int w = tid; w < count * 4 * 32; w += 256) {
+            const int ch = w >> 5, l32 = w & 31, sidx = base + (ch >> 2), comp = ch & 3;
+            const int p = q + (q ? sidx >> 1 : sidx);
+            const int h = p * p + p + ((q && (sidx & 1)) ? -q : q);
+            TABH[(((int64_t)h * NB16 + ib) * NB32 + jb) * 256 + comp * 64 + jhalf * 32 + l32] = sv[ch * 32 + l32];
+        }
+        __syncthreads();
+    };
     __syncthreads();
     for (int q = 0; q <= L2; ++q) {
         // (p, +q) and then (p, -q) for p = q, q + 1, ...: the staging index is the
-        // call count
-        int s = 0;
+        // call count; the staging holds nsub of them at a time
+        int s = 0, base = 0;
         auto put = [&](int h, cd v) {
             if (write64 && valid) {
                 typedef double d2v __attribute__((ext_vector_type(2)));
@@ -168,12 +185,15 @@ __global__ __launch_bounds__(256) void k_pair_tables_split(DevPlan pl, double k,
             }
             const int sh = shift[h * 16 + ii];
             const half2s pr = split16(v.re, sh), pi = split16(v.im, sh);
-            _Float16 *o = stage + (size_t)s * 1024 + slot;
+            _Float16 *o = stage + (size_t)(s - base) * 1024 + slot;
             o[0] = pr.hi;
             o[256] = pr.lo;
             o[512] = pi.hi;
             o[768] = pi.lo;
-            ++s;
+            if (++s - base == nsub) {
+                flush(q, base, nsub);
+                base = s;
+            }
         };
         if (q > 0) ladder = (-(double)(2 * q - 1)) * cmul_fma(ladder, xy);
         cd cur = ladder, prev(0.0);
@@ -187,26 +207,16 @@ __global__ __launch_bounds__(256) void k_pair_tables_split(DevPlan pl, double k,
                 prev = cur;
                 cur = next;
             }
-            if (live) {
-                put(p * p + p + q, cmul_fma(hv[p], cur));
-                if (q > 0) put(p * p + p - q, cmul_fma(hv[p], reflect(pl.refl[p * p + p - q], cur)));
-            } else {
-                put(p * p + p + q, cd(0.0));
-                if (q > 0) put(p * p + p - q, cd(0.0));
+            // values chosen per lane, put called by every thread alike: put may flush,
+            // and its barriers must not sit in a divergent branch
+            const cd vpos = live ? cmul_fma(hv[p], cur) : cd(0.0);
+            put(p * p + p + q, vpos);
+            if (q > 0) {
+                const cd vneg = live ? cmul_fma(hv[p], reflect(pl.refl[p * p + p - q], cur)) : cd(0.0);
+                put(p * p + p - q, vneg);
             }
         }
-        __syncthreads();
-        // write the order's staged fragments: chunk = (staging index, component), 32
-        // lanes of 16 bytes each
-        const int chunks = (L2 + 1 - q) * (q ? 2 : 1) * 4;
-        const h8v *sv = reinterpret_cast<const h8v *>(stage);
-        for (int w = tid; w < chunks * 32; w += 256) {
-            const int ch = w >> 5, l32 = w & 31, sidx = ch >> 2, comp = ch & 3;
-            const int p = q + (q ? sidx >> 1 : sidx);
-            const int h = p * p + p + ((q && (sidx & 1)) ? -q : q);
-            TABH[(((int64_t)h * NB16 + ib) * NB32 + jb) * 256 + comp * 64 + jhalf * 32 + l32] = sv[ch * 32 + l32];
-        }
-        __syncthreads();
+        if (s > base) flush(q, base, s - base);
     }
 }
 
@@ -445,19 +455,23 @@ int ensure_split_tables(tg_cluster *c) {
     TG_TRY(c->b_texp.ensure(sizeof(int) * (size_t)T * Np));
     TG_TRY(c->b_tabh.ensure((size_t)T * Np * Np * 8));
     int *E = c->b_texp.as<int>();
+    // staging for up to the most harmonics of one order, within 64 KB of dynamic LDS
+    // beside the coefficients and the row shifts
+    const size_t fixed = sizeof(double2) * (size_t)T + sizeof(int16_t) * (size_t)T * 16;
+    const size_t per = sizeof(_Float16) * 4 * 256;
     const int nstage = std::max(2 * L2, L2 + 1);
-    const size_t lds = sizeof(_Float16) * (size_t)nstage * 4 * 256 + sizeof(double2) * (size_t)T +
-                       sizeof(int16_t) * (size_t)T * 16;
-    if (lds > 160 * 1024) {
-        set_error("split tables: lmax %d needs more LDS than a workgroup has", c->plan.lmax);
+    const int nsub = (int)std::min<size_t>(nstage, fixed < 65536 ? (65536 - fixed) / per : 0);
+    if (nsub < 1) {
+        set_error("split tables: lmax %d needs more LDS than a workgroup may allocate", c->plan.lmax);
         return TG_INVALID_INPUT;
     }
+    const size_t lds = per * (size_t)nsub + fixed;
     k_split_rowexp_bound<<<(unsigned)Np, 64, 0, c->stream>>>(L2, c->k0, c->b_pos.as<double>(), N, Np, E);
     TG_HIP(hipGetLastError());
     const dim3 grid((unsigned)(Np / 16), (unsigned)(Np / 16));
 #define TG_TABLES_SPLIT(LB)                                                                                     \
     k_pair_tables_split<LB><<<grid, 256, lds, c->stream>>>(c->plan, c->k0, c->b_pos.as<double>(), N, Np, E,    \
-                                                          c->b_tab.as<cd>(), c->b_tabh.as<h8v>(), write64 ? 1 : 0)
+                                                          c->b_tab.as<cd>(), c->b_tabh.as<h8v>(), write64 ? 1 : 0, nsub)
     if (L2 <= 6) TG_TABLES_SPLIT(6);
     else if (L2 <= 12) TG_TABLES_SPLIT(12);
     else TG_TABLES_SPLIT(24);
