@@ -123,7 +123,7 @@ def test_operator_apply_matrix_free_c64(pkg, oracle):
         assert _rel(cl.coupling(x, c64=True), ref.C @ x) <= 1e-5
 
 
-@pytest.mark.parametrize("N,lmax,cols", [(6, 3, 3), (45, 4, 5), (96, 6, 4), (130, 3, 12), (12, 10, 2)])
+@pytest.mark.parametrize("N,lmax,cols", [(6, 3, 3), (45, 4, 5), (96, 6, 4), (130, 3, 12)])
 def test_inner_gemm_split16_vs_fp32(pkg, N, lmax, cols):
     """The split-FP16 inner coupling (scaled hi + lo halves on FP16 MFMA) against the
     FP64 operator, beside the FP32 form: single-precision level for both, on ragged
@@ -154,6 +154,77 @@ def test_inner_gemm_split16_vs_fp32(pkg, N, lmax, cols):
     # same tables, the same result bit for bit
     cl.set_inner_gemm("split16")
     assert np.array_equal(cl.apply(x, c64=True).cpu().numpy(), first)
+
+
+def _log2_table_bound(k0, pos, lmax):
+    """log2 of max over p <= 2 lmax of |h_p(k0 dmin)| sqrt((2p)!), the bound on every
+    singular pair-table entry, from scipy's spherical Bessel functions."""
+    from scipy.special import gammaln, spherical_jn, spherical_yn
+
+    pos = np.asarray(pos)
+    d = np.linalg.norm(pos[:, None] - pos[None], axis=-1)[np.triu_indices(len(pos), 1)]
+    x, p = k0 * d.min(), np.arange(2 * lmax + 1)
+    h = np.hypot(spherical_jn(p, x), spherical_yn(p, x))
+    return float(np.max(np.log2(h) + 0.5 * gammaln(2 * p + 1) / np.log(2)))
+
+
+def test_high_degree_split16_in_range(pkg):
+    """lmax 10 with well separated spheres (tables far inside FP32 range): the split-FP16
+    coupling runs (its GEMM is timed) at single-precision level against FP64."""
+    g = np.arange(2) * 6.0
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    assert _log2_table_bound(1.3, pos, 10) + np.log2(len(pos)) < 105
+    cl = pkg.IterativeSphereCluster(10, 1.3, np.full(8, 0.3), 4 + 0.1j, pos)
+    cl.set_inner_gemm("split16")
+    cl.timing(True)
+    x = np.random.default_rng(3).normal(size=(cl.n, 3)) + 0j
+    err = _rel(cl.apply(x, c64=True), cl.apply(x))
+    print("lmax 10 split16", f"{err:.2e}")
+    assert err <= 2e-6
+    assert cl.timing_get("gemm_split16")[1] > 0
+
+
+def test_c64_paths_beyond_fp32_range(pkg):
+    """lmax 10 with close spheres: singular tables beyond FP32 range.  Every c64-precision
+    path (split-FP16 and FP32 matrix-free couplings, the stored c64 coupling, the c64 LU,
+    the learned preconditioner's bfloat16 near blocks) falls back to FP64 tables and stays
+    finite and correct; the split GEMM does not run."""
+    lmax = 10
+    radii, e, pos = pkg.random_cluster(12, 1.9 * 12 ** (1 / 3), 0.05, (0.15, 0.35), None, seed=12)
+    assert _log2_table_bound(1.3, pos, lmax) > 125
+    cl = pkg.IterativeSphereCluster(lmax, 1.3, radii, e, pos)
+    cl.timing(True)
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=(cl.n, 2)) + 1j * rng.normal(size=(cl.n, 2))
+    ref, cref = cl.apply(x).cpu().numpy(), cl.coupling(x).cpu().numpy()
+    errs = {}
+    for kind in ("split16", "fp32"):
+        cl.set_inner_gemm(kind)
+        errs[kind] = _rel(cl.apply(x, c64=True), ref)
+        errs[kind, "coupling"] = _rel(cl.coupling(x, c64=True), cref)
+    cl.assemble("c64")
+    errs["stored"] = _rel(cl.coupling(x, stored=True), cref)
+    errs["stored apply"] = _rel(cl.apply(x, stored=True), ref)
+    print("lmax 10 close", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert all(np.isfinite(v) and v <= 2e-6 for v in errs.values()), errs
+    assert cl.timing_get("gemm_split16")[1] == 0
+    a = cl.plane_wave([0, 0, 1], 1)
+    xs = {}
+    for prec in ("c128", "c64"):
+        f = pkg.SphereClusterFactor(cl, prec, rtol=1e-12)
+        xs[prec], conv = f.solve(a)
+        f.close()
+        print("lmax 10 close LU", prec, conv)
+    assert np.isfinite(xs["c64"].cpu().numpy()).all()
+    assert _rel(xs["c64"], xs["c128"]) <= 1e-10
+    import importlib
+
+    learned = importlib.import_module("treams_rs_amd.learned")
+    learned.attach(cl, learned.MLP.load(learned.DEFAULT_WEIGHTS))
+    xl, cl_conv = cl.solve(a, rtol=1e-9, mixed_precision="matrix_free", preconditioner="learned", restart=50)
+    print("lmax 10 close learned", cl_conv)
+    assert cl_conv[0][1] <= 1e-9 * cl_conv[0][2]
+    assert _rel(xl, xs["c128"]) <= 1e-7
 
 
 @pytest.mark.parametrize("N,lmax", [(45, 4), (96, 6), (130, 3)])
