@@ -119,19 +119,23 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 }
 
 // Factor the nb x nb diagonal block at k0 with partial pivoting inside the block;
-// piv[k0 + k] = row exchanged with row k0 + k.  Four waves: lane r holds row r, wave w
-// its columns 16 w .. 16 w + 15 in registers.  Rows never move during the
-// factorisation (implicit pivoting): ord = the step at which a row became the pivot
-// (-1 before), and each row lands at position ord when the block is stored, which is
-// the arrangement of the equivalent exchange sequence piv (tracked in cur = the row at
-// each position).  Step k: the wave owning column k finds the pivot among the rows
-// not yet pivoted (one 32-bit wave max over keys = the high word of |a|^2 with the
-// low 6 bits replaced by 63 - row: magnitudes within 2^-14 of the largest tie, and
-// the lowest row wins) and the multipliers, and publishes them in LDS (double
+// piv[k0 + k] = row exchanged with row k0 + k.  W waves: lane r holds row r, wave w
+// the columns w, w + W, w + 2W, ... in registers (dealt cyclically, so that every
+// wave has about the same number of columns right of any step, and two waves share a
+// SIMD: one's pivot search runs beside the other's update).  Rows never move during
+// the factorisation (implicit pivoting): ord = the step at which a row became the
+// pivot (-1 before), and each row lands at position ord when the block is stored,
+// which is the arrangement of the equivalent exchange sequence piv (tracked in cur =
+// the row at each position).  Step k: the wave owning column k finds the pivot among
+// the rows not yet pivoted (one 32-bit wave max over keys = the high word of |a|^2
+// with the low 6 bits replaced by 63 - row: magnitudes within 2^-14 of the largest
+// tie, and the lowest row wins) and the multipliers, and publishes them in LDS (double
 // buffered: one barrier per step); every wave then updates its columns right of k
 // (multipliers zero on the rows already pivoted), reading the pivot row with readlane.
+// Each entry sees the same operations in the same order whatever W.
 template <class T>
-__global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0, int *piv0, int *status) {
+__global__ __launch_bounds__(512) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0, int *piv0, int *status) {
+    constexpr int W = 8, J = NB / W;
     const int z = blockIdx.z;
     const int64_t n = bt.n[z];
     if (k0 >= n) return;
@@ -142,71 +146,69 @@ __global__ __launch_bounds__(256) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     __shared__ lcx<T> lmul[2][NB];
     __shared__ int spiv[2];
-    cx<T> v[16];
+    cx<T> v[J];
     const int64_t row = k0 + min(lane, nb - 1);
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int c = 16 * wave + j;
+    for (int j = 0; j < J; ++j) {
+        const int c = W * j + wave;
         const cx<T> e = A[(k0 + min(c, nb - 1)) * n + row];
         v[j] = (lane < nb && c < nb) ? e : cx<T>();
     }
     int ord = lane < nb ? -1 : NB;  // rows beyond the block never pivot
     int cur = lane;
-    for (int ko = 0; ko < 4 && 16 * ko < nb; ++ko) {
-#pragma unroll
-        for (int kc = 0; kc < 16; ++kc) {
-            const int k = 16 * ko + kc;
-            if (k >= nb) break;
-            const int buf = k & 1;
-            if (wave == ko) {
-                const double m = (double)abs2(v[kc]);
-                const uint32_t key =
-                    ord < 0 ? ((uint32_t)((uint64_t)__double_as_longlong(m) >> 32) & ~63u) | (uint32_t)(63 - lane) : 0u;
-                const int p = 63 - (int)(wave_max_u32(key) & 63u);
-                const cx<T> pv = read_lane(v[kc], p);
-                if (!((double)abs2(pv) > 0.0)) {
-                    if (lane == 0) spiv[buf] = -1;
-                } else {
-                    const cx<T> inv = recip_scaled(pv);
-                    const bool below = ord < 0 && lane != p;
-                    const cx<T> l = below ? v[kc] * inv : cx<T>();
-                    lmul[buf][lane].set(l);
-                    if (below) v[kc] = l;  // the stored L entry
-                    if (lane == 0) spiv[buf] = p;
-                }
+    // fully unrolled (the register slots are indexed by step): steps past nb are skipped
+#pragma clang loop unroll(full)
+    for (int k = 0; k < NB; ++k) {
+        if (k >= nb) continue;
+        const int buf = k & 1, owner = k % W, q = k / W;
+        if (wave == owner) {
+            const double m = (double)abs2(v[q]);
+            const uint32_t key =
+                ord < 0 ? ((uint32_t)((uint64_t)__double_as_longlong(m) >> 32) & ~63u) | (uint32_t)(63 - lane) : 0u;
+            const int p = 63 - (int)(wave_max_u32(key) & 63u);
+            const cx<T> pv = read_lane(v[q], p);
+            if (!((double)abs2(pv) > 0.0)) {
+                if (lane == 0) spiv[buf] = -1;
+            } else {
+                const cx<T> inv = recip_scaled(pv);
+                const bool below = ord < 0 && lane != p;
+                const cx<T> l = below ? v[q] * inv : cx<T>();
+                lmul[buf][lane].set(l);
+                if (below) v[q] = l;  // the stored L entry
+                if (lane == 0) spiv[buf] = p;
             }
-            __syncthreads();
-            // both LDS reads issued together (the multipliers are unused after a zero pivot)
-            const int praw = spiv[buf];
-            const cx<T> l = lmul[buf][lane].get();
-            const int p = __builtin_amdgcn_readfirstlane(praw);
-            if (p < 0) {
-                if (threadIdx.x == 0) status[z] = 1;
-                return;
-            }
-            if (lane == p) ord = k;
-            // the exchange sequence: position j of row p swaps with position k
-            const int j = __ffsll((long long)__ballot(cur == p)) - 1;
-            const int ck = __builtin_amdgcn_readlane(cur, k);
-            cur = lane == j ? ck : (lane == k ? p : cur);
-            if (threadIdx.x == 0) piv[k0 + k] = (int)(k0 + j);
-            // l = 0 on the rows already pivoted and on the pivot row: they keep their
-            // values (v - 0 rp = v for finite pivot rows).  A wave right of column k
-            // updates all its columns, the wave of column k those right of kc (known at
-            // compile time), one left of it none: three straight-line paths.
-            if (wave > ko) {
+        }
+        __syncthreads();
+        // both LDS reads issued together (the multipliers are unused after a zero pivot)
+        const int praw = spiv[buf];
+        const cx<T> l = lmul[buf][lane].get();
+        const int p = __builtin_amdgcn_readfirstlane(praw);
+        if (p < 0) {
+            if (threadIdx.x == 0) status[z] = 1;
+            return;
+        }
+        if (lane == p) ord = k;
+        // the exchange sequence: position j of row p swaps with position k
+        const int j = __ffsll((long long)__ballot(cur == p)) - 1;
+        const int ck = __builtin_amdgcn_readlane(cur, k);
+        cur = lane == j ? ck : (lane == k ? p : cur);
+        if (threadIdx.x == 0) piv[k0 + k] = (int)(k0 + j);
+        // l = 0 on the rows already pivoted and on the pivot row: they keep their
+        // values (v - 0 rp = v for finite pivot rows).  Columns right of k: slots from q
+        // on in the waves after the owner, from q + 1 on in the owner and the waves
+        // before it (two straight-line paths).
+        if (wave > owner) {
 #pragma unroll
-                for (int jj = 0; jj < 16; ++jj) sub_mul(v[jj], l, read_lane(v[jj], p));
-            } else if (wave == ko && kc < 15) {
+            for (int jj = q; jj < J; ++jj) sub_mul(v[jj], l, read_lane(v[jj], p));
+        } else {
 #pragma unroll
-                for (int jj = kc + 1; jj < 16; ++jj) sub_mul(v[jj], l, read_lane(v[jj], p));
-            }
+            for (int jj = q + 1; jj < J; ++jj) sub_mul(v[jj], l, read_lane(v[jj], p));
         }
     }
     if (lane < nb) {
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int c = 16 * wave + j;
+        for (int j = 0; j < J; ++j) {
+            const int c = W * j + wave;
             if (c < nb) A[(k0 + c) * n + k0 + ord] = v[j];
         }
     }
@@ -277,15 +279,16 @@ __global__ __launch_bounds__(256) void k_lu_swap_left(cx<T> *A0, Batch bt, const
     }
 }
 
-// L21 = A21 U11^{-1}, in place.  Each wave solves 8 consecutive rows below the block
+// L21 = A21 U11^{-1}, in place.  Each wave solves R consecutive rows below the block
 // with lane c holding column c of every row (for c128 one 128-byte segment of each
 // column); step k scales entry k by 1 / u_kk and broadcasts it (readlane), and the
 // columns right of k subtract x_k u_kc, with U11 staged in LDS by rows (lane c reads
 // u_kc).  A chain of 64 short steps per wave, all rows in parallel; the sums run over
-// k in ascending order.
-template <class T>
+// k in ascending order.  R = 8 amortises the staging of U11 over many rows; with few
+// rows below the block, R = 2 spreads them over four times the workgroups and
+// shortens each step of the chain.
+template <class T, int R>
 __global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A0, Batch bt, int64_t k0) {
-    constexpr int R = 8;
     const int z = blockIdx.z;
     const int64_t n = bt.n[z];
     const int64_t rw = k0 + NB + (int64_t)blockIdx.x * 4 * R;  // first row of the workgroup
@@ -423,15 +426,16 @@ __global__ void k_lu_permute(const cd *src, cd *dst, int64_t ld, int cols, const
         dst[col * ld + p] = src[col * ld + i];
 }
 
-// Triangular solve with the diagonal block at k0: one workgroup per matrix, wave w
-// solving vector column w (w < cols <= 8), lane r holding row r (FP64 vectors,
-// factors in T).  which = 0: L y = b (unit lower); 1: U x = y; 2: U^H z = g;
+// Triangular solve with the diagonal block at k0: 8-column workgroups (grid.x), wave
+// w solving vector column 8 blockIdx.x + w, lane r holding row r (FP64 vectors,
+// factors in T).  WHICH = 0: L y = b (unit lower); 1: U x = y; 2: U^H z = g;
 // 3: L^H w = z (unit upper).  The block is staged in LDS oriented so that step k
-// reads S[k][lane] (conflict-free): S[k][r] = A(r, k) for which 0 / 1 and
-// S[k][r] = conj(A(k, r)) for which 2 / 3.
-template <class T>
+// reads S[k][lane] (conflict-free): S[k][r] = A(r, k) for WHICH 0 / 1 and
+// S[k][r] = conj(A(k, r)) for WHICH 2 / 3.  The substitution is unrolled by 8, so
+// the LDS reads of 8 steps are issued ahead of their readlane / FMA chain.
+template <class T, int WHICH>
 __global__ __launch_bounds__(512) void k_lu_block_solve(const cx<T> *A0, Batch bt, int64_t k0, cd *x0, int64_t ld,
-                                                        int cols, int which) {
+                                                        int cols) {
     const int z = blockIdx.z;
     const int64_t n = bt.n[z];
     if (k0 >= n) return;
@@ -439,7 +443,7 @@ __global__ __launch_bounds__(512) void k_lu_block_solve(const cx<T> *A0, Batch b
     const int nb = (int)min((int64_t)NB, n - k0);
     __shared__ lcx<T> S[NB][NB];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const bool trans = which >= 2;
+    constexpr bool trans = WHICH >= 2;
 #pragma unroll 8
     for (int t = threadIdx.x; t < nb * nb; t += 512) {
         const int c = t / nb, r = t - c * nb;  // coalesced over r
@@ -450,7 +454,6 @@ __global__ __launch_bounds__(512) void k_lu_block_solve(const cx<T> *A0, Batch b
             S[r][c].set(cx<T>(e.re, -e.im));  // conj(A(r, c)) at S[r][c]: (A^H)(c, r)
     }
     __syncthreads();
-    // 8 columns per workgroup: grid.x selects the chunk of columns
     const int c0 = blockIdx.x * 8;
     if (wave >= cols - c0) return;
     cd *v = x0 + (int64_t)(c0 + wave) * ld + bt.voff[z] + k0;
@@ -462,21 +465,23 @@ __global__ __launch_bounds__(512) void k_lu_block_solve(const cx<T> *A0, Batch b
     // the reciprocal of this lane's diagonal entry, for the non-unit triangles (every
     // lane at once, off the substitution's latency chain)
     cd rdiag(0.0);
-    if ((which == 1 || which == 2) && lane < nb) {
+    if ((WHICH == 1 || WHICH == 2) && lane < nb) {
         const cx<T> e = S[lane][lane].get();
         rdiag = cd(1.0) / cd((double)e.re, (double)e.im);
     }
-    if (which == 0 || which == 2) {
+    if (WHICH == 0 || WHICH == 2) {
         // lower triangular (L, or U^H with S[k][r] = conj(U(k, r)))
-for (int k = 0; k < nb; ++k) {
-            if (which == 2 && lane == k) val = val * rdiag;
+#pragma unroll 8
+        for (int k = 0; k < nb; ++k) {
+            if (WHICH == 2 && lane == k) val = val * rdiag;
             const cd vk = read_lane(val, k);
             if (lane > k && lane < nb) val -= s(k) * vk;
         }
     } else {
         // upper triangular (U, or L^H with S[k][r] = conj(L(k, r)))
-for (int k = nb - 1; k >= 0; --k) {
-            if (which == 1 && lane == k) val = val * rdiag;
+#pragma unroll 8
+        for (int k = nb - 1; k >= 0; --k) {
+            if (WHICH == 1 && lane == k) val = val * rdiag;
             const cd vk = read_lane(val, k);
             if (lane < k) val -= s(k) * vk;
         }
@@ -708,10 +713,14 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
                 A, bt, k, ls.piv, cb, ce);
     };
     auto diag = [&](hipStream_t q, int64_t k) {
-        k_lu_diag<T><<<dim3(1, 1, z), 256, 0, q>>>(A, bt, k, ls.piv, ls.status);
+        k_lu_diag<T><<<dim3(1, 1, z), 512, 0, q>>>(A, bt, k, ls.piv, ls.status);
     };
     auto lower = [&](hipStream_t q, int64_t k) {  // rows below the block at k
-        k_trsm_lower_panel<T><<<dim3((unsigned)((nmax - k - NB + 31) / 32), 1, z), 256, 0, q>>>(A, bt, k);
+        const int64_t rows = nmax - k - NB;
+        if (rows <= 4096)
+            k_trsm_lower_panel<T, 2><<<dim3((unsigned)((rows + 7) / 8), 1, z), 256, 0, q>>>(A, bt, k);
+        else
+            k_trsm_lower_panel<T, 8><<<dim3((unsigned)((rows + 31) / 32), 1, z), 256, 0, q>>>(A, bt, k);
     };
     auto upper = [&](hipStream_t q, int64_t k, int64_t cb, int64_t ce) {
         k_trsm_upper_panel<T><<<dim3((unsigned)std::min<int64_t>((ce - cb + 3) / 4, 1024), 1, z), 256, 0, q>>>(
@@ -986,23 +995,23 @@ static int luset_solve8(LuSet &ls, hipStream_t s, cd *x, int cols, int adjoint) 
         k_lu_permute<<<grid(ld * cols), 256, 0, s>>>(x, work, ld, cols, ls.perm, 0);
         TG_HIP(hipMemcpyAsync(x, work, sizeof(cd) * ld * cols, hipMemcpyDeviceToDevice, s));
         for (int64_t k0 = 0; k0 < nmax; k0 += NB) {
-            k_lu_block_solve<T><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 0);
+            k_lu_block_solve<T, 0><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols);
             if (nmax - k0 - NB > 0)
                 k_lu_panel_rows<T><<<dim3(grid(nmax - k0 - NB), 1, z), 256, 0, s>>>(A, bt, k0, x, ld, cols, 0);
         }
         for (int64_t k0 = last; k0 >= 0; k0 -= NB) {
-            k_lu_block_solve<T><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 1);
+            k_lu_block_solve<T, 1><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols);
             if (k0 > 0) k_lu_panel_rows<T><<<dim3(grid(k0), 1, z), 256, 0, s>>>(A, bt, k0, x, ld, cols, 1);
         }
     } else {
         // A^H = U^H L^H P: U^H z = g, L^H w = z, x = P^T w
         for (int64_t k0 = 0; k0 < nmax; k0 += NB) {
-            k_lu_block_solve<T><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 2);
+            k_lu_block_solve<T, 2><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols);
             if (nmax - k0 - NB > 0)
                 k_lu_panel_cols<T><<<dim3(wgrid(nmax - k0 - NB), 1, z), 256, 0, s>>>(A, bt, k0, x, ld, cols, 2);
         }
         for (int64_t k0 = last; k0 >= 0; k0 -= NB) {
-            k_lu_block_solve<T><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 3);
+            k_lu_block_solve<T, 3><<<dim3(1, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols);
             if (k0 > 0) k_lu_panel_cols<T><<<dim3(wgrid(k0), 1, z), 256, 0, s>>>(A, bt, k0, x, ld, cols, 3);
         }
         k_lu_permute<<<grid(ld * cols), 256, 0, s>>>(x, work, ld, cols, ls.perm, 1);
@@ -1063,7 +1072,7 @@ static int luset_solve_many(LuSet &ls, hipStream_t s, cd *x, int cols) {
     TG_HIP(hipMemcpyAsync(x, tmp.as<cd>(), sizeof(cd) * ld * cols, hipMemcpyDeviceToDevice, s));
     const unsigned cchunks = (unsigned)((cols + 7) / 8), ctiles = (unsigned)((cols + 63) / 64);
     for (int64_t k0 = 0; k0 < nmax; k0 += NB) {
-        k_lu_block_solve<T><<<dim3(cchunks, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 0);
+        k_lu_block_solve<T, 0><<<dim3(cchunks, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols);
         const int64_t below = nmax - k0 - NB;
         if (below > 0)
             k_lu_solve_update<T><<<dim3(ctiles, (unsigned)((below + 63) / 64), z), 256, 0, s>>>(
@@ -1071,7 +1080,7 @@ static int luset_solve_many(LuSet &ls, hipStream_t s, cd *x, int cols) {
     }
     const int64_t last = ((nmax - 1) / NB) * NB;
     for (int64_t k0 = last; k0 >= 0; k0 -= NB) {
-        k_lu_block_solve<T><<<dim3(cchunks, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols, 1);
+        k_lu_block_solve<T, 1><<<dim3(cchunks, 1, z), 512, 0, s>>>(A, bt, k0, x, ld, cols);
         if (k0 > 0)
             k_lu_solve_update<T><<<dim3(ctiles, (unsigned)((k0 + 63) / 64), z), 256, 0, s>>>(A, bt, k0, 0, k0, x, ld,
                                                                                             cols);
