@@ -122,17 +122,21 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // piv[k0 + k] = row exchanged with row k0 + k.  W waves: lane r holds row r, wave w
 // the columns w, w + W, w + 2W, ... in registers (dealt cyclically, so that every
 // wave has about the same number of columns right of any step, and two waves share a
-// SIMD: one's pivot search runs beside the other's update).  Rows never move during
-// the factorisation (implicit pivoting): ord = the step at which a row became the
-// pivot (-1 before), and each row lands at position ord when the block is stored,
-// which is the arrangement of the equivalent exchange sequence piv (tracked in cur =
-// the row at each position).  Step k: the wave owning column k finds the pivot among
-// the rows not yet pivoted (one 32-bit wave max over keys = the high word of |a|^2
+// SIMD).  Rows never move during the factorisation (implicit pivoting): ord = the step
+// at which a row became the pivot (-1 before), and each row lands at position ord
+// when the block is stored, which is the arrangement of the equivalent exchange
+// sequence piv (tracked in cur = the row at each position).  The pivot of column k is
+// found by the wave owning it (one 32-bit wave max over keys = the high word of |a|^2
 // with the low 6 bits replaced by 63 - row: magnitudes within 2^-14 of the largest
-// tie, and the lowest row wins) and the multipliers, and publishes them in LDS (double
-// buffered: one barrier per step); every wave then updates its columns right of k
-// (multipliers zero on the rows already pivoted), reading the pivot row with readlane.
-// Each entry sees the same operations in the same order whatever W.
+// tie, and the lowest row wins), which publishes it with the multipliers in LDS
+// (double buffered: one barrier per step).  Look-ahead: after the barrier of step k
+// the owner of column k + 1 applies step k to that column first and finds its pivot
+// before its other updates (branch-free, so the two interleave); every wave updates
+// its columns right of k (multipliers zero on the rows already pivoted), reading the
+// pivot row with readlane.  Each entry sees the same operations in the same order
+// whatever W.  Measured at lmax 3 (one 64 x 64 block): 2.8 us of loads, stores and
+// launch, 13.6 us of the 64 barrier / LDS steps alone, 9 us of pivot searches and
+// 12 us of updates; a rolled step loop (slots picked by uniform index) is slower.
 template <class T>
 __global__ __launch_bounds__(512) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0, int *piv0, int *status) {
     constexpr int W = 8, J = NB / W;
@@ -156,28 +160,30 @@ __global__ __launch_bounds__(512) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0
     }
     int ord = lane < nb ? -1 : NB;  // rows beyond the block never pivot
     int cur = lane;
+    // pivot of column k (slot q of the calling wave) among the rows not yet pivoted;
+    // a zero pivot publishes -1 and zero multipliers
+    auto pivot = [&](int k, int q) {
+        const int buf = k & 1;
+        const double m = (double)abs2(v[q]);
+        const uint32_t key =
+            ord < 0 ? ((uint32_t)((uint64_t)__double_as_longlong(m) >> 32) & ~63u) | (uint32_t)(63 - lane) : 0u;
+        const int p = 63 - (int)(wave_max_u32(key) & 63u);
+        const cx<T> pv = read_lane(v[q], p);
+        const bool ok = (double)abs2(pv) > 0.0;  // uniform over the wave
+        const cx<T> inv = recip_scaled(ok ? pv : cx<T>((T)1));
+        const bool below = ok && ord < 0 && lane != p;
+        const cx<T> l = below ? v[q] * inv : cx<T>();
+        lmul[buf][lane].set(l);
+        if (below) v[q] = l;  // the stored L entry
+        if (lane == 0) spiv[buf] = ok ? p : -1;
+    };
+    if (wave == 0) pivot(0, 0);
     // fully unrolled (the register slots are indexed by step): steps past nb are skipped
 #pragma clang loop unroll(full)
     for (int k = 0; k < NB; ++k) {
         if (k >= nb) continue;
         const int buf = k & 1, owner = k % W, q = k / W;
-        if (wave == owner) {
-            const double m = (double)abs2(v[q]);
-            const uint32_t key =
-                ord < 0 ? ((uint32_t)((uint64_t)__double_as_longlong(m) >> 32) & ~63u) | (uint32_t)(63 - lane) : 0u;
-            const int p = 63 - (int)(wave_max_u32(key) & 63u);
-            const cx<T> pv = read_lane(v[q], p);
-            if (!((double)abs2(pv) > 0.0)) {
-                if (lane == 0) spiv[buf] = -1;
-            } else {
-                const cx<T> inv = recip_scaled(pv);
-                const bool below = ord < 0 && lane != p;
-                const cx<T> l = below ? v[q] * inv : cx<T>();
-                lmul[buf][lane].set(l);
-                if (below) v[q] = l;  // the stored L entry
-                if (lane == 0) spiv[buf] = p;
-            }
-        }
+        const int k1 = k + 1, owner1 = k1 % W, q1 = k1 / W;
         __syncthreads();
         // both LDS reads issued together (the multipliers are unused after a zero pivot)
         const int praw = spiv[buf];
@@ -196,8 +202,14 @@ __global__ __launch_bounds__(512) void k_lu_diag(cx<T> *A0, Batch bt, int64_t k0
         // l = 0 on the rows already pivoted and on the pivot row: they keep their
         // values (v - 0 rp = v for finite pivot rows).  Columns right of k: slots from q
         // on in the waves after the owner, from q + 1 on in the owner and the waves
-        // before it (two straight-line paths).
-        if (wave > owner) {
+        // before it; in the owner of column k + 1 (a wave after the owner, or wave 0
+        // when k + 1 starts a new round) they start at that column's slot q1.
+        if (wave == owner1 && k1 < nb) {
+            sub_mul(v[q1], l, read_lane(v[q1], p));
+            pivot(k1, q1);
+#pragma unroll
+            for (int jj = q1 + 1; jj < J; ++jj) sub_mul(v[jj], l, read_lane(v[jj], p));
+        } else if (wave > owner) {
 #pragma unroll
             for (int jj = q; jj < J; ++jj) sub_mul(v[jj], l, read_lane(v[jj], p));
         } else {
