@@ -716,9 +716,17 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
             st = TG_DEVICE_ERROR;
         }
     };
-    if (!ls.panel && (st = panel_acquire(ls))) return st;
-    hipStream_t ps = ls.panel;
+    // small operators: everything on the main stream (the look-ahead's event waits cost
+    // more there than the overlap saves)
+    const bool ahead = nmax > 1024;
+    if (ahead && !ls.panel && (st = panel_acquire(ls))) return st;
+    hipStream_t ps = ahead ? ls.panel : s;
     hipEvent_t ready = ls.ready, done = ls.done;
+    auto join = [&](hipEvent_t e, hipStream_t from, hipStream_t to) {  // to waits for from
+        if (!ahead) return;
+        check(hipEventRecord(e, from));
+        check(hipStreamWaitEvent(to, e, 0));
+    };
     auto swap = [&](hipStream_t q, int64_t k, int64_t cb, int64_t ce) {  // one wave per column
         if (ce > cb)
             k_lu_swap<T><<<dim3((unsigned)std::min<int64_t>((ce - cb + 3) / 4, 2048), 1, z), 256, 0, q>>>(
@@ -743,10 +751,7 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
             k_lu_update<T><<<dim3((unsigned)((ce - cb + 63) / 64), (unsigned)((re - rb + 63) / 64), z), 256, 0, q>>>(
                 A, bt, k, K, rb, re, cb, ce);
     };
-    if (st == TG_OK) {
-        check(hipEventRecord(ready, s));  // the panel stream starts after the work before
-        check(hipStreamWaitEvent(ps, ready, 0));
-    }
+    if (st == TG_OK) join(ready, s, ps);  // the panel stream starts after the work before
     for (int64_t p0 = 0; p0 < nmax && st == TG_OK; p0 += 2 * NB) {
         const int64_t p1 = p0 + NB, p2 = p1 + NB, p4 = p2 + 2 * NB;
         const int64_t pe = std::min(p2, nmax);
@@ -762,8 +767,7 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
             if (p2 < nmax) lower(ps, p1);
         }
         check(hipGetLastError());
-        check(hipEventRecord(done, ps));
-        check(hipStreamWaitEvent(s, done, 0));
+        join(done, ps, s);
         if (p2 >= nmax) continue;
         // right of the panel, on the main stream
         swap(s, p0, p2, nmax);
@@ -772,8 +776,7 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
         update(s, p0, NB, p1, p2, p2, nmax);        // rows of b: A12_b -= L_ba U12_a
         upper(s, p1, p2, nmax);                     // U12 of block b
         update(s, p0, 2 * NB, p2, nmax, p2, std::min(p4, nmax));  // the next panel first
-        check(hipEventRecord(ready, s));
-        check(hipStreamWaitEvent(ps, ready, 0));
+        join(ready, s, ps);
         update(s, p0, 2 * NB, p2, nmax, p4, nmax);  // overlaps the next panel
         check(hipGetLastError());
     }
@@ -787,9 +790,8 @@ static int luset_factor_t(LuSet &ls, hipStream_t s) {
     if (st == TG_OK) {
         // the panel stream's last work joins the main stream (the last pair's panel
         // has no main-stream work after it)
-        check(hipEventRecord(done, ps));
-        check(hipStreamWaitEvent(s, done, 0));
-    } else {
+        join(done, ps, s);
+    } else if (ahead) {
         (void)hipStreamSynchronize(ps);
     }
     if (st != TG_OK) return st;
