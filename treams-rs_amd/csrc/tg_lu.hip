@@ -45,6 +45,9 @@ struct tg_factor {
     // assembled (before equilibration): the refinement residual of forward solves is
     // then one GEMM instead of a matrix-free apply
     tg::DevBuf op;
+    // pinned host squares of the right-hand side and residual norms of a solve: the
+    // right-hand side's are read with the first residual's (one synchronisation)
+    tg::HostBuf norms;
     // pivoting: 0 inside 64-row diagonal blocks, 1 over whole columns (GETRF order)
     int pivoting = 0;
     // fallbacks taken: 1 when a stalled or singular solve refactored the operator (in
@@ -1536,15 +1539,22 @@ static int factor_solve_once(tg_factor *f, const cd *b, cd *x, int cols, int adj
     // D_r (b - A x) against D_r b (D_c for the adjoint)
     const double *dnorm = f->equilibrated ? f->eq.as<double>() + (adjoint ? f->n : 0) : nullptr;
     const unsigned sgrid = (unsigned)((count + 255) / 256);
-    auto norms = [&](const cd *v, double *out) -> int {
-        if (!dnorm) return vec_norms(c, v, c->n, cols, c->n, out);
-        TG_HIP(hipMemcpyAsync(ax, v, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream));
-        k_scale_rows<<<sgrid, 256, 0, c->stream>>>(ax, c->n, cols, dnorm);
-        TG_HIP(hipGetLastError());
-        return vec_norms(c, ax, c->n, cols, c->n, out);
+    TG_TRY(f->norms.ensure(sizeof(cd) * 2 * (size_t)cols));
+    cd *bsq = f->norms.as<cd>(), *rsq = bsq + cols;
+    // squared norms into pinned host memory, read after the next synchronisation
+    auto norms = [&](const cd *v, cd *host) -> int {
+        if (dnorm) {
+            TG_HIP(hipMemcpyAsync(ax, v, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream));
+            k_scale_rows<<<sgrid, 256, 0, c->stream>>>(ax, c->n, cols, dnorm);
+            TG_HIP(hipGetLastError());
+            v = ax;
+        }
+        return vec_dots_dev(c, v, 0, 1, v, c->n, cols, c->n, c->n, nullptr, nullptr, host);
     };
+    // a NaN stays NaN (a clamp would turn it into 0 and pass a non-finite vector as zero)
+    auto norm = [](cd d) { return d.re > 0.0 ? sqrt(d.re) : d.re == d.re ? 0.0 : d.re; };
     do {
-        if ((st = norms(b, bn.data()))) break;
+        if ((st = norms(b, bsq))) break;
         if (hipMemcpyAsync(x, b, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
             set_error("copy of the right-hand side failed");
             st = TG_DEVICE_ERROR;
@@ -1584,7 +1594,16 @@ static int factor_solve_once(tg_factor *f, const cd *b, cd *x, int cols, int adj
                 break;
             }
             if ((st = vec_axpby(c->stream, cd(-1.0), ax, cd(1.0), r, count))) break;
-            if ((st = norms(r, rn.data()))) break;
+            if ((st = norms(r, rsq))) break;
+            if (hipStreamSynchronize(c->stream) != hipSuccess) {
+                set_error("residual norms of the LU solve failed");
+                st = TG_DEVICE_ERROR;
+                break;
+            }
+            for (int k = 0; k < cols; ++k) {
+                if (refinements == 0) bn[k] = norm(bsq[k]);
+                rn[k] = norm(rsq[k]);
+            }
             bool done = true;
             for (int k = 0; k < cols; ++k) {
                 conv[k].residual_norm = rn[k];
@@ -1666,6 +1685,7 @@ int tg_factor_destroy(tg_factor *f) {
     f->lu.release();
     f->eq.release();
     f->op.release();
+    f->norms.release();
     luset_release(f->set);
     if (f->dense) {
         for (auto &w : f->cluster->ws) w.release();
@@ -1762,13 +1782,13 @@ int tg_factor_solve(tg_factor *f, const tg_c128 *incident, int cols, double rtol
     }
     if (!st) st = factor_solve_split(f, b, xs, cols, adjoint, rtol, conv);
     if (!st) st = split_to_ref(c, xs, (cd *)x, cols);
-    if (!st && hipStreamSynchronize(c->stream) != hipSuccess) {
+    const bool idle = hipStreamSynchronize(c->stream) == hipSuccess;
+    if (!st && !idle) {
         set_error("device failure in the LU solve");
         st = TG_DEVICE_ERROR;
     }
     if (conv_host)
         for (int k = 0; k < cols && k < (int)conv.size(); ++k) conv_host[k] = conv[k];
-    const bool idle = hipStreamSynchronize(c->stream) == hipSuccess;
     if (idle) {
         CleanReleases clean;
         io.release();
