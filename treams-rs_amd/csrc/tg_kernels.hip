@@ -82,22 +82,22 @@ __device__ __forceinline__ void entry_values(const DevPlan &pl, int e, const cd 
 
 // One thread per sphere: 2x2 helicity Mie blocks (mode order, row-major) of every
 // degree and their derivatives along the size parameter and the permittivity.
+// One thread per (sphere, degree): the helicity blocks of the Mie coefficients and
+// their size and permittivity derivatives (NaN for a zero denominator).
 __global__ void k_mie(int N, int lmax, double k0, const double *radii, const double *eps, cd *val, cd *dsz, cd *deps) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    tg_dual a[kMaxLmax], b[kMaxLmax];
-    int rc = tg_mie_bh(lmax, k0 * radii[i], cd(eps[2 * i], eps[2 * i + 1]), a, b);
-    for (int l = 0; l < lmax; ++l) {
-        cd d = -0.5 * (a[l].v + b[l].v), o = -0.5 * (a[l].v - b[l].v);
-        cd ddx = -0.5 * (a[l].dx + b[l].dx), odx = -0.5 * (a[l].dx - b[l].dx);
-        cd dde = -0.5 * (a[l].de + b[l].de), ode = -0.5 * (a[l].de - b[l].de);
-        if (rc) d = o = ddx = odx = dde = ode = cd(NAN, NAN);
-        cd *v = val + (size_t)(i * lmax + l) * 4, *x = dsz + (size_t)(i * lmax + l) * 4,
-           *e = deps + (size_t)(i * lmax + l) * 4;
-        v[0] = d; v[1] = o; v[2] = o; v[3] = d;
-        x[0] = ddx; x[1] = odx; x[2] = odx; x[3] = ddx;
-        e[0] = dde; e[1] = ode; e[2] = ode; e[3] = dde;
-    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * lmax) return;
+    const int i = t / lmax, l = t - i * lmax;
+    tg_dual a, b;
+    const int rc = tg_mie_bh(l + 1, lmax, k0 * radii[i], cd(eps[2 * i], eps[2 * i + 1]), a, b);
+    cd d = -0.5 * (a.v + b.v), o = -0.5 * (a.v - b.v);
+    cd ddx = -0.5 * (a.dx + b.dx), odx = -0.5 * (a.dx - b.dx);
+    cd dde = -0.5 * (a.de + b.de), ode = -0.5 * (a.de - b.de);
+    if (rc) d = o = ddx = odx = dde = ode = cd(NAN, NAN);
+    cd *v = val + (size_t)t * 4, *x = dsz + (size_t)t * 4, *e = deps + (size_t)t * 4;
+    v[0] = d; v[1] = o; v[2] = o; v[3] = d;
+    x[0] = ddx; x[1] = odx; x[2] = odx; x[3] = ddx;
+    e[0] = dde; e[1] = ode; e[2] = ode; e[3] = dde;
 }
 
 int launch_mie(tg_cluster *c) {
@@ -106,7 +106,8 @@ int launch_mie(tg_cluster *c) {
     const double *eps = radii + c->N;
     cd *val = c->b_mie.as<cd>();
     size_t blk = (size_t)c->N * c->lmax * 4;
-    k_mie<<<(c->N + 63) / 64, 64, 0, c->stream>>>(c->N, c->lmax, c->k0, radii, eps, val, val + blk, val + 2 * blk);
+    k_mie<<<(c->N * c->lmax + 63) / 64, 64, 0, c->stream>>>(c->N, c->lmax, c->k0, radii, eps, val, val + blk,
+                                                             val + 2 * blk);
     TG_HIP(hipGetLastError());
     return TG_OK;
 }

@@ -83,53 +83,59 @@ TG_HD tg_dual operator/(tg_dual a, tg_dual b) {
 }
 TG_HD tg_dual operator/(double s, tg_dual b) { return tg_dual(cd(s)) / b; }
 
-// Mie coefficients a_n, b_n (n = 1..lmax) of a nonmagnetic sphere of size x and
-// relative permittivity eps in vacuum, with derivatives along x and eps.
-// lmax <= 40.  Returns 0, or -1 for a zero denominator.
-TG_HD int tg_mie_bh(int lmax, double xv, cd eps, tg_dual *a, tg_dual *b) {
+// Mie coefficients a_l, b_l of one degree l (1 <= l <= lmax <= 40) of a nonmagnetic
+// sphere of size x and relative permittivity eps in vacuum, with derivatives along x
+// and eps (Bohren and Huffman's recurrences; the starting degrees depend on lmax).
+// Each degree is computed alone, without arrays:
+// * D_l(mx), the logarithmic derivative, downward from zero at degree
+//   max(lmax, |mx|) + 16;
+// * psi_n(x) = x j_n(x) as psi_0 (or psi_1 in closed form) times the ratios
+//   psi_n / psi_{n-1}, themselves downward from zero at degree lmax + x + 40;
+// * xi_n(x) = x h_n(x) upward from xi_0, xi_-1.
+// Returns 0, or -1 for a zero denominator.
+TG_HD int tg_mie_bh(int l, int lmax, double xv, cd eps, tg_dual &a, tg_dual &b) {
     cd mv = csqrt_(eps);
     tg_dual x(cd(xv), cd(1.0), cd(0.0));
     tg_dual m(mv, cd(0.0), cd(0.5) / mv);  // dm/deps = 1/(2m)
     tg_dual mx = m * x;
-    // D_n(mx) downward from zero.
     double amx = cabs_(mx.v);
     int nstart = (lmax > (int)amx ? lmax : (int)amx) + 16;
-    tg_dual D[48];
     tg_dual d(cd(0.0));
-    for (int n = nstart; n >= 1; --n) {
+    for (int n = nstart; n > l; --n) {
         tg_dual nz = (double)n / mx;
-        if (n <= lmax) D[n] = d;
         d = nz - 1.0 / (d + nz);
     }
-    // psi_n(x) = x j_n(x) from the ratio recurrence, xi_n(x) = x h_n(x) upward.
+    const tg_dual Dl = d;
+    // ratios r_n = psi_n / psi_{n-1}; products of r_2 .. r_l and r_2 .. r_{l-1}
     int jstart = lmax + (int)xv + 40;
-    tg_dual r(cd(0.0));
-    tg_dual ratio[48];
+    tg_dual r(cd(0.0)), q_l(cd(1.0), cd(0.0), cd(0.0)), q_lm(cd(1.0), cd(0.0), cd(0.0)), r1;
     for (int n = jstart; n >= 1; --n) {
         r = x / (tg_dual(cd((double)(2 * n + 1))) - x * r);
-        if (n <= lmax) ratio[n] = r;
+        if (n >= 2 && n <= l) q_l = q_l * r;
+        if (n >= 2 && n <= l - 1) q_lm = q_lm * r;
+        if (n == 1) r1 = r;
     }
     double s = sin(xv), c = cos(xv);
-    tg_dual psi_prev(cd(s), cd(c), cd(0.0));                   // psi_0 = sin x
-    tg_dual xi_prev(cd(s, -c), cd(c, s), cd(0.0));             // xi_0 = -i e^{ix}
-    tg_dual xi_prev2(cd(c, s), cd(-s, c), cd(0.0));            // xi_-1 = e^{ix}
+    tg_dual psi0(cd(s), cd(c), cd(0.0));                   // psi_0 = sin x
+    tg_dual xi_prev(cd(s, -c), cd(c, s), cd(0.0));         // xi_0 = -i e^{ix}
+    tg_dual xi_prev2(cd(c, s), cd(-s, c), cd(0.0));        // xi_-1 = e^{ix}
     // psi_1 = sin x / x - cos x in closed form where it exceeds psi_0 (near x = k pi
     // the ratio chain would scale a vanishing psi_0)
     const double p1 = s / xv - c;
     const bool psi1_closed = fabs(p1) > fabs(s);
-    const tg_dual psi1(cd(p1), cd(c / xv - s / (xv * xv) + s), cd(0.0));
-    for (int n = 1; n <= lmax; ++n) {
-        tg_dual psi = (n == 1 && psi1_closed) ? psi1 : psi_prev * ratio[n];
+    const tg_dual psi1 = psi1_closed ? tg_dual(cd(p1), cd(c / xv - s / (xv * xv) + s), cd(0.0)) : psi0 * r1;
+    const tg_dual psi = psi1 * q_l, psi_prev = l == 1 ? psi0 : psi1 * q_lm;
+    for (int n = 1; n < l; ++n) {
         tg_dual xi = ((double)(2 * n - 1) / x) * xi_prev - xi_prev2;
-        tg_dual nx = (double)n / x;
-        tg_dual ga = D[n] / m + nx, gb = m * D[n] + nx;
-        tg_dual da = ga * xi - xi_prev, db = gb * xi - xi_prev;
-        if (abs2(da.v) == 0.0 || abs2(db.v) == 0.0) return -1;
-        a[n - 1] = (ga * psi - psi_prev) / da;
-        b[n - 1] = (gb * psi - psi_prev) / db;
-        psi_prev = psi;
         xi_prev2 = xi_prev;
         xi_prev = xi;
     }
+    tg_dual xi = ((double)(2 * l - 1) / x) * xi_prev - xi_prev2;
+    tg_dual nx = (double)l / x;
+    tg_dual ga = Dl / m + nx, gb = m * Dl + nx;
+    tg_dual da = ga * xi - xi_prev, db = gb * xi - xi_prev;
+    if (abs2(da.v) == 0.0 || abs2(db.v) == 0.0) return -1;
+    a = (ga * psi - psi_prev) / da;
+    b = (gb * psi - psi_prev) / db;
     return 0;
 }
