@@ -1485,6 +1485,44 @@ __global__ __launch_bounds__(256, 2) void k_zgemm_plain(const cx<T> *A, int64_t 
             }
 }
 
+// Slice z of a split-K product: P_z = A[:, z ks : (z + 1) ks] B[z ks : (z + 1) ks, :]
+// (the last slice ragged) into P + z pstride, one 64 x 64 tile per workgroup.  Small
+// products (few tiles) run as several slices side by side instead of one long K loop
+// per tile.
+template <class T>
+__global__ __launch_bounds__(256, 2) void k_zgemm_splitk(const cx<T> *A, int64_t lda, const cx<T> *B, int64_t ldb,
+                                                         cx<T> *P, int64_t ldp, int64_t pstride, int64_t M,
+                                                         int64_t N, int K, int ks) {
+    const int z = blockIdx.z, k0 = z * ks, kz = min(ks, K - k0);
+    if (kz <= 0) return;  // uniform over the workgroup
+    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    __shared__ TileLds<T> sm;
+    Tile3M<T> acc;
+    zgemm_tile_3m<T, true>(A + (int64_t)k0 * lda, lda, B + k0, ldb, r0, c0, M, N, kz, sm, acc);
+    cx<T> *C = P + (int64_t)z * pstride;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32, rr = lane & 15;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t gr = r0 + wm + a * 16 + rr, gc = c0 + wn + b * 16 + mfma_out_row<T>(lane, v);
+                const T t1 = acc.t1[a][b][v], t2 = acc.t2[a][b][v], t3 = acc.t3[a][b][v];
+                if (gr < M && gc < N) C[gc * ldp + gr] = cx<T>(t1 - t2, t3 - t1 - t2);
+            }
+}
+
+// r = b - sum over the S slices of P (slices in order).
+__global__ void k_residual_slices(const cd *b, const cd *P, int64_t count, int S, int64_t pstride, cd *r) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    cd s = b[t];
+    for (int z = 0; z < S; ++z) s -= P[(int64_t)z * pstride + t];
+    r[t] = s;
+}
+
 // y = A^H x for the stored FP64 operator (n x n, column-major): one workgroup per
 // output row j = column j of A, reduced over its contiguous entries, `cols` columns.
 __global__ __launch_bounds__(256) void k_gemv_adjoint(const cd *A, int64_t n, const cd *x, cd *y, int cols) {
@@ -1529,8 +1567,18 @@ static int factor_solve_once(tg_factor *f, const cd *b, cd *x, int cols, int adj
                              std::vector<tg_convergence> &conv) {
     tg_cluster *c = f->cluster;
     const int64_t count = (int64_t)cols * c->n;
+    // the forward residual with the stored operator: split-K slices so that about 128
+    // tiles run at once (at most 8 slices of a multiple of 16 columns)
+    const int64_t n = f->n;
+    int S = 1, ks = (int)n;
+    if (f->op.ptr && !adjoint) {
+        const int64_t tiles = ((cols + 63) / 64) * ((n + 63) / 64);
+        S = (int)std::min<int64_t>(8, std::max<int64_t>(1, 128 / tiles));
+        ks = (int)(((n + S - 1) / S + 15) / 16 * 16);
+        S = (int)((n + ks - 1) / ks);
+    }
     DevBuf tmp;
-    TG_TRY(tmp.ensure(sizeof(cd) * count * 2));
+    TG_TRY(tmp.ensure(sizeof(cd) * count * (1 + std::max(S, 1))));
     cd *r = tmp.as<cd>(), *ax = r + count;
     std::vector<double> bn(cols), rn(cols);
     int st = TG_OK;
@@ -1565,14 +1613,12 @@ static int factor_solve_once(tg_factor *f, const cd *b, cd *x, int cols, int adj
         const int max_refine = f->precision == TG_STORE_C64 ? 12 : 4;
         while (true) {
             if (f->op.ptr && !adjoint) {
-                // ax = A x with the stored FP64 operator (n x n times n x cols)
-                const int64_t n = f->n;
-                const dim3 grid((unsigned)((cols + 63) / 64), (unsigned)((n + 63) / 64));
-                const cd *op = f->op.as<cd>();
-                if (n % 16)
-                    k_zgemm_plain<double, true><<<grid, 256, 0, c->stream>>>(op, n, x, n, ax, n, n, cols, (int)n);
-                else
-                    k_zgemm_plain<double, false><<<grid, 256, 0, c->stream>>>(op, n, x, n, ax, n, n, cols, (int)n);
+                // r = b - A x with the stored FP64 operator (n x n times n x cols), the
+                // product in S split-K slices (ax .. ax + S count)
+                const dim3 grid((unsigned)((cols + 63) / 64), (unsigned)((n + 63) / 64), (unsigned)S);
+                k_zgemm_splitk<double><<<grid, 256, 0, c->stream>>>(f->op.as<cd>(), n, x, n, ax, n, count, n, cols,
+                                                                    (int)n, ks);
+                k_residual_slices<<<(unsigned)((count + 255) / 256), 256, 0, c->stream>>>(b, ax, count, S, count, r);
                 if (hipGetLastError() != hipSuccess) {
                     set_error("launch of the residual GEMM failed");
                     st = TG_DEVICE_ERROR;
@@ -1588,12 +1634,14 @@ static int factor_solve_once(tg_factor *f, const cd *b, cd *x, int cols, int adj
             } else if ((st = apply_operator(c, x, ax, cols, adjoint, TG_OP_MATRIX_FREE_F64))) {
                 break;
             }
-            if (hipMemcpyAsync(r, b, sizeof(cd) * count, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
-                set_error("copy of the right-hand side failed");
-                st = TG_DEVICE_ERROR;
-                break;
+            if (!(f->op.ptr && !adjoint)) {
+                k_residual_slices<<<(unsigned)((count + 255) / 256), 256, 0, c->stream>>>(b, ax, count, 1, count, r);
+                if (hipGetLastError() != hipSuccess) {
+                    set_error("launch of the residual failed");
+                    st = TG_DEVICE_ERROR;
+                    break;
+                }
             }
-            if ((st = vec_axpby(c->stream, cd(-1.0), ax, cd(1.0), r, count))) break;
             if ((st = norms(r, rsq))) break;
             if (hipStreamSynchronize(c->stream) != hipSuccess) {
                 set_error("residual norms of the LU solve failed");
