@@ -1230,13 +1230,24 @@ __global__ __launch_bounds__(256) void k_bal_cols(const cx<T> *S, int64_t n, con
     if (threadIdx.x == 0) bal_advance(y[c], red[0] * y[c], flags);
 }
 
-// Row pass: x_i from sum_j |s_ij| y_j; one thread per row (coalesced over rows).
+// Row pass: x_i from sum_j |s_ij| y_j.  Partial sums over column chunks (blockIdx.y),
+// one thread per row (coalesced over rows), then the chunks added in order per row.
 template <class T>
-__global__ void k_bal_rows(const cx<T> *S, int64_t n, double *x, const double *y, int *flags) {
+__global__ void k_bal_rows_part(const cx<T> *S, int64_t n, int chunk, const double *y, double *part) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t c0 = (int64_t)blockIdx.y * chunk, c1 = min(c0 + chunk, n);
+    double sum = 0.0;
+#pragma unroll 4
+    for (int64_t c = c0; c < c1; ++c) sum += eq_mag(S[c * n + r]) * y[c];
+    part[(int64_t)blockIdx.y * n + r] = sum;
+}
+
+__global__ void k_bal_rows(const double *part, int parts, int64_t n, double *x, int *flags) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     double sum = 0.0;
-    for (int64_t c = 0; c < n; ++c) sum += eq_mag(S[c * n + r]) * y[c];
+    for (int q = 0; q < parts; ++q) sum += part[(int64_t)q * n + r];
     bal_advance(x[r], sum * x[r], flags);
 }
 
@@ -1257,17 +1268,22 @@ constexpr int kBalanceSweeps = 64;  // dense operators take at most three (linal
 // sum or a factor beyond 2^+-256 keeps the max scales.
 template <class T>
 static int balance_t(cx<T> *S, int64_t n, double *dr, double *dc, hipStream_t s) {
+    // column chunks of the row pass: about 2048 workgroups on small operators
+    const int64_t rblocks = (n + 255) / 256;
+    const int chunk = (int)std::min<int64_t>(256, std::max<int64_t>(4, (n * rblocks + 2047) / 2048));
+    const int parts = (int)((n + chunk - 1) / chunk);
     DevBuf work;
-    TG_TRY(work.ensure(sizeof(double) * 2 * (size_t)n + 256));
-    double *x = work.as<double>(), *y = x + n;
-    int *flags = reinterpret_cast<int *>(y + n);
+    TG_TRY(work.ensure(sizeof(double) * (2 + (size_t)parts) * (size_t)n + 256));
+    double *x = work.as<double>(), *y = x + n, *part = y + n;
+    int *flags = reinterpret_cast<int *>(part + (size_t)parts * n);
     const unsigned g = (unsigned)((n + 255) / 256);
     k_fill_f64<<<(unsigned)((2 * n + 255) / 256), 256, 0, s>>>(x, 2 * n, 1.0);
     int hf[2] = {0, 0};
     for (int sweep = 0; sweep < kBalanceSweeps; ++sweep) {
         k_fill_i32<<<1, 64, 0, s>>>(flags, 2, 0);
         k_bal_cols<T><<<(unsigned)n, 256, 0, s>>>(S, n, x, y, flags);
-        k_bal_rows<T><<<g, 256, 0, s>>>(S, n, x, y, flags);
+        k_bal_rows_part<T><<<dim3(g, (unsigned)parts), 256, 0, s>>>(S, n, chunk, y, part);
+        k_bal_rows<<<g, 256, 0, s>>>(part, parts, n, x, flags);
         TG_HIP(hipGetLastError());
         TG_HIP(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, s));
         TG_HIP(hipStreamSynchronize(s));
