@@ -74,10 +74,19 @@ TG_HD tg_dual operator*(tg_dual a, tg_dual b) {
     return {a.v * b.v, a.dx * b.v + a.v * b.dx, a.de * b.v + a.v * b.de};
 }
 TG_HD tg_dual operator*(double s, tg_dual a) { return {s * a.v, s * a.dx, s * a.de}; }
-// One complex reciprocal for the value and both derivatives (the recurrences below
-// are chains of these divisions).
+// 1 / z with one real division: z scaled exactly by the power of two 2^-e that brings
+// its larger component into [0.5, 1), so |z 2^-e|^2 lies in [0.25, 2] whatever |z|.
+TG_HD cd crecip_scaled(cd z) {
+    int e;
+    (void)frexp(fmax(fabs(z.re), fabs(z.im)), &e);
+    const double a = ldexp(z.re, -e), b = ldexp(z.im, -e);
+    const double r = 1.0 / (a * a + b * b);
+    return cd(ldexp(a * r, -e), ldexp(-b * r, -e));
+}
+// One complex reciprocal (one real division) for the value and both derivatives (the
+// recurrences below are chains of these divisions).
 TG_HD tg_dual operator/(tg_dual a, tg_dual b) {
-    const cd r = cd(1.0) / b.v;
+    const cd r = crecip_scaled(b.v);
     const cd q = a.v * r;
     return {q, (a.dx - q * b.dx) * r, (a.de - q * b.de) * r};
 }
