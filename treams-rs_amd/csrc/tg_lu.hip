@@ -1268,9 +1268,9 @@ constexpr int kBalanceSweeps = 64;  // dense operators take at most three (linal
 // sum or a factor beyond 2^+-256 keeps the max scales.
 template <class T>
 static int balance_t(cx<T> *S, int64_t n, double *dr, double *dc, hipStream_t s) {
-    // column chunks of the row pass: about 2048 workgroups on small operators
+    // column chunks of the row pass: about 2048 workgroups, at least 32 columns each
     const int64_t rblocks = (n + 255) / 256;
-    const int chunk = (int)std::min<int64_t>(256, std::max<int64_t>(4, (n * rblocks + 2047) / 2048));
+    const int chunk = (int)std::min<int64_t>(256, std::max<int64_t>(32, (n * rblocks + 2047) / 2048));
     const int parts = (int)((n + chunk - 1) / chunk);
     DevBuf work;
     TG_TRY(work.ensure(sizeof(double) * (2 + (size_t)parts) * (size_t)n + 256));
@@ -1327,10 +1327,11 @@ static int equilibrate_t(tg_factor *f) {
     hipStream_t s = c->stream;
     const int64_t n = f->n;
     const cx<T> *A = reinterpret_cast<const cx<T> *>(f->lu.ptr);
-    // columns per row-partial: enough partials for about 2048 workgroups on small
-    // operators (a 240-row operator otherwise runs one workgroup over all columns)
+    // columns per row-partial: enough partials for about 2048 workgroups, at least 32
+    // columns each (a 240-row operator otherwise runs one workgroup over all columns;
+    // narrower chunks make the in-order reduction of the partials the slow part)
     const int64_t rblocks = (n + 255) / 256;
-    const int chunk = (int)std::min<int64_t>(256, std::max<int64_t>(4, (n * rblocks + 2047) / 2048));
+    const int chunk = (int)std::min<int64_t>(256, std::max<int64_t>(32, (n * rblocks + 2047) / 2048));
     const int parts = (int)((n + chunk - 1) / chunk);
     DevBuf &work = f->eqwork;
     TG_TRY(work.ensure(sizeof(double) * ((size_t)2 * parts * n + 6 * (size_t)n)));
