@@ -321,6 +321,7 @@ __global__ __launch_bounds__(256) void k_trsm_lower_panel(cx<T> *A0, Batch bt, i
     cx<T> a[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r] = A[(k0 + lane) * n + min(r0 + r, n - 1)];
+#pragma unroll 8
     for (int k = 0; k < NB; ++k) {
         const cx<T> u = U[k][lane].get();
         const cx<T> rk = read_lane(rd, k);
@@ -361,7 +362,7 @@ __global__ __launch_bounds__(256) void k_trsm_upper_panel(cx<T> *A0, Batch bt, i
          col += waves) {
         cx<T> *c = A + col * n + k0;
         cx<T> v = c[lane];
-#pragma unroll 4
+#pragma unroll 8
         for (int k = 0; k < nb - 1; ++k) {
             const cx<T> ck = read_lane(v, k);
             if (lane > k) v -= L[k][lane].get() * ck;
