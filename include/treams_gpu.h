@@ -463,17 +463,29 @@ int tg_factor_pullback_local(tg_factor *factor, const tg_c128 *incident_dev, con
  * ------------------------------------------------------------------------- */
 
 /*
- * Electric field of outgoing (singular) helicity vector spherical waves about every
- * sphere of the cluster, sum_i sum_modes x[i M + mode] wave_mode(point - p_i), at
- * `npoints` points (device FP64, xyz per point) for `cols` coefficient columns
- * (device c128, n x cols, reference layout): out_dev is (3 npoints) x cols
- * column-major, rows 3 p + axis.  Replaces fields::field (fields.rs:947) with the
- * cluster's multi-centre helicity basis (Wave.efield of a scattered wave,
- * python/treams_rs/_waves.py).  Points at a sphere centre give NaN (the reference
- * raises there).  regular != 0 (incoming waves) is not built: TG_INVALID_INPUT.
+ * Electric field of helicity vector spherical waves about every sphere of the
+ * cluster, sum_i sum_modes x[i M + mode] wave_mode(point - p_i), at `npoints` points
+ * (device FP64, xyz per point) for `cols` coefficient columns (device c128, n x cols,
+ * reference layout): out_dev is (3 npoints) x cols column-major, rows 3 p + axis.
+ * regular = 0: outgoing (singular) waves, h_l; points at a sphere centre give NaN
+ * (the reference raises there).  regular != 0: regular (incoming) waves, j_l, finite
+ * everywhere (the series of j_l(kr) / r^l below kr = 1/2, fields.rs:140-180).
+ * Replaces fields::field (fields.rs:947) with the cluster's multi-centre helicity
+ * basis (Wave.efield, python/treams_rs/_waves.py).
  */
 int tg_cluster_field(tg_cluster *cluster, const tg_c128 *x_dev, int cols, int64_t npoints, const double *points_dev,
                      int regular, tg_c128 *out_dev);
+
+/*
+ * The same for one expansion of degree up to lmax (1 .. 12) about `center` (host,
+ * xyz): x_dev holds 2 lmax (lmax + 2) modes per column in the reference order (a
+ * global T-matrix's incident or scattered coefficients, sw/expansion.rs:116, or a
+ * plane wave's regular expansion).  Waves of wavenumber k0 in vacuum, outgoing
+ * (regular = 0) or regular.  Replaces fields::field (fields.rs:947) for a
+ * single-origin SphericalBasis.
+ */
+int tg_expansion_field(int lmax, double k0, const double *center, const tg_c128 *x_dev, int cols, int64_t npoints,
+                       const double *points_dev, int regular, tg_c128 *out_dev, void *stream);
 
 /*
  * Field of a unit helicity plane wave, (N + (2 pol - 1) M) / sqrt(2) exp(i k.r), at

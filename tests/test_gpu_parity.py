@@ -539,6 +539,63 @@ def test_fields_match_oracle(pkg, oracle, golden_dir):
         assert _rel(pkg.plane_wave_field(ref.k0, d, p, pts), oracle.plane_wave_field(ref.k0, d, p, pts)) <= 1e-13
 
 
+def test_regular_field_reproduces_plane_wave(pkg):
+    """Regular waves against a closed form: a plane wave's regular expansion about a
+    centre (degrees to 10), evaluated as a regular field, is the plane wave itself at
+    k |r - c| <= 1 (truncation below 1e-10), including the centre (the series of
+    j_l(kr) / r^l) and points on both sides of k r = 1/2."""
+    k0, c = 1.3, np.array([0.4, -0.2, 0.7])
+    cl = pkg.IterativeSphereCluster(10, k0, [0.2], 4 + 0.1j, [c])
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=(40, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    rad = np.concatenate([[0.0], rng.uniform(0.0, 1.0, size=39)]) / k0
+    pts = c + u * rad[:, None]
+    for d, pol in (([0, 0, 1], 1), ([0.3, -0.5, 0.8], 0), ([-1, 0.2, 0.1], 1)):
+        a = cl.plane_wave(d, pol)
+        got = cl.field(a, pts, regular=True).cpu().numpy()
+        want = pkg.plane_wave_field(k0, d, pol, pts).cpu().numpy()
+        assert np.isfinite(got).all()
+        err = np.abs(got - want).max() / np.abs(want).max()
+        print("regular field vs plane wave", d, pol, f"{err:.2e}")
+        assert err <= 1e-9, (d, pol, err)
+
+
+def test_expansion_field_matches_cluster_field(pkg):
+    """One expansion about a centre (tg_expansion_field) against a one-sphere cluster
+    there: the same waves, outgoing and regular, several columns; and the scattered
+    field of a two-sphere cluster's global T-matrix (degree 12 about the origin)
+    against the cluster's multi-centre scattered field outside the circumscribing
+    sphere."""
+    k0, c, lmax = 1.1, np.array([0.3, 0.1, -0.4]), 5
+    cl = pkg.IterativeSphereCluster(lmax, k0, [0.3], 3 + 0.2j, [c])
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=(cl.n, 3)) + 1j * rng.normal(size=(cl.n, 3))
+    pts = c + rng.normal(size=(50, 3)) * 2.0
+    for regular in (False, True):
+        a = cl.field(x, pts, regular=regular)
+        b = pkg.expansion_field(lmax, k0, x, pts, regular=regular, center=c)
+        assert _rel(b, a) <= 1e-15, regular
+    # global expansion of a solved two-sphere cluster
+    pos = np.array([[0, 0, -0.6], [0, 0, 0.6]])
+    two = pkg.IterativeSphereCluster(4, k0, [0.3, 0.35], 4 + 0.1j, pos)
+    f = pkg.SphereClusterFactor(two, "c128")
+    ainc = two.plane_wave([0.2, 0.1, 1.0], 1)
+    xs, _ = f.solve(ainc)
+    Tg, _ = f.global_tmatrix(12)
+    f.close()
+    one = pkg.IterativeSphereCluster(12, k0, [0.1], 4 + 0.1j, [[0, 0, 0]])
+    ag = one.plane_wave([0.2, 0.1, 1.0], 1)
+    pg = torch.as_tensor(Tg).to(ag.device) @ ag
+    u = rng.normal(size=(30, 3))
+    far = u / np.linalg.norm(u, axis=1, keepdims=True) * 4.0
+    got = pkg.expansion_field(12, k0, pg, far).cpu().numpy()
+    want = two.field(xs, far).cpu().numpy()
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print("global expansion field", f"{err:.2e}")
+    assert err <= 1e-8  # measured 3.7e-10
+
+
 def test_cluster_example_intensity_map_golden(pkg, oracle, golden_dir):
     """docs/examples/cluster.py: total-field intensity 0.5 |E_inc + E_sca|^2 on the 7 x 7
     grid at y = 0 from the GPU solve and field kernels, against

@@ -19,6 +19,7 @@ from .cluster import (  # noqa: F401
     particle_cluster,
     plan_terms,
     plane_wave_field,
+    expansion_field,
     scattering_gradient,
     sphere_cluster_factor,
     translation_blocks,

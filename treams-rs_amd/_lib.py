@@ -39,7 +39,7 @@ EXPORTED = [
     "tg_mlp_get_params", "tg_mlp_set_params", "tg_mlp_get_grads", "tg_mlp_zero_grad", "tg_mlp_forward",
     "tg_mlp_backward", "tg_mlp_adam_step", "tg_factor_create", "tg_factor_destroy", "tg_factor_solve",
     "tg_factor_pullback", "tg_factor_equilibrated", "tg_factor_global_tmatrix", "tg_cluster_field",
-    "tg_plane_wave_field", "tg_translation_blocks_rect", "tg_translation_blocks_tangent", "tg_cluster_pushforward",
+    "tg_plane_wave_field", "tg_expansion_field", "tg_translation_blocks_rect", "tg_translation_blocks_tangent", "tg_cluster_pushforward",
     "tg_factor_pushforward", "tg_factor_info", "tg_dense_factor_create", "tg_dense_factor_solve",
     "tg_cluster_pushforward_local", "tg_factor_pushforward_local", "tg_cluster_pullback_local",
     "tg_factor_pullback_local",
@@ -145,6 +145,7 @@ def lib():
         "tg_translation_blocks_rect": (_i, [_i, _i, _d, _i64, _vp, _i, _vp, _vp]),
         "tg_cluster_field": (_i, [_vp, _vp, _i, _i64, _vp, _i, _vp]),
         "tg_plane_wave_field": (_i, [_d, _vp, _i, _i64, _vp, _vp, _vp]),
+        "tg_expansion_field": (_i, [_i, _d, _vp, _vp, _i, _i64, _vp, _i, _vp, _vp]),
         "tg_factor_pullback": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
         "tg_translation_blocks_tangent": (_i, [_i, _d, _i64, _vp, _vp, _d, _i, _vp, _vp]),
         "tg_cluster_pushforward": (_i, [_vp, _vp, _vp, _i, ctypes.POINTER(GmresOptions), _d, _vp, _vp, _vp, _vp,

@@ -224,16 +224,18 @@ class IterativeSphereCluster:
         check(lib().tg_cluster_plane_wave(self._h, d.ctypes.data, int(pol), out.data_ptr()))
         return out
 
-    def field(self, x, points) -> torch.Tensor:
-        """Electric field of outgoing helicity vector spherical waves with coefficients
-        x (n or n x P, reference layout) about every sphere, at points (Q x 3):
-        (Q, 3) or (Q, 3, P) complex128 (fields.rs:947, the efield of a scattered
-        wave).  NaN at a sphere centre."""
+    def field(self, x, points, regular: bool = False) -> torch.Tensor:
+        """Electric field of helicity vector spherical waves with coefficients x (n or
+        n x P, reference layout) about every sphere, at points (Q x 3): (Q, 3) or
+        (Q, 3, P) complex128 (fields.rs:947).  Outgoing waves (the efield of a
+        scattered wave; NaN at a sphere centre), or regular waves with ``regular``
+        (an incident expansion; finite everywhere)."""
         cm, vector = _columns(x, self.n)
         pts = torch.as_tensor(points, dtype=torch.float64).reshape(-1, 3).to(self._dev).contiguous()
         q = pts.shape[0]
         out = torch.empty((cm.shape[0], 3 * q), dtype=torch.complex128, device=self._dev)
-        check(lib().tg_cluster_field(self._h, cm.data_ptr(), cm.shape[0], q, pts.data_ptr(), 0, out.data_ptr()))
+        check(lib().tg_cluster_field(self._h, cm.data_ptr(), cm.shape[0], q, pts.data_ptr(), int(regular),
+                                     out.data_ptr()))
         out = out.reshape(cm.shape[0], q, 3).permute(1, 2, 0)
         return out[..., 0].contiguous() if vector else out.contiguous()
 
@@ -726,6 +728,26 @@ def scattering_gradient(cluster, direction, pol, **options):
     phase = torch.sum(torch.conj(ab) * 1j * ar, dim=1).real.cpu().numpy()
     gp = gp + phase[:, None] * kvec[None, :]
     return sca, gr, gp, dict(k0=gk, epsilon=ge, convergence=conv)
+
+
+def expansion_field(lmax: int, k0: float, coefficients, points, regular: bool = False,
+                    center=(0.0, 0.0, 0.0)) -> torch.Tensor:
+    """Electric field of one expansion in helicity vector spherical waves about
+    ``center`` (degrees 1 .. lmax, 2 lmax (lmax + 2) modes in the reference order;
+    M or M x P coefficients), outgoing or ``regular``, at points (Q x 3): (Q, 3) or
+    (Q, 3, P) complex128 (fields.rs:947 for a single-origin SphericalBasis: the
+    scattered or incident field of a global T-matrix, sw/expansion.rs:116)."""
+    M = int(lib().tg_modes_per_sphere(int(lmax)))
+    cm, vector = _columns(coefficients, M)
+    dev = _device()
+    pts = torch.as_tensor(points, dtype=torch.float64).reshape(-1, 3).to(dev).contiguous()
+    q = pts.shape[0]
+    c = np.ascontiguousarray(np.asarray(center, np.float64).reshape(3))
+    out = torch.empty((cm.shape[0], 3 * q), dtype=torch.complex128, device=dev)
+    check(lib().tg_expansion_field(int(lmax), float(k0), c.ctypes.data, cm.data_ptr(), cm.shape[0], q,
+                                   pts.data_ptr(), int(regular), out.data_ptr(), ctypes.c_void_p(_stream_ptr())))
+    out = out.reshape(cm.shape[0], q, 3).permute(1, 2, 0)
+    return out[..., 0].contiguous() if vector else out.contiguous()
 
 
 def plane_wave_field(k0: float, direction, pol, points) -> torch.Tensor:

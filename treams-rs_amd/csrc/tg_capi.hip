@@ -708,6 +708,36 @@ int tg_translation_blocks(int lmax, double k, int64_t count, const double *disp,
     return st;
 }
 
+int tg_expansion_field(int lmax, double k0, const double *center, const tg_c128 *x, int cols, int64_t npoints,
+                       const double *points, int regular, tg_c128 *out, void *stream) {
+    HostPlan hp;
+    if (lmax < 1 || lmax > kMaxLmax || !center || !x || !out || cols < 1 || npoints < 0 ||
+        (npoints > 0 && !points) || !std::isfinite(k0) || !(k0 > 0) || !std::isfinite(center[0]) ||
+        !std::isfinite(center[1]) || !std::isfinite(center[2]) || cached_plan(lmax, hp)) {
+        set_error("invalid lmax (1 .. %d), wavenumber, centre, columns, points or buffers", kMaxLmax);
+        return TG_INVALID_INPUT;
+    }
+    if (npoints == 0) return TG_OK;
+    DevPlan dp;
+    std::vector<int> ntiles;
+    TG_TRY(shared_device_plan(lmax, hp, dp, ntiles));
+    hipStream_t s = (hipStream_t)stream;
+    DevBuf c;
+    TG_TRY(c.ensure(sizeof(double) * 3));
+    TG_HIP(hipMemcpyAsync(c.ptr, center, sizeof(double) * 3, hipMemcpyHostToDevice, s));
+    const int64_t M = 2 * (int64_t)lmax * (lmax + 2);
+    const int st = launch_field(1, lmax, k0, c.as<double>(), dp.refl, (const cd *)x, M, cols, npoints, points,
+                                regular != 0, (cd *)out, s);
+    if (st == TG_OK) {
+        CleanReleases clean;  // launch_field synchronised the stream
+        c.release();
+    } else {
+        (void)hipStreamSynchronize(s);
+        c.release();
+    }
+    return st;
+}
+
 int tg_translation_blocks_tangent(int lmax, double k, int64_t count, const double *disp,
                                   const double *tangent_disp, double tangent_k, int regular, tg_c128 *out,
                                   void *stream) {

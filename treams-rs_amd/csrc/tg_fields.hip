@@ -1,19 +1,22 @@
-// Electric fields of mode expansions at sample points: the scattered field of a
-// solved cluster (outgoing helicity vector spherical waves about every sphere) and
-// the field of a helicity plane wave.
+// Electric fields of mode expansions at sample points: outgoing (singular) or
+// regular helicity vector spherical waves about every sphere of a cluster, or about
+// one centre (a global expansion), and the field of a helicity plane wave.
 //
 // Vector spherical waves (fields.rs:144-320): with the scaled radial function
-// c = h_l(k r) / r^l, e = h_{l+1}(k r) / r^{l+1} and the solid harmonic
-// S = r^l P_l^m(cos theta) e^{i m phi},
+// c = z_l(k r) / r^l, e = z_{l+1}(k r) / r^{l+1} (z = h for outgoing, j for regular
+// waves) and the solid harmonic S = r^l P_l^m(cos theta) e^{i m phi},
 //   n = d grad S + l e S r,  d = (l + 1) c / k - r^2 e,     m = c (r x grad S),
 //   wave(l, m, pol) = i sqrt((2l+1) / (4 pi l (l+1))) sqrt((l-m)!/(l+m)!) (n - (2 pol - 1) m) / sqrt(2).
 // Coordinates are divided by their largest component (the waves do not change
-// under a common change of length unit), so r^l stays near one.
+// under a common change of length unit), so r^l stays near one; for regular waves
+// the unit is at least 1 / k, and c comes from its series below k r = 1/2 (finite at
+// the centre, fields.rs:140-180).
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "tg_internal.h"
+#include "tg_special.h"
 
 namespace tg {
 
@@ -25,6 +28,37 @@ constexpr int kFieldCols = 8;  // coefficient columns per pass
 // recurrences of the solid harmonic (value and gradient) and of h_l run side by
 // side, so no table indexed by a run-time degree is kept.  The coefficients are read
 // with wave-uniform addresses.  A point at a sphere centre gets NaN.
+// j_l(x) / r^l for l = 0 .. last (last <= kMaxLmax + 1), x = k r: the series
+// k^l / (2l+1)!! sum_q (-x^2 / 2)^q / (q! (2l+3) ... (2l+2q+1)) below x = 1/2 (12
+// terms: the tail is below 4e-33 of the first), else j_l(x) / r^l.
+__device__ void regular_scaled(int last, double k, double r, double *out) {
+    const double x = k * r;
+    if (x < 0.5) {
+        double kl = 1.0, dfact = 1.0;  // k^l, (2l+1)!!
+        for (int l = 0; l <= last; ++l) {
+            if (l > 0) {
+                kl *= k;
+                dfact *= 2.0 * l + 1.0;
+            }
+            double coef = kl / dfact, sum = 0.0, x2q = 1.0;
+            for (int q = 0; q < 12; ++q) {
+                sum += coef * x2q;
+                x2q *= x * x;
+                coef /= -2.0 * (q + 1) * (2.0 * l + 2.0 * q + 3.0);
+            }
+            out[l] = sum;
+        }
+        return;
+    }
+    tg_bessel_j_seq(last, x, out);
+    double ir = 1.0;
+    for (int l = 0; l <= last; ++l) {
+        out[l] *= ir;
+        ir /= r;
+    }
+}
+
+template <bool REG>
 __global__ __launch_bounds__(128) void k_cluster_field(int N, int chunk, int lmax, double k, const double *pos,
                                                        const double *refl, const cd *x, int64_t n, int cols,
                                                        int64_t npts, const double *points, cd *part) {
@@ -43,6 +77,7 @@ __global__ __launch_bounds__(128) void k_cluster_field(int N, int chunk, int lma
             rs[a] = points[3 * p + a] - pos[3 * i + a];
             scale = fmax(scale, fabs(rs[a]));
         }
+        if (REG) scale = fmax(scale, 1.0 / k);
         if (!(scale > 0.0)) {
             singular = true;
             continue;
@@ -53,6 +88,8 @@ __global__ __launch_bounds__(128) void k_cluster_field(int N, int chunk, int lma
         const double xarg = ks * rr, inv = 1.0 / xarg, sx = sin(xarg), cxv = cos(xarg), iks = 1.0 / ks;
         const cd xy(rs[0], rs[1]);
         const cd *xi = x + (int64_t)i * M;
+        double jr[kMaxLmax + 2];  // regular waves: j_l(ks rr) / rr^l
+        if (REG) regular_scaled(lmax + 1, ks, rr, jr);
         // ladder over the order: P_q^q (-1)^q (2q-1)!! (x + i y)^q and its gradient
         cd lp(1.0), lg[3] = {cd(0.0), cd(0.0), cd(0.0)};
         for (int q = 0; q <= lmax; ++q) {
@@ -85,7 +122,7 @@ __global__ __launch_bounds__(128) void k_cluster_field(int N, int chunk, int lma
                     }
                 }
                 if (l >= q && l >= 1) {
-                    const cd cl = irpow * hcur, el = (irpow * irr) * hnext;
+                    const cd cl = REG ? cd(jr[l]) : irpow * hcur, el = REG ? cd(jr[l + 1]) : (irpow * irr) * hnext;
                     const cd d = ((double)(l + 1) * iks) * cl - r2 * el;
                     const double base = sqrt((2.0 * l + 1.0) / (4.0 * M_PI * l * (l + 1.0)));
                     // (l - q)! / (l + q)!: the plan's reflection factors hold it for q >= 1
@@ -167,6 +204,42 @@ __global__ void k_plane_wave_field(double k0, double d0, double d1, double d2, i
     for (int a = 0; a < 3; ++a) out[3 * p + a] = phase * ((Nv[a] + s * Mv[a]) * M_SQRT1_2);
 }
 
+// Fields of N expansions about the centres pos (device, xyz), M modes each, n = N M
+// rows per column: (point, centre chunk) threads, chunks summed in order.
+int launch_field(int N, int lmax, double k, const double *pos, const double *refl, const cd *x, int64_t n, int cols,
+                 int64_t npoints, const double *points, bool regular, cd *out, hipStream_t s) {
+    if (npoints == 0) return TG_OK;
+    // enough (point, centre chunk) threads to fill the chip: about 64 waves per CU
+    const int64_t target = (int64_t)256 * 64 * 64;
+    int parts = (int)std::min<int64_t>(N, std::max<int64_t>(1, target / std::max<int64_t>(npoints, 1)));
+    parts = std::min(parts, 64);
+    const int chunk = (N + parts - 1) / parts;
+    parts = (N + chunk - 1) / chunk;
+    DevBuf scratch;
+    TG_TRY(scratch.ensure(sizeof(cd) * (size_t)parts * kFieldCols * 3 * npoints));
+    for (int c0 = 0; c0 < cols; c0 += kFieldCols) {
+        const int nc = std::min(kFieldCols, cols - c0);
+        cd *dst = out + (int64_t)c0 * 3 * npoints;
+        const dim3 grid((unsigned)((npoints + 127) / 128), (unsigned)parts);
+        cd *target_buf = parts == 1 ? dst : scratch.as<cd>();
+        if (regular)
+            k_cluster_field<true><<<grid, 128, 0, s>>>(N, chunk, lmax, k, pos, refl, x + (int64_t)c0 * n, n, nc,
+                                                       npoints, points, target_buf);
+        else
+            k_cluster_field<false><<<grid, 128, 0, s>>>(N, chunk, lmax, k, pos, refl, x + (int64_t)c0 * n, n, nc,
+                                                        npoints, points, target_buf);
+        TG_HIP(hipGetLastError());
+        if (parts > 1) {
+            const int64_t count = (int64_t)nc * 3 * npoints;
+            k_sum_parts<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(scratch.as<cd>(), parts, count, dst);
+            TG_HIP(hipGetLastError());
+        }
+    }
+    TG_HIP(hipStreamSynchronize(s));
+    scratch.release();
+    return TG_OK;
+}
+
 }  // namespace tg
 
 using namespace tg;
@@ -179,36 +252,8 @@ int tg_cluster_field(tg_cluster *c, const tg_c128 *x, int cols, int64_t npoints,
         set_error("null argument, no columns or negative point count");
         return TG_INVALID_INPUT;
     }
-    if (regular) {
-        set_error("fields of regular (incoming) wave expansions are not built; only outgoing waves");
-        return TG_INVALID_INPUT;
-    }
-    if (npoints == 0) return TG_OK;
-    const double *pos = c->b_pos.as<double>();
-    // enough (point, sphere chunk) threads to fill the chip: about 64 waves per CU
-    const int64_t target = (int64_t)256 * 64 * 64;
-    int parts = (int)std::min<int64_t>(c->N, std::max<int64_t>(1, target / std::max<int64_t>(npoints, 1)));
-    parts = std::min(parts, 64);
-    const int chunk = (c->N + parts - 1) / parts;
-    parts = (c->N + chunk - 1) / chunk;
-    DevBuf scratch;
-    TG_TRY(scratch.ensure(sizeof(cd) * (size_t)parts * kFieldCols * 3 * npoints));
-    for (int c0 = 0; c0 < cols; c0 += kFieldCols) {
-        const int nc = std::min(kFieldCols, cols - c0);
-        cd *dst = (cd *)out + (int64_t)c0 * 3 * npoints;
-        k_cluster_field<<<dim3((unsigned)((npoints + 127) / 128), (unsigned)parts), 128, 0, c->stream>>>(
-            c->N, chunk, c->lmax, c->k0, pos, c->plan.refl, (const cd *)x + (int64_t)c0 * c->n, c->n, nc, npoints,
-            points, parts == 1 ? dst : scratch.as<cd>());
-        TG_HIP(hipGetLastError());
-        if (parts > 1) {
-            const int64_t count = (int64_t)nc * 3 * npoints;
-            k_sum_parts<<<(unsigned)((count + 255) / 256), 256, 0, c->stream>>>(scratch.as<cd>(), parts, count, dst);
-            TG_HIP(hipGetLastError());
-        }
-    }
-    TG_HIP(hipStreamSynchronize(c->stream));
-    scratch.release();
-    return TG_OK;
+    return launch_field(c->N, c->lmax, c->k0, c->b_pos.as<double>(), c->plan.refl, (const cd *)x, c->n, cols,
+                        npoints, points, regular != 0, (cd *)out, c->stream);
 }
 
 int tg_plane_wave_field(double k0, const double *direction, int pol, int64_t npoints, const double *points,

@@ -203,6 +203,10 @@ int tloc_from_ref(tg_cluster *c, const cd *ref, cd *panels);
 int local_cotangent(tg_cluster *c, const cd *Y, const cd *R, int cols, cd *out_ref);
 // error for operations that need the spheres' Mie blocks (radii, permittivities)
 int require_spheres(const tg_cluster *c, const char *what);
+// fields of N expansions (outgoing or regular waves) about the centres pos (device,
+// xyz), n = N M rows per column; synchronises the stream (tg_fields.hip)
+int launch_field(int N, int lmax, double k, const double *pos, const double *refl, const cd *x, int64_t n, int cols,
+                 int64_t npoints, const double *points, bool regular, cd *out, hipStream_t s);
 // singular blocks as their two same-helicity H x H parts, each entry a pair of bfloat16
 // (real part in the low half), [count][h][a][b]
 // (fp32_ok: the tables of these pairs fit FP32; fp32_tables_in_range of their cluster)
