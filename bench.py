@@ -416,8 +416,8 @@ def main() -> int:
                     help="bytes of a plain device copy run first (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only "
-                                   "(see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU: the path is built for one GPU's share "
+                                   "(pair-table row blocks per GPU are a design, not code), so --gpus must be 1"}))
         return 2
 
     import torch
