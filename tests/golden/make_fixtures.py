@@ -103,15 +103,19 @@ REFERENCE = "/root/reference"
 
 
 def cluster_example_fields():
-    """The 7 x 7 total-field intensity map (local expansion, y = 0, x and z on
-    linspace(-300, 300, 7) nm; NaN inside a sphere) and the cross sections of the
-    expansion about the origin with lmax 6 of docs/examples/cluster.py, read from
-    docs/examples/output/cluster.txt into reference_examples.json."""
+    """The 7 x 7 total-field intensity maps (y = 0, x and z on linspace(-300, 300, 7)
+    nm) of the local expansion (NaN inside a sphere) and of the expansion about the
+    origin with lmax 6 (NaN within radius 260 nm), and that expansion's cross sections,
+    of docs/examples/cluster.py, read from docs/examples/output/cluster.txt into
+    reference_examples.json."""
     path = os.path.join(REFERENCE, "docs", "examples", "output", "cluster.txt")
     with open(path) as f:
         lines = f.read().splitlines()
-    start = lines.index("Intensity at y = 0, local expansion (rows: x, columns: z)") + 1
-    rows = [[None if v == "nan" else float(v) for v in line.split()] for line in lines[start:start + 7]]
+    def grid_map(label):
+        start = lines.index(f"Intensity at y = 0, {label} expansion (rows: x, columns: z)") + 1
+        return [[None if v == "nan" else float(v) for v in line.split()] for line in lines[start:start + 7]]
+
+    rows = grid_map("local")
     out = os.path.join(HERE, "reference_examples.json")
     with open(out) as f:
         data = json.load(f)
@@ -122,6 +126,10 @@ def cluster_example_fields():
     data["cluster_example"]["intensity_local"] = {
         "grid": "x, z = linspace(-300, 300, 7) nm, y = 0; rows x, columns z; null inside a sphere",
         "values": rows, "significant_digits": 6}
+    data["cluster_example"]["intensity_global"] = {
+        "grid": "x, z = linspace(-300, 300, 7) nm, y = 0; rows x, columns z; null within radius 260 nm "
+                "(the expansion about the origin, lmax 6)",
+        "values": grid_map("global"), "significant_digits": 6}
     with open(out, "w") as f:
         json.dump(data, f, indent=1)
 

@@ -657,3 +657,26 @@ def test_cluster_example_global_cross_sections_golden(pkg, oracle, golden_dir):
     assert float(f"{ext:.6g}") == e["extinction"]
     assert float(f"{ext - sca:.6g}") == e["absorption"]
     f.close()
+
+
+def test_cluster_example_global_intensity_map_golden(pkg, oracle, golden_dir):
+    """docs/examples/cluster.py: total-field intensity 0.5 |E_inc + E_sca|^2 on the 7 x 7
+    grid at y = 0 from the expansion about the origin (lmax 6): the scattered field of
+    T_g a, one outgoing expansion at the origin (tg_expansion_field), against
+    docs/examples/output/cluster.txt's "global" map at its 6 printed digits (points
+    within radius 260 nm, where the expansion does not hold, are NaN there and
+    skipped)."""
+    cl, ref, g = _example(pkg, oracle, golden_dir)
+    f = pkg.SphereClusterFactor(cl, "c128")
+    Tg, _ = f.global_tmatrix(6)
+    f.close()
+    one = pkg.IterativeSphereCluster(6, ref.k0, np.array([1.0]), np.array([2.0 + 0j]), np.zeros((1, 3)))
+    a = one.plane_wave(g["direction"], g["pol"])
+    gx = np.linspace(-300, 300, 7)
+    pts = np.array([[u, 0.0, w] for u in gx for w in gx])
+    want = np.array([v for row in g["intensity_global"]["values"] for v in row], dtype=object)
+    keep = np.array([v is not None for v in want])
+    out = pts[keep]
+    e = pkg.plane_wave_field(ref.k0, g["direction"], g["pol"], out) + pkg.expansion_field(6, ref.k0, Tg @ a, out)
+    got = 0.5 * torch.sum(torch.abs(e) ** 2, dim=1).cpu().numpy()
+    assert [float(f"{v:.6g}") for v in got] == [float(v) for v in want[keep]]
