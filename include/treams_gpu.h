@@ -107,6 +107,17 @@ int tg_plan_terms(int lmax, int32_t *starts, uint8_t *nsame, double *w, uint16_t
 double tg_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3);
 
 /*
+ * Rotation matrix of z-y-z Euler angles (phi, theta, psi) in one spherical basis of
+ * degrees 1 .. lmax (2 lmax (lmax + 2) modes, reference order): row (l, m', pol),
+ * column (l, m, pol) holds D^l_(m' m) = e^(-i m' phi) d^l_(m' m)(theta) e^(-i m psi),
+ * zero between different degrees or polarisations.  out (host) is M x M, row-major.
+ * A T-matrix rotates as R T R^H.  Replaces sw_rotate / diff.rotation
+ * (rotation.rs:458-488, special/wigner.rs:411-449, python/treams_rs/diff.py:1743) for
+ * a single-origin basis (TMatrix.rotate, python/treams_rs/_framework_tmatrix.py:356).
+ */
+int tg_rotation_matrix(int lmax, double phi, double theta, double psi, tg_c128 *out_host);
+
+/*
  * Create a sphere cluster: homogeneous nonmagnetic spheres in vacuum.  Replaces
  * IterativeSphereCluster::new (cluster/iterative.rs:53-86) and the shared parts of
  * sphere_cluster / sphere_cluster_factor (cluster/spheres.rs:56-184): validation,

@@ -126,6 +126,15 @@ def cluster_example_fields():
     data["cluster_example"]["intensity_local"] = {
         "grid": "x, z = linspace(-300, 300, 7) nm, y = 0; rows x, columns z; null inside a sphere",
         "values": rows, "significant_digits": 6}
+    rot = next(line.split() for line in lines if line.split()[:1] == ["rotated"])
+    data["cluster_example"]["expected_rotated"] = {
+        "lmax": 6, "euler": [0.0, 1.5707963267948966, 0.0], "direction": [0, 0, -1],
+        "scattering": float(rot[1]), "extinction": float(rot[2]), "absorption": float(rot[3]),
+        "significant_digits": 6}
+    data["cluster_example"]["intensity_rotated"] = {
+        "grid": "x, z = linspace(-300, 300, 7) nm, y = 0; rows x, columns z; null within radius 260 nm "
+                "(the expansion about the origin rotated by z-y-z Euler angles (0, pi/2, 0), lit along -z)",
+        "values": grid_map("rotated"), "significant_digits": 6}
     data["cluster_example"]["intensity_global"] = {
         "grid": "x, z = linspace(-300, 300, 7) nm, y = 0; rows x, columns z; null within radius 260 nm "
                 "(the expansion about the origin, lmax 6)",

@@ -680,3 +680,32 @@ def test_cluster_example_global_intensity_map_golden(pkg, oracle, golden_dir):
     e = pkg.plane_wave_field(ref.k0, g["direction"], g["pol"], out) + pkg.expansion_field(6, ref.k0, Tg @ a, out)
     got = 0.5 * torch.sum(torch.abs(e) ** 2, dim=1).cpu().numpy()
     assert [float(f"{v:.6g}") for v in got] == [float(v) for v in want[keep]]
+
+
+def test_cluster_example_rotated_golden(pkg, oracle, golden_dir):
+    """docs/examples/cluster.py, third row: the expansion about the origin (lmax 6)
+    rotated by z-y-z Euler angles (0, pi/2, 0) (R T R^H with Wigner D blocks) and lit
+    along -z reproduces the first illumination in the rotated frame: its cross
+    sections and its 7 x 7 intensity map against docs/examples/output/cluster.txt's
+    "rotated" row and map at 6 printed digits."""
+    cl, ref, g = _example(pkg, oracle, golden_dir)
+    f = pkg.SphereClusterFactor(cl, "c128")
+    Tg, _ = f.global_tmatrix(6)
+    f.close()
+    e = g["expected_rotated"]
+    Tr = pkg.rotate_tmatrix(Tg, 6, *e["euler"])
+    one = pkg.IterativeSphereCluster(6, ref.k0, np.array([1.0]), np.array([2.0 + 0j]), np.zeros((1, 3)))
+    a = one.plane_wave(e["direction"], g["pol"])
+    p = Tr @ a
+    sca, ext = one.cross_sections(a, p)
+    assert float(f"{sca:.6g}") == e["scattering"]
+    assert float(f"{ext:.6g}") == e["extinction"]
+    assert float(f"{ext - sca:.6g}") == e["absorption"]
+    gx = np.linspace(-300, 300, 7)
+    pts = np.array([[u, 0.0, w] for u in gx for w in gx])
+    want = np.array([v for row in g["intensity_rotated"]["values"] for v in row], dtype=object)
+    keep = np.array([v is not None for v in want])
+    out = pts[keep]
+    fld = pkg.plane_wave_field(ref.k0, e["direction"], g["pol"], out) + pkg.expansion_field(6, ref.k0, p, out)
+    got = 0.5 * torch.sum(torch.abs(fld) ** 2, dim=1).cpu().numpy()
+    assert [float(f"{v:.6g}") for v in got] == [float(v) for v in want[keep]]

@@ -33,6 +33,44 @@ def test_wigner3j_against_sympy(pkg, golden_dir):
         assert abs(actual - expected) <= 1e-13 * max(abs(expected), 1e-3)
 
 
+def test_rotation_matrix_closed_forms_and_group(pkg):
+    """Rotation matrices of z-y-z Euler angles (host): the degree-1 small-d elements
+    in closed form (d_11 = (1 + cos t) / 2, d_10 = -sin t / sqrt 2, d_00 = cos t,
+    d_1-1 = (1 - cos t) / 2), e^(-i m phi) on the diagonal for rotations about z, the
+    identity at zero angles, unitarity, zero coupling between degrees and
+    polarisations, and the group law for successive rotations about y."""
+    t = 0.7
+    R = pkg.rotation_matrix(1, 0.0, t, 0.0)
+
+    def idx(l, m, pol):
+        return 2 * (l * l + l + m - 1) + (1 - pol)
+
+    for pol in (0, 1):
+        d = lambda mp, m: R[idx(1, mp, pol), idx(1, m, pol)]
+        assert abs(d(1, 1) - (1 + np.cos(t)) / 2) <= 1e-15
+        assert abs(d(1, 0) + np.sin(t) / np.sqrt(2)) <= 1e-15
+        assert abs(d(0, 1) - np.sin(t) / np.sqrt(2)) <= 1e-15
+        assert abs(d(0, 0) - np.cos(t)) <= 1e-15
+        assert abs(d(1, -1) - (1 - np.cos(t)) / 2) <= 1e-15
+    lmax, phi = 6, 0.4
+    Rz = pkg.rotation_matrix(lmax, phi)
+    for l in range(1, lmax + 1):
+        for m in range(-l, l + 1):
+            for pol in (0, 1):
+                assert abs(Rz[idx(l, m, pol), idx(l, m, pol)] - np.exp(-1j * m * phi)) <= 1e-14
+    assert np.abs(Rz - np.diag(np.diag(Rz))).max() == 0.0
+    assert np.abs(pkg.rotation_matrix(lmax, 0.0, 0.0, 0.0) - np.eye(Rz.shape[0])).max() <= 1e-15
+    Ra = pkg.rotation_matrix(lmax, 0.3, 1.1, -0.8)
+    assert np.abs(Ra @ Ra.conj().T - np.eye(Ra.shape[0])).max() <= 1e-13
+    pols = np.array([1 - (k % 2) for k in range(Ra.shape[0])])
+    degs = np.array([int(np.sqrt(k // 2 + 1)) for k in range(Ra.shape[0])])
+    mask = (pols[:, None] != pols[None, :]) | (degs[:, None] != degs[None, :])
+    assert np.abs(Ra[mask]).max() == 0.0
+    a, b = 0.5, 0.9
+    assert np.abs(pkg.rotation_matrix(lmax, 0, a, 0) @ pkg.rotation_matrix(lmax, 0, b, 0)
+                  - pkg.rotation_matrix(lmax, 0, a + b, 0)).max() <= 1e-13
+
+
 @pytest.mark.parametrize("lmax", [1, 3, 6])
 def test_plan_matches_oracle(pkg, oracle, lmax):
     starts, nsame, w, hidx = pkg.plan_terms(lmax)

@@ -20,6 +20,8 @@ from .cluster import (  # noqa: F401
     plan_terms,
     plane_wave_field,
     expansion_field,
+    rotation_matrix,
+    rotate_tmatrix,
     scattering_gradient,
     sphere_cluster_factor,
     translation_blocks,

@@ -26,7 +26,7 @@ TG_PC_NONE, TG_PC_BLOCK_JACOBI, TG_PC_LEARNED = 0, 1, 2
 
 #: every symbol include/treams_gpu.h declares
 EXPORTED = [
-    "tg_last_error", "tg_modes_per_sphere", "tg_plan_terms", "tg_wigner3j", "tg_cluster_create", "tg_cluster_create_particles",
+    "tg_last_error", "tg_modes_per_sphere", "tg_plan_terms", "tg_wigner3j", "tg_rotation_matrix", "tg_cluster_create", "tg_cluster_create_particles",
     "tg_cluster_destroy", "tg_cluster_dimension", "tg_cluster_mie_blocks", "tg_pair_matrix",
     "tg_translation_blocks", "tg_translation_blocks_noalloc", "tg_cluster_assemble", "tg_cluster_release", "tg_cluster_reset", "tg_cluster_apply", "tg_cluster_coupling",
     "tg_cluster_local", "tg_cluster_overlap", "tg_cluster_plane_wave", "tg_cluster_solve", "tg_cluster_cross_sections",
@@ -92,6 +92,7 @@ def lib():
         "tg_modes_per_sphere": (_i, [_i]),
         "tg_plan_terms": (_i, [_i, _vp, _vp, _vp, _vp, ctypes.POINTER(_i64)]),
         "tg_wigner3j": (_d, [_i] * 6),
+        "tg_rotation_matrix": (_i, [_i, _d, _d, _d, _vp]),
         "tg_cluster_create": (_i, [_i, _d, _i, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
         "tg_cluster_create_particles": (_i, [_i, _d, _i, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
         "tg_cluster_destroy": (_i, [_vp]),

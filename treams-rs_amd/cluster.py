@@ -777,6 +777,25 @@ def plan_terms(lmax: int):
     return starts, nsame, w, hidx
 
 
+def rotation_matrix(lmax: int, phi: float, theta: float = 0.0, psi: float = 0.0) -> np.ndarray:
+    """Rotation matrix of z-y-z Euler angles in one spherical basis of degrees
+    1 .. lmax (reference mode order): D^l_(m' m)(phi, theta, psi) between equal
+    degrees and polarisations (rotation.rs:458-488, diff.rotation)."""
+    M = int(lib().tg_modes_per_sphere(int(lmax)))
+    out = np.zeros((M, M), np.complex128)
+    check(lib().tg_rotation_matrix(int(lmax), float(phi), float(theta), float(psi), out.ctypes.data))
+    return out
+
+
+def rotate_tmatrix(tmatrix, lmax: int, phi: float, theta: float = 0.0, psi: float = 0.0):
+    """A single-origin T-matrix rotated by z-y-z Euler angles, R T R^H
+    (TMatrix.rotate, python/treams_rs/_framework_tmatrix.py:356-364); the product
+    runs on the tensor's device."""
+    t = torch.as_tensor(tmatrix)
+    r = torch.as_tensor(rotation_matrix(lmax, phi, theta, psi)).to(t.device)
+    return r @ t.to(torch.complex128) @ r.conj().T.contiguous()
+
+
 def wigner3j(j1, j2, j3, m1, m2, m3) -> float:
     return lib().tg_wigner3j(j1, j2, j3, m1, m2, m3)
 

@@ -520,6 +520,27 @@ int tg_modes_per_sphere(int lmax) { return 2 * lmax * (lmax + 2); }
 
 double tg_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) { return wigner3j(j1, j2, j3, m1, m2, m3); }
 
+int tg_rotation_matrix(int lmax, double phi, double theta, double psi, tg_c128 *out) {
+    if (lmax < 1 || lmax > kMaxLmax || !out || !std::isfinite(phi) || !std::isfinite(theta) || !std::isfinite(psi)) {
+        set_error("require 1 <= lmax <= %d, finite Euler angles and an output", kMaxLmax);
+        return TG_INVALID_INPUT;
+    }
+    const int M = 2 * lmax * (lmax + 2);
+    std::fill(out, out + (size_t)M * M, tg_c128{0.0, 0.0});
+    for (int l = 1; l <= lmax; ++l)
+        for (int mp = -l; mp <= l; ++mp)
+            for (int m = -l; m <= l; ++m) {
+                // D^l_(m' m) = e^(-i m' phi) d^l_(m' m)(theta) e^(-i m psi)
+                const double d = wigner_small_d(l, mp, m, theta), arg = -(mp * phi + m * psi);
+                const tg_c128 v{d * std::cos(arg), d * std::sin(arg)};
+                for (int pol = 0; pol < 2; ++pol) {
+                    const int r = 2 * (l * l + l + mp - 1) + (1 - pol), c = 2 * (l * l + l + m - 1) + (1 - pol);
+                    out[(size_t)r * M + c] = v;
+                }
+            }
+    return TG_OK;
+}
+
 int tg_plan_terms(int lmax, int32_t *starts, uint8_t *nsame, double *w, uint16_t *hidx, int64_t *count) {
     HostPlan hp;
     if (build_plan(lmax, hp)) {

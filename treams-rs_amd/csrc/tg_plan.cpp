@@ -57,6 +57,31 @@ double wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) {
     return (double)(sign * delta * pre * sum);
 }
 
+// Wigner's small d^l_(m' m)(theta) by his finite sum (z-y-z convention, the element
+// <l m'| exp(-i theta J_y) |l m>), in long double:
+//   sum_s (-1)^(m' - m + s) sqrt((l+m')! (l-m')! (l+m)! (l-m)!)
+//     / ((l+m-s)! s! (m'-m+s)! (l-m'-s)!) cos(theta/2)^(2l+m-m'-2s) sin(theta/2)^(m'-m+2s).
+double wigner_small_d(int l, int mp, int m, double theta) {
+    if (l < 0 || std::abs(mp) > l || std::abs(m) > l || 2 * l >= 190) return 0.0;
+    const long double c = std::cos(0.5L * (long double)theta), s = std::sin(0.5L * (long double)theta);
+    const long double pre =
+        std::sqrt(factorial_ld(l + mp) * factorial_ld(l - mp) * factorial_ld(l + m) * factorial_ld(l - m));
+    int smin = 0, smax = l + m;
+    if (m - mp > smin) smin = m - mp;
+    if (l - mp < smax) smax = l - mp;
+    long double sum = 0.0L;
+    for (int k = smin; k <= smax; ++k) {
+        const long double den = factorial_ld(l + m - k) * factorial_ld(k) * factorial_ld(mp - m + k) *
+                                factorial_ld(l - mp - k);
+        const int pc = 2 * l + m - mp - 2 * k, ps = mp - m + 2 * k;
+        long double term = pre / den;
+        for (int i = 0; i < pc; ++i) term *= c;
+        for (int i = 0; i < ps; ++i) term *= s;
+        sum += (((mp - m + k) % 2 + 2) % 2 ? -1.0L : 1.0L) * term;
+    }
+    return (double)sum;
+}
+
 // sqrt((p - q)! / (p + q)!)
 static double factorial_ratio_sqrt(int p, int q) {
     return (double)std::sqrt(factorial_ld(p - q) / factorial_ld(p + q));

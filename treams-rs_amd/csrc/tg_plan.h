@@ -17,6 +17,8 @@ struct HostPlan {
 };
 
 double wigner3j(int j1, int j2, int j3, int m1, int m2, int m3);
+// d^l_(m' m)(theta), z-y-z convention (special/wigner.rs:411)
+double wigner_small_d(int l, int mp, int m, double theta);
 int build_plan(int lmax, HostPlan &plan);
 // Destination degrees up to lmax_dst, source degrees up to lmax_src (harmonic degree
 // up to their sum, at most 2 kMaxLmax).
